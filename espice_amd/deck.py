@@ -1,0 +1,158 @@
+"""Deck: a circuit template as flat device arrays, the layout of `esl_deck`.
+
+The reference builds circuits from netlist text (src/frontend/) and binds
+device parameters to `.param`s named in `live_params` (include/espice.h:
+86-109); every point then supplies one row of live values
+(`espice_append_points`, include/espice.h:325). Here a circuit is built
+programmatically and a binding ties one device parameter to one live
+column. Node 0 is ground.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+NPARAM = 32
+NTERM = 4
+
+R, C, V, I, D, MOS1 = range(6)
+KIND_NAMES = {R: "R", C: "C", V: "V", I: "I", D: "D", MOS1: "MOS1"}
+
+# MOS1 parameter slots (include/espice_lanes.h ESL_M_*), models/mos1.va:17-80.
+M_SLOTS = {
+    "type": 0, "vto": 1, "kp": 2, "gamma": 3, "phi": 4, "lambda": 5, "w": 6, "l": 7,
+    "is": 8, "ld": 9, "tox": 10, "u0": 11, "m": 12, "cbd": 13, "cbs": 14, "pb": 15,
+    "mj": 16, "fc": 17, "cgso": 18, "cgdo": 19, "cgbo": 20, "gmin": 21, "temp": 22,
+    "tnom": 23, "given": 24,
+}
+M_DEFAULTS = {
+    "type": 1.0, "vto": 0.0, "kp": 2.0e-5, "gamma": 0.0, "phi": 0.6, "lambda": 0.0,
+    "w": 1.0e-4, "l": 1.0e-4, "is": 1.0e-14, "ld": 0.0, "tox": 1.0e-7, "u0": 600.0,
+    "m": 1.0, "cbd": 0.0, "cbs": 0.0, "pb": 0.8, "mj": 0.5, "fc": 0.5, "cgso": 0.0,
+    "cgdo": 0.0, "cgbo": 0.0, "gmin": 1.0e-12, "temp": 27.0, "tnom": 27.0,
+}
+# $param_given bits (ESL_MG_*)
+MG_TOX, MG_KP, MG_PHI, MG_GAMMA = 1, 2, 4, 8
+# Parameters a binding may move per point on the GPU lanes.
+BINDABLE = {R: {0}, C: {0}, V: {0}, I: {0}, MOS1: {M_SLOTS["vto"], M_SLOTS["kp"]}}
+
+
+class Deck:
+    """Builder for a circuit template with live parameters."""
+
+    def __init__(self, title: str = ""):
+        self.title = title
+        self._nodes: dict[str, int] = {"0": 0, "gnd": 0}
+        self.kind: list[int] = []
+        self.terms: list[list[int]] = []
+        self.params: list[np.ndarray] = []
+        self.names: list[str] = []
+        self._by_name: dict[str, int] = {}
+        self.live: list[str] = []
+        self.binds: list[tuple[int, int, int]] = []
+
+    # -- nodes --------------------------------------------------------------
+    def node(self, name) -> int:
+        key = str(name).lower()
+        if key not in self._nodes:
+            self._nodes[key] = len(set(self._nodes.values()))
+        return self._nodes[key]
+
+    @property
+    def n_nodes(self) -> int:
+        return len(set(self._nodes.values()))
+
+    def node_index(self, name) -> int:
+        return self._nodes[str(name).lower()]
+
+    # -- devices ------------------------------------------------------------
+    def _add(self, name: str, kind: int, nodes, p: np.ndarray) -> int:
+        if name.lower() in self._by_name:
+            raise ValueError(f"duplicate device {name}")
+        t = [self.node(x) for x in nodes] + [0] * (NTERM - len(nodes))
+        self.kind.append(kind)
+        self.terms.append(t)
+        self.params.append(p)
+        self.names.append(name)
+        idx = len(self.kind) - 1
+        self._by_name[name.lower()] = idx
+        return idx
+
+    def resistor(self, name, p, n, r):
+        a = np.zeros(NPARAM)
+        a[0] = r
+        return self._add(name, R, (p, n), a)
+
+    def capacitor(self, name, p, n, c):
+        a = np.zeros(NPARAM)
+        a[0] = c
+        return self._add(name, C, (p, n), a)
+
+    def vsource(self, name, p, n, dc):
+        a = np.zeros(NPARAM)
+        a[0] = dc
+        return self._add(name, V, (p, n), a)
+
+    def isource(self, name, p, n, dc):
+        a = np.zeros(NPARAM)
+        a[0] = dc
+        return self._add(name, I, (p, n), a)
+
+    def mos1(self, name, d, g, s, b, model: dict, **inst):
+        """MOS1 with a model dict (SPICE names, lower case) and instance
+        overrides (w, l, m). `type` is +1 NMOS, -1 PMOS."""
+        vals = dict(M_DEFAULTS)
+        given = 0
+        for k, v in {**model, **inst}.items():
+            k = k.lower()
+            if k == "uo":
+                k = "u0"
+            if k not in M_SLOTS or k == "given":
+                raise ValueError(f"unsupported MOS1 parameter {k}")
+            vals[k] = float(v)
+            given |= {"tox": MG_TOX, "kp": MG_KP, "phi": MG_PHI, "gamma": MG_GAMMA}.get(k, 0)
+        a = np.zeros(NPARAM)
+        for k, slot in M_SLOTS.items():
+            if k != "given":
+                a[slot] = vals[k]
+        a[M_SLOTS["given"]] = float(given)
+        return self._add(name, MOS1, (d, g, s, b), a)
+
+    # -- live parameters ----------------------------------------------------
+    def live_param(self, name: str) -> int:
+        if name in self.live:
+            return self.live.index(name)
+        self.live.append(name)
+        return len(self.live) - 1
+
+    def bind(self, device: str, param, live: str):
+        """params[device][param] = live column `live` for every point."""
+        dev = self._by_name[device.lower()]
+        slot = param if isinstance(param, int) else (M_SLOTS[param] if self.kind[dev] == MOS1 else 0)
+        self.binds.append((dev, slot, self.live_param(live)))
+
+    def param(self, device: str, name) -> float:
+        dev = self._by_name[device.lower()]
+        slot = name if isinstance(name, int) else (M_SLOTS[name] if self.kind[dev] == MOS1 else 0)
+        return float(self.params[dev][slot])
+
+    # -- flat arrays --------------------------------------------------------
+    def arrays(self) -> dict:
+        nd = len(self.kind)
+        b = np.array(self.binds, dtype=np.uint32).reshape(-1, 3)
+        return {
+            "n_nodes": self.n_nodes,
+            "kind": np.array(self.kind, dtype=np.uint32),
+            "nodes": np.array(self.terms, dtype=np.uint32).reshape(nd, NTERM),
+            "params": np.array(self.params, dtype=np.float64).reshape(nd, NPARAM),
+            "n_live": len(self.live),
+            "bind_dev": np.ascontiguousarray(b[:, 0]),
+            "bind_param": np.ascontiguousarray(b[:, 1]),
+            "bind_live": np.ascontiguousarray(b[:, 2]),
+        }
+
+    def nominal(self) -> np.ndarray:
+        """The live row that reproduces the template's own parameter values."""
+        row = np.zeros(len(self.live))
+        for dev, slot, li in self.binds:
+            row[li] = self.params[dev][slot]
+        return row

@@ -1,0 +1,237 @@
+/* espice_lanes: sweep points as MI355X (gfx950) lanes behind a C ABI.
+ *
+ * This is the drop-in for ESPice's GPU-lane path. It replaces two boundaries
+ * of the reference (OmarSiwy/ESPice 1.2.0):
+ *
+ *   - the internal device-lane context `GpuLanes` (src/analysis/gpu_lanes.zig:
+ *     251-1027) and its raw kernels `arp_lane_*` / `arp_l*_<model>`
+ *     (src/solver/lane_device.zig:26-258, src/device/eval.zig:2998-3203),
+ *     which a caller such as `tran.runGpuLanes` (src/analysis/tran/tran.zig:
+ *     1654-1820) drives tick by tick; and
+ *   - the points API of include/espice.h:244-360 (`espice_append_points`,
+ *     `espice_copy_point_measures`, `espice_release_points`,
+ *     `espice_lane_width`), whose rows of live-parameter values become lanes.
+ *
+ * Conventions follow include/espice.h:139-158: every call returns an
+ * esl_status; handles are generation-checked tokens, never addresses; calls
+ * on one handle must be serialized, different handles are independent; no
+ * call keeps a caller pointer after it returns; caller-owned output buffers
+ * are sized by the documented counts. There is no CPU fallback: a handle is
+ * created only when the HIP device and the kernels are present, and every
+ * numerical call runs on the GPU.
+ *
+ * The circuit is described as a deck of devices (esl_deck) rather than
+ * netlist text: netlist parsing is not part of the lane path. Units are SI.
+ */
+#ifndef ESPICE_LANES_H
+#define ESPICE_LANES_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ESL_ABI_VERSION 1u
+
+/* esl_status: the values of espice_status (include/espice.h:21-38). */
+typedef uint32_t esl_status;
+enum {
+    ESL_OK = 0, ESL_INVALID_ARGUMENT = 1, ESL_BUFFER_TOO_SMALL = 2,
+    ESL_OUT_OF_MEMORY = 3, ESL_INVALID_QUERY = 4, ESL_RESULT_UNAVAILABLE = 5,
+    ESL_FAILED = 6, ESL_ABI_MISMATCH = 7, ESL_INVALID_HANDLE = 8,
+    /* No HIP device, or the kernels failed to load or launch: the caller
+     * runs its points elsewhere, as tran.zig:1702-1716 does on a refused
+     * GpuLanes.init. */
+    ESL_DEVICE_UNAVAILABLE = 9
+};
+
+/* Per-point reasons: the values of espice_reason (include/espice.h:259-284). */
+enum {
+    ESL_REASON_OK = 0, ESL_REASON_NO_CONVERGENCE = 1,
+    ESL_REASON_SINGULAR_MATRIX = 3, ESL_REASON_INVALID_PARAMETER = 9,
+    ESL_REASON_OTHER = 255
+};
+
+/* Device kinds. The stamp order of a deck is kind-major in this order, then
+ * deck order within a kind (the reference stamps batch by batch, instances
+ * in tape order: src/device/eval.zig:782-922). */
+enum {
+    ESL_DEV_R = 0,     /* models/resistor.va  nodes p n          */
+    ESL_DEV_C = 1,     /* models/capacitor.va nodes p n          */
+    ESL_DEV_V = 2,     /* models/vsource.va   nodes p n (DC)     */
+    ESL_DEV_I = 3,     /* models/isource.va   nodes p n (DC)     */
+    ESL_DEV_D = 4,     /* models/diode.va     nodes a k          */
+    ESL_DEV_MOS1 = 5,  /* models/mos1.va      nodes d g s b      */
+    ESL_DEV_KINDS = 6
+};
+
+#define ESL_NPARAM 32   /* doubles per device in esl_deck.params */
+#define ESL_NTERM 4     /* terminals per device in esl_deck.nodes */
+
+/* Parameter slots. R/C/V/I use slot 0 (r, c, dc, dc). */
+enum {
+    ESL_P_VALUE = 0,
+    /* MOS1 (models/mos1.va:17-80): */
+    ESL_M_TYPE = 0, ESL_M_VTO = 1, ESL_M_KP = 2, ESL_M_GAMMA = 3,
+    ESL_M_PHI = 4, ESL_M_LAMBDA = 5, ESL_M_W = 6, ESL_M_L = 7, ESL_M_IS = 8,
+    ESL_M_LD = 9, ESL_M_TOX = 10, ESL_M_U0 = 11, ESL_M_M = 12,
+    ESL_M_CBD = 13, ESL_M_CBS = 14, ESL_M_PB = 15, ESL_M_MJ = 16,
+    ESL_M_FC = 17, ESL_M_CGSO = 18, ESL_M_CGDO = 19, ESL_M_CGBO = 20,
+    ESL_M_GMIN = 21, ESL_M_TEMP = 22, ESL_M_TNOM = 23,
+    /* flag bits of which optional parameters were given (ESL_MG_*) */
+    ESL_M_GIVEN = 24,
+    /* Diode level 1 (models/diode.va): */
+    ESL_D_IS = 0, ESL_D_N = 1, ESL_D_CJO = 2, ESL_D_VJ = 3, ESL_D_M = 4,
+    ESL_D_TT = 5, ESL_D_FC = 6, ESL_D_AREA = 7, ESL_D_GMIN = 8,
+    ESL_D_TEMP = 9, ESL_D_TNOM = 10
+};
+/* Bits of params[ESL_M_GIVEN] (stored as a double holding a small integer):
+ * the reference's $param_given tests (models/mos1.va:177-217). */
+enum { ESL_MG_TOX = 1, ESL_MG_KP = 2, ESL_MG_PHI = 4, ESL_MG_GAMMA = 8 };
+
+/* A deck: n_nodes nodes (node 0 is ground), n_dev devices, and P live
+ * parameters. Every binding sets params[dev][param] = live[point][live_idx]
+ * for each point, the role of a `.param` named in live_params
+ * (include/espice.h:86-109). Arrays are row-major. */
+typedef struct {
+    uint32_t n_nodes;
+    uint32_t n_dev;
+    const uint32_t *kind;    /* [n_dev] ESL_DEV_* */
+    const uint32_t *nodes;   /* [n_dev][ESL_NTERM], unused terminals 0 */
+    const double *params;    /* [n_dev][ESL_NPARAM] */
+    uint32_t n_live;
+    uint32_t n_bind;
+    const uint32_t *bind_dev;    /* [n_bind] */
+    const uint32_t *bind_param;  /* [n_bind] parameter slot */
+    const uint32_t *bind_live;   /* [n_bind] live index < n_live */
+} esl_deck;
+
+/* Newton tolerances: src/core/numerics.zig:377-404 defaults. */
+typedef struct {
+    double reltol, abstol, vntol, residual_tol;
+    uint32_t max_iter;      /* max(itl1, 100), converger.zig:85-96 */
+    uint32_t reserved;
+} esl_tolerances;
+
+/* esl_op_options.precision */
+enum {
+    ESL_FP64 = 0,          /* FP64 factors throughout (the drop-in) */
+    ESL_FP32_REFINED = 1   /* FP32 factors, FP64 residual + refinement */
+};
+/* esl_op_options.start */
+enum {
+    ESL_START_COLD = 0,    /* zero x + junction seeds, INITFIX (dc/op.zig:24-27) */
+    ESL_START_GIVEN = 1    /* x0 supplied per point, no INITFIX */
+};
+
+typedef struct {
+    size_t struct_size;    /* sizeof(esl_op_options) */
+    esl_tolerances tol;
+    uint32_t precision;    /* ESL_FP64 | ESL_FP32_REFINED */
+    uint32_t start;        /* ESL_START_* */
+    /* ESL_START_GIVEN: [count][n] start vectors (n = esl_engine_info.n). */
+    const double *x0;
+    /* FP32_REFINED: refinement stops once max|r| <= refine_tol * max|F|
+     * (relative residual), or after refine_max steps. */
+    double refine_tol;
+    uint32_t refine_max;
+    uint32_t reserved;
+} esl_op_options;
+
+/* Sizes of a compiled engine. */
+typedef struct {
+    uint32_t n;            /* MNA unknowns, ground (0) included */
+    uint32_t nnz;          /* pattern entries */
+    uint32_t n_nodes;      /* node unknowns 0..n_nodes-1; branches follow */
+    uint32_t n_live;
+    uint32_t nnz_l, nnz_u; /* strict L and U of the lead tape */
+    uint64_t flops;        /* F: refactor multiply-adds of the lead tape */
+    uint32_t levels;       /* column levels of the lead tape's refactor */
+    uint32_t lds_bytes;    /* LDS per lane of the wavefront kernel */
+    uint32_t tapes;        /* distinct pivot tapes in use */
+    uint32_t reserved;
+} esl_engine_info;
+
+typedef struct esl_engine esl_engine;
+
+uint32_t esl_abi_version(void);
+void esl_default_tolerances(esl_tolerances *tol);
+void esl_default_op_options(esl_op_options *opt);
+
+/* Compiles the deck into MNA form on the host (pattern, slots, the fill-
+ * reducing column order), and allocates nothing on the device yet.
+ * INVALID_ARGUMENT for a malformed deck (unknown kind, node out of range,
+ * binding out of range, an unsupported parameter such as MOS1 rd/rs > 0);
+ * DEVICE_UNAVAILABLE when no HIP device is present. diag receives a
+ * NUL-terminated reason when capacity > 0. */
+esl_status esl_engine_create(const esl_deck *deck, esl_engine **out,
+                             char *diag, size_t diag_capacity);
+esl_status esl_engine_destroy(esl_engine *e);
+esl_status esl_engine_info_get(esl_engine *e, esl_engine_info *out);
+/* The MNA pattern (CSC, n + 1 column pointers, nnz row indices) and the
+ * column order q (n entries) the lead factor used. Sizes from info. */
+esl_status esl_engine_pattern(esl_engine *e, uint32_t *col_ptr,
+                              uint32_t *row_idx, uint32_t *q);
+
+/* Solves the operating point of `count` points: values is
+ * [count][n_live] row-major. Outputs: x [count][n] (the converged iterate,
+ * or the last one), iterations [count] (Newton iterations, counted as
+ * converger.Result.iterations), reasons [count] (ESL_REASON_*). Lanes run
+ * on the GPU as one wavefront each. The first point's first Jacobian fixes
+ * the pivot tape (gpu_lanes.zig:361-374); a lane whose replay fails the
+ * growth, pivot or wide-multiplier test gets its own tape from a host
+ * symbolic re-pivot of its matrix and resumes on the GPU. */
+esl_status esl_op_solve(esl_engine *e, const double *values, size_t count,
+                        const esl_op_options *opt, double *x,
+                        uint32_t *iterations, uint32_t *reasons);
+
+/* Device-time split of the last esl_op_solve, milliseconds. */
+typedef struct {
+    double kernel_ms;      /* Newton kernel(s), HIP events */
+    double total_ms;       /* upload + kernels + download */
+    uint32_t launches;     /* Newton kernel launches (1 + re-pivot rounds) */
+    uint32_t repivots;     /* lanes re-pivoted on the host */
+    uint64_t lane_iterations; /* sum of Newton iterations over lanes */
+} esl_timing;
+esl_status esl_last_timing(esl_engine *e, esl_timing *out);
+
+/* ---- The LU boundary alone (arp_lane_refactor + arp_lane_solve,
+ * src/solver/lane_device.zig:26-38; bodies lane_kernels.zig:49-143). ----
+ * A tape is built on the host from one matrix (the lead lane) by the
+ * threshold-pivoting Gilbert-Peierls factor (sparse_lu.zig:260-622) over
+ * the caller's column order q; each lane then replays it on the GPU. */
+typedef struct esl_tape esl_tape;
+/* pivot_tol 1e-3 (direct.zig:32). SINGULAR (as ESL_FAILED) when the lead
+ * matrix has no usable pivot in some non-void column. */
+esl_status esl_tape_create(uint32_t n, const uint32_t *col_ptr,
+                           const uint32_t *row_idx, const double *lead_vals,
+                           const uint32_t *q, double pivot_tol,
+                           esl_tape **out);
+esl_status esl_tape_destroy(esl_tape *t);
+/* nnz_l, nnz_u, flops (F), levels. */
+esl_status esl_tape_sizes(esl_tape *t, uint32_t *nnz_l, uint32_t *nnz_u,
+                          uint64_t *flops, uint32_t *levels);
+/* The frozen tape: pinv [n], lp [n+1], li [nnz_l] (rows as pivot steps),
+ * up [n+1], ui [nnz_u], flags [n] (bit 0 void column, bit 1 scaled pivot),
+ * prow [nnz]. */
+esl_status esl_tape_export(esl_tape *t, uint32_t *pinv, uint32_t *lp,
+                           uint32_t *li, uint32_t *up, uint32_t *ui,
+                           uint32_t *flags, uint32_t *prow);
+/* For `lanes` matrices a [lanes][nnz] (pattern order) and right-hand sides
+ * rhs [lanes][n]: replays the tape on the GPU (FP64, growth limit `growth`,
+ * 1e-12 in Newton: direct.zig:53; wide bound 1/pivot_tol) and solves
+ * dx = -A^-1 rhs. Outputs lx [lanes][nnz_l], ux [lanes][nnz_u],
+ * udiag [lanes][n], dx [lanes][n], fail [lanes] (1 where the lane must
+ * re-pivot: dead/non-finite pivot, growth, non-finite L/U, wide multiplier,
+ * nonzero void slot). Any output may be NULL. */
+esl_status esl_tape_replay(esl_tape *t, const double *a, const double *rhs,
+                           size_t lanes, double growth, double *lx,
+                           double *ux, double *udiag, double *dx,
+                           uint32_t *fail);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

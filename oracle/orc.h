@@ -1,0 +1,126 @@
+/* The oracle: a CPU FP64 restatement of ESPice's sweep-lane algorithms.
+ *
+ * TEST INFRASTRUCTURE ONLY. Nothing in the espice_amd package links, loads
+ * or calls this library; only tests/, __graft_entry__.smoke() and the CPU
+ * baseline leg of bench.py do, and only as the checker.
+ *
+ * Restates (file:line in OmarSiwy/ESPice 1.2.0):
+ *   - SparseLu.factor / refactorColumns / solve (src/solver/sparse_lu.zig:
+ *     260-622, 916-1024, 1140-1184), without supernode panels, which the
+ *     reference documents as bitwise the column loop (sparse_lu.zig:772-776);
+ *   - the lane replay and its peel tests (src/solver/lane_kernels.zig:49-143);
+ *   - Newton with the reference's gate order (src/solver/converger.zig:
+ *     199-315, 454-522, 794-819, 336-346, 377-384);
+ *   - MNA planes with a ground pin (src/analysis/Circuit.zig:570-573);
+ *   - the device models of models/{resistor,vsource,isource,mos1,diode}.va
+ *     with ngspice's DEVpnjlim / DEVfetlim / DEVlimvds limiters
+ *     (dev/solvers/newton-raphson-convergence.md:36-69,151-162) applied as
+ *     limited evaluation points plus the residual correction of
+ *     src/device/eval.zig:837-841, and the MODEINITJCT seeds
+ *     (src/device/eval.zig:1841-1853, models/mos1.va:282-304).
+ *
+ * Parity status of each part is stated beside its function below.
+ */
+#ifndef ORC_H
+#define ORC_H
+#include <stddef.h>
+#include <stdint.h>
+
+#define ORC_NONE 0xffffffffu
+
+/* ---- math (orc_math.c) ---- */
+double orc_exp(double x);
+double orc_log(double x);
+
+/* ---- sparse LU (orc_lu.c) ---- */
+typedef struct {
+    uint32_t n, nnz;
+    uint32_t *q, *pinv, *prow;
+    uint32_t *lp, *li, *up, *ui;   /* li/ui as pivot steps */
+    double *lx, *ux, *udiag;
+    uint8_t *void_col, *scaled;
+    uint32_t *void_slots;
+    uint32_t nnz_l, nnz_u, n_void_slots;
+    uint32_t cap_l, cap_u;
+    int factored;
+    double *w, *y;                 /* n each, zero between calls (w) */
+} orc_lu;
+
+orc_lu *orc_lu_new(uint32_t n, uint32_t nnz);
+void orc_lu_free(orc_lu *lu);
+/* Full factor with threshold pivoting over column order q (copied).
+ * 0 on success, -1 SingularMatrix. */
+int orc_lu_factor(orc_lu *lu, const uint32_t *col_ptr, const uint32_t *row_idx,
+                  const double *vals, const uint32_t *q, double pivot_tol);
+/* refactorColumns: 0 ok, -1 SingularMatrix. */
+int orc_lu_refactor(orc_lu *lu, const uint32_t *col_ptr, const double *vals,
+                    double growth_limit);
+/* The lane replay (lane_kernels.refactor): returns 1 when the lane must
+ * re-pivot (bad pivot, growth, non-finite, wide multiplier, void slot). */
+int orc_lu_lane_refactor(orc_lu *lu, const uint32_t *col_ptr, const double *vals,
+                         double growth_limit, double wide);
+/* x = A^-1 b (solve). */
+void orc_lu_solve(orc_lu *lu, const double *b, double *x);
+/* dx = -A^-1 rhs (lane_kernels.solveNeg). */
+void orc_lu_solve_neg(orc_lu *lu, const double *rhs, double *dx);
+uint64_t orc_lu_flops(const orc_lu *lu);
+
+/* Flat C entry points for ctypes (tests): */
+int orc_lu_factor_export(uint32_t n, const uint32_t *col_ptr, const uint32_t *row_idx,
+                         const double *vals, const uint32_t *q, double pivot_tol,
+                         uint32_t *sizes /* nnz_l, nnz_u, n_void_slots */,
+                         uint32_t *pinv, uint32_t *lp, uint32_t *li, uint32_t *up,
+                         uint32_t *ui, uint32_t *flags, uint32_t *prow,
+                         double *lx, double *ux, double *udiag,
+                         size_t cap_l, size_t cap_u);
+/* Replays the tape built from lead_vals on `lanes` matrices a [lanes][nnz]
+ * and solves -rhs; outputs as esl_tape_replay. */
+int orc_lu_replay_export(uint32_t n, const uint32_t *col_ptr, const uint32_t *row_idx,
+                         const double *lead_vals, const uint32_t *q, double pivot_tol,
+                         const double *a, const double *rhs, size_t lanes, double growth,
+                         double *lx, double *ux, double *udiag, double *dx, uint32_t *fail);
+/* x = A^-1 b via a fresh full factor (for dense cross-checks). */
+int orc_lu_solve_export(uint32_t n, const uint32_t *col_ptr, const uint32_t *row_idx,
+                        const double *vals, const uint32_t *q, double pivot_tol,
+                        const double *b, double *x);
+
+/* ---- circuits and Newton (orc_ckt.c) ---- */
+typedef struct {
+    uint32_t n_nodes, n_dev;
+    const uint32_t *kind, *nodes;
+    const double *params;
+    uint32_t n_live, n_bind;
+    const uint32_t *bind_dev, *bind_param, *bind_live;
+} orc_deck;                        /* the layout of esl_deck */
+
+typedef struct {
+    double reltol, abstol, vntol, residual_tol;
+    uint32_t max_iter, reserved;
+} orc_tol;                         /* the layout of esl_tolerances */
+
+typedef struct orc_ckt orc_ckt;
+/* NULL on a malformed deck (message in err). */
+orc_ckt *orc_ckt_new(const orc_deck *deck, char *err, size_t err_cap);
+void orc_ckt_free(orc_ckt *c);
+uint32_t orc_ckt_n(const orc_ckt *c);
+uint32_t orc_ckt_nnz(const orc_ckt *c);
+void orc_ckt_pattern(const orc_ckt *c, uint32_t *col_ptr, uint32_t *row_idx,
+                     uint32_t *diag_slots, uint8_t *current_row);
+
+/* Assembles the Newton planes of one point at x with limiting state as a
+ * cold start leaves it (seeded, iteration 0): g [nnz], rhs [n]. */
+void orc_ckt_assemble_cold(orc_ckt *c, const double *live, double *g, double *rhs);
+/* Assembles at x given (no limiting: a warm start's iteration 0). */
+void orc_ckt_assemble_at(orc_ckt *c, const double *live, const double *x,
+                         double *g, double *rhs);
+
+/* Operating point of `count` points over `threads` host threads (0 = one).
+ * q: the column order every lane's lead factor uses; lead tape from point 0's
+ * first Jacobian, re-pivot per lane on replay failure, as the GPU lanes do.
+ * start_x NULL: cold start (INITFIX + seeds); else [count][n] warm starts.
+ * Outputs x [count][n], iterations, reasons (ESL_REASON_*), repivots. */
+int orc_op_solve(orc_ckt *c, const double *values, size_t count,
+                 const orc_tol *tol, const uint32_t *q, const double *start_x,
+                 int threads, double *x, uint32_t *iterations, uint32_t *reasons,
+                 uint32_t *repivots);
+#endif

@@ -1,0 +1,794 @@
+/* Oracle circuits, device models and Newton for operating points.
+ *
+ * MNA (src/analysis/Circuit.zig:89-214, 570-573): unknown 0 is ground and is
+ * pinned by g[diag0] += 1, rhs[0] += x[0]; devices never stamp a ground row
+ * or column. Node unknowns come first, then one branch current per voltage
+ * source in stamp order. The pattern holds every entry a device can stamp
+ * plus every diagonal. rhs is the KCL residual F (currents leaving a node)
+ * and the branch rows' KVL residual; Newton solves dx = -J^-1 F.
+ *
+ * Stamp order: by kind (R, C, V, I, D, MOS1), deck order within a kind; each
+ * device adds its rows and Jacobian entries in a fixed entry order
+ * (src/device/eval.zig:860-880 stamps per instance, rows in order).
+ *
+ * Device models (restated from the models/ Verilog-A files):
+ *   R   resistor.va:44-52   I(p,n) = V(p,n)/r           (tnom = temp)
+ *   V   vsource.va:329      V(p,n) = dc; branch row F = (vp - vn) - dc
+ *   I   isource.va:328      I(p,n) = dc
+ *   MOS1 mos1.va:176-463    DC part: bias-independent prep (mos1temp), the
+ *       fetlimds/limvds/pnjlimds limiting, bulk junctions jcur, mode swap,
+ *       Shichman-Hodges current with sbody and lambda. Derivatives are the
+ *       analytic partials of those expressions (the reference takes them by
+ *       forward AD; both differentiate the same branches).
+ * Limiting (src/device/eval.zig:837-841, 1005-1039; ngspice mos1load.c:
+ * 361-407 as described in mos1.va:282-304): a device keeps limited branch
+ * voltages between iterations; after each update it limits the new branch
+ * voltages against them, snaps values within 16 eps of the probe back to it
+ * (eval.zig:984-993), and reports "limited" when any value moved. The next
+ * evaluation runs at the limited voltages and corrects its residual by the
+ * Jacobian times (probe - limited). A cold start seeds the limited voltages
+ * (eval.zig:1841-1853): vgs = type vbit + gamma sqrt(phit), vds = 0,
+ * vbs = -1 (mos1.va:297-300).
+ *
+ * Newton (converger.zig:199-315) with finalizeStep's gate order (454-522):
+ * update + delta score, limiting, first-iterate veto, delta < 1, runaway
+ * bound, row-scaled residual gate; INITFIX on a cold start; a converged
+ * point publishes x_old.
+ *
+ * Parity: the models are pinned against the ngspice-44.2 operating points in
+ * tests/fixtures of the reference (transcribed into tests/golden) and by
+ * finite differences of the residual; Newton by closed forms (a divider,
+ * inverter rails); the lane machinery by agreement with the GPU.
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include "orc.h"
+
+#define K_R 0
+#define K_C 1
+#define K_V 2
+#define K_I 3
+#define K_D 4
+#define K_MOS 5
+#define NKIND 6
+#define NPARAM 32
+#define TRASH 0xffffffffu
+
+#define KB 1.38064852e-23
+#define QE 1.6021766208e-19
+
+typedef struct {
+    uint32_t kind, dev;           /* deck index */
+    uint32_t t[4];                /* terminal unknowns */
+    uint32_t br;                  /* branch unknown (V) */
+    uint32_t slot[12];            /* pattern slot per entry, TRASH for ground */
+    uint32_t nslot;
+} orc_inst;
+
+struct orc_ckt {
+    orc_deck d;
+    uint32_t *kind, *nodes, *bind_dev, *bind_param, *bind_live;
+    double *params;
+    uint32_t n, nnz, n_nodes, n_inst;
+    uint32_t *col_ptr, *row_idx, *diag;
+    uint8_t *current_row;
+    orc_inst *inst;
+};
+
+/* ---- MOS1 bias-independent prep (mos1.va:176-263) ---- */
+typedef struct {
+    double type, vbit, gammax, phit, beta, lambda, vt, isats, isatd;
+    double vcrits, vcritd, gmin, seed_vgs;
+} mos_prep;
+
+static void mos1_prep(const double *p, mos_prep *o)
+{
+    const double tp = p[0];
+    const unsigned given = (unsigned)p[24];
+    const double tnom = p[23], temp = p[22];
+    const double tnomk = tnom + 273.15;
+    const double vtnom = tnomk * (KB / QE);
+    const double egfet1 = 1.16 - (7.02e-4 * tnomk * tnomk) / (tnomk + 1108.0);
+    const double tox = p[10];
+    double coxp = ((given & 1u) && tox > 0.0) ? 3.9 * 8.854214871e-12 / tox : 0.0;
+    const double phix = p[4];
+    const double gammax = p[3];
+    const double vtox = p[1];
+    double kpx;
+    if ((given & 2u) || coxp <= 0.0) kpx = p[2];
+    else kpx = p[11] * coxp * 1.0e-4;
+    const double tk = temp + 273.15;
+    const double vt = tk * (KB / QE);
+    const double fact1 = tnomk / 300.15;
+    const double fact2 = tk / 300.15;
+    const double egfet = 1.16 - (7.02e-4 * tk * tk) / (tk + 1108.0);
+    const double arg1 = -egfet1 / (2.0 * KB * tnomk) + 1.1150877 / (2.0 * KB * 300.15);
+    const double argt = -egfet / (2.0 * KB * tk) + 1.1150877 / (2.0 * KB * 300.15);
+    const double pbfact1 = -2.0 * vtnom * (1.5 * orc_log(fact1) + QE * arg1);
+    const double pbfact = -2.0 * vt * (1.5 * orc_log(fact2) + QE * argt);
+    const double ratio4 = (tk / tnomk) * sqrt(tk / tnomk);
+    const double kpt = kpx / ratio4;
+    const double phio = (phix - pbfact1) / fact1;
+    const double phit = fact2 * phio + pbfact;
+    const double vbit = vtox - tp * (gammax * sqrt(phix)) + 0.5 * (egfet1 - egfet)
+                        + tp * 0.5 * (phit - phix);
+    const double ist = p[8] * orc_exp(-egfet / vt + egfet1 / vtnom);
+    double leff = p[7] - 2.0 * p[9];
+    if (leff < 1.0e-10) leff = 1.0e-10;
+    const double m = p[12];
+    o->type = tp;
+    o->vbit = vbit;
+    o->gammax = gammax;
+    o->phit = phit;
+    o->beta = kpt * m * p[6] / leff;
+    o->lambda = p[5];
+    o->vt = vt;
+    o->isats = ist * m;
+    o->isatd = ist * m;
+    o->vcrits = vt * orc_log(vt / (1.4142135623730951 * o->isats));
+    o->vcritd = vt * orc_log(vt / (1.4142135623730951 * o->isatd));
+    o->gmin = p[21];
+    o->seed_vgs = tp * vbit + gammax * sqrt(phit);
+}
+
+/* ---- ngspice limiters (devsup.c as given in
+ * dev/solvers/newton-raphson-convergence.md:42-69,151-162) ---- */
+static double pnjlim(double vnew, double vold, double vt, double vcrit)
+{
+    if (vnew > vcrit && fabs(vnew - vold) > vt + vt) {
+        if (vold > 0) {
+            double arg = (vnew - vold) / vt;
+            if (arg > 0) vnew = vold + vt * (2.0 + orc_log(arg - 2.0));
+            else vnew = vold - vt * (2.0 + orc_log(2.0 - arg));
+        } else {
+            vnew = vt * orc_log(vnew / vt);
+        }
+    } else if (vnew < 0) {
+        double arg = vold > 0 ? -1.0 * vold - 1.0 : 2.0 * vold - 1.0;
+        if (vnew < arg) vnew = arg;
+    }
+    return vnew;
+}
+
+static double fetlim(double vnew, double vold, double vto)
+{
+    double vtsthi = fabs(2.0 * (vold - vto)) + 2.0;
+    double vtstlo = fabs(vold - vto) + 1.0;
+    double vtox = vto + 3.5;
+    double delv = vnew - vold;
+    if (vold >= vto) {
+        if (vold >= vtox) {
+            if (delv <= 0) {
+                if (vnew >= vtox) {
+                    if (-delv > vtstlo) vnew = vold - vtstlo;
+                } else {
+                    vnew = fmax(vnew, vto + 2.0);
+                }
+            } else {
+                if (delv >= vtsthi) vnew = vold + vtsthi;
+            }
+        } else {
+            if (delv <= 0) vnew = fmax(vnew, vto - 0.5);
+            else vnew = fmin(vnew, vto + 4.0);
+        }
+    } else {
+        if (delv <= 0) {
+            if (-delv > vtsthi) vnew = vold - vtsthi;
+        } else {
+            double vtemp = vto + 0.5;
+            if (vnew <= vtemp) {
+                if (delv > vtstlo) vnew = vold + vtstlo;
+            } else {
+                vnew = vtemp;
+            }
+        }
+    }
+    return vnew;
+}
+
+static double limvds(double vnew, double vold)
+{
+    if (vold >= 3.5) {
+        if (vnew > vold) vnew = fmin(vnew, 3.0 * vold + 2.0);
+        else if (vnew < 3.5) vnew = fmax(vnew, 2.0);
+    } else {
+        if (vnew > vold) vnew = fmin(vnew, 4.0);
+        else vnew = fmax(vnew, -0.5);
+    }
+    return vnew;
+}
+
+/* jcur (mos1.va:99-110) and its derivative. */
+static void jcur(double v, double isat, double vtx, double gm, double *i, double *g)
+{
+    if (v > -3.0 * vtx) {
+        double u = fmin(v / vtx, 80.0);
+        double e = orc_exp(u);
+        *i = isat * (e * (1.0 + (v / vtx - u)) - 1.0) + gm * v;
+        *g = isat * e / vtx + gm;
+    } else {
+        *i = gm * v - isat;
+        *g = gm;
+    }
+}
+
+/* MOS1 limit step: limited (vgs, vds, vbs) from the probe and the last
+ * limited values; returns 1 when any value moved. lim: [vgs, vds, vbs, von]. */
+static int mos1_limit(const mos_prep *m, const double *x, const uint32_t *t, double *lim)
+{
+    const double xd = x[t[0]], xg = x[t[1]], xs = x[t[2]], xb = x[t[3]];
+    const double tp = m->type;
+    const double cgs = tp * (xg - xs), cds = tp * (xd - xs), cbs = tp * (xb - xs);
+    const double ogs = lim[0], ods = lim[1], obs = lim[2], von = lim[3];
+    double vgs = cgs, vds = cds, vbs = cbs;
+    double vgd = vgs - vds;
+    const double vgdo = ogs - ods;
+    if (ods >= 0) {
+        vgs = fetlim(vgs, ogs, von);
+        vds = vgs - vgd;
+        vds = limvds(vds, ods);
+    } else {
+        vgd = fetlim(vgd, vgdo, von);
+        vds = vgs - vgd;
+        vds = -limvds(-vds, -ods);
+        vgs = vgd + vds;
+    }
+    if (vds >= 0) {
+        vbs = pnjlim(vbs, obs, m->vt, m->vcrits);
+    } else {
+        double vbd = pnjlim(vbs - vds, obs - ods, m->vt, m->vcritd);
+        vbs = vbd + vds;
+    }
+    /* snapLimited: 16 eps of the largest node voltage the device reads. */
+    double scale = fmax(fmax(fabs(xd), fabs(xg)), fmax(fabs(xs), fabs(xb)));
+    double tol = 16.0 * 2.220446049250313e-16 * scale;
+    if (fabs(vgs - cgs) <= tol) vgs = cgs;
+    if (fabs(vds - cds) <= tol) vds = cds;
+    if (fabs(vbs - cbs) <= tol) vbs = cbs;
+    lim[0] = vgs;
+    lim[1] = vds;
+    lim[2] = vbs;
+    return vgs != cgs || vds != cds || vbs != cbs;
+}
+
+/* MOS1 DC evaluation at the limited voltages with the residual correction.
+ * Stamps rows d, s, b and 11 Jacobian entries (see entry order below). */
+static void mos1_eval(const mos_prep *m, const double *x, const uint32_t *t, double *lim,
+                      double *f3, double *j11)
+{
+    const double tp = m->type;
+    const double xd = x[t[0]], xg = x[t[1]], xs = x[t[2]], xb = x[t[3]];
+    const double vgs = lim[0], vds = lim[1], vbs = lim[2];
+    const double dgs = tp * (xg - xs) - vgs;
+    const double dds = tp * (xd - xs) - vds;
+    const double dbs = tp * (xb - xs) - vbs;
+    const double vbd = vbs - vds, vgd = vgs - vds;
+    double ibs, gbs, ibd, gbd;
+    jcur(vbs, m->isats, m->vt, m->gmin, &ibs, &gbs);
+    jcur(vbd, m->isatd, m->vt, m->gmin, &ibd, &gbd);
+    double mode, vbx, vgsx;
+    if (vds >= 0) { mode = 1.0; vbx = vbs; vgsx = vgs; }
+    else { mode = -1.0; vbx = vbd; vgsx = vgd; }
+    const double vdsx = mode * vds;
+    double sarg;
+    if (vbx <= 0) sarg = sqrt(m->phit - vbx);
+    else sarg = fmax(sqrt(m->phit) - vbx / (2.0 * sqrt(m->phit)), 0.0);
+    const double dsarg = sarg > 0 ? -0.5 / sarg : 0.0;
+    const double von = m->vbit * tp + m->gammax * sarg;
+    const double vgst = vgsx - von;
+    const double bp = m->beta * (1.0 + m->lambda * vdsx);
+    double id, gm, gds;
+    if (vgst <= 0) {
+        id = 0; gm = 0; gds = 0;
+    } else if (vgst <= vdsx) {
+        id = 0.5 * bp * vgst * vgst;
+        gm = bp * vgst;
+        gds = 0.5 * m->beta * m->lambda * vgst * vgst;
+    } else {
+        id = bp * vdsx * (vgst - 0.5 * vdsx);
+        gm = bp * vdsx;
+        gds = m->beta * m->lambda * vdsx * (vgst - 0.5 * vdsx) + bp * (vgst - vdsx);
+    }
+    const double gmbs = -gm * (m->gammax * dsarg);
+    lim[3] = von;
+    /* frame partials of id */
+    double ags, ads, abs_;
+    if (mode > 0) { ags = gm; ads = gds; abs_ = gmbs; }
+    else { ags = gm; ads = -(gm + gds + gmbs); abs_ = gmbs; }
+    double ich = tp * mode * id;
+    double ib_d = tp * ibd, ib_s = tp * ibs;
+    if (dgs != 0 || dds != 0 || dbs != 0) {
+        ich = ich + tp * mode * ((ags * dgs + ads * dds) + abs_ * dbs);
+        ib_d = ib_d + tp * (gbd * (dbs - dds));
+        ib_s = ib_s + tp * (gbs * dbs);
+    }
+    const double cg = mode * ags, cd = mode * ads, cb = mode * abs_;
+    const double cs = -((cg + cd) + cb);
+    f3[0] = ich - ib_d;
+    f3[1] = -ich - ib_s;
+    f3[2] = ib_d + ib_s;
+    j11[0] = cd + gbd;   /* (d,d) */
+    j11[1] = cg;         /* (d,g) */
+    j11[2] = cs;         /* (d,s) */
+    j11[3] = cb - gbd;   /* (d,b) */
+    j11[4] = -cd;        /* (s,d) */
+    j11[5] = -cg;        /* (s,g) */
+    j11[6] = -cs + gbs;  /* (s,s) */
+    j11[7] = -cb - gbs;  /* (s,b) */
+    j11[8] = -gbd;       /* (b,d) */
+    j11[9] = -gbs;       /* (b,s) */
+    j11[10] = gbd + gbs; /* (b,b) */
+}
+
+/* Entry (row terminal, col terminal) lists per kind. */
+static const uint8_t R_ENT[4][2] = {{0, 0}, {0, 1}, {1, 0}, {1, 1}};
+static const uint8_t MOS_ENT[11][2] = {{0, 0}, {0, 1}, {0, 2}, {0, 3}, {2, 0}, {2, 1},
+                                       {2, 2}, {2, 3}, {3, 0}, {3, 2}, {3, 3}};
+
+/* ---- construction ---- */
+typedef struct { uint32_t r, c; } rc_t;
+static int rc_cmp(const void *a, const void *b)
+{
+    const rc_t *x = a, *y = b;
+    if (x->c != y->c) return x->c < y->c ? -1 : 1;
+    if (x->r != y->r) return x->r < y->r ? -1 : 1;
+    return 0;
+}
+
+static uint32_t find_slot(const orc_ckt *c, uint32_t r, uint32_t col)
+{
+    if (r == 0 || col == 0) return TRASH;
+    uint32_t lo = c->col_ptr[col], hi = c->col_ptr[col + 1];
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) / 2;
+        if (c->row_idx[mid] < r) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+static void set_err(char *err, size_t cap, const char *msg)
+{
+    if (err && cap) { strncpy(err, msg, cap - 1); err[cap - 1] = 0; }
+}
+
+orc_ckt *orc_ckt_new(const orc_deck *deck, char *err, size_t err_cap)
+{
+    if (deck->n_nodes < 1) { set_err(err, err_cap, "no nodes"); return NULL; }
+    for (uint32_t i = 0; i < deck->n_dev; i++) {
+        if (deck->kind[i] >= NKIND) { set_err(err, err_cap, "unknown device kind"); return NULL; }
+        for (int t = 0; t < 4; t++)
+            if (deck->nodes[i * 4 + t] >= deck->n_nodes) { set_err(err, err_cap, "node out of range"); return NULL; }
+        if (deck->kind[i] == K_D) { set_err(err, err_cap, "diode not supported by the operating-point oracle"); return NULL; }
+    }
+    for (uint32_t b = 0; b < deck->n_bind; b++)
+        if (deck->bind_dev[b] >= deck->n_dev || deck->bind_param[b] >= NPARAM || deck->bind_live[b] >= deck->n_live) {
+            set_err(err, err_cap, "binding out of range");
+            return NULL;
+        }
+    orc_ckt *c = calloc(1, sizeof(orc_ckt));
+    c->d = *deck;
+    size_t nd = deck->n_dev;
+    c->kind = malloc(nd * 4 + 4); memcpy(c->kind, deck->kind, nd * 4);
+    c->nodes = malloc(nd * 16 + 4); memcpy(c->nodes, deck->nodes, nd * 16);
+    c->params = malloc(nd * NPARAM * 8 + 8); memcpy(c->params, deck->params, nd * NPARAM * 8);
+    size_t nb = deck->n_bind;
+    c->bind_dev = malloc(nb * 4 + 4); memcpy(c->bind_dev, deck->bind_dev, nb * 4);
+    c->bind_param = malloc(nb * 4 + 4); memcpy(c->bind_param, deck->bind_param, nb * 4);
+    c->bind_live = malloc(nb * 4 + 4); memcpy(c->bind_live, deck->bind_live, nb * 4);
+    c->d.kind = c->kind; c->d.nodes = c->nodes; c->d.params = c->params;
+    c->d.bind_dev = c->bind_dev; c->d.bind_param = c->bind_param; c->d.bind_live = c->bind_live;
+    c->n_nodes = deck->n_nodes;
+
+    /* instances in stamp order; V branches numbered in that order */
+    c->inst = calloc(nd + 1, sizeof(orc_inst));
+    uint32_t ni = 0, nbr = 0;
+    for (uint32_t k = 0; k < NKIND; k++)
+        for (uint32_t i = 0; i < nd; i++) {
+            if (c->kind[i] != k) continue;
+            orc_inst *in = &c->inst[ni++];
+            in->kind = k;
+            in->dev = i;
+            for (int t = 0; t < 4; t++) in->t[t] = c->nodes[i * 4 + t];
+            in->br = k == K_V ? deck->n_nodes + nbr++ : 0;
+        }
+    c->n_inst = ni;
+    c->n = deck->n_nodes + nbr;
+    const uint32_t n = c->n;
+    c->current_row = calloc(n, 1);
+    for (uint32_t i = deck->n_nodes; i < n; i++) c->current_row[i] = 1;
+
+    /* pattern */
+    size_t cap = n + ni * 16 + 16, cnt = 0;
+    rc_t *ent = malloc(cap * sizeof(rc_t));
+    for (uint32_t i = 0; i < n; i++) ent[cnt++] = (rc_t){i, i};
+    for (uint32_t i = 0; i < ni; i++) {
+        orc_inst *in = &c->inst[i];
+        if (in->kind == K_R) {
+            for (int e = 0; e < 4; e++) ent[cnt++] = (rc_t){in->t[R_ENT[e][0]], in->t[R_ENT[e][1]]};
+        } else if (in->kind == K_V) {
+            ent[cnt++] = (rc_t){in->t[0], in->br};
+            ent[cnt++] = (rc_t){in->t[1], in->br};
+            ent[cnt++] = (rc_t){in->br, in->t[0]};
+            ent[cnt++] = (rc_t){in->br, in->t[1]};
+        } else if (in->kind == K_MOS) {
+            for (int e = 0; e < 11; e++) ent[cnt++] = (rc_t){in->t[MOS_ENT[e][0]], in->t[MOS_ENT[e][1]]};
+        }
+    }
+    size_t keep = 0;
+    for (size_t e = 0; e < cnt; e++)
+        if ((ent[e].r != 0 && ent[e].c != 0) || (ent[e].r == 0 && ent[e].c == 0)) ent[keep++] = ent[e];
+    qsort(ent, keep, sizeof(rc_t), rc_cmp);
+    size_t u = 0;
+    for (size_t e = 0; e < keep; e++)
+        if (u == 0 || ent[e].r != ent[u - 1].r || ent[e].c != ent[u - 1].c) ent[u++] = ent[e];
+    c->nnz = (uint32_t)u;
+    c->col_ptr = calloc(n + 1, 4);
+    c->row_idx = malloc(u * 4 + 4);
+    for (size_t e = 0; e < u; e++) { c->col_ptr[ent[e].c + 1]++; c->row_idx[e] = ent[e].r; }
+    for (uint32_t j = 0; j < n; j++) c->col_ptr[j + 1] += c->col_ptr[j];
+    free(ent);
+    c->diag = malloc(n * 4);
+    for (uint32_t i = 0; i < n; i++) c->diag[i] = find_slot(c, i, i);
+    c->diag[0] = c->col_ptr[0];   /* (0,0) is column 0's only entry */
+    for (uint32_t i = 0; i < ni; i++) {
+        orc_inst *in = &c->inst[i];
+        if (in->kind == K_R) {
+            in->nslot = 4;
+            for (int e = 0; e < 4; e++) in->slot[e] = find_slot(c, in->t[R_ENT[e][0]], in->t[R_ENT[e][1]]);
+        } else if (in->kind == K_V) {
+            in->nslot = 4;
+            in->slot[0] = find_slot(c, in->t[0], in->br);
+            in->slot[1] = find_slot(c, in->t[1], in->br);
+            in->slot[2] = find_slot(c, in->br, in->t[0]);
+            in->slot[3] = find_slot(c, in->br, in->t[1]);
+        } else if (in->kind == K_MOS) {
+            in->nslot = 11;
+            for (int e = 0; e < 11; e++) in->slot[e] = find_slot(c, in->t[MOS_ENT[e][0]], in->t[MOS_ENT[e][1]]);
+        }
+    }
+    return c;
+}
+
+void orc_ckt_free(orc_ckt *c)
+{
+    if (!c) return;
+    free(c->kind); free(c->nodes); free(c->params); free(c->bind_dev); free(c->bind_param);
+    free(c->bind_live); free(c->inst); free(c->current_row); free(c->col_ptr); free(c->row_idx);
+    free(c->diag); free(c);
+}
+
+uint32_t orc_ckt_n(const orc_ckt *c) { return c->n; }
+uint32_t orc_ckt_nnz(const orc_ckt *c) { return c->nnz; }
+
+void orc_ckt_pattern(const orc_ckt *c, uint32_t *col_ptr, uint32_t *row_idx,
+                     uint32_t *diag_slots, uint8_t *current_row)
+{
+    if (col_ptr) memcpy(col_ptr, c->col_ptr, (size_t)(c->n + 1) * 4);
+    if (row_idx) memcpy(row_idx, c->row_idx, (size_t)c->nnz * 4);
+    if (diag_slots) memcpy(diag_slots, c->diag, (size_t)c->n * 4);
+    if (current_row) memcpy(current_row, c->current_row, c->n);
+}
+
+/* ---- one point's state ---- */
+typedef struct {
+    double *p;          /* [n_dev][NPARAM] with bindings applied */
+    mos_prep *mp;       /* per instance (MOS only used) */
+    double *lim;        /* [n_inst][4] */
+} lane_dev;
+
+static void lane_dev_init(const orc_ckt *c, const double *live, lane_dev *ld)
+{
+    size_t nd = c->d.n_dev;
+    ld->p = malloc(nd * NPARAM * 8 + 8);
+    memcpy(ld->p, c->params, nd * NPARAM * 8);
+    for (uint32_t b = 0; b < c->d.n_bind; b++)
+        ld->p[(size_t)c->bind_dev[b] * NPARAM + c->bind_param[b]] = live[c->bind_live[b]];
+    ld->mp = calloc(c->n_inst + 1, sizeof(mos_prep));
+    ld->lim = calloc((size_t)c->n_inst * 4 + 4, 8);
+    for (uint32_t i = 0; i < c->n_inst; i++)
+        if (c->inst[i].kind == K_MOS) mos1_prep(ld->p + (size_t)c->inst[i].dev * NPARAM, &ld->mp[i]);
+}
+
+static void lane_dev_free(lane_dev *ld) { free(ld->p); free(ld->mp); free(ld->lim); }
+
+/* cold start: MODEINITJCT seeds into the limiting state */
+static void seed_cold(const orc_ckt *c, lane_dev *ld)
+{
+    for (uint32_t i = 0; i < c->n_inst; i++) {
+        if (c->inst[i].kind != K_MOS) continue;
+        double *lim = ld->lim + (size_t)i * 4;
+        lim[0] = ld->mp[i].seed_vgs;
+        lim[1] = 0.0;
+        lim[2] = -1.0;
+        lim[3] = 0.0;
+    }
+}
+
+/* warm start: the limiting state is the start point's own branch voltages */
+static void seed_at(const orc_ckt *c, lane_dev *ld, const double *x)
+{
+    for (uint32_t i = 0; i < c->n_inst; i++) {
+        const orc_inst *in = &c->inst[i];
+        if (in->kind != K_MOS) continue;
+        double *lim = ld->lim + (size_t)i * 4;
+        const double tp = ld->mp[i].type;
+        const double xd = x[in->t[0]], xg = x[in->t[1]], xs = x[in->t[2]], xb = x[in->t[3]];
+        lim[0] = tp * (xg - xs);
+        lim[1] = tp * (xd - xs);
+        lim[2] = tp * (xb - xs);
+        lim[3] = 0.0;
+    }
+}
+
+static void assemble(const orc_ckt *c, lane_dev *ld, const double *x, double *g, double *rhs)
+{
+    const uint32_t n = c->n;
+    memset(g, 0, (size_t)(c->nnz + 1) * 8);
+    memset(rhs, 0, (size_t)(n + 1) * 8);
+    /* ground rows land in rhs[n], ground entries in g[nnz]: the trash */
+#define ROW(u) ((u) == 0 ? n : (u))
+#define SLOT(s) ((s) == TRASH ? c->nnz : (s))
+    for (uint32_t i = 0; i < c->n_inst; i++) {
+        const orc_inst *in = &c->inst[i];
+        const double *p = ld->p + (size_t)in->dev * NPARAM;
+        switch (in->kind) {
+        case K_R: {
+            const double r = p[0];
+            const double cur = (x[in->t[0]] - x[in->t[1]]) / r;
+            const double gg = 1.0 / r;
+            rhs[ROW(in->t[0])] += cur;
+            rhs[ROW(in->t[1])] += -cur;
+            g[SLOT(in->slot[0])] += gg;
+            g[SLOT(in->slot[1])] += -gg;
+            g[SLOT(in->slot[2])] += -gg;
+            g[SLOT(in->slot[3])] += gg;
+            break;
+        }
+        case K_V: {
+            const double ib = x[in->br];
+            rhs[ROW(in->t[0])] += ib;
+            rhs[ROW(in->t[1])] += -ib;
+            rhs[in->br] += (x[in->t[0]] - x[in->t[1]]) - p[0];
+            g[SLOT(in->slot[0])] += 1.0;
+            g[SLOT(in->slot[1])] += -1.0;
+            g[SLOT(in->slot[2])] += 1.0;
+            g[SLOT(in->slot[3])] += -1.0;
+            break;
+        }
+        case K_I:
+            rhs[ROW(in->t[0])] += p[0];
+            rhs[ROW(in->t[1])] += -p[0];
+            break;
+        case K_MOS: {
+            double f3[3], j11[11];
+            mos1_eval(&ld->mp[i], x, in->t, ld->lim + (size_t)i * 4, f3, j11);
+            rhs[ROW(in->t[0])] += f3[0];
+            rhs[ROW(in->t[2])] += f3[1];
+            rhs[ROW(in->t[3])] += f3[2];
+            for (int e = 0; e < 11; e++) g[SLOT(in->slot[e])] += j11[e];
+            break;
+        }
+        default:
+            break;
+        }
+    }
+#undef ROW
+#undef SLOT
+    g[c->diag[0]] += 1.0;
+    rhs[0] += x[0];
+}
+
+static int apply_limits(const orc_ckt *c, lane_dev *ld, const double *x)
+{
+    int any = 0;
+    for (uint32_t i = 0; i < c->n_inst; i++)
+        if (c->inst[i].kind == K_MOS) any |= mos1_limit(&ld->mp[i], x, c->inst[i].t, ld->lim + (size_t)i * 4);
+    return any;
+}
+
+void orc_ckt_assemble_cold(orc_ckt *c, const double *live, double *g, double *rhs)
+{
+    lane_dev ld;
+    lane_dev_init(c, live, &ld);
+    seed_cold(c, &ld);
+    double *x = calloc(c->n + 1, 8);
+    double *gg = malloc((size_t)(c->nnz + 1) * 8), *rr = malloc((size_t)(c->n + 1) * 8);
+    assemble(c, &ld, x, gg, rr);
+    memcpy(g, gg, (size_t)c->nnz * 8);
+    memcpy(rhs, rr, (size_t)c->n * 8);
+    free(x); free(gg); free(rr);
+    lane_dev_free(&ld);
+}
+
+void orc_ckt_assemble_at(orc_ckt *c, const double *live, const double *x, double *g, double *rhs)
+{
+    lane_dev ld;
+    lane_dev_init(c, live, &ld);
+    seed_at(c, &ld, x);
+    double *gg = malloc((size_t)(c->nnz + 1) * 8), *rr = malloc((size_t)(c->n + 1) * 8);
+    assemble(c, &ld, x, gg, rr);
+    memcpy(g, gg, (size_t)c->nnz * 8);
+    memcpy(rhs, rr, (size_t)c->n * 8);
+    free(gg); free(rr);
+    lane_dev_free(&ld);
+}
+
+/* ---- Newton ---- */
+static uint64_t db(double d) { uint64_t b; memcpy(&b, &d, 8); return b; }
+static double bd(uint64_t b) { double d; memcpy(&d, &b, 8); return d; }
+
+/* plausibleLimit (converger.zig:377-384) */
+static double plausible_limit(const orc_ckt *c, const double *x, const double *f, const double *g)
+{
+    double gm = 1.0;
+    for (uint32_t i = 0; i < c->n; i++) gm = fmax(gm, fabs(g[c->diag[i]]));
+    uint64_t xm = 0, fm = 0;
+    for (uint32_t i = 0; i < c->n; i++) {
+        uint64_t a = db(fabs(x[i])), b = db(fabs(f[i]));
+        if (a > xm) xm = a;
+        if (b > fm) fm = b;
+    }
+    double scale = fmin(fmax(bd(xm), bd(fm) * gm), 1e18);
+    return !(1e3 * scale > 1e9) ? 1e9 : 1e3 * scale;
+}
+
+typedef struct {
+    const orc_ckt *c;
+    const double *values;
+    size_t first, count;
+    const orc_tol *tol;
+    const orc_lu *lead;        /* the lead tape (read-only) */
+    const uint32_t *q;
+    const double *start_x;
+    double *x;
+    uint32_t *iterations, *reasons, *repivots;
+} job_t;
+
+/* one lane's Newton (converger.newtonLanes at W = 1 over a lane solver) */
+static void solve_lane(const job_t *j, size_t l)
+{
+    const orc_ckt *c = j->c;
+    const uint32_t n = c->n, nnz = c->nnz;
+    const orc_tol *tol = j->tol;
+    lane_dev ld;
+    lane_dev_init(c, j->values + l * c->d.n_live, &ld);
+    double *x = calloc(n + 1, 8), *x_old = calloc(n + 1, 8), *dx = calloc(n + 1, 8);
+    double *g = calloc(nnz + 1, 8), *rhs = calloc(n + 1, 8), *sdiag = calloc(n + 1, 8);
+    int init_fix;
+    if (j->start_x) {
+        memcpy(x, j->start_x + l * n, (size_t)n * 8);
+        seed_at(c, &ld, x);
+        init_fix = 0;
+    } else {
+        seed_cold(c, &ld);
+        init_fix = 1;
+    }
+    /* this lane's tape: a private copy of the lead's pattern, re-pivoted on
+     * a failed replay */
+    orc_lu *lu = orc_lu_new(n, nnz);
+    {
+        const orc_lu *s = j->lead;
+        memcpy(lu->q, s->q, n * 4); memcpy(lu->pinv, s->pinv, n * 4);
+        memcpy(lu->prow, s->prow, nnz * 4); memcpy(lu->lp, s->lp, (n + 1) * 4);
+        memcpy(lu->up, s->up, (n + 1) * 4);
+        memcpy(lu->void_col, s->void_col, n); memcpy(lu->scaled, s->scaled, n);
+        memcpy(lu->void_slots, s->void_slots, (size_t)s->n_void_slots * 4);
+        lu->n_void_slots = s->n_void_slots;
+        lu->cap_l = s->nnz_l + 16; lu->cap_u = s->nnz_u + 16;
+        lu->li = realloc(lu->li, lu->cap_l * 4); lu->lx = realloc(lu->lx, lu->cap_l * 8);
+        lu->ui = realloc(lu->ui, lu->cap_u * 4); lu->ux = realloc(lu->ux, lu->cap_u * 8);
+        memcpy(lu->li, s->li, s->nnz_l * 4); memcpy(lu->ui, s->ui, s->nnz_u * 4);
+        lu->nnz_l = s->nnz_l; lu->nnz_u = s->nnz_u;
+        lu->factored = 1;
+    }
+    uint32_t repiv = 0;
+    uint32_t reason = 1, iters = tol->max_iter;
+    double limit = 1e9;
+    const double wide = 1.0 / 1e-3;
+    for (uint32_t iter = 0; iter < tol->max_iter; iter++) {
+        assemble(c, &ld, x, g, rhs);
+        if (iter == 0) limit = plausible_limit(c, x, rhs, g);
+        for (uint32_t i = 0; i < n; i++) sdiag[i] = g[c->diag[i]];
+        if (orc_lu_lane_refactor(lu, c->col_ptr, g, 1e-12, wide)) {
+            repiv++;
+            if (orc_lu_factor(lu, c->col_ptr, c->row_idx, g, j->q, 1e-3) != 0) {
+                reason = 3;
+                iters = iter + 1;
+                break;
+            }
+        }
+        orc_lu_solve_neg(lu, rhs, dx);
+        /* updateAndNorm */
+        uint64_t worst = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            const double xo = x[i], d = dx[i], xn = xo + d;
+            x_old[i] = xo;
+            x[i] = xn;
+            const double t = tol->reltol * fmax(fabs(xn), fabs(xo)) + (c->current_row[i] ? tol->abstol : tol->vntol);
+            const int finite = fabs(xn) < INFINITY && fabs(d) < INFINITY;
+            const double qq = fabs(d) / t;
+            const double score = !finite ? INFINITY : (qq == qq ? qq : 0.0);
+            uint64_t sb = db(score);
+            if (sb > worst) worst = sb;
+        }
+        const double scaled = bd(worst);
+        const int limited = apply_limits(c, &ld, x);
+        int ok = !limited;
+        if (ok && iter == 0) ok = 0;
+        if (ok && !(scaled < 1.0)) ok = 0;
+        if (ok)
+            for (uint32_t i = 0; i < n; i++)
+                if (!(fabs(x[i]) <= limit)) { ok = 0; break; }
+        if (ok)
+            for (uint32_t i = 0; i < n; i++) {
+                const double s0 = fabs(sdiag[i]);
+                const double s = s0 == 0 ? 1.0 : s0;
+                const double rt = fmax(tol->residual_tol, 10.0 * s * (tol->reltol * fabs(x[i]) + tol->vntol));
+                if (!(fabs(rhs[i]) <= rt)) { ok = 0; break; }
+            }
+        if (ok && init_fix) {
+            init_fix = 0;
+        } else if (ok) {
+            memcpy(x, x_old, (size_t)n * 8);
+            reason = 0;
+            iters = iter + 1;
+            break;
+        }
+    }
+    memcpy(j->x + l * n, x, (size_t)n * 8);
+    j->iterations[l] = iters;
+    j->reasons[l] = reason;
+    if (j->repivots) j->repivots[l] = repiv;
+    orc_lu_free(lu);
+    free(x); free(x_old); free(dx); free(g); free(rhs); free(sdiag);
+    lane_dev_free(&ld);
+}
+
+static void *run_job(void *arg)
+{
+    job_t *j = arg;
+    for (size_t l = j->first; l < j->first + j->count; l++) solve_lane(j, l);
+    return NULL;
+}
+
+int orc_op_solve(orc_ckt *c, const double *values, size_t count, const orc_tol *tol,
+                 const uint32_t *q, const double *start_x, int threads, double *x,
+                 uint32_t *iterations, uint32_t *reasons, uint32_t *repivots)
+{
+    if (count == 0) return 0;
+    const uint32_t n = c->n, nnz = c->nnz;
+    /* lead tape: point 0's first Jacobian */
+    double *g = calloc(nnz + 1, 8), *rhs = calloc(n + 1, 8);
+    if (start_x) orc_ckt_assemble_at(c, values, start_x, g, rhs);
+    else orc_ckt_assemble_cold(c, values, g, rhs);
+    orc_lu *lead = orc_lu_new(n, nnz);
+    int rc = orc_lu_factor(lead, c->col_ptr, c->row_idx, g, q, 1e-3);
+    free(g); free(rhs);
+    if (rc != 0) {
+        for (size_t l = 0; l < count; l++) { reasons[l] = 3; iterations[l] = 0; }
+        orc_lu_free(lead);
+        return -1;
+    }
+    if (threads < 1) threads = 1;
+    if ((size_t)threads > count) threads = (int)count;
+    job_t *jobs = calloc(threads, sizeof(job_t));
+    pthread_t *th = calloc(threads, sizeof(pthread_t));
+    size_t per = count / threads, rem = count % threads, at = 0;
+    for (int t = 0; t < threads; t++) {
+        size_t k = per + ((size_t)t < rem ? 1 : 0);
+        jobs[t] = (job_t){c, values, at, k, tol, lead, q, start_x, x, iterations, reasons, repivots};
+        at += k;
+    }
+    if (threads == 1) {
+        run_job(&jobs[0]);
+    } else {
+        for (int t = 0; t < threads; t++) pthread_create(&th[t], NULL, run_job, &jobs[t]);
+        for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+    }
+    free(jobs); free(th);
+    orc_lu_free(lead);
+    return 0;
+}
