@@ -1,0 +1,153 @@
+"""Circuit families of the lane workloads, with seeded per-point samplers.
+
+Draws are keyed by (seed, lane, variable) through a counter-based hash, so a
+point's values depend only on its own key and never on how many points run
+or in what order (the keying idea of src/analysis/sweep/mc.zig:40-44; the
+reference's own stream is Zig's Wyhash + Xoshiro256 and is not reproduced).
+
+Families:
+  inverter_chain  16-stage MOS1 CMOS inverter chain, Monte Carlo .op
+  ota_array       64 MOS1 5-transistor OTAs in unity-gain feedback, .op,
+                  32 live parameters per point
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .deck import Deck
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def _mix(z: np.ndarray) -> np.ndarray:
+    """splitmix64 finalizer (Steele, Lea, Flood, "Fast splittable pseudorandom
+    number generators", OOPSLA 2014)."""
+    z = z.astype(np.uint64)
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform01(seed: int, lanes: np.ndarray, var: int, sub: int = 0) -> np.ndarray:
+    """U(0,1), open at both ends, keyed by (seed, lane, var, sub)."""
+    lanes = np.asarray(lanes, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        k = _mix(np.uint64(seed) * np.uint64(0x9E3779B97F4A7C15) + np.uint64(var * 4 + sub + 1))
+        z = _mix(k ^ (lanes * np.uint64(0xD1B54A32D192ED03) + np.uint64(0x632BE59BD9B4E019)))
+    return ((z >> np.uint64(11)).astype(np.float64) + 0.5) * (1.0 / 9007199254740992.0)
+
+
+def normal(seed: int, lanes: np.ndarray, var: int) -> np.ndarray:
+    """N(0,1) by Box-Muller on two keyed uniforms."""
+    u1 = uniform01(seed, lanes, var, 0)
+    u2 = uniform01(seed, lanes, var, 1)
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+
+
+@dataclass
+class Family:
+    name: str
+    deck: Deck
+    # (live name, kind, a, b): kind "normal" (mean a, sigma b), "rel_normal"
+    # (mean a, relative sigma b), "uniform_of" (U[0,1] times live column a)
+    draws: list = field(default_factory=list)
+
+    def sample(self, seed: int, lanes: int, first: int = 0) -> np.ndarray:
+        idx = np.arange(first, first + lanes, dtype=np.uint64)
+        out = np.zeros((lanes, len(self.deck.live)))
+        for var, (name, kind, a, b) in enumerate(self.draws):
+            col = self.deck.live.index(name)
+            if kind == "normal":
+                out[:, col] = a + b * normal(seed, idx, var)
+            elif kind == "rel_normal":
+                out[:, col] = a * (1.0 + b * normal(seed, idx, var))
+            elif kind == "uniform_of":
+                out[:, col] = uniform01(seed, idx, var) * out[:, self.deck.live.index(a)]
+            else:
+                raise ValueError(kind)
+        return out
+
+    def nominal(self) -> np.ndarray:
+        return self.deck.nominal()
+
+
+def inverter_chain(stages: int = 16) -> Family:
+    """16-stage CMOS inverter chain (MOS1). NMOS W/L = 1u/0.1u, KP = 200u,
+    VTO ~ N(0.5, 10 mV); PMOS W/L = 2u/0.1u, KP = 100u, VTO ~ N(-0.5, 10 mV);
+    VDD ~ N(1.0 V, 2%); input DC level ~ U[0, VDD]. LAMBDA = 0.05 on both
+    (the workload statement leaves it open; zero would make the switching
+    point an infinite-gain singular pair)."""
+    d = Deck(f"inverter_chain_{stages}")
+    nm = {"type": 1, "vto": 0.5, "kp": 200e-6, "lambda": 0.05}
+    pm = {"type": -1, "vto": -0.5, "kp": 100e-6, "lambda": 0.05}
+    d.vsource("vdd", "vdd", 0, 1.0)
+    d.vsource("vin", "in", 0, 0.5)
+    prev = "in"
+    for k in range(stages):
+        out = f"s{k + 1}"
+        d.mos1(f"mn{k}", out, prev, 0, 0, nm, w=1e-6, l=0.1e-6)
+        d.mos1(f"mp{k}", out, prev, "vdd", "vdd", pm, w=2e-6, l=0.1e-6)
+        prev = out
+    d.bind("vdd", 0, "vdd")
+    d.bind("vin", 0, "vin")
+    for k in range(stages):
+        d.bind(f"mn{k}", "vto", f"vto_n{k}")
+        d.bind(f"mp{k}", "vto", f"vto_p{k}")
+    draws = [("vdd", "rel_normal", 1.0, 0.02), ("vin", "uniform_of", "vdd", None)]
+    draws += [(f"vto_n{k}", "normal", 0.5, 0.01) for k in range(stages)]
+    draws += [(f"vto_p{k}", "normal", -0.5, 0.01) for k in range(stages)]
+    return Family(d.title, d, draws)
+
+
+OTA_ROLES = ("m1", "m2", "m3", "m4", "m5")
+
+
+def ota_array(count: int = 64, groups: int = 3) -> Family:
+    """`count` MOS1 5-transistor OTAs in unity-gain feedback sharing VDD and a
+    bias mirror (about 320 unknowns at 64). Per OTA k: M1 (diff pair, gate =
+    in+_k), M2 (gate = out_k), PMOS mirror M3/M4 and tail M5 mirrored from a
+    shared diode-connected reference fed by IBIAS. in+_k is a DC source at
+    0.8 + 0.4 k/(count-1) V. 32 live parameters per point: VDD ~ N(1.8 V, 2%),
+    IBIAS ~ N(20 uA, 5%), and for each of `groups` device groups (OTA k is in
+    group k mod groups) and each of the 5 roles, VTO (sigma 10 mV) and KP
+    (sigma 3%)."""
+    d = Deck(f"ota_array_{count}")
+    kpn, kpp = 200e-6, 100e-6
+    lam = 0.05
+    nm = {"type": 1, "vto": 0.5, "kp": kpn, "lambda": lam}
+    pm = {"type": -1, "vto": -0.5, "kp": kpp, "lambda": lam}
+    d.vsource("vdd", "vdd", 0, 1.8)
+    d.isource("ibias", "vdd", "vbias", 20e-6)
+    d.mos1("mref", "vbias", "vbias", 0, 0, nm, w=2e-6, l=0.2e-6)
+    sizes = {"m1": (nm, 4e-6), "m2": (nm, 4e-6), "m3": (pm, 4e-6), "m4": (pm, 4e-6), "m5": (nm, 2e-6)}
+    for k in range(count):
+        inp, out, tail, mir = f"inp{k}", f"out{k}", f"tail{k}", f"mir{k}"
+        d.vsource(f"vin{k}", inp, 0, 0.8 + 0.4 * k / max(count - 1, 1))
+        d.mos1(f"m1_{k}", mir, inp, tail, 0, sizes["m1"][0], w=sizes["m1"][1], l=0.2e-6)
+        d.mos1(f"m2_{k}", out, out, tail, 0, sizes["m2"][0], w=sizes["m2"][1], l=0.2e-6)
+        d.mos1(f"m3_{k}", mir, mir, "vdd", "vdd", sizes["m3"][0], w=sizes["m3"][1], l=0.2e-6)
+        d.mos1(f"m4_{k}", out, mir, "vdd", "vdd", sizes["m4"][0], w=sizes["m4"][1], l=0.2e-6)
+        d.mos1(f"m5_{k}", tail, "vbias", 0, 0, sizes["m5"][0], w=sizes["m5"][1], l=0.2e-6)
+    d.bind("vdd", 0, "vdd")
+    d.bind("ibias", 0, "ibias")
+    draws = [("vdd", "rel_normal", 1.8, 0.02), ("ibias", "rel_normal", 20e-6, 0.05)]
+    for g in range(groups):
+        for r in OTA_ROLES:
+            m = sizes[r][0]
+            draws.append((f"vto_{r}_g{g}", "normal", m["vto"], 0.01))
+            draws.append((f"kp_{r}_g{g}", "rel_normal", m["kp"], 0.03))
+    for name, *_ in draws:
+        d.live_param(name)
+    for k in range(count):
+        g = k % groups
+        for r in OTA_ROLES:
+            d.bind(f"{r}_{k}", "vto", f"vto_{r}_g{g}")
+            d.bind(f"{r}_{k}", "kp", f"kp_{r}_g{g}")
+    return Family(d.title, d, draws)
+
+
+FAMILIES = {"inverter_chain": inverter_chain, "ota_array": ota_array}

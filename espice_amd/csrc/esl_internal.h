@@ -1,0 +1,122 @@
+// Layouts shared by the host compiler of a deck (host.cpp) and the lane
+// kernels (lanes.hip). Everything here lives in device memory, read-only to
+// the kernels and shared by every lane.
+//
+// A deck program describes how one lane evaluates its devices: the devices
+// in stamp order (kind-major, deck order within a kind), cut into chunks of
+// 64 instances, one per thread of the lane's wavefront. Each thread stages
+// its instance's contributions in LDS; a reduce task then sums the cells of
+// one target (a pattern slot or a residual row) in instance order, so every
+// slot sees its additions in the same order as a serial stamp.
+//
+// A tape program describes one frozen pivot sequence: where each pattern
+// slot lands among the factor's slots (U values, then diagonals, then L
+// values, then a discard slot), the refactor as columns grouped by level,
+// and the two triangular solves as rows grouped by level. Every column (row)
+// is processed by one thread in the serial order of its operations, so the
+// parallel replay is bitwise the serial one.
+#pragma once
+#include <stdint.h>
+
+namespace esl {
+
+constexpr int kWave = 64;
+constexpr int kStageEntries = 14;   // most cells one instance stages (MOS1)
+constexpr uint32_t kRhsBit = 0x80000000u;
+constexpr uint32_t kVoidBit = 1u, kScaledBit = 2u;
+
+enum Kind : uint32_t { kR = 0, kC = 1, kV = 2, kI = 3, kD = 4, kMos = 5, kKinds = 6 };
+
+// MOS1 per-instance constants (structure of arrays, index [field * n_mos + j]).
+enum MosField : uint32_t {
+    MF_TYPE = 0, MF_VTO, MF_VBIT_A, MF_VBIT_B, MF_VBIT_C, MF_GAMMA, MF_PHIT,
+    MF_LAMBDA, MF_VT, MF_ISATS, MF_ISATD, MF_VCRITS, MF_VCRITD, MF_GMIN,
+    MF_KP, MF_RATIO4, MF_M, MF_W, MF_LEFF, MF_COUNT
+};
+
+struct Chunk {
+    uint32_t kind, first, count, task_begin;
+};
+
+struct DeckProg {
+    uint32_t n, nnz, n_nodes, n_live;
+    uint32_t n_chunks, n_mos, diag0_slot, pad0;
+    const Chunk *chunks;        // [n_chunks + 1]: task_begin of the last is the task count
+    const uint32_t *task_target;
+    const uint32_t *task_cell;  // [n_tasks + 1] offsets into cells
+    const uint16_t *cells;      // staging index e * 64 + t
+    const uint32_t *diag_slot;  // [n]
+    const uint8_t *current_row; // [n]
+    // R: t[2], value, bind
+    const uint32_t *r_t; const double *r_val; const int32_t *r_bind;
+    // V: t[3] (p, n, branch), value, bind
+    const uint32_t *v_t; const double *v_val; const int32_t *v_bind;
+    // I: t[2], value, bind
+    const uint32_t *i_t; const double *i_val; const int32_t *i_bind;
+    // MOS1: t[4] (d, g, s, b), fields, bindings of vto and kp
+    const uint32_t *m_t; const double *m_par; const int32_t *m_bind_vto; const int32_t *m_bind_kp;
+    // first instance index of each kind in the stamp order
+    uint32_t kind_first[kKinds + 1];
+};
+
+struct TapeProg {
+    uint32_t n_tv, n_u, n_l, lx0;   // tv = [U | diag | L | discard]
+    uint32_t discard, n_levels, n_flev, n_blev;
+    uint32_t n_void, pad0, pad1, pad2;
+    double growth, wide;
+    const uint32_t *amap;      // [nnz] pattern slot -> tv slot
+    const uint32_t *pinv;      // [n] row -> pivot step
+    const uint32_t *qinv;      // [n] column -> pivot step
+    const uint32_t *flags;     // [n] per pivot step
+    const uint32_t *lev_ptr;   // [n_levels + 1]
+    const uint32_t *lev_cols;  // [n] pivot steps by level
+    const uint32_t *up;        // [n + 1]
+    const uint32_t *ui;        // [n_u]
+    const uint32_t *lp;        // [n + 1]
+    const uint32_t *fptr;      // [n + 1] first flop of each column
+    const uint32_t *dst;       // [F] tv slot each flop updates
+    const uint32_t *flev_ptr, *flev_rows;            // forward solve rows by level
+    const uint32_t *lrow_ptr, *lrow_k, *lrow_slot;   // L by rows, ascending column
+    const uint32_t *blev_ptr, *blev_rows;            // backward solve rows by level
+    const uint32_t *urow_ptr, *urow_k, *urow_slot;   // U by rows, descending column
+    const uint32_t *void_slots;                      // pattern slots a void pivot needs zero
+};
+
+// Per-lane results of the Newton kernel.
+enum LaneStatus : uint32_t {
+    kConverged = 0, kNoConvergence = 1, kSingular = 3, kRepivot = 100
+};
+
+struct LaneResume {
+    uint32_t iter, init_fix, tape, pad;
+    double limit;
+};
+
+// Arguments of the Newton kernel (op_newton).
+struct OpArgs {
+    DeckProg dp;
+    const TapeProg *tapes;
+    const uint32_t *lane_ids;   // [grid] the lane each workgroup solves
+    const double *live;         // [N][n_live]
+    double *x;                  // [N][n]: start, resume and final iterate
+    double *lim;                // [N][4][n_mos] limited (vgs, vds, vbs) and von
+    LaneResume *resume;         // [N]
+    uint32_t *status;           // [N]
+    uint32_t *iterations;       // [N]
+    double *a_out;              // dump: [grid][nnz] Jacobian in pattern order
+    double *f_out;              // dump: [grid][n] residual
+    double reltol, abstol, vntol, residual_tol;
+    uint32_t max_iter, mode;    // mode: 0 cold, 1 given x, 2 resume
+    uint32_t tv_cap, dump;
+};
+
+// Arguments of the LU-only replay (tape_replay).
+struct ReplayArgs {
+    TapeProg T;
+    uint32_t n, nnz;
+    const double *a, *rhs;
+    double *lx, *ux, *udiag, *dx;
+    uint32_t *fail;
+};
+
+}  // namespace esl

@@ -1,0 +1,1429 @@
+// The C ABI of include/espice_lanes.h: deck compile, symbolic analysis, tape
+// programs, device buffers and the Newton launch loop.
+//
+// Host work is symbolic and once per batch: the MNA pattern and stamp order
+// (src/analysis/Circuit.zig:89-214), the fill-reducing column order (the
+// reference's BTF + AMD, src/solver/order.zig:78, here minimum degree on the
+// symmetrized pattern), and the threshold-pivoting Gilbert-Peierls factor of
+// one lane's Jacobian that freezes a pivot tape (src/solver/sparse_lu.zig:
+// 260-622, gpu_lanes.zig:361-374 for the lead lane). Every numerical step of
+// every point runs in the HIP kernels (lanes.hip).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/espice_lanes.h"
+#include "esl_internal.h"
+#include "eslmath.h"
+
+extern "C" hipError_t esl_launch_op_newton(const void *a, uint32_t grid, uint32_t lds_bytes, hipStream_t s);
+extern "C" hipError_t esl_launch_tape_replay(const void *r, uint32_t lanes, uint32_t lds_bytes, hipStream_t s);
+extern "C" hipError_t esl_kernel_attrs(uint32_t lds_bytes);
+
+using std::vector;
+using u32 = uint32_t;
+
+namespace {
+
+constexpr u32 NONE = 0xffffffffu;
+constexpr int NPARAM = ESL_NPARAM;
+
+using HostOpArgs = esl::OpArgs;
+using HostReplayArgs = esl::ReplayArgs;
+
+#define HIPCHK(x)                                       \
+    do {                                                \
+        hipError_t e_ = (x);                            \
+        if (e_ != hipSuccess) return fail_hip(e_, #x);  \
+    } while (0)
+
+thread_local std::string g_last_error;
+esl_status fail_hip(hipError_t e, const char *what)
+{
+    g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+    return ESL_DEVICE_UNAVAILABLE;
+}
+
+// ---------------------------------------------------------------- device memory
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t ensure(size_t b)
+    {
+        if (b <= bytes && p) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc(&p, b ? b : 8);
+        if (e == hipSuccess) bytes = b;
+        return e;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Packs host arrays into one device allocation.
+struct Packer {
+    vector<unsigned char> blob;
+    size_t add(const void *src, size_t bytes)
+    {
+        size_t off = (blob.size() + 255) & ~size_t(255);
+        blob.resize(off + bytes + 1);
+        if (bytes) memcpy(blob.data() + off, src, bytes);
+        return off;
+    }
+    template <class T> size_t add(const vector<T> &v) { return add(v.data(), v.size() * sizeof(T)); }
+};
+
+// ---------------------------------------------------------------- host tape
+struct HostTape {
+    u32 n = 0, nnz_l = 0, nnz_u = 0;
+    vector<u32> q, pinv, lp, li, up, ui, flags, prow, void_slots;
+    vector<double> lx, ux, udiag;
+};
+
+// Threshold-pivoting left-looking Gilbert-Peierls factor (sparse_lu.zig:
+// 260-622 without supernode panels, which it documents as bitwise the same).
+// Returns false on SingularMatrix.
+bool gp_factor(u32 n, const u32 *col_ptr, const u32 *row_idx, const double *vals, const u32 *q,
+               double pivot_tol, HostTape &t)
+{
+    const u32 nnz = col_ptr[n];
+    t.n = n;
+    t.q.assign(q, q + n);
+    t.pinv.assign(n, NONE);
+    t.lp.assign(n + 1, 0);
+    t.up.assign(n + 1, 0);
+    t.udiag.assign(n, 0.0);
+    t.flags.assign(n, 0);
+    t.li.clear(); t.lx.clear(); t.ui.clear(); t.ux.clear(); t.void_slots.clear();
+    vector<u32> flag(n, 0), topo(n), stack(n), pstack(n), want(n), want_of(n);
+    vector<double> rscale(n, 0.0), w(n, 0.0);
+    uint64_t amaxb = 0;
+    for (u32 p = 0; p < nnz; p++) amaxb = std::max(amaxb, esl::double_to_bits(std::fabs(vals[p])));
+    if (!(esl::bits_to_double(amaxb) <= 1.7976931348623157e308)) return false;
+    for (u32 j = 0; j < n; j++)
+        for (u32 p = col_ptr[j]; p < col_ptr[j + 1]; p++)
+            rscale[row_idx[p]] = std::fmax(rscale[row_idx[p]], std::fabs(vals[p]));
+    for (u32 i = 0; i < n; i++) rscale[i] = (rscale[i] > 0 && std::isfinite(rscale[i])) ? 1.0 / rscale[i] : 1.0;
+    for (u32 i = 0; i < n; i++) want[i] = want_of[i] = i;
+    uint64_t nonfinite = 0;
+    bool has_void = false;
+    for (u32 k = 0; k < n; k++) {
+        const u32 c = q[k];
+        t.lp[k] = (u32)t.li.size();
+        t.up[k] = (u32)t.ui.size();
+        const u32 mark = k + 1;
+        u32 nt = 0;
+        for (u32 p = col_ptr[c]; p < col_ptr[c + 1]; p++) {
+            u32 r = row_idx[p];
+            if (flag[r] == mark) continue;
+            u32 sp = 0;
+            stack[0] = r;
+            pstack[0] = t.pinv[r] == NONE ? 0 : t.lp[t.pinv[r]];
+            flag[r] = mark;
+            for (;;) {
+                r = stack[sp];
+                const u32 kc = t.pinv[r];
+                const u32 end = kc == NONE ? 0 : t.lp[kc + 1];
+                u32 pos = pstack[sp];
+                bool descended = false;
+                while (pos < end) {
+                    const u32 child = t.li[pos];
+                    pos++;
+                    if (flag[child] != mark) {
+                        pstack[sp] = pos;
+                        flag[child] = mark;
+                        sp++;
+                        stack[sp] = child;
+                        pstack[sp] = t.pinv[child] == NONE ? 0 : t.lp[t.pinv[child]];
+                        descended = true;
+                        break;
+                    }
+                }
+                if (descended) continue;
+                topo[nt++] = r;
+                if (sp == 0) break;
+                sp--;
+            }
+        }
+        for (u32 p = col_ptr[c]; p < col_ptr[c + 1]; p++) w[row_idx[p]] = vals[p];
+        for (u32 idx = nt; idx > 0;) {
+            idx--;
+            const u32 r = topo[idx], kc = t.pinv[r];
+            if (kc == NONE) continue;
+            const double ukr = w[r];
+            t.ui.push_back(kc);
+            t.ux.push_back(ukr);
+            for (u32 pl = t.lp[kc]; pl < t.lp[kc + 1]; pl++) w[t.li[pl]] -= t.lx[pl] * ukr;
+        }
+        double amax = 0, smax = 0;
+        u32 piv = NONE;
+        for (u32 s = 0; s < nt; s++) {
+            const u32 r = topo[s];
+            if (t.pinv[r] != NONE) continue;
+            const double a = std::fabs(w[r]);
+            if (a > amax) {
+                amax = a;
+                piv = r;
+            }
+            smax = std::fmax(smax, a * rscale[r]);
+        }
+        if (piv == NONE || amax == 0 || !std::isfinite(amax)) {
+            // voidUnknown (sparse_lu.zig:847-871)
+            bool is_void = t.pinv[c] == NONE;
+            for (u32 p = col_ptr[c]; is_void && p < col_ptr[c + 1]; p++)
+                if (vals[p] != 0) is_void = false;
+            for (u32 j = 0; is_void && j < n; j++) {
+                if (j == c) continue;
+                for (u32 p = col_ptr[j]; p < col_ptr[j + 1]; p++)
+                    if (row_idx[p] == c && vals[p] != 0) {
+                        is_void = false;
+                        break;
+                    }
+            }
+            for (u32 s = 0; s < nt; s++) w[topo[s]] = 0;
+            if (!is_void) return false;
+            t.flags[k] |= esl::kVoidBit;
+            has_void = true;
+            t.udiag[k] = 1;
+            t.pinv[c] = k;
+            continue;
+        }
+        const u32 pr = want[c];
+        if (t.pinv[pr] == NONE) {
+            const double dmag = std::fabs(w[pr]);
+            if (dmag > 0 && dmag >= pivot_tol * amax) {
+                piv = pr;
+            } else if (dmag > 0 && dmag * rscale[pr] >= pivot_tol * smax) {
+                piv = pr;
+                t.flags[k] |= esl::kScaledBit;
+            }
+        }
+        if (piv != pr) {
+            const u32 c2 = want_of[piv];
+            want[c2] = pr;
+            want_of[pr] = c2;
+        }
+        const double d = w[piv];
+        t.udiag[k] = d;
+        t.pinv[piv] = k;
+        for (u32 s = 0; s < nt; s++) {
+            const u32 r = topo[s];
+            const double v = w[r];
+            nonfinite |= esl::double_to_bits(v - v);
+            if (t.pinv[r] == NONE) {
+                const double l = v / d;
+                nonfinite |= esl::double_to_bits(l - l);
+                t.li.push_back(r);
+                t.lx.push_back(l);
+            }
+            w[r] = 0;
+        }
+    }
+    t.lp[n] = (u32)t.li.size();
+    t.up[n] = (u32)t.ui.size();
+    if (nonfinite != 0) return false;
+    for (auto &r : t.li) r = t.pinv[r];
+    t.prow.resize(nnz);
+    for (u32 p = 0; p < nnz; p++) t.prow[p] = t.pinv[row_idx[p]];
+    if (has_void)
+        for (u32 j = 0; j < n; j++) {
+            const u32 c = q[j];
+            for (u32 p = col_ptr[c]; p < col_ptr[c + 1]; p++)
+                if ((t.flags[j] & esl::kVoidBit) || (t.flags[t.prow[p]] & esl::kVoidBit)) t.void_slots.push_back(p);
+        }
+    t.nnz_l = (u32)t.li.size();
+    t.nnz_u = (u32)t.ui.size();
+    return true;
+}
+
+// Minimum degree on the pattern of A + A^T (ties to the lowest index).
+vector<u32> min_degree(u32 n, const vector<u32> &col_ptr, const vector<u32> &row_idx)
+{
+    vector<vector<u32>> adj(n);
+    for (u32 j = 0; j < n; j++)
+        for (u32 p = col_ptr[j]; p < col_ptr[j + 1]; p++) {
+            const u32 i = row_idx[p];
+            if (i == j) continue;
+            adj[i].push_back(j);
+            adj[j].push_back(i);
+        }
+    for (auto &a : adj) {
+        std::sort(a.begin(), a.end());
+        a.erase(std::unique(a.begin(), a.end()), a.end());
+    }
+    vector<char> gone(n, 0);
+    vector<u32> q;
+    q.reserve(n);
+    vector<u32> merged;
+    for (u32 step = 0; step < n; step++) {
+        u32 best = NONE;
+        size_t bd = SIZE_MAX;
+        for (u32 v = 0; v < n; v++)
+            if (!gone[v] && adj[v].size() < bd) {
+                bd = adj[v].size();
+                best = v;
+            }
+        const u32 v = best;
+        q.push_back(v);
+        gone[v] = 1;
+        const vector<u32> nb = adj[v];
+        for (u32 u : nb) {
+            merged.clear();
+            std::set_union(adj[u].begin(), adj[u].end(), nb.begin(), nb.end(), std::back_inserter(merged));
+            adj[u].clear();
+            for (u32 w : merged)
+                if (w != u && !gone[w]) adj[u].push_back(w);
+        }
+        adj[v].clear();
+    }
+    return q;
+}
+
+// ---------------------------------------------------------------- tape program
+struct TapeHostProg {
+    esl::TapeProg prog{};      // pointers filled after upload
+    vector<u32> amap, pinv, qinv, flags, lev_ptr, lev_cols, up, ui, lp, fptr, dst;
+    vector<u32> flev_ptr, flev_rows, lrow_ptr, lrow_k, lrow_slot;
+    vector<u32> blev_ptr, blev_rows, urow_ptr, urow_k, urow_slot, void_slots;
+    uint64_t flops = 0;
+    DevBuf buf;
+};
+
+void levels_to_lists(const vector<u32> &lev, vector<u32> &ptr, vector<u32> &items, u32 &count)
+{
+    u32 nl = 0;
+    for (u32 v : lev) nl = std::max(nl, v + 1);
+    ptr.assign(nl + 1, 0);
+    for (u32 v : lev) ptr[v + 1]++;
+    for (u32 i = 0; i < nl; i++) ptr[i + 1] += ptr[i];
+    items.assign(lev.size(), 0);
+    vector<u32> at(ptr.begin(), ptr.end() - 1);
+    for (u32 i = 0; i < lev.size(); i++) items[at[lev[i]]++] = i;
+    count = nl;
+}
+
+void build_tape_prog(const HostTape &t, const vector<u32> &col_ptr, double growth, double wide, TapeHostProg &o)
+{
+    const u32 n = t.n, nnz = col_ptr[n];
+    const u32 nu = t.nnz_u, nl = t.nnz_l;
+    const u32 lx0 = nu + n, discard = lx0 + nl;
+    u32 next_slot = discard + 1;
+    o.amap.assign(nnz, 0);
+    o.pinv = t.pinv;
+    o.qinv.assign(n, 0);
+    for (u32 j = 0; j < n; j++) o.qinv[t.q[j]] = j;
+    o.flags = t.flags;
+    o.up = t.up;
+    o.ui = t.ui;
+    o.lp = t.lp;
+    o.fptr.assign(n + 1, 0);
+    o.dst.clear();
+    vector<u32> pos(n, NONE);
+    for (u32 k = 0; k < n; k++) {
+        const bool void_k = t.flags[k] & esl::kVoidBit;
+        for (u32 p = t.up[k]; p < t.up[k + 1]; p++) pos[t.ui[p]] = p;
+        pos[k] = nu + k;
+        for (u32 qq = t.lp[k]; qq < t.lp[k + 1]; qq++) pos[t.li[qq]] = lx0 + qq;
+        o.fptr[k] = (u32)o.dst.size();
+        for (u32 p = t.up[k]; p < t.up[k + 1]; p++) {
+            const u32 i = t.ui[p];
+            for (u32 qq = t.lp[i]; qq < t.lp[i + 1]; qq++) {
+                const u32 r = t.li[qq];
+                o.dst.push_back(void_k && r > k ? discard : pos[r]);
+            }
+        }
+        const u32 c = t.q[k];
+        for (u32 a = col_ptr[c]; a < col_ptr[c + 1]; a++) {
+            const u32 r = t.prow[a];
+            o.amap[a] = (void_k && r > k) ? next_slot++ : pos[r];
+        }
+    }
+    o.fptr[n] = (u32)o.dst.size();
+    o.flops = o.dst.size();
+    // column levels
+    vector<u32> lev(n, 0);
+    for (u32 k = 0; k < n; k++) {
+        u32 l = 0;
+        for (u32 p = t.up[k]; p < t.up[k + 1]; p++) l = std::max(l, lev[t.ui[p]] + 1);
+        lev[k] = l;
+    }
+    u32 nlev = 0, nflev = 0, nblev = 0;
+    levels_to_lists(lev, o.lev_ptr, o.lev_cols, nlev);
+    // L by rows ascending column, U by rows descending column
+    vector<vector<std::pair<u32, u32>>> lrow(n), urow(n);
+    for (u32 k = 0; k < n; k++) {
+        for (u32 qq = t.lp[k]; qq < t.lp[k + 1]; qq++) lrow[t.li[qq]].push_back({k, lx0 + qq});
+        for (u32 p = t.up[k]; p < t.up[k + 1]; p++) urow[t.ui[p]].push_back({k, p});
+    }
+    o.lrow_ptr.assign(n + 1, 0);
+    o.urow_ptr.assign(n + 1, 0);
+    o.lrow_k.clear(); o.lrow_slot.clear(); o.urow_k.clear(); o.urow_slot.clear();
+    vector<u32> flev(n, 0), blev(n, 0);
+    for (u32 r = 0; r < n; r++) {
+        auto &v = lrow[r];
+        std::sort(v.begin(), v.end());
+        u32 l = 0;
+        for (auto &e : v) {
+            o.lrow_k.push_back(e.first);
+            o.lrow_slot.push_back(e.second);
+            l = std::max(l, flev[e.first] + 1);
+        }
+        flev[r] = l;
+        o.lrow_ptr[r + 1] = (u32)o.lrow_k.size();
+    }
+    for (u32 i = n; i > 0;) {
+        i--;
+        auto &v = urow[i];
+        std::sort(v.begin(), v.end(), [](auto &a, auto &b) { return a.first > b.first; });
+        u32 l = 0;
+        for (auto &e : v) l = std::max(l, blev[e.first] + 1);
+        blev[i] = l;
+    }
+    for (u32 i = 0; i < n; i++) {
+        for (auto &e : urow[i]) {
+            o.urow_k.push_back(e.first);
+            o.urow_slot.push_back(e.second);
+        }
+        o.urow_ptr[i + 1] = (u32)o.urow_k.size();
+    }
+    levels_to_lists(flev, o.flev_ptr, o.flev_rows, nflev);
+    levels_to_lists(blev, o.blev_ptr, o.blev_rows, nblev);
+    o.void_slots = t.void_slots;
+    esl::TapeProg &P = o.prog;
+    P.n_tv = next_slot;
+    P.n_u = nu;
+    P.n_l = nl;
+    P.lx0 = lx0;
+    P.discard = discard;
+    P.n_levels = nlev;
+    P.n_flev = nflev;
+    P.n_blev = nblev;
+    P.n_void = (u32)o.void_slots.size();
+    P.growth = growth;
+    P.wide = wide;
+}
+
+hipError_t upload_tape_prog(TapeHostProg &o)
+{
+    Packer pk;
+    size_t f[24];
+    int i = 0;
+    f[i++] = pk.add(o.amap); f[i++] = pk.add(o.pinv); f[i++] = pk.add(o.qinv); f[i++] = pk.add(o.flags);
+    f[i++] = pk.add(o.lev_ptr); f[i++] = pk.add(o.lev_cols); f[i++] = pk.add(o.up); f[i++] = pk.add(o.ui);
+    f[i++] = pk.add(o.lp); f[i++] = pk.add(o.fptr); f[i++] = pk.add(o.dst);
+    f[i++] = pk.add(o.flev_ptr); f[i++] = pk.add(o.flev_rows);
+    f[i++] = pk.add(o.lrow_ptr); f[i++] = pk.add(o.lrow_k); f[i++] = pk.add(o.lrow_slot);
+    f[i++] = pk.add(o.blev_ptr); f[i++] = pk.add(o.blev_rows);
+    f[i++] = pk.add(o.urow_ptr); f[i++] = pk.add(o.urow_k); f[i++] = pk.add(o.urow_slot);
+    f[i++] = pk.add(o.void_slots);
+    hipError_t e = o.buf.ensure(pk.blob.size());
+    if (e != hipSuccess) return e;
+    e = hipMemcpy(o.buf.p, pk.blob.data(), pk.blob.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+    auto P = [&](int k) { return reinterpret_cast<const u32 *>(o.buf.as<unsigned char>() + f[k]); };
+    esl::TapeProg &T = o.prog;
+    i = 0;
+    T.amap = P(i++); T.pinv = P(i++); T.qinv = P(i++); T.flags = P(i++);
+    T.lev_ptr = P(i++); T.lev_cols = P(i++); T.up = P(i++); T.ui = P(i++);
+    T.lp = P(i++); T.fptr = P(i++); T.dst = P(i++);
+    T.flev_ptr = P(i++); T.flev_rows = P(i++);
+    T.lrow_ptr = P(i++); T.lrow_k = P(i++); T.lrow_slot = P(i++);
+    T.blev_ptr = P(i++); T.blev_rows = P(i++);
+    T.urow_ptr = P(i++); T.urow_k = P(i++); T.urow_slot = P(i++);
+    T.void_slots = P(i++);
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------- deck compile
+struct Inst {
+    u32 kind, dev, t[4], br;
+    u32 slot[11];
+};
+
+const u32 R_ENT[4][2] = {{0, 0}, {0, 1}, {1, 0}, {1, 1}};
+const u32 MOS_ENT[11][2] = {{0, 0}, {0, 1}, {0, 2}, {0, 3}, {2, 0}, {2, 1}, {2, 2}, {2, 3}, {3, 0}, {3, 2}, {3, 3}};
+
+constexpr double KB = 1.38064852e-23;
+constexpr double QE = 1.6021766208e-19;
+
+// MOS1 bias-independent prep (models/mos1.va:176-263) into the per-instance
+// constants the kernels read (esl::MosField).
+void mos1_fields(const double *p, double *f)
+{
+    const double tp = p[ESL_M_TYPE];
+    const unsigned given = (unsigned)p[ESL_M_GIVEN];
+    const double tnomk = p[ESL_M_TNOM] + 273.15;
+    const double vtnom = tnomk * (KB / QE);
+    const double egfet1 = 1.16 - (7.02e-4 * tnomk * tnomk) / (tnomk + 1108.0);
+    const double tox = p[ESL_M_TOX];
+    const double coxp = ((given & ESL_MG_TOX) && tox > 0.0) ? 3.9 * 8.854214871e-12 / tox : 0.0;
+    const double phix = p[ESL_M_PHI];
+    const double gammax = p[ESL_M_GAMMA];
+    double kpx;
+    if ((given & ESL_MG_KP) || coxp <= 0.0) kpx = p[ESL_M_KP];
+    else kpx = p[ESL_M_U0] * coxp * 1.0e-4;
+    const double tk = p[ESL_M_TEMP] + 273.15;
+    const double vt = tk * (KB / QE);
+    const double fact1 = tnomk / 300.15;
+    const double fact2 = tk / 300.15;
+    const double egfet = 1.16 - (7.02e-4 * tk * tk) / (tk + 1108.0);
+    const double arg1 = -egfet1 / (2.0 * KB * tnomk) + 1.1150877 / (2.0 * KB * 300.15);
+    const double argt = -egfet / (2.0 * KB * tk) + 1.1150877 / (2.0 * KB * 300.15);
+    const double pbfact1 = -2.0 * vtnom * (1.5 * esl::log(fact1) + QE * arg1);
+    const double pbfact = -2.0 * vt * (1.5 * esl::log(fact2) + QE * argt);
+    const double ratio4 = (tk / tnomk) * std::sqrt(tk / tnomk);
+    const double phio = (phix - pbfact1) / fact1;
+    const double phit = fact2 * phio + pbfact;
+    const double ist = p[ESL_M_IS] * esl::exp(-egfet / vt + egfet1 / vtnom);
+    double leff = p[ESL_M_L] - 2.0 * p[ESL_M_LD];
+    if (leff < 1.0e-10) leff = 1.0e-10;
+    const double m = p[ESL_M_M];
+    const double isat = ist * m;
+    f[esl::MF_TYPE] = tp;
+    f[esl::MF_VTO] = p[ESL_M_VTO];
+    f[esl::MF_VBIT_A] = tp * (gammax * std::sqrt(phix));
+    f[esl::MF_VBIT_B] = 0.5 * (egfet1 - egfet);
+    f[esl::MF_VBIT_C] = tp * 0.5 * (phit - phix);
+    f[esl::MF_GAMMA] = gammax;
+    f[esl::MF_PHIT] = phit;
+    f[esl::MF_LAMBDA] = p[ESL_M_LAMBDA];
+    f[esl::MF_VT] = vt;
+    f[esl::MF_ISATS] = isat;
+    f[esl::MF_ISATD] = isat;
+    f[esl::MF_VCRITS] = vt * esl::log(vt / (1.4142135623730951 * isat));
+    f[esl::MF_VCRITD] = vt * esl::log(vt / (1.4142135623730951 * isat));
+    f[esl::MF_GMIN] = p[ESL_M_GMIN];
+    f[esl::MF_KP] = kpx;
+    f[esl::MF_RATIO4] = ratio4;
+    f[esl::MF_M] = m;
+    f[esl::MF_W] = p[ESL_M_W];
+    f[esl::MF_LEFF] = leff;
+}
+
+struct Compiled {
+    u32 n = 0, nnz = 0, n_nodes = 0, n_live = 0;
+    vector<u32> col_ptr, row_idx, diag;
+    vector<uint8_t> current_row;
+    vector<Inst> inst;
+    vector<u32> q;
+    u32 kind_first[esl::kKinds + 1] = {};
+    // deck program host arrays
+    vector<esl::Chunk> chunks;
+    vector<u32> task_target, task_cell;
+    vector<uint16_t> cells;
+    vector<u32> r_t, v_t, i_t, m_t;
+    vector<double> r_val, v_val, i_val, m_par;
+    vector<int32_t> r_bind, v_bind, i_bind, m_bind_vto, m_bind_kp;
+    u32 n_mos = 0;
+};
+
+u32 find_slot(const Compiled &c, u32 r, u32 col)
+{
+    if (r == 0 || col == 0) return NONE;
+    auto b = c.row_idx.begin() + c.col_ptr[col], e = c.row_idx.begin() + c.col_ptr[col + 1];
+    auto it = std::lower_bound(b, e, r);
+    return (u32)(it - c.row_idx.begin());
+}
+
+bool compile_deck(const esl_deck *d, Compiled &c, std::string &err)
+{
+    if (!d || d->n_nodes < 1) { err = "deck has no nodes"; return false; }
+    if (d->n_dev && (!d->kind || !d->nodes || !d->params)) { err = "deck arrays missing"; return false; }
+    if (d->n_bind && (!d->bind_dev || !d->bind_param || !d->bind_live)) { err = "binding arrays missing"; return false; }
+    vector<vector<int32_t>> bind(d->n_dev, vector<int32_t>(NPARAM, -1));
+    for (u32 b = 0; b < d->n_bind; b++) {
+        if (d->bind_dev[b] >= d->n_dev || d->bind_param[b] >= NPARAM || d->bind_live[b] >= d->n_live) {
+            err = "binding out of range";
+            return false;
+        }
+        bind[d->bind_dev[b]][d->bind_param[b]] = (int32_t)d->bind_live[b];
+    }
+    for (u32 i = 0; i < d->n_dev; i++) {
+        const u32 k = d->kind[i];
+        if (k >= esl::kKinds) { err = "unknown device kind"; return false; }
+        if (k == esl::kD) { err = "diode lanes are not supported by the operating-point kernel"; return false; }
+        for (int t = 0; t < ESL_NTERM; t++)
+            if (d->nodes[i * ESL_NTERM + t] >= d->n_nodes) { err = "node out of range"; return false; }
+        for (int p = 0; p < NPARAM; p++) {
+            if (bind[i][p] < 0) continue;
+            bool ok = (k == esl::kR || k == esl::kC || k == esl::kV || k == esl::kI) ? p == 0
+                      : (k == esl::kMos) ? (p == ESL_M_VTO || p == ESL_M_KP) : false;
+            if (k == esl::kMos && p == ESL_M_KP) {
+                const double *pp = d->params + (size_t)i * NPARAM;
+                const unsigned given = (unsigned)pp[ESL_M_GIVEN];
+                if (!(given & ESL_MG_KP) && (given & ESL_MG_TOX) && pp[ESL_M_TOX] > 0) ok = false;
+            }
+            if (!ok) { err = "binding of a parameter the lanes cannot move per point"; return false; }
+        }
+        if (k == esl::kR && !(d->params[(size_t)i * NPARAM] != 0) && bind[i][0] < 0) { err = "zero resistance"; return false; }
+    }
+    c.n_nodes = d->n_nodes;
+    c.n_live = d->n_live;
+    u32 nbr = 0;
+    for (u32 k = 0; k < esl::kKinds; k++) {
+        c.kind_first[k] = (u32)c.inst.size();
+        for (u32 i = 0; i < d->n_dev; i++) {
+            if (d->kind[i] != k) continue;
+            Inst in{};
+            in.kind = k;
+            in.dev = i;
+            for (int t = 0; t < 4; t++) in.t[t] = d->nodes[i * 4 + t];
+            in.br = k == esl::kV ? d->n_nodes + nbr++ : 0;
+            c.inst.push_back(in);
+        }
+    }
+    c.kind_first[esl::kKinds] = (u32)c.inst.size();
+    c.n = d->n_nodes + nbr;
+    const u32 n = c.n;
+    c.current_row.assign(n, 0);
+    for (u32 i = d->n_nodes; i < n; i++) c.current_row[i] = 1;
+    vector<std::pair<u32, u32>> ent;   // (col, row)
+    for (u32 i = 0; i < n; i++) ent.push_back({i, i});
+    for (auto &in : c.inst) {
+        if (in.kind == esl::kR)
+            for (auto &e : R_ENT) ent.push_back({in.t[e[1]], in.t[e[0]]});
+        else if (in.kind == esl::kV) {
+            ent.push_back({in.br, in.t[0]});
+            ent.push_back({in.br, in.t[1]});
+            ent.push_back({in.t[0], in.br});
+            ent.push_back({in.t[1], in.br});
+        } else if (in.kind == esl::kMos)
+            for (auto &e : MOS_ENT) ent.push_back({in.t[e[1]], in.t[e[0]]});
+    }
+    vector<std::pair<u32, u32>> keep;
+    for (auto &e : ent)
+        if ((e.first != 0 && e.second != 0) || (e.first == 0 && e.second == 0)) keep.push_back(e);
+    std::sort(keep.begin(), keep.end());
+    keep.erase(std::unique(keep.begin(), keep.end()), keep.end());
+    c.nnz = (u32)keep.size();
+    c.col_ptr.assign(n + 1, 0);
+    c.row_idx.resize(c.nnz);
+    for (size_t e = 0; e < keep.size(); e++) {
+        c.col_ptr[keep[e].first + 1]++;
+        c.row_idx[e] = keep[e].second;
+    }
+    for (u32 j = 0; j < n; j++) c.col_ptr[j + 1] += c.col_ptr[j];
+    c.diag.resize(n);
+    for (u32 i = 0; i < n; i++) c.diag[i] = find_slot(c, i, i);
+    c.diag[0] = c.col_ptr[0];
+    for (auto &in : c.inst) {
+        for (u32 &s : in.slot) s = NONE;
+        if (in.kind == esl::kR)
+            for (int e = 0; e < 4; e++) in.slot[e] = find_slot(c, in.t[R_ENT[e][0]], in.t[R_ENT[e][1]]);
+        else if (in.kind == esl::kV) {
+            in.slot[0] = find_slot(c, in.t[0], in.br);
+            in.slot[1] = find_slot(c, in.t[1], in.br);
+            in.slot[2] = find_slot(c, in.br, in.t[0]);
+            in.slot[3] = find_slot(c, in.br, in.t[1]);
+        } else if (in.kind == esl::kMos)
+            for (int e = 0; e < 11; e++) in.slot[e] = find_slot(c, in.t[MOS_ENT[e][0]], in.t[MOS_ENT[e][1]]);
+    }
+    c.q = min_degree(n, c.col_ptr, c.row_idx);
+
+    // per-kind instance arrays
+    for (auto &in : c.inst) {
+        const double *p = d->params + (size_t)in.dev * NPARAM;
+        const auto &bd = bind[in.dev];
+        switch (in.kind) {
+        case esl::kR:
+            c.r_t.push_back(in.t[0]); c.r_t.push_back(in.t[1]);
+            c.r_val.push_back(p[0]); c.r_bind.push_back(bd[0]);
+            break;
+        case esl::kV:
+            c.v_t.push_back(in.t[0]); c.v_t.push_back(in.t[1]); c.v_t.push_back(in.br);
+            c.v_val.push_back(p[0]); c.v_bind.push_back(bd[0]);
+            break;
+        case esl::kI:
+            c.i_t.push_back(in.t[0]); c.i_t.push_back(in.t[1]);
+            c.i_val.push_back(p[0]); c.i_bind.push_back(bd[0]);
+            break;
+        case esl::kMos:
+            for (int t = 0; t < 4; t++) c.m_t.push_back(in.t[t]);
+            c.m_bind_vto.push_back(bd[ESL_M_VTO]);
+            c.m_bind_kp.push_back(bd[ESL_M_KP]);
+            break;
+        default:
+            break;
+        }
+    }
+    c.n_mos = c.kind_first[esl::kMos + 1] - c.kind_first[esl::kMos];
+    c.m_par.assign((size_t)esl::MF_COUNT * c.n_mos, 0.0);
+    for (u32 j = 0; j < c.n_mos; j++) {
+        const Inst &in = c.inst[c.kind_first[esl::kMos] + j];
+        double f[esl::MF_COUNT];
+        mos1_fields(d->params + (size_t)in.dev * NPARAM, f);
+        for (u32 k = 0; k < esl::MF_COUNT; k++) c.m_par[(size_t)k * c.n_mos + j] = f[k];
+    }
+    // chunks and reduce tasks
+    for (u32 k = 0; k < esl::kKinds; k++) {
+        const u32 b = c.kind_first[k], e = c.kind_first[k + 1];
+        if (k == esl::kC) continue;   // no DC stamps
+        for (u32 first = b; first < e; first += esl::kWave) {
+            const u32 cnt = std::min<u32>(esl::kWave, e - first);
+            esl::Chunk ch{k, first - b, cnt, (u32)c.task_target.size()};
+            // (target, cell) in instance then entry order
+            vector<std::pair<u32, u32>> tc;
+            for (u32 t = 0; t < cnt; t++) {
+                const Inst &in = c.inst[first + t];
+                auto jac = [&](u32 ei, u32 slot) {
+                    if (slot != NONE) tc.push_back({slot, ei * esl::kWave + t});
+                };
+                auto row = [&](u32 ei, u32 r) {
+                    if (r != 0) tc.push_back({esl::kRhsBit | r, ei * esl::kWave + t});
+                };
+                if (k == esl::kR) {
+                    for (u32 ei = 0; ei < 4; ei++) jac(ei, in.slot[ei]);
+                    row(4, in.t[0]);
+                    row(5, in.t[1]);
+                } else if (k == esl::kV) {
+                    for (u32 ei = 0; ei < 4; ei++) jac(ei, in.slot[ei]);
+                    row(4, in.t[0]);
+                    row(5, in.t[1]);
+                    row(6, in.br);
+                } else if (k == esl::kI) {
+                    row(0, in.t[0]);
+                    row(1, in.t[1]);
+                } else if (k == esl::kMos) {
+                    for (u32 ei = 0; ei < 11; ei++) jac(ei, in.slot[ei]);
+                    row(11, in.t[0]);
+                    row(12, in.t[2]);
+                    row(13, in.t[3]);
+                }
+            }
+            std::map<u32, vector<u32>> groups;
+            vector<u32> order;
+            for (auto &x : tc) {
+                auto it = groups.find(x.first);
+                if (it == groups.end()) {
+                    order.push_back(x.first);
+                    groups[x.first] = {x.second};
+                } else {
+                    it->second.push_back(x.second);
+                }
+            }
+            for (u32 tg : order) {
+                c.task_target.push_back(tg);
+                c.task_cell.push_back((u32)c.cells.size());
+                for (u32 cell : groups[tg]) c.cells.push_back((uint16_t)cell);
+            }
+            c.chunks.push_back(ch);
+        }
+    }
+    c.task_cell.push_back((u32)c.cells.size());
+    c.chunks.push_back(esl::Chunk{0, 0, 0, (u32)c.task_target.size()});
+    return true;
+}
+
+// ---------------------------------------------------------------- engine
+struct Engine {
+    u32 gen = 0;
+    Compiled c;
+    DevBuf deck_buf;
+    esl::DeckProg dp{};
+    vector<TapeHostProg *> tapes;
+    std::map<vector<u32>, u32> tape_index;
+    DevBuf tape_table;
+    // per-solve buffers
+    DevBuf live, x, lim, resume, status, iters, lane_ids, a_out, f_out;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    esl_timing timing{};
+    u32 lds_max = 0;
+    uint64_t lead_flops = 0;
+    u32 lead_levels = 0, lead_nl = 0, lead_nu = 0, lead_lds = 0;
+
+    ~Engine()
+    {
+        for (auto *t : tapes) delete t;
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+hipError_t upload_deck(Engine &e)
+{
+    Compiled &c = e.c;
+    Packer pk;
+    size_t o_chunks = pk.add(c.chunks), o_tt = pk.add(c.task_target), o_tc = pk.add(c.task_cell),
+           o_cells = pk.add(c.cells), o_diag = pk.add(c.diag), o_cur = pk.add(c.current_row),
+           o_rt = pk.add(c.r_t), o_rv = pk.add(c.r_val), o_rb = pk.add(c.r_bind), o_vt = pk.add(c.v_t),
+           o_vv = pk.add(c.v_val), o_vb = pk.add(c.v_bind), o_it = pk.add(c.i_t), o_iv = pk.add(c.i_val),
+           o_ib = pk.add(c.i_bind), o_mt = pk.add(c.m_t), o_mp = pk.add(c.m_par), o_mbv = pk.add(c.m_bind_vto),
+           o_mbk = pk.add(c.m_bind_kp);
+    hipError_t err = e.deck_buf.ensure(pk.blob.size());
+    if (err != hipSuccess) return err;
+    err = hipMemcpy(e.deck_buf.p, pk.blob.data(), pk.blob.size(), hipMemcpyHostToDevice);
+    if (err != hipSuccess) return err;
+    unsigned char *b = e.deck_buf.as<unsigned char>();
+    esl::DeckProg &dp = e.dp;
+    dp.n = c.n;
+    dp.nnz = c.nnz;
+    dp.n_nodes = c.n_nodes;
+    dp.n_live = c.n_live;
+    dp.n_chunks = (u32)c.chunks.size() - 1;
+    dp.n_mos = c.n_mos;
+    dp.diag0_slot = c.diag[0];
+    dp.chunks = reinterpret_cast<const esl::Chunk *>(b + o_chunks);
+    dp.task_target = reinterpret_cast<const u32 *>(b + o_tt);
+    dp.task_cell = reinterpret_cast<const u32 *>(b + o_tc);
+    dp.cells = reinterpret_cast<const uint16_t *>(b + o_cells);
+    dp.diag_slot = reinterpret_cast<const u32 *>(b + o_diag);
+    dp.current_row = reinterpret_cast<const uint8_t *>(b + o_cur);
+    dp.r_t = reinterpret_cast<const u32 *>(b + o_rt);
+    dp.r_val = reinterpret_cast<const double *>(b + o_rv);
+    dp.r_bind = reinterpret_cast<const int32_t *>(b + o_rb);
+    dp.v_t = reinterpret_cast<const u32 *>(b + o_vt);
+    dp.v_val = reinterpret_cast<const double *>(b + o_vv);
+    dp.v_bind = reinterpret_cast<const int32_t *>(b + o_vb);
+    dp.i_t = reinterpret_cast<const u32 *>(b + o_it);
+    dp.i_val = reinterpret_cast<const double *>(b + o_iv);
+    dp.i_bind = reinterpret_cast<const int32_t *>(b + o_ib);
+    dp.m_t = reinterpret_cast<const u32 *>(b + o_mt);
+    dp.m_par = reinterpret_cast<const double *>(b + o_mp);
+    dp.m_bind_vto = reinterpret_cast<const int32_t *>(b + o_mbv);
+    dp.m_bind_kp = reinterpret_cast<const int32_t *>(b + o_mbk);
+    for (u32 k = 0; k <= esl::kKinds; k++) dp.kind_first[k] = c.kind_first[k];
+    return hipSuccess;
+}
+
+u32 lds_bytes_for(u32 tv_cap, u32 n)
+{
+    return (u32)((size_t)(tv_cap + 5 * n + esl::kWave * esl::kStageEntries) * sizeof(double));
+}
+
+
+// ---------------------------------------------------------------- validation
+// Every index a kernel reads is checked on the host before the first launch,
+// so a malformed program is an error here rather than a GPU memory fault.
+bool validate_tape(const TapeHostProg &o, u32 n, u32 nnz)
+{
+    const esl::TapeProg &P = o.prog;
+    if (o.amap.size() != nnz || o.pinv.size() != n || o.qinv.size() != n || o.flags.size() != n) return false;
+    for (u32 v : o.amap) if (v >= P.n_tv) return false;
+    for (u32 v : o.pinv) if (v >= n) return false;
+    for (u32 v : o.qinv) if (v >= n) return false;
+    if (o.up.size() != n + 1 || o.lp.size() != n + 1 || o.fptr.size() != n + 1) return false;
+    if (o.up[n] != P.n_u || o.lp[n] != P.n_l || o.fptr[n] != o.dst.size()) return false;
+    for (u32 k = 0; k < n; k++)
+        if (o.up[k] > o.up[k + 1] || o.lp[k] > o.lp[k + 1] || o.fptr[k] > o.fptr[k + 1]) return false;
+    for (u32 v : o.ui) if (v >= n) return false;
+    for (u32 v : o.dst) if (v >= P.n_tv) return false;
+    if (o.lev_ptr.size() != P.n_levels + 1 || o.lev_ptr[P.n_levels] != n || o.lev_cols.size() != n) return false;
+    for (u32 v : o.lev_cols) if (v >= n) return false;
+    if (o.flev_ptr.size() != P.n_flev + 1 || o.flev_ptr[P.n_flev] != n) return false;
+    if (o.blev_ptr.size() != P.n_blev + 1 || o.blev_ptr[P.n_blev] != n) return false;
+    for (u32 v : o.flev_rows) if (v >= n) return false;
+    for (u32 v : o.blev_rows) if (v >= n) return false;
+    if (o.lrow_ptr.size() != n + 1 || o.urow_ptr.size() != n + 1) return false;
+    if (o.lrow_ptr[n] != o.lrow_k.size() || o.urow_ptr[n] != o.urow_k.size()) return false;
+    for (u32 v : o.lrow_k) if (v >= n) return false;
+    for (u32 v : o.urow_k) if (v >= n) return false;
+    for (u32 v : o.lrow_slot) if (v >= P.n_tv) return false;
+    for (u32 v : o.urow_slot) if (v >= P.n_tv) return false;
+    for (u32 v : o.void_slots) if (v >= nnz) return false;
+    if (P.lx0 + P.n_l > P.n_tv || P.n_u + n > P.n_tv) return false;
+    return true;
+}
+
+bool validate_deck(const Compiled &c)
+{
+    const u32 n = c.n, nnz = c.nnz;
+    if (c.diag.size() != n || c.diag[0] >= nnz) return false;
+    for (u32 v : c.diag) if (v >= nnz) return false;
+    if (c.chunks.empty() || c.task_cell.size() != c.task_target.size() + 1) return false;
+    if (c.chunks.back().task_begin != c.task_target.size()) return false;
+    for (size_t k = 0; k + 1 < c.chunks.size(); k++) {
+        const esl::Chunk &ch = c.chunks[k];
+        if (ch.count == 0 || ch.count > (u32)esl::kWave) return false;
+        if (ch.task_begin > c.chunks[k + 1].task_begin) return false;
+        const u32 kind_n = c.kind_first[ch.kind + 1] - c.kind_first[ch.kind];
+        if (ch.first + ch.count > kind_n) return false;
+    }
+    for (u32 t : c.task_target) {
+        if (t & esl::kRhsBit) {
+            if ((t & ~esl::kRhsBit) >= n) return false;
+        } else if (t >= nnz) {
+            return false;
+        }
+    }
+    for (u32 v : c.cells) if (v >= (u32)(esl::kWave * esl::kStageEntries)) return false;
+    auto terms_ok = [&](const vector<u32> &t) {
+        for (u32 v : t) if (v >= n) return false;
+        return true;
+    };
+    if (!terms_ok(c.r_t) || !terms_ok(c.v_t) || !terms_ok(c.i_t) || !terms_ok(c.m_t)) return false;
+    auto binds_ok = [&](const vector<int32_t> &b) {
+        for (int32_t v : b) if (v >= (int32_t)c.n_live) return false;
+        return true;
+    };
+    return binds_ok(c.r_bind) && binds_ok(c.v_bind) && binds_ok(c.i_bind) && binds_ok(c.m_bind_vto) &&
+           binds_ok(c.m_bind_kp);
+}
+
+u32 device_lds_max()
+{
+    static u32 cached = 0;
+    if (cached) return cached;
+    int v = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 65536;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0) v = 65536;
+    cached = (u32)v;
+    return cached;
+}
+
+// ---------------------------------------------------------------- handles
+struct Slot {
+    u32 gen = 0;
+    void *ptr = nullptr;
+    int type = 0;   // 1 engine, 2 tape
+};
+std::mutex g_mu;
+vector<Slot> g_slots;
+
+void *make_handle(void *obj, int type)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (u32 i = 0; i < g_slots.size(); i++)
+        if (!g_slots[i].ptr && g_slots[i].gen < 0xfffff) {
+            g_slots[i].gen++;
+            g_slots[i].ptr = obj;
+            g_slots[i].type = type;
+            return reinterpret_cast<void *>((uintptr_t)(((uint64_t)g_slots[i].gen << 20) | (i + 1)));
+        }
+    g_slots.push_back(Slot{1, obj, type});
+    return reinterpret_cast<void *>((uintptr_t)(((uint64_t)1 << 20) | g_slots.size()));
+}
+
+void *lookup(void *h, int type)
+{
+    if (!h) return nullptr;
+    const uint64_t v = (uint64_t)(uintptr_t)h;
+    const u32 idx = (u32)(v & 0xfffff), gen = (u32)(v >> 20);
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (idx == 0 || idx > g_slots.size()) return nullptr;
+    Slot &s = g_slots[idx - 1];
+    if (s.gen != gen || !s.ptr || s.type != type) return nullptr;
+    return s.ptr;
+}
+
+void *release(void *h, int type)
+{
+    if (!h) return nullptr;
+    const uint64_t v = (uint64_t)(uintptr_t)h;
+    const u32 idx = (u32)(v & 0xfffff), gen = (u32)(v >> 20);
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (idx == 0 || idx > g_slots.size()) return nullptr;
+    Slot &s = g_slots[idx - 1];
+    if (s.gen != gen || !s.ptr || s.type != type) return nullptr;
+    void *p = s.ptr;
+    s.ptr = nullptr;
+    return p;
+}
+
+void put_diag(char *diag, size_t cap, const std::string &msg)
+{
+    if (diag && cap) {
+        size_t n = std::min(cap - 1, msg.size());
+        memcpy(diag, msg.data(), n);
+        diag[n] = 0;
+    }
+}
+
+bool device_present()
+{
+    int count = 0;
+    return hipGetDeviceCount(&count) == hipSuccess && count > 0;
+}
+
+// Registers a tape program (deduplicated by pivot sequence and flags).
+esl_status add_tape(Engine &e, const HostTape &t, u32 &id)
+{
+    vector<u32> key = t.pinv;
+    key.insert(key.end(), t.flags.begin(), t.flags.end());
+    auto it = e.tape_index.find(key);
+    if (it != e.tape_index.end()) {
+        id = it->second;
+        return ESL_OK;
+    }
+    auto *tp = new TapeHostProg();
+    build_tape_prog(t, e.c.col_ptr, 1e-12, 1.0 / 1e-3, *tp);
+    if (!validate_tape(*tp, e.c.n, e.c.nnz)) {
+        delete tp;
+        g_last_error = "tape program failed validation";
+        return ESL_FAILED;
+    }
+    if (lds_bytes_for(tp->prog.n_tv, e.c.n) > device_lds_max()) {
+        delete tp;
+        g_last_error = "lane state exceeds the LDS of one workgroup";
+        return ESL_FAILED;
+    }
+    hipError_t er = upload_tape_prog(*tp);
+    if (er != hipSuccess) {
+        delete tp;
+        return fail_hip(er, "tape upload");
+    }
+    id = (u32)e.tapes.size();
+    e.tapes.push_back(tp);
+    e.tape_index[key] = id;
+    vector<esl::TapeProg> table;
+    for (auto *p : e.tapes) table.push_back(p->prog);
+    er = e.tape_table.ensure(table.size() * sizeof(esl::TapeProg));
+    if (er != hipSuccess) return fail_hip(er, "tape table");
+    er = hipMemcpy(e.tape_table.p, table.data(), table.size() * sizeof(esl::TapeProg), hipMemcpyHostToDevice);
+    if (er != hipSuccess) return fail_hip(er, "tape table");
+    e.lds_max = std::max(e.lds_max, lds_bytes_for(tp->prog.n_tv, e.c.n));
+    return ESL_OK;
+}
+
+void clear_tapes(Engine &e)
+{
+    for (auto *t : e.tapes) delete t;
+    e.tapes.clear();
+    e.tape_index.clear();
+    e.lds_max = 0;
+}
+
+}  // namespace
+
+// ============================================================== C ABI
+extern "C" {
+
+uint32_t esl_abi_version(void) { return ESL_ABI_VERSION; }
+
+void esl_default_tolerances(esl_tolerances *t)
+{
+    if (!t) return;
+    t->reltol = 1e-3;
+    t->abstol = 1e-12;
+    t->vntol = 1e-6;
+    t->residual_tol = 1e-9;
+    t->max_iter = 100;
+    t->reserved = 0;
+}
+
+void esl_default_op_options(esl_op_options *o)
+{
+    if (!o) return;
+    memset(o, 0, sizeof(*o));
+    o->struct_size = sizeof(*o);
+    esl_default_tolerances(&o->tol);
+    o->precision = ESL_FP64;
+    o->start = ESL_START_COLD;
+    o->refine_tol = 1e-12;
+    o->refine_max = 8;
+}
+
+esl_status esl_engine_create(const esl_deck *deck, esl_engine **out, char *diag, size_t cap)
+{
+    if (!out) return ESL_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (!device_present()) {
+        put_diag(diag, cap, "no HIP device");
+        return ESL_DEVICE_UNAVAILABLE;
+    }
+    auto *e = new Engine();
+    std::string err;
+    if (!compile_deck(deck, e->c, err)) {
+        put_diag(diag, cap, err);
+        delete e;
+        return ESL_INVALID_ARGUMENT;
+    }
+    if (!validate_deck(e->c)) {
+        put_diag(diag, cap, "deck program failed validation");
+        delete e;
+        return ESL_FAILED;
+    }
+    if (lds_bytes_for(e->c.nnz + 1, e->c.n) > device_lds_max()) {
+        put_diag(diag, cap, "lane state exceeds the LDS of one workgroup");
+        delete e;
+        return ESL_FAILED;
+    }
+    hipError_t er = upload_deck(*e);
+    if (er == hipSuccess) er = hipStreamCreate(&e->stream);
+    if (er == hipSuccess) er = hipEventCreate(&e->ev0);
+    if (er == hipSuccess) er = hipEventCreate(&e->ev1);
+    if (er != hipSuccess) {
+        put_diag(diag, cap, hipGetErrorString(er));
+        delete e;
+        return ESL_DEVICE_UNAVAILABLE;
+    }
+    *out = static_cast<esl_engine *>(make_handle(e, 1));
+    return ESL_OK;
+}
+
+esl_status esl_engine_destroy(esl_engine *h)
+{
+    if (!h) return ESL_OK;
+    auto *e = static_cast<Engine *>(release(h, 1));
+    if (!e) return ESL_INVALID_HANDLE;
+    delete e;
+    return ESL_OK;
+}
+
+esl_status esl_engine_info_get(esl_engine *h, esl_engine_info *out)
+{
+    auto *e = static_cast<Engine *>(lookup(h, 1));
+    if (!e) return ESL_INVALID_HANDLE;
+    if (!out) return ESL_INVALID_ARGUMENT;
+    memset(out, 0, sizeof(*out));
+    out->n = e->c.n;
+    out->nnz = e->c.nnz;
+    out->n_nodes = e->c.n_nodes;
+    out->n_live = e->c.n_live;
+    out->nnz_l = e->lead_nl;
+    out->nnz_u = e->lead_nu;
+    out->flops = e->lead_flops;
+    out->levels = e->lead_levels;
+    out->lds_bytes = e->lead_lds;
+    out->tapes = (u32)e->tapes.size();
+    return ESL_OK;
+}
+
+esl_status esl_engine_pattern(esl_engine *h, uint32_t *col_ptr, uint32_t *row_idx, uint32_t *q)
+{
+    auto *e = static_cast<Engine *>(lookup(h, 1));
+    if (!e) return ESL_INVALID_HANDLE;
+    if (col_ptr) memcpy(col_ptr, e->c.col_ptr.data(), (e->c.n + 1) * 4);
+    if (row_idx) memcpy(row_idx, e->c.row_idx.data(), e->c.nnz * 4);
+    if (q) memcpy(q, e->c.q.data(), e->c.n * 4);
+    return ESL_OK;
+}
+
+esl_status esl_last_timing(esl_engine *h, esl_timing *out)
+{
+    auto *e = static_cast<Engine *>(lookup(h, 1));
+    if (!e) return ESL_INVALID_HANDLE;
+    if (!out) return ESL_INVALID_ARGUMENT;
+    *out = e->timing;
+    return ESL_OK;
+}
+
+esl_status esl_op_solve(esl_engine *h, const double *values, size_t count, const esl_op_options *opt,
+                        double *x, uint32_t *iterations, uint32_t *reasons)
+{
+    auto *e = static_cast<Engine *>(lookup(h, 1));
+    if (!e) return ESL_INVALID_HANDLE;
+    if (count == 0 || count > 0x7fffffff) return ESL_INVALID_ARGUMENT;
+    if (!x || !iterations || !reasons) return ESL_INVALID_ARGUMENT;
+    if (e->c.n_live && !values) return ESL_INVALID_ARGUMENT;
+    esl_op_options o;
+    esl_default_op_options(&o);
+    if (opt) {
+        if (opt->struct_size < sizeof(esl_op_options)) return ESL_ABI_MISMATCH;
+        o = *opt;
+    }
+    if (o.start != ESL_START_COLD && o.start != ESL_START_GIVEN) return ESL_INVALID_ARGUMENT;
+    if (o.start == ESL_START_GIVEN && !o.x0) return ESL_INVALID_ARGUMENT;
+    if (o.precision != ESL_FP64) return ESL_INVALID_ARGUMENT;
+    if (o.tol.max_iter == 0) return ESL_INVALID_ARGUMENT;
+    for (size_t i = 0; i < count * e->c.n_live; i++)
+        if (!std::isfinite(values[i])) return ESL_INVALID_ARGUMENT;
+    const auto t_start = std::chrono::steady_clock::now();
+    const Compiled &c = e->c;
+    const u32 n = c.n, nnz = c.nnz, P = c.n_live;
+    e->timing = esl_timing{};
+    HIPCHK(e->live.ensure(std::max<size_t>(1, count * P) * 8));
+    HIPCHK(e->x.ensure(count * n * 8));
+    HIPCHK(e->lim.ensure(std::max<size_t>(1, count * 4 * c.n_mos) * 8));
+    HIPCHK(e->resume.ensure(count * sizeof(esl::LaneResume)));
+    HIPCHK(e->status.ensure(count * 4));
+    HIPCHK(e->iters.ensure(count * 4));
+    HIPCHK(e->lane_ids.ensure(count * 4));
+    hipStream_t s = e->stream;
+    if (P) HIPCHK(hipMemcpyAsync(e->live.p, values, count * P * 8, hipMemcpyHostToDevice, s));
+    if (o.start == ESL_START_GIVEN) HIPCHK(hipMemcpyAsync(e->x.p, o.x0, count * n * 8, hipMemcpyHostToDevice, s));
+    vector<u32> ids(count);
+    for (size_t i = 0; i < count; i++) ids[i] = (u32)i;
+    HIPCHK(hipMemcpyAsync(e->lane_ids.p, ids.data(), count * 4, hipMemcpyHostToDevice, s));
+
+    HostOpArgs a{};
+    a.dp = e->dp;
+    a.live = e->live.as<double>();
+    a.x = e->x.as<double>();
+    a.lim = e->lim.as<double>();
+    a.resume = e->resume.as<esl::LaneResume>();
+    a.status = e->status.as<u32>();
+    a.iterations = e->iters.as<u32>();
+    a.reltol = o.tol.reltol;
+    a.abstol = o.tol.abstol;
+    a.vntol = o.tol.vntol;
+    a.residual_tol = o.tol.residual_tol;
+    a.max_iter = o.tol.max_iter;
+
+    // Lead tape: point 0's first Jacobian, factored on the host.
+    clear_tapes(*e);
+    {
+        HIPCHK(e->a_out.ensure(std::max<u32>(nnz, 1) * 8));
+        HIPCHK(e->f_out.ensure(std::max<u32>(n, 1) * 8));
+        HostOpArgs d = a;
+        d.lane_ids = e->lane_ids.as<u32>();
+        d.mode = o.start == ESL_START_GIVEN ? 1 : 0;
+        d.dump = 1;
+        d.tv_cap = nnz + 1;
+        d.a_out = e->a_out.as<double>();
+        d.f_out = e->f_out.as<double>();
+        const u32 lds = lds_bytes_for(nnz + 1, n);
+        HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
+        HIPCHK(esl_launch_op_newton(&d, 1, lds, s));
+        vector<double> a0(nnz);
+        HIPCHK(hipMemcpyAsync(a0.data(), e->a_out.p, nnz * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        HostTape t;
+        if (!gp_factor(n, c.col_ptr.data(), c.row_idx.data(), a0.data(), c.q.data(), 1e-3, t)) {
+            for (size_t l = 0; l < count; l++) {
+                reasons[l] = ESL_REASON_SINGULAR_MATRIX;
+                iterations[l] = 0;
+            }
+            memset(x, 0, count * n * 8);
+            return ESL_OK;
+        }
+        u32 id;
+        esl_status st = add_tape(*e, t, id);
+        if (st != ESL_OK) return st;
+        e->lead_nl = t.nnz_l;
+        e->lead_nu = t.nnz_u;
+        e->lead_flops = e->tapes[0]->flops;
+        e->lead_levels = e->tapes[0]->prog.n_levels;
+        e->lead_lds = lds_bytes_for(e->tapes[0]->prog.n_tv, n);
+    }
+    vector<esl::LaneResume> rs(count);
+    for (auto &r : rs) r = esl::LaneResume{0, 0, 0, 0, 1e9};
+    HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
+
+    vector<u32> st(count), its(count);
+    vector<u32> active = ids;
+    u32 mode = o.start == ESL_START_GIVEN ? 1 : 0;
+    float kernel_ms = 0;
+    for (int round = 0; !active.empty(); round++) {
+        if (round > 0) HIPCHK(hipMemcpyAsync(e->lane_ids.p, active.data(), active.size() * 4, hipMemcpyHostToDevice, s));
+        a.tapes = e->tape_table.as<esl::TapeProg>();
+        a.lane_ids = e->lane_ids.as<u32>();
+        a.mode = mode;
+        a.dump = 0;
+        u32 tv_cap = 0;
+        for (auto *t : e->tapes) tv_cap = std::max(tv_cap, t->prog.n_tv);
+        a.tv_cap = tv_cap;
+        const u32 lds = lds_bytes_for(tv_cap, n);
+        HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
+        HIPCHK(hipEventRecord(e->ev0, s));
+        HIPCHK(esl_launch_op_newton(&a, (u32)active.size(), lds, s));
+        HIPCHK(hipEventRecord(e->ev1, s));
+        HIPCHK(hipMemcpyAsync(st.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+        kernel_ms += ms;
+        e->timing.launches++;
+        vector<u32> again;
+        for (u32 l : active)
+            if (st[l] == esl::kRepivot) again.push_back(l);
+        if (again.empty() || round > 1000) break;
+        // dump the re-pivoting lanes' Jacobians at their saved iterates
+        HIPCHK(e->a_out.ensure(again.size() * nnz * 8));
+        HIPCHK(e->f_out.ensure(again.size() * n * 8));
+        HIPCHK(hipMemcpyAsync(e->lane_ids.p, again.data(), again.size() * 4, hipMemcpyHostToDevice, s));
+        HostOpArgs d = a;
+        d.mode = 2;
+        d.dump = 1;
+        d.tv_cap = std::max(tv_cap, nnz + 1);
+        d.a_out = e->a_out.as<double>();
+        d.f_out = e->f_out.as<double>();
+        const u32 dl = lds_bytes_for(d.tv_cap, n);
+        HIPCHK(esl_kernel_attrs(std::max<u32>(dl, 65536)));
+        HIPCHK(esl_launch_op_newton(&d, (u32)again.size(), dl, s));
+        vector<double> A(again.size() * (size_t)nnz);
+        HIPCHK(hipMemcpyAsync(A.data(), e->a_out.p, A.size() * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(rs.data(), e->resume.p, count * sizeof(esl::LaneResume), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        vector<u32> next;
+        for (size_t k = 0; k < again.size(); k++) {
+            const u32 l = again[k];
+            HostTape t;
+            e->timing.repivots++;
+            if (!gp_factor(n, c.col_ptr.data(), c.row_idx.data(), A.data() + k * nnz, c.q.data(), 1e-3, t)) {
+                st[l] = esl::kSingular;
+                its[l] = rs[l].iter + 1;
+                continue;
+            }
+            u32 id;
+            esl_status s2 = add_tape(*e, t, id);
+            if (s2 != ESL_OK) return s2;
+            rs[l].tape = id;
+            next.push_back(l);
+        }
+        HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
+        // singular lanes: record their status on the device copy too
+        HIPCHK(hipMemcpyAsync(e->status.p, st.data(), count * 4, hipMemcpyHostToDevice, s));
+        active = next;
+        mode = 2;
+    }
+    HIPCHK(hipMemcpyAsync(x, e->x.p, count * n * 8, hipMemcpyDeviceToHost, s));
+    vector<u32> dev_its(count);
+    HIPCHK(hipMemcpyAsync(dev_its.data(), e->iters.p, count * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(st.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    uint64_t lane_it = 0;
+    for (size_t l = 0; l < count; l++) {
+        iterations[l] = dev_its[l];
+        switch (st[l]) {
+        case esl::kConverged: reasons[l] = ESL_REASON_OK; break;
+        case esl::kSingular: reasons[l] = ESL_REASON_SINGULAR_MATRIX; iterations[l] = its[l]; break;
+        case esl::kNoConvergence: reasons[l] = ESL_REASON_NO_CONVERGENCE; break;
+        default: reasons[l] = ESL_REASON_OTHER; break;
+        }
+        lane_it += iterations[l];
+    }
+    e->timing.kernel_ms = kernel_ms;
+    e->timing.lane_iterations = lane_it;
+    e->timing.total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    return ESL_OK;
+}
+
+// ---------------------------------------------------------------- tape boundary
+struct TapeObj {
+    u32 n = 0, nnz = 0;
+    vector<u32> col_ptr;
+    HostTape t;
+    TapeHostProg prog;
+    hipStream_t stream = nullptr;
+    DevBuf a, rhs, lx, ux, udiag, dx, fail;
+    ~TapeObj()
+    {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+esl_status esl_tape_create(uint32_t n, const uint32_t *col_ptr, const uint32_t *row_idx, const double *lead_vals,
+                           const uint32_t *q, double pivot_tol, esl_tape **out)
+{
+    if (!out || !col_ptr || !row_idx || !lead_vals || !q || n == 0) return ESL_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (!(pivot_tol >= 0 && pivot_tol <= 1)) return ESL_INVALID_ARGUMENT;
+    if (!device_present()) return ESL_DEVICE_UNAVAILABLE;
+    vector<char> seen(n, 0);
+    for (u32 i = 0; i < n; i++) {
+        if (q[i] >= n || seen[q[i]]) return ESL_INVALID_ARGUMENT;
+        seen[q[i]] = 1;
+    }
+    for (u32 j = 0; j < n; j++)
+        if (col_ptr[j + 1] < col_ptr[j]) return ESL_INVALID_ARGUMENT;
+    for (u32 p = 0; p < col_ptr[n]; p++)
+        if (row_idx[p] >= n) return ESL_INVALID_ARGUMENT;
+    auto *t = new TapeObj();
+    t->n = n;
+    t->nnz = col_ptr[n];
+    t->col_ptr.assign(col_ptr, col_ptr + n + 1);
+    if (!gp_factor(n, col_ptr, row_idx, lead_vals, q, pivot_tol, t->t)) {
+        delete t;
+        return ESL_FAILED;
+    }
+    build_tape_prog(t->t, t->col_ptr, 1e-12, 1.0 / pivot_tol, t->prog);
+    if (!validate_tape(t->prog, n, t->nnz) || (size_t)(t->prog.prog.n_tv + 2 * n) * 8 > device_lds_max()) {
+        delete t;
+        return ESL_FAILED;
+    }
+    hipError_t er = upload_tape_prog(t->prog);
+    if (er == hipSuccess) er = hipStreamCreate(&t->stream);
+    if (er != hipSuccess) {
+        delete t;
+        return ESL_DEVICE_UNAVAILABLE;
+    }
+    *out = static_cast<esl_tape *>(make_handle(t, 2));
+    return ESL_OK;
+}
+
+esl_status esl_tape_destroy(esl_tape *h)
+{
+    if (!h) return ESL_OK;
+    auto *t = static_cast<TapeObj *>(release(h, 2));
+    if (!t) return ESL_INVALID_HANDLE;
+    delete t;
+    return ESL_OK;
+}
+
+esl_status esl_tape_sizes(esl_tape *h, uint32_t *nnz_l, uint32_t *nnz_u, uint64_t *flops, uint32_t *levels)
+{
+    auto *t = static_cast<TapeObj *>(lookup(h, 2));
+    if (!t) return ESL_INVALID_HANDLE;
+    if (nnz_l) *nnz_l = t->t.nnz_l;
+    if (nnz_u) *nnz_u = t->t.nnz_u;
+    if (flops) *flops = t->prog.flops;
+    if (levels) *levels = t->prog.prog.n_levels;
+    return ESL_OK;
+}
+
+esl_status esl_tape_export(esl_tape *h, uint32_t *pinv, uint32_t *lp, uint32_t *li, uint32_t *up, uint32_t *ui,
+                           uint32_t *flags, uint32_t *prow)
+{
+    auto *t = static_cast<TapeObj *>(lookup(h, 2));
+    if (!t) return ESL_INVALID_HANDLE;
+    const HostTape &T = t->t;
+    if (pinv) memcpy(pinv, T.pinv.data(), t->n * 4);
+    if (lp) memcpy(lp, T.lp.data(), (t->n + 1) * 4);
+    if (li) memcpy(li, T.li.data(), T.nnz_l * 4);
+    if (up) memcpy(up, T.up.data(), (t->n + 1) * 4);
+    if (ui) memcpy(ui, T.ui.data(), T.nnz_u * 4);
+    if (flags) memcpy(flags, T.flags.data(), t->n * 4);
+    if (prow) memcpy(prow, T.prow.data(), t->nnz * 4);
+    return ESL_OK;
+}
+
+esl_status esl_tape_replay(esl_tape *h, const double *a, const double *rhs, size_t lanes, double growth, double *lx,
+                           double *ux, double *udiag, double *dx, uint32_t *fail)
+{
+    auto *t = static_cast<TapeObj *>(lookup(h, 2));
+    if (!t) return ESL_INVALID_HANDLE;
+    if (!a || !rhs || lanes == 0 || lanes > 0x7fffffff) return ESL_INVALID_ARGUMENT;
+    const u32 n = t->n, nnz = t->nnz;
+    const esl::TapeProg &P = t->prog.prog;
+    HIPCHK(t->a.ensure(lanes * nnz * 8));
+    HIPCHK(t->rhs.ensure(lanes * n * 8));
+    HIPCHK(t->lx.ensure(lanes * std::max<u32>(P.n_l, 1) * 8));
+    HIPCHK(t->ux.ensure(lanes * std::max<u32>(P.n_u, 1) * 8));
+    HIPCHK(t->udiag.ensure(lanes * n * 8));
+    HIPCHK(t->dx.ensure(lanes * n * 8));
+    HIPCHK(t->fail.ensure(lanes * 4));
+    hipStream_t s = t->stream;
+    HIPCHK(hipMemcpyAsync(t->a.p, a, lanes * nnz * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(t->rhs.p, rhs, lanes * n * 8, hipMemcpyHostToDevice, s));
+    HostReplayArgs r{};
+    r.T = P;
+    r.T.growth = growth;
+    r.n = n;
+    r.nnz = nnz;
+    r.a = t->a.as<double>();
+    r.rhs = t->rhs.as<double>();
+    r.lx = t->lx.as<double>();
+    r.ux = t->ux.as<double>();
+    r.udiag = t->udiag.as<double>();
+    r.dx = t->dx.as<double>();
+    r.fail = t->fail.as<u32>();
+    const u32 lds = (u32)((size_t)(P.n_tv + 2 * n) * 8);
+    HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
+    HIPCHK(esl_launch_tape_replay(&r, (u32)lanes, lds, s));
+    if (lx && P.n_l) HIPCHK(hipMemcpyAsync(lx, t->lx.p, lanes * P.n_l * 8, hipMemcpyDeviceToHost, s));
+    if (ux && P.n_u) HIPCHK(hipMemcpyAsync(ux, t->ux.p, lanes * P.n_u * 8, hipMemcpyDeviceToHost, s));
+    if (udiag) HIPCHK(hipMemcpyAsync(udiag, t->udiag.p, lanes * n * 8, hipMemcpyDeviceToHost, s));
+    if (dx) HIPCHK(hipMemcpyAsync(dx, t->dx.p, lanes * n * 8, hipMemcpyDeviceToHost, s));
+    if (fail) HIPCHK(hipMemcpyAsync(fail, t->fail.p, lanes * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return ESL_OK;
+}
+
+}  // extern "C"

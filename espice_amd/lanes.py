@@ -1,0 +1,252 @@
+"""Python host mirror of the lane engine's C ABI (include/espice_lanes.h).
+
+`Engine` plays the role of the reference's points API for one template
+(include/espice.h:244-360: a deck with live parameters, rows of values in,
+one result per point out) and of `GpuLanes` underneath it
+(src/analysis/gpu_lanes.zig:251-1027). `Tape` is the LU boundary alone
+(`arp_lane_refactor` + `arp_lane_solve`, src/solver/lane_device.zig:26-38).
+
+There is no fallback: without the built library or without a HIP device the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libespice_lanes.so")
+
+OK, INVALID_ARGUMENT, BUFFER_TOO_SMALL, OUT_OF_MEMORY = 0, 1, 2, 3
+FAILED, ABI_MISMATCH, INVALID_HANDLE, DEVICE_UNAVAILABLE = 6, 7, 8, 9
+REASON_OK, REASON_NO_CONVERGENCE, REASON_SINGULAR = 0, 1, 3
+FP64, FP32_REFINED = 0, 1
+START_COLD, START_GIVEN = 0, 1
+
+STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "BUFFER_TOO_SMALL", 3: "OUT_OF_MEMORY",
+                4: "INVALID_QUERY", 5: "RESULT_UNAVAILABLE", 6: "FAILED", 7: "ABI_MISMATCH",
+                8: "INVALID_HANDLE", 9: "DEVICE_UNAVAILABLE"}
+
+EXPORTS = [
+    "esl_abi_version", "esl_default_tolerances", "esl_default_op_options", "esl_engine_create",
+    "esl_engine_destroy", "esl_engine_info_get", "esl_engine_pattern", "esl_op_solve", "esl_last_timing",
+    "esl_tape_create", "esl_tape_destroy", "esl_tape_sizes", "esl_tape_export", "esl_tape_replay",
+]
+
+
+class EslError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        super().__init__(f"{what}: {STATUS_NAMES.get(status, status)}")
+        self.status = status
+
+
+class _Deck(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_uint32), ("n_dev", C.c_uint32),
+        ("kind", C.c_void_p), ("nodes", C.c_void_p), ("params", C.c_void_p),
+        ("n_live", C.c_uint32), ("n_bind", C.c_uint32),
+        ("bind_dev", C.c_void_p), ("bind_param", C.c_void_p), ("bind_live", C.c_void_p),
+    ]
+
+
+class Tolerances(C.Structure):
+    _fields_ = [("reltol", C.c_double), ("abstol", C.c_double), ("vntol", C.c_double),
+                ("residual_tol", C.c_double), ("max_iter", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class OpOptions(C.Structure):
+    _fields_ = [("struct_size", C.c_size_t), ("tol", Tolerances), ("precision", C.c_uint32),
+                ("start", C.c_uint32), ("x0", C.c_void_p), ("refine_tol", C.c_double),
+                ("refine_max", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class EngineInfo(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("nnz", C.c_uint32), ("n_nodes", C.c_uint32), ("n_live", C.c_uint32),
+                ("nnz_l", C.c_uint32), ("nnz_u", C.c_uint32), ("flops", C.c_uint64), ("levels", C.c_uint32),
+                ("lds_bytes", C.c_uint32), ("tapes", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("launches", C.c_uint32),
+                ("repivots", C.c_uint32), ("lane_iterations", C.c_uint64)]
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Loads the built library; raises when it is missing (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp, u32p = C.c_void_p, C.POINTER(C.c_uint32)
+    L.esl_abi_version.restype = C.c_uint32
+    L.esl_default_op_options.argtypes = [C.POINTER(OpOptions)]
+    L.esl_engine_create.restype = C.c_uint32
+    L.esl_engine_create.argtypes = [C.POINTER(_Deck), C.POINTER(vp), C.c_char_p, C.c_size_t]
+    L.esl_engine_destroy.restype = C.c_uint32
+    L.esl_engine_destroy.argtypes = [vp]
+    L.esl_engine_info_get.restype = C.c_uint32
+    L.esl_engine_info_get.argtypes = [vp, C.POINTER(EngineInfo)]
+    L.esl_engine_pattern.restype = C.c_uint32
+    L.esl_engine_pattern.argtypes = [vp, vp, vp, vp]
+    L.esl_op_solve.restype = C.c_uint32
+    L.esl_op_solve.argtypes = [vp, vp, C.c_size_t, C.POINTER(OpOptions), vp, vp, vp]
+    L.esl_last_timing.restype = C.c_uint32
+    L.esl_last_timing.argtypes = [vp, C.POINTER(Timing)]
+    L.esl_tape_create.restype = C.c_uint32
+    L.esl_tape_create.argtypes = [C.c_uint32, vp, vp, vp, vp, C.c_double, C.POINTER(vp)]
+    L.esl_tape_destroy.restype = C.c_uint32
+    L.esl_tape_destroy.argtypes = [vp]
+    L.esl_tape_sizes.restype = C.c_uint32
+    L.esl_tape_sizes.argtypes = [vp, u32p, u32p, C.POINTER(C.c_uint64), u32p]
+    L.esl_tape_export.restype = C.c_uint32
+    L.esl_tape_export.argtypes = [vp] * 8
+    L.esl_tape_replay.restype = C.c_uint32
+    L.esl_tape_replay.argtypes = [vp, vp, vp, C.c_size_t, C.c_double, vp, vp, vp, vp, vp]
+    _lib = L
+    return L
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data
+
+
+def _check(st: int, what: str):
+    if st != OK:
+        raise EslError(st, what)
+
+
+def default_options(precision=FP64, start=START_COLD) -> OpOptions:
+    o = OpOptions()
+    load().esl_default_op_options(C.byref(o))
+    o.precision = precision
+    o.start = start
+    return o
+
+
+class Engine:
+    """A compiled deck on the GPU: solves operating points of many points."""
+
+    def __init__(self, arrays: dict):
+        L = load()
+        self._a = {k: (np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v) for k, v in arrays.items()}
+        a = self._a
+        d = _Deck()
+        d.n_nodes = a["n_nodes"]
+        d.n_dev = len(a["kind"])
+        d.kind, d.nodes, d.params = _ptr(a["kind"]), _ptr(a["nodes"]), _ptr(a["params"])
+        d.n_live = a["n_live"]
+        d.n_bind = len(a["bind_dev"])
+        d.bind_dev, d.bind_param, d.bind_live = _ptr(a["bind_dev"]), _ptr(a["bind_param"]), _ptr(a["bind_live"])
+        h = C.c_void_p()
+        diag = C.create_string_buffer(256)
+        st = L.esl_engine_create(C.byref(d), C.byref(h), diag, 256)
+        if st != OK:
+            raise EslError(st, f"esl_engine_create ({diag.value.decode()})")
+        self.h = h
+        info = self.info()
+        self.n, self.nnz, self.n_live = info.n, info.nnz, info.n_live
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().esl_engine_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def info(self) -> EngineInfo:
+        i = EngineInfo()
+        _check(load().esl_engine_info_get(self.h, C.byref(i)), "esl_engine_info_get")
+        return i
+
+    def pattern(self):
+        info = self.info()
+        cp = np.zeros(info.n + 1, np.uint32)
+        ri = np.zeros(max(info.nnz, 1), np.uint32)
+        q = np.zeros(info.n, np.uint32)
+        _check(load().esl_engine_pattern(self.h, _ptr(cp), _ptr(ri), _ptr(q)), "esl_engine_pattern")
+        return cp, ri[: info.nnz], q
+
+    def solve_op(self, values, x0=None, precision=FP64, tol: Tolerances | None = None):
+        """Operating points of rows `values` [count][n_live]. Returns dict
+        with x [count][n], iterations, reasons."""
+        values = np.ascontiguousarray(np.atleast_2d(np.asarray(values, dtype=np.float64)))
+        count = values.shape[0]
+        o = default_options(precision, START_COLD if x0 is None else START_GIVEN)
+        if tol is not None:
+            o.tol = tol
+        xs = None
+        if x0 is not None:
+            xs = np.ascontiguousarray(x0, dtype=np.float64).reshape(count, self.n)
+            o.x0 = xs.ctypes.data
+        x = np.zeros((count, self.n))
+        it = np.zeros(count, np.uint32)
+        rs = np.zeros(count, np.uint32)
+        _check(load().esl_op_solve(self.h, _ptr(values), count, C.byref(o), _ptr(x), _ptr(it), _ptr(rs)),
+               "esl_op_solve")
+        return {"x": x, "iterations": it, "reasons": rs}
+
+    def timing(self) -> Timing:
+        t = Timing()
+        _check(load().esl_last_timing(self.h, C.byref(t)), "esl_last_timing")
+        return t
+
+
+class Tape:
+    """A frozen pivot tape replayed by many lanes on the GPU."""
+
+    def __init__(self, col_ptr, row_idx, lead_vals, q, pivot_tol=1e-3):
+        L = load()
+        self._cp = np.ascontiguousarray(col_ptr, dtype=np.uint32)
+        self._ri = np.ascontiguousarray(row_idx, dtype=np.uint32)
+        lv = np.ascontiguousarray(lead_vals, dtype=np.float64)
+        q = np.ascontiguousarray(q, dtype=np.uint32)
+        self.n = len(self._cp) - 1
+        self.nnz = int(self._cp[-1])
+        h = C.c_void_p()
+        _check(L.esl_tape_create(self.n, _ptr(self._cp), _ptr(self._ri), _ptr(lv), _ptr(q), pivot_tol,
+                                 C.byref(h)), "esl_tape_create")
+        self.h = h
+        nl, nu, lev = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        fl = C.c_uint64()
+        _check(L.esl_tape_sizes(self.h, C.byref(nl), C.byref(nu), C.byref(fl), C.byref(lev)), "esl_tape_sizes")
+        self.nnz_l, self.nnz_u, self.flops, self.levels = nl.value, nu.value, fl.value, lev.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().esl_tape_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def export(self) -> dict:
+        n, nnz = self.n, self.nnz
+        out = {k: np.zeros(s, np.uint32) for k, s in [("pinv", n), ("lp", n + 1), ("li", max(self.nnz_l, 1)),
+                                                      ("up", n + 1), ("ui", max(self.nnz_u, 1)),
+                                                      ("flags", n), ("prow", max(nnz, 1))]}
+        _check(load().esl_tape_export(self.h, *[_ptr(out[k]) for k in
+                                               ["pinv", "lp", "li", "up", "ui", "flags", "prow"]]),
+               "esl_tape_export")
+        out["li"] = out["li"][: self.nnz_l]
+        out["ui"] = out["ui"][: self.nnz_u]
+        out["prow"] = out["prow"][:nnz]
+        return out
+
+    def replay(self, a, rhs, growth=1e-12) -> dict:
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        lanes = a.shape[0]
+        lx = np.zeros((lanes, max(self.nnz_l, 1)))
+        ux = np.zeros((lanes, max(self.nnz_u, 1)))
+        ud = np.zeros((lanes, self.n))
+        dx = np.zeros((lanes, self.n))
+        fail = np.zeros(lanes, np.uint32)
+        _check(load().esl_tape_replay(self.h, _ptr(a), _ptr(rhs), lanes, growth, _ptr(lx), _ptr(ux), _ptr(ud),
+                                      _ptr(dx), _ptr(fail)), "esl_tape_replay")
+        return {"lx": lx[:, : self.nnz_l], "ux": ux[:, : self.nnz_u], "udiag": ud, "dx": dx, "fail": fail}

@@ -1,0 +1,155 @@
+"""GPU lanes against the oracle, through the C ABI.
+
+The LU boundary must be bitwise the oracle's replay; operating points must be
+bitwise the oracle's Newton on the same points, pivot tape and start (the
+device physics, exp/log and the LU all round identically); and the golden
+ngspice cases must match to the fixtures' own tolerances.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import golden_cases as G
+from espice_amd import circuits
+from test_oracle_lu import random_system
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+@pytest.fixture(scope="module")
+def lanes():
+    from espice_amd import lanes as L
+    L.load()
+    return L
+
+
+def test_tape_replay_bitwise_random_systems(lanes, oracle):
+    rng = np.random.default_rng(11)
+    checked = 0
+    for t in range(48):
+        n = int(rng.integers(2, 64))
+        cp, ri, vals = random_system(rng, n, rng.random() * 0.3)
+        q = rng.permutation(n).astype(np.uint32)
+        if oracle.lu_factor(cp, ri, vals, q) is None:
+            continue
+        say(f"tape replay case {t} n={n} nnz={len(ri)}")
+        lanes_n = 64
+        a = vals[None, :] * (0.8 + 0.4 * rng.random((lanes_n, len(vals))))
+        a[5] = vals
+        a[7, 0] = np.nan
+        a[9] = 0.0
+        rhs = 2 * rng.random((lanes_n, n)) - 1
+        tape = lanes.Tape(cp, ri, vals, q)
+        ref = oracle.lu_replay(cp, ri, vals, q, a, rhs)
+        tx = tape.export()
+        t0 = ref["tape"]
+        for k in ["pinv", "lp", "li", "up", "ui", "prow"]:
+            assert np.array_equal(tx[k], t0[k]), k
+        got = tape.replay(a, rhs)
+        assert np.array_equal(got["fail"], ref["fail"])
+        ok = ref["fail"] == 0
+        for k in ["lx", "ux", "udiag", "dx"]:
+            assert np.array_equal(bits(got[k][ok]), bits(ref[k][ok])), (t, k)
+        tape.close()
+        checked += 1
+    assert checked >= 24
+
+
+@pytest.mark.parametrize("name", sorted(G.load()))
+def test_golden_cases_on_gpu(lanes, oracle, name):
+    case = G.load()[name]
+    deck, live, reader = G.build(case)
+    say(f"golden {name} rows={live.shape[0]}")
+    eng = lanes.Engine(deck.arrays())
+    cp, ri, q = eng.pattern()
+    c = oracle.Circuit(deck.arrays())
+    ocp, ori, _, _ = c.pattern()
+    assert np.array_equal(cp, ocp) and np.array_equal(ri, ori)
+    got = eng.solve_op(live)
+    assert (got["reasons"] == 0).all()
+    misses, worst = G.check(case, got["x"], reader)
+    assert misses == 0, worst
+    ref = c.op(live, q, threads=4)
+    assert np.array_equal(got["iterations"], ref["iterations"])
+    assert np.array_equal(bits(got["x"]), bits(ref["x"]))
+    eng.close()
+
+
+@pytest.mark.parametrize("family,lanes_n", [("inverter_chain", 256), ("ota_array", 256)])
+def test_family_bitwise_against_oracle(lanes, oracle, family, lanes_n):
+    fam = circuits.FAMILIES[family]()
+    vals = fam.sample(2026, lanes_n)
+    say(f"family {family} lanes={lanes_n}")
+    eng = lanes.Engine(fam.deck.arrays())
+    _, _, q = eng.pattern()
+    got = eng.solve_op(vals)
+    ref = oracle.Circuit(fam.deck.arrays()).op(vals, q, threads=8)
+    assert np.array_equal(got["reasons"], ref["reasons"])
+    assert np.array_equal(got["iterations"], ref["iterations"])
+    assert np.array_equal(bits(got["x"]), bits(ref["x"]))
+    say(f"  converged {(got['reasons'] == 0).sum()}/{lanes_n} mean iters {got['iterations'].mean():.2f} "
+        f"repivots {eng.timing().repivots}")
+    eng.close()
+
+
+def test_warm_start_bitwise_against_oracle(lanes, oracle):
+    fam = circuits.ota_array()
+    vals = fam.sample(5, 128)
+    eng = lanes.Engine(fam.deck.arrays())
+    _, _, q = eng.pattern()
+    c = oracle.Circuit(fam.deck.arrays())
+    nom = c.op(fam.nominal()[None, :], q)
+    x0 = np.repeat(nom["x"], 128, 0)
+    say("warm start from the nominal operating point, 128 lanes")
+    got = eng.solve_op(vals, x0=x0)
+    ref = c.op(vals, q, start_x=x0, threads=8)
+    assert np.array_equal(got["reasons"], ref["reasons"])
+    assert np.array_equal(got["iterations"], ref["iterations"])
+    assert np.array_equal(bits(got["x"]), bits(ref["x"]))
+    assert got["iterations"].mean() < 6
+    eng.close()
+
+
+def test_replicas_are_identical(lanes):
+    # tools/gpu_lane_parity.py:1-19: replicas of one point agree bitwise.
+    fam = circuits.ota_array()
+    vals = np.repeat(fam.sample(9, 1), 200, 0)
+    eng = lanes.Engine(fam.deck.arrays())
+    got = eng.solve_op(vals)
+    assert (got["x"] == got["x"][0]).all()
+    assert (got["iterations"] == got["iterations"][0]).all()
+    eng.close()
+
+
+def test_full_size_ota_converges_and_satisfies_kcl(lanes, oracle):
+    """16,384 lanes of the OTA array: every lane converges, the published
+    points satisfy the residual gate when re-assembled by the oracle, and a
+    sample of lanes is bitwise the oracle."""
+    fam = circuits.ota_array()
+    n_lanes = 16384
+    vals = fam.sample(4, n_lanes)
+    say(f"ota_array full size {n_lanes}")
+    eng = lanes.Engine(fam.deck.arrays())
+    _, _, q = eng.pattern()
+    got = eng.solve_op(vals)
+    assert (got["reasons"] == 0).all()
+    c = oracle.Circuit(fam.deck.arrays())
+    pick = np.arange(0, n_lanes, 1021)
+    ref = c.op(vals[pick], q, threads=8)
+    # both runs take their lead tape from point 0, so every lane is bitwise
+    assert np.array_equal(got["iterations"][pick], ref["iterations"])
+    assert np.array_equal(bits(got["x"][pick]), bits(ref["x"]))
+    for l in pick[:8]:
+        g, f = c.assemble_at(vals[l], got["x"][l])
+        assert np.abs(f).max() < 1e-5
+    eng.close()

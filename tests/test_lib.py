@@ -1,0 +1,47 @@
+"""The C-ABI library loads and exports every entry point include/espice_lanes.h
+declares (no compute calls here: they need a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    text = open(os.path.join(ROOT, "include", "espice_lanes.h")).read()
+    return sorted(set(re.findall(r"\b(esl_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_boundary():
+    names = declared()
+    for n in ["esl_engine_create", "esl_op_solve", "esl_tape_create", "esl_tape_replay", "esl_engine_destroy"]:
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    from espice_amd import lanes
+    lib = lanes.load()
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(lanes.EXPORTS) <= set(declared())
+
+
+def test_abi_version_and_defaults():
+    from espice_amd import lanes
+    lib = lanes.load()
+    assert lib.esl_abi_version() == 1
+    o = lanes.default_options()
+    assert o.struct_size == ctypes.sizeof(lanes.OpOptions)
+    assert o.tol.reltol == 1e-3 and o.tol.abstol == 1e-12 and o.tol.vntol == 1e-6
+    assert o.tol.residual_tol == 1e-9 and o.tol.max_iter == 100
+
+
+def test_invalid_handles_are_rejected():
+    from espice_amd import lanes
+    lib = lanes.load()
+    info = lanes.EngineInfo()
+    assert lib.esl_engine_info_get(ctypes.c_void_p(12345), ctypes.byref(info)) == lanes.INVALID_HANDLE
+    assert lib.esl_engine_destroy(ctypes.c_void_p(12345)) == lanes.INVALID_HANDLE
+    assert lib.esl_engine_destroy(None) == lanes.OK

@@ -1,0 +1,74 @@
+"""The oracle against ngspice-44.2 operating points from the reference's own
+fixtures (tests/golden/ngspice_mos1.json), and its Jacobian against finite
+differences of its residual."""
+import numpy as np
+import pytest
+
+import golden_cases as G
+from espice_amd import circuits
+
+CASES = G.load()
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_golden_case(oracle, name):
+    case = CASES[name]
+    deck, live, reader = G.build(case)
+    c = oracle.Circuit(deck.arrays())
+    r = c.op(live, oracle.natural_order(c.n), threads=4)
+    assert (r["reasons"] == 0).all()
+    misses, worst = G.check(case, r["x"], reader)
+    assert misses == 0, (name, worst)
+
+
+@pytest.mark.parametrize("family", ["inverter_chain", "ota_array"])
+def test_jacobian_matches_finite_differences(oracle, family):
+    fam = circuits.FAMILIES[family]()
+    c = oracle.Circuit(fam.deck.arrays())
+    cp, ri, dg, cur = c.pattern()
+    vals = fam.sample(7, 4)
+    rng = np.random.default_rng(3)
+    for l in range(4):
+        # a point near the operating point, away from region boundaries
+        r = c.op(vals[l:l + 1], oracle.natural_order(c.n))
+        x = r["x"][0] + 1e-3 * rng.standard_normal(c.n)
+        x[0] = 0.0
+        g, f = c.assemble_at(vals[l], x)
+        J = np.zeros((c.n, c.n))
+        for j in range(c.n):
+            for p in range(cp[j], cp[j + 1]):
+                J[ri[p], j] = g[p]
+        for j in range(1, c.n):
+            h = 1e-6 * max(1.0, abs(x[j]))
+            xp, xm = x.copy(), x.copy()
+            xp[j] += h
+            xm[j] -= h
+            _, fp = c.assemble_at(vals[l], xp)
+            _, fm = c.assemble_at(vals[l], xm)
+            fd = (fp - fm) / (2 * h)
+            scale = np.abs(J[:, j]).max() + 1e-9
+            assert np.abs(fd - J[:, j]).max() <= 1e-5 * scale + 1e-12, (family, l, j)
+
+
+def test_zero_variation_is_nominal(oracle):
+    # analysis/tests/sweep.zig:710-750: a point at the nominal values solves
+    # to the template's own operating point.
+    fam = circuits.ota_array(8)
+    c = oracle.Circuit(fam.deck.arrays())
+    nom = fam.nominal()
+    r1 = c.op(nom[None, :], oracle.natural_order(c.n))
+    r2 = c.op(np.repeat(nom[None, :], 3, 0), oracle.natural_order(c.n))
+    assert np.array_equal(r1["x"][0], r2["x"][2])
+
+
+def test_unity_gain_ota_follows_its_input(oracle):
+    # Closed-form check of the OTA family: in unity gain, out tracks in+ to
+    # within the offset that the group's mismatch creates (tens of mV).
+    fam = circuits.ota_array(16)
+    c = oracle.Circuit(fam.deck.arrays())
+    r = c.op(fam.nominal()[None, :], oracle.natural_order(c.n))
+    x = r["x"][0]
+    d = fam.deck
+    for k in range(16):
+        vin = 0.8 + 0.4 * k / 15
+        assert abs(x[d.node_index(f"out{k}")] - vin) < 0.05
