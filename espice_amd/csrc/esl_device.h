@@ -1,0 +1,235 @@
+// Device physics shared by the lane kernels (one body for every tiling):
+// the ngspice limiters and the MOS1 DC evaluation of models/mos1.va.
+// Compiled without floating-point contraction, like the oracle.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "esl_internal.h"
+#include "eslmath.h"
+
+namespace esl {
+
+// ---- ngspice limiters (devsup.c, as quoted in the reference's
+// dev/solvers/newton-raphson-convergence.md:42-69,151-162) ----
+__device__ inline double pnjlim(double vnew, double vold, double vt, double vcrit)
+{
+    if (vnew > vcrit && fabs(vnew - vold) > vt + vt) {
+        if (vold > 0) {
+            const double arg = (vnew - vold) / vt;
+            if (arg > 0) vnew = vold + vt * (2.0 + esl::log(arg - 2.0));
+            else vnew = vold - vt * (2.0 + esl::log(2.0 - arg));
+        } else {
+            vnew = vt * esl::log(vnew / vt);
+        }
+    } else if (vnew < 0) {
+        const double arg = vold > 0 ? -1.0 * vold - 1.0 : 2.0 * vold - 1.0;
+        if (vnew < arg) vnew = arg;
+    }
+    return vnew;
+}
+
+__device__ inline double fetlim(double vnew, double vold, double vto)
+{
+    const double vtsthi = fabs(2.0 * (vold - vto)) + 2.0;
+    const double vtstlo = fabs(vold - vto) + 1.0;
+    const double vtox = vto + 3.5;
+    const double delv = vnew - vold;
+    if (vold >= vto) {
+        if (vold >= vtox) {
+            if (delv <= 0) {
+                if (vnew >= vtox) {
+                    if (-delv > vtstlo) vnew = vold - vtstlo;
+                } else {
+                    vnew = fmax(vnew, vto + 2.0);
+                }
+            } else if (delv >= vtsthi) {
+                vnew = vold + vtsthi;
+            }
+        } else {
+            if (delv <= 0) vnew = fmax(vnew, vto - 0.5);
+            else vnew = fmin(vnew, vto + 4.0);
+        }
+    } else {
+        if (delv <= 0) {
+            if (-delv > vtsthi) vnew = vold - vtsthi;
+        } else {
+            const double vtemp = vto + 0.5;
+            if (vnew <= vtemp) {
+                if (delv > vtstlo) vnew = vold + vtstlo;
+            } else {
+                vnew = vtemp;
+            }
+        }
+    }
+    return vnew;
+}
+
+__device__ inline double limvds(double vnew, double vold)
+{
+    if (vold >= 3.5) {
+        if (vnew > vold) vnew = fmin(vnew, 3.0 * vold + 2.0);
+        else if (vnew < 3.5) vnew = fmax(vnew, 2.0);
+    } else {
+        if (vnew > vold) vnew = fmin(vnew, 4.0);
+        else vnew = fmax(vnew, -0.5);
+    }
+    return vnew;
+}
+
+// ---- MOS1 (models/mos1.va) ----
+struct Mos {
+    double tp, vbit, gammax, phit, beta, lambda, vt, isats, isatd, vcrits, vcritd, gmin;
+};
+
+__device__ inline Mos mos_load(const DeckProg &dp, uint32_t j, const double *live, uint32_t live_stride = 1)
+{
+    const uint32_t nm = dp.n_mos;
+    const double *f = dp.m_par;
+    Mos m;
+    m.tp = f[MF_TYPE * nm + j];
+    const int bv = dp.m_bind_vto[j], bk = dp.m_bind_kp[j];
+    const double vtox = bv >= 0 ? live[(size_t)bv * live_stride] : f[MF_VTO * nm + j];
+    const double kpx = bk >= 0 ? live[(size_t)bk * live_stride] : f[MF_KP * nm + j];
+    m.vbit = ((vtox - f[MF_VBIT_A * nm + j]) + f[MF_VBIT_B * nm + j]) + f[MF_VBIT_C * nm + j];
+    const double kpt = kpx / f[MF_RATIO4 * nm + j];
+    m.beta = kpt * f[MF_M * nm + j] * f[MF_W * nm + j] / f[MF_LEFF * nm + j];
+    m.gammax = f[MF_GAMMA * nm + j];
+    m.phit = f[MF_PHIT * nm + j];
+    m.lambda = f[MF_LAMBDA * nm + j];
+    m.vt = f[MF_VT * nm + j];
+    m.isats = f[MF_ISATS * nm + j];
+    m.isatd = f[MF_ISATD * nm + j];
+    m.vcrits = f[MF_VCRITS * nm + j];
+    m.vcritd = f[MF_VCRITD * nm + j];
+    m.gmin = f[MF_GMIN * nm + j];
+    return m;
+}
+
+__device__ inline void jcur(double v, double isat, double vtx, double gm, double &i, double &g)
+{
+    if (v > -3.0 * vtx) {
+        const double u = fmin(v / vtx, 80.0);
+        const double e = esl::exp(u);
+        i = isat * (e * (1.0 + (v / vtx - u)) - 1.0) + gm * v;
+        g = isat * e / vtx + gm;
+    } else {
+        i = gm * v - isat;
+        g = gm;
+    }
+}
+
+// Limit step after an update: (vgs, vds, vbs) are the last limited branch
+// voltages on entry and the new ones on return; true when any value moved.
+__device__ inline bool mos_limit_v(const Mos &m, double xd, double xg, double xs, double xb, double &lgs,
+                                   double &lds, double &lbs, double von)
+{
+    const double tp = m.tp;
+    const double cgs = tp * (xg - xs), cds = tp * (xd - xs), cbs = tp * (xb - xs);
+    const double ogs = lgs, ods = lds, obs = lbs;
+    double vgs = cgs, vds = cds, vbs = cbs;
+    double vgd = vgs - vds;
+    const double vgdo = ogs - ods;
+    if (ods >= 0) {
+        vgs = fetlim(vgs, ogs, von);
+        vds = vgs - vgd;
+        vds = limvds(vds, ods);
+    } else {
+        vgd = fetlim(vgd, vgdo, von);
+        vds = vgs - vgd;
+        vds = -limvds(-vds, -ods);
+        vgs = vgd + vds;
+    }
+    if (vds >= 0) {
+        vbs = pnjlim(vbs, obs, m.vt, m.vcrits);
+    } else {
+        const double vbd = pnjlim(vbs - vds, obs - ods, m.vt, m.vcritd);
+        vbs = vbd + vds;
+    }
+    const double scale = fmax(fmax(fabs(xd), fabs(xg)), fmax(fabs(xs), fabs(xb)));
+    const double tol = 16.0 * 2.220446049250313e-16 * scale;
+    if (fabs(vgs - cgs) <= tol) vgs = cgs;
+    if (fabs(vds - cds) <= tol) vds = cds;
+    if (fabs(vbs - cbs) <= tol) vbs = cbs;
+    lgs = vgs;
+    lds = vds;
+    lbs = vbs;
+    return vgs != cgs || vds != cds || vbs != cbs;
+}
+
+// DC evaluation at the limited voltages (vgs, vds, vbs) plus the residual
+// correction toward the probe x; out[0..10] are the Jacobian entries in the
+// order (d,d) (d,g) (d,s) (d,b) (s,d) (s,g) (s,s) (s,b) (b,d) (b,s) (b,b) and
+// out[11..13] the residuals of rows d, s, b. Returns von for the next limit.
+__device__ inline double mos_eval_v(const Mos &m, double xd, double xg, double xs, double xb, double vgs, double vds,
+                                    double vbs, double *out)
+{
+    const double tp = m.tp;
+    const double dgs = tp * (xg - xs) - vgs;
+    const double dds = tp * (xd - xs) - vds;
+    const double dbs = tp * (xb - xs) - vbs;
+    const double vbd = vbs - vds, vgd = vgs - vds;
+    double ibs, gbs, ibd, gbd;
+    jcur(vbs, m.isats, m.vt, m.gmin, ibs, gbs);
+    jcur(vbd, m.isatd, m.vt, m.gmin, ibd, gbd);
+    double mode, vbx, vgsx;
+    if (vds >= 0) {
+        mode = 1.0; vbx = vbs; vgsx = vgs;
+    } else {
+        mode = -1.0; vbx = vbd; vgsx = vgd;
+    }
+    const double vdsx = mode * vds;
+    double sarg;
+    if (vbx <= 0) sarg = sqrt(m.phit - vbx);
+    else sarg = fmax(sqrt(m.phit) - vbx / (2.0 * sqrt(m.phit)), 0.0);
+    const double dsarg = sarg > 0 ? -0.5 / sarg : 0.0;
+    const double von = m.vbit * tp + m.gammax * sarg;
+    const double vgst = vgsx - von;
+    const double bp = m.beta * (1.0 + m.lambda * vdsx);
+    double id, gm, gds;
+    if (vgst <= 0) {
+        id = 0; gm = 0; gds = 0;
+    } else if (vgst <= vdsx) {
+        id = 0.5 * bp * vgst * vgst;
+        gm = bp * vgst;
+        gds = 0.5 * m.beta * m.lambda * vgst * vgst;
+    } else {
+        id = bp * vdsx * (vgst - 0.5 * vdsx);
+        gm = bp * vdsx;
+        gds = m.beta * m.lambda * vdsx * (vgst - 0.5 * vdsx) + bp * (vgst - vdsx);
+    }
+    const double gmbs = -gm * (m.gammax * dsarg);
+    double ags, ads, abs_;
+    if (mode > 0) {
+        ags = gm; ads = gds; abs_ = gmbs;
+    } else {
+        ags = gm; ads = -(gm + gds + gmbs); abs_ = gmbs;
+    }
+    double ich = tp * mode * id;
+    double ib_d = tp * ibd, ib_s = tp * ibs;
+    if (dgs != 0 || dds != 0 || dbs != 0) {
+        ich = ich + tp * mode * ((ags * dgs + ads * dds) + abs_ * dbs);
+        ib_d = ib_d + tp * (gbd * (dbs - dds));
+        ib_s = ib_s + tp * (gbs * dbs);
+    }
+    const double cg = mode * ags, cd = mode * ads, cb = mode * abs_;
+    const double cs = -((cg + cd) + cb);
+    out[0] = cd + gbd;
+    out[1] = cg;
+    out[2] = cs;
+    out[3] = cb - gbd;
+    out[4] = -cd;
+    out[5] = -cg;
+    out[6] = -cs + gbs;
+    out[7] = -cb - gbs;
+    out[8] = -gbd;
+    out[9] = -gbs;
+    out[10] = gbd + gbs;
+    out[11] = ich - ib_d;
+    out[12] = -ich - ib_s;
+    out[13] = ib_d + ib_s;
+    return von;
+}
+
+
+}  // namespace esl

@@ -20,6 +20,10 @@
 
 namespace esl {
 
+struct uint2_t {
+    uint32_t x, y;
+};
+
 constexpr int kWave = 64;
 constexpr int kStageEntries = 14;   // most cells one instance stages (MOS1)
 constexpr uint32_t kRhsBit = 0x80000000u;
@@ -57,12 +61,42 @@ struct DeckProg {
     const uint32_t *m_t; const double *m_par; const int32_t *m_bind_vto; const int32_t *m_bind_kp;
     // first instance index of each kind in the stamp order
     uint32_t kind_first[kKinds + 1];
+    // pattern slot of each Jacobian entry (kNoSlot for a ground entry):
+    // R [4 * nR], V [4 * nV], MOS1 [11 * n_mos], in the stamp entry order
+    const uint32_t *r_slot, *v_slot, *m_slot;
 };
+constexpr uint32_t kNoSlot = 0xffffffffu;
+
+// Words of the refactor and solve chain streams, uint2 each:
+// x = a | (b << 16), y = c | (kind << 16); slots are 16-bit LDS indices.
+// A chain is one slot's whole sequence of updates, accumulated in a
+// register: START loads the slot, each term updates the accumulator with
+// final operands, and the closing word stores it (or divides and stores).
+enum WordKind : uint32_t {
+    W_NOP = 0,
+    W_TSTART = 1,  // acc = tv[a]; target = a
+    W_FLOP = 2,    // acc = acc - tv[b] * tv[c]
+    W_TSTORE = 3,  // tv[target] = acc
+    W_DIVU = 4,    // unscaled L entry: tv[target] = acc / tv[c]; growth + wide tests
+    W_DIVS = 5,    // scaled-pivot L entry: tv[target] = acc / tv[c]; non-finite test
+    W_VOID = 6,    // tv[a] = 1 (fabricated pivot)
+    W_YSTART = 7,  // acc = y[a]; target = a
+    W_FWD = 8,     // acc = y[c] != 0 ? acc - tv[b] * y[c] : acc
+    W_YSTORE = 9,  // y[target] = acc
+    W_BDIV = 10,   // y[target] = acc / tv[b]
+    W_KINDS = 11
+};
+// Reduce-stream words: a target word sets the accumulator's slot (bit 30:
+// residual row, else factor slot); a cell word adds one staged value.
+constexpr uint32_t kRedTarget = 0x80000000u, kRedRhs = 0x40000000u, kRedNop = 0x3fffffffu;
+constexpr int kGroup = 4;      // rows per step group
+constexpr int kRing = 16;      // rows in flight; streams carry kRing pad rows
 
 struct TapeProg {
     uint32_t n_tv, n_u, n_l, lx0;   // tv = [U | diag | L | discard]
     uint32_t discard, n_levels, n_flev, n_blev;
-    uint32_t n_void, pad0, pad1, pad2;
+    uint32_t n_void, n_rlev, n_slev, pad2;
+    uint32_t rw_rows, sw_rows, pad3, pad4;   // stream rows (multiples of kRing)
     double growth, wide;
     const uint32_t *amap;      // [nnz] pattern slot -> tv slot
     const uint32_t *pinv;      // [n] row -> pivot step
@@ -80,6 +114,17 @@ struct TapeProg {
     const uint32_t *blev_ptr, *blev_rows;            // backward solve rows by level
     const uint32_t *urow_ptr, *urow_k, *urow_slot;   // U by rows, descending column
     const uint32_t *void_slots;                      // pattern slots a void pivot needs zero
+    // chain streams: [rows + kRing][64] words, levels back to back, each level
+    // a multiple of kGroup rows; bar[g] != 0 puts a barrier after group g
+    const uint2_t *rw;          // refactor words
+    const uint32_t *rw_bar;
+    const uint2_t *sw;          // forward + backward solve words
+    const uint32_t *sw_bar;
+    const uint32_t *red;        // assembly reduce words, per chunk [len + kRing][64]
+    const uint32_t *red_off;    // [n_chunks] word offset of each chunk's stream
+    const uint32_t *red_len;    // [n_chunks] rows of each chunk's stream (multiple of kRing)
+    const uint32_t *diag_tv;    // [n] tv slot of each row's diagonal
+    const uint32_t *li;         // [n_l] row (pivot step) of each L entry
 };
 
 // Per-lane results of the Newton kernel.
@@ -108,6 +153,30 @@ struct OpArgs {
     double reltol, abstol, vntol, residual_tol;
     uint32_t max_iter, mode;    // mode: 0 cold, 1 given x, 2 resume
     uint32_t tv_cap, dump;
+    // optional: [N][kProfPhases] shader cycles per phase (assemble, refactor,
+    // solve, update + limits + gates), summed over the lane's iterations
+    unsigned long long *prof;
+};
+constexpr int kProfPhases = 4;
+
+// Arguments of the lane-per-thread Newton kernel (op_newton_lpt): thread t
+// solves lane lane_ids[t]; its state is lane-major, element i at i * N + t.
+struct LptArgs {
+    DeckProg dp;
+    TapeProg T;
+    const uint32_t *lane_ids;   // [N]
+    uint32_t N, n_mos;
+    const double *live_t;       // [n_live][N]
+    double *tvg;                // [n_tv][N]
+    double *rhsg, *xg, *xold, *yg, *sdiag;   // [n][N]
+    double *limg;               // [4 * n_mos][N]
+    double *x;                  // [lanes][n]: start (mode 1) and final iterate
+    double *lim;                // [lanes][4][n_mos]: state handed to a re-pivot
+    LaneResume *resume;         // [lanes]
+    uint32_t *status, *iterations;   // [lanes]
+    double reltol, abstol, vntol, residual_tol;
+    uint32_t max_iter, mode;    // mode: 0 cold, 1 given x
+    unsigned long long *prof;   // optional [lanes][kProfPhases]
 };
 
 // Arguments of the LU-only replay (tape_replay).

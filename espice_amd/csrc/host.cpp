@@ -27,6 +27,8 @@
 extern "C" hipError_t esl_launch_op_newton(const void *a, uint32_t grid, uint32_t lds_bytes, hipStream_t s);
 extern "C" hipError_t esl_launch_tape_replay(const void *r, uint32_t lanes, uint32_t lds_bytes, hipStream_t s);
 extern "C" hipError_t esl_kernel_attrs(uint32_t lds_bytes);
+extern "C" hipError_t esl_launch_op_newton_lpt(const void *a, hipStream_t s);
+extern "C" hipError_t esl_launch_transpose_live(const double *in, double *out, uint32_t N, uint32_t P, hipStream_t s);
 
 using std::vector;
 using u32 = uint32_t;
@@ -293,15 +295,68 @@ vector<u32> min_degree(u32 n, const vector<u32> &col_ptr, const vector<u32> &row
     return q;
 }
 
+// A chain: one slot's updates in serial order, run by one thread into a
+// register. Chains are grouped by level (a chain's level is one more than the
+// level of every chain that produces one of its operands) and, within a
+// level, dealt to the 64 threads longest first; each thread's words are laid
+// out row by row so a wavefront reads one coalesced row per step.
+struct Chain {
+    vector<esl::uint2_t> words;
+};
+
+struct ChainStreams {
+    vector<esl::uint2_t> words;
+    vector<u32> off, len, bar;
+    u32 rows = 0;
+};
+
+void emit_chains(vector<vector<Chain>> &by_level, ChainStreams &out)
+{
+    out.words.clear();
+    out.off.clear();
+    out.len.clear();
+    size_t row = 0;
+    for (auto &lev : by_level) {
+        if (lev.empty()) continue;
+        std::stable_sort(lev.begin(), lev.end(),
+                         [](const Chain &a, const Chain &b) { return a.words.size() > b.words.size(); });
+        vector<vector<esl::uint2_t>> thr(esl::kWave);
+        for (auto &ch : lev) {
+            size_t best = 0;
+            for (size_t t = 1; t < thr.size(); t++)
+                if (thr[t].size() < thr[best].size()) best = t;
+            thr[best].insert(thr[best].end(), ch.words.begin(), ch.words.end());
+        }
+        size_t L = 0;
+        for (auto &t : thr) L = std::max(L, t.size());
+        L = (L + esl::kGroup - 1) / esl::kGroup * esl::kGroup;
+        out.off.push_back((u32)row);
+        out.len.push_back((u32)L);
+        out.words.resize((row + L) * esl::kWave, esl::uint2_t{0, 0});
+        for (size_t t = 0; t < thr.size(); t++)
+            for (size_t w = 0; w < thr[t].size(); w++) out.words[(row + w) * esl::kWave + t] = thr[t][w];
+        row += L;
+    }
+    const size_t rows = (row + esl::kRing - 1) / esl::kRing * esl::kRing;
+    out.rows = (u32)rows;
+    out.words.resize((rows + esl::kRing) * esl::kWave, esl::uint2_t{0, 0});
+    out.bar.assign(rows / esl::kGroup, 0);
+    for (size_t l = 0; l < out.off.size(); l++) out.bar[(out.off[l] + out.len[l]) / esl::kGroup - 1] = 1;
+}
+
 // ---------------------------------------------------------------- tape program
 struct TapeHostProg {
     esl::TapeProg prog{};      // pointers filled after upload
-    vector<u32> amap, pinv, qinv, flags, lev_ptr, lev_cols, up, ui, lp, fptr, dst;
+    vector<u32> amap, pinv, qinv, flags, lev_ptr, lev_cols, up, ui, lp, fptr, dst, li;
     vector<u32> flev_ptr, flev_rows, lrow_ptr, lrow_k, lrow_slot;
     vector<u32> blev_ptr, blev_rows, urow_ptr, urow_k, urow_slot, void_slots;
+    ChainStreams rch, sch;
+    vector<u32> red, red_off, red_len, diag_tv;
     uint64_t flops = 0;
     DevBuf buf;
 };
+
+inline esl::uint2_t mkop(u32 kind, u32 a, u32 b, u32 c) { return esl::uint2_t{a | (b << 16), c | (kind << 16)}; }
 
 void levels_to_lists(const vector<u32> &lev, vector<u32> &ptr, vector<u32> &items, u32 &count)
 {
@@ -330,6 +385,7 @@ void build_tape_prog(const HostTape &t, const vector<u32> &col_ptr, double growt
     o.up = t.up;
     o.ui = t.ui;
     o.lp = t.lp;
+    o.li = t.li;
     o.fptr.assign(n + 1, 0);
     o.dst.clear();
     vector<u32> pos(n, NONE);
@@ -403,6 +459,119 @@ void build_tape_prog(const HostTape &t, const vector<u32> &col_ptr, double growt
     levels_to_lists(flev, o.flev_ptr, o.flev_rows, nflev);
     levels_to_lists(blev, o.blev_ptr, o.blev_rows, nblev);
     o.void_slots = t.void_slots;
+    // Refactor chains (the column replay of sparse_lu.zig:916-1024): every
+    // slot starts from its assembled value and takes its multiply-subtracts in
+    // stored topological order; an L slot ends with its pivot division. Each
+    // operand is final when read (a U entry before its column's later
+    // entries, an L column before any later column), so a chain depends only
+    // on the chains that produce its operands.
+    const u32 n_tv = next_slot;
+    {
+        vector<vector<esl::uint2_t>> terms(n_tv);
+        for (u32 k = 0; k < n; k++) {
+            u32 f = o.fptr[k];
+            for (u32 p = t.up[k]; p < t.up[k + 1]; p++) {
+                const u32 i = t.ui[p];
+                for (u32 qq = t.lp[i]; qq < t.lp[i + 1]; qq++, f++)
+                    terms[o.dst[f]].push_back(mkop(esl::W_FLOP, 0, lx0 + qq, p));
+            }
+        }
+        vector<int> level(n_tv, -1);
+        vector<vector<Chain>> by_level;
+        auto put = [&](u32 slot, int lev, Chain &&ch) {
+            level[slot] = lev;
+            if ((size_t)lev >= by_level.size()) by_level.resize(lev + 1);
+            by_level[lev].push_back(std::move(ch));
+        };
+        auto chain_of = [&](u32 slot, u32 close_kind, u32 piv) {
+            int lev = 0;
+            for (auto &w : terms[slot]) {
+                lev = std::max(lev, level[w.x >> 16] + 1);
+                lev = std::max(lev, level[w.y & 0xffff] + 1);
+            }
+            if (close_kind == esl::W_DIVU || close_kind == esl::W_DIVS) lev = std::max(lev, level[piv] + 1);
+            Chain ch;
+            ch.words.push_back(mkop(esl::W_TSTART, slot, slot, 0));
+            ch.words.insert(ch.words.end(), terms[slot].begin(), terms[slot].end());
+            ch.words.push_back(mkop(close_kind, slot, piv, 0));
+            put(slot, lev, std::move(ch));
+        };
+        for (u32 k = 0; k < n; k++) {
+            const bool void_k = t.flags[k] & esl::kVoidBit;
+            for (u32 p = t.up[k]; p < t.up[k + 1]; p++)
+                if (!terms[p].empty()) chain_of(p, esl::W_TSTORE, 0);
+            const u32 piv = nu + k;
+            if (void_k) {
+                Chain ch;
+                ch.words.push_back(mkop(esl::W_VOID, piv, 0, 0));
+                int lev = 0;
+                for (auto &w : terms[piv]) lev = std::max(lev, std::max(level[w.x >> 16], level[w.y & 0xffff]) + 1);
+                put(piv, lev, std::move(ch));
+            } else if (!terms[piv].empty()) {
+                chain_of(piv, esl::W_TSTORE, 0);
+            }
+            if (!void_k) {
+                const u32 kind = (t.flags[k] & esl::kScaledBit) ? esl::W_DIVS : esl::W_DIVU;
+                for (u32 qq = t.lp[k]; qq < t.lp[k + 1]; qq++) chain_of(lx0 + qq, kind, piv);
+            }
+        }
+        emit_chains(by_level, o.rch);
+        o.prog.n_rlev = (u32)o.rch.off.size();
+    }
+    // Solve chains (lane_kernels.zig:123-143 by rows): y row r takes the
+    // forward terms L(r, k) y(k) in ascending k, then the backward terms
+    // U(r, k) z(k) in descending k and the division by U(r, r); the forward
+    // and backward halves are separate chains because forward terms read the
+    // forward value of y(k).
+    {
+        const u32 yb = n_tv;   // y(j) sits at factor-slot index n_tv + j
+        vector<vector<std::pair<u32, u32>>> lrow(n), urow(n);
+        for (u32 k = 0; k < n; k++) {
+            for (u32 qq = t.lp[k]; qq < t.lp[k + 1]; qq++) lrow[t.li[qq]].push_back({k, lx0 + qq});
+            for (u32 p = t.up[k]; p < t.up[k + 1]; p++) urow[t.ui[p]].push_back({k, p});
+        }
+        vector<int> lf(n, -1), lb(n, -1), fwd_readers(n, -1);
+        vector<vector<Chain>> by_level;
+        auto put = [&](int lev, Chain &&ch) {
+            if ((size_t)lev >= by_level.size()) by_level.resize(lev + 1);
+            by_level[lev].push_back(std::move(ch));
+        };
+        for (u32 r = 0; r < n; r++) {
+            auto &v = lrow[r];
+            if (v.empty()) continue;
+            std::sort(v.begin(), v.end());
+            int lev = 0;
+            Chain ch;
+            ch.words.push_back(mkop(esl::W_YSTART, yb + r, yb + r, 0));
+            for (auto &e : v) {
+                lev = std::max(lev, lf[e.first] + 1);
+                ch.words.push_back(mkop(esl::W_FWD, 0, e.second, yb + e.first));
+            }
+            ch.words.push_back(mkop(esl::W_YSTORE, 0, 0, 0));
+            lf[r] = lev;
+            for (auto &e : v) fwd_readers[e.first] = std::max(fwd_readers[e.first], lev);
+            put(lev, std::move(ch));
+        }
+        for (u32 i = n; i > 0;) {
+            i--;
+            auto &v = urow[i];
+            std::sort(v.begin(), v.end(), [](auto &a, auto &b) { return a.first > b.first; });
+            // after its own forward chain, and after every forward chain that
+            // reads the forward value of y(i), which this chain overwrites
+            int lev = std::max(lf[i], fwd_readers[i]) + 1;
+            Chain ch;
+            ch.words.push_back(mkop(esl::W_YSTART, yb + i, yb + i, 0));
+            for (auto &e : v) {
+                lev = std::max(lev, lb[e.first] + 1);
+                ch.words.push_back(mkop(esl::W_FWD, 0, e.second, yb + e.first));
+            }
+            ch.words.push_back(mkop(esl::W_BDIV, 0, nu + i, 0));
+            lb[i] = lev;
+            put(lev, std::move(ch));
+        }
+        emit_chains(by_level, o.sch);
+        o.prog.n_slev = (u32)o.sch.off.size();
+    }
     esl::TapeProg &P = o.prog;
     P.n_tv = next_slot;
     P.n_u = nu;
@@ -420,7 +589,7 @@ void build_tape_prog(const HostTape &t, const vector<u32> &col_ptr, double growt
 hipError_t upload_tape_prog(TapeHostProg &o)
 {
     Packer pk;
-    size_t f[24];
+    size_t f[32];
     int i = 0;
     f[i++] = pk.add(o.amap); f[i++] = pk.add(o.pinv); f[i++] = pk.add(o.qinv); f[i++] = pk.add(o.flags);
     f[i++] = pk.add(o.lev_ptr); f[i++] = pk.add(o.lev_cols); f[i++] = pk.add(o.up); f[i++] = pk.add(o.ui);
@@ -430,6 +599,11 @@ hipError_t upload_tape_prog(TapeHostProg &o)
     f[i++] = pk.add(o.blev_ptr); f[i++] = pk.add(o.blev_rows);
     f[i++] = pk.add(o.urow_ptr); f[i++] = pk.add(o.urow_k); f[i++] = pk.add(o.urow_slot);
     f[i++] = pk.add(o.void_slots);
+    f[i++] = pk.add(o.rch.words); f[i++] = pk.add(o.rch.bar);
+    f[i++] = pk.add(o.sch.words); f[i++] = pk.add(o.sch.bar);
+    f[i++] = pk.add(o.red);
+    f[i++] = pk.add(o.red_off); f[i++] = pk.add(o.red_len); f[i++] = pk.add(o.diag_tv);
+    f[i++] = pk.add(o.li);
     hipError_t e = o.buf.ensure(pk.blob.size());
     if (e != hipSuccess) return e;
     e = hipMemcpy(o.buf.p, pk.blob.data(), pk.blob.size(), hipMemcpyHostToDevice);
@@ -445,6 +619,17 @@ hipError_t upload_tape_prog(TapeHostProg &o)
     T.blev_ptr = P(i++); T.blev_rows = P(i++);
     T.urow_ptr = P(i++); T.urow_k = P(i++); T.urow_slot = P(i++);
     T.void_slots = P(i++);
+    T.rw = reinterpret_cast<const esl::uint2_t *>(P(i++));
+    T.rw_bar = P(i++);
+    T.sw = reinterpret_cast<const esl::uint2_t *>(P(i++));
+    T.sw_bar = P(i++);
+    T.rw_rows = o.rch.rows;
+    T.sw_rows = o.sch.rows;
+    T.red = P(i++);
+    T.red_off = P(i++);
+    T.red_len = P(i++);
+    T.diag_tv = P(i++);
+    T.li = P(i++);
     return hipSuccess;
 }
 
@@ -528,6 +713,7 @@ struct Compiled {
     vector<u32> r_t, v_t, i_t, m_t;
     vector<double> r_val, v_val, i_val, m_par;
     vector<int32_t> r_bind, v_bind, i_bind, m_bind_vto, m_bind_kp;
+    vector<u32> r_slot, v_slot, m_slot;
     u32 n_mos = 0;
 };
 
@@ -642,10 +828,12 @@ bool compile_deck(const esl_deck *d, Compiled &c, std::string &err)
         case esl::kR:
             c.r_t.push_back(in.t[0]); c.r_t.push_back(in.t[1]);
             c.r_val.push_back(p[0]); c.r_bind.push_back(bd[0]);
+            for (int e = 0; e < 4; e++) c.r_slot.push_back(in.slot[e]);
             break;
         case esl::kV:
             c.v_t.push_back(in.t[0]); c.v_t.push_back(in.t[1]); c.v_t.push_back(in.br);
             c.v_val.push_back(p[0]); c.v_bind.push_back(bd[0]);
+            for (int e = 0; e < 4; e++) c.v_slot.push_back(in.slot[e]);
             break;
         case esl::kI:
             c.i_t.push_back(in.t[0]); c.i_t.push_back(in.t[1]);
@@ -653,6 +841,7 @@ bool compile_deck(const esl_deck *d, Compiled &c, std::string &err)
             break;
         case esl::kMos:
             for (int t = 0; t < 4; t++) c.m_t.push_back(in.t[t]);
+            for (int e = 0; e < 11; e++) c.m_slot.push_back(in.slot[e]);
             c.m_bind_vto.push_back(bd[ESL_M_VTO]);
             c.m_bind_kp.push_back(bd[ESL_M_KP]);
             break;
@@ -739,9 +928,15 @@ struct Engine {
     DevBuf tape_table;
     // per-solve buffers
     DevBuf live, x, lim, resume, status, iters, lane_ids, a_out, f_out;
+    // lane-per-thread state, lane-major
+    DevBuf live_t, tvg, rhsg, xg, xoldg, yg, sdiagg, limg;
+    u32 tuned = 0;   // ESL_TILING_* chosen by the autotuner
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     esl_timing timing{};
+    DevBuf prof;
+    bool profiling = false;
+    vector<unsigned long long> prof_host;
     u32 lds_max = 0;
     uint64_t lead_flops = 0;
     u32 lead_levels = 0, lead_nl = 0, lead_nu = 0, lead_lds = 0;
@@ -764,7 +959,7 @@ hipError_t upload_deck(Engine &e)
            o_rt = pk.add(c.r_t), o_rv = pk.add(c.r_val), o_rb = pk.add(c.r_bind), o_vt = pk.add(c.v_t),
            o_vv = pk.add(c.v_val), o_vb = pk.add(c.v_bind), o_it = pk.add(c.i_t), o_iv = pk.add(c.i_val),
            o_ib = pk.add(c.i_bind), o_mt = pk.add(c.m_t), o_mp = pk.add(c.m_par), o_mbv = pk.add(c.m_bind_vto),
-           o_mbk = pk.add(c.m_bind_kp);
+           o_mbk = pk.add(c.m_bind_kp), o_rs = pk.add(c.r_slot), o_vs = pk.add(c.v_slot), o_ms = pk.add(c.m_slot);
     hipError_t err = e.deck_buf.ensure(pk.blob.size());
     if (err != hipSuccess) return err;
     err = hipMemcpy(e.deck_buf.p, pk.blob.data(), pk.blob.size(), hipMemcpyHostToDevice);
@@ -798,6 +993,9 @@ hipError_t upload_deck(Engine &e)
     dp.m_bind_vto = reinterpret_cast<const int32_t *>(b + o_mbv);
     dp.m_bind_kp = reinterpret_cast<const int32_t *>(b + o_mbk);
     for (u32 k = 0; k <= esl::kKinds; k++) dp.kind_first[k] = c.kind_first[k];
+    dp.r_slot = reinterpret_cast<const u32 *>(b + o_rs);
+    dp.v_slot = reinterpret_cast<const u32 *>(b + o_vs);
+    dp.m_slot = reinterpret_cast<const u32 *>(b + o_ms);
     return hipSuccess;
 }
 
@@ -806,6 +1004,48 @@ u32 lds_bytes_for(u32 tv_cap, u32 n)
     return (u32)((size_t)(tv_cap + 5 * n + esl::kWave * esl::kStageEntries) * sizeof(double));
 }
 
+
+
+// Assembly reduce streams for one tape: each chunk's reduce tasks with their
+// targets mapped to factor slots, dealt to the 64 threads longest first, each
+// thread's words laid out row by row ([row][64]) so a wavefront reads one
+// coalesced row per step.
+void build_reduce_streams(const Compiled &c, TapeHostProg &o)
+{
+    o.red.clear();
+    o.red_off.clear();
+    o.red_len.clear();
+    const size_t nch = c.chunks.size() - 1;
+    for (size_t ci = 0; ci < nch; ci++) {
+        const u32 t0 = c.chunks[ci].task_begin, t1 = c.chunks[ci + 1].task_begin;
+        vector<u32> order;
+        for (u32 t = t0; t < t1; t++) order.push_back(t);
+        auto len = [&](u32 t) { return 1 + c.task_cell[t + 1] - c.task_cell[t]; };
+        std::stable_sort(order.begin(), order.end(), [&](u32 a, u32 b) { return len(a) > len(b); });
+        vector<vector<u32>> words(esl::kWave);
+        for (u32 t : order) {
+            size_t best = 0;
+            for (size_t w = 1; w < words.size(); w++)
+                if (words[w].size() < words[best].size()) best = w;
+            const u32 tg = c.task_target[t];
+            const u32 word = (tg & esl::kRhsBit) ? (esl::kRedTarget | esl::kRedRhs | (tg & ~esl::kRhsBit))
+                                                 : (esl::kRedTarget | o.amap[tg]);
+            words[best].push_back(word);
+            for (u32 cc = c.task_cell[t]; cc < c.task_cell[t + 1]; cc++) words[best].push_back(c.cells[cc]);
+        }
+        size_t L = 0;
+        for (auto &w : words) L = std::max(L, w.size());
+        L = (L + esl::kRing - 1) / esl::kRing * esl::kRing;
+        o.red_off.push_back((u32)o.red.size());
+        o.red_len.push_back((u32)L);
+        const size_t base = o.red.size();
+        o.red.resize(base + (L + esl::kRing) * esl::kWave, esl::kRedNop);
+        for (size_t t = 0; t < words.size(); t++)
+            for (size_t w = 0; w < words[t].size(); w++) o.red[base + w * esl::kWave + t] = words[t][w];
+    }
+    o.diag_tv.assign(c.n, 0);
+    for (u32 i = 0; i < c.n; i++) o.diag_tv[i] = o.amap[c.diag[i]];
+}
 
 // ---------------------------------------------------------------- validation
 // Every index a kernel reads is checked on the host before the first launch,
@@ -822,6 +1062,8 @@ bool validate_tape(const TapeHostProg &o, u32 n, u32 nnz)
     for (u32 k = 0; k < n; k++)
         if (o.up[k] > o.up[k + 1] || o.lp[k] > o.lp[k + 1] || o.fptr[k] > o.fptr[k + 1]) return false;
     for (u32 v : o.ui) if (v >= n) return false;
+    if (o.li.size() != P.n_l) return false;
+    for (u32 v : o.li) if (v >= n) return false;
     for (u32 v : o.dst) if (v >= P.n_tv) return false;
     if (o.lev_ptr.size() != P.n_levels + 1 || o.lev_ptr[P.n_levels] != n || o.lev_cols.size() != n) return false;
     for (u32 v : o.lev_cols) if (v >= n) return false;
@@ -837,6 +1079,44 @@ bool validate_tape(const TapeHostProg &o, u32 n, u32 nnz)
     for (u32 v : o.urow_slot) if (v >= P.n_tv) return false;
     for (u32 v : o.void_slots) if (v >= nnz) return false;
     if (P.lx0 + P.n_l > P.n_tv || P.n_u + n > P.n_tv) return false;
+    if (P.n_tv + n >= 65536) return false;
+    auto chains_ok = [&](const ChainStreams &cs, u32 nlev, bool solve) {
+        if (cs.off.size() != nlev || cs.len.size() != nlev) return false;
+        size_t rows = 0;
+        for (u32 l = 0; l < nlev; l++) {
+            if (cs.off[l] != rows || cs.len[l] % esl::kGroup) return false;
+            rows += cs.len[l];
+        }
+        if (cs.rows % esl::kRing || cs.rows < rows) return false;
+        if (cs.words.size() != (size_t)(cs.rows + esl::kRing) * esl::kWave) return false;
+        if (cs.bar.size() != cs.rows / esl::kGroup) return false;
+        for (const auto &w : cs.words) {
+            const u32 kind = w.y >> 16, a = w.x & 0xffff, b = w.x >> 16, cc = w.y & 0xffff;
+            if (kind >= esl::W_KINDS) return false;
+            const bool y_kind = kind >= esl::W_YSTART;
+            if (kind != esl::W_NOP && y_kind != solve) return false;
+            const u32 lim = solve ? P.n_tv + n : P.n_tv;
+            if (a >= lim || b >= lim || cc >= lim) return false;
+        }
+        return true;
+    };
+    if (!chains_ok(o.rch, P.n_rlev, false) || !chains_ok(o.sch, P.n_slev, true)) return false;
+    if (!o.red_off.empty()) {
+        if (o.red_len.size() != o.red_off.size() || o.diag_tv.size() != n) return false;
+        for (size_t ci = 0; ci < o.red_off.size(); ci++) {
+            if ((size_t)o.red_off[ci] + (size_t)(o.red_len[ci] + esl::kRing) * esl::kWave > o.red.size()) return false;
+            if (o.red_len[ci] % esl::kRing) return false;
+        }
+        for (u32 w : o.red) {
+            if (w & esl::kRedTarget) {
+                const u32 idx = w & 0xfffff;
+                if ((w & esl::kRedRhs) ? idx >= n : idx >= P.n_tv) return false;
+            } else if (w != esl::kRedNop && w >= (u32)(esl::kWave * esl::kStageEntries)) {
+                return false;
+            }
+        }
+        for (u32 v : o.diag_tv) if (v >= P.n_tv) return false;
+    }
     return true;
 }
 
@@ -871,6 +1151,11 @@ bool validate_deck(const Compiled &c)
         for (int32_t v : b) if (v >= (int32_t)c.n_live) return false;
         return true;
     };
+    auto slots_ok = [&](const vector<u32> &s) {
+        for (u32 v : s) if (v != NONE && v >= nnz) return false;
+        return true;
+    };
+    if (!slots_ok(c.r_slot) || !slots_ok(c.v_slot) || !slots_ok(c.m_slot)) return false;
     return binds_ok(c.r_bind) && binds_ok(c.v_bind) && binds_ok(c.i_bind) && binds_ok(c.m_bind_vto) &&
            binds_ok(c.m_bind_kp);
 }
@@ -962,6 +1247,7 @@ esl_status add_tape(Engine &e, const HostTape &t, u32 &id)
     }
     auto *tp = new TapeHostProg();
     build_tape_prog(t, e.c.col_ptr, 1e-12, 1.0 / 1e-3, *tp);
+    build_reduce_streams(e.c, *tp);
     if (!validate_tape(*tp, e.c.n, e.c.nnz)) {
         delete tp;
         g_last_error = "tape program failed validation";
@@ -1091,6 +1377,13 @@ esl_status esl_engine_info_get(esl_engine *h, esl_engine_info *out)
     out->levels = e->lead_levels;
     out->lds_bytes = e->lead_lds;
     out->tapes = (u32)e->tapes.size();
+    out->tiling = e->tuned;
+    if (!e->tapes.empty()) {
+        const TapeHostProg &tp = *e->tapes[0];
+        out->refactor_steps = tp.rch.rows;
+        out->solve_steps = tp.sch.rows;
+        out->levels = tp.prog.n_rlev;
+    }
     return ESL_OK;
 }
 
@@ -1104,6 +1397,20 @@ esl_status esl_engine_pattern(esl_engine *h, uint32_t *col_ptr, uint32_t *row_id
     return ESL_OK;
 }
 
+// Development hook, not part of the drop-in: per-lane shader cycles by phase
+// of the next solves (enable != 0), and the last solve's counters.
+esl_status esl_debug_profile(esl_engine *h, int enable, unsigned long long *out, size_t capacity)
+{
+    auto *e = static_cast<Engine *>(lookup(h, 1));
+    if (!e) return ESL_INVALID_HANDLE;
+    e->profiling = enable != 0;
+    if (out) {
+        if (capacity < e->prof_host.size()) return ESL_BUFFER_TOO_SMALL;
+        memcpy(out, e->prof_host.data(), e->prof_host.size() * 8);
+    }
+    return ESL_OK;
+}
+
 esl_status esl_last_timing(esl_engine *h, esl_timing *out)
 {
     auto *e = static_cast<Engine *>(lookup(h, 1));
@@ -1113,8 +1420,81 @@ esl_status esl_last_timing(esl_engine *h, esl_timing *out)
     return ESL_OK;
 }
 
-esl_status esl_op_solve(esl_engine *h, const double *values, size_t count, const esl_op_options *opt,
-                        double *x, uint32_t *iterations, uint32_t *reasons)
+}  // extern "C"
+
+namespace {
+
+// One lane-per-thread launch over `lanes` (lane ids already on the device in
+// e.lane_ids, live values lane-major in e.live_t).
+esl_status launch_lpt(Engine &e, const HostOpArgs &a, u32 lanes, u32 mode, float &ms)
+{
+    const Compiled &c = e.c;
+    const TapeHostProg &tp = *e.tapes[0];
+    const size_t N = lanes;
+    HIPCHK(e.tvg.ensure(N * tp.prog.n_tv * 8));
+    HIPCHK(e.rhsg.ensure(N * c.n * 8));
+    HIPCHK(e.xg.ensure(N * c.n * 8));
+    HIPCHK(e.xoldg.ensure(N * c.n * 8));
+    HIPCHK(e.yg.ensure(N * c.n * 8));
+    HIPCHK(e.sdiagg.ensure(N * c.n * 8));
+    HIPCHK(e.limg.ensure(std::max<size_t>(1, N * 4 * c.n_mos) * 8));
+    esl::LptArgs L{};
+    L.dp = a.dp;
+    L.T = tp.prog;
+    L.lane_ids = a.lane_ids;
+    L.N = lanes;
+    L.n_mos = c.n_mos;
+    L.live_t = e.live_t.as<double>();
+    L.tvg = e.tvg.as<double>();
+    L.rhsg = e.rhsg.as<double>();
+    L.xg = e.xg.as<double>();
+    L.xold = e.xoldg.as<double>();
+    L.yg = e.yg.as<double>();
+    L.sdiag = e.sdiagg.as<double>();
+    L.limg = e.limg.as<double>();
+    L.x = a.x;
+    L.lim = a.lim;
+    L.resume = a.resume;
+    L.status = a.status;
+    L.iterations = a.iterations;
+    L.reltol = a.reltol;
+    L.abstol = a.abstol;
+    L.vntol = a.vntol;
+    L.residual_tol = a.residual_tol;
+    L.max_iter = a.max_iter;
+    L.mode = mode;
+    L.prof = a.prof;
+    HIPCHK(hipEventRecord(e.ev0, e.stream));
+    HIPCHK(esl_launch_op_newton_lpt(&L, e.stream));
+    HIPCHK(hipEventRecord(e.ev1, e.stream));
+    HIPCHK(hipEventSynchronize(e.ev1));
+    HIPCHK(hipEventElapsedTime(&ms, e.ev0, e.ev1));
+    return ESL_OK;
+}
+
+// One wavefront-per-lane launch over `lanes` lanes listed in e.lane_ids.
+esl_status launch_lpw(Engine &e, HostOpArgs a, u32 lanes, u32 mode, float &ms)
+{
+    a.tapes = e.tape_table.as<esl::TapeProg>();
+    a.mode = mode;
+    a.dump = 0;
+    u32 tv_cap = 0;
+    for (auto *t : e.tapes) tv_cap = std::max(tv_cap, t->prog.n_tv);
+    a.tv_cap = tv_cap;
+    const u32 lds = lds_bytes_for(tv_cap, e.c.n);
+    HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
+    HIPCHK(hipEventRecord(e.ev0, e.stream));
+    HIPCHK(esl_launch_op_newton(&a, lanes, lds, e.stream));
+    HIPCHK(hipEventRecord(e.ev1, e.stream));
+    HIPCHK(hipEventSynchronize(e.ev1));
+    HIPCHK(hipEventElapsedTime(&ms, e.ev0, e.ev1));
+    return ESL_OK;
+}
+
+}  // namespace
+
+extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t count, const esl_op_options *opt,
+                                   double *x, uint32_t *iterations, uint32_t *reasons)
 {
     auto *e = static_cast<Engine *>(lookup(h, 1));
     if (!e) return ESL_INVALID_HANDLE;
@@ -1131,6 +1511,7 @@ esl_status esl_op_solve(esl_engine *h, const double *values, size_t count, const
     if (o.start == ESL_START_GIVEN && !o.x0) return ESL_INVALID_ARGUMENT;
     if (o.precision != ESL_FP64) return ESL_INVALID_ARGUMENT;
     if (o.tol.max_iter == 0) return ESL_INVALID_ARGUMENT;
+    if (o.tiling > ESL_TILING_THREAD) return ESL_INVALID_ARGUMENT;
     for (size_t i = 0; i < count * e->c.n_live; i++)
         if (!std::isfinite(values[i])) return ESL_INVALID_ARGUMENT;
     const auto t_start = std::chrono::steady_clock::now();
@@ -1138,6 +1519,7 @@ esl_status esl_op_solve(esl_engine *h, const double *values, size_t count, const
     const u32 n = c.n, nnz = c.nnz, P = c.n_live;
     e->timing = esl_timing{};
     HIPCHK(e->live.ensure(std::max<size_t>(1, count * P) * 8));
+    HIPCHK(e->live_t.ensure(std::max<size_t>(1, count * P) * 8));
     HIPCHK(e->x.ensure(count * n * 8));
     HIPCHK(e->lim.ensure(std::max<size_t>(1, count * 4 * c.n_mos) * 8));
     HIPCHK(e->resume.ensure(count * sizeof(esl::LaneResume)));
@@ -1146,6 +1528,7 @@ esl_status esl_op_solve(esl_engine *h, const double *values, size_t count, const
     HIPCHK(e->lane_ids.ensure(count * 4));
     hipStream_t s = e->stream;
     if (P) HIPCHK(hipMemcpyAsync(e->live.p, values, count * P * 8, hipMemcpyHostToDevice, s));
+    if (P) HIPCHK(esl_launch_transpose_live(e->live.as<double>(), e->live_t.as<double>(), (u32)count, P, s));
     if (o.start == ESL_START_GIVEN) HIPCHK(hipMemcpyAsync(e->x.p, o.x0, count * n * 8, hipMemcpyHostToDevice, s));
     vector<u32> ids(count);
     for (size_t i = 0; i < count; i++) ids[i] = (u32)i;
@@ -1159,11 +1542,18 @@ esl_status esl_op_solve(esl_engine *h, const double *values, size_t count, const
     a.resume = e->resume.as<esl::LaneResume>();
     a.status = e->status.as<u32>();
     a.iterations = e->iters.as<u32>();
+    a.lane_ids = e->lane_ids.as<u32>();
     a.reltol = o.tol.reltol;
     a.abstol = o.tol.abstol;
     a.vntol = o.tol.vntol;
     a.residual_tol = o.tol.residual_tol;
     a.max_iter = o.tol.max_iter;
+    a.prof = nullptr;
+    if (e->profiling) {
+        HIPCHK(e->prof.ensure(count * esl::kProfPhases * 8));
+        HIPCHK(hipMemsetAsync(e->prof.p, 0, count * esl::kProfPhases * 8, s));
+        a.prof = e->prof.as<unsigned long long>();
+    }
 
     // Lead tape: point 0's first Jacobian, factored on the host.
     clear_tapes(*e);
@@ -1171,7 +1561,7 @@ esl_status esl_op_solve(esl_engine *h, const double *values, size_t count, const
         HIPCHK(e->a_out.ensure(std::max<u32>(nnz, 1) * 8));
         HIPCHK(e->f_out.ensure(std::max<u32>(n, 1) * 8));
         HostOpArgs d = a;
-        d.lane_ids = e->lane_ids.as<u32>();
+        d.prof = nullptr;
         d.mode = o.start == ESL_START_GIVEN ? 1 : 0;
         d.dump = 1;
         d.tv_cap = nnz + 1;
@@ -1204,40 +1594,60 @@ esl_status esl_op_solve(esl_engine *h, const double *values, size_t count, const
     vector<esl::LaneResume> rs(count);
     for (auto &r : rs) r = esl::LaneResume{0, 0, 0, 0, 1e9};
     HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
+    const u32 mode0 = o.start == ESL_START_GIVEN ? 1 : 0;
+
+    // Tiling: explicit, or the autotuner's choice. The autotuner times both
+    // tilings once per engine on the first batch of at least 1024 points
+    // (kernel time of the first launch over every lane) and keeps the faster;
+    // below that size a batch runs one wavefront per lane.
+    u32 tiling = o.tiling;
+    if (tiling == ESL_TILING_AUTO) {
+        if (e->tuned) {
+            tiling = e->tuned;
+        } else if (count >= 1024) {
+            float ms_w = 0, ms_t = 0;
+            if (launch_lpw(*e, a, (u32)count, mode0, ms_w) != ESL_OK) return ESL_DEVICE_UNAVAILABLE;
+            HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
+            if (mode0 == 1) HIPCHK(hipMemcpyAsync(e->x.p, o.x0, count * n * 8, hipMemcpyHostToDevice, s));
+            if (launch_lpt(*e, a, (u32)count, mode0, ms_t) != ESL_OK) return ESL_DEVICE_UNAVAILABLE;
+            HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
+            if (mode0 == 1) HIPCHK(hipMemcpyAsync(e->x.p, o.x0, count * n * 8, hipMemcpyHostToDevice, s));
+            e->tuned = ms_t < ms_w ? ESL_TILING_THREAD : ESL_TILING_WAVE;
+            tiling = e->tuned;
+        } else {
+            tiling = ESL_TILING_WAVE;
+        }
+    }
+    e->timing.tiling = tiling;
 
     vector<u32> st(count), its(count);
-    vector<u32> active = ids;
-    u32 mode = o.start == ESL_START_GIVEN ? 1 : 0;
-    float kernel_ms = 0;
-    for (int round = 0; !active.empty(); round++) {
-        if (round > 0) HIPCHK(hipMemcpyAsync(e->lane_ids.p, active.data(), active.size() * 4, hipMemcpyHostToDevice, s));
-        a.tapes = e->tape_table.as<esl::TapeProg>();
-        a.lane_ids = e->lane_ids.as<u32>();
-        a.mode = mode;
-        a.dump = 0;
-        u32 tv_cap = 0;
-        for (auto *t : e->tapes) tv_cap = std::max(tv_cap, t->prog.n_tv);
-        a.tv_cap = tv_cap;
-        const u32 lds = lds_bytes_for(tv_cap, n);
-        HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
-        HIPCHK(hipEventRecord(e->ev0, s));
-        HIPCHK(esl_launch_op_newton(&a, (u32)active.size(), lds, s));
-        HIPCHK(hipEventRecord(e->ev1, s));
-        HIPCHK(hipMemcpyAsync(st.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
-        kernel_ms += ms;
-        e->timing.launches++;
+    float kernel_ms = 0, ms = 0;
+    if (tiling == ESL_TILING_THREAD) {
+        esl_status r = launch_lpt(*e, a, (u32)count, mode0, ms);
+        if (r != ESL_OK) return r;
+    } else {
+        esl_status r = launch_lpw(*e, a, (u32)count, mode0, ms);
+        if (r != ESL_OK) return r;
+    }
+    kernel_ms += ms;
+    e->timing.launches++;
+    HIPCHK(hipMemcpyAsync(st.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    // Re-pivot rounds: dump each failing lane's Jacobian at its saved iterate,
+    // factor it on the host into its own tape, resume it one wavefront per lane.
+    for (int round = 0; round < 1000; round++) {
         vector<u32> again;
-        for (u32 l : active)
-            if (st[l] == esl::kRepivot) again.push_back(l);
-        if (again.empty() || round > 1000) break;
-        // dump the re-pivoting lanes' Jacobians at their saved iterates
+        for (size_t l = 0; l < count; l++)
+            if (st[l] == esl::kRepivot) again.push_back((u32)l);
+        if (again.empty()) break;
         HIPCHK(e->a_out.ensure(again.size() * nnz * 8));
         HIPCHK(e->f_out.ensure(again.size() * n * 8));
         HIPCHK(hipMemcpyAsync(e->lane_ids.p, again.data(), again.size() * 4, hipMemcpyHostToDevice, s));
+        u32 tv_cap = 0;
+        for (auto *t : e->tapes) tv_cap = std::max(tv_cap, t->prog.n_tv);
         HostOpArgs d = a;
+        d.prof = nullptr;
+        d.tapes = e->tape_table.as<esl::TapeProg>();
         d.mode = 2;
         d.dump = 1;
         d.tv_cap = std::max(tv_cap, nnz + 1);
@@ -1267,10 +1677,15 @@ esl_status esl_op_solve(esl_engine *h, const double *values, size_t count, const
             next.push_back(l);
         }
         HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
-        // singular lanes: record their status on the device copy too
         HIPCHK(hipMemcpyAsync(e->status.p, st.data(), count * 4, hipMemcpyHostToDevice, s));
-        active = next;
-        mode = 2;
+        if (next.empty()) break;
+        HIPCHK(hipMemcpyAsync(e->lane_ids.p, next.data(), next.size() * 4, hipMemcpyHostToDevice, s));
+        esl_status r = launch_lpw(*e, a, (u32)next.size(), 2, ms);
+        if (r != ESL_OK) return r;
+        kernel_ms += ms;
+        e->timing.launches++;
+        HIPCHK(hipMemcpyAsync(st.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
     }
     HIPCHK(hipMemcpyAsync(x, e->x.p, count * n * 8, hipMemcpyDeviceToHost, s));
     vector<u32> dev_its(count);
@@ -1288,12 +1703,18 @@ esl_status esl_op_solve(esl_engine *h, const double *values, size_t count, const
         }
         lane_it += iterations[l];
     }
+    if (e->profiling) {
+        e->prof_host.assign(count * esl::kProfPhases, 0);
+        HIPCHK(hipMemcpy(e->prof_host.data(), e->prof.p, count * esl::kProfPhases * 8, hipMemcpyDeviceToHost));
+    }
     e->timing.kernel_ms = kernel_ms;
     e->timing.lane_iterations = lane_it;
     e->timing.total_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     return ESL_OK;
 }
+
+extern "C" {
 
 // ---------------------------------------------------------------- tape boundary
 struct TapeObj {

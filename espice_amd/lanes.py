@@ -24,6 +24,7 @@ FAILED, ABI_MISMATCH, INVALID_HANDLE, DEVICE_UNAVAILABLE = 6, 7, 8, 9
 REASON_OK, REASON_NO_CONVERGENCE, REASON_SINGULAR = 0, 1, 3
 FP64, FP32_REFINED = 0, 1
 START_COLD, START_GIVEN = 0, 1
+TILING_AUTO, TILING_WAVE, TILING_THREAD = 0, 1, 2
 
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "BUFFER_TOO_SMALL", 3: "OUT_OF_MEMORY",
                 4: "INVALID_QUERY", 5: "RESULT_UNAVAILABLE", 6: "FAILED", 7: "ABI_MISMATCH",
@@ -59,18 +60,21 @@ class Tolerances(C.Structure):
 class OpOptions(C.Structure):
     _fields_ = [("struct_size", C.c_size_t), ("tol", Tolerances), ("precision", C.c_uint32),
                 ("start", C.c_uint32), ("x0", C.c_void_p), ("refine_tol", C.c_double),
-                ("refine_max", C.c_uint32), ("reserved", C.c_uint32)]
+                ("refine_max", C.c_uint32), ("tiling", C.c_uint32)]
 
 
 class EngineInfo(C.Structure):
     _fields_ = [("n", C.c_uint32), ("nnz", C.c_uint32), ("n_nodes", C.c_uint32), ("n_live", C.c_uint32),
                 ("nnz_l", C.c_uint32), ("nnz_u", C.c_uint32), ("flops", C.c_uint64), ("levels", C.c_uint32),
-                ("lds_bytes", C.c_uint32), ("tapes", C.c_uint32), ("reserved", C.c_uint32)]
+                ("lds_bytes", C.c_uint32), ("tapes", C.c_uint32), ("reserved", C.c_uint32),
+                ("refactor_steps", C.c_uint32), ("solve_steps", C.c_uint32), ("tiling", C.c_uint32),
+                ("reserved2", C.c_uint32)]
 
 
 class Timing(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("launches", C.c_uint32),
-                ("repivots", C.c_uint32), ("lane_iterations", C.c_uint64)]
+                ("repivots", C.c_uint32), ("lane_iterations", C.c_uint64), ("tiling", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 _lib = None
@@ -173,12 +177,13 @@ class Engine:
         _check(load().esl_engine_pattern(self.h, _ptr(cp), _ptr(ri), _ptr(q)), "esl_engine_pattern")
         return cp, ri[: info.nnz], q
 
-    def solve_op(self, values, x0=None, precision=FP64, tol: Tolerances | None = None):
+    def solve_op(self, values, x0=None, precision=FP64, tol: Tolerances | None = None, tiling=TILING_AUTO):
         """Operating points of rows `values` [count][n_live]. Returns dict
         with x [count][n], iterations, reasons."""
         values = np.ascontiguousarray(np.atleast_2d(np.asarray(values, dtype=np.float64)))
         count = values.shape[0]
         o = default_options(precision, START_COLD if x0 is None else START_GIVEN)
+        o.tiling = tiling
         if tol is not None:
             o.tol = tol
         xs = None

@@ -120,6 +120,13 @@ enum {
     ESL_FP64 = 0,          /* FP64 factors throughout (the drop-in) */
     ESL_FP32_REFINED = 1   /* FP32 factors, FP64 residual + refinement */
 };
+/* esl_op_options.tiling: how a lane maps onto the GPU. AUTO times both
+ * tilings on the engine's first large batch and keeps the faster. */
+enum {
+    ESL_TILING_AUTO = 0,
+    ESL_TILING_WAVE = 1,    /* one wavefront per lane, factors in LDS */
+    ESL_TILING_THREAD = 2   /* one thread per lane, state lane-major in HBM */
+};
 /* esl_op_options.start */
 enum {
     ESL_START_COLD = 0,    /* zero x + junction seeds, INITFIX (dc/op.zig:24-27) */
@@ -137,7 +144,7 @@ typedef struct {
      * (relative residual), or after refine_max steps. */
     double refine_tol;
     uint32_t refine_max;
-    uint32_t reserved;
+    uint32_t tiling;       /* ESL_TILING_* */
 } esl_op_options;
 
 /* Sizes of a compiled engine. */
@@ -152,6 +159,10 @@ typedef struct {
     uint32_t lds_bytes;    /* LDS per lane of the wavefront kernel */
     uint32_t tapes;        /* distinct pivot tapes in use */
     uint32_t reserved;
+    uint32_t refactor_steps;  /* rows of the lead tape's refactor chain stream */
+    uint32_t solve_steps;     /* rows of its forward + backward solve stream */
+    uint32_t tiling;          /* ESL_TILING_* the autotuner settled on, 0 if none */
+    uint32_t reserved2;
 } esl_engine_info;
 
 typedef struct esl_engine esl_engine;
@@ -194,8 +205,17 @@ typedef struct {
     uint32_t launches;     /* Newton kernel launches (1 + re-pivot rounds) */
     uint32_t repivots;     /* lanes re-pivoted on the host */
     uint64_t lane_iterations; /* sum of Newton iterations over lanes */
+    uint32_t tiling;       /* ESL_TILING_* the last solve ran (first launch) */
+    uint32_t reserved;
 } esl_timing;
 esl_status esl_last_timing(esl_engine *e, esl_timing *out);
+
+/* Development hook (no counterpart in the reference): enable != 0 makes the
+ * following esl_op_solve calls count shader cycles per lane in four phases
+ * (assembly, refactor, solve, update + limiting + gates); out receives the
+ * last solve's counters [count][4] when capacity allows. */
+esl_status esl_debug_profile(esl_engine *e, int enable, unsigned long long *out,
+                             size_t capacity);
 
 /* ---- The LU boundary alone (arp_lane_refactor + arp_lane_solve,
  * src/solver/lane_device.zig:26-38; bodies lane_kernels.zig:49-143). ----

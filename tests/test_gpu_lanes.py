@@ -65,17 +65,18 @@ def test_tape_replay_bitwise_random_systems(lanes, oracle):
     assert checked >= 24
 
 
+@pytest.mark.parametrize("tiling", [1, 2])
 @pytest.mark.parametrize("name", sorted(G.load()))
-def test_golden_cases_on_gpu(lanes, oracle, name):
+def test_golden_cases_on_gpu(lanes, oracle, name, tiling):
     case = G.load()[name]
     deck, live, reader = G.build(case)
-    say(f"golden {name} rows={live.shape[0]}")
+    say(f"golden {name} rows={live.shape[0]} tiling={tiling}")
     eng = lanes.Engine(deck.arrays())
     cp, ri, q = eng.pattern()
     c = oracle.Circuit(deck.arrays())
     ocp, ori, _, _ = c.pattern()
     assert np.array_equal(cp, ocp) and np.array_equal(ri, ori)
-    got = eng.solve_op(live)
+    got = eng.solve_op(live, tiling=tiling)
     assert (got["reasons"] == 0).all()
     misses, worst = G.check(case, got["x"], reader)
     assert misses == 0, worst
@@ -85,14 +86,15 @@ def test_golden_cases_on_gpu(lanes, oracle, name):
     eng.close()
 
 
+@pytest.mark.parametrize("tiling", [1, 2])
 @pytest.mark.parametrize("family,lanes_n", [("inverter_chain", 256), ("ota_array", 256)])
-def test_family_bitwise_against_oracle(lanes, oracle, family, lanes_n):
+def test_family_bitwise_against_oracle(lanes, oracle, family, lanes_n, tiling):
     fam = circuits.FAMILIES[family]()
     vals = fam.sample(2026, lanes_n)
-    say(f"family {family} lanes={lanes_n}")
+    say(f"family {family} lanes={lanes_n} tiling={tiling}")
     eng = lanes.Engine(fam.deck.arrays())
     _, _, q = eng.pattern()
-    got = eng.solve_op(vals)
+    got = eng.solve_op(vals, tiling=tiling)
     ref = oracle.Circuit(fam.deck.arrays()).op(vals, q, threads=8)
     assert np.array_equal(got["reasons"], ref["reasons"])
     assert np.array_equal(got["iterations"], ref["iterations"])
@@ -102,7 +104,8 @@ def test_family_bitwise_against_oracle(lanes, oracle, family, lanes_n):
     eng.close()
 
 
-def test_warm_start_bitwise_against_oracle(lanes, oracle):
+@pytest.mark.parametrize("tiling", [1, 2])
+def test_warm_start_bitwise_against_oracle(lanes, oracle, tiling):
     fam = circuits.ota_array()
     vals = fam.sample(5, 128)
     eng = lanes.Engine(fam.deck.arrays())
@@ -111,7 +114,7 @@ def test_warm_start_bitwise_against_oracle(lanes, oracle):
     nom = c.op(fam.nominal()[None, :], q)
     x0 = np.repeat(nom["x"], 128, 0)
     say("warm start from the nominal operating point, 128 lanes")
-    got = eng.solve_op(vals, x0=x0)
+    got = eng.solve_op(vals, x0=x0, tiling=tiling)
     ref = c.op(vals, q, start_x=x0, threads=8)
     assert np.array_equal(got["reasons"], ref["reasons"])
     assert np.array_equal(got["iterations"], ref["iterations"])
@@ -131,17 +134,18 @@ def test_replicas_are_identical(lanes):
     eng.close()
 
 
-def test_full_size_ota_converges_and_satisfies_kcl(lanes, oracle):
+@pytest.mark.parametrize("tiling", [0, 1, 2])
+def test_full_size_ota_converges_and_satisfies_kcl(lanes, oracle, tiling):
     """16,384 lanes of the OTA array: every lane converges, the published
     points satisfy the residual gate when re-assembled by the oracle, and a
     sample of lanes is bitwise the oracle."""
     fam = circuits.ota_array()
     n_lanes = 16384
     vals = fam.sample(4, n_lanes)
-    say(f"ota_array full size {n_lanes}")
+    say(f"ota_array full size {n_lanes} tiling={tiling}")
     eng = lanes.Engine(fam.deck.arrays())
     _, _, q = eng.pattern()
-    got = eng.solve_op(vals)
+    got = eng.solve_op(vals, tiling=tiling)
     assert (got["reasons"] == 0).all()
     c = oracle.Circuit(fam.deck.arrays())
     pick = np.arange(0, n_lanes, 1021)
