@@ -50,15 +50,32 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=2048)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tiling", type=int, default=0, help="0 autotune, 1 lane per wavefront, 2 lane per thread")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only run one plain device copy of this many MiB (scales the HBM byte counters) and exit")
     return ap.parse_args()
+
+
+def calibrate_copy(mib):
+    import torch
+    src = torch.empty(mib << 20, dtype=torch.uint8, device="cuda")
+    src.fill_(1)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": mib << 20, "copy_bytes_written": mib << 20}))
 
 
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
-              "(lanes are independent points: see DESIGN.md for how they shard)", file=sys.stderr)
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported: sweep points are "
+              "independent, so more GPUs would each take a slice of the points", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy_mib:
+        calibrate_copy(args.calibrate_copy_mib)
+        return
     from espice_amd import circuits, lanes
 
     fam = circuits.ota_array()
@@ -82,11 +99,11 @@ def main():
     for mode in args.modes.split(","):
         x0 = x_nom if mode == "fp64_nominal" else None
         for _ in range(args.warmup):
-            eng.solve_op(vals, x0=x0)
+            eng.solve_op(vals, x0=x0, tiling=args.tiling)
         t_kernel, t_total, conv, iters, lane_it = 0.0, 0.0, 0, 0.0, 0
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            r = eng.solve_op(vals, x0=x0)
+            r = eng.solve_op(vals, x0=x0, tiling=args.tiling)
             tm = eng.timing()
             t_kernel += tm.kernel_ms
             t_total += tm.total_ms

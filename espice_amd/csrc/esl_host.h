@@ -1,0 +1,53 @@
+// Host-side helpers shared by the lane engine and the warm-start network:
+// device buffers, HIP error capture and generation-checked handles.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/espice_lanes.h"
+
+namespace eslh {
+
+extern thread_local std::string g_last_error;
+esl_status fail_hip(hipError_t e, const char *what);
+
+#define HIPCHK(x)                                             \
+    do {                                                      \
+        hipError_t e_ = (x);                                  \
+        if (e_ != hipSuccess) return eslh::fail_hip(e_, #x);  \
+    } while (0)
+
+// One device allocation that grows on demand.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t ensure(size_t b)
+    {
+        if (b <= bytes && p) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc(&p, b ? b : 8);
+        if (e == hipSuccess) bytes = b;
+        return e;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Handles: generation-checked tokens (never addresses); type 1 engine,
+// 2 tape, 3 network.
+void *make_handle(void *obj, int type);
+void *lookup(void *h, int type);
+void *release(void *h, int type);
+bool device_present();
+
+}  // namespace eslh

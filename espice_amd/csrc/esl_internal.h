@@ -64,39 +64,39 @@ struct DeckProg {
     // pattern slot of each Jacobian entry (kNoSlot for a ground entry):
     // R [4 * nR], V [4 * nV], MOS1 [11 * n_mos], in the stamp entry order
     const uint32_t *r_slot, *v_slot, *m_slot;
+    // the pattern by rows, columns ascending: [n + 1], [nnz], [nnz] (pattern slot)
+    const uint32_t *csr_ptr, *csr_col, *csr_slot;
 };
 constexpr uint32_t kNoSlot = 0xffffffffu;
 
-// Words of the refactor and solve chain streams, uint2 each:
-// x = a | (b << 16), y = c | (kind << 16); slots are 16-bit LDS indices.
-// A chain is one slot's whole sequence of updates, accumulated in a
-// register: START loads the slot, each term updates the accumulator with
-// final operands, and the closing word stores it (or divides and stores).
-enum WordKind : uint32_t {
-    W_NOP = 0,
-    W_TSTART = 1,  // acc = tv[a]; target = a
-    W_FLOP = 2,    // acc = acc - tv[b] * tv[c]
-    W_TSTORE = 3,  // tv[target] = acc
-    W_DIVU = 4,    // unscaled L entry: tv[target] = acc / tv[c]; growth + wide tests
-    W_DIVS = 5,    // scaled-pivot L entry: tv[target] = acc / tv[c]; non-finite test
-    W_VOID = 6,    // tv[a] = 1 (fabricated pivot)
-    W_YSTART = 7,  // acc = y[a]; target = a
-    W_FWD = 8,     // acc = y[c] != 0 ? acc - tv[b] * y[c] : acc
-    W_YSTORE = 9,  // y[target] = acc
-    W_BDIV = 10,   // y[target] = acc / tv[b]
-    W_KINDS = 11
+// Words of the refactor and solve row programs, uint2 each:
+// x = d | (b << 16), y = c | (kind << 16) | (flags << 20); d, b, c are
+// 16-bit LDS indices into [factor slots | solve vector]. One row holds one
+// word per thread and every word of a row has the same kind, so a row runs
+// as one branch-free wavefront step; rows run in order, and a word's operands
+// were final in an earlier row. Idle threads of a row work on a private pad
+// slot.
+enum RowKind : uint32_t {
+    R_NOP = 0,    // whole row idle (stream padding)
+    R_MAC = 1,    // M[d] = M[d] - M[b] * M[c]
+    R_MACZ = 2,   // M[d] = M[c] != 0 ? M[d] - M[b] * M[c] : M[d]
+    R_DIV = 3,    // M[d] = M[d] / M[b]
+    R_KINDS = 4
 };
+constexpr uint32_t kFlagUnscaled = 1;   // R_DIV on an unscaled column: growth + wide tests
+constexpr uint32_t kFlagScaled = 2;     // R_DIV on a scaled column: non-finite test
+inline constexpr uint32_t row_kind(uint32_t y) { return (y >> 16) & 0xfu; }
+inline constexpr uint32_t row_flags(uint32_t y) { return (y >> 20) & 0x3u; }
 // Reduce-stream words: a target word sets the accumulator's slot (bit 30:
 // residual row, else factor slot); a cell word adds one staged value.
 constexpr uint32_t kRedTarget = 0x80000000u, kRedRhs = 0x40000000u, kRedNop = 0x3fffffffu;
-constexpr int kGroup = 4;      // rows per step group
 constexpr int kRing = 16;      // rows in flight; streams carry kRing pad rows
 
 struct TapeProg {
-    uint32_t n_tv, n_u, n_l, lx0;   // tv = [U | diag | L | discard]
+    uint32_t n_tv, n_u, n_l, lx0;   // tv = [U | diag | L | discard | void extras | 64 pads]
     uint32_t discard, n_levels, n_flev, n_blev;
-    uint32_t n_void, n_rlev, n_slev, pad2;
-    uint32_t rw_rows, sw_rows, pad3, pad4;   // stream rows (multiples of kRing)
+    uint32_t n_void, n_rlev, n_slev, pad_slot;   // n_rlev/n_slev: critical paths in rows
+    uint32_t rw_rows, sw_rows, pad3, pad4;       // stream rows (multiples of kRing)
     double growth, wide;
     const uint32_t *amap;      // [nnz] pattern slot -> tv slot
     const uint32_t *pinv;      // [n] row -> pivot step
@@ -114,12 +114,9 @@ struct TapeProg {
     const uint32_t *blev_ptr, *blev_rows;            // backward solve rows by level
     const uint32_t *urow_ptr, *urow_k, *urow_slot;   // U by rows, descending column
     const uint32_t *void_slots;                      // pattern slots a void pivot needs zero
-    // chain streams: [rows + kRing][64] words, levels back to back, each level
-    // a multiple of kGroup rows; bar[g] != 0 puts a barrier after group g
-    const uint2_t *rw;          // refactor words
-    const uint32_t *rw_bar;
-    const uint2_t *sw;          // forward + backward solve words
-    const uint32_t *sw_bar;
+    // row programs: [rows + kRing][64] words (the kRing tail rows are R_NOP)
+    const uint2_t *rw;          // refactor
+    const uint2_t *sw;          // forward + backward solve
     const uint32_t *red;        // assembly reduce words, per chunk [len + kRing][64]
     const uint32_t *red_off;    // [n_chunks] word offset of each chunk's stream
     const uint32_t *red_len;    // [n_chunks] rows of each chunk's stream (multiple of kRing)
@@ -129,12 +126,20 @@ struct TapeProg {
 
 // Per-lane results of the Newton kernel.
 enum LaneStatus : uint32_t {
-    kConverged = 0, kNoConvergence = 1, kSingular = 3, kRepivot = 100
+    kConverged = 0, kNoConvergence = 1, kSingular = 3, kRepivot = 100,
+    kToFp64 = 101   // FP32 factors failed on a fresh tape, or refinement stalled
 };
 
+// Where a lane that left the kernel early resumes. fresh != 0: the host just
+// gave the lane a tape built from its matrix at iteration `iter`.
+// k0: the iteration the lane's current start began at; warm: 0 no learned
+// start, 1 learned start active (f0 = its first residual max), 2 restarted
+// from the nominal point.
 struct LaneResume {
-    uint32_t iter, init_fix, tape, pad;
+    uint32_t iter, init_fix, tape, fresh;
     double limit;
+    uint32_t k0, warm;
+    double f0;
 };
 
 // Arguments of the Newton kernel (op_newton).
@@ -151,8 +156,16 @@ struct OpArgs {
     double *a_out;              // dump: [grid][nnz] Jacobian in pattern order
     double *f_out;              // dump: [grid][n] residual
     double reltol, abstol, vntol, residual_tol;
-    uint32_t max_iter, mode;    // mode: 0 cold, 1 given x, 2 resume
+    uint32_t max_iter, mode;    // mode: 0 cold, 1 given x, 2 resume, 3 learned x + nominal fallback
     uint32_t tv_cap, dump;
+    const double *x_nom;        // [n] mode 3: the nominal operating point
+    uint8_t *fell_back;         // [N] mode 3: 1 where the lane restarted from x_nom
+    // FP32 factors with FP64 refinement: lanes with lane64[lane] == 0 factor
+    // in FP32; a64 [N][nnz] holds their FP64 Jacobian for the residual
+    uint32_t fp32, refine_max;
+    double refine_tol;
+    const uint8_t *lane64;      // [N]
+    double *a64;                // [N][nnz]
     // optional: [N][kProfPhases] shader cycles per phase (assemble, refactor,
     // solve, update + limits + gates), summed over the lane's iterations
     unsigned long long *prof;

@@ -17,12 +17,15 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <queue>
 #include <string>
 #include <vector>
 
 #include "../../include/espice_lanes.h"
 #include "esl_internal.h"
 #include "eslmath.h"
+#include "esl_host.h"
+#include "esl_warm.h"
 
 extern "C" hipError_t esl_launch_op_newton(const void *a, uint32_t grid, uint32_t lds_bytes, hipStream_t s);
 extern "C" hipError_t esl_launch_tape_replay(const void *r, uint32_t lanes, uint32_t lds_bytes, hipStream_t s);
@@ -33,19 +36,7 @@ extern "C" hipError_t esl_launch_transpose_live(const double *in, double *out, u
 using std::vector;
 using u32 = uint32_t;
 
-namespace {
-
-constexpr u32 NONE = 0xffffffffu;
-constexpr int NPARAM = ESL_NPARAM;
-
-using HostOpArgs = esl::OpArgs;
-using HostReplayArgs = esl::ReplayArgs;
-
-#define HIPCHK(x)                                       \
-    do {                                                \
-        hipError_t e_ = (x);                            \
-        if (e_ != hipSuccess) return fail_hip(e_, #x);  \
-    } while (0)
+namespace eslh {
 
 thread_local std::string g_last_error;
 esl_status fail_hip(hipError_t e, const char *what)
@@ -54,27 +45,81 @@ esl_status fail_hip(hipError_t e, const char *what)
     return ESL_DEVICE_UNAVAILABLE;
 }
 
-// ---------------------------------------------------------------- device memory
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    ~DevBuf() { release(); }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    hipError_t ensure(size_t b)
-    {
-        if (b <= bytes && p) return hipSuccess;
-        release();
-        hipError_t e = hipMalloc(&p, b ? b : 8);
-        if (e == hipSuccess) bytes = b;
-        return e;
-    }
-    template <class T> T *as() const { return static_cast<T *>(p); }
+namespace {
+struct Slot {
+    u32 gen = 0;
+    void *ptr = nullptr;
+    int type = 0;   // 1 engine, 2 tape, 3 network
 };
+std::mutex g_mu;
+vector<Slot> g_slots;
+}  // namespace
+void *make_handle(void *obj, int type)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (u32 i = 0; i < g_slots.size(); i++)
+        if (!g_slots[i].ptr && g_slots[i].gen < 0xfffff) {
+            g_slots[i].gen++;
+            g_slots[i].ptr = obj;
+            g_slots[i].type = type;
+            return reinterpret_cast<void *>((uintptr_t)(((uint64_t)g_slots[i].gen << 20) | (i + 1)));
+        }
+    g_slots.push_back(Slot{1, obj, type});
+    return reinterpret_cast<void *>((uintptr_t)(((uint64_t)1 << 20) | g_slots.size()));
+}
+
+void *lookup(void *h, int type)
+{
+    if (!h) return nullptr;
+    const uint64_t v = (uint64_t)(uintptr_t)h;
+    const u32 idx = (u32)(v & 0xfffff), gen = (u32)(v >> 20);
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (idx == 0 || idx > g_slots.size()) return nullptr;
+    Slot &s = g_slots[idx - 1];
+    if (s.gen != gen || !s.ptr || s.type != type) return nullptr;
+    return s.ptr;
+}
+
+void *release(void *h, int type)
+{
+    if (!h) return nullptr;
+    const uint64_t v = (uint64_t)(uintptr_t)h;
+    const u32 idx = (u32)(v & 0xfffff), gen = (u32)(v >> 20);
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (idx == 0 || idx > g_slots.size()) return nullptr;
+    Slot &s = g_slots[idx - 1];
+    if (s.gen != gen || !s.ptr || s.type != type) return nullptr;
+    void *p = s.ptr;
+    s.ptr = nullptr;
+    return p;
+}
+
+void put_diag(char *diag, size_t cap, const std::string &msg)
+{
+    if (diag && cap) {
+        size_t n = std::min(cap - 1, msg.size());
+        memcpy(diag, msg.data(), n);
+        diag[n] = 0;
+    }
+}
+
+bool device_present()
+{
+    int count = 0;
+    return hipGetDeviceCount(&count) == hipSuccess && count > 0;
+}
+
+
+}  // namespace eslh
+using namespace eslh;
+
+namespace {
+
+constexpr u32 NONE = 0xffffffffu;
+constexpr int NPARAM = ESL_NPARAM;
+
+using HostOpArgs = esl::OpArgs;
+using HostReplayArgs = esl::ReplayArgs;
 
 // Packs host arrays into one device allocation.
 struct Packer {
@@ -295,54 +340,112 @@ vector<u32> min_degree(u32 n, const vector<u32> &col_ptr, const vector<u32> &row
     return q;
 }
 
-// A chain: one slot's updates in serial order, run by one thread into a
-// register. Chains are grouped by level (a chain's level is one more than the
-// level of every chain that produces one of its operands) and, within a
-// level, dealt to the 64 threads longest first; each thread's words are laid
-// out row by row so a wavefront reads one coalesced row per step.
-struct Chain {
-    vector<esl::uint2_t> words;
+// Row programs. Every update of the refactor or the solve is one op
+// (d, b, c, kind); an op depends on the previous op on its target and on the
+// ops that finish its operands (plus, in the solve, on the forward readers of
+// a value the backward pass overwrites). A list scheduler deals the ops into
+// rows of 64, one kind per row: each new row takes the kind of the ready op
+// with the longest path to the end and fills up with ready ops of that kind,
+// longest path first. Each target's ops stay in their serial order, so every
+// slot sees exactly the sequence of roundings the column-by-column replay
+// makes.
+struct SchedOp {
+    u32 d, b, c, kind, flags;
 };
 
-struct ChainStreams {
-    vector<esl::uint2_t> words;
-    vector<u32> off, len, bar;
-    u32 rows = 0;
+struct RowProgram {
+    vector<esl::uint2_t> words;   // [rows + kRing][64]
+    u32 rows = 0;                  // multiple of kRing
+    u32 used = 0;                  // rows before padding
+    u32 critical = 0;              // longest dependency chain, in ops
 };
 
-void emit_chains(vector<vector<Chain>> &by_level, ChainStreams &out)
-{
-    out.words.clear();
-    out.off.clear();
-    out.len.clear();
-    size_t row = 0;
-    for (auto &lev : by_level) {
-        if (lev.empty()) continue;
-        std::stable_sort(lev.begin(), lev.end(),
-                         [](const Chain &a, const Chain &b) { return a.words.size() > b.words.size(); });
-        vector<vector<esl::uint2_t>> thr(esl::kWave);
-        for (auto &ch : lev) {
-            size_t best = 0;
-            for (size_t t = 1; t < thr.size(); t++)
-                if (thr[t].size() < thr[best].size()) best = t;
-            thr[best].insert(thr[best].end(), ch.words.begin(), ch.words.end());
-        }
-        size_t L = 0;
-        for (auto &t : thr) L = std::max(L, t.size());
-        L = (L + esl::kGroup - 1) / esl::kGroup * esl::kGroup;
-        out.off.push_back((u32)row);
-        out.len.push_back((u32)L);
-        out.words.resize((row + L) * esl::kWave, esl::uint2_t{0, 0});
-        for (size_t t = 0; t < thr.size(); t++)
-            for (size_t w = 0; w < thr[t].size(); w++) out.words[(row + w) * esl::kWave + t] = thr[t][w];
-        row += L;
+class OpGraph {
+  public:
+    explicit OpGraph(u32 n_index) : last_(n_index, NONE) {}
+    // add an op whose operands are read now; extra_pred adds one more edge
+    u32 add(u32 d, u32 b, u32 c, u32 kind, u32 flags, const vector<u32> &extra = {})
+    {
+        const u32 id = (u32)ops_.size();
+        ops_.push_back({d, b, c, kind, flags});
+        succ_.emplace_back();
+        indeg_.push_back(0);
+        auto edge = [&](u32 from) {
+            if (from == NONE || (!succ_[from].empty() && succ_[from].back() == id)) return;
+            succ_[from].push_back(id);
+            indeg_[id]++;
+        };
+        edge(last_[d]);
+        edge(last_[b]);
+        edge(last_[c]);
+        for (u32 e : extra) edge(e);
+        last_[d] = id;
+        return id;
     }
-    const size_t rows = (row + esl::kRing - 1) / esl::kRing * esl::kRing;
-    out.rows = (u32)rows;
-    out.words.resize((rows + esl::kRing) * esl::kWave, esl::uint2_t{0, 0});
-    out.bar.assign(rows / esl::kGroup, 0);
-    for (size_t l = 0; l < out.off.size(); l++) out.bar[(out.off[l] + out.len[l]) / esl::kGroup - 1] = 1;
-}
+    u32 last(u32 slot) const { return last_[slot]; }
+    size_t size() const { return ops_.size(); }
+
+    void schedule(u32 pad_base, RowProgram &out) const
+    {
+        const size_t m = ops_.size();
+        vector<u32> height(m, 1);
+        for (size_t i = m; i > 0;) {
+            i--;
+            for (u32 s : succ_[i]) height[i] = std::max(height[i], height[s] + 1);
+        }
+        u32 crit = 0;
+        for (u32 h : height) crit = std::max(crit, h);
+        vector<u32> indeg = indeg_;
+        auto cmp = [&](u32 a, u32 b) { return height[a] != height[b] ? height[a] < height[b] : a > b; };
+        vector<std::priority_queue<u32, vector<u32>, decltype(cmp)>> ready;
+        for (u32 k = 0; k < esl::R_KINDS; k++) ready.emplace_back(cmp);
+        for (u32 i = 0; i < m; i++)
+            if (indeg[i] == 0) ready[ops_[i].kind].push(i);
+        vector<vector<u32>> rows;
+        size_t done = 0;
+        while (done < m) {
+            u32 kind = esl::R_KINDS;
+            for (u32 k = 1; k < esl::R_KINDS; k++) {
+                if (ready[k].empty()) continue;
+                if (kind == esl::R_KINDS || cmp(ready[kind].top(), ready[k].top())) kind = k;
+            }
+            vector<u32> row;
+            while (!ready[kind].empty() && row.size() < (size_t)esl::kWave) {
+                row.push_back(ready[kind].top());
+                ready[kind].pop();
+            }
+            done += row.size();
+            for (u32 i : row)
+                for (u32 s : succ_[i])
+                    if (--indeg[s] == 0) ready[ops_[s].kind].push(s);
+            rows.push_back(std::move(row));
+        }
+        const u32 used = (u32)rows.size();
+        const u32 padded = (used + esl::kRing - 1) / esl::kRing * esl::kRing;
+        out.words.assign((size_t)(padded + esl::kRing) * esl::kWave, esl::uint2_t{0, 0});
+        for (u32 r = 0; r < used; r++) {
+            const u32 kind = ops_[rows[r][0]].kind;
+            for (u32 t = 0; t < (u32)esl::kWave; t++) {
+                const u32 pad = pad_base + t;
+                esl::uint2_t w{pad | (pad << 16), pad | (kind << 16)};
+                if (t < rows[r].size()) {
+                    const SchedOp &o = ops_[rows[r][t]];
+                    w = esl::uint2_t{o.d | (o.b << 16), o.c | (o.kind << 16) | (o.flags << 20)};
+                }
+                out.words[(size_t)r * esl::kWave + t] = w;
+            }
+        }
+        out.rows = padded;
+        out.used = used;
+        out.critical = crit;
+    }
+
+  private:
+    vector<SchedOp> ops_;
+    vector<vector<u32>> succ_;
+    vector<u32> indeg_;
+    vector<u32> last_;
+};
 
 // ---------------------------------------------------------------- tape program
 struct TapeHostProg {
@@ -350,7 +453,7 @@ struct TapeHostProg {
     vector<u32> amap, pinv, qinv, flags, lev_ptr, lev_cols, up, ui, lp, fptr, dst, li;
     vector<u32> flev_ptr, flev_rows, lrow_ptr, lrow_k, lrow_slot;
     vector<u32> blev_ptr, blev_rows, urow_ptr, urow_k, urow_slot, void_slots;
-    ChainStreams rch, sch;
+    RowProgram rch, sch;
     vector<u32> red, red_off, red_len, diag_tv;
     uint64_t flops = 0;
     DevBuf buf;
@@ -459,121 +562,77 @@ void build_tape_prog(const HostTape &t, const vector<u32> &col_ptr, double growt
     levels_to_lists(flev, o.flev_ptr, o.flev_rows, nflev);
     levels_to_lists(blev, o.blev_ptr, o.blev_rows, nblev);
     o.void_slots = t.void_slots;
-    // Refactor chains (the column replay of sparse_lu.zig:916-1024): every
-    // slot starts from its assembled value and takes its multiply-subtracts in
-    // stored topological order; an L slot ends with its pivot division. Each
-    // operand is final when read (a U entry before its column's later
-    // entries, an L column before any later column), so a chain depends only
-    // on the chains that produce its operands.
-    const u32 n_tv = next_slot;
+    // Refactor ops (the column replay of sparse_lu.zig:916-1024): every slot
+    // starts from its assembled value and takes its multiply-subtracts in
+    // stored topological order; an L slot ends with its pivot division. A
+    // void column's pivot is set to 1 before the rows run and its L slots
+    // stay at their assembled zero.
+    const u32 pad_base = next_slot;
+    const u32 n_tv = pad_base + esl::kWave;
+    const u32 yb = n_tv;   // y(j) sits at index n_tv + j
     {
-        vector<vector<esl::uint2_t>> terms(n_tv);
+        vector<vector<std::pair<u32, u32>>> terms(n_tv);   // (L slot, U slot) per target
         for (u32 k = 0; k < n; k++) {
             u32 f = o.fptr[k];
             for (u32 p = t.up[k]; p < t.up[k + 1]; p++) {
                 const u32 i = t.ui[p];
-                for (u32 qq = t.lp[i]; qq < t.lp[i + 1]; qq++, f++)
-                    terms[o.dst[f]].push_back(mkop(esl::W_FLOP, 0, lx0 + qq, p));
+                for (u32 qq = t.lp[i]; qq < t.lp[i + 1]; qq++, f++) terms[o.dst[f]].push_back({lx0 + qq, p});
             }
         }
-        vector<int> level(n_tv, -1);
-        vector<vector<Chain>> by_level;
-        auto put = [&](u32 slot, int lev, Chain &&ch) {
-            level[slot] = lev;
-            if ((size_t)lev >= by_level.size()) by_level.resize(lev + 1);
-            by_level[lev].push_back(std::move(ch));
-        };
-        auto chain_of = [&](u32 slot, u32 close_kind, u32 piv) {
-            int lev = 0;
-            for (auto &w : terms[slot]) {
-                lev = std::max(lev, level[w.x >> 16] + 1);
-                lev = std::max(lev, level[w.y & 0xffff] + 1);
-            }
-            if (close_kind == esl::W_DIVU || close_kind == esl::W_DIVS) lev = std::max(lev, level[piv] + 1);
-            Chain ch;
-            ch.words.push_back(mkop(esl::W_TSTART, slot, slot, 0));
-            ch.words.insert(ch.words.end(), terms[slot].begin(), terms[slot].end());
-            ch.words.push_back(mkop(close_kind, slot, piv, 0));
-            put(slot, lev, std::move(ch));
+        OpGraph g(n_tv);
+        auto macs = [&](u32 slot) {
+            for (auto &e : terms[slot]) g.add(slot, e.first, e.second, esl::R_MAC, 0);
         };
         for (u32 k = 0; k < n; k++) {
             const bool void_k = t.flags[k] & esl::kVoidBit;
-            for (u32 p = t.up[k]; p < t.up[k + 1]; p++)
-                if (!terms[p].empty()) chain_of(p, esl::W_TSTORE, 0);
+            for (u32 p = t.up[k]; p < t.up[k + 1]; p++) macs(p);
+            if (void_k) continue;
             const u32 piv = nu + k;
-            if (void_k) {
-                Chain ch;
-                ch.words.push_back(mkop(esl::W_VOID, piv, 0, 0));
-                int lev = 0;
-                for (auto &w : terms[piv]) lev = std::max(lev, std::max(level[w.x >> 16], level[w.y & 0xffff]) + 1);
-                put(piv, lev, std::move(ch));
-            } else if (!terms[piv].empty()) {
-                chain_of(piv, esl::W_TSTORE, 0);
-            }
-            if (!void_k) {
-                const u32 kind = (t.flags[k] & esl::kScaledBit) ? esl::W_DIVS : esl::W_DIVU;
-                for (u32 qq = t.lp[k]; qq < t.lp[k + 1]; qq++) chain_of(lx0 + qq, kind, piv);
+            macs(piv);
+            const u32 fl = (t.flags[k] & esl::kScaledBit) ? esl::kFlagScaled : esl::kFlagUnscaled;
+            for (u32 qq = t.lp[k]; qq < t.lp[k + 1]; qq++) {
+                macs(lx0 + qq);
+                g.add(lx0 + qq, piv, lx0 + qq, esl::R_DIV, fl);
             }
         }
-        emit_chains(by_level, o.rch);
-        o.prog.n_rlev = (u32)o.rch.off.size();
+        g.schedule(pad_base, o.rch);
+        o.prog.n_rlev = o.rch.critical;
     }
-    // Solve chains (lane_kernels.zig:123-143 by rows): y row r takes the
+    // Solve ops (lane_kernels.zig:123-143 by rows): y row r takes the
     // forward terms L(r, k) y(k) in ascending k, then the backward terms
-    // U(r, k) z(k) in descending k and the division by U(r, r); the forward
-    // and backward halves are separate chains because forward terms read the
-    // forward value of y(k).
+    // U(r, k) z(k) in descending k and the division by U(r, r). The backward
+    // ops of a row overwrite y(r), so they wait for every forward op that
+    // reads y(r).
     {
-        const u32 yb = n_tv;   // y(j) sits at factor-slot index n_tv + j
         vector<vector<std::pair<u32, u32>>> lrow(n), urow(n);
         for (u32 k = 0; k < n; k++) {
             for (u32 qq = t.lp[k]; qq < t.lp[k + 1]; qq++) lrow[t.li[qq]].push_back({k, lx0 + qq});
             for (u32 p = t.up[k]; p < t.up[k + 1]; p++) urow[t.ui[p]].push_back({k, p});
         }
-        vector<int> lf(n, -1), lb(n, -1), fwd_readers(n, -1);
-        vector<vector<Chain>> by_level;
-        auto put = [&](int lev, Chain &&ch) {
-            if ((size_t)lev >= by_level.size()) by_level.resize(lev + 1);
-            by_level[lev].push_back(std::move(ch));
-        };
+        OpGraph g(n_tv + n);
+        vector<vector<u32>> readers(n);
         for (u32 r = 0; r < n; r++) {
             auto &v = lrow[r];
-            if (v.empty()) continue;
             std::sort(v.begin(), v.end());
-            int lev = 0;
-            Chain ch;
-            ch.words.push_back(mkop(esl::W_YSTART, yb + r, yb + r, 0));
-            for (auto &e : v) {
-                lev = std::max(lev, lf[e.first] + 1);
-                ch.words.push_back(mkop(esl::W_FWD, 0, e.second, yb + e.first));
-            }
-            ch.words.push_back(mkop(esl::W_YSTORE, 0, 0, 0));
-            lf[r] = lev;
-            for (auto &e : v) fwd_readers[e.first] = std::max(fwd_readers[e.first], lev);
-            put(lev, std::move(ch));
+            for (auto &e : v) readers[e.first].push_back(g.add(yb + r, e.second, yb + e.first, esl::R_MACZ, 0));
         }
         for (u32 i = n; i > 0;) {
             i--;
             auto &v = urow[i];
             std::sort(v.begin(), v.end(), [](auto &a, auto &b) { return a.first > b.first; });
-            // after its own forward chain, and after every forward chain that
-            // reads the forward value of y(i), which this chain overwrites
-            int lev = std::max(lf[i], fwd_readers[i]) + 1;
-            Chain ch;
-            ch.words.push_back(mkop(esl::W_YSTART, yb + i, yb + i, 0));
+            bool first = true;
             for (auto &e : v) {
-                lev = std::max(lev, lb[e.first] + 1);
-                ch.words.push_back(mkop(esl::W_FWD, 0, e.second, yb + e.first));
+                g.add(yb + i, e.second, yb + e.first, esl::R_MACZ, 0, first ? readers[i] : vector<u32>{});
+                first = false;
             }
-            ch.words.push_back(mkop(esl::W_BDIV, 0, nu + i, 0));
-            lb[i] = lev;
-            put(lev, std::move(ch));
+            g.add(yb + i, nu + i, yb + i, esl::R_DIV, 0, first ? readers[i] : vector<u32>{});
         }
-        emit_chains(by_level, o.sch);
-        o.prog.n_slev = (u32)o.sch.off.size();
+        g.schedule(pad_base, o.sch);
+        o.prog.n_slev = o.sch.critical;
     }
     esl::TapeProg &P = o.prog;
-    P.n_tv = next_slot;
+    P.n_tv = n_tv;
+    P.pad_slot = pad_base;
     P.n_u = nu;
     P.n_l = nl;
     P.lx0 = lx0;
@@ -599,8 +658,8 @@ hipError_t upload_tape_prog(TapeHostProg &o)
     f[i++] = pk.add(o.blev_ptr); f[i++] = pk.add(o.blev_rows);
     f[i++] = pk.add(o.urow_ptr); f[i++] = pk.add(o.urow_k); f[i++] = pk.add(o.urow_slot);
     f[i++] = pk.add(o.void_slots);
-    f[i++] = pk.add(o.rch.words); f[i++] = pk.add(o.rch.bar);
-    f[i++] = pk.add(o.sch.words); f[i++] = pk.add(o.sch.bar);
+    f[i++] = pk.add(o.rch.words);
+    f[i++] = pk.add(o.sch.words);
     f[i++] = pk.add(o.red);
     f[i++] = pk.add(o.red_off); f[i++] = pk.add(o.red_len); f[i++] = pk.add(o.diag_tv);
     f[i++] = pk.add(o.li);
@@ -620,9 +679,7 @@ hipError_t upload_tape_prog(TapeHostProg &o)
     T.urow_ptr = P(i++); T.urow_k = P(i++); T.urow_slot = P(i++);
     T.void_slots = P(i++);
     T.rw = reinterpret_cast<const esl::uint2_t *>(P(i++));
-    T.rw_bar = P(i++);
     T.sw = reinterpret_cast<const esl::uint2_t *>(P(i++));
-    T.sw_bar = P(i++);
     T.rw_rows = o.rch.rows;
     T.sw_rows = o.sch.rows;
     T.red = P(i++);
@@ -714,8 +771,26 @@ struct Compiled {
     vector<double> r_val, v_val, i_val, m_par;
     vector<int32_t> r_bind, v_bind, i_bind, m_bind_vto, m_bind_kp;
     vector<u32> r_slot, v_slot, m_slot;
+    vector<u32> csr_ptr, csr_col, csr_slot;   // the pattern by rows, columns ascending
     u32 n_mos = 0;
 };
+
+void build_csr(Compiled &c)
+{
+    const u32 n = c.n, nnz = c.nnz;
+    c.csr_ptr.assign(n + 1, 0);
+    c.csr_col.assign(nnz, 0);
+    c.csr_slot.assign(nnz, 0);
+    for (u32 p = 0; p < nnz; p++) c.csr_ptr[c.row_idx[p] + 1]++;
+    for (u32 i = 0; i < n; i++) c.csr_ptr[i + 1] += c.csr_ptr[i];
+    vector<u32> at(c.csr_ptr.begin(), c.csr_ptr.end() - 1);
+    for (u32 j = 0; j < n; j++)
+        for (u32 p = c.col_ptr[j]; p < c.col_ptr[j + 1]; p++) {
+            const u32 e = at[c.row_idx[p]]++;
+            c.csr_col[e] = j;
+            c.csr_slot[e] = p;
+        }
+}
 
 u32 find_slot(const Compiled &c, u32 r, u32 col)
 {
@@ -927,7 +1002,9 @@ struct Engine {
     std::map<vector<u32>, u32> tape_index;
     DevBuf tape_table;
     // per-solve buffers
-    DevBuf live, x, lim, resume, status, iters, lane_ids, a_out, f_out;
+    DevBuf live, x, lim, resume, status, iters, lane_ids, a_out, f_out, a64, lane64, fell_back, x_nom;
+    esl_mlp *warm_net = nullptr;   // handle of the attached warm-start network
+    bool has_nominal = false;
     // lane-per-thread state, lane-major
     DevBuf live_t, tvg, rhsg, xg, xoldg, yg, sdiagg, limg;
     u32 tuned = 0;   // ESL_TILING_* chosen by the autotuner
@@ -960,6 +1037,8 @@ hipError_t upload_deck(Engine &e)
            o_vv = pk.add(c.v_val), o_vb = pk.add(c.v_bind), o_it = pk.add(c.i_t), o_iv = pk.add(c.i_val),
            o_ib = pk.add(c.i_bind), o_mt = pk.add(c.m_t), o_mp = pk.add(c.m_par), o_mbv = pk.add(c.m_bind_vto),
            o_mbk = pk.add(c.m_bind_kp), o_rs = pk.add(c.r_slot), o_vs = pk.add(c.v_slot), o_ms = pk.add(c.m_slot);
+    build_csr(c);
+    size_t o_cp = pk.add(c.csr_ptr), o_cc = pk.add(c.csr_col), o_cs = pk.add(c.csr_slot);
     hipError_t err = e.deck_buf.ensure(pk.blob.size());
     if (err != hipSuccess) return err;
     err = hipMemcpy(e.deck_buf.p, pk.blob.data(), pk.blob.size(), hipMemcpyHostToDevice);
@@ -996,6 +1075,9 @@ hipError_t upload_deck(Engine &e)
     dp.r_slot = reinterpret_cast<const u32 *>(b + o_rs);
     dp.v_slot = reinterpret_cast<const u32 *>(b + o_vs);
     dp.m_slot = reinterpret_cast<const u32 *>(b + o_ms);
+    dp.csr_ptr = reinterpret_cast<const u32 *>(b + o_cp);
+    dp.csr_col = reinterpret_cast<const u32 *>(b + o_cc);
+    dp.csr_slot = reinterpret_cast<const u32 *>(b + o_cs);
     return hipSuccess;
 }
 
@@ -1080,27 +1162,26 @@ bool validate_tape(const TapeHostProg &o, u32 n, u32 nnz)
     for (u32 v : o.void_slots) if (v >= nnz) return false;
     if (P.lx0 + P.n_l > P.n_tv || P.n_u + n > P.n_tv) return false;
     if (P.n_tv + n >= 65536) return false;
-    auto chains_ok = [&](const ChainStreams &cs, u32 nlev, bool solve) {
-        if (cs.off.size() != nlev || cs.len.size() != nlev) return false;
-        size_t rows = 0;
-        for (u32 l = 0; l < nlev; l++) {
-            if (cs.off[l] != rows || cs.len[l] % esl::kGroup) return false;
-            rows += cs.len[l];
-        }
-        if (cs.rows % esl::kRing || cs.rows < rows) return false;
-        if (cs.words.size() != (size_t)(cs.rows + esl::kRing) * esl::kWave) return false;
-        if (cs.bar.size() != cs.rows / esl::kGroup) return false;
-        for (const auto &w : cs.words) {
-            const u32 kind = w.y >> 16, a = w.x & 0xffff, b = w.x >> 16, cc = w.y & 0xffff;
-            if (kind >= esl::W_KINDS) return false;
-            const bool y_kind = kind >= esl::W_YSTART;
-            if (kind != esl::W_NOP && y_kind != solve) return false;
-            const u32 lim = solve ? P.n_tv + n : P.n_tv;
-            if (a >= lim || b >= lim || cc >= lim) return false;
+    if (P.pad_slot + esl::kWave != P.n_tv) return false;
+    auto rows_ok = [&](const RowProgram &rp, bool solve) {
+        if (rp.rows % esl::kRing || rp.used > rp.rows) return false;
+        if (rp.words.size() != (size_t)(rp.rows + esl::kRing) * esl::kWave) return false;
+        const u32 lim = solve ? P.n_tv + n : P.n_tv;
+        for (size_t r = 0; r < rp.rows + (size_t)esl::kRing; r++) {
+            const u32 kind = esl::row_kind(rp.words[r * esl::kWave].y);
+            if (kind >= esl::R_KINDS || (kind == esl::R_MAC && solve)) return false;
+            if (kind == esl::R_MACZ && !solve) return false;
+            for (u32 t = 0; t < (u32)esl::kWave; t++) {
+                const esl::uint2_t w = rp.words[r * esl::kWave + t];
+                if (esl::row_kind(w.y) != kind || (w.y >> 22) != 0) return false;
+                if (kind == esl::R_NOP) continue;
+                const u32 d = w.x & 0xffff, b = w.x >> 16, c = w.y & 0xffff;
+                if (d >= lim || b >= lim || c >= lim) return false;
+            }
         }
         return true;
     };
-    if (!chains_ok(o.rch, P.n_rlev, false) || !chains_ok(o.sch, P.n_slev, true)) return false;
+    if (!rows_ok(o.rch, false) || !rows_ok(o.sch, true)) return false;
     if (!o.red_off.empty()) {
         if (o.red_len.size() != o.red_off.size() || o.diag_tv.size() != n) return false;
         for (size_t ci = 0; ci < o.red_off.size(); ci++) {
@@ -1169,70 +1250,6 @@ u32 device_lds_max()
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0) v = 65536;
     cached = (u32)v;
     return cached;
-}
-
-// ---------------------------------------------------------------- handles
-struct Slot {
-    u32 gen = 0;
-    void *ptr = nullptr;
-    int type = 0;   // 1 engine, 2 tape
-};
-std::mutex g_mu;
-vector<Slot> g_slots;
-
-void *make_handle(void *obj, int type)
-{
-    std::lock_guard<std::mutex> lk(g_mu);
-    for (u32 i = 0; i < g_slots.size(); i++)
-        if (!g_slots[i].ptr && g_slots[i].gen < 0xfffff) {
-            g_slots[i].gen++;
-            g_slots[i].ptr = obj;
-            g_slots[i].type = type;
-            return reinterpret_cast<void *>((uintptr_t)(((uint64_t)g_slots[i].gen << 20) | (i + 1)));
-        }
-    g_slots.push_back(Slot{1, obj, type});
-    return reinterpret_cast<void *>((uintptr_t)(((uint64_t)1 << 20) | g_slots.size()));
-}
-
-void *lookup(void *h, int type)
-{
-    if (!h) return nullptr;
-    const uint64_t v = (uint64_t)(uintptr_t)h;
-    const u32 idx = (u32)(v & 0xfffff), gen = (u32)(v >> 20);
-    std::lock_guard<std::mutex> lk(g_mu);
-    if (idx == 0 || idx > g_slots.size()) return nullptr;
-    Slot &s = g_slots[idx - 1];
-    if (s.gen != gen || !s.ptr || s.type != type) return nullptr;
-    return s.ptr;
-}
-
-void *release(void *h, int type)
-{
-    if (!h) return nullptr;
-    const uint64_t v = (uint64_t)(uintptr_t)h;
-    const u32 idx = (u32)(v & 0xfffff), gen = (u32)(v >> 20);
-    std::lock_guard<std::mutex> lk(g_mu);
-    if (idx == 0 || idx > g_slots.size()) return nullptr;
-    Slot &s = g_slots[idx - 1];
-    if (s.gen != gen || !s.ptr || s.type != type) return nullptr;
-    void *p = s.ptr;
-    s.ptr = nullptr;
-    return p;
-}
-
-void put_diag(char *diag, size_t cap, const std::string &msg)
-{
-    if (diag && cap) {
-        size_t n = std::min(cap - 1, msg.size());
-        memcpy(diag, msg.data(), n);
-        diag[n] = 0;
-    }
-}
-
-bool device_present()
-{
-    int count = 0;
-    return hipGetDeviceCount(&count) == hipSuccess && count > 0;
 }
 
 // Registers a tape program (deduplicated by pivot sequence and flags).
@@ -1380,8 +1397,8 @@ esl_status esl_engine_info_get(esl_engine *h, esl_engine_info *out)
     out->tiling = e->tuned;
     if (!e->tapes.empty()) {
         const TapeHostProg &tp = *e->tapes[0];
-        out->refactor_steps = tp.rch.rows;
-        out->solve_steps = tp.sch.rows;
+        out->refactor_steps = tp.rch.used;
+        out->solve_steps = tp.sch.used;
         out->levels = tp.prog.n_rlev;
     }
     return ESL_OK;
@@ -1507,9 +1524,19 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
         if (opt->struct_size < sizeof(esl_op_options)) return ESL_ABI_MISMATCH;
         o = *opt;
     }
-    if (o.start != ESL_START_COLD && o.start != ESL_START_GIVEN) return ESL_INVALID_ARGUMENT;
+    if (o.start != ESL_START_COLD && o.start != ESL_START_GIVEN && o.start != ESL_START_LEARNED)
+        return ESL_INVALID_ARGUMENT;
+    EslMlp *net = nullptr;
+    if (o.start == ESL_START_LEARNED) {
+        net = e->warm_net ? eslw::lookup_mlp(e->warm_net) : nullptr;
+        if (!net || !e->has_nominal) return ESL_INVALID_ARGUMENT;
+        if (o.tiling == ESL_TILING_THREAD) return ESL_INVALID_ARGUMENT;
+    }
     if (o.start == ESL_START_GIVEN && !o.x0) return ESL_INVALID_ARGUMENT;
-    if (o.precision != ESL_FP64) return ESL_INVALID_ARGUMENT;
+    if (o.precision != ESL_FP64 && o.precision != ESL_FP32_REFINED) return ESL_INVALID_ARGUMENT;
+    if (o.precision == ESL_FP32_REFINED && (o.refine_max == 0 || !(o.refine_tol >= 0))) return ESL_INVALID_ARGUMENT;
+    // FP32 factors run in the wavefront tiling only
+    if (o.precision == ESL_FP32_REFINED && o.tiling == ESL_TILING_THREAD) return ESL_INVALID_ARGUMENT;
     if (o.tol.max_iter == 0) return ESL_INVALID_ARGUMENT;
     if (o.tiling > ESL_TILING_THREAD) return ESL_INVALID_ARGUMENT;
     for (size_t i = 0; i < count * e->c.n_live; i++)
@@ -1526,10 +1553,27 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
     HIPCHK(e->status.ensure(count * 4));
     HIPCHK(e->iters.ensure(count * 4));
     HIPCHK(e->lane_ids.ensure(count * 4));
+    const bool fp32 = o.precision == ESL_FP32_REFINED;
+    vector<uint8_t> lane64(count, 0);
+    HIPCHK(e->lane64.ensure(count));
+    if (fp32) HIPCHK(e->a64.ensure(count * (size_t)nnz * 8));
     hipStream_t s = e->stream;
+    HIPCHK(hipMemcpyAsync(e->lane64.p, lane64.data(), count, hipMemcpyHostToDevice, s));
     if (P) HIPCHK(hipMemcpyAsync(e->live.p, values, count * P * 8, hipMemcpyHostToDevice, s));
     if (P) HIPCHK(esl_launch_transpose_live(e->live.as<double>(), e->live_t.as<double>(), (u32)count, P, s));
     if (o.start == ESL_START_GIVEN) HIPCHK(hipMemcpyAsync(e->x.p, o.x0, count * n * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(e->fell_back.ensure(count));
+    if (net) {
+        // the network's guesses straight into the start vectors, on the device
+        HIPCHK(hipEventRecord(e->ev0, s));
+        esl_status ws = eslw::predict_device(*net, e->live.as<double>(), (u32)count, e->x.as<double>(), s);
+        if (ws != ESL_OK) return ws;
+        HIPCHK(hipEventRecord(e->ev1, s));
+        HIPCHK(hipEventSynchronize(e->ev1));
+        float wms = 0;
+        HIPCHK(hipEventElapsedTime(&wms, e->ev0, e->ev1));
+        e->timing.warm_ms = wms;
+    }
     vector<u32> ids(count);
     for (size_t i = 0; i < count; i++) ids[i] = (u32)i;
     HIPCHK(hipMemcpyAsync(e->lane_ids.p, ids.data(), count * 4, hipMemcpyHostToDevice, s));
@@ -1548,6 +1592,13 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
     a.vntol = o.tol.vntol;
     a.residual_tol = o.tol.residual_tol;
     a.max_iter = o.tol.max_iter;
+    a.fp32 = fp32 ? 1u : 0u;
+    a.refine_max = o.refine_max;
+    a.refine_tol = o.refine_tol;
+    a.lane64 = e->lane64.as<uint8_t>();
+    a.x_nom = e->x_nom.as<double>();
+    a.fell_back = net ? e->fell_back.as<uint8_t>() : nullptr;
+    a.a64 = fp32 ? e->a64.as<double>() : nullptr;
     a.prof = nullptr;
     if (e->profiling) {
         HIPCHK(e->prof.ensure(count * esl::kProfPhases * 8));
@@ -1562,7 +1613,7 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
         HIPCHK(e->f_out.ensure(std::max<u32>(n, 1) * 8));
         HostOpArgs d = a;
         d.prof = nullptr;
-        d.mode = o.start == ESL_START_GIVEN ? 1 : 0;
+        d.mode = o.start == ESL_START_COLD ? 0 : 1;
         d.dump = 1;
         d.tv_cap = nnz + 1;
         d.a_out = e->a_out.as<double>();
@@ -1592,15 +1643,15 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
         e->lead_lds = lds_bytes_for(e->tapes[0]->prog.n_tv, n);
     }
     vector<esl::LaneResume> rs(count);
-    for (auto &r : rs) r = esl::LaneResume{0, 0, 0, 0, 1e9};
+    for (auto &r : rs) r = esl::LaneResume{0, 0, 0, 0, 1e9, 0, 0, 0.0};
     HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
-    const u32 mode0 = o.start == ESL_START_GIVEN ? 1 : 0;
+    const u32 mode0 = o.start == ESL_START_COLD ? 0u : o.start == ESL_START_GIVEN ? 1u : 3u;
 
     // Tiling: explicit, or the autotuner's choice. The autotuner times both
     // tilings once per engine on the first batch of at least 1024 points
     // (kernel time of the first launch over every lane) and keeps the faster;
     // below that size a batch runs one wavefront per lane.
-    u32 tiling = o.tiling;
+    u32 tiling = (fp32 || net) ? (u32)ESL_TILING_WAVE : o.tiling;
     if (tiling == ESL_TILING_AUTO) {
         if (e->tuned) {
             tiling = e->tuned;
@@ -1635,11 +1686,22 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
     HIPCHK(hipStreamSynchronize(s));
     // Re-pivot rounds: dump each failing lane's Jacobian at its saved iterate,
     // factor it on the host into its own tape, resume it one wavefront per lane.
+    // Lanes whose FP32 factors failed on a fresh tape, or whose refinement
+    // stalled, redo that iteration and finish in FP64.
     for (int round = 0; round < 1000; round++) {
-        vector<u32> again;
-        for (size_t l = 0; l < count; l++)
+        vector<u32> again, to64;
+        for (size_t l = 0; l < count; l++) {
             if (st[l] == esl::kRepivot) again.push_back((u32)l);
-        if (again.empty()) break;
+            if (st[l] == esl::kToFp64) to64.push_back((u32)l);
+        }
+        if (again.empty() && to64.empty()) break;
+        if (!to64.empty()) {
+            for (u32 l : to64) lane64[l] = 1;
+            e->timing.fp64_lanes += (u32)to64.size();
+            HIPCHK(hipMemcpyAsync(e->lane64.p, lane64.data(), count, hipMemcpyHostToDevice, s));
+        }
+        vector<u32> next = to64;
+        if (!again.empty()) {
         HIPCHK(e->a_out.ensure(again.size() * nnz * 8));
         HIPCHK(e->f_out.ensure(again.size() * n * 8));
         HIPCHK(hipMemcpyAsync(e->lane_ids.p, again.data(), again.size() * 4, hipMemcpyHostToDevice, s));
@@ -1660,7 +1722,6 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
         HIPCHK(hipMemcpyAsync(A.data(), e->a_out.p, A.size() * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(rs.data(), e->resume.p, count * sizeof(esl::LaneResume), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        vector<u32> next;
         for (size_t k = 0; k < again.size(); k++) {
             const u32 l = again[k];
             HostTape t;
@@ -1674,10 +1735,12 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
             esl_status s2 = add_tape(*e, t, id);
             if (s2 != ESL_OK) return s2;
             rs[l].tape = id;
+            rs[l].fresh = 1;
             next.push_back(l);
         }
         HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(e->status.p, st.data(), count * 4, hipMemcpyHostToDevice, s));
+        }
         if (next.empty()) break;
         HIPCHK(hipMemcpyAsync(e->lane_ids.p, next.data(), next.size() * 4, hipMemcpyHostToDevice, s));
         esl_status r = launch_lpw(*e, a, (u32)next.size(), 2, ms);
@@ -1703,6 +1766,11 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
         }
         lane_it += iterations[l];
     }
+    if (net) {
+        vector<uint8_t> fb(count);
+        HIPCHK(hipMemcpy(fb.data(), e->fell_back.p, count, hipMemcpyDeviceToHost));
+        for (size_t l = 0; l < count; l++) e->timing.warm_fallbacks += fb[l];
+    }
     if (e->profiling) {
         e->prof_host.assign(count * esl::kProfPhases, 0);
         HIPCHK(hipMemcpy(e->prof_host.data(), e->prof.p, count * esl::kProfPhases * 8, hipMemcpyDeviceToHost));
@@ -1711,6 +1779,29 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
     e->timing.lane_iterations = lane_it;
     e->timing.total_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    return ESL_OK;
+}
+
+extern "C" esl_status esl_engine_set_warm_start(esl_engine *h, esl_mlp *m, const double *x_nominal)
+{
+    auto *e = static_cast<Engine *>(lookup(h, 1));
+    if (!e) return ESL_INVALID_HANDLE;
+    if (!m) {
+        e->warm_net = nullptr;
+        e->has_nominal = false;
+        return ESL_OK;
+    }
+    EslMlp *net = eslw::lookup_mlp(m);
+    if (!net) return ESL_INVALID_HANDLE;
+    const u32 n = e->c.n;
+    if (!x_nominal || eslw::mlp_n_in(*net) != e->c.n_live || eslw::mlp_n_out(*net) + 1 != n)
+        return ESL_INVALID_ARGUMENT;
+    for (u32 i = 0; i < n; i++)
+        if (!std::isfinite(x_nominal[i])) return ESL_INVALID_ARGUMENT;
+    HIPCHK(e->x_nom.ensure((size_t)n * 8));
+    HIPCHK(hipMemcpy(e->x_nom.p, x_nominal, (size_t)n * 8, hipMemcpyHostToDevice));
+    e->warm_net = m;
+    e->has_nominal = true;
     return ESL_OK;
 }
 

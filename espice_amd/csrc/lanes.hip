@@ -7,11 +7,12 @@
 // syncing twice per Newton tick (src/analysis/gpu_lanes.zig:782-832). Here a
 // lane is a wavefront: its factor slots, residual and vectors sit in LDS;
 // devices evaluate 64 at a time (one per thread) and reduce in instance
-// order; the refactor runs column levels with one thread per column and the
-// solves run row levels with one thread per row, each thread repeating the
-// serial operation order of its column or row; the Newton loop, its gates and
-// the limiting stay on the device until the lane converges, fails, or needs a
-// new pivot sequence.
+// order; the refactor and the solves run as row programs (64 independent
+// updates per wavefront step, each slot's updates in the serial order of the
+// column replay); the Newton loop, its gates and the limiting stay on the
+// device until the lane converges, fails, or needs a new pivot sequence. In
+// the FP32 mode the factors and solves run in FP32 on the same programs and
+// each Newton solve is refined against the FP64 Jacobian.
 //
 // Compiled with -ffp-contract=off: no fused multiply-add anywhere, so every
 // value is bitwise the serial CPU restatement's.
@@ -223,93 +224,73 @@ struct LuFlags {
     uint64_t nonfinite, worst;
 };
 
-struct ChainAcc {
-    double acc;
-    uint32_t tgt;
-};
-
-// One word with its operands A = M[b] and B = M[c] already loaded (M holds
-// the factor slots followed by the solve vector).
-__device__ inline void chain_word(esl::uint2_t w, double A, double B, ChainAcc &s, double *M, double growth,
-                                  LuFlags &f)
+// One row of a row program: every thread runs one op of the row's kind on
+// M = [factor slots | solve vector], in the factor precision R. Rows run in
+// order within the wavefront (LDS accesses of one wavefront complete in issue
+// order), so a row sees the results of every earlier row. The peel tests are
+// computed in FP64 on the values.
+template <typename R>
+__device__ inline void run_row(esl::uint2_t w, R *M, double growth, LuFlags &f)
 {
-    const uint32_t kind = w.y >> 16;
-    if (kind == W_FLOP || kind == W_FWD) {
-        const double t = s.acc - A * B;
-        s.acc = (kind == W_FLOP || B != 0.0) ? t : s.acc;
-    } else if (kind == W_TSTART || kind == W_YSTART) {
-        s.acc = A;
-        s.tgt = w.x & 0xffffu;
-    } else if (kind == W_TSTORE || kind == W_YSTORE) {
-        M[s.tgt] = s.acc;
-    } else if (kind == W_DIVU || kind == W_DIVS || kind == W_BDIV) {
-        const double v = s.acc;
-        const double q = v / A;
-        M[s.tgt] = q;
-        if (kind == W_DIVU) {
-            if (growth > 0) f.bad |= !(fabs(A) >= growth * fabs(v));
-            else f.nonfinite |= dbits(q - q);
-            const uint64_t lb = dbits(fabs(q));
-            f.worst = lb > f.worst ? lb : f.worst;
-        } else if (kind == W_DIVS) {
-            f.nonfinite |= dbits(q - q);
-        }
-    } else if (kind == W_VOID) {
-        M[w.x & 0xffffu] = 1.0;
+    const uint32_t kind = __builtin_amdgcn_readfirstlane(esl::row_kind(w.y));
+    const uint32_t d = w.x & 0xffffu, b = w.x >> 16, c = w.y & 0xffffu;
+    if (kind == R_MAC) {
+        const R v = M[d], A = M[b], B = M[c];
+        M[d] = v - A * B;
+    } else if (kind == R_MACZ) {
+        const R v = M[d], A = M[b], B = M[c];
+        if (B != R(0)) M[d] = v - A * B;
+    } else if (kind == R_DIV) {
+        const R v = M[d], A = M[b];
+        const R q = v / A;
+        M[d] = q;
+        const uint32_t fl = esl::row_flags(w.y);
+        const bool u = fl & kFlagUnscaled;
+        const uint64_t nf = dbits((double)(q - q));
+        const bool check_growth = growth > 0;
+        f.bad |= u && check_growth && !(fabs((double)A) >= growth * fabs((double)v));
+        f.nonfinite |= ((u && !check_growth) || (fl & kFlagScaled)) ? nf : 0;
+        const uint64_t lb = u ? dbits(fabs((double)q)) : 0;
+        f.worst = lb > f.worst ? lb : f.worst;
     }
 }
 
-#define ESL_GROUP(R0, R1, R2, R3, ROW)                                                     \
-    do {                                                                                   \
-        const double a0 = M[R0.x >> 16], b0 = M[R0.y & 0xffffu];                           \
-        const double a1 = M[R1.x >> 16], b1 = M[R1.y & 0xffffu];                           \
-        const double a2 = M[R2.x >> 16], b2 = M[R2.y & 0xffffu];                           \
-        const double a3 = M[R3.x >> 16], b3 = M[R3.y & 0xffffu];                           \
-        chain_word(R0, a0, b0, s, M, growth, f);                                           \
-        chain_word(R1, a1, b1, s, M, growth, f);                                           \
-        chain_word(R2, a2, b2, s, M, growth, f);                                           \
-        chain_word(R3, a3, b3, s, M, growth, f);                                           \
-        R0 = w[((ROW) + 16) * kWave];                                                      \
-        R1 = w[((ROW) + 17) * kWave];                                                      \
-        R2 = w[((ROW) + 18) * kWave];                                                      \
-        R3 = w[((ROW) + 19) * kWave];                                                      \
-        if (bar[(ROW) / kGroup]) __syncthreads();                                          \
-    } while (0)
-
-// Runs a chain stream: 16 rows in flight, a barrier after each level.
-__device__ void run_chains(const esl::uint2_t *words, const uint32_t *bar, uint32_t rows, double *M, double growth,
-                           LuFlags &f, uint32_t tid)
+// Runs a row program with kRing rows of words in flight.
+template <typename R>
+__device__ void run_rows(const esl::uint2_t *words, uint32_t rows, R *M, double growth, LuFlags &f, uint32_t tid)
 {
     const esl::uint2_t *w = words + tid;
-    ChainAcc s{0.0, 0};
-    esl::uint2_t r0 = w[0 * kWave], r1 = w[1 * kWave], r2 = w[2 * kWave], r3 = w[3 * kWave];
-    esl::uint2_t r4 = w[4 * kWave], r5 = w[5 * kWave], r6 = w[6 * kWave], r7 = w[7 * kWave];
-    esl::uint2_t r8 = w[8 * kWave], r9 = w[9 * kWave], r10 = w[10 * kWave], r11 = w[11 * kWave];
-    esl::uint2_t r12 = w[12 * kWave], r13 = w[13 * kWave], r14 = w[14 * kWave], r15 = w[15 * kWave];
+    esl::uint2_t q[kRing];
+#pragma unroll
+    for (int k = 0; k < kRing; k++) q[k] = w[k * kWave];
     for (uint32_t row = 0; row < rows; row += kRing) {
-        ESL_GROUP(r0, r1, r2, r3, row);
-        ESL_GROUP(r4, r5, r6, r7, row + 4);
-        ESL_GROUP(r8, r9, r10, r11, row + 8);
-        ESL_GROUP(r12, r13, r14, r15, row + 12);
+#pragma unroll
+        for (int k = 0; k < kRing; k++) {
+            const esl::uint2_t cur = q[k];
+            q[k] = w[(row + kRing + k) * kWave];
+            run_row<R>(cur, M, growth, f);
+        }
     }
 }
 
-// ---- refactor: true when the lane must re-pivot (lane_kernels.zig:49-118).
-// The growth test |d| >= growth * max(|d|, |v|) runs per L entry, the same
-// predicate because multiplication by growth is monotone.
-__device__ bool refactor(const TapeProg &T, uint32_t n, double *tv, uint32_t tid)
+// ---- the lane replay (lane_kernels.zig:49-118) on tv = [U | diag | L | ...]
+// in precision R; true when the lane must re-pivot ----
+template <typename R>
+__device__ bool refactor(const TapeProg &T, uint32_t n, R *tv, uint32_t tid)
 {
     LuFlags f{false, 0, 0};
-    for (uint32_t v = tid; v < T.n_void; v += kWave) f.bad |= tv[T.amap[T.void_slots[v]]] != 0.0;
-    run_chains(T.rw, T.rw_bar, T.rw_rows, tv, T.growth, f, tid);
+    for (uint32_t v = tid; v < T.n_void; v += kWave) f.bad |= tv[T.amap[T.void_slots[v]]] != R(0);
+    for (uint32_t k = tid; k < n; k += kWave)
+        if (T.flags[k] & kVoidBit) tv[T.n_u + k] = R(1);
+    run_rows<R>(T.rw, T.rw_rows, tv, T.growth, f, tid);
     for (uint32_t p = tid; p < T.n_u; p += kWave) {
-        const double u = tv[p];
-        f.nonfinite |= dbits(u - u);
+        const R u = tv[p];
+        f.nonfinite |= dbits((double)(u - u));
     }
     for (uint32_t k = tid; k < n; k += kWave) {
         if (T.flags[k] & kVoidBit) continue;
-        const double d = tv[T.n_u + k];
-        f.bad |= d == 0.0 || !(fabs(d) < __builtin_inf());
+        const R d = tv[T.n_u + k];
+        f.bad |= d == R(0) || !(fabs((double)d) < __builtin_inf());
     }
     const uint64_t worst = wave_max_u64(f.worst);
     const bool fail = f.bad || f.nonfinite != 0 || !(bitsd(worst) <= T.wide);
@@ -323,7 +304,75 @@ __device__ void solve_neg(const TapeProg &T, uint32_t n, double *tv, const doubl
     for (uint32_t r = tid; r < n; r += kWave) y[T.pinv[r]] = -rhs[r];
     __syncthreads();
     LuFlags f{false, 0, 0};
-    run_chains(T.sw, T.sw_bar, T.sw_rows, tv, 0.0, f, tid);
+    run_rows<double>(T.sw, T.sw_rows, tv, 0.0, f, tid);
+}
+
+// ---- FP32 factors + FP64 refinement of one Newton solve: dx = -A^-1 F.
+// tv32 = [FP32 factor slots | y32]; r64 and dx64 are FP64 vectors; the FP64
+// Jacobian a64 is in pattern order. r = -F - A dx row by row over the
+// pattern in ascending column order. Stops when max|r| <= refine_tol *
+// max(max|F|, max_i sum_j |A_ij dx_j|); a step that does not halve max|r|,
+// or refine_max steps without success, is a stall. ----
+__device__ bool refine_solve(const OpArgs &a, const TapeProg &T, uint32_t n, float *tv32, const double *rhs,
+                             const double *a64, double *dx, double *r, uint32_t tid)
+{
+    const DeckProg &dp = a.dp;
+    float *y32 = tv32 + T.n_tv;
+    uint64_t fm = 0;
+    for (uint32_t i = tid; i < n; i += kWave) {
+        r[i] = -rhs[i];
+        dx[i] = 0.0;
+        const uint64_t b = dbits(fabs(rhs[i]));
+        fm = b > fm ? b : fm;
+    }
+    fm = wave_max_u64(fm);
+    double prev = __builtin_inf();
+    for (uint32_t s = 0; s < a.refine_max; s++) {
+        __syncthreads();
+        for (uint32_t i = tid; i < n; i += kWave) y32[T.pinv[i]] = (float)r[i];
+        __syncthreads();
+        LuFlags f{false, 0, 0};
+        run_rows<float>(T.sw, T.sw_rows, tv32, 0.0, f, tid);
+        __syncthreads();
+        for (uint32_t i = tid; i < n; i += kWave) dx[i] = dx[i] + (double)y32[T.qinv[i]];
+        __syncthreads();
+        uint64_t rm = 0, am = 0;
+        for (uint32_t i = tid; i < n; i += kWave) {
+            double acc = -rhs[i], aa = 0.0;
+            for (uint32_t e = dp.csr_ptr[i]; e < dp.csr_ptr[i + 1]; e++) {
+                const double t = a64[dp.csr_slot[e]] * dx[dp.csr_col[e]];
+                acc = acc - t;
+                aa = aa + fabs(t);
+            }
+            r[i] = acc;
+            const uint64_t rb = dbits(fabs(acc)), ab = dbits(aa);
+            rm = rb > rm ? rb : rm;
+            am = ab > am ? ab : am;
+        }
+        const double rmax = bitsd(wave_max_u64(rm));
+        const double target = a.refine_tol * fmax(bitsd(fm), bitsd(wave_max_u64(am)));
+        if (rmax <= target) return true;
+        if (!(rmax <= 0.5 * prev)) return false;
+        prev = rmax;
+    }
+    return false;
+}
+
+// Limiting state of a start at x (no junction seeds): the previous terminal
+// voltages are x's own.
+__device__ inline void seed_lim_at(const DeckProg &dp, const double *live, const double *x, double *lim,
+                                   uint32_t tid)
+{
+    const uint32_t nm = dp.n_mos;
+    for (uint32_t j = tid; j < nm; j += kWave) {
+        const Mos m = mos_load(dp, j, live);
+        const uint32_t *t4 = dp.m_t + 4 * j;
+        const double xd = x[t4[0]], xgg = x[t4[1]], xs = x[t4[2]], xb = x[t4[3]];
+        lim[j] = m.tp * (xgg - xs);
+        lim[nm + j] = m.tp * (xd - xs);
+        lim[2 * nm + j] = m.tp * (xb - xs);
+        lim[3 * nm + j] = 0.0;
+    }
 }
 
 __device__ inline void lane_init(const OpArgs &a, uint32_t lane, const double *live, double *x, double *lim,
@@ -339,20 +388,16 @@ __device__ inline void lane_init(const OpArgs &a, uint32_t lane, const double *l
     }
     __syncthreads();
     if (a.mode == 2) return;
-    for (uint32_t j = tid; j < nm; j += kWave) {
-        const Mos m = mos_load(dp, j, live);
-        if (a.mode == 0) {
+    if (a.mode == 0) {
+        for (uint32_t j = tid; j < nm; j += kWave) {
+            const Mos m = mos_load(dp, j, live);
             lim[j] = m.tp * m.vbit + m.gammax * sqrt(m.phit);
             lim[nm + j] = 0.0;
             lim[2 * nm + j] = -1.0;
-        } else {
-            const uint32_t *t4 = dp.m_t + 4 * j;
-            const double xd = x[t4[0]], xgg = x[t4[1]], xs = x[t4[2]], xb = x[t4[3]];
-            lim[j] = m.tp * (xgg - xs);
-            lim[nm + j] = m.tp * (xd - xs);
-            lim[2 * nm + j] = m.tp * (xb - xs);
+            lim[3 * nm + j] = 0.0;
         }
-        lim[3 * nm + j] = 0.0;
+    } else {
+        seed_lim_at(dp, live, x, lim, tid);
     }
     __syncthreads();
 }
@@ -383,15 +428,31 @@ __global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
 
     LaneResume rs = a.resume[lane];
     const TapeProg T = a.tapes[rs.tape];
-    double *y = tv + T.n_tv;
-    uint32_t iter0 = 0;
+    double *y = tv + T.n_tv;   // FP64 solve vector; in FP32 lanes the refined dx (row order)
+    const bool fp32 = a.fp32 && !a.lane64[lane];
+    float *tv32 = reinterpret_cast<float *>(tv);
+    double *a64 = a.a64 + (size_t)lane * dp.nnz;
+    uint32_t iter0 = 0, k0 = 0, warm = a.mode == 3 ? 1u : 0u;
     bool init_fix = a.mode == 0;
-    double limit = 1e9;
+    double limit = 1e9, f0 = 0.0;
     if (a.mode == 2) {
         iter0 = rs.iter;
         init_fix = rs.init_fix != 0;
         limit = rs.limit;
+        k0 = rs.k0;
+        warm = rs.warm;
+        f0 = rs.f0;
     }
+    // a learned start that stalls restarts from the nominal point: its
+    // residual did not fall in the first iteration, or max_iter passed
+    auto restart_nominal = [&](uint32_t at) {
+        for (uint32_t i = tid; i < n; i += kWave) x[i] = a.x_nom[i];
+        __syncthreads();
+        seed_lim_at(dp, live, x, lim, tid);
+        __syncthreads();
+        k0 = at;
+        warm = 2;
+    };
     uint32_t status = kNoConvergence, iters = a.max_iter;
     unsigned long long pc[kProfPhases] = {0, 0, 0, 0};
     unsigned long long tc = clock64();
@@ -402,10 +463,30 @@ __global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
             tc = now;
         }
     };
-    for (uint32_t iter = iter0; iter < a.max_iter; iter++) {
+    uint32_t iter = iter0;
+    for (;; iter++) {
+        if (iter - k0 >= a.max_iter) {
+            if (warm != 1) break;
+            restart_nominal(iter);
+        }
         lap(3);
         assemble(dp, T, live, x, lim, nm, tv, rhs, st, tid);
-        if (iter == 0) {
+        if (warm == 1) {
+            uint64_t fb = 0;
+            for (uint32_t i = tid; i < n; i += kWave) {
+                const uint64_t b = dbits(fabs(rhs[i]));
+                fb = b > fb ? b : fb;
+            }
+            const double fmax = bitsd(wave_max_u64(fb));
+            if (iter == k0) {
+                f0 = fmax;
+            } else if (iter == k0 + 1 && !(fmax <= f0)) {
+                restart_nominal(iter);
+                iter--;   // this assembly is discarded, not an iteration
+                continue;
+            }
+        }
+        if (iter == k0) {
             double gm = 1.0;
             uint64_t xm = 0, fm = 0;
             for (uint32_t i = tid; i < n; i += kWave) {
@@ -423,29 +504,58 @@ __global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
         for (uint32_t i = tid; i < n; i += kWave) sdiag[i] = tv[T.diag_tv[i]];
         __syncthreads();
         lap(0);
-        const bool must_repivot = refactor(T, n, tv, tid);
-        lap(1);
-        if (must_repivot) {
+        // FP32 lanes: keep the FP64 Jacobian for the residual, round the
+        // factor slots to FP32 in place (the FP32 slots and y32 occupy the
+        // low half of the FP64 slot area), replay and refine.
+        uint32_t exit_status = kConverged;
+        bool have_dx = false;
+        if (fp32) {
+            for (uint32_t p = tid; p < dp.nnz; p += kWave) a64[p] = tv[T.amap[p]];
+            __threadfence_block();   // the refinement reads a64 back from other threads
+            for (uint32_t sl = tid; sl < T.n_tv; sl += kWave) {
+                const float v = (float)tv[sl];
+                tv32[sl] = v;
+            }
+            __syncthreads();
+            if (refactor<float>(T, n, tv32, tid)) {
+                exit_status = (rs.fresh && iter == iter0) ? kToFp64 : kRepivot;
+            } else {
+                lap(1);
+                if (refine_solve(a, T, n, tv32, rhs, a64, y, xold, tid)) have_dx = true;
+                else exit_status = kToFp64;
+            }
+        } else if (refactor<double>(T, n, tv, tid)) {
+            exit_status = kRepivot;
+        }
+        if (exit_status != kConverged) {
+            lap(1);
             for (uint32_t i = tid; i < n; i += kWave) a.x[(size_t)lane * n + i] = x[i];
             if (tid == 0) {
                 LaneResume o;
                 o.iter = iter;
                 o.init_fix = init_fix ? 1u : 0u;
                 o.tape = rs.tape;
-                o.pad = 0;
+                o.fresh = 0;
                 o.limit = limit;
+                o.k0 = k0;
+                o.warm = warm;
+                o.f0 = f0;
                 a.resume[lane] = o;
             }
-            status = kRepivot;
+            status = exit_status;
             iters = iter;
             break;
         }
-        solve_neg(T, n, tv, rhs, y, tid);
+        if (!have_dx) {
+            lap(1);
+            solve_neg(T, n, tv, rhs, y, tid);
+        }
         lap(2);
+        __syncthreads();
         uint64_t worst = 0;
         for (uint32_t i = tid; i < n; i += kWave) {
             const double xo = x[i];
-            const double d = y[T.qinv[i]];
+            const double d = have_dx ? y[i] : y[T.qinv[i]];
             const double xn = xo + d;
             xold[i] = xo;
             x[i] = xn;
@@ -465,7 +575,7 @@ __global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
             limited |= mos_limit(m, x[t4[0]], x[t4[1]], x[t4[2]], x[t4[3]], lim + j, nm);
         }
         limited = wave_any(limited);
-        bool ok = !limited && iter != 0 && scaled < 1.0;
+        bool ok = !limited && iter != k0 && scaled < 1.0;
         if (ok) {
             bool miss = false;
             for (uint32_t i = tid; i < n; i += kWave) {
@@ -488,9 +598,11 @@ __global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
         }
         __syncthreads();
     }
+    if (status == kNoConvergence) iters = iter;
     lap(3);
     __syncthreads();
-    if (status != kRepivot)
+    if (a.fell_back && tid == 0) a.fell_back[lane] = warm == 2 ? 1 : 0;
+    if (status != kRepivot && status != kToFp64)
         for (uint32_t i = tid; i < n; i += kWave) a.x[(size_t)lane * n + i] = x[i];
     if (tid == 0) {
         a.status[lane] = status;
@@ -517,7 +629,7 @@ __global__ __launch_bounds__(kWave) void tape_replay(ReplayArgs r)
     for (uint32_t p = tid; p < r.nnz; p += kWave) tv[T.amap[p]] = r.a[(size_t)lane * r.nnz + p];
     for (uint32_t i = tid; i < n; i += kWave) rhs[i] = r.rhs[(size_t)lane * n + i];
     __syncthreads();
-    const bool bad = refactor(T, n, tv, tid);
+    const bool bad = refactor<double>(T, n, tv, tid);
     solve_neg(T, n, tv, rhs, y, tid);
     if (r.lx)
         for (uint32_t q = tid; q < T.n_l; q += kWave) r.lx[(size_t)lane * T.n_l + q] = tv[T.lx0 + q];

@@ -200,8 +200,11 @@ __global__ __launch_bounds__(256) void op_newton_lpt(LptArgs a)
             o.iter = iter;
             o.init_fix = init_fix ? 1u : 0u;
             o.tape = 0;
-            o.pad = 0;
+            o.fresh = 0;
             o.limit = limit;
+            o.k0 = 0;
+            o.warm = 0;
+            o.f0 = 0.0;
             a.resume[lane] = o;
             status = kRepivot;
             iters = iter;

@@ -23,7 +23,7 @@ OK, INVALID_ARGUMENT, BUFFER_TOO_SMALL, OUT_OF_MEMORY = 0, 1, 2, 3
 FAILED, ABI_MISMATCH, INVALID_HANDLE, DEVICE_UNAVAILABLE = 6, 7, 8, 9
 REASON_OK, REASON_NO_CONVERGENCE, REASON_SINGULAR = 0, 1, 3
 FP64, FP32_REFINED = 0, 1
-START_COLD, START_GIVEN = 0, 1
+START_COLD, START_GIVEN, START_LEARNED = 0, 1, 2
 TILING_AUTO, TILING_WAVE, TILING_THREAD = 0, 1, 2
 
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "BUFFER_TOO_SMALL", 3: "OUT_OF_MEMORY",
@@ -34,6 +34,9 @@ EXPORTS = [
     "esl_abi_version", "esl_default_tolerances", "esl_default_op_options", "esl_engine_create",
     "esl_engine_destroy", "esl_engine_info_get", "esl_engine_pattern", "esl_op_solve", "esl_last_timing",
     "esl_tape_create", "esl_tape_destroy", "esl_tape_sizes", "esl_tape_export", "esl_tape_replay",
+    "esl_mlp_default_train_options", "esl_mlp_create", "esl_mlp_destroy", "esl_mlp_param_count",
+    "esl_mlp_get_params", "esl_mlp_set_params", "esl_mlp_set_scaling", "esl_mlp_gradient", "esl_mlp_train",
+    "esl_mlp_predict", "esl_engine_set_warm_start",
 ]
 
 
@@ -74,7 +77,14 @@ class EngineInfo(C.Structure):
 class Timing(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("launches", C.c_uint32),
                 ("repivots", C.c_uint32), ("lane_iterations", C.c_uint64), ("tiling", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("fp64_lanes", C.c_uint32), ("warm_fallbacks", C.c_uint32), ("reserved", C.c_uint32),
+                ("warm_ms", C.c_double)]
+
+
+class MlpTrainOptions(C.Structure):
+    _fields_ = [("struct_size", C.c_size_t), ("epochs", C.c_uint32), ("batch", C.c_uint32),
+                ("lr", C.c_float), ("lr_final", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
+                ("eps", C.c_float), ("weight_decay", C.c_float), ("seed", C.c_uint64)]
 
 
 _lib = None
@@ -113,6 +123,27 @@ def load() -> C.CDLL:
     L.esl_tape_export.argtypes = [vp] * 8
     L.esl_tape_replay.restype = C.c_uint32
     L.esl_tape_replay.argtypes = [vp, vp, vp, C.c_size_t, C.c_double, vp, vp, vp, vp, vp]
+    L.esl_mlp_default_train_options.argtypes = [C.POINTER(MlpTrainOptions)]
+    L.esl_mlp_create.restype = C.c_uint32
+    L.esl_mlp_create.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(vp)]
+    L.esl_mlp_destroy.restype = C.c_uint32
+    L.esl_mlp_destroy.argtypes = [vp]
+    L.esl_mlp_param_count.restype = C.c_uint32
+    L.esl_mlp_param_count.argtypes = [vp, C.POINTER(C.c_size_t)]
+    L.esl_mlp_get_params.restype = C.c_uint32
+    L.esl_mlp_get_params.argtypes = [vp, vp, C.c_size_t]
+    L.esl_mlp_set_params.restype = C.c_uint32
+    L.esl_mlp_set_params.argtypes = [vp, vp, C.c_size_t]
+    L.esl_mlp_set_scaling.restype = C.c_uint32
+    L.esl_mlp_set_scaling.argtypes = [vp, vp, vp, vp, vp]
+    L.esl_mlp_gradient.restype = C.c_uint32
+    L.esl_mlp_gradient.argtypes = [vp, vp, vp, C.c_size_t, C.POINTER(C.c_float), vp]
+    L.esl_mlp_train.restype = C.c_uint32
+    L.esl_mlp_train.argtypes = [vp, vp, vp, C.c_size_t, C.POINTER(MlpTrainOptions), vp]
+    L.esl_mlp_predict.restype = C.c_uint32
+    L.esl_mlp_predict.argtypes = [vp, vp, C.c_size_t, vp]
+    L.esl_engine_set_warm_start.restype = C.c_uint32
+    L.esl_engine_set_warm_start.argtypes = [vp, vp, vp]
     _lib = L
     return L
 
@@ -177,17 +208,24 @@ class Engine:
         _check(load().esl_engine_pattern(self.h, _ptr(cp), _ptr(ri), _ptr(q)), "esl_engine_pattern")
         return cp, ri[: info.nnz], q
 
-    def solve_op(self, values, x0=None, precision=FP64, tol: Tolerances | None = None, tiling=TILING_AUTO):
+    def solve_op(self, values, x0=None, precision=FP64, tol: Tolerances | None = None, tiling=TILING_AUTO,
+                 refine_tol=None, refine_max=None, learned=False):
         """Operating points of rows `values` [count][n_live]. Returns dict
-        with x [count][n], iterations, reasons."""
+        with x [count][n], iterations, reasons. x0 [count][n]: given starts;
+        learned=True: starts from the attached network (set_warm_start)."""
         values = np.ascontiguousarray(np.atleast_2d(np.asarray(values, dtype=np.float64)))
         count = values.shape[0]
-        o = default_options(precision, START_COLD if x0 is None else START_GIVEN)
+        start = START_LEARNED if learned else (START_COLD if x0 is None else START_GIVEN)
+        o = default_options(precision, start)
         o.tiling = tiling
+        if refine_tol is not None:
+            o.refine_tol = refine_tol
+        if refine_max is not None:
+            o.refine_max = refine_max
         if tol is not None:
             o.tol = tol
         xs = None
-        if x0 is not None:
+        if x0 is not None and not learned:
             xs = np.ascontiguousarray(x0, dtype=np.float64).reshape(count, self.n)
             o.x0 = xs.ctypes.data
         x = np.zeros((count, self.n))
@@ -196,6 +234,16 @@ class Engine:
         _check(load().esl_op_solve(self.h, _ptr(values), count, C.byref(o), _ptr(x), _ptr(it), _ptr(rs)),
                "esl_op_solve")
         return {"x": x, "iterations": it, "reasons": rs}
+
+    def set_warm_start(self, net: "Mlp | None", x_nominal=None):
+        """Attaches the warm-start network and the nominal operating point."""
+        if net is None:
+            _check(load().esl_engine_set_warm_start(self.h, None, None), "esl_engine_set_warm_start")
+            self._warm = None
+            return
+        xn = np.ascontiguousarray(x_nominal, dtype=np.float64).reshape(self.n)
+        _check(load().esl_engine_set_warm_start(self.h, net.h, _ptr(xn)), "esl_engine_set_warm_start")
+        self._warm = net   # keep the network alive while attached
 
     def timing(self) -> Timing:
         t = Timing()
@@ -255,3 +303,69 @@ class Tape:
         _check(load().esl_tape_replay(self.h, _ptr(a), _ptr(rhs), lanes, growth, _ptr(lx), _ptr(ux), _ptr(ud),
                                       _ptr(dx), _ptr(fail)), "esl_tape_replay")
         return {"lx": lx[:, : self.nnz_l], "ux": ux[:, : self.nnz_u], "udiag": ud, "dx": dx, "fail": fail}
+
+
+def default_train_options(**kw) -> MlpTrainOptions:
+    o = MlpTrainOptions()
+    load().esl_mlp_default_train_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Mlp:
+    """The warm-start network on the GPU (BF16 MFMA, FP32 master weights)."""
+
+    def __init__(self, n_in, n_out, n_hidden=256, n_hidden_layers=3, seed=1):
+        h = C.c_void_p()
+        _check(load().esl_mlp_create(n_in, n_hidden, n_hidden_layers, n_out, seed, C.byref(h)), "esl_mlp_create")
+        self.h = h
+        self.n_in, self.n_out = n_in, n_out
+        self.dims = [n_in] + [n_hidden] * n_hidden_layers + [n_out]
+        c = C.c_size_t()
+        _check(load().esl_mlp_param_count(self.h, C.byref(c)), "esl_mlp_param_count")
+        self.n_params = c.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().esl_mlp_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, np.float32)
+        _check(load().esl_mlp_get_params(self.h, _ptr(p), p.size), "esl_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(load().esl_mlp_set_params(self.h, _ptr(p), p.size), "esl_mlp_set_params")
+
+    def set_scaling(self, in_mean, in_scale, out_mean, out_scale):
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (in_mean, in_scale, out_mean, out_scale)]
+        _check(load().esl_mlp_set_scaling(self.h, *[_ptr(v) for v in a]), "esl_mlp_set_scaling")
+
+    def gradient(self, inputs, targets):
+        inputs = np.ascontiguousarray(inputs, dtype=np.float64)
+        targets = np.ascontiguousarray(targets, dtype=np.float64)
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_float()
+        _check(load().esl_mlp_gradient(self.h, _ptr(inputs), _ptr(targets), inputs.shape[0], C.byref(loss),
+                                       _ptr(g)), "esl_mlp_gradient")
+        return float(loss.value), g
+
+    def train(self, inputs, targets, **kw):
+        inputs = np.ascontiguousarray(inputs, dtype=np.float64)
+        targets = np.ascontiguousarray(targets, dtype=np.float64)
+        o = default_train_options(**kw)
+        losses = np.zeros(o.epochs, np.float32)
+        _check(load().esl_mlp_train(self.h, _ptr(inputs), _ptr(targets), inputs.shape[0], C.byref(o),
+                                    _ptr(losses)), "esl_mlp_train")
+        return losses
+
+    def predict(self, inputs):
+        inputs = np.ascontiguousarray(np.atleast_2d(inputs), dtype=np.float64)
+        out = np.zeros((inputs.shape[0], self.n_out + 1))
+        _check(load().esl_mlp_predict(self.h, _ptr(inputs), inputs.shape[0], _ptr(out)), "esl_mlp_predict")
+        return out

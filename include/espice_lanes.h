@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define ESL_ABI_VERSION 1u
+#define ESL_ABI_VERSION 2u
 
 /* esl_status: the values of espice_status (include/espice.h:21-38). */
 typedef uint32_t esl_status;
@@ -130,7 +130,13 @@ enum {
 /* esl_op_options.start */
 enum {
     ESL_START_COLD = 0,    /* zero x + junction seeds, INITFIX (dc/op.zig:24-27) */
-    ESL_START_GIVEN = 1    /* x0 supplied per point, no INITFIX */
+    ESL_START_GIVEN = 1,   /* x0 supplied per point, no INITFIX */
+    /* x0 from the network attached by esl_engine_set_warm_start, evaluated
+     * on the device from each point's live values; a lane whose residual
+     * does not fall in its first iteration, or that has not converged after
+     * max_iter iterations, restarts from the attached nominal operating
+     * point (max_iter more iterations). No INITFIX. */
+    ESL_START_LEARNED = 2
 };
 
 typedef struct {
@@ -140,8 +146,11 @@ typedef struct {
     uint32_t start;        /* ESL_START_* */
     /* ESL_START_GIVEN: [count][n] start vectors (n = esl_engine_info.n). */
     const double *x0;
-    /* FP32_REFINED: refinement stops once max|r| <= refine_tol * max|F|
-     * (relative residual), or after refine_max steps. */
+    /* FP32_REFINED: each Newton solve dx = -J^-1 F is refined in FP64
+     * until max|r| <= refine_tol * max(max|F|, max_i sum_j |J_ij dx_j|)
+     * with r = -F - J dx (defaults 1e-12, 8 steps). A step that does not
+     * halve max|r|, or refine_max steps without success, sends the lane to
+     * FP64 factors for that iteration and the rest of its Newton. */
     double refine_tol;
     uint32_t refine_max;
     uint32_t tiling;       /* ESL_TILING_* */
@@ -202,11 +211,14 @@ esl_status esl_op_solve(esl_engine *e, const double *values, size_t count,
 typedef struct {
     double kernel_ms;      /* Newton kernel(s), HIP events */
     double total_ms;       /* upload + kernels + download */
-    uint32_t launches;     /* Newton kernel launches (1 + re-pivot rounds) */
+    uint32_t launches;     /* Newton kernel launches (1 + re-pivot / fallback rounds) */
     uint32_t repivots;     /* lanes re-pivoted on the host */
     uint64_t lane_iterations; /* sum of Newton iterations over lanes */
     uint32_t tiling;       /* ESL_TILING_* the last solve ran (first launch) */
+    uint32_t fp64_lanes;   /* FP32_REFINED: lanes that fell back to FP64 factors */
+    uint32_t warm_fallbacks;  /* START_LEARNED: lanes restarted from the nominal point */
     uint32_t reserved;
+    double warm_ms;        /* START_LEARNED: network inference, HIP events */
 } esl_timing;
 esl_status esl_last_timing(esl_engine *e, esl_timing *out);
 
@@ -250,6 +262,54 @@ esl_status esl_tape_replay(esl_tape *t, const double *a, const double *rhs,
                            size_t lanes, double growth, double *lx,
                            double *ux, double *udiag, double *dx,
                            uint32_t *fail);
+
+/* ---- Learned warm start (no counterpart in the reference) ----
+ * A multilayer perceptron n_in -> n_hidden (x n_hidden_layers, SiLU) -> n_out
+ * that maps a point's live parameter values to its operating point
+ * (unknowns 1..n-1; ground is 0). Trained and evaluated on the device: BF16
+ * MFMA products with FP32 accumulation, FP32 master weights, Adam, mean
+ * squared error on standardized outputs. Parameters are ordered layer by
+ * layer, each as W [out][in] row-major then b [out]. */
+typedef struct esl_mlp esl_mlp;
+
+typedef struct {
+    size_t struct_size;    /* sizeof(esl_mlp_train_options) */
+    uint32_t epochs;       /* passes over the training rows */
+    uint32_t batch;        /* rows per Adam step */
+    float lr, lr_final;    /* learning rate, cosine-decayed to lr_final */
+    float beta1, beta2, eps, weight_decay;
+    uint64_t seed;         /* row shuffling */
+} esl_mlp_train_options;
+
+void esl_mlp_default_train_options(esl_mlp_train_options *opt);
+/* Weights drawn uniform in +-sqrt(6 / (fan_in + fan_out)) from seed,
+ * biases zero. Scaling defaults to identity. */
+esl_status esl_mlp_create(uint32_t n_in, uint32_t n_hidden, uint32_t n_hidden_layers,
+                          uint32_t n_out, uint64_t seed, esl_mlp **out);
+esl_status esl_mlp_destroy(esl_mlp *m);
+esl_status esl_mlp_param_count(esl_mlp *m, size_t *count);
+esl_status esl_mlp_get_params(esl_mlp *m, float *params, size_t count);
+esl_status esl_mlp_set_params(esl_mlp *m, const float *params, size_t count);
+/* Inputs are standardized as (v - in_mean) / in_scale, outputs produced as
+ * y * out_scale + out_mean. Scales must be positive and finite. */
+esl_status esl_mlp_set_scaling(esl_mlp *m, const double *in_mean, const double *in_scale,
+                               const double *out_mean, const double *out_scale);
+/* One forward and backward pass over rows (inputs [rows][n_in], targets
+ * [rows][n_out + 1] operating points with ground first, as esl_op_solve
+ * writes them); loss = mean squared error of the standardized outputs;
+ * grads [param_count]. No update. */
+esl_status esl_mlp_gradient(esl_mlp *m, const double *inputs, const double *targets,
+                            size_t rows, float *loss, float *grads);
+/* Adam over the rows (same layouts); loss_per_epoch [epochs] may be NULL. */
+esl_status esl_mlp_train(esl_mlp *m, const double *inputs, const double *targets,
+                         size_t rows, const esl_mlp_train_options *opt,
+                         float *loss_per_epoch);
+/* out [rows][n_out + 1]: ground 0, then the network's unknowns. */
+esl_status esl_mlp_predict(esl_mlp *m, const double *inputs, size_t rows, double *out);
+/* Attaches a network (n_in = n_live, n_out = n - 1) and the nominal
+ * operating point x_nominal [n] for ESL_START_LEARNED. The engine keeps
+ * references: the network must outlive its use; NULL detaches. */
+esl_status esl_engine_set_warm_start(esl_engine *e, esl_mlp *m, const double *x_nominal);
 
 #ifdef __cplusplus
 }

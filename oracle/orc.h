@@ -61,6 +61,32 @@ int orc_lu_lane_refactor(orc_lu *lu, const uint32_t *col_ptr, const double *vals
                          double growth_limit, double wide);
 /* x = A^-1 b (solve). */
 void orc_lu_solve(orc_lu *lu, const double *b, double *x);
+
+/* FP32 factors on a tape of orc_lu (the precision mode of the GPU lanes:
+ * LU in FP32, residual and refinement in FP64). Parity pinned by tests: a
+ * matrix whose entries are exact in FP32 and whose factorization rounds
+ * nowhere (integer factors) gives the FP64 factors exactly; FP32 factors of
+ * random systems solve to FP32 accuracy; refinement reaches the FP64
+ * solution. */
+typedef struct {
+    uint32_t n, cap_l, cap_u;
+    float *lx, *ux, *udiag, *w, *y;
+} orc_lu32;
+orc_lu32 *orc_lu32_new(uint32_t n);
+void orc_lu32_free(orc_lu32 *f);
+/* The lane replay of orc_lu_lane_refactor in FP32 on (float)vals; the
+ * growth and wide tests are computed in FP64 on the FP32 values. Returns 1
+ * when the lane must re-pivot. */
+int orc_lu_lane_refactor32(const orc_lu *lu, orc_lu32 *f, const uint32_t *col_ptr,
+                           const double *vals, double growth_limit, double wide);
+/* x = A32^-1 b in FP32 (no negation). */
+void orc_lu_solve32(const orc_lu *lu, orc_lu32 *f, const float *b, float *x);
+/* FP32 replay of the tape built from lead_vals over a [lanes][nnz] and
+ * x = A32^-1 (float)b. */
+int orc_lu_replay32_export(uint32_t n, const uint32_t *col_ptr, const uint32_t *row_idx,
+                           const double *lead_vals, const uint32_t *q, double pivot_tol,
+                           const double *a, const double *b, size_t lanes, double growth,
+                           float *lx, float *ux, float *udiag, float *x, uint32_t *fail);
 /* dx = -A^-1 rhs (lane_kernels.solveNeg). */
 void orc_lu_solve_neg(orc_lu *lu, const double *rhs, double *dx);
 uint64_t orc_lu_flops(const orc_lu *lu);
@@ -123,4 +149,25 @@ int orc_op_solve(orc_ckt *c, const double *values, size_t count,
                  const orc_tol *tol, const uint32_t *q, const double *start_x,
                  int threads, double *x, uint32_t *iterations, uint32_t *reasons,
                  uint32_t *repivots);
+
+/* Options of orc_op_solve2. precision 0 FP64, 1 FP32 factors with FP64
+ * iterative refinement of each Newton solve (refine until max|r| <=
+ * refine_tol * max(max|F|, max_i sum_j |A_ij dx_j|); a stall (max|r| not
+ * halving) or refine_max steps without success sends the lane to FP64 for
+ * the rest of its Newton, redoing the current iteration). nominal_x [n]
+ * (with start_x): start_x is a learned guess, and a lane whose residual max
+ * does not fall in its first iteration, or that has not converged after
+ * max_iter iterations, restarts from nominal_x (the first check's assembly
+ * is discarded, not counted). fp64_lanes / fell_back [count] receive 1 for
+ * lanes that fell back to FP64 / restarted from nominal_x (may be NULL). */
+typedef struct {
+    uint32_t precision, refine_max;
+    double refine_tol;
+    const double *nominal_x;
+} orc_prec;
+int orc_op_solve2(orc_ckt *c, const double *values, size_t count,
+                  const orc_tol *tol, const orc_prec *prec, const uint32_t *q,
+                  const double *start_x, int threads, double *x,
+                  uint32_t *iterations, uint32_t *reasons, uint32_t *repivots,
+                  uint32_t *fp64_lanes, uint32_t *fell_back);
 #endif

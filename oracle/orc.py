@@ -18,6 +18,7 @@ _LIB = None
 u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 
 
 class Deck(C.Structure):
@@ -34,6 +35,11 @@ class Tol(C.Structure):
         ("reltol", C.c_double), ("abstol", C.c_double), ("vntol", C.c_double),
         ("residual_tol", C.c_double), ("max_iter", C.c_uint32), ("reserved", C.c_uint32),
     ]
+
+
+class Prec(C.Structure):
+    _fields_ = [("precision", C.c_uint32), ("refine_max", C.c_uint32), ("refine_tol", C.c_double),
+                ("nominal_x", C.c_void_p)]
 
 
 def build(force: bool = False) -> str:
@@ -74,6 +80,12 @@ def lib():
         L.orc_ckt_pattern.argtypes = [C.c_void_p, u32p, u32p, u32p, u8p]
         L.orc_ckt_assemble_cold.argtypes = [C.c_void_p, f64p, f64p, f64p]
         L.orc_ckt_assemble_at.argtypes = [C.c_void_p, f64p, f64p, f64p, f64p]
+        L.orc_op_solve2.restype = C.c_int
+        L.orc_op_solve2.argtypes = [C.c_void_p, f64p, C.c_size_t, C.POINTER(Tol), C.POINTER(Prec), u32p,
+                                    C.c_void_p, C.c_int, f64p, u32p, u32p, u32p, u32p, u32p]
+        L.orc_lu_replay32_export.restype = C.c_int
+        L.orc_lu_replay32_export.argtypes = [C.c_uint32, u32p, u32p, f64p, u32p, C.c_double, f64p, f64p,
+                                             C.c_size_t, C.c_double, f32p, f32p, f32p, f32p, u32p]
         L.orc_op_solve.restype = C.c_int
         L.orc_op_solve.argtypes = [C.c_void_p, f64p, C.c_size_t, C.POINTER(Tol), u32p, C.c_void_p,
                                    C.c_int, f64p, u32p, u32p, u32p]
@@ -146,6 +158,28 @@ def lu_replay(col_ptr, row_idx, lead_vals, q, a, rhs, growth=1e-12, pivot_tol=1e
     return dict(lx=lx[:, :t["nnz_l"]], ux=ux[:, :t["nnz_u"]], udiag=ud, dx=dx, fail=fail, tape=t)
 
 
+def lu_replay32(col_ptr, row_idx, lead_vals, q, a, b, growth=1e-12, pivot_tol=1e-3):
+    """FP32 lane replay of the (FP64) tape built from lead_vals over a
+    [lanes][nnz]; x = A32^-1 b in FP32."""
+    col_ptr, row_idx, q = _u32(col_ptr), _u32(row_idx), _u32(q)
+    lead_vals, a, b = _f64(lead_vals), _f64(a), _f64(b)
+    t = lu_factor(col_ptr, row_idx, lead_vals, q, pivot_tol)
+    if t is None:
+        return None
+    n = len(col_ptr) - 1
+    lanes = a.shape[0]
+    lx = np.zeros((lanes, max(t["nnz_l"], 1)), np.float32)
+    ux = np.zeros((lanes, max(t["nnz_u"], 1)), np.float32)
+    ud = np.zeros((lanes, n), np.float32)
+    x = np.zeros((lanes, n), np.float32)
+    fail = np.zeros(lanes, np.uint32)
+    rc = lib().orc_lu_replay32_export(n, col_ptr, row_idx, lead_vals, q, pivot_tol, a, b, lanes, growth,
+                                      lx, ux, ud, x, fail)
+    if rc != 0:
+        return None
+    return dict(lx=lx[:, :t["nnz_l"]], ux=ux[:, :t["nnz_u"]], udiag=ud, x=x, fail=fail, tape=t)
+
+
 def lu_solve(col_ptr, row_idx, vals, q, b, pivot_tol=1e-3):
     col_ptr, row_idx, q = _u32(col_ptr), _u32(row_idx), _u32(q)
     n = len(col_ptr) - 1
@@ -204,18 +238,25 @@ class Circuit:
         lib().orc_ckt_assemble_at(self.h, _f64(live), _f64(x), g, r)
         return g, r
 
-    def op(self, values, q, start_x=None, threads=1, tol=None):
+    def op(self, values, q, start_x=None, threads=1, tol=None, precision=0, refine_tol=1e-12, refine_max=8,
+           nominal_x=None):
+        """Operating points; precision 1 = FP32 factors + FP64 refinement;
+        nominal_x with start_x = a learned start with the nominal fallback."""
         values = _f64(np.atleast_2d(values))
         count = values.shape[0]
         t = Tol(1e-3, 1e-12, 1e-6, 1e-9, 100, 0) if tol is None else tol
+        nom = None if nominal_x is None else _f64(nominal_x)
+        p = Prec(int(precision), int(refine_max), float(refine_tol), None if nom is None else nom.ctypes.data)
         x = np.zeros((count, self.n))
         it = np.zeros(count, np.uint32)
         rs = np.zeros(count, np.uint32)
         rp = np.zeros(count, np.uint32)
+        fb = np.zeros(count, np.uint32)
+        wb = np.zeros(count, np.uint32)
         sx = None if start_x is None else _f64(start_x)
-        lib().orc_op_solve(self.h, values, count, C.byref(t), _u32(q),
-                           None if sx is None else sx.ctypes.data, int(threads), x, it, rs, rp)
-        return dict(x=x, iterations=it, reasons=rs, repivots=rp)
+        lib().orc_op_solve2(self.h, values, count, C.byref(t), C.byref(p), _u32(q),
+                            None if sx is None else sx.ctypes.data, int(threads), x, it, rs, rp, fb, wb)
+        return dict(x=x, iterations=it, reasons=rs, repivots=rp, fp64_lanes=fb, fell_back=wb)
 
 
 def natural_order(n):

@@ -73,6 +73,7 @@ struct orc_ckt {
     double *params;
     uint32_t n, nnz, n_nodes, n_inst;
     uint32_t *col_ptr, *row_idx, *diag;
+    uint32_t *csr_ptr, *csr_col, *csr_slot;   /* the pattern by rows, columns ascending */
     uint8_t *current_row;
     orc_inst *inst;
 };
@@ -433,6 +434,22 @@ orc_ckt *orc_ckt_new(const orc_deck *deck, char *err, size_t err_cap)
     c->diag = malloc(n * 4);
     for (uint32_t i = 0; i < n; i++) c->diag[i] = find_slot(c, i, i);
     c->diag[0] = c->col_ptr[0];   /* (0,0) is column 0's only entry */
+    c->csr_ptr = calloc(n + 1, 4);
+    c->csr_col = malloc((c->nnz + 1) * 4);
+    c->csr_slot = malloc((c->nnz + 1) * 4);
+    for (uint32_t p = 0; p < c->nnz; p++) c->csr_ptr[c->row_idx[p] + 1]++;
+    for (uint32_t i = 0; i < n; i++) c->csr_ptr[i + 1] += c->csr_ptr[i];
+    {
+        uint32_t *at = malloc((n + 1) * 4);
+        memcpy(at, c->csr_ptr, n * 4);
+        for (uint32_t j = 0; j < n; j++)
+            for (uint32_t p = c->col_ptr[j]; p < c->col_ptr[j + 1]; p++) {
+                const uint32_t e = at[c->row_idx[p]]++;
+                c->csr_col[e] = j;
+                c->csr_slot[e] = p;
+            }
+        free(at);
+    }
     for (uint32_t i = 0; i < ni; i++) {
         orc_inst *in = &c->inst[i];
         if (in->kind == K_R) {
@@ -457,7 +474,7 @@ void orc_ckt_free(orc_ckt *c)
     if (!c) return;
     free(c->kind); free(c->nodes); free(c->params); free(c->bind_dev); free(c->bind_param);
     free(c->bind_live); free(c->inst); free(c->current_row); free(c->col_ptr); free(c->row_idx);
-    free(c->diag); free(c);
+    free(c->diag); free(c->csr_ptr); free(c->csr_col); free(c->csr_slot); free(c);
 }
 
 uint32_t orc_ckt_n(const orc_ckt *c) { return c->n; }
@@ -645,7 +662,50 @@ typedef struct {
     const double *start_x;
     double *x;
     uint32_t *iterations, *reasons, *repivots;
+    orc_prec prec;
+    uint32_t *fp64_lanes, *fell_back;
 } job_t;
+
+/* One Newton linear solve with FP32 factors and FP64 refinement:
+ * dx = -A^-1 F. r = -F - A dx in FP64, row by row over the pattern in
+ * ascending column order. Returns 0 when the refinement converged, 1 when it
+ * stalled or ran out of steps. */
+static int refine_solve(const orc_ckt *c, const orc_lu *lu, orc_lu32 *f, const double *g, const double *rhs,
+                        const orc_prec *pr, double *dx, double *r, float *b32, float *x32)
+{
+    const uint32_t n = c->n;
+    uint64_t fm = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        r[i] = -rhs[i];
+        dx[i] = 0;
+        const uint64_t b = db(fabs(rhs[i]));
+        if (b > fm) fm = b;
+    }
+    double prev = INFINITY;
+    for (uint32_t s = 0; s < pr->refine_max; s++) {
+        for (uint32_t i = 0; i < n; i++) b32[i] = (float)r[i];
+        orc_lu_solve32(lu, f, b32, x32);
+        for (uint32_t i = 0; i < n; i++) dx[i] = dx[i] + (double)x32[i];
+        uint64_t rm = 0, am = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            double acc = -rhs[i], aa = 0;
+            for (uint32_t e = c->csr_ptr[i]; e < c->csr_ptr[i + 1]; e++) {
+                const double t = g[c->csr_slot[e]] * dx[c->csr_col[e]];
+                acc = acc - t;
+                aa = aa + fabs(t);
+            }
+            r[i] = acc;
+            const uint64_t rb = db(fabs(acc)), ab = db(aa);
+            if (rb > rm) rm = rb;
+            if (ab > am) am = ab;
+        }
+        const double target = pr->refine_tol * fmax(bd(fm), bd(am));
+        if (bd(rm) <= target) return 0;
+        if (!(bd(rm) <= 0.5 * prev)) return 1;
+        prev = bd(rm);
+    }
+    return 1;
+}
 
 /* one lane's Newton (converger.newtonLanes at W = 1 over a lane solver) */
 static void solve_lane(const job_t *j, size_t l)
@@ -657,6 +717,11 @@ static void solve_lane(const job_t *j, size_t l)
     lane_dev_init(c, j->values + l * c->d.n_live, &ld);
     double *x = calloc(n + 1, 8), *x_old = calloc(n + 1, 8), *dx = calloc(n + 1, 8);
     double *g = calloc(nnz + 1, 8), *rhs = calloc(n + 1, 8), *sdiag = calloc(n + 1, 8);
+    double *rr = calloc(n + 1, 8);
+    float *b32 = calloc(n + 1, 4), *x32 = calloc(n + 1, 4);
+    orc_lu32 *f32 = orc_lu32_new(n);
+    int lane64 = j->prec.precision == 0;
+    int fresh_iter = -1;
     int init_fix;
     if (j->start_x) {
         memcpy(x, j->start_x + l * n, (size_t)n * 8);
@@ -688,19 +753,78 @@ static void solve_lane(const job_t *j, size_t l)
     uint32_t reason = 1, iters = tol->max_iter;
     double limit = 1e9;
     const double wide = 1.0 / 1e-3;
-    for (uint32_t iter = 0; iter < tol->max_iter; iter++) {
+    /* learned start: warm 1 while active, 2 after the restart from nominal */
+    uint32_t warm = (j->start_x && j->prec.nominal_x) ? 1 : 0, k0 = 0;
+    double f0 = 0;
+    uint32_t iter = 0;
+    for (;; iter++) {
+        if (iter - k0 >= tol->max_iter) {
+            if (warm != 1) break;
+            memcpy(x, j->prec.nominal_x, (size_t)n * 8);
+            seed_at(c, &ld, x);
+            k0 = iter;
+            warm = 2;
+        }
         assemble(c, &ld, x, g, rhs);
-        if (iter == 0) limit = plausible_limit(c, x, rhs, g);
-        for (uint32_t i = 0; i < n; i++) sdiag[i] = g[c->diag[i]];
-        if (orc_lu_lane_refactor(lu, c->col_ptr, g, 1e-12, wide)) {
-            repiv++;
-            if (orc_lu_factor(lu, c->col_ptr, c->row_idx, g, j->q, 1e-3) != 0) {
-                reason = 3;
-                iters = iter + 1;
-                break;
+        if (warm == 1) {
+            uint64_t fb = 0;
+            for (uint32_t i = 0; i < n; i++) {
+                const uint64_t b = db(fabs(rhs[i]));
+                if (b > fb) fb = b;
+            }
+            if (iter == k0) {
+                f0 = bd(fb);
+            } else if (iter == k0 + 1 && !(bd(fb) <= f0)) {
+                memcpy(x, j->prec.nominal_x, (size_t)n * 8);
+                seed_at(c, &ld, x);
+                k0 = iter;
+                warm = 2;
+                iter--;
+                continue;
             }
         }
-        orc_lu_solve_neg(lu, rhs, dx);
+        if (iter == k0) limit = plausible_limit(c, x, rhs, g);
+        for (uint32_t i = 0; i < n; i++) sdiag[i] = g[c->diag[i]];
+        int singular = 0, done = 0;
+        if (!lane64) {
+            /* FP32 factors: a failed replay re-pivots (from the FP64
+             * matrix); a replay that fails on the tape just built from this
+             * very matrix, or a refinement that stalls, sends the lane to
+             * FP64 for this iteration and the rest. */
+            int fail = orc_lu_lane_refactor32(lu, f32, c->col_ptr, g, 1e-12, wide);
+            if (fail && fresh_iter != (int)iter) {
+                repiv++;
+                if (orc_lu_factor(lu, c->col_ptr, c->row_idx, g, j->q, 1e-3) != 0) {
+                    singular = 1;
+                } else {
+                    fresh_iter = (int)iter;
+                    fail = orc_lu_lane_refactor32(lu, f32, c->col_ptr, g, 1e-12, wide);
+                    if (fail) {
+                        lane64 = 1;   /* FP64 replay of the fresh tape == the factor just made */
+                        orc_lu_solve_neg(lu, rhs, dx);
+                        done = 1;
+                    }
+                }
+            } else if (fail) {
+                lane64 = 1;
+            }
+            if (!singular && !done && !lane64) {
+                if (refine_solve(c, lu, f32, g, rhs, &j->prec, dx, rr, b32, x32) == 0) done = 1;
+                else lane64 = 1;
+            }
+        }
+        if (!singular && !done) {
+            if (orc_lu_lane_refactor(lu, c->col_ptr, g, 1e-12, wide)) {
+                repiv++;
+                if (orc_lu_factor(lu, c->col_ptr, c->row_idx, g, j->q, 1e-3) != 0) singular = 1;
+            }
+            if (!singular) orc_lu_solve_neg(lu, rhs, dx);
+        }
+        if (singular) {
+            reason = 3;
+            iters = iter + 1;
+            break;
+        }
         /* updateAndNorm */
         uint64_t worst = 0;
         for (uint32_t i = 0; i < n; i++) {
@@ -717,7 +841,7 @@ static void solve_lane(const job_t *j, size_t l)
         const double scaled = bd(worst);
         const int limited = apply_limits(c, &ld, x);
         int ok = !limited;
-        if (ok && iter == 0) ok = 0;
+        if (ok && iter == k0) ok = 0;
         if (ok && !(scaled < 1.0)) ok = 0;
         if (ok)
             for (uint32_t i = 0; i < n; i++)
@@ -738,12 +862,16 @@ static void solve_lane(const job_t *j, size_t l)
             break;
         }
     }
+    if (reason == 1) iters = iter;
     memcpy(j->x + l * n, x, (size_t)n * 8);
     j->iterations[l] = iters;
+    if (j->fell_back) j->fell_back[l] = warm == 2;
     j->reasons[l] = reason;
     if (j->repivots) j->repivots[l] = repiv;
+    if (j->fp64_lanes) j->fp64_lanes[l] = (uint32_t)(lane64 && j->prec.precision != 0);
     orc_lu_free(lu);
-    free(x); free(x_old); free(dx); free(g); free(rhs); free(sdiag);
+    orc_lu32_free(f32);
+    free(x); free(x_old); free(dx); free(g); free(rhs); free(sdiag); free(rr); free(b32); free(x32);
     lane_dev_free(&ld);
 }
 
@@ -757,6 +885,15 @@ static void *run_job(void *arg)
 int orc_op_solve(orc_ckt *c, const double *values, size_t count, const orc_tol *tol,
                  const uint32_t *q, const double *start_x, int threads, double *x,
                  uint32_t *iterations, uint32_t *reasons, uint32_t *repivots)
+{
+    const orc_prec p64 = {0, 0, 0.0, NULL};
+    return orc_op_solve2(c, values, count, tol, &p64, q, start_x, threads, x, iterations, reasons, repivots, NULL,
+                         NULL);
+}
+
+int orc_op_solve2(orc_ckt *c, const double *values, size_t count, const orc_tol *tol, const orc_prec *prec,
+                  const uint32_t *q, const double *start_x, int threads, double *x, uint32_t *iterations,
+                  uint32_t *reasons, uint32_t *repivots, uint32_t *fp64_lanes, uint32_t *fell_back)
 {
     if (count == 0) return 0;
     const uint32_t n = c->n, nnz = c->nnz;
@@ -779,7 +916,8 @@ int orc_op_solve(orc_ckt *c, const double *values, size_t count, const orc_tol *
     size_t per = count / threads, rem = count % threads, at = 0;
     for (int t = 0; t < threads; t++) {
         size_t k = per + ((size_t)t < rem ? 1 : 0);
-        jobs[t] = (job_t){c, values, at, k, tol, lead, q, start_x, x, iterations, reasons, repivots};
+        jobs[t] = (job_t){c, values, at, k, tol, lead, q, start_x, x, iterations, reasons, repivots, *prec, fp64_lanes,
+                          fell_back};
         at += k;
     }
     if (threads == 1) {

@@ -516,3 +516,160 @@ int orc_lu_solve_export(uint32_t n, const uint32_t *col_ptr, const uint32_t *row
     orc_lu_free(lu);
     return rc;
 }
+
+/* ---- FP32 factors on an FP64 tape ---- */
+static uint64_t fbits64(float f) { return dbits((double)f); }
+
+orc_lu32 *orc_lu32_new(uint32_t n)
+{
+    orc_lu32 *f = calloc(1, sizeof(orc_lu32));
+    f->n = n;
+    f->udiag = calloc(n + 1, 4);
+    f->w = calloc(n + 1, 4);
+    f->y = calloc(n + 1, 4);
+    return f;
+}
+
+void orc_lu32_free(orc_lu32 *f)
+{
+    if (!f) return;
+    free(f->lx); free(f->ux); free(f->udiag); free(f->w); free(f->y);
+    free(f);
+}
+
+static void lu32_fit(const orc_lu *lu, orc_lu32 *f)
+{
+    if (f->cap_l < lu->nnz_l + 1) {
+        f->cap_l = lu->nnz_l + 1;
+        f->lx = realloc(f->lx, (size_t)f->cap_l * 4);
+    }
+    if (f->cap_u < lu->nnz_u + 1) {
+        f->cap_u = lu->nnz_u + 1;
+        f->ux = realloc(f->ux, (size_t)f->cap_u * 4);
+    }
+}
+
+/* orc_lu_lane_refactor in FP32: the same column loop with a float
+ * accumulator; growth (cmax as the bit-pattern max) and wide in FP64. */
+int orc_lu_lane_refactor32(const orc_lu *lu, orc_lu32 *f, const uint32_t *col_ptr,
+                           const double *vals, double growth_limit, double wide)
+{
+    const uint32_t n = lu->n;
+    lu32_fit(lu, f);
+    float *w = f->w;
+    int bad = 0;
+    uint64_t nonfinite = 0;
+    for (uint32_t v = 0; v < lu->n_void_slots; v++)
+        if ((float)vals[lu->void_slots[v]] != 0) return 1;
+    for (uint32_t k = 0; k < n; k++) {
+        const uint32_t c = lu->q[k];
+        for (uint32_t p = col_ptr[c]; p < col_ptr[c + 1]; p++) w[lu->prow[p]] = (float)vals[p];
+        for (uint32_t p = lu->up[k]; p < lu->up[k + 1]; p++) {
+            uint32_t i = lu->ui[p];
+            float uki = w[i];
+            w[i] = 0;
+            f->ux[p] = uki;
+            nonfinite |= fbits64(uki - uki);
+            for (uint32_t pl = lu->lp[i]; pl < lu->lp[i + 1]; pl++)
+                w[lu->li[pl]] = w[lu->li[pl]] - f->lx[pl] * uki;
+        }
+        if (lu->void_col[k]) {
+            f->udiag[k] = 1;
+            for (uint32_t p = lu->lp[k]; p < lu->lp[k + 1]; p++) { f->lx[p] = 0; w[lu->li[p]] = 0; }
+            w[k] = 0;
+            for (uint32_t p = lu->up[k]; p < lu->up[k + 1]; p++) {
+                uint32_t i = lu->ui[p];
+                for (uint32_t pl = lu->lp[i]; pl < lu->lp[i + 1]; pl++) w[lu->li[pl]] = 0;
+            }
+            continue;
+        }
+        float d = w[k];
+        w[k] = 0;
+        int dead = d == 0 || !(fabs((double)d) < INFINITY);
+        bad = bad || dead;
+        if (dead) d = 1;
+        f->udiag[k] = d;
+        if (growth_limit > 0 && !lu->scaled[k]) {
+            int64_t cbits = (int64_t)fbits64(fabsf(d));
+            for (uint32_t p = lu->lp[k]; p < lu->lp[k + 1]; p++) {
+                uint32_t r = lu->li[p];
+                float v = w[r];
+                w[r] = 0;
+                int64_t vb = (int64_t)fbits64(fabsf(v));
+                if (vb > cbits) cbits = vb;
+                f->lx[p] = v / d;
+            }
+            double cmax = bitsd((uint64_t)cbits);
+            bad = bad || !(fabs((double)d) >= growth_limit * cmax);
+        } else {
+            for (uint32_t p = lu->lp[k]; p < lu->lp[k + 1]; p++) {
+                uint32_t r = lu->li[p];
+                float lv = w[r] / d;
+                nonfinite |= fbits64(lv - lv);
+                f->lx[p] = lv;
+                w[r] = 0;
+            }
+        }
+    }
+    if (nonfinite != 0) return 1;
+    uint64_t worst = 0;
+    for (uint32_t k = 0; k < n; k++) {
+        if (lu->scaled[k]) continue;
+        for (uint32_t p = lu->lp[k]; p < lu->lp[k + 1]; p++) {
+            uint64_t b = fbits64(fabsf(f->lx[p]));
+            if (b > worst) worst = b;
+        }
+    }
+    return bad || !(bitsd(worst) <= wide);
+}
+
+void orc_lu_solve32(const orc_lu *lu, orc_lu32 *f, const float *b, float *x)
+{
+    const uint32_t n = lu->n;
+    float *y = f->y;
+    for (uint32_t r = 0; r < n; r++) y[lu->pinv[r]] = b[r];
+    for (uint32_t k = 0; k < n; k++) {
+        float yk = y[k];
+        if (yk != 0)
+            for (uint32_t p = lu->lp[k]; p < lu->lp[k + 1]; p++) y[lu->li[p]] = y[lu->li[p]] - f->lx[p] * yk;
+    }
+    for (uint32_t k = n; k > 0;) {
+        k--;
+        float zk = y[k] / f->udiag[k];
+        y[k] = zk;
+        if (zk != 0)
+            for (uint32_t p = lu->up[k]; p < lu->up[k + 1]; p++) y[lu->ui[p]] = y[lu->ui[p]] - f->ux[p] * zk;
+    }
+    for (uint32_t j = 0; j < n; j++) x[lu->q[j]] = y[j];
+}
+
+int orc_lu_replay32_export(uint32_t n, const uint32_t *col_ptr, const uint32_t *row_idx,
+                           const double *lead_vals, const uint32_t *q, double pivot_tol,
+                           const double *a, const double *b, size_t lanes, double growth,
+                           float *lx, float *ux, float *udiag, float *x, uint32_t *fail)
+{
+    const uint32_t nnz = col_ptr[n];
+    orc_lu *lu = orc_lu_new(n, nnz);
+    if (orc_lu_factor(lu, col_ptr, row_idx, lead_vals, q, pivot_tol) != 0) {
+        orc_lu_free(lu);
+        return -1;
+    }
+    orc_lu32 *f = orc_lu32_new(n);
+    float *b32 = calloc(n + 1, 4);
+    const double wide = 1.0 / pivot_tol;
+    for (size_t l = 0; l < lanes; l++) {
+        int bad = orc_lu_lane_refactor32(lu, f, col_ptr, a + l * nnz, growth, wide);
+        if (fail) fail[l] = (uint32_t)bad;
+        if (lx) memcpy(lx + l * lu->nnz_l, f->lx, (size_t)lu->nnz_l * 4);
+        if (ux) memcpy(ux + l * lu->nnz_u, f->ux, (size_t)lu->nnz_u * 4);
+        if (udiag) memcpy(udiag + l * n, f->udiag, (size_t)n * 4);
+        if (x && b) {
+            for (uint32_t i = 0; i < n; i++) b32[i] = (float)b[l * n + i];
+            orc_lu_solve32(lu, f, b32, x + l * n);
+        }
+    }
+    free(b32);
+    orc_lu32_free(f);
+    orc_lu_free(lu);
+    return 0;
+}

@@ -157,3 +157,63 @@ def test_full_size_ota_converges_and_satisfies_kcl(lanes, oracle, tiling):
         g, f = c.assemble_at(vals[l], got["x"][l])
         assert np.abs(f).max() < 1e-5
     eng.close()
+
+
+# ---- FP32 factors with FP64 refinement ----
+
+@pytest.mark.parametrize("family,lanes_n", [("inverter_chain", 256), ("ota_array", 256)])
+def test_fp32_refined_bitwise_against_oracle(lanes, oracle, family, lanes_n):
+    """The FP32 mode is bitwise the oracle's FP32 mode: FP32 replay, FP32
+    solves, FP64 residual and refinement, re-pivots and FP64 fallbacks."""
+    fam = circuits.FAMILIES[family]()
+    vals = fam.sample(2027, lanes_n)
+    say(f"fp32 refined {family} lanes={lanes_n}")
+    eng = lanes.Engine(fam.deck.arrays())
+    _, _, q = eng.pattern()
+    got = eng.solve_op(vals, precision=lanes.FP32_REFINED)
+    ref = oracle.Circuit(fam.deck.arrays()).op(vals, q, threads=8, precision=1)
+    assert np.array_equal(got["reasons"], ref["reasons"])
+    assert np.array_equal(got["iterations"], ref["iterations"])
+    assert np.array_equal(bits(got["x"]), bits(ref["x"]))
+    assert eng.timing().fp64_lanes == int(ref["fp64_lanes"].sum())
+    say(f"  converged {(got['reasons'] == 0).sum()}/{lanes_n} fp64 fallbacks {eng.timing().fp64_lanes} "
+        f"repivots {eng.timing().repivots}")
+    eng.close()
+
+
+def test_fp32_refinement_never_succeeding_is_the_fp64_newton(lanes):
+    """refine_tol 0 with one step: every lane stalls in its first solve,
+    redoes that iteration in FP64 and finishes in FP64, so the results are
+    bitwise the FP64 lanes'."""
+    fam = circuits.ota_array()
+    vals = fam.sample(8, 256)
+    eng = lanes.Engine(fam.deck.arrays())
+    a = eng.solve_op(vals, tiling=lanes.TILING_WAVE)
+    b = eng.solve_op(vals, precision=lanes.FP32_REFINED, refine_tol=0.0, refine_max=1)
+    assert eng.timing().fp64_lanes == 256
+    assert np.array_equal(a["iterations"], b["iterations"])
+    assert np.array_equal(bits(a["x"]), bits(b["x"]))
+    eng.close()
+
+
+def test_fp32_full_size_ota_error_against_fp64(lanes, oracle):
+    """16,384 OTA lanes: FP32+refined against plain FP64, per node value
+    |x32 - x64| <= 1e-7 max(|x64|, 1e-6) (relative, floored at vntol); a sample
+    is bitwise the oracle's FP32 mode."""
+    fam = circuits.ota_array()
+    n_lanes = 16384
+    vals = fam.sample(4, n_lanes)
+    say(f"fp32 refined ota_array full size {n_lanes}")
+    eng = lanes.Engine(fam.deck.arrays())
+    _, _, q = eng.pattern()
+    a = eng.solve_op(vals, tiling=lanes.TILING_WAVE)
+    b = eng.solve_op(vals, precision=lanes.FP32_REFINED)
+    assert (a["reasons"] == 0).all() and (b["reasons"] == 0).all()
+    err = np.abs(a["x"] - b["x"]) / np.maximum(np.abs(a["x"]), 1e-6)
+    say(f"  max relative error {err.max():.3e}, fp64 fallbacks {eng.timing().fp64_lanes}")
+    assert err.max() <= 1e-7
+    pick = np.arange(0, n_lanes, 2039)
+    ref = oracle.Circuit(fam.deck.arrays()).op(vals[pick], q, threads=8, precision=1)
+    assert np.array_equal(b["iterations"][pick], ref["iterations"])
+    assert np.array_equal(bits(b["x"][pick]), bits(ref["x"]))
+    eng.close()

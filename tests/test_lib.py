@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
 def test_abi_version_and_defaults():
     from espice_amd import lanes
     lib = lanes.load()
-    assert lib.esl_abi_version() == 1
+    assert lib.esl_abi_version() == 2
     o = lanes.default_options()
     assert o.struct_size == ctypes.sizeof(lanes.OpOptions)
     assert o.tol.reltol == 1e-3 and o.tol.abstol == 1e-12 and o.tol.vntol == 1e-6
@@ -45,3 +45,13 @@ def test_invalid_handles_are_rejected():
     assert lib.esl_engine_info_get(ctypes.c_void_p(12345), ctypes.byref(info)) == lanes.INVALID_HANDLE
     assert lib.esl_engine_destroy(ctypes.c_void_p(12345)) == lanes.INVALID_HANDLE
     assert lib.esl_engine_destroy(None) == lanes.OK
+
+
+def test_invalid_network_handles_are_rejected():
+    from espice_amd import lanes
+    lib = lanes.load()
+    c = ctypes.c_size_t()
+    assert lib.esl_mlp_param_count(ctypes.c_void_p(777), ctypes.byref(c)) == lanes.INVALID_HANDLE
+    assert lib.esl_mlp_destroy(ctypes.c_void_p(777)) == lanes.INVALID_HANDLE
+    o = lanes.default_train_options()
+    assert o.struct_size == ctypes.sizeof(lanes.MlpTrainOptions) and o.epochs > 0 and o.batch > 0
