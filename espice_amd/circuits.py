@@ -9,6 +9,9 @@ Families:
   inverter_chain  16-stage MOS1 CMOS inverter chain, Monte Carlo .op
   ota_array       64 MOS1 5-transistor OTAs in unity-gain feedback, .op,
                   32 live parameters per point
+  diode_stack     a resistor into a stack of junction diodes, .op
+  rc_mesh         k x k RC mesh with diode clamps and a step source, the
+                  transient workload (every R, C and IS live)
 """
 from __future__ import annotations
 
@@ -67,6 +70,9 @@ class Family:
                 out[:, col] = a * (1.0 + b * normal(seed, idx, var))
             elif kind == "uniform_of":
                 out[:, col] = uniform01(seed, idx, var) * out[:, self.deck.live.index(a)]
+            elif kind == "log_uniform":
+                la, lb = np.log(a), np.log(b)
+                out[:, col] = np.exp(la + (lb - la) * uniform01(seed, idx, var))
             else:
                 raise ValueError(kind)
         return out
@@ -150,4 +156,64 @@ def ota_array(count: int = 64, groups: int = 3) -> Family:
     return Family(d.title, d, draws)
 
 
-FAMILIES = {"inverter_chain": inverter_chain, "ota_array": ota_array}
+def diode_stack(stages: int = 8) -> Family:
+    """A DC supply through a resistor into a stack of `stages` junction
+    diodes (the reference's convergence/diode_stack_* decks): VDD ~ N(5 V,
+    2%), R ~ N(1 kOhm, 1%), each diode's IS ~ log-uniform [1e-16, 1e-14] A."""
+    d = Deck(f"diode_stack_{stages}")
+    d.vsource("vdd", "vdd", 0, 5.0)
+    d.resistor("r1", "vdd", "a0", 1e3)
+    for k in range(stages):
+        nxt = 0 if k == stages - 1 else f"a{k + 1}"
+        d.diode(f"d{k}", f"a{k}", nxt, {"is": 1e-15})
+    d.bind("vdd", 0, "vdd")
+    d.bind("r1", 0, "r1")
+    draws = [("vdd", "rel_normal", 5.0, 0.02), ("r1", "rel_normal", 1e3, 0.01)]
+    for k in range(stages):
+        d.bind(f"d{k}", "is", f"is{k}")
+        draws.append((f"is{k}", "log_uniform", 1e-16, 1e-14))
+    return Family(d.title, d, draws)
+
+
+def rc_mesh(k: int = 64, diode_frac: float = 0.05, layout_seed: int = 1) -> Family:
+    """k x k RC mesh (5-point stencil), every edge resistor R ~ log-uniform
+    [100, 1e4] Ohm, every node a grounded C ~ log-uniform [1 fF, 100 fF],
+    diode clamps to ground on a seeded `diode_frac` of the nodes with IS ~
+    log-uniform [1e-16, 1e-14] A; a 1 V step (0 V before t = 0) into node
+    (0, 0) through 50 Ohm. Every R, C and IS is a live parameter, so each
+    point is its own mesh on one shared pattern."""
+    d = Deck(f"rc_mesh_{k}")
+    node = lambda i, j: f"n{i}_{j}"
+    d.vsource("vs", "src", 0, 0.0, step_to=1.0, step_at=0.0)
+    d.resistor("rs", "src", node(0, 0), 50.0)
+    draws = []
+    for i in range(k):
+        for j in range(k):
+            if j + 1 < k:
+                nm = f"rh{i}_{j}"
+                d.resistor(nm, node(i, j), node(i, j + 1), 1e3)
+                d.bind(nm, 0, nm)
+                draws.append((nm, "log_uniform", 100.0, 1e4))
+            if i + 1 < k:
+                nm = f"rv{i}_{j}"
+                d.resistor(nm, node(i, j), node(i + 1, j), 1e3)
+                d.bind(nm, 0, nm)
+                draws.append((nm, "log_uniform", 100.0, 1e4))
+    for i in range(k):
+        for j in range(k):
+            nm = f"c{i}_{j}"
+            d.capacitor(nm, node(i, j), 0, 1e-14)
+            d.bind(nm, 0, nm)
+            draws.append((nm, "log_uniform", 1e-15, 1e-13))
+    pick = uniform01(layout_seed, np.arange(k * k, dtype=np.uint64), 0) < diode_frac
+    for e in np.nonzero(pick)[0]:
+        i, j = divmod(int(e), k)
+        nm = f"d{i}_{j}"
+        d.diode(nm, node(i, j), 0, {"is": 1e-15})
+        d.bind(nm, "is", nm)
+        draws.append((nm, "log_uniform", 1e-16, 1e-14))
+    return Family(d.title, d, draws)
+
+
+FAMILIES = {"inverter_chain": inverter_chain, "ota_array": ota_array, "diode_stack": diode_stack,
+            "rc_mesh": rc_mesh}

@@ -232,4 +232,120 @@ __device__ inline double mos_eval_v(const Mos &m, double xd, double xg, double x
 }
 
 
+// ---- junction diode (models/diode.va, the RS = 0 subset without breakdown,
+// recombination, knee currents or sidewall) ----
+struct Dio {
+    double is, vte, vcrit, gmin, cjo, vj, mt, qexp, vfc, f1, inv_f2, f3, tt;
+};
+
+__device__ inline Dio dio_load(const DeckProg &dp, uint32_t j, const double *live, uint32_t live_stride = 1)
+{
+    const uint32_t nd = dp.n_dio;
+    const double *f = dp.d_par;
+    Dio d;
+    const int b = dp.d_bind_is[j];
+    const double isx = b >= 0 ? live[(size_t)b * live_stride] : f[DF_IS * nd + j];
+    d.is = isx * f[DF_IS_SCALE * nd + j];
+    d.vte = f[DF_VTE * nd + j];
+    d.vcrit = d.vte * esl::log(d.vte / (1.4142135623730951 * d.is));
+    d.gmin = f[DF_GMIN * nd + j];
+    d.cjo = f[DF_CJO * nd + j];
+    d.vj = f[DF_VJ * nd + j];
+    d.mt = f[DF_MT * nd + j];
+    d.qexp = f[DF_QEXP * nd + j];
+    d.vfc = f[DF_VFC * nd + j];
+    d.f1 = f[DF_F1 * nd + j];
+    d.inv_f2 = f[DF_INVF2 * nd + j];
+    d.f3 = f[DF_F3 * nd + j];
+    d.tt = f[DF_TT * nd + j];
+    return d;
+}
+
+// Junction current with the 80 vte tangent continuation and the cubic
+// reverse tail, plus gmin * vd; g = dI/dvd.
+__device__ inline double dio_current(const Dio &d, double vd, double &g)
+{
+    double id, gd;
+    if (vd >= -3.0 * d.vte) {
+        const double u = vd / d.vte;
+        const double ul = fmin(u, 80.0);
+        const double e = esl::exp(ul);
+        id = d.is * (e * (1.0 + (u - ul)) - 1.0);
+        gd = d.is * e / d.vte;
+    } else {
+        const double w = 3.0 * d.vte / (vd * 2.718281828459045);
+        const double w3 = w * w * w;
+        id = -d.is * (1.0 + w3);
+        gd = 3.0 * d.is * w3 / vd;
+    }
+    g = gd + d.gmin;
+    return id + d.gmin * vd;
+}
+
+// Junction charge (depletion, two-branch SPICE form) plus tt * I; c = dq/dvd.
+__device__ inline double dio_charge(const Dio &d, double vd, double i, double g, double &c)
+{
+    double q = 0.0, cap = 0.0;
+    if (d.cjo > 0) {
+        if (vd < d.vfc) {
+            const double a = 1.0 - vd / d.vj;
+            const double p = esl::exp(d.qexp * esl::log(a));
+            q = d.cjo * d.vj * (1.0 - p) / d.qexp;
+            cap = d.cjo * p / a;
+        } else {
+            q = d.cjo * (d.f1 + d.inv_f2 * (d.f3 * (vd - d.vfc) + (d.mt / (2.0 * d.vj)) * (vd * vd - d.vfc * d.vfc)));
+            cap = d.cjo * d.inv_f2 * (d.f3 + (d.mt / d.vj) * vd);
+        }
+    }
+    if (d.tt > 0) {
+        q = q + d.tt * i;
+        cap = cap + d.tt * g;
+    }
+    c = cap;
+    return q;
+}
+
+// pnjlim of the junction voltage against the previous limited value, with
+// the same snap to the probe as the MOS limiter; true when it moved.
+__device__ inline bool dio_limit_v(const Dio &d, double xa, double xk, double &lvd)
+{
+    const double vd = xa - xk;
+    double v = pnjlim(vd, lvd, d.vte, d.vcrit);
+    const double tol = 16.0 * 2.220446049250313e-16 * fmax(fabs(xa), fabs(xk));
+    if (fabs(v - vd) <= tol) v = vd;
+    lvd = v;
+    return v != vd;
+}
+
+// Stamps at the limited voltage vl with the residual correction toward the
+// probe: out[0..3] Jacobian (a,a) (a,k) (k,a) (k,k), out[4..5] residual rows
+// a, k; with charges, out[6..9] the charge Jacobian and out[10..11] the row
+// charges, corrected the same way.
+__device__ inline void dio_eval_v(const Dio &d, double xa, double xk, double vl, bool charges, double *out)
+{
+    double g;
+    double i = dio_current(d, vl, g);
+    const double dv = (xa - xk) - vl;
+    double q = 0.0, c = 0.0;
+    if (charges) q = dio_charge(d, vl, i, g, c);
+    if (dv != 0) {
+        i = i + g * dv;
+        if (charges) q = q + c * dv;
+    }
+    out[0] = g;
+    out[1] = -g;
+    out[2] = -g;
+    out[3] = g;
+    out[4] = i;
+    out[5] = -i;
+    if (charges) {
+        out[6] = c;
+        out[7] = -c;
+        out[8] = -c;
+        out[9] = c;
+        out[10] = q;
+        out[11] = -q;
+    }
+}
+
 }  // namespace esl

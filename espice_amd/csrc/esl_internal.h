@@ -26,7 +26,10 @@ struct uint2_t {
 
 constexpr int kWave = 64;
 constexpr int kStageEntries = 14;   // most cells one instance stages (MOS1)
-constexpr uint32_t kRhsBit = 0x80000000u;
+constexpr uint32_t kRhsBit = 0x80000000u;   // task target is a residual row
+constexpr uint32_t kCBit = 0x40000000u;     // ... a charge-Jacobian slot (transient only)
+constexpr uint32_t kQBit = 0x20000000u;     // ... a charge row (transient only)
+constexpr uint32_t kTargetMask = 0x1fffffffu;
 constexpr uint32_t kVoidBit = 1u, kScaledBit = 2u;
 
 enum Kind : uint32_t { kR = 0, kC = 1, kV = 2, kI = 3, kD = 4, kMos = 5, kKinds = 6 };
@@ -36,6 +39,11 @@ enum MosField : uint32_t {
     MF_TYPE = 0, MF_VTO, MF_VBIT_A, MF_VBIT_B, MF_VBIT_C, MF_GAMMA, MF_PHIT,
     MF_LAMBDA, MF_VT, MF_ISATS, MF_ISATD, MF_VCRITS, MF_VCRITD, MF_GMIN,
     MF_KP, MF_RATIO4, MF_M, MF_W, MF_LEFF, MF_COUNT
+};
+// Diode fields (prepared on the host): is_t = IS * DF_IS_SCALE.
+enum DioField : uint32_t {
+    DF_IS = 0, DF_IS_SCALE, DF_VTE, DF_GMIN, DF_CJO, DF_VJ, DF_MT, DF_QEXP, DF_VFC, DF_F1, DF_INVF2, DF_F3, DF_TT,
+    DF_COUNT
 };
 
 struct Chunk {
@@ -66,6 +74,15 @@ struct DeckProg {
     const uint32_t *r_slot, *v_slot, *m_slot;
     // the pattern by rows, columns ascending: [n + 1], [nnz], [nnz] (pattern slot)
     const uint32_t *csr_ptr, *csr_col, *csr_slot;
+    // diodes: t[2] (a, k), fields [DF_COUNT][n_dio], binding of IS
+    uint32_t n_dio, n_cap, n_cslot, pad1;
+    const uint32_t *d_t; const double *d_par; const int32_t *d_bind_is;
+    // capacitors: t[2], value, bind
+    const uint32_t *c_t; const double *c_val; const int32_t *c_bind;
+    // V source waveforms: kind (0 DC, 1 step), v2, td per source
+    const uint32_t *v_wkind; const double *v_v2, *v_td;
+    // charge-Jacobian slots: the k-th is pattern slot cslot_pat[k]
+    const uint32_t *cslot_pat;
 };
 constexpr uint32_t kNoSlot = 0xffffffffu;
 

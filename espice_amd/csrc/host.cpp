@@ -772,7 +772,10 @@ struct Compiled {
     vector<int32_t> r_bind, v_bind, i_bind, m_bind_vto, m_bind_kp;
     vector<u32> r_slot, v_slot, m_slot;
     vector<u32> csr_ptr, csr_col, csr_slot;   // the pattern by rows, columns ascending
-    u32 n_mos = 0;
+    vector<u32> d_t, c_t, v_wkind, cslot_pat;
+    vector<double> d_par, c_val, v_v2, v_td;
+    vector<int32_t> d_bind_is, c_bind;
+    u32 n_mos = 0, n_dio = 0, n_cap = 0;
 };
 
 void build_csr(Compiled &c)
@@ -790,6 +793,46 @@ void build_csr(Compiled &c)
             c.csr_col[e] = j;
             c.csr_slot[e] = p;
         }
+}
+
+// Diode prep (models/diode.va, TLEV 0 / TLEVC 0 temperature map): the
+// fields of esl::DioField. is_t = IS * is_scale, so a per-point IS scales.
+void dio_fields(const double *p, double *f)
+{
+    const double n = p[ESL_D_N], area = p[ESL_D_AREA], mt = p[ESL_D_M], fc = p[ESL_D_FC];
+    const double tk = p[ESL_D_TEMP] + 273.15, tnomk = p[ESL_D_TNOM] + 273.15;
+    const double ratio = tk / tnomk;
+    const double vt = KB / QE * tk, vtnom = KB / QE * tnomk;
+    const double vte = n * vt;
+    const double egfet = 1.16 - (7.02e-4 * tk * tk) / (tk + 1108.0);
+    const double egfet1 = 1.16 - (7.02e-4 * tnomk * tnomk) / (tnomk + 1108.0);
+    const double fact1 = tnomk / 300.15, fact2 = tk / 300.15;
+    const double arg1 = -egfet1 / (2.0 * KB * tnomk) + 1.1150877 / (2.0 * KB * 300.15);
+    const double argt = -egfet / (2.0 * KB * tk) + 1.1150877 / (2.0 * KB * 300.15);
+    const double pbfact1 = -2.0 * vtnom * (1.5 * esl::log(fact1) + QE * arg1);
+    const double pbfact = -2.0 * vt * (1.5 * esl::log(fact2) + QE * argt);
+    const double vj = p[ESL_D_VJ];
+    const double pbo = (vj - pbfact1) / fact1;
+    const double gmaold = (vj - pbo) / pbo;
+    const double vj_t = pbfact + fact2 * pbo;
+    const double gmanew = (vj_t - pbo) / pbo;
+    const double cjo_t = p[ESL_D_CJO] * (1.0 + mt * (400e-6 * (tk - 300.15) - gmanew)) /
+                         (1.0 + mt * (400e-6 * (tnomk - 300.15) - gmaold)) * area;
+    const double xti = p[ESL_D_XTI], eg = p[ESL_D_EG];
+    auto pw = [](double a, double b) { return esl::exp(b * esl::log(a)); };
+    f[esl::DF_IS] = p[ESL_D_IS];
+    f[esl::DF_IS_SCALE] = area * esl::exp((ratio - 1.0) * eg / vte + (xti / n) * esl::log(ratio));
+    f[esl::DF_VTE] = vte;
+    f[esl::DF_GMIN] = p[ESL_D_GMIN];
+    f[esl::DF_CJO] = cjo_t;
+    f[esl::DF_VJ] = vj_t;
+    f[esl::DF_MT] = mt;
+    f[esl::DF_QEXP] = 1.0 - mt;
+    f[esl::DF_VFC] = fc * vj_t;
+    f[esl::DF_F1] = vj_t * (1.0 - pw(1.0 - fc, 1.0 - mt)) / (1.0 - mt);
+    f[esl::DF_INVF2] = 1.0 / pw(1.0 - fc, 1.0 + mt);
+    f[esl::DF_F3] = 1.0 - fc * (1.0 + mt);
+    f[esl::DF_TT] = p[ESL_D_TT];
 }
 
 u32 find_slot(const Compiled &c, u32 r, u32 col)
@@ -816,13 +859,13 @@ bool compile_deck(const esl_deck *d, Compiled &c, std::string &err)
     for (u32 i = 0; i < d->n_dev; i++) {
         const u32 k = d->kind[i];
         if (k >= esl::kKinds) { err = "unknown device kind"; return false; }
-        if (k == esl::kD) { err = "diode lanes are not supported by the operating-point kernel"; return false; }
         for (int t = 0; t < ESL_NTERM; t++)
             if (d->nodes[i * ESL_NTERM + t] >= d->n_nodes) { err = "node out of range"; return false; }
         for (int p = 0; p < NPARAM; p++) {
             if (bind[i][p] < 0) continue;
             bool ok = (k == esl::kR || k == esl::kC || k == esl::kV || k == esl::kI) ? p == 0
-                      : (k == esl::kMos) ? (p == ESL_M_VTO || p == ESL_M_KP) : false;
+                      : (k == esl::kMos) ? (p == ESL_M_VTO || p == ESL_M_KP)
+                      : (k == esl::kD) ? p == ESL_D_IS : false;
             if (k == esl::kMos && p == ESL_M_KP) {
                 const double *pp = d->params + (size_t)i * NPARAM;
                 const unsigned given = (unsigned)pp[ESL_M_GIVEN];
@@ -831,6 +874,19 @@ bool compile_deck(const esl_deck *d, Compiled &c, std::string &err)
             if (!ok) { err = "binding of a parameter the lanes cannot move per point"; return false; }
         }
         if (k == esl::kR && !(d->params[(size_t)i * NPARAM] != 0) && bind[i][0] < 0) { err = "zero resistance"; return false; }
+        if (k == esl::kD) {
+            const double *pp = d->params + (size_t)i * NPARAM;
+            if (!(pp[ESL_D_N] > 0) || !(pp[ESL_D_AREA] > 0) || !(pp[ESL_D_IS] > 0) || !(pp[ESL_D_VJ] > 0) ||
+                !(pp[ESL_D_FC] >= 0 && pp[ESL_D_FC] < 1) || !(pp[ESL_D_M] > 0 && pp[ESL_D_M] < 1) ||
+                !(pp[ESL_D_CJO] >= 0) || !(pp[ESL_D_TT] >= 0) || !(pp[ESL_D_GMIN] >= 0)) {
+                err = "diode parameter out of range";
+                return false;
+            }
+        }
+        if (k == esl::kV) {
+            const double w = d->params[(size_t)i * NPARAM + ESL_V_WAVE];
+            if (w != 0.0 && w != 1.0) { err = "unknown source waveform"; return false; }
+        }
     }
     c.n_nodes = d->n_nodes;
     c.n_live = d->n_live;
@@ -855,7 +911,7 @@ bool compile_deck(const esl_deck *d, Compiled &c, std::string &err)
     vector<std::pair<u32, u32>> ent;   // (col, row)
     for (u32 i = 0; i < n; i++) ent.push_back({i, i});
     for (auto &in : c.inst) {
-        if (in.kind == esl::kR)
+        if (in.kind == esl::kR || in.kind == esl::kC || in.kind == esl::kD)
             for (auto &e : R_ENT) ent.push_back({in.t[e[1]], in.t[e[0]]});
         else if (in.kind == esl::kV) {
             ent.push_back({in.br, in.t[0]});
@@ -883,7 +939,7 @@ bool compile_deck(const esl_deck *d, Compiled &c, std::string &err)
     c.diag[0] = c.col_ptr[0];
     for (auto &in : c.inst) {
         for (u32 &s : in.slot) s = NONE;
-        if (in.kind == esl::kR)
+        if (in.kind == esl::kR || in.kind == esl::kC || in.kind == esl::kD)
             for (int e = 0; e < 4; e++) in.slot[e] = find_slot(c, in.t[R_ENT[e][0]], in.t[R_ENT[e][1]]);
         else if (in.kind == esl::kV) {
             in.slot[0] = find_slot(c, in.t[0], in.br);
@@ -908,11 +964,20 @@ bool compile_deck(const esl_deck *d, Compiled &c, std::string &err)
         case esl::kV:
             c.v_t.push_back(in.t[0]); c.v_t.push_back(in.t[1]); c.v_t.push_back(in.br);
             c.v_val.push_back(p[0]); c.v_bind.push_back(bd[0]);
+            c.v_wkind.push_back((u32)p[ESL_V_WAVE]); c.v_v2.push_back(p[ESL_V_V2]); c.v_td.push_back(p[ESL_V_TD]);
             for (int e = 0; e < 4; e++) c.v_slot.push_back(in.slot[e]);
             break;
         case esl::kI:
             c.i_t.push_back(in.t[0]); c.i_t.push_back(in.t[1]);
             c.i_val.push_back(p[0]); c.i_bind.push_back(bd[0]);
+            break;
+        case esl::kC:
+            c.c_t.push_back(in.t[0]); c.c_t.push_back(in.t[1]);
+            c.c_val.push_back(p[0]); c.c_bind.push_back(bd[0]);
+            break;
+        case esl::kD:
+            c.d_t.push_back(in.t[0]); c.d_t.push_back(in.t[1]);
+            c.d_bind_is.push_back(bd[ESL_D_IS]);
             break;
         case esl::kMos:
             for (int t = 0; t < 4; t++) c.m_t.push_back(in.t[t]);
@@ -925,6 +990,24 @@ bool compile_deck(const esl_deck *d, Compiled &c, std::string &err)
         }
     }
     c.n_mos = c.kind_first[esl::kMos + 1] - c.kind_first[esl::kMos];
+    c.n_dio = c.kind_first[esl::kD + 1] - c.kind_first[esl::kD];
+    c.n_cap = c.kind_first[esl::kC + 1] - c.kind_first[esl::kC];
+    c.d_par.assign((size_t)esl::DF_COUNT * c.n_dio, 0.0);
+    for (u32 j = 0; j < c.n_dio; j++) {
+        const Inst &in = c.inst[c.kind_first[esl::kD] + j];
+        double f[esl::DF_COUNT];
+        dio_fields(d->params + (size_t)in.dev * NPARAM, f);
+        for (u32 k = 0; k < esl::DF_COUNT; k++) c.d_par[(size_t)k * c.n_dio + j] = f[k];
+    }
+    // charge-Jacobian slots (capacitor and diode entries), in first-use order
+    std::map<u32, u32> cslot_of;
+    for (auto &in : c.inst)
+        if (in.kind == esl::kC || in.kind == esl::kD)
+            for (int e = 0; e < 4; e++)
+                if (in.slot[e] != NONE && !cslot_of.count(in.slot[e])) {
+                    cslot_of[in.slot[e]] = (u32)c.cslot_pat.size();
+                    c.cslot_pat.push_back(in.slot[e]);
+                }
     c.m_par.assign((size_t)esl::MF_COUNT * c.n_mos, 0.0);
     for (u32 j = 0; j < c.n_mos; j++) {
         const Inst &in = c.inst[c.kind_first[esl::kMos] + j];
@@ -935,7 +1018,6 @@ bool compile_deck(const esl_deck *d, Compiled &c, std::string &err)
     // chunks and reduce tasks
     for (u32 k = 0; k < esl::kKinds; k++) {
         const u32 b = c.kind_first[k], e = c.kind_first[k + 1];
-        if (k == esl::kC) continue;   // no DC stamps
         for (u32 first = b; first < e; first += esl::kWave) {
             const u32 cnt = std::min<u32>(esl::kWave, e - first);
             esl::Chunk ch{k, first - b, cnt, (u32)c.task_target.size()};
@@ -949,7 +1031,24 @@ bool compile_deck(const esl_deck *d, Compiled &c, std::string &err)
                 auto row = [&](u32 ei, u32 r) {
                     if (r != 0) tc.push_back({esl::kRhsBit | r, ei * esl::kWave + t});
                 };
-                if (k == esl::kR) {
+                auto cjac = [&](u32 ei, u32 slot) {
+                    if (slot != NONE) tc.push_back({esl::kCBit | cslot_of.at(slot), ei * esl::kWave + t});
+                };
+                auto qrow = [&](u32 ei, u32 r) {
+                    if (r != 0) tc.push_back({esl::kQBit | r, ei * esl::kWave + t});
+                };
+                if (k == esl::kC) {
+                    for (u32 ei = 0; ei < 4; ei++) cjac(6 + ei, in.slot[ei]);
+                    qrow(10, in.t[0]);
+                    qrow(11, in.t[1]);
+                } else if (k == esl::kD) {
+                    for (u32 ei = 0; ei < 4; ei++) jac(ei, in.slot[ei]);
+                    row(4, in.t[0]);
+                    row(5, in.t[1]);
+                    for (u32 ei = 0; ei < 4; ei++) cjac(6 + ei, in.slot[ei]);
+                    qrow(10, in.t[0]);
+                    qrow(11, in.t[1]);
+                } else if (k == esl::kR) {
                     for (u32 ei = 0; ei < 4; ei++) jac(ei, in.slot[ei]);
                     row(4, in.t[0]);
                     row(5, in.t[1]);
@@ -1039,6 +1138,9 @@ hipError_t upload_deck(Engine &e)
            o_mbk = pk.add(c.m_bind_kp), o_rs = pk.add(c.r_slot), o_vs = pk.add(c.v_slot), o_ms = pk.add(c.m_slot);
     build_csr(c);
     size_t o_cp = pk.add(c.csr_ptr), o_cc = pk.add(c.csr_col), o_cs = pk.add(c.csr_slot);
+    size_t o_dt = pk.add(c.d_t), o_dp = pk.add(c.d_par), o_db = pk.add(c.d_bind_is), o_ct = pk.add(c.c_t),
+           o_cv = pk.add(c.c_val), o_cb = pk.add(c.c_bind), o_vw = pk.add(c.v_wkind), o_v2 = pk.add(c.v_v2),
+           o_vtd = pk.add(c.v_td), o_csp = pk.add(c.cslot_pat);
     hipError_t err = e.deck_buf.ensure(pk.blob.size());
     if (err != hipSuccess) return err;
     err = hipMemcpy(e.deck_buf.p, pk.blob.data(), pk.blob.size(), hipMemcpyHostToDevice);
@@ -1078,6 +1180,19 @@ hipError_t upload_deck(Engine &e)
     dp.csr_ptr = reinterpret_cast<const u32 *>(b + o_cp);
     dp.csr_col = reinterpret_cast<const u32 *>(b + o_cc);
     dp.csr_slot = reinterpret_cast<const u32 *>(b + o_cs);
+    dp.n_dio = c.n_dio;
+    dp.n_cap = c.n_cap;
+    dp.n_cslot = (u32)c.cslot_pat.size();
+    dp.d_t = reinterpret_cast<const u32 *>(b + o_dt);
+    dp.d_par = reinterpret_cast<const double *>(b + o_dp);
+    dp.d_bind_is = reinterpret_cast<const int32_t *>(b + o_db);
+    dp.c_t = reinterpret_cast<const u32 *>(b + o_ct);
+    dp.c_val = reinterpret_cast<const double *>(b + o_cv);
+    dp.c_bind = reinterpret_cast<const int32_t *>(b + o_cb);
+    dp.v_wkind = reinterpret_cast<const u32 *>(b + o_vw);
+    dp.v_v2 = reinterpret_cast<const double *>(b + o_v2);
+    dp.v_td = reinterpret_cast<const double *>(b + o_vtd);
+    dp.cslot_pat = reinterpret_cast<const u32 *>(b + o_csp);
     return hipSuccess;
 }
 
@@ -1101,7 +1216,8 @@ void build_reduce_streams(const Compiled &c, TapeHostProg &o)
     for (size_t ci = 0; ci < nch; ci++) {
         const u32 t0 = c.chunks[ci].task_begin, t1 = c.chunks[ci + 1].task_begin;
         vector<u32> order;
-        for (u32 t = t0; t < t1; t++) order.push_back(t);
+        for (u32 t = t0; t < t1; t++)
+            if (!(c.task_target[t] & (esl::kCBit | esl::kQBit))) order.push_back(t);
         auto len = [&](u32 t) { return 1 + c.task_cell[t + 1] - c.task_cell[t]; };
         std::stable_sort(order.begin(), order.end(), [&](u32 a, u32 b) { return len(a) > len(b); });
         vector<vector<u32>> words(esl::kWave);
@@ -1215,19 +1331,31 @@ bool validate_deck(const Compiled &c)
         const u32 kind_n = c.kind_first[ch.kind + 1] - c.kind_first[ch.kind];
         if (ch.first + ch.count > kind_n) return false;
     }
+    const u32 ncs = (u32)c.cslot_pat.size();
     for (u32 t : c.task_target) {
-        if (t & esl::kRhsBit) {
-            if ((t & ~esl::kRhsBit) >= n) return false;
+        const u32 kinds = ((t & esl::kRhsBit) != 0) + ((t & esl::kCBit) != 0) + ((t & esl::kQBit) != 0);
+        const u32 idx = t & esl::kTargetMask;
+        if (kinds > 1) return false;
+        if ((t & esl::kRhsBit) || (t & esl::kQBit)) {
+            if (idx >= n) return false;
+        } else if (t & esl::kCBit) {
+            if (idx >= ncs) return false;
         } else if (t >= nnz) {
             return false;
         }
     }
+    for (u32 v : c.cslot_pat) if (v >= nnz) return false;
     for (u32 v : c.cells) if (v >= (u32)(esl::kWave * esl::kStageEntries)) return false;
     auto terms_ok = [&](const vector<u32> &t) {
         for (u32 v : t) if (v >= n) return false;
         return true;
     };
-    if (!terms_ok(c.r_t) || !terms_ok(c.v_t) || !terms_ok(c.i_t) || !terms_ok(c.m_t)) return false;
+    if (!terms_ok(c.r_t) || !terms_ok(c.v_t) || !terms_ok(c.i_t) || !terms_ok(c.m_t) || !terms_ok(c.d_t) ||
+        !terms_ok(c.c_t))
+        return false;
+    if (c.d_t.size() != 2 * (size_t)c.n_dio || c.c_t.size() != 2 * (size_t)c.n_cap) return false;
+    if (c.d_par.size() != (size_t)esl::DF_COUNT * c.n_dio || c.d_bind_is.size() != c.n_dio) return false;
+    if (c.c_val.size() != c.n_cap || c.c_bind.size() != c.n_cap) return false;
     auto binds_ok = [&](const vector<int32_t> &b) {
         for (int32_t v : b) if (v >= (int32_t)c.n_live) return false;
         return true;
@@ -1238,7 +1366,7 @@ bool validate_deck(const Compiled &c)
     };
     if (!slots_ok(c.r_slot) || !slots_ok(c.v_slot) || !slots_ok(c.m_slot)) return false;
     return binds_ok(c.r_bind) && binds_ok(c.v_bind) && binds_ok(c.i_bind) && binds_ok(c.m_bind_vto) &&
-           binds_ok(c.m_bind_kp);
+           binds_ok(c.m_bind_kp) && binds_ok(c.d_bind_is) && binds_ok(c.c_bind);
 }
 
 u32 device_lds_max()
@@ -1548,7 +1676,7 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
     HIPCHK(e->live.ensure(std::max<size_t>(1, count * P) * 8));
     HIPCHK(e->live_t.ensure(std::max<size_t>(1, count * P) * 8));
     HIPCHK(e->x.ensure(count * n * 8));
-    HIPCHK(e->lim.ensure(std::max<size_t>(1, count * 4 * c.n_mos) * 8));
+    HIPCHK(e->lim.ensure(std::max<size_t>(1, count * (4 * c.n_mos + c.n_dio)) * 8));
     HIPCHK(e->resume.ensure(count * sizeof(esl::LaneResume)));
     HIPCHK(e->status.ensure(count * 4));
     HIPCHK(e->iters.ensure(count * 4));
@@ -1651,7 +1779,10 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
     // tilings once per engine on the first batch of at least 1024 points
     // (kernel time of the first launch over every lane) and keeps the faster;
     // below that size a batch runs one wavefront per lane.
-    u32 tiling = (fp32 || net) ? (u32)ESL_TILING_WAVE : o.tiling;
+    // the thread tiling covers R, V, I and MOS1 decks in FP64
+    const bool wave_only = fp32 || net || c.n_dio > 0;
+    if (wave_only && o.tiling == ESL_TILING_THREAD) return ESL_INVALID_ARGUMENT;
+    u32 tiling = wave_only ? (u32)ESL_TILING_WAVE : o.tiling;
     if (tiling == ESL_TILING_AUTO) {
         if (e->tuned) {
             tiling = e->tuned;

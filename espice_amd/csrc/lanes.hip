@@ -71,7 +71,8 @@ __device__ inline void mos_eval(const Mos &m, double xd, double xg, double xs, d
 // ---- device evaluation of one chunk (one instance per thread) into the
 // staging cells st[e * 64 + t]
 __device__ inline void eval_chunk(const DeckProg &dp, const Chunk &ch, const double *live, const double *x,
-                                  double *lim, uint32_t lim_stride, double *st, uint32_t tid)
+                                  double *lim, uint32_t lim_stride, double *st, uint32_t tid, double t = 0.0,
+                                  bool charges = false)
 {
     if (tid >= ch.count) return;
     const uint32_t j = ch.first + tid;
@@ -89,9 +90,23 @@ __device__ inline void eval_chunk(const DeckProg &dp, const Chunk &ch, const dou
         st[5 * kWave + tid] = -cur;
         break;
     }
+    case kC: {
+        if (!charges) break;
+        const uint32_t p = dp.c_t[2 * j], q = dp.c_t[2 * j + 1];
+        const double cv = dp.c_bind[j] >= 0 ? live[dp.c_bind[j]] : dp.c_val[j];
+        const double qq = cv * (x[p] - x[q]);
+        st[6 * kWave + tid] = cv;
+        st[7 * kWave + tid] = -cv;
+        st[8 * kWave + tid] = -cv;
+        st[9 * kWave + tid] = cv;
+        st[10 * kWave + tid] = qq;
+        st[11 * kWave + tid] = -qq;
+        break;
+    }
     case kV: {
         const uint32_t p = dp.v_t[3 * j], q = dp.v_t[3 * j + 1], br = dp.v_t[3 * j + 2];
-        const double dc = dp.v_bind[j] >= 0 ? live[dp.v_bind[j]] : dp.v_val[j];
+        const double v1 = dp.v_bind[j] >= 0 ? live[dp.v_bind[j]] : dp.v_val[j];
+        const double dc = (dp.v_wkind[j] == 1 && t > dp.v_td[j]) ? dp.v_v2[j] : v1;
         const double ib = x[br];
         st[0 * kWave + tid] = 1.0;
         st[1 * kWave + tid] = -1.0;
@@ -106,6 +121,15 @@ __device__ inline void eval_chunk(const DeckProg &dp, const Chunk &ch, const dou
         const double dc = dp.i_bind[j] >= 0 ? live[dp.i_bind[j]] : dp.i_val[j];
         st[0 * kWave + tid] = dc;
         st[1 * kWave + tid] = -dc;
+        break;
+    }
+    case kD: {
+        const Dio d = dio_load(dp, j, live);
+        const uint32_t a = dp.d_t[2 * j], k = dp.d_t[2 * j + 1];
+        double out[12];
+        dio_eval_v(d, x[a], x[k], lim[4 * lim_stride + j], charges, out);
+        const int ne = charges ? 12 : 6;
+        for (int e = 0; e < ne; e++) st[e * kWave + tid] = out[e];
         break;
     }
     case kMos: {
@@ -130,11 +154,13 @@ __device__ void assemble_pattern(const DeckProg &dp, const double *live, const d
     __syncthreads();
     for (uint32_t c = 0; c < dp.n_chunks; c++) {
         const Chunk ch = dp.chunks[c];
+        if (ch.kind == kC) continue;   // no static stamps
         eval_chunk(dp, ch, live, x, lim, lim_stride, st, tid);
         __syncthreads();
         const uint32_t t1 = dp.chunks[c + 1].task_begin;
         for (uint32_t t = ch.task_begin + tid; t < t1; t += kWave) {
             const uint32_t tgt = dp.task_target[t];
+            if (tgt & (kCBit | kQBit)) continue;
             double *d = (tgt & kRhsBit) ? &rhs[tgt & ~kRhsBit] : &a[tgt];
             double acc = *d;
             const uint32_t c1 = dp.task_cell[t + 1];
@@ -188,6 +214,7 @@ __device__ void assemble(const DeckProg &dp, const TapeProg &T, const double *li
     __syncthreads();
     for (uint32_t c = 0; c < dp.n_chunks; c++) {
         const Chunk ch = dp.chunks[c];
+        if (ch.kind == kC) continue;   // no static stamps
         eval_chunk(dp, ch, live, x, lim, lim_stride, st, tid);
         __syncthreads();
         const uint32_t *rs = T.red + T.red_off[c] + tid;
@@ -373,6 +400,7 @@ __device__ inline void seed_lim_at(const DeckProg &dp, const double *live, const
         lim[2 * nm + j] = m.tp * (xb - xs);
         lim[3 * nm + j] = 0.0;
     }
+    for (uint32_t j = tid; j < dp.n_dio; j += kWave) lim[4 * nm + j] = x[dp.d_t[2 * j]] - x[dp.d_t[2 * j + 1]];
 }
 
 __device__ inline void lane_init(const OpArgs &a, uint32_t lane, const double *live, double *x, double *lim,
@@ -396,6 +424,7 @@ __device__ inline void lane_init(const OpArgs &a, uint32_t lane, const double *l
             lim[2 * nm + j] = -1.0;
             lim[3 * nm + j] = 0.0;
         }
+        for (uint32_t j = tid; j < dp.n_dio; j += kWave) lim[4 * nm + j] = dio_load(dp, j, live).vcrit;
     } else {
         seed_lim_at(dp, live, x, lim, tid);
     }
@@ -416,7 +445,7 @@ __global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
     double *sdiag = xold + n;
     double *st = sdiag + n;
     const double *live = a.live + (size_t)lane * dp.n_live;
-    double *lim = a.lim + (size_t)lane * 4 * nm;
+    double *lim = a.lim + (size_t)lane * (4 * nm + dp.n_dio);
     lane_init(a, lane, live, x, lim, tid);
 
     if (a.dump) {
@@ -573,6 +602,10 @@ __global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
             const Mos m = mos_load(dp, j, live);
             const uint32_t *t4 = dp.m_t + 4 * j;
             limited |= mos_limit(m, x[t4[0]], x[t4[1]], x[t4[2]], x[t4[3]], lim + j, nm);
+        }
+        for (uint32_t j = tid; j < dp.n_dio; j += kWave) {
+            const Dio d = dio_load(dp, j, live);
+            limited |= dio_limit_v(d, x[dp.d_t[2 * j]], x[dp.d_t[2 * j + 1]], lim[4 * nm + j]);
         }
         limited = wave_any(limited);
         bool ok = !limited && iter != k0 && scaled < 1.0;

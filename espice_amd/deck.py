@@ -32,8 +32,15 @@ M_DEFAULTS = {
 }
 # $param_given bits (ESL_MG_*)
 MG_TOX, MG_KP, MG_PHI, MG_GAMMA = 1, 2, 4, 8
+# Diode level 1 slots (ESL_D_*), models/diode.va (RS = 0 subset).
+D_SLOTS = {"is": 0, "n": 1, "cjo": 2, "vj": 3, "m": 4, "tt": 5, "fc": 6, "area": 7, "gmin": 8,
+           "temp": 9, "tnom": 10, "eg": 11, "xti": 12}
+D_DEFAULTS = {"is": 1e-14, "n": 1.0, "cjo": 0.0, "vj": 1.0, "m": 0.5, "tt": 0.0, "fc": 0.5, "area": 1.0,
+              "gmin": 1e-12, "temp": 27.0, "tnom": 27.0, "eg": 1.11, "xti": 3.0}
+# V source waveform slots (ESL_V_*): value (t <= td), wave (1 = step), v2, td.
+V_WAVE, V_V2, V_TD = 1, 2, 3
 # Parameters a binding may move per point on the GPU lanes.
-BINDABLE = {R: {0}, C: {0}, V: {0}, I: {0}, MOS1: {M_SLOTS["vto"], M_SLOTS["kp"]}}
+BINDABLE = {R: {0}, C: {0}, V: {0}, I: {0}, D: {D_SLOTS["is"]}, MOS1: {M_SLOTS["vto"], M_SLOTS["kp"]}}
 
 
 class Deck:
@@ -87,10 +94,25 @@ class Deck:
         a[0] = c
         return self._add(name, C, (p, n), a)
 
-    def vsource(self, name, p, n, dc):
+    def vsource(self, name, p, n, dc, step_to=None, step_at=0.0):
+        """DC source; with step_to, the value becomes step_to for t > step_at."""
         a = np.zeros(NPARAM)
         a[0] = dc
+        if step_to is not None:
+            a[V_WAVE], a[V_V2], a[V_TD] = 1.0, step_to, step_at
         return self._add(name, V, (p, n), a)
+
+    def diode(self, name, a_node, k_node, model: dict | None = None, **inst):
+        vals = dict(D_DEFAULTS)
+        for k, v in {**(model or {}), **inst}.items():
+            k = k.lower()
+            if k not in D_SLOTS:
+                raise ValueError(f"unsupported diode parameter {k}")
+            vals[k] = float(v)
+        a = np.zeros(NPARAM)
+        for k, slot in D_SLOTS.items():
+            a[slot] = vals[k]
+        return self._add(name, D, (a_node, k_node), a)
 
     def isource(self, name, p, n, dc):
         a = np.zeros(NPARAM)
@@ -127,12 +149,19 @@ class Deck:
     def bind(self, device: str, param, live: str):
         """params[device][param] = live column `live` for every point."""
         dev = self._by_name[device.lower()]
-        slot = param if isinstance(param, int) else (M_SLOTS[param] if self.kind[dev] == MOS1 else 0)
+        slot = param if isinstance(param, int) else self._slot(dev, param)
         self.binds.append((dev, slot, self.live_param(live)))
+
+    def _slot(self, dev, name):
+        if self.kind[dev] == MOS1:
+            return M_SLOTS[name]
+        if self.kind[dev] == D:
+            return D_SLOTS[name]
+        return 0
 
     def param(self, device: str, name) -> float:
         dev = self._by_name[device.lower()]
-        slot = name if isinstance(name, int) else (M_SLOTS[name] if self.kind[dev] == MOS1 else 0)
+        slot = name if isinstance(name, int) else self._slot(dev, name)
         return float(self.params[dev][slot])
 
     # -- flat arrays --------------------------------------------------------

@@ -62,7 +62,7 @@ enum {
     ESL_DEV_C = 1,     /* models/capacitor.va nodes p n          */
     ESL_DEV_V = 2,     /* models/vsource.va   nodes p n (DC)     */
     ESL_DEV_I = 3,     /* models/isource.va   nodes p n (DC)     */
-    ESL_DEV_D = 4,     /* models/diode.va     nodes a k          */
+    ESL_DEV_D = 4,     /* models/diode.va     nodes a k (RS = 0) */
     ESL_DEV_MOS1 = 5,  /* models/mos1.va      nodes d g s b      */
     ESL_DEV_KINDS = 6
 };
@@ -85,7 +85,11 @@ enum {
     /* Diode level 1 (models/diode.va): */
     ESL_D_IS = 0, ESL_D_N = 1, ESL_D_CJO = 2, ESL_D_VJ = 3, ESL_D_M = 4,
     ESL_D_TT = 5, ESL_D_FC = 6, ESL_D_AREA = 7, ESL_D_GMIN = 8,
-    ESL_D_TEMP = 9, ESL_D_TNOM = 10
+    ESL_D_TEMP = 9, ESL_D_TNOM = 10, ESL_D_EG = 11, ESL_D_XTI = 12,
+    /* V source waveform (transient): slot 0 is the value at t <= td (and
+     * the operating point's); ESL_V_WAVE 1 = step to ESL_V_V2 after
+     * ESL_V_TD seconds. */
+    ESL_V_WAVE = 1, ESL_V_V2 = 2, ESL_V_TD = 3
 };
 /* Bits of params[ESL_M_GIVEN] (stored as a double holding a small integer):
  * the reference's $param_given tests (models/mos1.va:177-217). */

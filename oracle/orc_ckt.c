@@ -202,6 +202,127 @@ static double limvds(double vnew, double vold)
 }
 
 /* jcur (mos1.va:99-110) and its derivative. */
+/* ---- junction diode (diode.va, the RS = 0 subset: no breakdown,
+ * recombination, knee currents or sidewall; TLEV 0 / TLEVC 0) ---- */
+typedef struct {
+    double is, vte, vcrit, gmin, cjo, vj, mt, qexp, vfc, f1, inv_f2, f3, tt;
+} dio_prep;
+
+static double pw(double a, double b) { return orc_exp(b * orc_log(a)); }
+
+static void dio1_prep(const double *p, dio_prep *o)
+{
+    const double n = p[1], area = p[7], mt = p[4], fc = p[6];
+    const double tk = p[9] + 273.15, tnomk = p[10] + 273.15;
+    const double ratio = tk / tnomk;
+    const double vt = KB / QE * tk, vtnom = KB / QE * tnomk;
+    const double vte = n * vt;
+    const double egfet = 1.16 - (7.02e-4 * tk * tk) / (tk + 1108.0);
+    const double egfet1 = 1.16 - (7.02e-4 * tnomk * tnomk) / (tnomk + 1108.0);
+    const double fact1 = tnomk / 300.15, fact2 = tk / 300.15;
+    const double arg1 = -egfet1 / (2.0 * KB * tnomk) + 1.1150877 / (2.0 * KB * 300.15);
+    const double argt = -egfet / (2.0 * KB * tk) + 1.1150877 / (2.0 * KB * 300.15);
+    const double pbfact1 = -2.0 * vtnom * (1.5 * orc_log(fact1) + QE * arg1);
+    const double pbfact = -2.0 * vt * (1.5 * orc_log(fact2) + QE * argt);
+    const double vj = p[3];
+    const double pbo = (vj - pbfact1) / fact1;
+    const double gmaold = (vj - pbo) / pbo;
+    const double vj_t = pbfact + fact2 * pbo;
+    const double gmanew = (vj_t - pbo) / pbo;
+    const double cjo_t = p[2] * (1.0 + mt * (400e-6 * (tk - 300.15) - gmanew)) /
+                         (1.0 + mt * (400e-6 * (tnomk - 300.15) - gmaold)) * area;
+    const double xti = p[12], eg = p[11];
+    const double scale = area * orc_exp((ratio - 1.0) * eg / vte + (xti / n) * orc_log(ratio));
+    o->is = p[0] * scale;
+    o->vte = vte;
+    o->vcrit = vte * orc_log(vte / (1.4142135623730951 * o->is));
+    o->gmin = p[8];
+    o->cjo = cjo_t;
+    o->vj = vj_t;
+    o->mt = mt;
+    o->qexp = 1.0 - mt;
+    o->vfc = fc * vj_t;
+    o->f1 = vj_t * (1.0 - pw(1.0 - fc, 1.0 - mt)) / (1.0 - mt);
+    o->inv_f2 = 1.0 / pw(1.0 - fc, 1.0 + mt);
+    o->f3 = 1.0 - fc * (1.0 + mt);
+    o->tt = p[5];
+}
+
+/* I = id + gmin vd with the 80 vte tangent and the cubic reverse tail */
+static double dio1_current(const dio_prep *d, double vd, double *g)
+{
+    double id, gd;
+    if (vd >= -3.0 * d->vte) {
+        const double u = vd / d->vte;
+        const double ul = fmin(u, 80.0);
+        const double e = orc_exp(ul);
+        id = d->is * (e * (1.0 + (u - ul)) - 1.0);
+        gd = d->is * e / d->vte;
+    } else {
+        const double w = 3.0 * d->vte / (vd * 2.718281828459045);
+        const double w3 = w * w * w;
+        id = -d->is * (1.0 + w3);
+        gd = 3.0 * d->is * w3 / vd;
+    }
+    *g = gd + d->gmin;
+    return id + d->gmin * vd;
+}
+
+/* depletion charge (two-branch form) + tt I; *c = dq/dvd */
+static double dio1_charge(const dio_prep *d, double vd, double i, double g, double *c)
+{
+    double q = 0.0, cap = 0.0;
+    if (d->cjo > 0) {
+        if (vd < d->vfc) {
+            const double a = 1.0 - vd / d->vj;
+            const double pp = orc_exp(d->qexp * orc_log(a));
+            q = d->cjo * d->vj * (1.0 - pp) / d->qexp;
+            cap = d->cjo * pp / a;
+        } else {
+            q = d->cjo * (d->f1 + d->inv_f2 * (d->f3 * (vd - d->vfc) + (d->mt / (2.0 * d->vj)) * (vd * vd - d->vfc * d->vfc)));
+            cap = d->cjo * d->inv_f2 * (d->f3 + (d->mt / d->vj) * vd);
+        }
+    }
+    if (d->tt > 0) {
+        q = q + d->tt * i;
+        cap = cap + d->tt * g;
+    }
+    *c = cap;
+    return q;
+}
+
+static int dio1_limit(const dio_prep *d, const double *x, const uint32_t *t, double *lvd)
+{
+    const double xa = x[t[0]], xk = x[t[1]];
+    const double vd = xa - xk;
+    double v = pnjlim(vd, *lvd, d->vte, d->vcrit);
+    const double tol = 16.0 * 2.220446049250313e-16 * fmax(fabs(xa), fabs(xk));
+    if (fabs(v - vd) <= tol) v = vd;
+    *lvd = v;
+    return v != vd;
+}
+
+/* out[0..3] Jacobian (a,a) (a,k) (k,a) (k,k), out[4..5] rows a, k; with
+ * charges out[6..9] the charge Jacobian, out[10..11] the row charges */
+static void dio1_eval(const dio_prep *d, double xa, double xk, double vl, int charges, double *out)
+{
+    double g;
+    double i = dio1_current(d, vl, &g);
+    const double dv = (xa - xk) - vl;
+    double q = 0.0, c = 0.0;
+    if (charges) q = dio1_charge(d, vl, i, g, &c);
+    if (dv != 0) {
+        i = i + g * dv;
+        if (charges) q = q + c * dv;
+    }
+    out[0] = g; out[1] = -g; out[2] = -g; out[3] = g;
+    out[4] = i; out[5] = -i;
+    if (charges) {
+        out[6] = c; out[7] = -c; out[8] = -c; out[9] = c;
+        out[10] = q; out[11] = -q;
+    }
+}
+
 static void jcur(double v, double isat, double vtx, double gm, double *i, double *g)
 {
     if (v > -3.0 * vtx) {
@@ -362,7 +483,6 @@ orc_ckt *orc_ckt_new(const orc_deck *deck, char *err, size_t err_cap)
         if (deck->kind[i] >= NKIND) { set_err(err, err_cap, "unknown device kind"); return NULL; }
         for (int t = 0; t < 4; t++)
             if (deck->nodes[i * 4 + t] >= deck->n_nodes) { set_err(err, err_cap, "node out of range"); return NULL; }
-        if (deck->kind[i] == K_D) { set_err(err, err_cap, "diode not supported by the operating-point oracle"); return NULL; }
     }
     for (uint32_t b = 0; b < deck->n_bind; b++)
         if (deck->bind_dev[b] >= deck->n_dev || deck->bind_param[b] >= NPARAM || deck->bind_live[b] >= deck->n_live) {
@@ -407,7 +527,7 @@ orc_ckt *orc_ckt_new(const orc_deck *deck, char *err, size_t err_cap)
     for (uint32_t i = 0; i < n; i++) ent[cnt++] = (rc_t){i, i};
     for (uint32_t i = 0; i < ni; i++) {
         orc_inst *in = &c->inst[i];
-        if (in->kind == K_R) {
+        if (in->kind == K_R || in->kind == K_C || in->kind == K_D) {
             for (int e = 0; e < 4; e++) ent[cnt++] = (rc_t){in->t[R_ENT[e][0]], in->t[R_ENT[e][1]]};
         } else if (in->kind == K_V) {
             ent[cnt++] = (rc_t){in->t[0], in->br};
@@ -452,7 +572,7 @@ orc_ckt *orc_ckt_new(const orc_deck *deck, char *err, size_t err_cap)
     }
     for (uint32_t i = 0; i < ni; i++) {
         orc_inst *in = &c->inst[i];
-        if (in->kind == K_R) {
+        if (in->kind == K_R || in->kind == K_C || in->kind == K_D) {
             in->nslot = 4;
             for (int e = 0; e < 4; e++) in->slot[e] = find_slot(c, in->t[R_ENT[e][0]], in->t[R_ENT[e][1]]);
         } else if (in->kind == K_V) {
@@ -493,7 +613,8 @@ void orc_ckt_pattern(const orc_ckt *c, uint32_t *col_ptr, uint32_t *row_idx,
 typedef struct {
     double *p;          /* [n_dev][NPARAM] with bindings applied */
     mos_prep *mp;       /* per instance (MOS only used) */
-    double *lim;        /* [n_inst][4] */
+    dio_prep *dp;       /* per instance (diodes only used) */
+    double *lim;        /* [n_inst][4]: MOS (vgs, vds, vbs, von), diode vd */
 } lane_dev;
 
 static void lane_dev_init(const orc_ckt *c, const double *live, lane_dev *ld)
@@ -505,16 +626,20 @@ static void lane_dev_init(const orc_ckt *c, const double *live, lane_dev *ld)
         ld->p[(size_t)c->bind_dev[b] * NPARAM + c->bind_param[b]] = live[c->bind_live[b]];
     ld->mp = calloc(c->n_inst + 1, sizeof(mos_prep));
     ld->lim = calloc((size_t)c->n_inst * 4 + 4, 8);
-    for (uint32_t i = 0; i < c->n_inst; i++)
+    ld->dp = calloc(c->n_inst + 1, sizeof(dio_prep));
+    for (uint32_t i = 0; i < c->n_inst; i++) {
         if (c->inst[i].kind == K_MOS) mos1_prep(ld->p + (size_t)c->inst[i].dev * NPARAM, &ld->mp[i]);
+        if (c->inst[i].kind == K_D) dio1_prep(ld->p + (size_t)c->inst[i].dev * NPARAM, &ld->dp[i]);
+    }
 }
 
-static void lane_dev_free(lane_dev *ld) { free(ld->p); free(ld->mp); free(ld->lim); }
+static void lane_dev_free(lane_dev *ld) { free(ld->p); free(ld->mp); free(ld->dp); free(ld->lim); }
 
 /* cold start: MODEINITJCT seeds into the limiting state */
 static void seed_cold(const orc_ckt *c, lane_dev *ld)
 {
     for (uint32_t i = 0; i < c->n_inst; i++) {
+        if (c->inst[i].kind == K_D) ld->lim[(size_t)i * 4] = ld->dp[i].vcrit;
         if (c->inst[i].kind != K_MOS) continue;
         double *lim = ld->lim + (size_t)i * 4;
         lim[0] = ld->mp[i].seed_vgs;
@@ -529,6 +654,7 @@ static void seed_at(const orc_ckt *c, lane_dev *ld, const double *x)
 {
     for (uint32_t i = 0; i < c->n_inst; i++) {
         const orc_inst *in = &c->inst[i];
+        if (in->kind == K_D) ld->lim[(size_t)i * 4] = x[in->t[0]] - x[in->t[1]];
         if (in->kind != K_MOS) continue;
         double *lim = ld->lim + (size_t)i * 4;
         const double tp = ld->mp[i].type;
@@ -579,6 +705,14 @@ static void assemble(const orc_ckt *c, lane_dev *ld, const double *x, double *g,
             rhs[ROW(in->t[0])] += p[0];
             rhs[ROW(in->t[1])] += -p[0];
             break;
+        case K_D: {
+            double o[12];
+            dio1_eval(&ld->dp[i], x[in->t[0]], x[in->t[1]], ld->lim[(size_t)i * 4], 0, o);
+            for (int e = 0; e < 4; e++) g[SLOT(in->slot[e])] += o[e];
+            rhs[ROW(in->t[0])] += o[4];
+            rhs[ROW(in->t[1])] += o[5];
+            break;
+        }
         case K_MOS: {
             double f3[3], j11[11];
             mos1_eval(&ld->mp[i], x, in->t, ld->lim + (size_t)i * 4, f3, j11);
@@ -601,8 +735,10 @@ static void assemble(const orc_ckt *c, lane_dev *ld, const double *x, double *g,
 static int apply_limits(const orc_ckt *c, lane_dev *ld, const double *x)
 {
     int any = 0;
-    for (uint32_t i = 0; i < c->n_inst; i++)
+    for (uint32_t i = 0; i < c->n_inst; i++) {
         if (c->inst[i].kind == K_MOS) any |= mos1_limit(&ld->mp[i], x, c->inst[i].t, ld->lim + (size_t)i * 4);
+        if (c->inst[i].kind == K_D) any |= dio1_limit(&ld->dp[i], x, c->inst[i].t, ld->lim + (size_t)i * 4);
+    }
     return any;
 }
 
