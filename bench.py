@@ -4,7 +4,9 @@ Workload: Monte Carlo operating points of the 64-OTA MOS1 array (324 MNA
 unknowns, 32 live parameters per point), 16,384 points per step; one step =
 one esl_op_solve of the whole batch, timed from the values on the host to
 the results on the host (upload, network inference where used, lead factor,
-the Newton kernel, re-pivot and fallback rounds, download). Modes:
+the Newton kernel, re-pivot and fallback rounds, download). A point's
+results are its measures, the 64 OTA output voltages, as the points API
+returns per-point measures (include/espice.h:338). Modes:
   fp64_cold      plain FP64 lanes, cold start (INITFIX + junction seeds): the
                  baseline of the headline ratio
   fp64_nominal   plain FP64 lanes warm-started from the template's nominal
@@ -79,6 +81,23 @@ def calibrate_copy(mib):
     print(json.dumps({"copy_bytes_read": mib << 20, "copy_bytes_written": mib << 20}))
 
 
+def measured_traffic(mode, points):
+    """HBM bytes per op_newton launch from the committed rocprofv3 counter
+    summary for this workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or
+    None when no summary covers it."""
+    if mode != "learned_fp32" or points != 16384:
+        return None
+    path = os.path.join(ROOT, "profiles", "op_newton_ota_array_64_learned_fp32_hbm_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
 def train_network(lanes, fam, eng, x_nom, rows, epochs):
     """The warm-start network, trained on operating points the engine solves."""
     t0 = time.perf_counter()
@@ -129,8 +148,14 @@ def main():
         net, training = train_network(lanes, fam, eng, x_nom, args.train_points, args.train_epochs)
         eng.set_warm_start(net, x_nom)
 
-    def run(mode):
+    # measures per point: the 64 OTA outputs (the points API returns
+    # per-point measures, include/espice.h:338, not whole state vectors)
+    probes = np.array([deck.node_index(f"out{k}") for k in range(64)], dtype=np.uint32)
+
+    def run(mode, measures=True):
         kw = {"tiling": args.tiling}
+        if measures:
+            kw["probes"] = probes
         if mode == "fp64_nominal":
             kw["x0"] = np.repeat(x_nom[None, :], n_pts, 0)
         if mode.startswith("learned"):
@@ -146,6 +171,7 @@ def main():
         for _ in range(args.warmup):
             run(mode)
         t_kernel, t_total, t_warm, conv, iters, lane_it, fb64, fbw = 0.0, 0.0, 0.0, 0, 0.0, 0, 0, 0
+        t_lead, t_dl = 0.0, 0.0
         t0 = time.perf_counter()
         for _ in range(args.steps):
             r = run(mode)
@@ -153,6 +179,8 @@ def main():
             t_kernel += tm.kernel_ms
             t_total += tm.total_ms
             t_warm += tm.warm_ms if mode.startswith("learned") else 0.0
+            t_lead += tm.lead_ms
+            t_dl += tm.download_ms
             conv += int((r["reasons"] == 0).sum())
             iters += float(r["iterations"].mean())
             lane_it += int(tm.lane_iterations)
@@ -168,6 +196,8 @@ def main():
             "kernel_ms_per_step": t_kernel / args.steps,
             "library_ms_per_step": t_total / args.steps,
             "inference_ms_per_step": t_warm / args.steps,
+            "lead_tape_ms_per_step": t_lead / args.steps,
+            "download_ms_per_step": t_dl / args.steps,
             "converged_fraction": conv / (n_pts * args.steps),
             "newton_iterations_per_point": iters / args.steps,
             "fp64_fallback_rate": fb64 / (n_pts * args.steps),
@@ -176,7 +206,7 @@ def main():
             "flops_per_lane_iteration": flops_iter,
             "repivots": int(tm.repivots),
             "tiling": int(tm.tiling),
-            "last": r,
+            "last": run(mode, measures=False),
         }
     head = HEADLINE if HEADLINE in results else max(results, key=lambda m: results[m]["points_per_s"])
     res = results[head]
@@ -269,7 +299,7 @@ def main():
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(res["tflops"] / PEAK_FP64_TFLOPS, 5),
-            "traffic": None,
+            "traffic": measured_traffic(head, n_pts),
         },
         "cpu_baseline": cpu,
         "training": training,

@@ -109,6 +109,16 @@ inline constexpr uint32_t row_flags(uint32_t y) { return (y >> 20) & 0x3u; }
 constexpr uint32_t kRedTarget = 0x80000000u, kRedRhs = 0x40000000u, kRedNop = 0x3fffffffu;
 constexpr int kRing = 16;      // rows in flight; streams carry kRing pad rows
 
+// Doubles of a wavefront lane's LDS: tv_cap factor slots, the Y block (y,
+// sdiag, xold after the assembly; the stamp staging area during it), rhs
+// and x.
+__host__ __device__ inline uint32_t lds_y_block(uint32_t n)
+{
+    const uint32_t stage = kWave * kStageEntries;
+    return 3 * n > stage ? 3 * n : stage;
+}
+__host__ __device__ inline uint32_t lds_doubles(uint32_t tv_cap, uint32_t n) { return tv_cap + lds_y_block(n) + 2 * n; }
+
 struct TapeProg {
     uint32_t n_tv, n_u, n_l, lx0;   // tv = [U | diag | L | discard | void extras | 64 pads]
     uint32_t discard, n_levels, n_flev, n_blev;

@@ -32,6 +32,8 @@ extern "C" hipError_t esl_launch_tape_replay(const void *r, uint32_t lanes, uint
 extern "C" hipError_t esl_kernel_attrs(uint32_t lds_bytes);
 extern "C" hipError_t esl_launch_op_newton_lpt(const void *a, hipStream_t s);
 extern "C" hipError_t esl_launch_transpose_live(const double *in, double *out, uint32_t N, uint32_t P, hipStream_t s);
+extern "C" hipError_t esl_launch_gather(const double *x, uint32_t n, const uint32_t *ids, uint32_t k, uint32_t count,
+                                        double *out, hipStream_t s);
 
 using std::vector;
 using u32 = uint32_t;
@@ -1103,6 +1105,8 @@ struct Engine {
     // per-solve buffers
     DevBuf live, x, lim, resume, status, iters, lane_ids, a_out, f_out, a64, lane64, fell_back, x_nom;
     esl_mlp *warm_net = nullptr;   // handle of the attached warm-start network
+    u32 lead_id = 0;               // tape of the last solve's lead point
+    DevBuf probe_ids, probe_out;
     bool has_nominal = false;
     // lane-per-thread state, lane-major
     DevBuf live_t, tvg, rhsg, xg, xoldg, yg, sdiagg, limg;
@@ -1198,7 +1202,7 @@ hipError_t upload_deck(Engine &e)
 
 u32 lds_bytes_for(u32 tv_cap, u32 n)
 {
-    return (u32)((size_t)(tv_cap + 5 * n + esl::kWave * esl::kStageEntries) * sizeof(double));
+    return (u32)((size_t)esl::lds_doubles(tv_cap, n) * sizeof(double));
 }
 
 
@@ -1524,7 +1528,7 @@ esl_status esl_engine_info_get(esl_engine *h, esl_engine_info *out)
     out->tapes = (u32)e->tapes.size();
     out->tiling = e->tuned;
     if (!e->tapes.empty()) {
-        const TapeHostProg &tp = *e->tapes[0];
+        const TapeHostProg &tp = *e->tapes[std::min<size_t>(e->lead_id, e->tapes.size() - 1)];
         out->refactor_steps = tp.rch.used;
         out->solve_steps = tp.sch.used;
         out->levels = tp.prog.n_rlev;
@@ -1574,7 +1578,7 @@ namespace {
 esl_status launch_lpt(Engine &e, const HostOpArgs &a, u32 lanes, u32 mode, float &ms)
 {
     const Compiled &c = e.c;
-    const TapeHostProg &tp = *e.tapes[0];
+    const TapeHostProg &tp = *e.tapes[e.lead_id];
     const size_t N = lanes;
     HIPCHK(e.tvg.ensure(N * tp.prog.n_tv * 8));
     HIPCHK(e.rhsg.ensure(N * c.n * 8));
@@ -1618,13 +1622,11 @@ esl_status launch_lpt(Engine &e, const HostOpArgs &a, u32 lanes, u32 mode, float
 }
 
 // One wavefront-per-lane launch over `lanes` lanes listed in e.lane_ids.
-esl_status launch_lpw(Engine &e, HostOpArgs a, u32 lanes, u32 mode, float &ms)
+esl_status launch_lpw(Engine &e, HostOpArgs a, u32 lanes, u32 mode, u32 tv_cap, float &ms)
 {
     a.tapes = e.tape_table.as<esl::TapeProg>();
     a.mode = mode;
     a.dump = 0;
-    u32 tv_cap = 0;
-    for (auto *t : e.tapes) tv_cap = std::max(tv_cap, t->prog.n_tv);
     a.tv_cap = tv_cap;
     const u32 lds = lds_bytes_for(tv_cap, e.c.n);
     HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
@@ -1667,6 +1669,9 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
     if (o.precision == ESL_FP32_REFINED && o.tiling == ESL_TILING_THREAD) return ESL_INVALID_ARGUMENT;
     if (o.tol.max_iter == 0) return ESL_INVALID_ARGUMENT;
     if (o.tiling > ESL_TILING_THREAD) return ESL_INVALID_ARGUMENT;
+    if (o.n_probes && !o.probes) return ESL_INVALID_ARGUMENT;
+    for (u32 k = 0; k < o.n_probes; k++)
+        if (o.probes[k] >= e->c.n) return ESL_INVALID_ARGUMENT;
     for (size_t i = 0; i < count * e->c.n_live; i++)
         if (!std::isfinite(values[i])) return ESL_INVALID_ARGUMENT;
     const auto t_start = std::chrono::steady_clock::now();
@@ -1735,7 +1740,10 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
     }
 
     // Lead tape: point 0's first Jacobian, factored on the host.
-    clear_tapes(*e);
+    // Tapes stay cached across solves (deduplicated by pivot sequence), so a
+    // batch whose lead point pivots like an earlier one reuses its program.
+    if (e->tapes.size() > 128) clear_tapes(*e);
+    const auto t_lead0 = std::chrono::steady_clock::now();
     {
         HIPCHK(e->a_out.ensure(std::max<u32>(nnz, 1) * 8));
         HIPCHK(e->f_out.ensure(std::max<u32>(n, 1) * 8));
@@ -1758,20 +1766,24 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
                 reasons[l] = ESL_REASON_SINGULAR_MATRIX;
                 iterations[l] = 0;
             }
-            memset(x, 0, count * n * 8);
+            memset(x, 0, count * (o.n_probes ? o.n_probes : n) * 8);
             return ESL_OK;
         }
         u32 id;
         esl_status st = add_tape(*e, t, id);
         if (st != ESL_OK) return st;
+        e->lead_id = id;
         e->lead_nl = t.nnz_l;
         e->lead_nu = t.nnz_u;
-        e->lead_flops = e->tapes[0]->flops;
-        e->lead_levels = e->tapes[0]->prog.n_levels;
-        e->lead_lds = lds_bytes_for(e->tapes[0]->prog.n_tv, n);
+        e->lead_flops = e->tapes[id]->flops;
+        e->lead_levels = e->tapes[id]->prog.n_levels;
+        e->lead_lds = lds_bytes_for(e->tapes[id]->prog.n_tv, n);
     }
+    e->timing.lead_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_lead0).count();
+    const u32 lead_cap = e->tapes[e->lead_id]->prog.n_tv;
     vector<esl::LaneResume> rs(count);
-    for (auto &r : rs) r = esl::LaneResume{0, 0, 0, 0, 1e9, 0, 0, 0.0};
+    for (auto &r : rs) r = esl::LaneResume{0, 0, e->lead_id, 0, 1e9, 0, 0, 0.0};
     HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
     const u32 mode0 = o.start == ESL_START_COLD ? 0u : o.start == ESL_START_GIVEN ? 1u : 3u;
 
@@ -1788,7 +1800,7 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
             tiling = e->tuned;
         } else if (count >= 1024) {
             float ms_w = 0, ms_t = 0;
-            if (launch_lpw(*e, a, (u32)count, mode0, ms_w) != ESL_OK) return ESL_DEVICE_UNAVAILABLE;
+            if (launch_lpw(*e, a, (u32)count, mode0, lead_cap, ms_w) != ESL_OK) return ESL_DEVICE_UNAVAILABLE;
             HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
             if (mode0 == 1) HIPCHK(hipMemcpyAsync(e->x.p, o.x0, count * n * 8, hipMemcpyHostToDevice, s));
             if (launch_lpt(*e, a, (u32)count, mode0, ms_t) != ESL_OK) return ESL_DEVICE_UNAVAILABLE;
@@ -1808,7 +1820,7 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
         esl_status r = launch_lpt(*e, a, (u32)count, mode0, ms);
         if (r != ESL_OK) return r;
     } else {
-        esl_status r = launch_lpw(*e, a, (u32)count, mode0, ms);
+        esl_status r = launch_lpw(*e, a, (u32)count, mode0, lead_cap, ms);
         if (r != ESL_OK) return r;
     }
     kernel_ms += ms;
@@ -1874,14 +1886,26 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
         }
         if (next.empty()) break;
         HIPCHK(hipMemcpyAsync(e->lane_ids.p, next.data(), next.size() * 4, hipMemcpyHostToDevice, s));
-        esl_status r = launch_lpw(*e, a, (u32)next.size(), 2, ms);
+        u32 cap = 0;
+        for (u32 l : next) cap = std::max(cap, e->tapes[rs[l].tape]->prog.n_tv);
+        esl_status r = launch_lpw(*e, a, (u32)next.size(), 2, cap, ms);
         if (r != ESL_OK) return r;
         kernel_ms += ms;
         e->timing.launches++;
         HIPCHK(hipMemcpyAsync(st.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
     }
-    HIPCHK(hipMemcpyAsync(x, e->x.p, count * n * 8, hipMemcpyDeviceToHost, s));
+    const auto t_dl0 = std::chrono::steady_clock::now();
+    if (o.n_probes) {
+        HIPCHK(e->probe_ids.ensure(o.n_probes * 4));
+        HIPCHK(e->probe_out.ensure(count * o.n_probes * 8));
+        HIPCHK(hipMemcpyAsync(e->probe_ids.p, o.probes, o.n_probes * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(esl_launch_gather(e->x.as<double>(), n, e->probe_ids.as<u32>(), o.n_probes, (u32)count,
+                                 e->probe_out.as<double>(), s));
+        HIPCHK(hipMemcpyAsync(x, e->probe_out.p, count * o.n_probes * 8, hipMemcpyDeviceToHost, s));
+    } else {
+        HIPCHK(hipMemcpyAsync(x, e->x.p, count * n * 8, hipMemcpyDeviceToHost, s));
+    }
     vector<u32> dev_its(count);
     HIPCHK(hipMemcpyAsync(dev_its.data(), e->iters.p, count * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(st.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
@@ -1906,6 +1930,8 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
         e->prof_host.assign(count * esl::kProfPhases, 0);
         HIPCHK(hipMemcpy(e->prof_host.data(), e->prof.p, count * esl::kProfPhases * 8, hipMemcpyDeviceToHost));
     }
+    e->timing.download_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_dl0).count();
     e->timing.kernel_ms = kernel_ms;
     e->timing.lane_iterations = lane_it;
     e->timing.total_ms =

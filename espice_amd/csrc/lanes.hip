@@ -431,6 +431,28 @@ __device__ inline void lane_init(const OpArgs &a, uint32_t lane, const double *l
     __syncthreads();
 }
 
+// A lane's Jacobian (pattern order) and residual at its start or saved
+// iterate: the lead point's matrix for the host factor, or a failing lane's
+// for its re-pivot.
+__global__ __launch_bounds__(kWave) void op_dump(OpArgs a)
+{
+    extern __shared__ double sm[];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = a.lane_ids[blockIdx.x];
+    const DeckProg &dp = a.dp;
+    const uint32_t n = dp.n, nm = dp.n_mos;
+    double *tv = sm;
+    double *st = tv + a.tv_cap;
+    double *rhs = tv + a.tv_cap + lds_y_block(n);
+    double *x = rhs + n;
+    const double *live = a.live + (size_t)lane * dp.n_live;
+    double *lim = a.lim + (size_t)lane * (4 * nm + dp.n_dio);
+    lane_init(a, lane, live, x, lim, tid);
+    assemble_pattern(dp, live, x, lim, nm, tv, rhs, st, tid);
+    for (uint32_t p = tid; p < dp.nnz; p += kWave) a.a_out[(size_t)blockIdx.x * dp.nnz + p] = tv[p];
+    for (uint32_t i = tid; i < n; i += kWave) a.f_out[(size_t)blockIdx.x * n + i] = rhs[i];
+}
+
 __global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
 {
     extern __shared__ double sm[];
@@ -438,22 +460,20 @@ __global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
     const uint32_t lane = a.lane_ids[blockIdx.x];
     const DeckProg &dp = a.dp;
     const uint32_t n = dp.n, nm = dp.n_mos;
-    double *tv = sm;   // the tape's factor slots, then y (solve vector) at tv + n_tv
-    double *rhs = tv + a.tv_cap + n;
+    // LDS: [tv (tv_cap) | Y block | rhs | x]. The lane's factor slots start
+    // at tv and its solve vector follows at tv + n_tv (one index space for
+    // the row programs); the Y block holds y, sdiag and xold after the
+    // assembly and the stamp staging area st during it.
+    double *tv = sm;
+    double *sdiag = tv + a.tv_cap + n;
+    double *xold = sdiag + n;
+    double *st = tv + a.tv_cap;
+    double *rhs = tv + a.tv_cap + lds_y_block(n);
     double *x = rhs + n;
-    double *xold = x + n;
-    double *sdiag = xold + n;
-    double *st = sdiag + n;
     const double *live = a.live + (size_t)lane * dp.n_live;
     double *lim = a.lim + (size_t)lane * (4 * nm + dp.n_dio);
     lane_init(a, lane, live, x, lim, tid);
 
-    if (a.dump) {
-        assemble_pattern(dp, live, x, lim, nm, tv, rhs, st, tid);
-        for (uint32_t p = tid; p < dp.nnz; p += kWave) a.a_out[(size_t)blockIdx.x * dp.nnz + p] = tv[p];
-        for (uint32_t i = tid; i < n; i += kWave) a.f_out[(size_t)blockIdx.x * n + i] = rhs[i];
-        return;
-    }
 
     LaneResume rs = a.resume[lane];
     const TapeProg T = a.tapes[rs.tape];
@@ -675,12 +695,34 @@ __global__ __launch_bounds__(kWave) void tape_replay(ReplayArgs r)
     if (tid == 0 && r.fail) r.fail[lane] = bad ? 1u : 0u;
 }
 
+// out[p][k] = x[p][ids[k]]: a solve's measures.
+__global__ void gather_probes(const double *x, uint32_t n, const uint32_t *ids, uint32_t k, uint32_t count,
+                              double *out)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)count * k) return;
+    const size_t p = e / k, j = e % k;
+    out[e] = x[p * n + ids[j]];
+}
+
 }  // namespace esl
+
+extern "C" hipError_t esl_launch_gather(const double *x, uint32_t n, const uint32_t *ids, uint32_t k, uint32_t count,
+                                        double *out, hipStream_t s)
+{
+    const size_t total = (size_t)count * k;
+    hipLaunchKernelGGL(esl::gather_probes, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, s, x, n, ids, k, count,
+                       out);
+    return hipGetLastError();
+}
 
 extern "C" hipError_t esl_launch_op_newton(const esl::OpArgs *a, uint32_t grid, uint32_t lds_bytes,
                                            hipStream_t s)
 {
-    hipLaunchKernelGGL(esl::op_newton, dim3(grid), dim3(esl::kWave), lds_bytes, s, *a);
+    if (a->dump)
+        hipLaunchKernelGGL(esl::op_dump, dim3(grid), dim3(esl::kWave), lds_bytes, s, *a);
+    else
+        hipLaunchKernelGGL(esl::op_newton, dim3(grid), dim3(esl::kWave), lds_bytes, s, *a);
     return hipGetLastError();
 }
 
@@ -695,6 +737,8 @@ extern "C" hipError_t esl_kernel_attrs(uint32_t lds_bytes)
 {
     hipError_t e = hipFuncSetAttribute((const void *)esl::op_newton,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void *)esl::op_dump, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) return e;
     return hipFuncSetAttribute((const void *)esl::tape_replay, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds_bytes);

@@ -63,7 +63,8 @@ class Tolerances(C.Structure):
 class OpOptions(C.Structure):
     _fields_ = [("struct_size", C.c_size_t), ("tol", Tolerances), ("precision", C.c_uint32),
                 ("start", C.c_uint32), ("x0", C.c_void_p), ("refine_tol", C.c_double),
-                ("refine_max", C.c_uint32), ("tiling", C.c_uint32)]
+                ("refine_max", C.c_uint32), ("tiling", C.c_uint32), ("probes", C.c_void_p),
+                ("n_probes", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class EngineInfo(C.Structure):
@@ -78,7 +79,7 @@ class Timing(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("launches", C.c_uint32),
                 ("repivots", C.c_uint32), ("lane_iterations", C.c_uint64), ("tiling", C.c_uint32),
                 ("fp64_lanes", C.c_uint32), ("warm_fallbacks", C.c_uint32), ("reserved", C.c_uint32),
-                ("warm_ms", C.c_double)]
+                ("warm_ms", C.c_double), ("lead_ms", C.c_double), ("download_ms", C.c_double)]
 
 
 class MlpTrainOptions(C.Structure):
@@ -209,10 +210,11 @@ class Engine:
         return cp, ri[: info.nnz], q
 
     def solve_op(self, values, x0=None, precision=FP64, tol: Tolerances | None = None, tiling=TILING_AUTO,
-                 refine_tol=None, refine_max=None, learned=False):
+                 refine_tol=None, refine_max=None, learned=False, probes=None):
         """Operating points of rows `values` [count][n_live]. Returns dict
         with x [count][n], iterations, reasons. x0 [count][n]: given starts;
-        learned=True: starts from the attached network (set_warm_start)."""
+        learned=True: starts from the attached network (set_warm_start);
+        probes: unknown indices to return instead of all of x."""
         values = np.ascontiguousarray(np.atleast_2d(np.asarray(values, dtype=np.float64)))
         count = values.shape[0]
         start = START_LEARNED if learned else (START_COLD if x0 is None else START_GIVEN)
@@ -228,7 +230,11 @@ class Engine:
         if x0 is not None and not learned:
             xs = np.ascontiguousarray(x0, dtype=np.float64).reshape(count, self.n)
             o.x0 = xs.ctypes.data
-        x = np.zeros((count, self.n))
+        pr = None
+        if probes is not None:
+            pr = np.ascontiguousarray(probes, dtype=np.uint32)
+            o.probes, o.n_probes = pr.ctypes.data, len(pr)
+        x = np.zeros((count, self.n if pr is None else len(pr)))
         it = np.zeros(count, np.uint32)
         rs = np.zeros(count, np.uint32)
         _check(load().esl_op_solve(self.h, _ptr(values), count, C.byref(o), _ptr(x), _ptr(it), _ptr(rs)),

@@ -158,6 +158,12 @@ typedef struct {
     double refine_tol;
     uint32_t refine_max;
     uint32_t tiling;       /* ESL_TILING_* */
+    /* Measures: with n_probes > 0 the solve returns x [count][n_probes],
+     * x[p][k] = unknown probes[k] of point p (the role of the points API's
+     * per-point measures, include/espice.h:338), instead of [count][n]. */
+    const uint32_t *probes;
+    uint32_t n_probes;
+    uint32_t reserved;
 } esl_op_options;
 
 /* Sizes of a compiled engine. */
@@ -201,7 +207,7 @@ esl_status esl_engine_pattern(esl_engine *e, uint32_t *col_ptr,
 
 /* Solves the operating point of `count` points: values is
  * [count][n_live] row-major. Outputs: x [count][n] (the converged iterate,
- * or the last one), iterations [count] (Newton iterations, counted as
+ * or the last one; [count][n_probes] with probes), iterations [count] (Newton iterations, counted as
  * converger.Result.iterations), reasons [count] (ESL_REASON_*). Lanes run
  * on the GPU as one wavefront each. The first point's first Jacobian fixes
  * the pivot tape (gpu_lanes.zig:361-374); a lane whose replay fails the
@@ -223,6 +229,8 @@ typedef struct {
     uint32_t warm_fallbacks;  /* START_LEARNED: lanes restarted from the nominal point */
     uint32_t reserved;
     double warm_ms;        /* START_LEARNED: network inference, HIP events */
+    double lead_ms;        /* host: lead point's Jacobian, factor and tape program */
+    double download_ms;    /* host: results to the caller */
 } esl_timing;
 esl_status esl_last_timing(esl_engine *e, esl_timing *out);
 

@@ -217,3 +217,16 @@ def test_fp32_full_size_ota_error_against_fp64(lanes, oracle):
     assert np.array_equal(b["iterations"][pick], ref["iterations"])
     assert np.array_equal(bits(b["x"][pick]), bits(ref["x"]))
     eng.close()
+
+
+def test_probes_return_the_selected_unknowns(lanes):
+    fam = circuits.ota_array()
+    vals = fam.sample(12, 300)
+    eng = lanes.Engine(fam.deck.arrays())
+    probes = np.array([fam.deck.node_index(f"out{k}") for k in range(64)] + [0, eng.n - 1], np.uint32)
+    full = eng.solve_op(vals)
+    got = eng.solve_op(vals, probes=probes)
+    assert got["x"].shape == (300, len(probes))
+    assert np.array_equal(bits(got["x"]), bits(full["x"][:, probes]))
+    assert np.array_equal(got["iterations"], full["iterations"])
+    eng.close()
