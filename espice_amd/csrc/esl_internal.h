@@ -199,6 +199,23 @@ struct OpArgs {
 };
 constexpr int kProfPhases = 4;
 
+// Transient lanes (tran_steps): fixed-step integration, one wavefront per
+// lane, state between launches in HBM. Step s (0-based) ends at
+// t = (s + 1) dt; step 0 is backward Euler (ag0 = 1/dt, ag1 = 0), later
+// steps the trapezoid with xmu = 0.5 (ag0 = 2/dt, ag1 = 1).
+enum : uint32_t { kTranRunning = 102 };
+struct TranArgs {
+    OpArgs op;                  // deck, tapes, lane ids, live, x, lim, resume, status, iterations, tolerances
+    double *q1, *ip;            // [N][n] charges of the last accepted step, dynamic currents
+    uint32_t *step;             // [N] next step to run
+    const uint32_t *probe_ids;  // [n_probes]
+    double *probes;             // [N][steps][n_probes]
+    double dt;
+    uint32_t steps, n_probes, s0, s1;   // this launch runs steps [s0, s1)
+    uint32_t init;              // 1: the first launch (q1 from the operating point, ip = 0)
+    uint32_t dump;              // 1: only write the first step's first Jacobian to op.a_out
+};
+
 // Arguments of the lane-per-thread Newton kernel (op_newton_lpt): thread t
 // solves lane lane_ids[t]; its state is lane-major, element i at i * N + t.
 struct LptArgs {

@@ -32,6 +32,7 @@ extern "C" hipError_t esl_launch_tape_replay(const void *r, uint32_t lanes, uint
 extern "C" hipError_t esl_kernel_attrs(uint32_t lds_bytes);
 extern "C" hipError_t esl_launch_op_newton_lpt(const void *a, hipStream_t s);
 extern "C" hipError_t esl_launch_transpose_live(const double *in, double *out, uint32_t N, uint32_t P, hipStream_t s);
+extern "C" hipError_t esl_launch_tran_steps(const void *ta, uint32_t grid, uint32_t lds_bytes, hipStream_t s);
 extern "C" hipError_t esl_launch_gather(const double *x, uint32_t n, const uint32_t *ids, uint32_t k, uint32_t count,
                                         double *out, hipStream_t s);
 
@@ -1107,6 +1108,7 @@ struct Engine {
     esl_mlp *warm_net = nullptr;   // handle of the attached warm-start network
     u32 lead_id = 0;               // tape of the last solve's lead point
     DevBuf probe_ids, probe_out;
+    DevBuf tq1, tip, tstep, tprobes;   // transient state
     bool has_nominal = false;
     // lane-per-thread state, lane-major
     DevBuf live_t, tvg, rhsg, xg, xoldg, yg, sdiagg, limg;
@@ -1933,6 +1935,276 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
     e->timing.download_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_dl0).count();
     e->timing.kernel_ms = kernel_ms;
+    e->timing.lane_iterations = lane_it;
+    e->timing.total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    return ESL_OK;
+}
+
+extern "C" void esl_default_tran_options(esl_tran_options *o)
+{
+    if (!o) return;
+    memset(o, 0, sizeof(*o));
+    o->struct_size = sizeof(*o);
+    esl_default_tolerances(&o->tol);
+    o->tol.max_iter = 50;
+    o->dt = 1e-12;
+    o->steps = 1;
+    o->steps_per_launch = 10;
+    o->use_graph = 1;
+}
+
+namespace {
+u32 tran_lds_bytes(const Compiled &c, u32 tv_cap)
+{
+    return (u32)(((size_t)esl::lds_doubles(tv_cap, c.n) + 4 * (size_t)c.n + c.cslot_pat.size()) * sizeof(double));
+}
+}  // namespace
+
+extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t count, const esl_tran_options *opt,
+                                     double *x, double *probe_out, uint32_t *iterations, uint32_t *reasons)
+{
+    auto *e = static_cast<Engine *>(lookup(h, 1));
+    if (!e) return ESL_INVALID_HANDLE;
+    if (!opt || opt->struct_size < sizeof(esl_tran_options)) return ESL_ABI_MISMATCH;
+    const esl_tran_options o = *opt;
+    if (count == 0 || count > 0x7fffffff || !x || !iterations || !reasons) return ESL_INVALID_ARGUMENT;
+    if (!(o.dt > 0) || !std::isfinite(o.dt) || o.steps == 0 || o.steps_per_launch == 0 || o.tol.max_iter == 0)
+        return ESL_INVALID_ARGUMENT;
+    if (o.n_probes && (!o.probes || !probe_out)) return ESL_INVALID_ARGUMENT;
+    const Compiled &c = e->c;
+    const u32 n = c.n, nnz = c.nnz;
+    for (u32 k = 0; k < o.n_probes; k++)
+        if (o.probes[k] >= n) return ESL_INVALID_ARGUMENT;
+    const auto t_start = std::chrono::steady_clock::now();
+    // 1. operating points (cold start, FP64, wavefront tiling): x stays on the device
+    esl_op_options oo;
+    esl_default_op_options(&oo);
+    oo.tol = o.tol;
+    oo.tol.max_iter = std::max<u32>(o.tol.max_iter, 100);
+    oo.tiling = ESL_TILING_WAVE;
+    vector<u32> op_it(count);
+    esl_status st0 = esl_op_solve(h, values, count, &oo, x, op_it.data(), reasons);
+    if (st0 != ESL_OK) return st0;
+    float op_ms = (float)e->timing.kernel_ms;
+    hipStream_t s = e->stream;
+    size_t first_ok = count;
+    for (size_t l = 0; l < count; l++)
+        if (reasons[l] == ESL_REASON_OK) { first_ok = l; break; }
+    for (size_t l = 0; l < count; l++) iterations[l] = 0;
+    if (first_ok == count) return ESL_OK;   // no operating point converged
+
+    // 2. transient state
+    HIPCHK(e->tq1.ensure(count * n * 8));
+    HIPCHK(e->tip.ensure(count * n * 8));
+    HIPCHK(e->tstep.ensure(count * 4));
+    HIPCHK(e->tprobes.ensure(std::max<size_t>(1, count * (size_t)o.steps * o.n_probes) * 8));
+    HIPCHK(e->probe_ids.ensure(std::max<u32>(1, o.n_probes) * 4));
+    if (o.n_probes) HIPCHK(hipMemcpyAsync(e->probe_ids.p, o.probes, o.n_probes * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(e->a_out.ensure(std::max<size_t>(1, count * (size_t)nnz) * 8));
+    HIPCHK(hipMemsetAsync(e->tstep.p, 0, count * 4, s));
+    HIPCHK(hipMemsetAsync(e->iters.p, 0, count * 4, s));
+    vector<u32> stt(count);
+    for (size_t l = 0; l < count; l++) stt[l] = reasons[l] == ESL_REASON_OK ? (u32)esl::kTranRunning : reasons[l];
+    HIPCHK(hipMemcpyAsync(e->status.p, stt.data(), count * 4, hipMemcpyHostToDevice, s));
+    vector<u32> ids(count);
+    for (size_t l = 0; l < count; l++) ids[l] = (u32)l;
+
+    esl::TranArgs ta{};
+    HostOpArgs &a = ta.op;
+    a.dp = e->dp;
+    a.live = e->live.as<double>();
+    a.x = e->x.as<double>();
+    a.lim = e->lim.as<double>();
+    a.resume = e->resume.as<esl::LaneResume>();
+    a.status = e->status.as<u32>();
+    a.iterations = e->iters.as<u32>();
+    a.lane_ids = e->lane_ids.as<u32>();
+    a.reltol = o.tol.reltol;
+    a.abstol = o.tol.abstol;
+    a.vntol = o.tol.vntol;
+    a.residual_tol = o.tol.residual_tol;
+    a.max_iter = o.tol.max_iter;
+    a.a_out = e->a_out.as<double>();
+    ta.q1 = e->tq1.as<double>();
+    ta.ip = e->tip.as<double>();
+    ta.step = e->tstep.as<u32>();
+    ta.probe_ids = e->probe_ids.as<u32>();
+    ta.probes = e->tprobes.as<double>();
+    ta.dt = o.dt;
+    ta.steps = o.steps;
+    ta.n_probes = o.n_probes;
+
+    // 3. the transient lead tape: the first converged point's first
+    // transient Jacobian (its resume record still names its .op tape)
+    vector<esl::LaneResume> rs(count);
+    HIPCHK(hipMemcpyAsync(rs.data(), e->resume.p, count * sizeof(esl::LaneResume), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    {
+        const u32 l0 = (u32)first_ok;
+        HIPCHK(hipMemcpyAsync(e->lane_ids.p, &l0, 4, hipMemcpyHostToDevice, s));
+        esl::TranArgs d = ta;
+        d.op.tapes = e->tape_table.as<esl::TapeProg>();
+        d.op.tv_cap = e->tapes[rs[l0].tape]->prog.n_tv;
+        d.init = 1;
+        d.dump = 1;
+        d.s0 = 0;
+        d.s1 = o.steps;
+        const u32 lds = tran_lds_bytes(c, d.op.tv_cap);
+        if (lds > device_lds_max()) {
+            g_last_error = "transient lane state exceeds the LDS of one workgroup";
+            return ESL_FAILED;
+        }
+        HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
+        HIPCHK(esl_launch_tran_steps(&d, 1, lds, s));
+        vector<double> A(nnz);
+        HIPCHK(hipMemcpyAsync(A.data(), e->a_out.p, nnz * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        HostTape t;
+        if (!gp_factor(n, c.col_ptr.data(), c.row_idx.data(), A.data(), c.q.data(), 1e-3, t)) {
+            for (size_t l = 0; l < count; l++)
+                if (reasons[l] == ESL_REASON_OK) reasons[l] = ESL_REASON_SINGULAR_MATRIX;
+            return ESL_OK;
+        }
+        u32 id;
+        esl_status s2 = add_tape(*e, t, id);
+        if (s2 != ESL_OK) return s2;
+        for (auto &r : rs) r = esl::LaneResume{0, 0, id, 0, 1e9, 0, 0, 0.0};
+        HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(e->lane_ids.p, ids.data(), count * 4, hipMemcpyHostToDevice, s));
+    }
+    ta.op.tapes = e->tape_table.as<esl::TapeProg>();
+
+    // 4. the step launches: blocks of steps_per_launch, captured in a graph
+    const u32 S = o.steps_per_launch, B = (o.steps + S - 1) / S;
+    auto block_args = [&](u32 b, u32 cap, u32 init) {
+        esl::TranArgs t = ta;
+        t.op.tapes = e->tape_table.as<esl::TapeProg>();
+        t.op.tv_cap = cap;
+        t.s0 = b * S;
+        t.s1 = std::min(o.steps, (b + 1) * S);
+        t.init = init;
+        t.dump = 0;
+        return t;
+    };
+    float kms = 0;
+    {
+        const u32 cap = e->tapes[rs[0].tape]->prog.n_tv;
+        const u32 lds = tran_lds_bytes(c, cap);
+        if (lds > device_lds_max()) {
+            g_last_error = "transient lane state exceeds the LDS of one workgroup";
+            return ESL_FAILED;
+        }
+        HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
+        HIPCHK(hipEventRecord(e->ev0, s));
+        if (o.use_graph) {
+            hipGraph_t g = nullptr;
+            hipGraphExec_t ge = nullptr;
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            hipError_t ce = hipSuccess;
+            for (u32 b = 0; b < B && ce == hipSuccess; b++) {
+                esl::TranArgs t = block_args(b, cap, b == 0);
+                ce = esl_launch_tran_steps(&t, (u32)count, lds, s);
+            }
+            hipError_t ee = hipStreamEndCapture(s, &g);
+            if (ce != hipSuccess) return fail_hip(ce, "tran capture");
+            HIPCHK(ee);
+            hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            if (ie != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                return fail_hip(ie, "graph instantiate");
+            }
+            hipError_t le = hipGraphLaunch(ge, s);
+            hipError_t se = hipStreamSynchronize(s);
+            (void)hipGraphExecDestroy(ge);
+            (void)hipGraphDestroy(g);
+            HIPCHK(le);
+            HIPCHK(se);
+        } else {
+            for (u32 b = 0; b < B; b++) {
+                esl::TranArgs t = block_args(b, cap, b == 0);
+                HIPCHK(esl_launch_tran_steps(&t, (u32)count, lds, s));
+            }
+        }
+        HIPCHK(hipEventRecord(e->ev1, s));
+        HIPCHK(hipEventSynchronize(e->ev1));
+        HIPCHK(hipEventElapsedTime(&kms, e->ev0, e->ev1));
+    }
+    e->timing.launches = B;
+    // 5. re-pivot rounds: factor each failing lane's dumped Jacobian, resume
+    for (int round = 0; round < 1000; round++) {
+        HIPCHK(hipMemcpyAsync(stt.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        vector<u32> again;
+        for (size_t l = 0; l < count; l++)
+            if (stt[l] == esl::kRepivot) again.push_back((u32)l);
+        if (again.empty()) break;
+        vector<u32> steps_at(count);
+        HIPCHK(hipMemcpyAsync(rs.data(), e->resume.p, count * sizeof(esl::LaneResume), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(steps_at.data(), e->tstep.p, count * 4, hipMemcpyDeviceToHost, s));
+        vector<double> A(nnz);
+        vector<u32> next;
+        u32 b0 = B;
+        for (u32 l : again) {
+            HIPCHK(hipMemcpyAsync(A.data(), e->a_out.as<double>() + (size_t)l * nnz, nnz * 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            HostTape t;
+            e->timing.repivots++;
+            if (!gp_factor(n, c.col_ptr.data(), c.row_idx.data(), A.data(), c.q.data(), 1e-3, t)) {
+                stt[l] = esl::kSingular;
+                continue;
+            }
+            u32 id;
+            esl_status s2 = add_tape(*e, t, id);
+            if (s2 != ESL_OK) return s2;
+            rs[l].tape = id;
+            stt[l] = esl::kTranRunning;
+            next.push_back(l);
+            b0 = std::min(b0, steps_at[l] / S);
+        }
+        HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(e->status.p, stt.data(), count * 4, hipMemcpyHostToDevice, s));
+        if (next.empty()) break;
+        HIPCHK(hipMemcpyAsync(e->lane_ids.p, next.data(), next.size() * 4, hipMemcpyHostToDevice, s));
+        u32 cap = 0;
+        for (u32 l : next) cap = std::max(cap, e->tapes[rs[l].tape]->prog.n_tv);
+        const u32 lds = tran_lds_bytes(c, cap);
+        if (lds > device_lds_max()) {
+            g_last_error = "transient lane state exceeds the LDS of one workgroup";
+            return ESL_FAILED;
+        }
+        HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
+        float ms = 0;
+        HIPCHK(hipEventRecord(e->ev0, s));
+        for (u32 b = b0; b < B; b++) {
+            esl::TranArgs t = block_args(b, cap, 0);
+            HIPCHK(esl_launch_tran_steps(&t, (u32)next.size(), lds, s));
+        }
+        HIPCHK(hipEventRecord(e->ev1, s));
+        HIPCHK(hipEventSynchronize(e->ev1));
+        HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+        kms += ms;
+        e->timing.launches += B - b0;
+    }
+    // 6. results
+    HIPCHK(hipMemcpyAsync(x, e->x.p, count * n * 8, hipMemcpyDeviceToHost, s));
+    if (o.n_probes)
+        HIPCHK(hipMemcpyAsync(probe_out, e->tprobes.p, count * (size_t)o.steps * o.n_probes * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(iterations, e->iters.p, count * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(stt.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    uint64_t lane_it = 0;
+    for (size_t l = 0; l < count; l++) {
+        if (reasons[l] != ESL_REASON_OK) { iterations[l] = 0; continue; }
+        switch (stt[l]) {
+        case esl::kConverged: reasons[l] = ESL_REASON_OK; break;
+        case esl::kSingular: reasons[l] = ESL_REASON_SINGULAR_MATRIX; break;
+        case esl::kNoConvergence: reasons[l] = ESL_REASON_NO_CONVERGENCE; break;
+        default: reasons[l] = ESL_REASON_OTHER; break;
+        }
+        lane_it += iterations[l];
+    }
+    e->timing.kernel_ms = op_ms + kms;
     e->timing.lane_iterations = lane_it;
     e->timing.total_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();

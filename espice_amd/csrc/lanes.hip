@@ -665,6 +665,251 @@ __global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
     }
 }
 
+// ---- transient lanes ----
+
+// Assembly of a transient Newton iterate: static stamps into tv (through the
+// tape's slot map) and rhs, charge-Jacobian cells into cq, row charges into q
+// (src/device/eval.zig:782-922 with the charge planes of LaneSink), the
+// ground pin, then A = G + ag0 C and F += ag0 (q - q1) - ag1 i_prev
+// (lane_kernels.zig:194-215).
+__device__ void assemble_tran(const DeckProg &dp, const TapeProg &T, const double *live, const double *x, double *lim,
+                              uint32_t nm, double *tv, double *rhs, double *q, double *cq, double *st, double t,
+                              double ag0, double ag1, const double *q1, const double *ip, uint32_t tid)
+{
+    const uint32_t n = dp.n;
+    for (uint32_t i = tid; i < T.n_tv; i += kWave) tv[i] = 0.0;
+    for (uint32_t i = tid; i < n; i += kWave) {
+        rhs[i] = 0.0;
+        q[i] = 0.0;
+    }
+    for (uint32_t k = tid; k < dp.n_cslot; k += kWave) cq[k] = 0.0;
+    __syncthreads();
+    for (uint32_t c = 0; c < dp.n_chunks; c++) {
+        const Chunk ch = dp.chunks[c];
+        eval_chunk(dp, ch, live, x, lim, nm, st, tid, t, true);
+        __syncthreads();
+        const uint32_t t1 = dp.chunks[c + 1].task_begin;
+        for (uint32_t k = ch.task_begin + tid; k < t1; k += kWave) {
+            const uint32_t tgt = dp.task_target[k];
+            const uint32_t idx = tgt & kTargetMask;
+            double *d = (tgt & kRhsBit) ? &rhs[idx] : (tgt & kQBit) ? &q[idx] : (tgt & kCBit) ? &cq[idx]
+                                                                                               : &tv[T.amap[tgt]];
+            double acc = *d;
+            const uint32_t c1 = dp.task_cell[k + 1];
+            for (uint32_t cc = dp.task_cell[k]; cc < c1; cc++) acc = acc + st[dp.cells[cc]];
+            *d = acc;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        tv[T.amap[dp.diag0_slot]] = tv[T.amap[dp.diag0_slot]] + 1.0;
+        rhs[0] = rhs[0] + x[0];
+    }
+    __syncthreads();
+    for (uint32_t k = tid; k < dp.n_cslot; k += kWave) {
+        const uint32_t sl = T.amap[dp.cslot_pat[k]];
+        tv[sl] = tv[sl] + ag0 * cq[k];
+    }
+    for (uint32_t i = tid; i < n; i += kWave) {
+        const double d = ag0 * (q[i] - q1[i]);
+        const double base = rhs[i] + d;
+        rhs[i] = base - ag1 * ip[i];
+    }
+    __syncthreads();
+}
+
+// Runs steps [s0, s1) of each still-running lane: per step a Newton solve
+// with the .op gate order (residual gate off: only a NaN row refuses, as
+// lane_device.zig:74-98), then acceptance (published x = x_old), the
+// dynamic currents i = ag0 (q - q1) - ag1 i_prev, q1 = q, the probe row,
+// and limiting state reset to the accepted point. A lane whose replay fails
+// leaves its Jacobian in a_out for a host re-pivot and resumes at the same
+// iterate.
+__global__ __launch_bounds__(kWave) void tran_steps(TranArgs ta)
+{
+    extern __shared__ double sm[];
+    const OpArgs &a = ta.op;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = a.lane_ids[blockIdx.x];
+    if (a.status[lane] != kTranRunning) return;
+    uint32_t s = ta.step[lane];
+    if (s >= ta.s1) return;
+    if (s < ta.s0) s = ta.s0;
+    const DeckProg &dp = a.dp;
+    const uint32_t n = dp.n, nm = dp.n_mos;
+    double *tv = sm;
+    double *sdiag = tv + a.tv_cap + n;
+    double *xold = sdiag + n;
+    double *st = tv + a.tv_cap;
+    double *rhs = tv + a.tv_cap + lds_y_block(n);
+    double *x = rhs + n;
+    double *q = x + n;
+    double *q1 = q + n;
+    double *ip = q1 + n;
+    double *cq = ip + n;
+    const double *live = a.live + (size_t)lane * dp.n_live;
+    double *lim = a.lim + (size_t)lane * (4 * nm + dp.n_dio);
+    LaneResume rs = a.resume[lane];
+    const TapeProg T = a.tapes[rs.tape];
+    double *y = tv + T.n_tv;
+    for (uint32_t i = tid; i < n; i += kWave) x[i] = a.x[(size_t)lane * n + i];
+    __syncthreads();
+    if (ta.init) {
+        // start at the operating point: limiting state at its own voltages,
+        // charges from one evaluation there, no dynamic current
+        seed_lim_at(dp, live, x, lim, tid);
+        for (uint32_t i = tid; i < n; i += kWave) {
+            q1[i] = 0.0;
+            ip[i] = 0.0;
+        }
+        __syncthreads();
+        assemble_tran(dp, T, live, x, lim, nm, tv, rhs, q, cq, st, 0.0, 0.0, 0.0, q1, ip, tid);
+        for (uint32_t i = tid; i < n; i += kWave) q1[i] = q[i];
+        __syncthreads();
+        if (ta.dump) {
+            // the transient lead point's first Jacobian (step 0, iteration 0)
+            assemble_tran(dp, T, live, x, lim, nm, tv, rhs, q, cq, st, ta.dt, 1.0 / ta.dt, 0.0, q1, ip, tid);
+            for (uint32_t p = tid; p < dp.nnz; p += kWave) a.a_out[(size_t)blockIdx.x * dp.nnz + p] = tv[T.amap[p]];
+            return;
+        }
+    } else {
+        for (uint32_t i = tid; i < n; i += kWave) {
+            q1[i] = ta.q1[(size_t)lane * n + i];
+            ip[i] = ta.ip[(size_t)lane * n + i];
+        }
+    }
+    __syncthreads();
+    uint32_t iter0 = rs.iter;   // nonzero only when resuming inside a step
+    double limit = rs.limit;
+    uint32_t status = kTranRunning;
+    uint32_t total_it = a.iterations[lane];
+    for (; s < ta.s1 && status == kTranRunning; s++) {
+        const double t = (double)(s + 1) * ta.dt;
+        const double ag0 = s == 0 ? 1.0 / ta.dt : 2.0 / ta.dt;
+        const double ag1 = s == 0 ? 0.0 : 1.0;
+        bool converged = false;
+        uint32_t iter = iter0;
+        for (; iter < a.max_iter; iter++) {
+            assemble_tran(dp, T, live, x, lim, nm, tv, rhs, q, cq, st, t, ag0, ag1, q1, ip, tid);
+            if (iter == 0) {
+                double gm = 1.0;
+                uint64_t xm = 0, fm = 0;
+                for (uint32_t i = tid; i < n; i += kWave) {
+                    gm = fmax(gm, fabs(tv[T.diag_tv[i]]));
+                    const uint64_t xb = dbits(fabs(x[i])), fb = dbits(fabs(rhs[i]));
+                    xm = xb > xm ? xb : xm;
+                    fm = fb > fm ? fb : fm;
+                }
+                gm = wave_fmax(gm);
+                xm = wave_max_u64(xm);
+                fm = wave_max_u64(fm);
+                const double scale = fmin(fmax(bitsd(xm), bitsd(fm) * gm), 1e18);
+                limit = !(1e3 * scale > 1e9) ? 1e9 : 1e3 * scale;
+            }
+            for (uint32_t i = tid; i < n; i += kWave) sdiag[i] = tv[T.diag_tv[i]];
+            __syncthreads();
+            if (refactor<double>(T, n, tv, tid)) {
+                // the Jacobian again, for the host's re-pivot
+                assemble_tran(dp, T, live, x, lim, nm, tv, rhs, q, cq, st, t, ag0, ag1, q1, ip, tid);
+                for (uint32_t p = tid; p < dp.nnz; p += kWave) a.a_out[(size_t)lane * dp.nnz + p] = tv[T.amap[p]];
+                if (tid == 0) {
+                    LaneResume o = rs;
+                    o.iter = iter;
+                    o.limit = limit;
+                    o.fresh = 0;
+                    a.resume[lane] = o;
+                }
+                status = kRepivot;
+                break;
+            }
+            solve_neg(T, n, tv, rhs, y, tid);
+            __syncthreads();
+            uint64_t worst = 0;
+            for (uint32_t i = tid; i < n; i += kWave) {
+                const double xo = x[i];
+                const double d = y[T.qinv[i]];
+                const double xn = xo + d;
+                xold[i] = xo;
+                x[i] = xn;
+                const double tl = a.reltol * fmax(fabs(xn), fabs(xo)) + (dp.current_row[i] ? a.abstol : a.vntol);
+                const bool finite = fabs(xn) < __builtin_inf() && fabs(d) < __builtin_inf();
+                const double qq = fabs(d) / tl;
+                const double score = !finite ? __builtin_inf() : (qq == qq ? qq : 0.0);
+                const uint64_t sb = dbits(score);
+                worst = sb > worst ? sb : worst;
+            }
+            const double scaled = bitsd(wave_max_u64(worst));
+            __syncthreads();
+            bool limited = false;
+            for (uint32_t j = tid; j < nm; j += kWave) {
+                const Mos m = mos_load(dp, j, live);
+                const uint32_t *t4 = dp.m_t + 4 * j;
+                limited |= mos_limit(m, x[t4[0]], x[t4[1]], x[t4[2]], x[t4[3]], lim + j, nm);
+            }
+            for (uint32_t j = tid; j < dp.n_dio; j += kWave) {
+                const Dio d = dio_load(dp, j, live);
+                limited |= dio_limit_v(d, x[dp.d_t[2 * j]], x[dp.d_t[2 * j + 1]], lim[4 * nm + j]);
+            }
+            limited = wave_any(limited);
+            bool ok = !limited && iter != 0 && scaled < 1.0;
+            if (ok) {
+                bool miss = false;
+                for (uint32_t i = tid; i < n; i += kWave) {
+                    miss |= !(fabs(x[i]) <= limit);
+                    miss |= !(fabs(rhs[i]) <= __builtin_inf());
+                }
+                ok = !wave_any(miss);
+            }
+            __syncthreads();
+            if (ok) {
+                for (uint32_t i = tid; i < n; i += kWave) x[i] = xold[i];
+                converged = true;
+                iter++;
+                break;
+            }
+        }
+        if (status == kRepivot) {
+            total_it += iter - iter0;
+            break;
+        }
+        total_it += iter - iter0;
+        iter0 = 0;
+        if (!converged) {
+            status = kNoConvergence;
+            break;
+        }
+        // accept: q holds the charges of the converged point (the last
+        // assembly ran at x_old)
+        __syncthreads();
+        for (uint32_t i = tid; i < n; i += kWave) {
+            const double d = ag0 * (q[i] - q1[i]);
+            ip[i] = d - ag1 * ip[i];
+            q1[i] = q[i];
+        }
+        for (uint32_t k = tid; k < ta.n_probes; k += kWave)
+            ta.probes[((size_t)lane * ta.steps + s) * ta.n_probes + k] = x[ta.probe_ids[k]];
+        __syncthreads();
+        seed_lim_at(dp, live, x, lim, tid);
+        __syncthreads();
+    }
+    if (status == kTranRunning && s >= ta.steps) status = kConverged;
+    for (uint32_t i = tid; i < n; i += kWave) {
+        a.x[(size_t)lane * n + i] = x[i];
+        ta.q1[(size_t)lane * n + i] = q1[i];
+        ta.ip[(size_t)lane * n + i] = ip[i];
+    }
+    if (tid == 0) {
+        if (status != kRepivot) {
+            LaneResume o = rs;
+            o.iter = 0;
+            a.resume[lane] = o;
+        }
+        ta.step[lane] = s;
+        a.status[lane] = status;
+        a.iterations[lane] = total_it;
+    }
+}
+
 // ---- the LU boundary alone: replay one tape over lanes of A, solve -rhs ----
 
 
@@ -733,12 +978,20 @@ extern "C" hipError_t esl_launch_tape_replay(const esl::ReplayArgs *r, uint32_t 
     return hipGetLastError();
 }
 
+extern "C" hipError_t esl_launch_tran_steps(const esl::TranArgs *ta, uint32_t grid, uint32_t lds_bytes, hipStream_t s)
+{
+    hipLaunchKernelGGL(esl::tran_steps, dim3(grid), dim3(esl::kWave), lds_bytes, s, *ta);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t esl_kernel_attrs(uint32_t lds_bytes)
 {
     hipError_t e = hipFuncSetAttribute((const void *)esl::op_newton,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void *)esl::op_dump, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void *)esl::tran_steps, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) return e;
     return hipFuncSetAttribute((const void *)esl::tape_replay, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds_bytes);

@@ -36,7 +36,7 @@ EXPORTS = [
     "esl_tape_create", "esl_tape_destroy", "esl_tape_sizes", "esl_tape_export", "esl_tape_replay",
     "esl_mlp_default_train_options", "esl_mlp_create", "esl_mlp_destroy", "esl_mlp_param_count",
     "esl_mlp_get_params", "esl_mlp_set_params", "esl_mlp_set_scaling", "esl_mlp_gradient", "esl_mlp_train",
-    "esl_mlp_predict", "esl_engine_set_warm_start",
+    "esl_mlp_predict", "esl_engine_set_warm_start", "esl_default_tran_options", "esl_tran_solve",
 ]
 
 
@@ -80,6 +80,12 @@ class Timing(C.Structure):
                 ("repivots", C.c_uint32), ("lane_iterations", C.c_uint64), ("tiling", C.c_uint32),
                 ("fp64_lanes", C.c_uint32), ("warm_fallbacks", C.c_uint32), ("reserved", C.c_uint32),
                 ("warm_ms", C.c_double), ("lead_ms", C.c_double), ("download_ms", C.c_double)]
+
+
+class TranOptions(C.Structure):
+    _fields_ = [("struct_size", C.c_size_t), ("tol", Tolerances), ("dt", C.c_double), ("steps", C.c_uint32),
+                ("steps_per_launch", C.c_uint32), ("probes", C.c_void_p), ("n_probes", C.c_uint32),
+                ("use_graph", C.c_uint32)]
 
 
 class MlpTrainOptions(C.Structure):
@@ -143,6 +149,9 @@ def load() -> C.CDLL:
     L.esl_mlp_train.argtypes = [vp, vp, vp, C.c_size_t, C.POINTER(MlpTrainOptions), vp]
     L.esl_mlp_predict.restype = C.c_uint32
     L.esl_mlp_predict.argtypes = [vp, vp, C.c_size_t, vp]
+    L.esl_default_tran_options.argtypes = [C.POINTER(TranOptions)]
+    L.esl_tran_solve.restype = C.c_uint32
+    L.esl_tran_solve.argtypes = [vp, vp, C.c_size_t, C.POINTER(TranOptions), vp, vp, vp, vp]
     L.esl_engine_set_warm_start.restype = C.c_uint32
     L.esl_engine_set_warm_start.argtypes = [vp, vp, vp]
     _lib = L
@@ -240,6 +249,28 @@ class Engine:
         _check(load().esl_op_solve(self.h, _ptr(values), count, C.byref(o), _ptr(x), _ptr(it), _ptr(rs)),
                "esl_op_solve")
         return {"x": x, "iterations": it, "reasons": rs}
+
+    def solve_tran(self, values, dt, steps, probes=(), steps_per_launch=10, use_graph=True, max_iter=None):
+        """Transient lanes: operating point, then `steps` fixed steps of dt.
+        Returns x [count][n] (last state), probes [count][steps][n_probes],
+        iterations, reasons."""
+        values = np.ascontiguousarray(np.atleast_2d(np.asarray(values, dtype=np.float64)))
+        count = values.shape[0]
+        o = TranOptions()
+        load().esl_default_tran_options(C.byref(o))
+        o.dt, o.steps, o.steps_per_launch, o.use_graph = dt, steps, steps_per_launch, 1 if use_graph else 0
+        if max_iter is not None:
+            o.tol.max_iter = max_iter
+        pr = np.ascontiguousarray(probes, dtype=np.uint32)
+        if len(pr):
+            o.probes, o.n_probes = pr.ctypes.data, len(pr)
+        x = np.zeros((count, self.n))
+        po = np.zeros((count, steps, max(len(pr), 1)))
+        it = np.zeros(count, np.uint32)
+        rs = np.zeros(count, np.uint32)
+        _check(load().esl_tran_solve(self.h, _ptr(values), count, C.byref(o), _ptr(x),
+                                     _ptr(po) if len(pr) else None, _ptr(it), _ptr(rs)), "esl_tran_solve")
+        return {"x": x, "probes": po[:, :, :len(pr)], "iterations": it, "reasons": rs}
 
     def set_warm_start(self, net: "Mlp | None", x_nominal=None):
         """Attaches the warm-start network and the nominal operating point."""

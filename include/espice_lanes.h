@@ -217,6 +217,36 @@ esl_status esl_op_solve(esl_engine *e, const double *values, size_t count,
                         const esl_op_options *opt, double *x,
                         uint32_t *iterations, uint32_t *reasons);
 
+/* ---- Transient lanes (gpu_lanes.zig / tran.runGpuLanes, tran.zig:
+ * 1654-1820, with fixed steps). Each point first solves its operating point
+ * (cold start, as esl_op_solve), then integrates `steps` fixed steps of dt:
+ * step 0 backward Euler, later steps the trapezoid (xmu = 0.5: ag0 = 2/dt,
+ * ag1 = 1, integrator.zig:40-53), each a Newton solve on the device with the
+ * .op gate order and the residual gate off (lane_device.zig:74-98). The
+ * lanes share one tape from the first converged point's first transient
+ * Jacobian; a lane whose replay fails is re-pivoted on the host and resumes.
+ * The launches, each `steps_per_launch` steps, are captured in one hipGraph
+ * and replayed. A step that does not converge in max_iter iterations ends
+ * the lane (ESL_REASON_NO_CONVERGENCE): fixed-step lanes do not cut the
+ * step. ---- */
+typedef struct {
+    size_t struct_size;        /* sizeof(esl_tran_options) */
+    esl_tolerances tol;        /* per step (max_iter default 50) */
+    double dt;                 /* step, s */
+    uint32_t steps;
+    uint32_t steps_per_launch; /* default 10 */
+    const uint32_t *probes;    /* unknowns recorded after every step */
+    uint32_t n_probes;
+    uint32_t use_graph;        /* 1 (default): capture the launches in a hipGraph */
+} esl_tran_options;
+void esl_default_tran_options(esl_tran_options *opt);
+/* values [count][n_live]; outputs x [count][n] (the last accepted state),
+ * probe_out [count][steps][n_probes] (may be NULL when n_probes = 0),
+ * iterations [count] (transient Newton iterations), reasons [count]. */
+esl_status esl_tran_solve(esl_engine *e, const double *values, size_t count,
+                          const esl_tran_options *opt, double *x, double *probe_out,
+                          uint32_t *iterations, uint32_t *reasons);
+
 /* Device-time split of the last esl_op_solve, milliseconds. */
 typedef struct {
     double kernel_ms;      /* Newton kernel(s), HIP events */

@@ -170,4 +170,19 @@ int orc_op_solve2(orc_ckt *c, const double *values, size_t count,
                   const double *start_x, int threads, double *x,
                   uint32_t *iterations, uint32_t *reasons, uint32_t *repivots,
                   uint32_t *fp64_lanes, uint32_t *fell_back);
+/* Transient lanes, fixed step (the GPU's esl_tran_solve): each point's
+ * operating point (cold, tol_op, lead tape from point 0), then `steps` steps
+ * of dt (step 0 backward Euler, then the trapezoid with xmu = 0.5) with the
+ * companion A = G + ag0 C, F += ag0 (q - q1) - ag1 i_prev
+ * (integrator.zig:40-53, lane_kernels.zig:194-215), each a Newton solve with
+ * the residual gate off; the lanes share the tape of the first converged
+ * point's first transient Jacobian and re-pivot on a failed replay. After a
+ * step: i = ag0 (q - q1) - ag1 i_prev, q1 = q, probes, limiting state reset
+ * to the accepted point. probe_out [count][steps][n_probes]. Parity pinned by
+ * tests: RC closed forms (a single RC charging, a ladder against the exact
+ * trapezoid recurrence computed densely) and agreement with the GPU. */
+int orc_tran_solve(orc_ckt *c, const double *values, size_t count, const orc_tol *tol_op,
+                   const orc_tol *tol, double dt, uint32_t steps, const uint32_t *probes,
+                   uint32_t n_probes, const uint32_t *q, int threads, double *x,
+                   double *probe_out, uint32_t *iterations, uint32_t *reasons);
 #endif

@@ -86,6 +86,9 @@ def lib():
         L.orc_lu_replay32_export.restype = C.c_int
         L.orc_lu_replay32_export.argtypes = [C.c_uint32, u32p, u32p, f64p, u32p, C.c_double, f64p, f64p,
                                              C.c_size_t, C.c_double, f32p, f32p, f32p, f32p, u32p]
+        L.orc_tran_solve.restype = C.c_int
+        L.orc_tran_solve.argtypes = [C.c_void_p, f64p, C.c_size_t, C.POINTER(Tol), C.POINTER(Tol), C.c_double,
+                                     C.c_uint32, C.c_void_p, C.c_uint32, u32p, C.c_int, f64p, C.c_void_p, u32p, u32p]
         L.orc_op_solve.restype = C.c_int
         L.orc_op_solve.argtypes = [C.c_void_p, f64p, C.c_size_t, C.POINTER(Tol), u32p, C.c_void_p,
                                    C.c_int, f64p, u32p, u32p, u32p]
@@ -257,6 +260,23 @@ class Circuit:
         lib().orc_op_solve2(self.h, values, count, C.byref(t), C.byref(p), _u32(q),
                             None if sx is None else sx.ctypes.data, int(threads), x, it, rs, rp, fb, wb)
         return dict(x=x, iterations=it, reasons=rs, repivots=rp, fp64_lanes=fb, fell_back=wb)
+
+
+def tran(circuit, values, q, dt, steps, probes=(), threads=1, tol=None, tol_op=None):
+    """Transient lanes on the oracle (see orc_tran_solve)."""
+    values = _f64(np.atleast_2d(values))
+    count = values.shape[0]
+    t = Tol(1e-3, 1e-12, 1e-6, 1e-9, 50, 0) if tol is None else tol
+    to = Tol(1e-3, 1e-12, 1e-6, 1e-9, 100, 0) if tol_op is None else tol_op
+    pr = np.ascontiguousarray(probes, dtype=np.uint32)
+    x = np.zeros((count, circuit.n))
+    po = np.zeros((count, steps, max(len(pr), 1)))
+    it = np.zeros(count, np.uint32)
+    rs = np.zeros(count, np.uint32)
+    lib().orc_tran_solve(circuit.h, values, count, C.byref(to), C.byref(t), float(dt), int(steps),
+                         pr.ctypes.data if len(pr) else None, len(pr), _u32(q), int(threads), x,
+                         po.ctypes.data, it, rs)
+    return dict(x=x, probes=po[:, :, :len(pr)], iterations=it, reasons=rs)
 
 
 def natural_order(n):

@@ -666,11 +666,19 @@ static void seed_at(const orc_ckt *c, lane_dev *ld, const double *x)
     }
 }
 
-static void assemble(const orc_ckt *c, lane_dev *ld, const double *x, double *g, double *rhs)
+/* Static stamps into g (pattern slots) and rhs; with charges, the charge
+ * Jacobian into cp (pattern slots) and the row charges into q. t is the
+ * source time (0 for the operating point). */
+static void assemble_t(const orc_ckt *c, lane_dev *ld, const double *x, double t, int charges, double *g,
+                       double *rhs, double *cp, double *q)
 {
     const uint32_t n = c->n;
     memset(g, 0, (size_t)(c->nnz + 1) * 8);
     memset(rhs, 0, (size_t)(n + 1) * 8);
+    if (charges) {
+        memset(cp, 0, (size_t)(c->nnz + 1) * 8);
+        memset(q, 0, (size_t)(n + 1) * 8);
+    }
     /* ground rows land in rhs[n], ground entries in g[nnz]: the trash */
 #define ROW(u) ((u) == 0 ? n : (u))
 #define SLOT(s) ((s) == TRASH ? c->nnz : (s))
@@ -690,11 +698,25 @@ static void assemble(const orc_ckt *c, lane_dev *ld, const double *x, double *g,
             g[SLOT(in->slot[3])] += gg;
             break;
         }
+        case K_C: {
+            if (!charges) break;
+            const double cv = p[0];
+            const double qq = cv * (x[in->t[0]] - x[in->t[1]]);
+            cp[SLOT(in->slot[0])] += cv;
+            cp[SLOT(in->slot[1])] += -cv;
+            cp[SLOT(in->slot[2])] += -cv;
+            cp[SLOT(in->slot[3])] += cv;
+            q[ROW(in->t[0])] += qq;
+            q[ROW(in->t[1])] += -qq;
+            break;
+        }
         case K_V: {
             const double ib = x[in->br];
+            /* waveform (ESL_V_WAVE 1: step to p[2] after p[3]) */
+            const double dc = (p[1] == 1.0 && t > p[3]) ? p[2] : p[0];
             rhs[ROW(in->t[0])] += ib;
             rhs[ROW(in->t[1])] += -ib;
-            rhs[in->br] += (x[in->t[0]] - x[in->t[1]]) - p[0];
+            rhs[in->br] += (x[in->t[0]] - x[in->t[1]]) - dc;
             g[SLOT(in->slot[0])] += 1.0;
             g[SLOT(in->slot[1])] += -1.0;
             g[SLOT(in->slot[2])] += 1.0;
@@ -706,11 +728,16 @@ static void assemble(const orc_ckt *c, lane_dev *ld, const double *x, double *g,
             rhs[ROW(in->t[1])] += -p[0];
             break;
         case K_D: {
-            double o[12];
-            dio1_eval(&ld->dp[i], x[in->t[0]], x[in->t[1]], ld->lim[(size_t)i * 4], 0, o);
+            double o[12] = {0};
+            dio1_eval(&ld->dp[i], x[in->t[0]], x[in->t[1]], ld->lim[(size_t)i * 4], charges, o);
             for (int e = 0; e < 4; e++) g[SLOT(in->slot[e])] += o[e];
             rhs[ROW(in->t[0])] += o[4];
             rhs[ROW(in->t[1])] += o[5];
+            if (charges) {
+                for (int e = 0; e < 4; e++) cp[SLOT(in->slot[e])] += o[6 + e];
+                q[ROW(in->t[0])] += o[10];
+                q[ROW(in->t[1])] += o[11];
+            }
             break;
         }
         case K_MOS: {
@@ -730,6 +757,11 @@ static void assemble(const orc_ckt *c, lane_dev *ld, const double *x, double *g,
 #undef SLOT
     g[c->diag[0]] += 1.0;
     rhs[0] += x[0];
+}
+
+static void assemble(const orc_ckt *c, lane_dev *ld, const double *x, double *g, double *rhs)
+{
+    assemble_t(c, ld, x, 0.0, 0, g, rhs, NULL, NULL);
 }
 
 static int apply_limits(const orc_ckt *c, lane_dev *ld, const double *x)
@@ -1065,4 +1097,218 @@ int orc_op_solve2(orc_ckt *c, const double *values, size_t count, const orc_tol 
     free(jobs); free(th);
     orc_lu_free(lead);
     return 0;
+}
+
+/* ---- transient lanes (fixed step; the GPU lanes' esl_tran_solve) ---- */
+typedef struct {
+    orc_ckt *c;
+    const double *values;
+    size_t first, count;
+    const orc_tol *tol;
+    double dt;
+    uint32_t steps, n_probes;
+    const uint32_t *probes;
+    const orc_lu *lead;            /* the transient lead tape */
+    const uint32_t *q;
+    const uint8_t *has_c;          /* [nnz] slots with charge-Jacobian entries */
+    const double *x_op;            /* [count][n] operating points */
+    const uint32_t *op_reason;
+    double *x, *probe_out;
+    uint32_t *iterations, *reasons;
+} tjob_t;
+
+/* A = G + ag0 C on the charge slots, F += ag0 (q - q1) - ag1 i_prev */
+static void companion(const orc_ckt *c, const uint8_t *has_c, double ag0, double ag1, double *g, const double *cp,
+                      double *rhs, const double *q, const double *q1, const double *ip)
+{
+    for (uint32_t p = 0; p < c->nnz; p++)
+        if (has_c[p]) g[p] = g[p] + ag0 * cp[p];
+    for (uint32_t i = 0; i < c->n; i++) {
+        const double d = ag0 * (q[i] - q1[i]);
+        const double base = rhs[i] + d;
+        rhs[i] = base - ag1 * ip[i];
+    }
+}
+
+static orc_lu *lu_copy(const orc_lu *s, uint32_t n, uint32_t nnz)
+{
+    orc_lu *lu = orc_lu_new(n, nnz);
+    memcpy(lu->q, s->q, n * 4); memcpy(lu->pinv, s->pinv, n * 4);
+    memcpy(lu->prow, s->prow, nnz * 4); memcpy(lu->lp, s->lp, (n + 1) * 4);
+    memcpy(lu->up, s->up, (n + 1) * 4);
+    memcpy(lu->void_col, s->void_col, n); memcpy(lu->scaled, s->scaled, n);
+    memcpy(lu->void_slots, s->void_slots, (size_t)s->n_void_slots * 4);
+    lu->n_void_slots = s->n_void_slots;
+    lu->cap_l = s->nnz_l + 16; lu->cap_u = s->nnz_u + 16;
+    lu->li = realloc(lu->li, lu->cap_l * 4); lu->lx = realloc(lu->lx, lu->cap_l * 8);
+    lu->ui = realloc(lu->ui, lu->cap_u * 4); lu->ux = realloc(lu->ux, lu->cap_u * 8);
+    memcpy(lu->li, s->li, s->nnz_l * 4); memcpy(lu->ui, s->ui, s->nnz_u * 4);
+    lu->nnz_l = s->nnz_l; lu->nnz_u = s->nnz_u;
+    lu->factored = 1;
+    return lu;
+}
+
+static void tran_lane(const tjob_t *j, size_t l)
+{
+    const orc_ckt *c = j->c;
+    const uint32_t n = c->n, nnz = c->nnz;
+    if (j->op_reason[l] != 0) {
+        memcpy(j->x + l * n, j->x_op + l * n, (size_t)n * 8);
+        j->iterations[l] = 0;
+        j->reasons[l] = j->op_reason[l];
+        return;
+    }
+    const orc_tol *tol = j->tol;
+    lane_dev ld;
+    lane_dev_init(c, j->values + l * c->d.n_live, &ld);
+    double *x = calloc(n + 1, 8), *x_old = calloc(n + 1, 8), *dx = calloc(n + 1, 8);
+    double *g = calloc(nnz + 1, 8), *cp = calloc(nnz + 1, 8), *rhs = calloc(n + 1, 8);
+    double *q = calloc(n + 1, 8), *q1 = calloc(n + 1, 8), *ip = calloc(n + 1, 8);
+    memcpy(x, j->x_op + l * n, (size_t)n * 8);
+    seed_at(c, &ld, x);
+    assemble_t(c, &ld, x, 0.0, 1, g, rhs, cp, q);
+    memcpy(q1, q, (size_t)n * 8);
+    orc_lu *lu = lu_copy(j->lead, n, nnz);
+    const double wide = 1.0 / 1e-3;
+    uint32_t reason = 0, total = 0;
+    for (uint32_t s = 0; s < j->steps && reason == 0; s++) {
+        const double t = (double)(s + 1) * j->dt;
+        const double ag0 = s == 0 ? 1.0 / j->dt : 2.0 / j->dt;
+        const double ag1 = s == 0 ? 0.0 : 1.0;
+        double limit = 1e9;
+        int converged = 0;
+        uint32_t iter = 0;
+        for (; iter < tol->max_iter; iter++) {
+            assemble_t(c, &ld, x, t, 1, g, rhs, cp, q);
+            companion(c, j->has_c, ag0, ag1, g, cp, rhs, q, q1, ip);
+            if (iter == 0) limit = plausible_limit(c, x, rhs, g);
+            if (orc_lu_lane_refactor(lu, c->col_ptr, g, 1e-12, wide)) {
+                if (orc_lu_factor(lu, c->col_ptr, c->row_idx, g, j->q, 1e-3) != 0) {
+                    reason = 3;
+                    break;
+                }
+            }
+            orc_lu_solve_neg(lu, rhs, dx);
+            uint64_t worst = 0;
+            for (uint32_t i = 0; i < n; i++) {
+                const double xo = x[i], d = dx[i], xn = xo + d;
+                x_old[i] = xo;
+                x[i] = xn;
+                const double tl = tol->reltol * fmax(fabs(xn), fabs(xo)) + (c->current_row[i] ? tol->abstol : tol->vntol);
+                const int finite = fabs(xn) < INFINITY && fabs(d) < INFINITY;
+                const double qq = fabs(d) / tl;
+                const double score = !finite ? INFINITY : (qq == qq ? qq : 0.0);
+                const uint64_t sb = db(score);
+                if (sb > worst) worst = sb;
+            }
+            const double scaled = bd(worst);
+            const int limited = apply_limits(c, &ld, x);
+            int ok = !limited && iter != 0 && scaled < 1.0;
+            if (ok)
+                for (uint32_t i = 0; i < n; i++)
+                    if (!(fabs(x[i]) <= limit) || !(fabs(rhs[i]) <= INFINITY)) { ok = 0; break; }
+            if (ok) {
+                memcpy(x, x_old, (size_t)n * 8);
+                converged = 1;
+                iter++;
+                break;
+            }
+        }
+        total += iter;
+        if (reason) break;
+        if (!converged) { reason = 1; break; }
+        for (uint32_t i = 0; i < n; i++) {
+            const double d = ag0 * (q[i] - q1[i]);
+            ip[i] = d - ag1 * ip[i];
+            q1[i] = q[i];
+        }
+        for (uint32_t k = 0; k < j->n_probes; k++)
+            j->probe_out[((size_t)l * j->steps + s) * j->n_probes + k] = x[j->probes[k]];
+        seed_at(c, &ld, x);
+    }
+    memcpy(j->x + l * n, x, (size_t)n * 8);
+    j->iterations[l] = total;
+    j->reasons[l] = reason;
+    orc_lu_free(lu);
+    free(x); free(x_old); free(dx); free(g); free(cp); free(rhs); free(q); free(q1); free(ip);
+    lane_dev_free(&ld);
+}
+
+static void *run_tjob(void *arg)
+{
+    tjob_t *j = arg;
+    for (size_t l = j->first; l < j->first + j->count; l++) tran_lane(j, l);
+    return NULL;
+}
+
+int orc_tran_solve(orc_ckt *c, const double *values, size_t count, const orc_tol *tol_op, const orc_tol *tol,
+                   double dt, uint32_t steps, const uint32_t *probes, uint32_t n_probes, const uint32_t *q,
+                   int threads, double *x, double *probe_out, uint32_t *iterations, uint32_t *reasons)
+{
+    if (count == 0) return 0;
+    const uint32_t n = c->n, nnz = c->nnz;
+    double *x_op = calloc(count * n + 1, 8);
+    uint32_t *op_it = calloc(count + 1, 4), *op_rs = calloc(count + 1, 4);
+    orc_op_solve(c, values, count, tol_op, q, NULL, threads, x_op, op_it, op_rs, NULL);
+    size_t l0 = count;
+    for (size_t l = 0; l < count; l++)
+        if (op_rs[l] == 0) { l0 = l; break; }
+    uint8_t *has_c = calloc(nnz + 1, 1);
+    for (uint32_t i = 0; i < c->n_inst; i++) {
+        const orc_inst *in = &c->inst[i];
+        if (in->kind != K_C && in->kind != K_D) continue;
+        for (int e = 0; e < 4; e++)
+            if (in->slot[e] != TRASH) has_c[in->slot[e]] = 1;
+    }
+    int rc = 0;
+    orc_lu *lead = NULL;
+    if (l0 < count) {
+        /* the transient lead tape: point l0's first transient Jacobian */
+        lane_dev ld;
+        lane_dev_init(c, values + l0 * c->d.n_live, &ld);
+        double *xx = malloc((size_t)(n + 1) * 8), *g = calloc(nnz + 1, 8), *cp = calloc(nnz + 1, 8);
+        double *rhs = calloc(n + 1, 8), *qq = calloc(n + 1, 8), *q1 = calloc(n + 1, 8), *ip = calloc(n + 1, 8);
+        memcpy(xx, x_op + l0 * n, (size_t)n * 8);
+        seed_at(c, &ld, xx);
+        assemble_t(c, &ld, xx, 0.0, 1, g, rhs, cp, qq);
+        memcpy(q1, qq, (size_t)n * 8);
+        assemble_t(c, &ld, xx, dt, 1, g, rhs, cp, qq);
+        companion(c, has_c, 1.0 / dt, 0.0, g, cp, rhs, qq, q1, ip);
+        lead = orc_lu_new(n, nnz);
+        if (orc_lu_factor(lead, c->col_ptr, c->row_idx, g, q, 1e-3) != 0) {
+            orc_lu_free(lead);
+            lead = NULL;
+        }
+        free(xx); free(g); free(cp); free(rhs); free(qq); free(q1); free(ip);
+        lane_dev_free(&ld);
+    }
+    if (!lead) {
+        for (size_t l = 0; l < count; l++) {
+            memcpy(x + l * n, x_op + l * n, (size_t)n * 8);
+            iterations[l] = 0;
+            reasons[l] = op_rs[l] != 0 ? op_rs[l] : 3;
+        }
+        rc = -1;
+    } else {
+        if (threads < 1) threads = 1;
+        if ((size_t)threads > count) threads = (int)count;
+        tjob_t *jobs = calloc(threads, sizeof(tjob_t));
+        pthread_t *th = calloc(threads, sizeof(pthread_t));
+        size_t per = count / threads, rem = count % threads, at = 0;
+        for (int t = 0; t < threads; t++) {
+            size_t k = per + ((size_t)t < rem ? 1 : 0);
+            jobs[t] = (tjob_t){c, values, at, k, tol, dt, steps, n_probes, probes, lead, q, has_c, x_op, op_rs,
+                               x, probe_out, iterations, reasons};
+            at += k;
+        }
+        if (threads == 1) run_tjob(&jobs[0]);
+        else {
+            for (int t = 0; t < threads; t++) pthread_create(&th[t], NULL, run_tjob, &jobs[t]);
+            for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+        }
+        free(jobs); free(th);
+        orc_lu_free(lead);
+    }
+    free(x_op); free(op_it); free(op_rs); free(has_c);
+    return rc;
 }
