@@ -12,6 +12,8 @@ Families:
   diode_stack     a resistor into a stack of junction diodes, .op
   rc_mesh         k x k RC mesh with diode clamps and a step source, the
                   transient workload (every R, C and IS live)
+  diffpair_array  64 Gummel-Poon differential pairs, .op (the FP32 +
+                  refinement workload)
 """
 from __future__ import annotations
 
@@ -70,6 +72,8 @@ class Family:
                 out[:, col] = a * (1.0 + b * normal(seed, idx, var))
             elif kind == "uniform_of":
                 out[:, col] = uniform01(seed, idx, var) * out[:, self.deck.live.index(a)]
+            elif kind == "log_normal":
+                out[:, col] = a * np.exp(b * normal(seed, idx, var))
             elif kind == "log_uniform":
                 la, lb = np.log(a), np.log(b)
                 out[:, col] = np.exp(la + (lb - la) * uniform01(seed, idx, var))
@@ -175,6 +179,38 @@ def diode_stack(stages: int = 8) -> Family:
     return Family(d.title, d, draws)
 
 
+def diffpair_array(count: int = 64) -> Family:
+    """`count` Gummel-Poon NPN differential pairs (128 BJTs at 64) with
+    resistive loads and tail current sources between VCC = 5 V and VEE = -5 V;
+    pair k is driven differentially by +-vd_k/2 with vd_k spread over
+    [-0.1, 0.1] V (the reference's bjt_diffpair decks). Per point: each BJT's
+    IS ~ log-normal (median 1e-16 A, sigma_ln 0.2) and BF ~ N(100, 10), each
+    load R ~ N(10 kOhm, 1%). Tail 0.2 mA: the loads drop about 1 V."""
+    d = Deck(f"diffpair_array_{count}")
+    d.vsource("vcc", "vcc", 0, 5.0)
+    d.vsource("vee", "vee", 0, -5.0)
+    qm = {"type": 1, "is": 1e-16, "bf": 100.0, "vaf": 100.0}
+    draws = []
+    for k in range(count):
+        vd = -0.1 + 0.2 * k / max(count - 1, 1)
+        d.vsource(f"vp{k}", f"bp{k}", 0, 0.5 * vd)
+        d.vsource(f"vn{k}", f"bn{k}", 0, -0.5 * vd)
+        d.resistor(f"rp{k}", "vcc", f"cp{k}", 10e3)
+        d.resistor(f"rn{k}", "vcc", f"cn{k}", 10e3)
+        d.isource(f"it{k}", f"e{k}", "vee", 0.2e-3)
+        d.bjt(f"q1_{k}", f"cp{k}", f"bp{k}", f"e{k}", 0, qm)
+        d.bjt(f"q2_{k}", f"cn{k}", f"bn{k}", f"e{k}", 0, qm)
+        for q in (f"q1_{k}", f"q2_{k}"):
+            d.bind(q, "is", f"is_{q}")
+            d.bind(q, "bf", f"bf_{q}")
+            draws.append((f"is_{q}", "log_normal", 1e-16, 0.2))
+            draws.append((f"bf_{q}", "normal", 100.0, 10.0))
+        for r in (f"rp{k}", f"rn{k}"):
+            d.bind(r, 0, r)
+            draws.append((r, "rel_normal", 10e3, 0.01))
+    return Family(d.title, d, draws)
+
+
 def rc_mesh(k: int = 64, diode_frac: float = 0.05, layout_seed: int = 1) -> Family:
     """k x k RC mesh (5-point stencil), every edge resistor R ~ log-uniform
     [100, 1e4] Ohm, every node a grounded C ~ log-uniform [1 fF, 100 fF],
@@ -216,4 +252,4 @@ def rc_mesh(k: int = 64, diode_frac: float = 0.05, layout_seed: int = 1) -> Fami
 
 
 FAMILIES = {"inverter_chain": inverter_chain, "ota_array": ota_array, "diode_stack": diode_stack,
-            "rc_mesh": rc_mesh}
+            "rc_mesh": rc_mesh, "diffpair_array": diffpair_array}

@@ -348,4 +348,133 @@ __device__ inline void dio_eval_v(const Dio &d, double xa, double xk, double vl,
     }
 }
 
+// ---- Gummel-Poon BJT (models/gummel_poon.va, DC part with RB = RC = RE = 0,
+// no excess phase; vertical substrate) ----
+struct Bjt {
+    double tp, is, bf, br, vt, nfvt, nrvt, ise, nevt, isc, ncvt, iss, nsvt, inv_vaf, inv_var, inv_ikf, inv_ikr, nkf,
+        mgmin, vert, vcrit;
+};
+
+__device__ inline Bjt bjt_load(const DeckProg &dp, uint32_t j, const double *live, uint32_t live_stride = 1)
+{
+    const uint32_t nq = dp.n_bjt;
+    const double *f = dp.q_par;
+    Bjt b;
+    b.tp = f[QF_TYPE * nq + j];
+    const int bi = dp.q_bind_is[j], bb = dp.q_bind_bf[j];
+    const double isx = bi >= 0 ? live[(size_t)bi * live_stride] : f[QF_IS * nq + j];
+    const double bfx = bb >= 0 ? live[(size_t)bb * live_stride] : f[QF_BF * nq + j];
+    b.is = isx * f[QF_IS_SCALE * nq + j];
+    b.bf = bfx * f[QF_BFAC * nq + j];
+    b.br = f[QF_BR_T * nq + j];
+    b.vt = f[QF_VT * nq + j];
+    b.nfvt = f[QF_NFVT * nq + j];
+    b.nrvt = f[QF_NRVT * nq + j];
+    b.ise = f[QF_ISE * nq + j];
+    b.nevt = f[QF_NEVT * nq + j];
+    b.isc = f[QF_ISC * nq + j];
+    b.ncvt = f[QF_NCVT * nq + j];
+    b.iss = f[QF_ISS * nq + j];
+    b.nsvt = f[QF_NSVT * nq + j];
+    b.inv_vaf = f[QF_INV_VAF * nq + j];
+    b.inv_var = f[QF_INV_VAR * nq + j];
+    b.inv_ikf = f[QF_INV_IKF * nq + j];
+    b.inv_ikr = f[QF_INV_IKR * nq + j];
+    b.nkf = f[QF_NKF * nq + j];
+    b.mgmin = f[QF_MGMIN * nq + j];
+    b.vert = f[QF_VERT * nq + j];
+    b.vcrit = b.vt * esl::log(b.vt / (1.4142135623730951 * b.is));
+    return b;
+}
+
+// Junction law with the 80 vtn tangent and the cubic reverse tail (the
+// model's pnj); g = di/dv.
+__device__ inline double bjt_pnj(double v, double isat, double vtn, double &g)
+{
+    if (v >= -3.0 * vtn) {
+        const double u = fmin(v / vtn, 80.0);
+        const double e = esl::exp(u);
+        g = isat * e / vtn;
+        return isat * (e * (1.0 + (v / vtn - u)) - 1.0);
+    }
+    const double u = 3.0 * vtn / (2.718281828459045 * v);
+    const double u3 = u * u * u;
+    g = 3.0 * isat * u3 / v;
+    return -isat * (1.0 + u3);
+}
+
+// pnjlim of vbe and vbc (npn frame) against the previous limited values,
+// with the snap to the probe; true when either moved.
+__device__ inline bool bjt_limit_v(const Bjt &b, double xc, double xb, double xe, double &lbe, double &lbc)
+{
+    const double vbe = b.tp * (xb - xe), vbc = b.tp * (xb - xc);
+    double ve = pnjlim(vbe, lbe, b.vt, b.vcrit);
+    double vc = pnjlim(vbc, lbc, b.vt, b.vcrit);
+    const double tol = 16.0 * 2.220446049250313e-16 * fmax(fmax(fabs(xc), fabs(xb)), fabs(xe));
+    if (fabs(ve - vbe) <= tol) ve = vbe;
+    if (fabs(vc - vbc) <= tol) vc = vbc;
+    lbe = ve;
+    lbc = vc;
+    return ve != vbe || vc != vbc;
+}
+
+// DC stamps at the limited (vbe, vbc) with the residual correction toward
+// the probe. out[0..11]: Jacobian (c,c) (c,b) (c,e) (c,s) (b,c) (b,b) (b,e)
+// (e,c) (e,b) (e,e) (s,c) (s,s); out[12..15]: residual rows c, b, e, s.
+__device__ inline void bjt_eval_v(const Bjt &b, double xc, double xb, double xe, double xs, double vbe, double vbc,
+                                  double *out)
+{
+    const double tp = b.tp;
+    const double dbe = tp * (xb - xe) - vbe, dbc = tp * (xb - xc) - vbc;
+    const double vsub = tp * (xs - xc);
+    double gbe, gben, gbc, gbcn, gsub;
+    const double cbe = bjt_pnj(vbe, b.is, b.nfvt, gbe);
+    const double cben = bjt_pnj(vbe, b.ise, b.nevt, gben) + b.mgmin * vbe;
+    gben = gben + b.mgmin;
+    const double cbc = bjt_pnj(vbc, b.is, b.nrvt, gbc);
+    const double cbcn = bjt_pnj(vbc, b.isc, b.ncvt, gbcn) + b.mgmin * vbc;
+    gbcn = gbcn + b.mgmin;
+    const double cdsub = bjt_pnj(vsub, b.iss, b.nsvt, gsub) + b.mgmin * vsub;
+    gsub = gsub + b.mgmin;
+    const double q1 = 1.0 / (1.0 - vbc * b.inv_vaf - vbe * b.inv_var);
+    const double q2 = cbe * b.inv_ikf + cbc * b.inv_ikr;
+    const double arg = fmax(1e-30, 1.0 + 4.0 * q2);
+    const double p = b.nkf == 0.5 ? sqrt(arg) : esl::exp(b.nkf * esl::log(arg));
+    const double qb = 0.5 * q1 * (1.0 + p);
+    const double dp_dq2 = (1.0 + 4.0 * q2 > 1e-30) ? 4.0 * b.nkf * p / arg : 0.0;
+    const double dqb_be = 0.5 * (q1 * q1 * b.inv_var * (1.0 + p) + q1 * dp_dq2 * (gbe * b.inv_ikf));
+    const double dqb_bc = 0.5 * (q1 * q1 * b.inv_vaf * (1.0 + p) + q1 * dp_dq2 * (gbc * b.inv_ikr));
+    double icc = cbe / qb - cbc / qb;
+    const double dcc = (cbe - cbc) / (qb * qb);
+    const double dicc_be = gbe / qb - dcc * dqb_be;
+    const double dicc_bc = -gbc / qb - dcc * dqb_bc;
+    double ibe = cbe / b.bf + cben;
+    const double gbe_t = gbe / b.bf + gben;
+    double ibc = cbc / b.br + cbcn;
+    const double gbc_t = gbc / b.br + gbcn;
+    const double isc_ = b.vert * cdsub;
+    const double gs = b.vert * gsub;
+    if (dbe != 0 || dbc != 0) {
+        icc = icc + (dicc_be * dbe + dicc_bc * dbc);
+        ibe = ibe + gbe_t * dbe;
+        ibc = ibc + gbc_t * dbc;
+    }
+    out[0] = (-dicc_bc + gbc_t) + gs;
+    out[1] = (dicc_be + dicc_bc) - gbc_t;
+    out[2] = -dicc_be;
+    out[3] = -gs;
+    out[4] = -gbc_t;
+    out[5] = gbe_t + gbc_t;
+    out[6] = -gbe_t;
+    out[7] = dicc_bc;
+    out[8] = -(dicc_be + dicc_bc) - gbe_t;
+    out[9] = dicc_be + gbe_t;
+    out[10] = -gs;
+    out[11] = gs;
+    out[12] = (tp * icc - tp * ibc) - tp * isc_;
+    out[13] = tp * ibe + tp * ibc;
+    out[14] = -tp * icc - tp * ibe;
+    out[15] = tp * isc_;
+}
+
 }  // namespace esl

@@ -25,20 +25,26 @@ struct uint2_t {
 };
 
 constexpr int kWave = 64;
-constexpr int kStageEntries = 14;   // most cells one instance stages (MOS1)
+constexpr int kStageEntries = 16;   // most cells one instance stages (Gummel-Poon: 12 + 4)
 constexpr uint32_t kRhsBit = 0x80000000u;   // task target is a residual row
 constexpr uint32_t kCBit = 0x40000000u;     // ... a charge-Jacobian slot (transient only)
 constexpr uint32_t kQBit = 0x20000000u;     // ... a charge row (transient only)
 constexpr uint32_t kTargetMask = 0x1fffffffu;
 constexpr uint32_t kVoidBit = 1u, kScaledBit = 2u;
 
-enum Kind : uint32_t { kR = 0, kC = 1, kV = 2, kI = 3, kD = 4, kMos = 5, kKinds = 6 };
+enum Kind : uint32_t { kR = 0, kC = 1, kV = 2, kI = 3, kD = 4, kMos = 5, kQ = 6, kKinds = 7 };
 
 // MOS1 per-instance constants (structure of arrays, index [field * n_mos + j]).
 enum MosField : uint32_t {
     MF_TYPE = 0, MF_VTO, MF_VBIT_A, MF_VBIT_B, MF_VBIT_C, MF_GAMMA, MF_PHIT,
     MF_LAMBDA, MF_VT, MF_ISATS, MF_ISATD, MF_VCRITS, MF_VCRITD, MF_GMIN,
     MF_KP, MF_RATIO4, MF_M, MF_W, MF_LEFF, MF_COUNT
+};
+// Gummel-Poon fields (prepared on the host): is_t * area = IS * QF_IS_SCALE,
+// bf_t = BF * QF_BFAC.
+enum BjtField : uint32_t {
+    QF_TYPE = 0, QF_IS, QF_IS_SCALE, QF_BF, QF_BFAC, QF_BR_T, QF_VT, QF_NFVT, QF_NRVT, QF_ISE, QF_NEVT, QF_ISC,
+    QF_NCVT, QF_ISS, QF_NSVT, QF_INV_VAF, QF_INV_VAR, QF_INV_IKF, QF_INV_IKR, QF_NKF, QF_MGMIN, QF_VERT, QF_COUNT
 };
 // Diode fields (prepared on the host): is_t = IS * DF_IS_SCALE.
 enum DioField : uint32_t {
@@ -77,6 +83,9 @@ struct DeckProg {
     // diodes: t[2] (a, k), fields [DF_COUNT][n_dio], binding of IS
     uint32_t n_dio, n_cap, n_cslot, pad1;
     const uint32_t *d_t; const double *d_par; const int32_t *d_bind_is;
+    // Gummel-Poon BJTs: t[4] (c, b, e, s), fields [QF_COUNT][n_bjt], bindings of IS and BF
+    uint32_t n_bjt, pad2;
+    const uint32_t *q_t; const double *q_par; const int32_t *q_bind_is, *q_bind_bf;
     // capacitors: t[2], value, bind
     const uint32_t *c_t; const double *c_val; const int32_t *c_bind;
     // V source waveforms: kind (0 DC, 1 step), v2, td per source

@@ -696,11 +696,13 @@ hipError_t upload_tape_prog(TapeHostProg &o)
 // ---------------------------------------------------------------- deck compile
 struct Inst {
     u32 kind, dev, t[4], br;
-    u32 slot[11];
+    u32 slot[12];
 };
 
 const u32 R_ENT[4][2] = {{0, 0}, {0, 1}, {1, 0}, {1, 1}};
 const u32 MOS_ENT[11][2] = {{0, 0}, {0, 1}, {0, 2}, {0, 3}, {2, 0}, {2, 1}, {2, 2}, {2, 3}, {3, 0}, {3, 2}, {3, 3}};
+// Gummel-Poon entries (row, col) in terminals (c, b, e, s)
+const u32 BJT_ENT[12][2] = {{0, 0}, {0, 1}, {0, 2}, {0, 3}, {1, 0}, {1, 1}, {1, 2}, {2, 0}, {2, 1}, {2, 2}, {3, 0}, {3, 3}};
 
 constexpr double KB = 1.38064852e-23;
 constexpr double QE = 1.6021766208e-19;
@@ -775,10 +777,10 @@ struct Compiled {
     vector<int32_t> r_bind, v_bind, i_bind, m_bind_vto, m_bind_kp;
     vector<u32> r_slot, v_slot, m_slot;
     vector<u32> csr_ptr, csr_col, csr_slot;   // the pattern by rows, columns ascending
-    vector<u32> d_t, c_t, v_wkind, cslot_pat;
-    vector<double> d_par, c_val, v_v2, v_td;
-    vector<int32_t> d_bind_is, c_bind;
-    u32 n_mos = 0, n_dio = 0, n_cap = 0;
+    vector<u32> d_t, c_t, v_wkind, cslot_pat, q_t;
+    vector<double> d_par, c_val, v_v2, v_td, q_par;
+    vector<int32_t> d_bind_is, c_bind, q_bind_is, q_bind_bf;
+    u32 n_mos = 0, n_dio = 0, n_cap = 0, n_bjt = 0;
 };
 
 void build_csr(Compiled &c)
@@ -796,6 +798,45 @@ void build_csr(Compiled &c)
             c.csr_col[e] = j;
             c.csr_slot[e] = p;
         }
+}
+
+// Gummel-Poon prep (models/gummel_poon.va:158-242, bjttemp.c): the fields
+// of esl::BjtField. is_t * area = IS * is_scale and bf_t = BF * bfac, so
+// per-point IS and BF scale.
+void bjt_fields(const double *p, double *f)
+{
+    const double tdev = p[ESL_Q_TEMP] + 273.15, tnom = p[ESL_Q_TNOM] + 273.15;
+    const double vt = KB / QE * tdev;
+    const double ratlog = esl::log(tdev / tnom);
+    const double factlog = (tdev / tnom - 1.0) * p[ESL_Q_EG] / vt + p[ESL_Q_XTI] * ratlog;
+    const double factor = esl::exp(factlog);
+    const double bfac = esl::exp(p[ESL_Q_XTB] * ratlog);
+    const double m = p[ESL_Q_M], ai = p[ESL_Q_AREA] * m;
+    const double vert = p[ESL_Q_SUBS] > 0 ? 1.0 : 0.0;
+    const double abc = m * (vert * p[ESL_Q_AREAB] + (1.0 - vert) * p[ESL_Q_AREAC]);
+    const double ne = p[ESL_Q_NE], nc = p[ESL_Q_NC];
+    f[esl::QF_TYPE] = p[ESL_Q_TYPE];
+    f[esl::QF_IS] = p[ESL_Q_IS];
+    f[esl::QF_IS_SCALE] = factor * ai;
+    f[esl::QF_BF] = p[ESL_Q_BF];
+    f[esl::QF_BFAC] = bfac;
+    f[esl::QF_BR_T] = p[ESL_Q_BR] * bfac;
+    f[esl::QF_VT] = vt;
+    f[esl::QF_NFVT] = p[ESL_Q_NF] * vt;
+    f[esl::QF_NRVT] = p[ESL_Q_NR] * vt;
+    f[esl::QF_ISE] = p[ESL_Q_ISE] * esl::exp(factlog / ne) / bfac * ai;
+    f[esl::QF_NEVT] = ne * vt;
+    f[esl::QF_ISC] = p[ESL_Q_ISC] * esl::exp(factlog / nc) / bfac * abc;
+    f[esl::QF_NCVT] = nc * vt;
+    f[esl::QF_ISS] = p[ESL_Q_ISS] * factor * ai;
+    f[esl::QF_NSVT] = p[ESL_Q_NS] * vt;
+    f[esl::QF_INV_VAF] = p[ESL_Q_VAF] > 0 ? 1.0 / p[ESL_Q_VAF] : 0.0;
+    f[esl::QF_INV_VAR] = p[ESL_Q_VAR] > 0 ? 1.0 / p[ESL_Q_VAR] : 0.0;
+    f[esl::QF_INV_IKF] = p[ESL_Q_IKF] > 0 ? 1.0 / (p[ESL_Q_IKF] * ai) : 0.0;
+    f[esl::QF_INV_IKR] = p[ESL_Q_IKR] > 0 ? 1.0 / (p[ESL_Q_IKR] * ai) : 0.0;
+    f[esl::QF_NKF] = p[ESL_Q_NKF];
+    f[esl::QF_MGMIN] = m * p[ESL_Q_GMIN];
+    f[esl::QF_VERT] = vert;
 }
 
 // Diode prep (models/diode.va, TLEV 0 / TLEVC 0 temperature map): the
@@ -868,7 +909,8 @@ bool compile_deck(const esl_deck *d, Compiled &c, std::string &err)
             if (bind[i][p] < 0) continue;
             bool ok = (k == esl::kR || k == esl::kC || k == esl::kV || k == esl::kI) ? p == 0
                       : (k == esl::kMos) ? (p == ESL_M_VTO || p == ESL_M_KP)
-                      : (k == esl::kD) ? p == ESL_D_IS : false;
+                      : (k == esl::kD) ? p == ESL_D_IS
+                      : (k == esl::kQ) ? (p == ESL_Q_IS || p == ESL_Q_BF) : false;
             if (k == esl::kMos && p == ESL_M_KP) {
                 const double *pp = d->params + (size_t)i * NPARAM;
                 const unsigned given = (unsigned)pp[ESL_M_GIVEN];
@@ -883,6 +925,16 @@ bool compile_deck(const esl_deck *d, Compiled &c, std::string &err)
                 !(pp[ESL_D_FC] >= 0 && pp[ESL_D_FC] < 1) || !(pp[ESL_D_M] > 0 && pp[ESL_D_M] < 1) ||
                 !(pp[ESL_D_CJO] >= 0) || !(pp[ESL_D_TT] >= 0) || !(pp[ESL_D_GMIN] >= 0)) {
                 err = "diode parameter out of range";
+                return false;
+            }
+        }
+        if (k == esl::kQ) {
+            const double *pp = d->params + (size_t)i * NPARAM;
+            if ((pp[ESL_Q_TYPE] != 1.0 && pp[ESL_Q_TYPE] != -1.0) || pp[ESL_Q_SUBS] != 1.0 || !(pp[ESL_Q_IS] > 0) ||
+                !(pp[ESL_Q_BF] > 0) || !(pp[ESL_Q_BR] > 0) || !(pp[ESL_Q_NF] > 0) || !(pp[ESL_Q_NR] > 0) ||
+                !(pp[ESL_Q_NE] > 0) || !(pp[ESL_Q_NC] > 0) || !(pp[ESL_Q_NS] > 0) || !(pp[ESL_Q_AREA] > 0) ||
+                !(pp[ESL_Q_M] > 0) || !(pp[ESL_Q_NKF] > 0 && pp[ESL_Q_NKF] <= 1)) {
+                err = "BJT parameter out of range (type +-1, vertical substrate, positive IS/BF/BR/N*/AREA/M)";
                 return false;
             }
         }
@@ -923,6 +975,8 @@ bool compile_deck(const esl_deck *d, Compiled &c, std::string &err)
             ent.push_back({in.t[1], in.br});
         } else if (in.kind == esl::kMos)
             for (auto &e : MOS_ENT) ent.push_back({in.t[e[1]], in.t[e[0]]});
+        else if (in.kind == esl::kQ)
+            for (auto &e : BJT_ENT) ent.push_back({in.t[e[1]], in.t[e[0]]});
     }
     vector<std::pair<u32, u32>> keep;
     for (auto &e : ent)
@@ -951,6 +1005,8 @@ bool compile_deck(const esl_deck *d, Compiled &c, std::string &err)
             in.slot[3] = find_slot(c, in.br, in.t[1]);
         } else if (in.kind == esl::kMos)
             for (int e = 0; e < 11; e++) in.slot[e] = find_slot(c, in.t[MOS_ENT[e][0]], in.t[MOS_ENT[e][1]]);
+        else if (in.kind == esl::kQ)
+            for (int e = 0; e < 12; e++) in.slot[e] = find_slot(c, in.t[BJT_ENT[e][0]], in.t[BJT_ENT[e][1]]);
     }
     c.q = min_degree(n, c.col_ptr, c.row_idx);
 
@@ -982,6 +1038,11 @@ bool compile_deck(const esl_deck *d, Compiled &c, std::string &err)
             c.d_t.push_back(in.t[0]); c.d_t.push_back(in.t[1]);
             c.d_bind_is.push_back(bd[ESL_D_IS]);
             break;
+        case esl::kQ:
+            for (int t = 0; t < 4; t++) c.q_t.push_back(in.t[t]);
+            c.q_bind_is.push_back(bd[ESL_Q_IS]);
+            c.q_bind_bf.push_back(bd[ESL_Q_BF]);
+            break;
         case esl::kMos:
             for (int t = 0; t < 4; t++) c.m_t.push_back(in.t[t]);
             for (int e = 0; e < 11; e++) c.m_slot.push_back(in.slot[e]);
@@ -995,6 +1056,14 @@ bool compile_deck(const esl_deck *d, Compiled &c, std::string &err)
     c.n_mos = c.kind_first[esl::kMos + 1] - c.kind_first[esl::kMos];
     c.n_dio = c.kind_first[esl::kD + 1] - c.kind_first[esl::kD];
     c.n_cap = c.kind_first[esl::kC + 1] - c.kind_first[esl::kC];
+    c.n_bjt = c.kind_first[esl::kQ + 1] - c.kind_first[esl::kQ];
+    c.q_par.assign((size_t)esl::QF_COUNT * c.n_bjt, 0.0);
+    for (u32 j = 0; j < c.n_bjt; j++) {
+        const Inst &in = c.inst[c.kind_first[esl::kQ] + j];
+        double f[esl::QF_COUNT];
+        bjt_fields(d->params + (size_t)in.dev * NPARAM, f);
+        for (u32 k = 0; k < esl::QF_COUNT; k++) c.q_par[(size_t)k * c.n_bjt + j] = f[k];
+    }
     c.d_par.assign((size_t)esl::DF_COUNT * c.n_dio, 0.0);
     for (u32 j = 0; j < c.n_dio; j++) {
         const Inst &in = c.inst[c.kind_first[esl::kD] + j];
@@ -1068,6 +1137,9 @@ bool compile_deck(const esl_deck *d, Compiled &c, std::string &err)
                     row(11, in.t[0]);
                     row(12, in.t[2]);
                     row(13, in.t[3]);
+                } else if (k == esl::kQ) {
+                    for (u32 ei = 0; ei < 12; ei++) jac(ei, in.slot[ei]);
+                    for (u32 r = 0; r < 4; r++) row(12 + r, in.t[r]);
                 }
             }
             std::map<u32, vector<u32>> groups;
@@ -1146,7 +1218,8 @@ hipError_t upload_deck(Engine &e)
     size_t o_cp = pk.add(c.csr_ptr), o_cc = pk.add(c.csr_col), o_cs = pk.add(c.csr_slot);
     size_t o_dt = pk.add(c.d_t), o_dp = pk.add(c.d_par), o_db = pk.add(c.d_bind_is), o_ct = pk.add(c.c_t),
            o_cv = pk.add(c.c_val), o_cb = pk.add(c.c_bind), o_vw = pk.add(c.v_wkind), o_v2 = pk.add(c.v_v2),
-           o_vtd = pk.add(c.v_td), o_csp = pk.add(c.cslot_pat);
+           o_vtd = pk.add(c.v_td), o_csp = pk.add(c.cslot_pat), o_qt = pk.add(c.q_t), o_qp = pk.add(c.q_par),
+           o_qbi = pk.add(c.q_bind_is), o_qbb = pk.add(c.q_bind_bf);
     hipError_t err = e.deck_buf.ensure(pk.blob.size());
     if (err != hipSuccess) return err;
     err = hipMemcpy(e.deck_buf.p, pk.blob.data(), pk.blob.size(), hipMemcpyHostToDevice);
@@ -1199,6 +1272,11 @@ hipError_t upload_deck(Engine &e)
     dp.v_v2 = reinterpret_cast<const double *>(b + o_v2);
     dp.v_td = reinterpret_cast<const double *>(b + o_vtd);
     dp.cslot_pat = reinterpret_cast<const u32 *>(b + o_csp);
+    dp.n_bjt = c.n_bjt;
+    dp.q_t = reinterpret_cast<const u32 *>(b + o_qt);
+    dp.q_par = reinterpret_cast<const double *>(b + o_qp);
+    dp.q_bind_is = reinterpret_cast<const int32_t *>(b + o_qbi);
+    dp.q_bind_bf = reinterpret_cast<const int32_t *>(b + o_qbb);
     return hipSuccess;
 }
 
@@ -1362,6 +1440,8 @@ bool validate_deck(const Compiled &c)
     if (c.d_t.size() != 2 * (size_t)c.n_dio || c.c_t.size() != 2 * (size_t)c.n_cap) return false;
     if (c.d_par.size() != (size_t)esl::DF_COUNT * c.n_dio || c.d_bind_is.size() != c.n_dio) return false;
     if (c.c_val.size() != c.n_cap || c.c_bind.size() != c.n_cap) return false;
+    if (c.q_t.size() != 4 * (size_t)c.n_bjt || c.q_par.size() != (size_t)esl::QF_COUNT * c.n_bjt) return false;
+    if (!terms_ok(c.q_t)) return false;
     auto binds_ok = [&](const vector<int32_t> &b) {
         for (int32_t v : b) if (v >= (int32_t)c.n_live) return false;
         return true;
@@ -1372,7 +1452,8 @@ bool validate_deck(const Compiled &c)
     };
     if (!slots_ok(c.r_slot) || !slots_ok(c.v_slot) || !slots_ok(c.m_slot)) return false;
     return binds_ok(c.r_bind) && binds_ok(c.v_bind) && binds_ok(c.i_bind) && binds_ok(c.m_bind_vto) &&
-           binds_ok(c.m_bind_kp) && binds_ok(c.d_bind_is) && binds_ok(c.c_bind);
+           binds_ok(c.m_bind_kp) && binds_ok(c.d_bind_is) && binds_ok(c.c_bind) && binds_ok(c.q_bind_is) &&
+           binds_ok(c.q_bind_bf);
 }
 
 u32 device_lds_max()
@@ -1683,7 +1764,7 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
     HIPCHK(e->live.ensure(std::max<size_t>(1, count * P) * 8));
     HIPCHK(e->live_t.ensure(std::max<size_t>(1, count * P) * 8));
     HIPCHK(e->x.ensure(count * n * 8));
-    HIPCHK(e->lim.ensure(std::max<size_t>(1, count * (4 * c.n_mos + c.n_dio)) * 8));
+    HIPCHK(e->lim.ensure(std::max<size_t>(1, count * (4 * c.n_mos + c.n_dio + 2 * c.n_bjt)) * 8));
     HIPCHK(e->resume.ensure(count * sizeof(esl::LaneResume)));
     HIPCHK(e->status.ensure(count * 4));
     HIPCHK(e->iters.ensure(count * 4));
@@ -1794,7 +1875,7 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
     // (kernel time of the first launch over every lane) and keeps the faster;
     // below that size a batch runs one wavefront per lane.
     // the thread tiling covers R, V, I and MOS1 decks in FP64
-    const bool wave_only = fp32 || net || c.n_dio > 0;
+    const bool wave_only = fp32 || net || c.n_dio > 0 || c.n_bjt > 0;
     if (wave_only && o.tiling == ESL_TILING_THREAD) return ESL_INVALID_ARGUMENT;
     u32 tiling = wave_only ? (u32)ESL_TILING_WAVE : o.tiling;
     if (tiling == ESL_TILING_AUTO) {

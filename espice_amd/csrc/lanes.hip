@@ -45,6 +45,11 @@ __device__ inline double wave_fmax(double v)
 
 __device__ inline bool wave_any(bool p) { return __any((int)p) != 0; }
 
+// Per-lane limiting state: MOS1 (vgs, vds, vbs, von) as [4][n_mos], then one
+// junction voltage per diode, then (vbe, vbc) per BJT.
+__device__ inline uint32_t lim_width(const DeckProg &dp) { return 4 * dp.n_mos + dp.n_dio + 2 * dp.n_bjt; }
+__device__ inline uint32_t lim_bjt(const DeckProg &dp) { return 4 * dp.n_mos + dp.n_dio; }
+
 // Wrappers over the shared physics for the wavefront kernel: limiting state
 // per lane at lim[j + k * stride] (vgs, vds, vbs, von), cells staged at
 // st[e * 64 + t].
@@ -136,6 +141,15 @@ __device__ inline void eval_chunk(const DeckProg &dp, const Chunk &ch, const dou
         const Mos m = mos_load(dp, j, live);
         const uint32_t *t4 = dp.m_t + 4 * j;
         mos_eval(m, x[t4[0]], x[t4[1]], x[t4[2]], x[t4[3]], lim + j, lim_stride, st, tid);
+        break;
+    }
+    case kQ: {
+        const Bjt b = bjt_load(dp, j, live);
+        const uint32_t *t4 = dp.q_t + 4 * j;
+        const double *ql = lim + lim_bjt(dp) + 2 * j;
+        double out[16];
+        bjt_eval_v(b, x[t4[0]], x[t4[1]], x[t4[2]], x[t4[3]], ql[0], ql[1], out);
+        for (int e = 0; e < 16; e++) st[e * kWave + tid] = out[e];
         break;
     }
     default:
@@ -401,6 +415,12 @@ __device__ inline void seed_lim_at(const DeckProg &dp, const double *live, const
         lim[3 * nm + j] = 0.0;
     }
     for (uint32_t j = tid; j < dp.n_dio; j += kWave) lim[4 * nm + j] = x[dp.d_t[2 * j]] - x[dp.d_t[2 * j + 1]];
+    for (uint32_t j = tid; j < dp.n_bjt; j += kWave) {
+        const uint32_t *t4 = dp.q_t + 4 * j;
+        const double tp = dp.q_par[QF_TYPE * dp.n_bjt + j];
+        lim[lim_bjt(dp) + 2 * j] = tp * (x[t4[1]] - x[t4[2]]);
+        lim[lim_bjt(dp) + 2 * j + 1] = tp * (x[t4[1]] - x[t4[0]]);
+    }
 }
 
 __device__ inline void lane_init(const OpArgs &a, uint32_t lane, const double *live, double *x, double *lim,
@@ -425,6 +445,10 @@ __device__ inline void lane_init(const OpArgs &a, uint32_t lane, const double *l
             lim[3 * nm + j] = 0.0;
         }
         for (uint32_t j = tid; j < dp.n_dio; j += kWave) lim[4 * nm + j] = dio_load(dp, j, live).vcrit;
+        for (uint32_t j = tid; j < dp.n_bjt; j += kWave) {
+            lim[lim_bjt(dp) + 2 * j] = bjt_load(dp, j, live).vcrit;
+            lim[lim_bjt(dp) + 2 * j + 1] = 0.0;
+        }
     } else {
         seed_lim_at(dp, live, x, lim, tid);
     }
@@ -446,7 +470,7 @@ __global__ __launch_bounds__(kWave) void op_dump(OpArgs a)
     double *rhs = tv + a.tv_cap + lds_y_block(n);
     double *x = rhs + n;
     const double *live = a.live + (size_t)lane * dp.n_live;
-    double *lim = a.lim + (size_t)lane * (4 * nm + dp.n_dio);
+    double *lim = a.lim + (size_t)lane * lim_width(dp);
     lane_init(a, lane, live, x, lim, tid);
     assemble_pattern(dp, live, x, lim, nm, tv, rhs, st, tid);
     for (uint32_t p = tid; p < dp.nnz; p += kWave) a.a_out[(size_t)blockIdx.x * dp.nnz + p] = tv[p];
@@ -471,7 +495,7 @@ __global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
     double *rhs = tv + a.tv_cap + lds_y_block(n);
     double *x = rhs + n;
     const double *live = a.live + (size_t)lane * dp.n_live;
-    double *lim = a.lim + (size_t)lane * (4 * nm + dp.n_dio);
+    double *lim = a.lim + (size_t)lane * lim_width(dp);
     lane_init(a, lane, live, x, lim, tid);
 
 
@@ -627,6 +651,12 @@ __global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
             const Dio d = dio_load(dp, j, live);
             limited |= dio_limit_v(d, x[dp.d_t[2 * j]], x[dp.d_t[2 * j + 1]], lim[4 * nm + j]);
         }
+        for (uint32_t j = tid; j < dp.n_bjt; j += kWave) {
+            const Bjt b = bjt_load(dp, j, live);
+            const uint32_t *t4 = dp.q_t + 4 * j;
+            double *ql = lim + lim_bjt(dp) + 2 * j;
+            limited |= bjt_limit_v(b, x[t4[0]], x[t4[1]], x[t4[2]], ql[0], ql[1]);
+        }
         limited = wave_any(limited);
         bool ok = !limited && iter != k0 && scaled < 1.0;
         if (ok) {
@@ -748,7 +778,7 @@ __global__ __launch_bounds__(kWave) void tran_steps(TranArgs ta)
     double *ip = q1 + n;
     double *cq = ip + n;
     const double *live = a.live + (size_t)lane * dp.n_live;
-    double *lim = a.lim + (size_t)lane * (4 * nm + dp.n_dio);
+    double *lim = a.lim + (size_t)lane * lim_width(dp);
     LaneResume rs = a.resume[lane];
     const TapeProg T = a.tapes[rs.tape];
     double *y = tv + T.n_tv;
@@ -849,6 +879,12 @@ __global__ __launch_bounds__(kWave) void tran_steps(TranArgs ta)
             for (uint32_t j = tid; j < dp.n_dio; j += kWave) {
                 const Dio d = dio_load(dp, j, live);
                 limited |= dio_limit_v(d, x[dp.d_t[2 * j]], x[dp.d_t[2 * j + 1]], lim[4 * nm + j]);
+            }
+            for (uint32_t j = tid; j < dp.n_bjt; j += kWave) {
+                const Bjt b = bjt_load(dp, j, live);
+                const uint32_t *t4 = dp.q_t + 4 * j;
+                double *ql = lim + lim_bjt(dp) + 2 * j;
+                limited |= bjt_limit_v(b, x[t4[0]], x[t4[1]], x[t4[2]], ql[0], ql[1]);
             }
             limited = wave_any(limited);
             bool ok = !limited && iter != 0 && scaled < 1.0;

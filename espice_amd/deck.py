@@ -14,8 +14,8 @@ import numpy as np
 NPARAM = 32
 NTERM = 4
 
-R, C, V, I, D, MOS1 = range(6)
-KIND_NAMES = {R: "R", C: "C", V: "V", I: "I", D: "D", MOS1: "MOS1"}
+R, C, V, I, D, MOS1, BJT = range(7)
+KIND_NAMES = {R: "R", C: "C", V: "V", I: "I", D: "D", MOS1: "MOS1", BJT: "BJT"}
 
 # MOS1 parameter slots (include/espice_lanes.h ESL_M_*), models/mos1.va:17-80.
 M_SLOTS = {
@@ -37,10 +37,20 @@ D_SLOTS = {"is": 0, "n": 1, "cjo": 2, "vj": 3, "m": 4, "tt": 5, "fc": 6, "area":
            "temp": 9, "tnom": 10, "eg": 11, "xti": 12}
 D_DEFAULTS = {"is": 1e-14, "n": 1.0, "cjo": 0.0, "vj": 1.0, "m": 0.5, "tt": 0.0, "fc": 0.5, "area": 1.0,
               "gmin": 1e-12, "temp": 27.0, "tnom": 27.0, "eg": 1.11, "xti": 3.0}
+# Gummel-Poon slots (ESL_Q_*), models/gummel_poon.va (RB = RC = RE = 0).
+Q_SLOTS = {"type": 0, "is": 1, "bf": 2, "nf": 3, "vaf": 4, "ikf": 5, "ise": 6, "ne": 7, "br": 8, "nr": 9,
+           "var": 10, "ikr": 11, "isc": 12, "nc": 13, "nkf": 14, "iss": 15, "ns": 16, "area": 17, "m": 18,
+           "gmin": 19, "temp": 20, "tnom": 21, "eg": 22, "xti": 23, "xtb": 24, "areab": 25, "areac": 26,
+           "subs": 27}
+Q_DEFAULTS = {"type": 1.0, "is": 1e-16, "bf": 100.0, "nf": 1.0, "vaf": 0.0, "ikf": 0.0, "ise": 0.0, "ne": 1.5,
+              "br": 1.0, "nr": 1.0, "var": 0.0, "ikr": 0.0, "isc": 0.0, "nc": 2.0, "nkf": 0.5, "iss": 0.0,
+              "ns": 1.0, "area": 1.0, "m": 1.0, "gmin": 1e-12, "temp": 27.0, "tnom": 27.0, "eg": 1.11,
+              "xti": 3.0, "xtb": 0.0, "areab": 1.0, "areac": 1.0, "subs": 1.0}
 # V source waveform slots (ESL_V_*): value (t <= td), wave (1 = step), v2, td.
 V_WAVE, V_V2, V_TD = 1, 2, 3
 # Parameters a binding may move per point on the GPU lanes.
-BINDABLE = {R: {0}, C: {0}, V: {0}, I: {0}, D: {D_SLOTS["is"]}, MOS1: {M_SLOTS["vto"], M_SLOTS["kp"]}}
+BINDABLE = {R: {0}, C: {0}, V: {0}, I: {0}, D: {D_SLOTS["is"]}, MOS1: {M_SLOTS["vto"], M_SLOTS["kp"]},
+            BJT: {Q_SLOTS["is"], Q_SLOTS["bf"]}}
 
 
 class Deck:
@@ -102,6 +112,20 @@ class Deck:
             a[V_WAVE], a[V_V2], a[V_TD] = 1.0, step_to, step_at
         return self._add(name, V, (p, n), a)
 
+    def bjt(self, name, c, b, e, s, model: dict | None = None, **inst):
+        """Gummel-Poon BJT (c, b, e, substrate); model in SPICE names, type
+        +1 NPN / -1 PNP."""
+        vals = dict(Q_DEFAULTS)
+        for k, v in {**(model or {}), **inst}.items():
+            k = k.lower()
+            if k not in Q_SLOTS:
+                raise ValueError(f"unsupported BJT parameter {k}")
+            vals[k] = float(v)
+        a = np.zeros(NPARAM)
+        for k, slot in Q_SLOTS.items():
+            a[slot] = vals[k]
+        return self._add(name, BJT, (c, b, e, s), a)
+
     def diode(self, name, a_node, k_node, model: dict | None = None, **inst):
         vals = dict(D_DEFAULTS)
         for k, v in {**(model or {}), **inst}.items():
@@ -157,6 +181,8 @@ class Deck:
             return M_SLOTS[name]
         if self.kind[dev] == D:
             return D_SLOTS[name]
+        if self.kind[dev] == BJT:
+            return Q_SLOTS[name]
         return 0
 
     def param(self, device: str, name) -> float:

@@ -64,7 +64,8 @@ enum {
     ESL_DEV_I = 3,     /* models/isource.va   nodes p n (DC)     */
     ESL_DEV_D = 4,     /* models/diode.va     nodes a k (RS = 0) */
     ESL_DEV_MOS1 = 5,  /* models/mos1.va      nodes d g s b      */
-    ESL_DEV_KINDS = 6
+    ESL_DEV_BJT = 6,   /* models/gummel_poon.va nodes c b e s (RB = RC = RE = 0) */
+    ESL_DEV_KINDS = 7
 };
 
 #define ESL_NPARAM 32   /* doubles per device in esl_deck.params */
@@ -89,7 +90,17 @@ enum {
     /* V source waveform (transient): slot 0 is the value at t <= td (and
      * the operating point's); ESL_V_WAVE 1 = step to ESL_V_V2 after
      * ESL_V_TD seconds. */
-    ESL_V_WAVE = 1, ESL_V_V2 = 2, ESL_V_TD = 3
+    ESL_V_WAVE = 1, ESL_V_V2 = 2, ESL_V_TD = 3,
+    /* Gummel-Poon (models/gummel_poon.va:24-94, DC subset: no RB/RC/RE,
+     * caps or excess phase; ESL_Q_SUBS +1 vertical only). Zero VAF, VAR,
+     * IKF, IKR mean infinite. */
+    ESL_Q_TYPE = 0, ESL_Q_IS = 1, ESL_Q_BF = 2, ESL_Q_NF = 3, ESL_Q_VAF = 4,
+    ESL_Q_IKF = 5, ESL_Q_ISE = 6, ESL_Q_NE = 7, ESL_Q_BR = 8, ESL_Q_NR = 9,
+    ESL_Q_VAR = 10, ESL_Q_IKR = 11, ESL_Q_ISC = 12, ESL_Q_NC = 13,
+    ESL_Q_NKF = 14, ESL_Q_ISS = 15, ESL_Q_NS = 16, ESL_Q_AREA = 17,
+    ESL_Q_M = 18, ESL_Q_GMIN = 19, ESL_Q_TEMP = 20, ESL_Q_TNOM = 21,
+    ESL_Q_EG = 22, ESL_Q_XTI = 23, ESL_Q_XTB = 24, ESL_Q_AREAB = 25,
+    ESL_Q_AREAC = 26, ESL_Q_SUBS = 27
 };
 /* Bits of params[ESL_M_GIVEN] (stored as a double holding a small integer):
  * the reference's $param_given tests (models/mos1.va:177-217). */

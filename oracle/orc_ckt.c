@@ -52,7 +52,8 @@
 #define K_I 3
 #define K_D 4
 #define K_MOS 5
-#define NKIND 6
+#define K_Q 6
+#define NKIND 7
 #define NPARAM 32
 #define TRASH 0xffffffffu
 
@@ -323,6 +324,139 @@ static void dio1_eval(const dio_prep *d, double xa, double xk, double vl, int ch
     }
 }
 
+/* ---- Gummel-Poon BJT (gummel_poon.va, DC part: RB = RC = RE = 0, no excess
+ * phase, vertical substrate) ---- */
+typedef struct {
+    double tp, is, bf, br, vt, nfvt, nrvt, ise, nevt, isc, ncvt, iss, nsvt, inv_vaf, inv_var, inv_ikf, inv_ikr, nkf,
+        mgmin, vert, vcrit;
+} bjt_prep;
+
+static void bjt1_prep(const double *p, bjt_prep *o)
+{
+    /* slots: 0 type 1 is 2 bf 3 nf 4 vaf 5 ikf 6 ise 7 ne 8 br 9 nr 10 var 11 ikr 12 isc 13 nc 14 nkf 15 iss
+     * 16 ns 17 area 18 m 19 gmin 20 temp 21 tnom 22 eg 23 xti 24 xtb 25 areab 26 areac 27 subs */
+    const double tdev = p[20] + 273.15, tnom = p[21] + 273.15;
+    const double vt = KB / QE * tdev;
+    const double ratlog = orc_log(tdev / tnom);
+    const double factlog = (tdev / tnom - 1.0) * p[22] / vt + p[23] * ratlog;
+    const double factor = orc_exp(factlog);
+    const double bfac = orc_exp(p[24] * ratlog);
+    const double m = p[18], ai = p[17] * m;
+    const double vert = p[27] > 0 ? 1.0 : 0.0;
+    const double abc = m * (vert * p[25] + (1.0 - vert) * p[26]);
+    const double ne = p[7], nc = p[13];
+    o->tp = p[0];
+    o->is = p[1] * (factor * ai);
+    o->bf = p[2] * bfac;
+    o->br = p[8] * bfac;
+    o->vt = vt;
+    o->nfvt = p[3] * vt;
+    o->nrvt = p[9] * vt;
+    o->ise = p[6] * orc_exp(factlog / ne) / bfac * ai;
+    o->nevt = ne * vt;
+    o->isc = p[12] * orc_exp(factlog / nc) / bfac * abc;
+    o->ncvt = nc * vt;
+    o->iss = p[15] * factor * ai;
+    o->nsvt = p[16] * vt;
+    o->inv_vaf = p[4] > 0 ? 1.0 / p[4] : 0.0;
+    o->inv_var = p[10] > 0 ? 1.0 / p[10] : 0.0;
+    o->inv_ikf = p[5] > 0 ? 1.0 / (p[5] * ai) : 0.0;
+    o->inv_ikr = p[11] > 0 ? 1.0 / (p[11] * ai) : 0.0;
+    o->nkf = p[14];
+    o->mgmin = m * p[19];
+    o->vert = vert;
+    o->vcrit = vt * orc_log(vt / (1.4142135623730951 * o->is));
+}
+
+static double bjt1_pnj(double v, double isat, double vtn, double *g)
+{
+    if (v >= -3.0 * vtn) {
+        const double u = fmin(v / vtn, 80.0);
+        const double e = orc_exp(u);
+        *g = isat * e / vtn;
+        return isat * (e * (1.0 + (v / vtn - u)) - 1.0);
+    }
+    const double u = 3.0 * vtn / (2.718281828459045 * v);
+    const double u3 = u * u * u;
+    *g = 3.0 * isat * u3 / v;
+    return -isat * (1.0 + u3);
+}
+
+static int bjt1_limit(const bjt_prep *b, const double *x, const uint32_t *t, double *lim)
+{
+    const double xc = x[t[0]], xb = x[t[1]], xe = x[t[2]];
+    const double vbe = b->tp * (xb - xe), vbc = b->tp * (xb - xc);
+    double ve = pnjlim(vbe, lim[0], b->vt, b->vcrit);
+    double vc = pnjlim(vbc, lim[1], b->vt, b->vcrit);
+    const double tol = 16.0 * 2.220446049250313e-16 * fmax(fmax(fabs(xc), fabs(xb)), fabs(xe));
+    if (fabs(ve - vbe) <= tol) ve = vbe;
+    if (fabs(vc - vbc) <= tol) vc = vbc;
+    lim[0] = ve;
+    lim[1] = vc;
+    return ve != vbe || vc != vbc;
+}
+
+/* out[0..11] Jacobian (c,c) (c,b) (c,e) (c,s) (b,c) (b,b) (b,e) (e,c) (e,b)
+ * (e,e) (s,c) (s,s); out[12..15] rows c, b, e, s */
+static void bjt1_eval(const bjt_prep *b, const double *x, const uint32_t *t, const double *lim, double *out)
+{
+    const double xc = x[t[0]], xb = x[t[1]], xe = x[t[2]], xs = x[t[3]];
+    const double tp = b->tp, vbe = lim[0], vbc = lim[1];
+    const double dbe = tp * (xb - xe) - vbe, dbc = tp * (xb - xc) - vbc;
+    const double vsub = tp * (xs - xc);
+    double gbe, gben, gbc, gbcn, gsub;
+    const double cbe = bjt1_pnj(vbe, b->is, b->nfvt, &gbe);
+    const double cben = bjt1_pnj(vbe, b->ise, b->nevt, &gben) + b->mgmin * vbe;
+    gben = gben + b->mgmin;
+    const double cbc = bjt1_pnj(vbc, b->is, b->nrvt, &gbc);
+    const double cbcn = bjt1_pnj(vbc, b->isc, b->ncvt, &gbcn) + b->mgmin * vbc;
+    gbcn = gbcn + b->mgmin;
+    const double cdsub = bjt1_pnj(vsub, b->iss, b->nsvt, &gsub) + b->mgmin * vsub;
+    gsub = gsub + b->mgmin;
+    const double q1 = 1.0 / (1.0 - vbc * b->inv_vaf - vbe * b->inv_var);
+    const double q2 = cbe * b->inv_ikf + cbc * b->inv_ikr;
+    const double arg = fmax(1e-30, 1.0 + 4.0 * q2);
+    const double pp = b->nkf == 0.5 ? sqrt(arg) : orc_exp(b->nkf * orc_log(arg));
+    const double qb = 0.5 * q1 * (1.0 + pp);
+    const double dp_dq2 = (1.0 + 4.0 * q2 > 1e-30) ? 4.0 * b->nkf * pp / arg : 0.0;
+    const double dqb_be = 0.5 * (q1 * q1 * b->inv_var * (1.0 + pp) + q1 * dp_dq2 * (gbe * b->inv_ikf));
+    const double dqb_bc = 0.5 * (q1 * q1 * b->inv_vaf * (1.0 + pp) + q1 * dp_dq2 * (gbc * b->inv_ikr));
+    double icc = cbe / qb - cbc / qb;
+    const double dcc = (cbe - cbc) / (qb * qb);
+    const double dicc_be = gbe / qb - dcc * dqb_be;
+    const double dicc_bc = -gbc / qb - dcc * dqb_bc;
+    double ibe = cbe / b->bf + cben;
+    const double gbe_t = gbe / b->bf + gben;
+    double ibc = cbc / b->br + cbcn;
+    const double gbc_t = gbc / b->br + gbcn;
+    const double isc_ = b->vert * cdsub;
+    const double gs = b->vert * gsub;
+    if (dbe != 0 || dbc != 0) {
+        icc = icc + (dicc_be * dbe + dicc_bc * dbc);
+        ibe = ibe + gbe_t * dbe;
+        ibc = ibc + gbc_t * dbc;
+    }
+    out[0] = (-dicc_bc + gbc_t) + gs;
+    out[1] = (dicc_be + dicc_bc) - gbc_t;
+    out[2] = -dicc_be;
+    out[3] = -gs;
+    out[4] = -gbc_t;
+    out[5] = gbe_t + gbc_t;
+    out[6] = -gbe_t;
+    out[7] = dicc_bc;
+    out[8] = -(dicc_be + dicc_bc) - gbe_t;
+    out[9] = dicc_be + gbe_t;
+    out[10] = -gs;
+    out[11] = gs;
+    out[12] = (tp * icc - tp * ibc) - tp * isc_;
+    out[13] = tp * ibe + tp * ibc;
+    out[14] = -tp * icc - tp * ibe;
+    out[15] = tp * isc_;
+}
+
+static const uint8_t BJT_ENT[12][2] = {{0, 0}, {0, 1}, {0, 2}, {0, 3}, {1, 0}, {1, 1},
+                                       {1, 2}, {2, 0}, {2, 1}, {2, 2}, {3, 0}, {3, 3}};
+
 static void jcur(double v, double isat, double vtx, double gm, double *i, double *g)
 {
     if (v > -3.0 * vtx) {
@@ -536,6 +670,8 @@ orc_ckt *orc_ckt_new(const orc_deck *deck, char *err, size_t err_cap)
             ent[cnt++] = (rc_t){in->br, in->t[1]};
         } else if (in->kind == K_MOS) {
             for (int e = 0; e < 11; e++) ent[cnt++] = (rc_t){in->t[MOS_ENT[e][0]], in->t[MOS_ENT[e][1]]};
+        } else if (in->kind == K_Q) {
+            for (int e = 0; e < 12; e++) ent[cnt++] = (rc_t){in->t[BJT_ENT[e][0]], in->t[BJT_ENT[e][1]]};
         }
     }
     size_t keep = 0;
@@ -584,6 +720,9 @@ orc_ckt *orc_ckt_new(const orc_deck *deck, char *err, size_t err_cap)
         } else if (in->kind == K_MOS) {
             in->nslot = 11;
             for (int e = 0; e < 11; e++) in->slot[e] = find_slot(c, in->t[MOS_ENT[e][0]], in->t[MOS_ENT[e][1]]);
+        } else if (in->kind == K_Q) {
+            in->nslot = 12;
+            for (int e = 0; e < 12; e++) in->slot[e] = find_slot(c, in->t[BJT_ENT[e][0]], in->t[BJT_ENT[e][1]]);
         }
     }
     return c;
@@ -614,6 +753,7 @@ typedef struct {
     double *p;          /* [n_dev][NPARAM] with bindings applied */
     mos_prep *mp;       /* per instance (MOS only used) */
     dio_prep *dp;       /* per instance (diodes only used) */
+    bjt_prep *bp;       /* per instance (BJTs only used) */
     double *lim;        /* [n_inst][4]: MOS (vgs, vds, vbs, von), diode vd */
 } lane_dev;
 
@@ -627,19 +767,25 @@ static void lane_dev_init(const orc_ckt *c, const double *live, lane_dev *ld)
     ld->mp = calloc(c->n_inst + 1, sizeof(mos_prep));
     ld->lim = calloc((size_t)c->n_inst * 4 + 4, 8);
     ld->dp = calloc(c->n_inst + 1, sizeof(dio_prep));
+    ld->bp = calloc(c->n_inst + 1, sizeof(bjt_prep));
     for (uint32_t i = 0; i < c->n_inst; i++) {
         if (c->inst[i].kind == K_MOS) mos1_prep(ld->p + (size_t)c->inst[i].dev * NPARAM, &ld->mp[i]);
         if (c->inst[i].kind == K_D) dio1_prep(ld->p + (size_t)c->inst[i].dev * NPARAM, &ld->dp[i]);
+        if (c->inst[i].kind == K_Q) bjt1_prep(ld->p + (size_t)c->inst[i].dev * NPARAM, &ld->bp[i]);
     }
 }
 
-static void lane_dev_free(lane_dev *ld) { free(ld->p); free(ld->mp); free(ld->dp); free(ld->lim); }
+static void lane_dev_free(lane_dev *ld) { free(ld->p); free(ld->mp); free(ld->dp); free(ld->bp); free(ld->lim); }
 
 /* cold start: MODEINITJCT seeds into the limiting state */
 static void seed_cold(const orc_ckt *c, lane_dev *ld)
 {
     for (uint32_t i = 0; i < c->n_inst; i++) {
         if (c->inst[i].kind == K_D) ld->lim[(size_t)i * 4] = ld->dp[i].vcrit;
+        if (c->inst[i].kind == K_Q) {
+            ld->lim[(size_t)i * 4] = ld->bp[i].vcrit;
+            ld->lim[(size_t)i * 4 + 1] = 0.0;
+        }
         if (c->inst[i].kind != K_MOS) continue;
         double *lim = ld->lim + (size_t)i * 4;
         lim[0] = ld->mp[i].seed_vgs;
@@ -655,6 +801,10 @@ static void seed_at(const orc_ckt *c, lane_dev *ld, const double *x)
     for (uint32_t i = 0; i < c->n_inst; i++) {
         const orc_inst *in = &c->inst[i];
         if (in->kind == K_D) ld->lim[(size_t)i * 4] = x[in->t[0]] - x[in->t[1]];
+        if (in->kind == K_Q) {
+            ld->lim[(size_t)i * 4] = ld->bp[i].tp * (x[in->t[1]] - x[in->t[2]]);
+            ld->lim[(size_t)i * 4 + 1] = ld->bp[i].tp * (x[in->t[1]] - x[in->t[0]]);
+        }
         if (in->kind != K_MOS) continue;
         double *lim = ld->lim + (size_t)i * 4;
         const double tp = ld->mp[i].type;
@@ -740,6 +890,13 @@ static void assemble_t(const orc_ckt *c, lane_dev *ld, const double *x, double t
             }
             break;
         }
+        case K_Q: {
+            double o[16];
+            bjt1_eval(&ld->bp[i], x, in->t, ld->lim + (size_t)i * 4, o);
+            for (int e = 0; e < 12; e++) g[SLOT(in->slot[e])] += o[e];
+            for (int r = 0; r < 4; r++) rhs[ROW(in->t[r])] += o[12 + r];
+            break;
+        }
         case K_MOS: {
             double f3[3], j11[11];
             mos1_eval(&ld->mp[i], x, in->t, ld->lim + (size_t)i * 4, f3, j11);
@@ -770,6 +927,7 @@ static int apply_limits(const orc_ckt *c, lane_dev *ld, const double *x)
     for (uint32_t i = 0; i < c->n_inst; i++) {
         if (c->inst[i].kind == K_MOS) any |= mos1_limit(&ld->mp[i], x, c->inst[i].t, ld->lim + (size_t)i * 4);
         if (c->inst[i].kind == K_D) any |= dio1_limit(&ld->dp[i], x, c->inst[i].t, ld->lim + (size_t)i * 4);
+        if (c->inst[i].kind == K_Q) any |= bjt1_limit(&ld->bp[i], x, c->inst[i].t, ld->lim + (size_t)i * 4);
     }
     return any;
 }

@@ -17,6 +17,7 @@ import os
 
 REF = "/root/reference/tests/fixtures"
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ngspice_mos1.json")
+OUT_BJT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ngspice_bjt.json")
 
 NM_REF = {"type": 1, "vto": 0.7, "kp": 200e-6, "gamma": 0.4, "phi": 0.6, "lambda": 0.02,
           "cgso": 100e-12, "cgdo": 100e-12}
@@ -83,16 +84,50 @@ CASES = {
 }
 
 
-def main():
+QM = {"type": 1, "is": 1e-15, "bf": 100, "br": 2, "vaf": 100}
+QD = {"type": 1, "is": 1e-15, "bf": 200, "vaf": 100}
+BJT_CASES = {
+    "bjt_active": ("reference/bjt_active", [
+        ("V", "vb", ["b", "0"], 0.7), ("V", "vc", ["supply", "0"], 5.0), ("R", "rc", ["supply", "c"], 1e3),
+        ("Q", "q1", ["c", "b", "0", "0"], QM)], []),
+    "bjt_cutoff": ("reference/bjt_cutoff", [
+        ("V", "vb", ["b", "0"], 0.0), ("V", "vc", ["supply", "0"], 5.0), ("R", "rc", ["supply", "c"], 1e3),
+        ("Q", "q1", ["c", "b", "0", "0"], QM)], []),
+    "bjt_saturation": ("reference/bjt_saturation", [
+        ("V", "vb", ["b", "0"], 0.9), ("V", "vc", ["supply", "0"], 0.2), ("R", "rc", ["supply", "c"], 1e3),
+        ("Q", "q1", ["c", "b", "0", "0"], QM)], []),
+    "bjt_reverse_active": ("reference/bjt_reverse_active", [
+        ("V", "vb", ["b", "0"], 0.8), ("V", "vc", ["supply", "0"], 0.0), ("R", "rc", ["supply", "c"], 1e3),
+        ("Q", "q1", ["c", "b", "0", "0"], QM)], []),
+}
+for tag, vp in (("0p1", 0.05), ("-0p1", -0.05), ("0", 0.0)):
+    BJT_CASES[f"bjt_diffpair_{tag}"] = (f"reference/bjt_diffpair_{tag}", [
+        ("V", "vcc", ["vcc", "0"], 5.0), ("V", "vp", ["bp", "0"], vp), ("V", "vn", ["bn", "0"], -vp),
+        ("I", "itail", ["emit", "neg"], 1e-3), ("V", "vneg", ["neg", "0"], -5.0),
+        ("R", "r1", ["vcc", "op"], 2e3), ("R", "r2", ["vcc", "on"], 2e3),
+        ("Q", "q1", ["op", "bp", "emit", "0"], QD), ("Q", "q2", ["on", "bn", "emit", "0"], QD)], [])
+BJT_CASES["bench_analog_diff_pair"] = ("op/bench_analog_diff_pair", [
+    ("V", "vcc", ["vcc", "0"], 5.0), ("V", "vee", ["vee", "0"], -5.0), ("V", "vp", ["inp", "0"], 0.1),
+    ("V", "vn", ["inn", "0"], -0.1), ("R", "rcp", ["vcc", "cp"], 4.7e3), ("R", "rcn", ["vcc", "cn"], 4.7e3),
+    ("R", "rt", ["tail", "vee"], 4.7e3),
+    ("Q", "q1", ["cp", "inp", "tail", "0"], {"type": 1, "is": 1e-16, "bf": 100}),
+    ("Q", "q2", ["cn", "inn", "tail", "0"], {"type": 1, "is": 1e-16, "bf": 100})], [])
+BJT_CASES["bench_analog_current_mirror"] = ("op/bench_analog_current_mirror", [
+    ("V", "vcc", ["vcc", "0"], 5.0), ("I", "iref", ["vcc", "ref"], 1e-3), ("R", "rl", ["vcc", "out"], 1e3),
+    ("Q", "q1", ["ref", "ref", "0", "0"], {"type": 1, "is": 1e-16, "bf": 200}),
+    ("Q", "q2", ["out", "ref", "0", "0"], {"type": 1, "is": 1e-16, "bf": 200})], [])
+
+
+def write(cases, path):
     out = {"source": "OmarSiwy/ESPice 1.2.0 tests/fixtures (ngspice-44.2 oracle values)", "cases": {}}
-    for name, (path, devices, live) in CASES.items():
-        exp = json.load(open(os.path.join(REF, path + ".expected.json")))
-        net = open(os.path.join(REF, path + ".sp"), "rb").read()
+    for name, (fixture, devices, live) in cases.items():
+        exp = json.load(open(os.path.join(REF, fixture + ".expected.json")))
+        net = open(os.path.join(REF, fixture + ".sp"), "rb").read()
         plot = exp["expect"]["plots"][0]
         cols = {k: {"values": v["values"], "rtol": v.get("rtol", 1e-3), "atol": v.get("atol", 0.0)}
                 for k, v in plot["columns"].items()}
         out["cases"][name] = {
-            "fixture": path,
+            "fixture": fixture,
             "netlist_sha256": hashlib.sha256(net).hexdigest(),
             "oracle": exp["oracle"].get("version", exp["oracle"].get("kind", "")),
             "devices": devices,
@@ -100,9 +135,14 @@ def main():
             "rows": plot.get("row_count", 1),
             "columns": cols,
         }
-    with open(OUT, "w") as f:
+    with open(path, "w") as f:
         json.dump(out, f, indent=1)
-    print(f"wrote {OUT}: {len(out['cases'])} cases")
+    print(f"wrote {path}: {len(out['cases'])} cases")
+
+
+def main():
+    write(CASES, OUT)
+    write(BJT_CASES, OUT_BJT)
 
 
 if __name__ == "__main__":

@@ -1,4 +1,5 @@
-"""Builds decks and expected columns from tests/golden/ngspice_mos1.json."""
+"""Builds decks and expected columns from the ngspice fixtures transcribed
+under tests/golden (MOS1 and Gummel-Poon operating points and sweeps)."""
 from __future__ import annotations
 
 import json
@@ -8,11 +9,15 @@ import numpy as np
 
 from espice_amd.deck import Deck
 
-GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ngspice_mos1.json")
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+GOLDEN = [os.path.join(HERE, "ngspice_mos1.json"), os.path.join(HERE, "ngspice_bjt.json")]
 
 
 def load():
-    return json.load(open(GOLDEN))["cases"]
+    cases = {}
+    for path in GOLDEN:
+        cases.update(json.load(open(path))["cases"])
+    return cases
 
 
 def build(case: dict):
@@ -30,6 +35,8 @@ def build(case: dict):
             d.capacitor(name, nodes[0], nodes[1], dev[3])
         elif kind == "M":
             d.mos1(name, *nodes, dev[3], **dev[4])
+        elif kind == "Q":
+            d.bjt(name, *nodes, dev[3])
     for src, _col, _sign in case["live"]:
         d.bind(src, 0, src)
     cols = case["columns"]

@@ -21,7 +21,7 @@ def test_golden_case(oracle, name):
     assert misses == 0, (name, worst)
 
 
-@pytest.mark.parametrize("family", ["inverter_chain", "ota_array"])
+@pytest.mark.parametrize("family", ["inverter_chain", "ota_array", "diode_stack", "diffpair_array"])
 def test_jacobian_matches_finite_differences(oracle, family):
     fam = circuits.FAMILIES[family]()
     c = oracle.Circuit(fam.deck.arrays())
