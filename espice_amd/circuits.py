@@ -14,6 +14,12 @@ Families:
                   transient workload (every R, C and IS live)
   diffpair_array  64 Gummel-Poon differential pairs, .op (the FP32 +
                   refinement workload)
+
+The uneven batch draws its instances from eight families built from four
+structures, each as an .op and as a 200-step transient variant:
+  inverter chains of 8-256 stages, RC ladders of 16-2048 nodes (with diode
+  clamps), BJT differential-pair arrays of 4-64 pairs, and MOS1 current-
+  mirror trees of depth 2-8 (`mixed_batch`).
 """
 from __future__ import annotations
 
@@ -62,8 +68,12 @@ class Family:
     draws: list = field(default_factory=list)
 
     def sample(self, seed: int, lanes: int, first: int = 0) -> np.ndarray:
-        idx = np.arange(first, first + lanes, dtype=np.uint64)
-        out = np.zeros((lanes, len(self.deck.live)))
+        return self.sample_at(seed, np.arange(first, first + lanes, dtype=np.uint64))
+
+    def sample_at(self, seed: int, idx: np.ndarray) -> np.ndarray:
+        """Rows for the points keyed by `idx` (any lane numbers)."""
+        idx = np.asarray(idx, dtype=np.uint64)
+        out = np.zeros((len(idx), len(self.deck.live)))
         for var, (name, kind, a, b) in enumerate(self.draws):
             col = self.deck.live.index(name)
             if kind == "normal":
@@ -251,5 +261,205 @@ def rc_mesh(k: int = 64, diode_frac: float = 0.05, layout_seed: int = 1) -> Fami
     return Family(d.title, d, draws)
 
 
+def inverter_chain_tran(stages: int = 16) -> Family:
+    """The inverter chain as a transient: a 10 fF load on every stage output
+    and the input stepping from 0 V to 1 V at t = 0; VDD ~ N(1.0 V, 2%),
+    per-device VTO as in `inverter_chain`."""
+    d = Deck(f"inverter_chain_tran_{stages}")
+    nm = {"type": 1, "vto": 0.5, "kp": 200e-6, "lambda": 0.05}
+    pm = {"type": -1, "vto": -0.5, "kp": 100e-6, "lambda": 0.05}
+    d.vsource("vdd", "vdd", 0, 1.0)
+    d.vsource("vin", "in", 0, 0.0, step_to=1.0, step_at=0.0)
+    prev = "in"
+    for k in range(stages):
+        out = f"s{k + 1}"
+        d.mos1(f"mn{k}", out, prev, 0, 0, nm, w=1e-6, l=0.1e-6)
+        d.mos1(f"mp{k}", out, prev, "vdd", "vdd", pm, w=2e-6, l=0.1e-6)
+        d.capacitor(f"cl{k}", out, 0, 10e-15)
+        prev = out
+    d.bind("vdd", 0, "vdd")
+    for k in range(stages):
+        d.bind(f"mn{k}", "vto", f"vto_n{k}")
+        d.bind(f"mp{k}", "vto", f"vto_p{k}")
+    draws = [("vdd", "rel_normal", 1.0, 0.02)]
+    draws += [(f"vto_n{k}", "normal", 0.5, 0.01) for k in range(stages)]
+    draws += [(f"vto_p{k}", "normal", -0.5, 0.01) for k in range(stages)]
+    return Family(d.title, d, draws)
+
+
+def rc_ladder(nodes: int = 64, tran: bool = False, clamp_every: int = 16) -> Family:
+    """RC ladder of `nodes` nodes: a source through R into node 1, R between
+    consecutive nodes, C from every node to ground, and a junction diode
+    clamp to ground on every `clamp_every`-th node. The source is 1 V DC
+    (.op) or steps from 0 V to 1 V at t = 0 (transient). Per point: every R ~
+    N(1 kOhm, 5%), every C ~ log-uniform [10 fF, 100 fF], the clamps' IS ~
+    log-uniform [1e-16, 1e-14] A (shared), the source level ~ N(1 V, 2%)."""
+    d = Deck(f"rc_ladder_{'tran_' if tran else ''}{nodes}")
+    if tran:
+        d.vsource("vs", "src", 0, 0.0, step_to=1.0, step_at=0.0)
+    else:
+        d.vsource("vs", "src", 0, 1.0)
+        d.bind("vs", 0, "vs")
+    draws = [] if tran else [("vs", "rel_normal", 1.0, 0.02)]
+    prev = "src"
+    for k in range(nodes):
+        nd = f"n{k}"
+        d.resistor(f"r{k}", prev, nd, 1e3)
+        d.bind(f"r{k}", 0, f"r{k}")
+        draws.append((f"r{k}", "rel_normal", 1e3, 0.05))
+        d.capacitor(f"c{k}", nd, 0, 30e-15)
+        d.bind(f"c{k}", 0, f"c{k}")
+        draws.append((f"c{k}", "log_uniform", 10e-15, 100e-15))
+        if (k + 1) % clamp_every == 0:
+            d.diode(f"d{k}", nd, 0, {"is": 1e-15})
+            d.bind(f"d{k}", "is", "is_clamp")
+        prev = nd
+    if nodes >= clamp_every:
+        draws.append(("is_clamp", "log_uniform", 1e-16, 1e-14))
+    return Family(d.title, d, draws)
+
+
+def diffpair_array_tran(count: int = 8) -> Family:
+    """`diffpair_array` as a transient: 100 fF on every collector, and each
+    pair's differential input flipping sign at t = 0 (vd_k/2 -> -vd_k/2 on
+    one side, the opposite on the other). Same per-point draws."""
+    d = Deck(f"diffpair_array_tran_{count}")
+    d.vsource("vcc", "vcc", 0, 5.0)
+    d.vsource("vee", "vee", 0, -5.0)
+    qm = {"type": 1, "is": 1e-16, "bf": 100.0, "vaf": 100.0}
+    draws = []
+    for k in range(count):
+        vd = -0.1 + 0.2 * k / max(count - 1, 1)
+        d.vsource(f"vp{k}", f"bp{k}", 0, 0.5 * vd, step_to=-0.5 * vd, step_at=0.0)
+        d.vsource(f"vn{k}", f"bn{k}", 0, -0.5 * vd, step_to=0.5 * vd, step_at=0.0)
+        d.resistor(f"rp{k}", "vcc", f"cp{k}", 10e3)
+        d.resistor(f"rn{k}", "vcc", f"cn{k}", 10e3)
+        d.capacitor(f"ccp{k}", f"cp{k}", 0, 100e-15)
+        d.capacitor(f"ccn{k}", f"cn{k}", 0, 100e-15)
+        d.isource(f"it{k}", f"e{k}", "vee", 0.2e-3)
+        d.bjt(f"q1_{k}", f"cp{k}", f"bp{k}", f"e{k}", 0, qm)
+        d.bjt(f"q2_{k}", f"cn{k}", f"bn{k}", f"e{k}", 0, qm)
+        for q in (f"q1_{k}", f"q2_{k}"):
+            d.bind(q, "is", f"is_{q}")
+            d.bind(q, "bf", f"bf_{q}")
+            draws.append((f"is_{q}", "log_normal", 1e-16, 0.2))
+            draws.append((f"bf_{q}", "normal", 100.0, 10.0))
+        for r in (f"rp{k}", f"rn{k}"):
+            d.bind(r, 0, r)
+            draws.append((r, "rel_normal", 10e3, 0.01))
+    return Family(d.title, d, draws)
+
+
+def mirror_tree(depth: int = 4, tran: bool = False) -> Family:
+    """A binary tree of MOS1 current mirrors `depth` levels deep. The root
+    current flows from a reference source through 20 kOhm into a diode-
+    connected NMOS; each tree node is a diode-connected device whose gate
+    drives two mirror outputs of the same type, each output feeding a
+    diode-connected device of the other type (NMOS sinks into PMOS diodes
+    at VDD, PMOS sources into NMOS diodes at ground); the last level's
+    outputs drive 10 kOhm loads to the opposite rail. The reference source
+    is 1.2 V DC (.op) or steps from 1.0 V to 1.2 V at t = 0 with 20 fF on
+    every tree node (transient). Per point: VDD ~ N(1.8 V, 2%), and per
+    level and type VTO (sigma 10 mV) and KP (sigma 3%)."""
+    d = Deck(f"mirror_tree_{'tran_' if tran else ''}{depth}")
+    nm = {"type": 1, "vto": 0.5, "kp": 200e-6, "lambda": 0.05}
+    pm = {"type": -1, "vto": -0.5, "kp": 100e-6, "lambda": 0.05}
+    d.vsource("vdd", "vdd", 0, 1.8)
+    if tran:
+        d.vsource("vref", "ref", 0, 1.0, step_to=1.2, step_at=0.0)
+    else:
+        d.vsource("vref", "ref", 0, 1.2)
+    d.resistor("rref", "ref", "g0_0", 20e3)
+    d.mos1("md0_0", "g0_0", "g0_0", 0, 0, nm, w=2e-6, l=0.2e-6)
+    d.bind("md0_0", "vto", "vto_n0")
+    d.bind("md0_0", "kp", "kp_n0")
+    if tran:
+        d.capacitor("c0_0", "g0_0", 0, 20e-15)
+    draws = [("vdd", "rel_normal", 1.8, 0.02)]
+    used = {"vto_n0", "kp_n0"}
+    for lev in range(depth):
+        ntype = lev % 2 == 0        # this level's diodes and outputs: NMOS on even levels
+        model, rail = (nm, "0") if ntype else (pm, "vdd")
+        other, orail = (pm, "vdd") if ntype else (nm, "0")
+        tname = "n" if ntype else "p"
+        oname = "p" if ntype else "n"
+        for i in range(2 ** lev):
+            gate = f"g{lev}_{i}"
+            for c in range(2):
+                j = 2 * i + c
+                out = f"g{lev + 1}_{j}"
+                mo = f"mo{lev}_{j}"
+                d.mos1(mo, out, gate, rail, rail, model, w=2e-6, l=0.2e-6)
+                d.bind(mo, "vto", f"vto_{tname}{lev}")
+                d.bind(mo, "kp", f"kp_{tname}{lev}")
+                if lev + 1 < depth:
+                    md = f"md{lev + 1}_{j}"
+                    d.mos1(md, out, out, orail, orail, other, w=2e-6, l=0.2e-6)
+                    d.bind(md, "vto", f"vto_{oname}{lev + 1}")
+                    d.bind(md, "kp", f"kp_{oname}{lev + 1}")
+                    used |= {f"vto_{oname}{lev + 1}", f"kp_{oname}{lev + 1}"}
+                else:
+                    d.resistor(f"rl{j}", out, orail, 10e3)
+                if tran:
+                    d.capacitor(f"c{lev + 1}_{j}", out, 0, 20e-15)
+        used |= {f"vto_{tname}{lev}", f"kp_{tname}{lev}"}
+    d.bind("vdd", 0, "vdd")
+    for name in sorted(used, key=lambda v: (int(v[v.index("_") + 2:]), v)):
+        t = name[name.index("_") + 1]
+        m = nm if t == "n" else pm
+        if name.startswith("vto"):
+            draws.append((name, "normal", m["vto"], 0.01))
+        else:
+            draws.append((name, "rel_normal", m["kp"], 0.03))
+    return Family(d.title, d, draws)
+
+
+# The uneven batch: eight families, each drawing its size per instance from
+# a geometric grid over its range (sizes repeat, so instances share
+# templates), .op or a 200-step transient with the step stated per family.
+MIXED_FAMILIES = (
+    # name, builder(size), size grid, analysis, dt
+    ("inverter_chain", inverter_chain, (8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256), "op", None),
+    ("inverter_chain_tran", inverter_chain_tran, (8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256), "tran", 2e-12),
+    ("rc_ladder", lambda k: rc_ladder(k), (16, 24, 36, 54, 81, 122, 182, 273, 410, 615, 923, 1384, 2048), "op", None),
+    ("rc_ladder_tran", lambda k: rc_ladder(k, tran=True),
+     (16, 24, 36, 54, 81, 122, 182, 273, 410, 615, 923, 1384, 2048), "tran", 10e-12),
+    ("diffpair_array", diffpair_array, (4, 6, 8, 12, 16, 24, 32, 48, 64), "op", None),
+    ("diffpair_array_tran", diffpair_array_tran, (4, 6, 8, 12, 16, 24, 32, 48, 64), "tran", 20e-12),
+    ("mirror_tree", lambda k: mirror_tree(k), (2, 3, 4, 5, 6, 7, 8), "op", None),
+    ("mirror_tree_tran", lambda k: mirror_tree(k, tran=True), (2, 3, 4, 5, 6, 7, 8), "tran", 20e-12),
+)
+MIXED_STEPS = 200
+
+
+def mixed_batch(instances: int = 10000, seed: int = 2024, families=None):
+    """The uneven batch: instance i belongs to family i mod 8 and draws its
+    size from the family's grid (keyed by (seed, i)); instances of one
+    (family, size) share a template and take their parameters from that
+    family's sampler keyed by (seed, instance) (Family.sample_at). Returns a list of groups
+    {"family", "size", "analysis", "dt", "steps", "instances", "build"}
+    ordered by family then size."""
+    fams = MIXED_FAMILIES if families is None else [f for f in MIXED_FAMILIES if f[0] in families]
+    idx = np.arange(instances, dtype=np.uint64)
+    u = uniform01(seed, idx, 0)
+    groups = {}
+    for i in range(instances):
+        f = fams[i % len(fams)]
+        grid = f[2]
+        size = grid[min(int(u[i] * len(grid)), len(grid) - 1)]
+        groups.setdefault((f[0], size), []).append(i)
+    out = []
+    for f in fams:
+        for size in f[2]:
+            inst = groups.get((f[0], size))
+            if not inst:
+                continue
+            out.append({"family": f[0], "size": size, "analysis": f[3], "dt": f[4],
+                        "steps": MIXED_STEPS if f[3] == "tran" else 0, "instances": np.array(inst, dtype=np.uint64),
+                        "build": (lambda b=f[1], s=size: b(s))})
+    return out
+
+
 FAMILIES = {"inverter_chain": inverter_chain, "ota_array": ota_array, "diode_stack": diode_stack,
-            "rc_mesh": rc_mesh, "diffpair_array": diffpair_array}
+            "rc_mesh": rc_mesh, "diffpair_array": diffpair_array, "inverter_chain_tran": inverter_chain_tran,
+            "rc_ladder": rc_ladder, "diffpair_array_tran": diffpair_array_tran, "mirror_tree": mirror_tree}

@@ -118,15 +118,30 @@ inline constexpr uint32_t row_flags(uint32_t y) { return (y >> 20) & 0x3u; }
 constexpr uint32_t kRedTarget = 0x80000000u, kRedRhs = 0x40000000u, kRedNop = 0x3fffffffu;
 constexpr int kRing = 16;      // rows in flight; streams carry kRing pad rows
 
-// Doubles of a wavefront lane's LDS: tv_cap factor slots, the Y block (y,
-// sdiag, xold after the assembly; the stamp staging area during it), rhs
-// and x.
-__host__ __device__ inline uint32_t lds_y_block(uint32_t n)
+// Doubles of a lane's working set (LDS, or global memory for a spilled
+// lane): tv_cap factor slots, the Y block (y, sdiag, xold after the
+// assembly; during it the stamp staging areas, one per wavefront of the
+// lane's crew), rhs and x. A transient lane's Y block holds y and xold,
+// and q and the charge-Jacobian cells follow x.
+constexpr uint32_t kStageDoubles = kWave * kStageEntries;
+__host__ __device__ inline uint32_t lds_y_block(uint32_t n, uint32_t nw = 1)
 {
-    const uint32_t stage = kWave * kStageEntries;
+    const uint32_t stage = nw * kStageDoubles;
     return 3 * n > stage ? 3 * n : stage;
 }
-__host__ __device__ inline uint32_t lds_doubles(uint32_t tv_cap, uint32_t n) { return tv_cap + lds_y_block(n) + 2 * n; }
+__host__ __device__ inline uint32_t lds_doubles(uint32_t tv_cap, uint32_t n, uint32_t nw = 1)
+{
+    return tv_cap + lds_y_block(n, nw) + 2 * n;
+}
+__host__ __device__ inline uint32_t tran_y_block(uint32_t n, uint32_t nw = 1)
+{
+    const uint32_t stage = nw * kStageDoubles;
+    return 2 * n > stage ? 2 * n : stage;
+}
+__host__ __device__ inline uint32_t tran_doubles(uint32_t tv_cap, uint32_t n, uint32_t n_cslot, uint32_t nw = 1)
+{
+    return tv_cap + tran_y_block(n, nw) + 3 * n + n_cslot;
+}
 
 struct TapeProg {
     uint32_t n_tv, n_u, n_l, lx0;   // tv = [U | diag | L | discard | void extras | 64 pads]
@@ -223,7 +238,36 @@ struct TranArgs {
     uint32_t steps, n_probes, s0, s1;   // this launch runs steps [s0, s1)
     uint32_t init;              // 1: the first launch (q1 from the operating point, ip = 0)
     uint32_t dump;              // 1: only write the first step's first Jacobian to op.a_out
+    uint32_t tran_tape;         // batch: the tape a lane's transient starts on
+    uint32_t pad;
 };
+
+// The persistent batch: lanes of several templates (engines) in one launch
+// of workgroups of kCrew wavefronts. A job is one lane: its operating point
+// (op >= 0 names the .op arguments), then, when tran names transient
+// arguments, its transient from that point. Workgroup g first runs its
+// block jobs (block_ptr[g] .. block_ptr[g + 1]) one at a time, each on the
+// whole workgroup (lane per block: wavefront 0 leads, the others evaluate
+// devices), then each wavefront w runs its own wave jobs (wave_ptr[g * kCrew
+// + w] ...) in a kCrew-th of the LDS (lane per wavefront). The host deals
+// the jobs; nothing is claimed at run time.
+constexpr uint32_t kCrew = 4;
+constexpr uint32_t kNoArgs = 0xffffffffu;
+struct BatchJob {
+    uint32_t op, tran, lane, flags;
+};
+struct BatchArgs {
+    const OpArgs *ops;          // [.op argument sets]
+    const TranArgs *trans;      // [transient argument sets]
+    const BatchJob *block_jobs;
+    const uint32_t *block_ptr;  // [grid + 1]
+    const BatchJob *wave_jobs;
+    const uint32_t *wave_ptr;   // [grid * kCrew + 1]
+    uint32_t wave_doubles;      // LDS doubles of one wavefront in the wave phase
+    uint32_t pad;
+    uint64_t *block_ticks, *wave_ticks;   // optional: each job's duration (wall-clock ticks)
+};
+constexpr uint32_t kStepOp = 0xffffffffu;   // step marker: the lane has not reached its transient
 
 // Arguments of the lane-per-thread Newton kernel (op_newton_lpt): thread t
 // solves lane lane_ids[t]; its state is lane-major, element i at i * N + t.

@@ -18,6 +18,7 @@
 #include <map>
 #include <mutex>
 #include <queue>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -33,6 +34,8 @@ extern "C" hipError_t esl_kernel_attrs(uint32_t lds_bytes);
 extern "C" hipError_t esl_launch_op_newton_lpt(const void *a, hipStream_t s);
 extern "C" hipError_t esl_launch_transpose_live(const double *in, double *out, uint32_t N, uint32_t P, hipStream_t s);
 extern "C" hipError_t esl_launch_tran_steps(const void *ta, uint32_t grid, uint32_t lds_bytes, hipStream_t s);
+extern "C" hipError_t esl_launch_batch(const void *b, uint32_t grid, uint32_t lds_bytes, hipStream_t s);
+extern "C" hipError_t esl_batch_occupancy(uint32_t lds_bytes, int *blocks_per_cu);
 extern "C" hipError_t esl_launch_gather(const double *x, uint32_t n, const uint32_t *ids, uint32_t k, uint32_t count,
                                         double *out, hipStream_t s);
 
@@ -1185,6 +1188,10 @@ struct Engine {
     // lane-per-thread state, lane-major
     DevBuf live_t, tvg, rhsg, xg, xoldg, yg, sdiagg, limg;
     u32 tuned = 0;   // ESL_TILING_* chosen by the autotuner
+    // persistent batches: the autotuner's tiling (ESL_TILING_WAVE or _BLOCK,
+    // 0 before tuning) and the measured mean job time per tiling (ns)
+    u32 bt_tiling = 0;
+    double bt_ns[2] = {0.0, 0.0};
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     esl_timing timing{};
@@ -1523,6 +1530,20 @@ extern "C" {
 
 uint32_t esl_abi_version(void) { return ESL_ABI_VERSION; }
 
+esl_status esl_error_message(char *buf, size_t capacity, size_t *required)
+{
+    const std::string &m = g_last_error;
+    if (required) *required = m.size() + 1;
+    if (!buf || capacity == 0) return required ? ESL_OK : ESL_INVALID_ARGUMENT;
+    if (capacity < m.size() + 1) {
+        memcpy(buf, m.data(), capacity - 1);
+        buf[capacity - 1] = 0;
+        return ESL_BUFFER_TOO_SMALL;
+    }
+    memcpy(buf, m.c_str(), m.size() + 1);
+    return ESL_OK;
+}
+
 void esl_default_tolerances(esl_tolerances *t)
 {
     if (!t) return;
@@ -1721,43 +1742,14 @@ esl_status launch_lpw(Engine &e, HostOpArgs a, u32 lanes, u32 mode, u32 tv_cap, 
     return ESL_OK;
 }
 
-}  // namespace
-
-extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t count, const esl_op_options *opt,
-                                   double *x, uint32_t *iterations, uint32_t *reasons)
+// The solve's uploads, start vectors and Newton arguments, and the lead
+// tape (point 0's first Jacobian, factored on the host; lead_singular when
+// that matrix is singular).
+esl_status op_prepare(Engine &E, const double *values, size_t count, const esl_op_options &o, EslMlp *net,
+                      HostOpArgs &a, vector<uint8_t> &lane64, bool &lead_singular)
 {
-    auto *e = static_cast<Engine *>(lookup(h, 1));
-    if (!e) return ESL_INVALID_HANDLE;
-    if (count == 0 || count > 0x7fffffff) return ESL_INVALID_ARGUMENT;
-    if (!x || !iterations || !reasons) return ESL_INVALID_ARGUMENT;
-    if (e->c.n_live && !values) return ESL_INVALID_ARGUMENT;
-    esl_op_options o;
-    esl_default_op_options(&o);
-    if (opt) {
-        if (opt->struct_size < sizeof(esl_op_options)) return ESL_ABI_MISMATCH;
-        o = *opt;
-    }
-    if (o.start != ESL_START_COLD && o.start != ESL_START_GIVEN && o.start != ESL_START_LEARNED)
-        return ESL_INVALID_ARGUMENT;
-    EslMlp *net = nullptr;
-    if (o.start == ESL_START_LEARNED) {
-        net = e->warm_net ? eslw::lookup_mlp(e->warm_net) : nullptr;
-        if (!net || !e->has_nominal) return ESL_INVALID_ARGUMENT;
-        if (o.tiling == ESL_TILING_THREAD) return ESL_INVALID_ARGUMENT;
-    }
-    if (o.start == ESL_START_GIVEN && !o.x0) return ESL_INVALID_ARGUMENT;
-    if (o.precision != ESL_FP64 && o.precision != ESL_FP32_REFINED) return ESL_INVALID_ARGUMENT;
-    if (o.precision == ESL_FP32_REFINED && (o.refine_max == 0 || !(o.refine_tol >= 0))) return ESL_INVALID_ARGUMENT;
-    // FP32 factors run in the wavefront tiling only
-    if (o.precision == ESL_FP32_REFINED && o.tiling == ESL_TILING_THREAD) return ESL_INVALID_ARGUMENT;
-    if (o.tol.max_iter == 0) return ESL_INVALID_ARGUMENT;
-    if (o.tiling > ESL_TILING_THREAD) return ESL_INVALID_ARGUMENT;
-    if (o.n_probes && !o.probes) return ESL_INVALID_ARGUMENT;
-    for (u32 k = 0; k < o.n_probes; k++)
-        if (o.probes[k] >= e->c.n) return ESL_INVALID_ARGUMENT;
-    for (size_t i = 0; i < count * e->c.n_live; i++)
-        if (!std::isfinite(values[i])) return ESL_INVALID_ARGUMENT;
-    const auto t_start = std::chrono::steady_clock::now();
+    Engine *e = &E;
+    lead_singular = false;
     const Compiled &c = e->c;
     const u32 n = c.n, nnz = c.nnz, P = c.n_live;
     e->timing = esl_timing{};
@@ -1770,7 +1762,7 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
     HIPCHK(e->iters.ensure(count * 4));
     HIPCHK(e->lane_ids.ensure(count * 4));
     const bool fp32 = o.precision == ESL_FP32_REFINED;
-    vector<uint8_t> lane64(count, 0);
+    lane64.assign(count, 0);
     HIPCHK(e->lane64.ensure(count));
     if (fp32) HIPCHK(e->a64.ensure(count * (size_t)nnz * 8));
     hipStream_t s = e->stream;
@@ -1794,7 +1786,7 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
     for (size_t i = 0; i < count; i++) ids[i] = (u32)i;
     HIPCHK(hipMemcpyAsync(e->lane_ids.p, ids.data(), count * 4, hipMemcpyHostToDevice, s));
 
-    HostOpArgs a{};
+    a = HostOpArgs{};
     a.dp = e->dp;
     a.live = e->live.as<double>();
     a.x = e->x.as<double>();
@@ -1845,11 +1837,7 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
         HIPCHK(hipStreamSynchronize(s));
         HostTape t;
         if (!gp_factor(n, c.col_ptr.data(), c.row_idx.data(), a0.data(), c.q.data(), 1e-3, t)) {
-            for (size_t l = 0; l < count; l++) {
-                reasons[l] = ESL_REASON_SINGULAR_MATRIX;
-                iterations[l] = 0;
-            }
-            memset(x, 0, count * (o.n_probes ? o.n_probes : n) * 8);
+            lead_singular = true;
             return ESL_OK;
         }
         u32 id;
@@ -1864,52 +1852,23 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
     }
     e->timing.lead_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_lead0).count();
-    const u32 lead_cap = e->tapes[e->lead_id]->prog.n_tv;
+    return ESL_OK;
+}
+
+// Re-pivot rounds after a Newton launch over `count` lanes with statuses st:
+// dump each failing lane's Jacobian at its saved iterate, factor it on the
+// host into its own tape, resume it one wavefront per lane. Lanes whose
+// FP32 factors failed on a fresh tape, or whose refinement stalled, redo
+// that iteration and finish in FP64. st is current on return.
+esl_status op_repivot(Engine &E, const HostOpArgs &a, size_t count, vector<u32> &st, vector<u32> &its,
+                      vector<uint8_t> &lane64, float &kernel_ms)
+{
+    Engine *e = &E;
+    const Compiled &c = e->c;
+    const u32 n = c.n, nnz = c.nnz;
+    hipStream_t s = e->stream;
     vector<esl::LaneResume> rs(count);
-    for (auto &r : rs) r = esl::LaneResume{0, 0, e->lead_id, 0, 1e9, 0, 0, 0.0};
-    HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
-    const u32 mode0 = o.start == ESL_START_COLD ? 0u : o.start == ESL_START_GIVEN ? 1u : 3u;
-
-    // Tiling: explicit, or the autotuner's choice. The autotuner times both
-    // tilings once per engine on the first batch of at least 1024 points
-    // (kernel time of the first launch over every lane) and keeps the faster;
-    // below that size a batch runs one wavefront per lane.
-    // the thread tiling covers R, V, I and MOS1 decks in FP64
-    const bool wave_only = fp32 || net || c.n_dio > 0 || c.n_bjt > 0;
-    if (wave_only && o.tiling == ESL_TILING_THREAD) return ESL_INVALID_ARGUMENT;
-    u32 tiling = wave_only ? (u32)ESL_TILING_WAVE : o.tiling;
-    if (tiling == ESL_TILING_AUTO) {
-        if (e->tuned) {
-            tiling = e->tuned;
-        } else if (count >= 1024) {
-            float ms_w = 0, ms_t = 0;
-            if (launch_lpw(*e, a, (u32)count, mode0, lead_cap, ms_w) != ESL_OK) return ESL_DEVICE_UNAVAILABLE;
-            HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
-            if (mode0 == 1) HIPCHK(hipMemcpyAsync(e->x.p, o.x0, count * n * 8, hipMemcpyHostToDevice, s));
-            if (launch_lpt(*e, a, (u32)count, mode0, ms_t) != ESL_OK) return ESL_DEVICE_UNAVAILABLE;
-            HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
-            if (mode0 == 1) HIPCHK(hipMemcpyAsync(e->x.p, o.x0, count * n * 8, hipMemcpyHostToDevice, s));
-            e->tuned = ms_t < ms_w ? ESL_TILING_THREAD : ESL_TILING_WAVE;
-            tiling = e->tuned;
-        } else {
-            tiling = ESL_TILING_WAVE;
-        }
-    }
-    e->timing.tiling = tiling;
-
-    vector<u32> st(count), its(count);
-    float kernel_ms = 0, ms = 0;
-    if (tiling == ESL_TILING_THREAD) {
-        esl_status r = launch_lpt(*e, a, (u32)count, mode0, ms);
-        if (r != ESL_OK) return r;
-    } else {
-        esl_status r = launch_lpw(*e, a, (u32)count, mode0, lead_cap, ms);
-        if (r != ESL_OK) return r;
-    }
-    kernel_ms += ms;
-    e->timing.launches++;
-    HIPCHK(hipMemcpyAsync(st.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    float ms = 0;
     // Re-pivot rounds: dump each failing lane's Jacobian at its saved iterate,
     // factor it on the host into its own tape, resume it one wavefront per lane.
     // Lanes whose FP32 factors failed on a fresh tape, or whose refinement
@@ -1978,6 +1937,115 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
         HIPCHK(hipMemcpyAsync(st.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
     }
+    return ESL_OK;
+}
+
+}  // namespace
+
+extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t count, const esl_op_options *opt,
+                                   double *x, uint32_t *iterations, uint32_t *reasons)
+{
+    auto *e = static_cast<Engine *>(lookup(h, 1));
+    if (!e) return ESL_INVALID_HANDLE;
+    if (count == 0 || count > 0x7fffffff) return ESL_INVALID_ARGUMENT;
+    if (!x || !iterations || !reasons) return ESL_INVALID_ARGUMENT;
+    if (e->c.n_live && !values) return ESL_INVALID_ARGUMENT;
+    esl_op_options o;
+    esl_default_op_options(&o);
+    if (opt) {
+        if (opt->struct_size < sizeof(esl_op_options)) return ESL_ABI_MISMATCH;
+        o = *opt;
+    }
+    if (o.start != ESL_START_COLD && o.start != ESL_START_GIVEN && o.start != ESL_START_LEARNED)
+        return ESL_INVALID_ARGUMENT;
+    EslMlp *net = nullptr;
+    if (o.start == ESL_START_LEARNED) {
+        net = e->warm_net ? eslw::lookup_mlp(e->warm_net) : nullptr;
+        if (!net || !e->has_nominal) return ESL_INVALID_ARGUMENT;
+        if (o.tiling == ESL_TILING_THREAD) return ESL_INVALID_ARGUMENT;
+    }
+    if (o.start == ESL_START_GIVEN && !o.x0) return ESL_INVALID_ARGUMENT;
+    if (o.precision != ESL_FP64 && o.precision != ESL_FP32_REFINED) return ESL_INVALID_ARGUMENT;
+    if (o.precision == ESL_FP32_REFINED && (o.refine_max == 0 || !(o.refine_tol >= 0))) return ESL_INVALID_ARGUMENT;
+    // FP32 factors run in the wavefront tiling only
+    if (o.precision == ESL_FP32_REFINED && o.tiling == ESL_TILING_THREAD) return ESL_INVALID_ARGUMENT;
+    if (o.tol.max_iter == 0) return ESL_INVALID_ARGUMENT;
+    if (o.tiling > ESL_TILING_THREAD) return ESL_INVALID_ARGUMENT;
+    if (o.n_probes && !o.probes) return ESL_INVALID_ARGUMENT;
+    for (u32 k = 0; k < o.n_probes; k++)
+        if (o.probes[k] >= e->c.n) return ESL_INVALID_ARGUMENT;
+    for (size_t i = 0; i < count * e->c.n_live; i++)
+        if (!std::isfinite(values[i])) return ESL_INVALID_ARGUMENT;
+    const auto t_start = std::chrono::steady_clock::now();
+    const Compiled &c = e->c;
+    const u32 n = c.n;
+    const bool fp32 = o.precision == ESL_FP32_REFINED;
+    hipStream_t s = e->stream;
+    HostOpArgs a{};
+    vector<uint8_t> lane64;
+    bool lead_singular = false;
+    {
+        esl_status r = op_prepare(*e, values, count, o, net, a, lane64, lead_singular);
+        if (r != ESL_OK) return r;
+    }
+    if (lead_singular) {
+        for (size_t l = 0; l < count; l++) {
+            reasons[l] = ESL_REASON_SINGULAR_MATRIX;
+            iterations[l] = 0;
+        }
+        memset(x, 0, count * (o.n_probes ? o.n_probes : n) * 8);
+        return ESL_OK;
+    }
+    const u32 lead_cap = e->tapes[e->lead_id]->prog.n_tv;
+    vector<esl::LaneResume> rs(count);
+    for (auto &r : rs) r = esl::LaneResume{0, 0, e->lead_id, 0, 1e9, 0, 0, 0.0};
+    HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
+    const u32 mode0 = o.start == ESL_START_COLD ? 0u : o.start == ESL_START_GIVEN ? 1u : 3u;
+
+    // Tiling: explicit, or the autotuner's choice. The autotuner times both
+    // tilings once per engine on the first batch of at least 1024 points
+    // (kernel time of the first launch over every lane) and keeps the faster;
+    // below that size a batch runs one wavefront per lane.
+    // the thread tiling covers R, V, I and MOS1 decks in FP64
+    const bool wave_only = fp32 || net || c.n_dio > 0 || c.n_bjt > 0;
+    if (wave_only && o.tiling == ESL_TILING_THREAD) return ESL_INVALID_ARGUMENT;
+    u32 tiling = wave_only ? (u32)ESL_TILING_WAVE : o.tiling;
+    if (tiling == ESL_TILING_AUTO) {
+        if (e->tuned) {
+            tiling = e->tuned;
+        } else if (count >= 1024) {
+            float ms_w = 0, ms_t = 0;
+            if (launch_lpw(*e, a, (u32)count, mode0, lead_cap, ms_w) != ESL_OK) return ESL_DEVICE_UNAVAILABLE;
+            HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
+            if (mode0 == 1) HIPCHK(hipMemcpyAsync(e->x.p, o.x0, count * n * 8, hipMemcpyHostToDevice, s));
+            if (launch_lpt(*e, a, (u32)count, mode0, ms_t) != ESL_OK) return ESL_DEVICE_UNAVAILABLE;
+            HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
+            if (mode0 == 1) HIPCHK(hipMemcpyAsync(e->x.p, o.x0, count * n * 8, hipMemcpyHostToDevice, s));
+            e->tuned = ms_t < ms_w ? ESL_TILING_THREAD : ESL_TILING_WAVE;
+            tiling = e->tuned;
+        } else {
+            tiling = ESL_TILING_WAVE;
+        }
+    }
+    e->timing.tiling = tiling;
+
+    vector<u32> st(count), its(count);
+    float kernel_ms = 0, ms = 0;
+    if (tiling == ESL_TILING_THREAD) {
+        esl_status r = launch_lpt(*e, a, (u32)count, mode0, ms);
+        if (r != ESL_OK) return r;
+    } else {
+        esl_status r = launch_lpw(*e, a, (u32)count, mode0, lead_cap, ms);
+        if (r != ESL_OK) return r;
+    }
+    kernel_ms += ms;
+    e->timing.launches++;
+    HIPCHK(hipMemcpyAsync(st.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    {
+        esl_status r = op_repivot(*e, a, count, st, its, lane64, kernel_ms);
+        if (r != ESL_OK) return r;
+    }
     const auto t_dl0 = std::chrono::steady_clock::now();
     if (o.n_probes) {
         HIPCHK(e->probe_ids.ensure(o.n_probes * 4));
@@ -2036,10 +2104,129 @@ extern "C" void esl_default_tran_options(esl_tran_options *o)
 }
 
 namespace {
-u32 tran_lds_bytes(const Compiled &c, u32 tv_cap)
+u32 tran_lds_bytes(const Compiled &c, u32 tv_cap, u32 nw = 1)
 {
-    return (u32)(((size_t)esl::lds_doubles(tv_cap, c.n) + 4 * (size_t)c.n + c.cslot_pat.size()) * sizeof(double));
+    return (u32)((size_t)esl::tran_doubles(tv_cap, c.n, (u32)c.cslot_pat.size(), nw) * sizeof(double));
 }
+}  // namespace
+
+namespace {
+// The transient lead tape: lane l0's first transient Jacobian (step 0,
+// iteration 0) at its converged operating point, factored on the host.
+// op_tape: the tape the lane's operating point ran on (its slot map places
+// the assembly). Needs status[l0] = kTranRunning and step[l0] = 0.
+esl_status tran_lead_tape(Engine &E, const esl::TranArgs &ta, u32 l0, u32 op_tape, u32 &id, bool &singular)
+{
+    Engine *e = &E;
+    const Compiled &c = e->c;
+    const u32 n = c.n, nnz = c.nnz;
+    hipStream_t s = e->stream;
+    singular = false;
+    HIPCHK(hipMemcpyAsync(e->lane_ids.p, &l0, 4, hipMemcpyHostToDevice, s));
+    esl::TranArgs d = ta;
+    d.op.tapes = e->tape_table.as<esl::TapeProg>();
+    d.op.tv_cap = e->tapes[op_tape]->prog.n_tv;
+    d.init = 1;
+    d.dump = 1;
+    d.s0 = 0;
+    d.s1 = ta.steps;
+    const u32 lds = tran_lds_bytes(c, d.op.tv_cap);
+    if (lds > device_lds_max()) {
+        g_last_error = "transient lane state exceeds the LDS of one workgroup";
+        return ESL_FAILED;
+    }
+    HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
+    HIPCHK(esl_launch_tran_steps(&d, 1, lds, s));
+    vector<double> A(nnz);
+    HIPCHK(hipMemcpyAsync(A.data(), e->a_out.p, nnz * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    HostTape t;
+    if (!gp_factor(n, c.col_ptr.data(), c.row_idx.data(), A.data(), c.q.data(), 1e-3, t)) {
+        singular = true;
+        return ESL_OK;
+    }
+    return add_tape(*e, t, id);
+}
+
+// Re-pivot rounds for transient lanes (statuses stt, current on return):
+// factor each failing lane's dumped Jacobian into its own tape and resume
+// the lane at its step, in launches of S steps up to B * S.
+esl_status tran_repivot(Engine &E, const esl::TranArgs &ta, size_t count, u32 S, u32 B, vector<u32> &stt,
+                        float &kms)
+{
+    Engine *e = &E;
+    const Compiled &c = e->c;
+    const u32 n = c.n, nnz = c.nnz;
+    hipStream_t s = e->stream;
+    const u32 steps = ta.steps;
+    vector<esl::LaneResume> rs(count);
+    auto block_args = [&](u32 b, u32 cap, u32 init) {
+        esl::TranArgs t = ta;
+        t.op.tapes = e->tape_table.as<esl::TapeProg>();
+        t.op.tv_cap = cap;
+        t.s0 = b * S;
+        t.s1 = std::min(steps, (b + 1) * S);
+        t.init = init;
+        t.dump = 0;
+        return t;
+    };
+    for (int round = 0; round < 1000; round++) {
+        HIPCHK(hipMemcpyAsync(stt.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        vector<u32> again;
+        for (size_t l = 0; l < count; l++)
+            if (stt[l] == esl::kRepivot) again.push_back((u32)l);
+        if (again.empty()) break;
+        vector<u32> steps_at(count);
+        HIPCHK(hipMemcpyAsync(rs.data(), e->resume.p, count * sizeof(esl::LaneResume), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(steps_at.data(), e->tstep.p, count * 4, hipMemcpyDeviceToHost, s));
+        vector<double> A(nnz);
+        vector<u32> next;
+        u32 b0 = B;
+        for (u32 l : again) {
+            HIPCHK(hipMemcpyAsync(A.data(), e->a_out.as<double>() + (size_t)l * nnz, nnz * 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            HostTape t;
+            e->timing.repivots++;
+            if (!gp_factor(n, c.col_ptr.data(), c.row_idx.data(), A.data(), c.q.data(), 1e-3, t)) {
+                stt[l] = esl::kSingular;
+                continue;
+            }
+            u32 id;
+            esl_status s2 = add_tape(*e, t, id);
+            if (s2 != ESL_OK) return s2;
+            rs[l].tape = id;
+            stt[l] = esl::kTranRunning;
+            next.push_back(l);
+            b0 = std::min(b0, steps_at[l] / S);
+        }
+        HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(e->status.p, stt.data(), count * 4, hipMemcpyHostToDevice, s));
+        if (next.empty()) break;
+        HIPCHK(hipMemcpyAsync(e->lane_ids.p, next.data(), next.size() * 4, hipMemcpyHostToDevice, s));
+        u32 cap = 0;
+        for (u32 l : next) cap = std::max(cap, e->tapes[rs[l].tape]->prog.n_tv);
+        const u32 lds = tran_lds_bytes(c, cap);
+        if (lds > device_lds_max()) {
+            g_last_error = "transient lane state exceeds the LDS of one workgroup";
+            return ESL_FAILED;
+        }
+        HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
+        float ms = 0;
+        HIPCHK(hipEventRecord(e->ev0, s));
+        for (u32 b = b0; b < B; b++) {
+            esl::TranArgs t = block_args(b, cap, 0);
+            HIPCHK(esl_launch_tran_steps(&t, (u32)next.size(), lds, s));
+        }
+        HIPCHK(hipEventRecord(e->ev1, s));
+        HIPCHK(hipEventSynchronize(e->ev1));
+        HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+        kms += ms;
+        e->timing.launches += B - b0;
+    }
+    return ESL_OK;
+}
+
 }  // namespace
 
 extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t count, const esl_tran_options *opt,
@@ -2122,34 +2309,15 @@ extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t
     HIPCHK(hipMemcpyAsync(rs.data(), e->resume.p, count * sizeof(esl::LaneResume), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     {
-        const u32 l0 = (u32)first_ok;
-        HIPCHK(hipMemcpyAsync(e->lane_ids.p, &l0, 4, hipMemcpyHostToDevice, s));
-        esl::TranArgs d = ta;
-        d.op.tapes = e->tape_table.as<esl::TapeProg>();
-        d.op.tv_cap = e->tapes[rs[l0].tape]->prog.n_tv;
-        d.init = 1;
-        d.dump = 1;
-        d.s0 = 0;
-        d.s1 = o.steps;
-        const u32 lds = tran_lds_bytes(c, d.op.tv_cap);
-        if (lds > device_lds_max()) {
-            g_last_error = "transient lane state exceeds the LDS of one workgroup";
-            return ESL_FAILED;
-        }
-        HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
-        HIPCHK(esl_launch_tran_steps(&d, 1, lds, s));
-        vector<double> A(nnz);
-        HIPCHK(hipMemcpyAsync(A.data(), e->a_out.p, nnz * 8, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        HostTape t;
-        if (!gp_factor(n, c.col_ptr.data(), c.row_idx.data(), A.data(), c.q.data(), 1e-3, t)) {
+        u32 id = 0;
+        bool singular = false;
+        esl_status s2 = tran_lead_tape(*e, ta, (u32)first_ok, rs[first_ok].tape, id, singular);
+        if (s2 != ESL_OK) return s2;
+        if (singular) {
             for (size_t l = 0; l < count; l++)
                 if (reasons[l] == ESL_REASON_OK) reasons[l] = ESL_REASON_SINGULAR_MATRIX;
             return ESL_OK;
         }
-        u32 id;
-        esl_status s2 = add_tape(*e, t, id);
-        if (s2 != ESL_OK) return s2;
         for (auto &r : rs) r = esl::LaneResume{0, 0, id, 0, 1e9, 0, 0, 0.0};
         HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(e->lane_ids.p, ids.data(), count * 4, hipMemcpyHostToDevice, s));
@@ -2213,59 +2381,9 @@ extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t
     }
     e->timing.launches = B;
     // 5. re-pivot rounds: factor each failing lane's dumped Jacobian, resume
-    for (int round = 0; round < 1000; round++) {
-        HIPCHK(hipMemcpyAsync(stt.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        vector<u32> again;
-        for (size_t l = 0; l < count; l++)
-            if (stt[l] == esl::kRepivot) again.push_back((u32)l);
-        if (again.empty()) break;
-        vector<u32> steps_at(count);
-        HIPCHK(hipMemcpyAsync(rs.data(), e->resume.p, count * sizeof(esl::LaneResume), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(steps_at.data(), e->tstep.p, count * 4, hipMemcpyDeviceToHost, s));
-        vector<double> A(nnz);
-        vector<u32> next;
-        u32 b0 = B;
-        for (u32 l : again) {
-            HIPCHK(hipMemcpyAsync(A.data(), e->a_out.as<double>() + (size_t)l * nnz, nnz * 8, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            HostTape t;
-            e->timing.repivots++;
-            if (!gp_factor(n, c.col_ptr.data(), c.row_idx.data(), A.data(), c.q.data(), 1e-3, t)) {
-                stt[l] = esl::kSingular;
-                continue;
-            }
-            u32 id;
-            esl_status s2 = add_tape(*e, t, id);
-            if (s2 != ESL_OK) return s2;
-            rs[l].tape = id;
-            stt[l] = esl::kTranRunning;
-            next.push_back(l);
-            b0 = std::min(b0, steps_at[l] / S);
-        }
-        HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(e->status.p, stt.data(), count * 4, hipMemcpyHostToDevice, s));
-        if (next.empty()) break;
-        HIPCHK(hipMemcpyAsync(e->lane_ids.p, next.data(), next.size() * 4, hipMemcpyHostToDevice, s));
-        u32 cap = 0;
-        for (u32 l : next) cap = std::max(cap, e->tapes[rs[l].tape]->prog.n_tv);
-        const u32 lds = tran_lds_bytes(c, cap);
-        if (lds > device_lds_max()) {
-            g_last_error = "transient lane state exceeds the LDS of one workgroup";
-            return ESL_FAILED;
-        }
-        HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
-        float ms = 0;
-        HIPCHK(hipEventRecord(e->ev0, s));
-        for (u32 b = b0; b < B; b++) {
-            esl::TranArgs t = block_args(b, cap, 0);
-            HIPCHK(esl_launch_tran_steps(&t, (u32)next.size(), lds, s));
-        }
-        HIPCHK(hipEventRecord(e->ev1, s));
-        HIPCHK(hipEventSynchronize(e->ev1));
-        HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
-        kms += ms;
-        e->timing.launches += B - b0;
+    {
+        esl_status s2 = tran_repivot(*e, ta, count, S, B, stt, kms);
+        if (s2 != ESL_OK) return s2;
     }
     // 6. results
     HIPCHK(hipMemcpyAsync(x, e->x.p, count * n * 8, hipMemcpyDeviceToHost, s));
@@ -2289,6 +2407,598 @@ extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t
     e->timing.lane_iterations = lane_it;
     e->timing.total_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    return ESL_OK;
+}
+
+// ---------------------------------------------------------------- batches
+namespace {
+
+// One batch item on its engine: Newton arguments for its operating points
+// and, for a transient item, for its steps.
+struct BatchItemRun {
+    Engine *e = nullptr;
+    const esl_batch_item *it = nullptr;
+    bool tran = false;
+    bool lead_singular = false;   // the operating points' lead matrix is singular: nothing runs
+    bool tran_singular = false;   // the transient lead matrix is singular: operating points only
+    HostOpArgs a{};
+    esl::TranArgs ta{};
+    vector<uint8_t> lane64;
+    u32 op_cap = 0, tran_cap = 0;
+    u32 need[2] = {0, 0};         // LDS bytes of one lane: [0] lane per wavefront, [1] lane per block
+    u32 tiling = 0;               // 0 wave, 1 block for this batch; 2 both (tuning)
+    double est_ns[2] = {0.0, 0.0};
+    u32 op_idx = 0, tran_idx = esl::kNoArgs;
+};
+
+// Model of one job's time before any measurement: the rows a Newton
+// iteration runs (refactor, solves, assembly reduce) times a typical
+// iteration count; the lane-per-block tiling evaluates devices on four
+// wavefronts.
+void model_job_ns(const BatchItemRun &r, double out[2])
+{
+    const Engine &e = *r.e;
+    auto rows = [&](u32 tape) {
+        const TapeHostProg &t = *e.tapes[tape];
+        double red = 0;
+        for (u32 v : t.red_len) red += v;
+        return (double)t.prog.rw_rows + t.prog.sw_rows + red;
+    };
+    const double chunks = (double)e.c.chunks.size() - 1;
+    double w = (rows(e.lead_id) + 40.0 * chunks) * 8.0, b = (rows(e.lead_id) + 10.0 * chunks) * 8.0;
+    if (r.tran && !r.tran_singular) {
+        const double per = rows(r.ta.tran_tape);
+        w += (per + 40.0 * chunks) * 3.0 * r.ta.steps;
+        b += (per + 10.0 * chunks) * 3.0 * r.ta.steps;
+    }
+    out[0] = 100.0 * w;   // ~100 ns per row of a latency-bound wavefront
+    out[1] = 100.0 * b;
+}
+
+struct Sched {
+    vector<vector<esl::BatchJob>> block, wave;   // [grid], [grid * kCrew]
+    double makespan_ns = 0;
+};
+
+// Deals the jobs longest first: lane-per-block jobs to the workgroup whose
+// block phase ends first, then lane-per-wavefront jobs to the wavefront
+// that is free first (its workgroup's block phase plus its own jobs).
+Sched schedule(const vector<std::pair<esl::BatchJob, std::pair<u32, double>>> &jobs, u32 grid)
+{
+    // jobs: (job, (tiling 0 wave / 1 block, predicted ns))
+    Sched S;
+    S.block.assign(grid, {});
+    S.wave.assign((size_t)grid * esl::kCrew, {});
+    vector<size_t> order(jobs.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t x, size_t y) { return jobs[x].second.second > jobs[y].second.second; });
+    vector<double> bt(grid, 0.0);
+    using QE = std::pair<double, u32>;
+    std::priority_queue<QE, vector<QE>, std::greater<QE>> qb;
+    for (u32 g = 0; g < grid; g++) qb.push({0.0, g});
+    for (size_t i : order) {
+        if (jobs[i].second.first != 1) continue;
+        QE top = qb.top();
+        qb.pop();
+        S.block[top.second].push_back(jobs[i].first);
+        top.first += jobs[i].second.second;
+        bt[top.second] = top.first;
+        qb.push(top);
+    }
+    std::priority_queue<QE, vector<QE>, std::greater<QE>> qw;
+    for (u32 w = 0; w < grid * esl::kCrew; w++) qw.push({bt[w / esl::kCrew], w});
+    for (size_t i : order) {
+        if (jobs[i].second.first != 0) continue;
+        QE top = qw.top();
+        qw.pop();
+        S.wave[top.second].push_back(jobs[i].first);
+        top.first += jobs[i].second.second;
+        qw.push(top);
+    }
+    while (!qw.empty()) {
+        S.makespan_ns = std::max(S.makespan_ns, qw.top().first);
+        qw.pop();
+    }
+    for (double v : bt) S.makespan_ns = std::max(S.makespan_ns, v);
+    return S;
+}
+
+u32 wall_clock_khz()
+{
+    static u32 cached = 0;
+    if (cached) return cached;
+    int v = 0, dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&v, hipDeviceAttributeWallClockRate, dev) == hipSuccess &&
+        v > 0)
+        cached = (u32)v;
+    else
+        cached = 100000;
+    return cached;
+}
+
+u32 device_cu_count()
+{
+    int v = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    return (u32)std::max(v, 0);
+}
+
+esl_status validate_batch_item(const esl_batch_item &it, Engine *&e)
+{
+    e = static_cast<Engine *>(lookup(it.engine, 1));
+    if (!e) return ESL_INVALID_HANDLE;
+    if (it.count == 0 || it.count > 0x7fffffff || !it.x || !it.iterations || !it.reasons) return ESL_INVALID_ARGUMENT;
+    if (e->c.n_live && !it.values) return ESL_INVALID_ARGUMENT;
+    for (size_t i = 0; i < it.count * e->c.n_live; i++)
+        if (!std::isfinite(it.values[i])) return ESL_INVALID_ARGUMENT;
+    if (it.tol.max_iter == 0) return ESL_INVALID_ARGUMENT;
+    if (it.analysis == ESL_BATCH_OP) return ESL_OK;
+    if (it.analysis != ESL_BATCH_TRAN || !it.tran) return ESL_INVALID_ARGUMENT;
+    const esl_tran_options &o = *it.tran;
+    if (o.struct_size < sizeof(esl_tran_options)) return ESL_ABI_MISMATCH;
+    if (!(o.dt > 0) || !std::isfinite(o.dt) || o.steps == 0 || o.tol.max_iter == 0) return ESL_INVALID_ARGUMENT;
+    if (o.n_probes && (!o.probes || !it.probe_out)) return ESL_INVALID_ARGUMENT;
+    for (u32 k = 0; k < o.n_probes; k++)
+        if (o.probes[k] >= e->c.n) return ESL_INVALID_ARGUMENT;
+    return ESL_OK;
+}
+
+// Operating points of one item: uploads, lead tape, Newton arguments; for a
+// transient item also the transient state, the transient lead tape (from
+// the first of the first eight points whose operating point converges on
+// the lead tape) and the step arguments.
+esl_status batch_item_setup(BatchItemRun &r)
+{
+    Engine *e = r.e;
+    const esl_batch_item &it = *r.it;
+    const Compiled &c = e->c;
+    const u32 n = c.n, nnz = c.nnz;
+    const size_t count = it.count;
+    hipStream_t s = e->stream;
+    r.tran = it.analysis == ESL_BATCH_TRAN;
+    esl_op_options oo;
+    esl_default_op_options(&oo);
+    oo.tol = it.tol;   // the operating points' Newton, in both analyses
+    oo.tiling = ESL_TILING_WAVE;
+    esl_status st = op_prepare(*e, it.values, count, oo, nullptr, r.a, r.lane64, r.lead_singular);
+    if (st != ESL_OK || r.lead_singular) return st;
+    r.op_cap = e->tapes[e->lead_id]->prog.n_tv;
+    vector<esl::LaneResume> rs(count, esl::LaneResume{0, 0, e->lead_id, 0, 1e9, 0, 0, 0.0});
+    HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
+    r.a.mode = 0;
+    r.a.dump = 0;
+    r.a.tv_cap = r.op_cap;
+    if (!r.tran) return ESL_OK;
+
+    const esl_tran_options &o = *it.tran;
+    HIPCHK(e->tq1.ensure(count * n * 8));
+    HIPCHK(e->tip.ensure(count * n * 8));
+    HIPCHK(e->tstep.ensure(count * 4));
+    HIPCHK(e->tprobes.ensure(std::max<size_t>(1, count * (size_t)o.steps * o.n_probes) * 8));
+    HIPCHK(e->probe_ids.ensure(std::max<u32>(1, o.n_probes) * 4));
+    if (o.n_probes) HIPCHK(hipMemcpyAsync(e->probe_ids.p, o.probes, o.n_probes * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(e->a_out.ensure(std::max<size_t>(1, count * (size_t)nnz) * 8));
+    esl::TranArgs &ta = r.ta;
+    ta = esl::TranArgs{};
+    ta.op = r.a;
+    ta.op.reltol = o.tol.reltol;
+    ta.op.abstol = o.tol.abstol;
+    ta.op.vntol = o.tol.vntol;
+    ta.op.residual_tol = o.tol.residual_tol;
+    ta.op.max_iter = o.tol.max_iter;
+    ta.op.a_out = e->a_out.as<double>();
+    ta.op.fell_back = nullptr;
+    ta.op.prof = nullptr;
+    ta.q1 = e->tq1.as<double>();
+    ta.ip = e->tip.as<double>();
+    ta.step = e->tstep.as<u32>();
+    ta.probe_ids = e->probe_ids.as<u32>();
+    ta.probes = e->tprobes.as<double>();
+    ta.dt = o.dt;
+    ta.steps = o.steps;
+    ta.n_probes = o.n_probes;
+
+    // the transient lead point
+    const u32 k = (u32)std::min<size_t>(count, 8);
+    float ms = 0;
+    esl_status r2 = launch_lpw(*e, r.a, k, 0, r.op_cap, ms);
+    if (r2 != ESL_OK) return r2;
+    vector<u32> stk(k);
+    HIPCHK(hipMemcpyAsync(stk.data(), e->status.p, k * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    u32 l0 = esl::kNoArgs;
+    for (u32 l = 0; l < k; l++)
+        if (stk[l] == esl::kConverged) { l0 = l; break; }
+    if (l0 == esl::kNoArgs) {
+        g_last_error = "no operating point among a transient item's first eight points converged on its lead tape";
+        return ESL_FAILED;
+    }
+    const u32 running = esl::kTranRunning, zero = 0;
+    HIPCHK(hipMemcpyAsync(e->status.as<u32>() + l0, &running, 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(e->tstep.as<u32>() + l0, &zero, 4, hipMemcpyHostToDevice, s));
+    u32 id = 0;
+    esl_status r3 = tran_lead_tape(*e, ta, l0, e->lead_id, id, r.tran_singular);
+    if (r3 != ESL_OK) return r3;
+    // the probe launches above wrote lanes < k: back to their cold start
+    HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), k * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemsetAsync(e->tstep.p, 0xff, count * 4, s));   // every lane before its transient (kStepOp)
+    vector<u32> ids(count);
+    for (size_t l = 0; l < count; l++) ids[l] = (u32)l;
+    HIPCHK(hipMemcpyAsync(e->lane_ids.p, ids.data(), count * 4, hipMemcpyHostToDevice, s));
+    if (r.tran_singular) return ESL_OK;
+    ta.tran_tape = id;
+    r.tran_cap = e->tapes[id]->prog.n_tv;
+    ta.op.tv_cap = r.tran_cap;
+    ta.init = 1;
+    ta.dump = 0;
+    ta.s0 = 0;
+    ta.s1 = o.steps;
+    return ESL_OK;
+}
+
+// After the persistent launch: re-pivot rounds for operating points, the
+// transient of points whose operating point finished in those rounds, then
+// re-pivot rounds for transient steps; results to the caller.
+esl_status batch_item_finish(BatchItemRun &r, float &ms_out, u32 &repivots)
+{
+    Engine *e = r.e;
+    const esl_batch_item &it = *r.it;
+    const Compiled &c = e->c;
+    const u32 n = c.n;
+    const size_t count = it.count;
+    hipStream_t s = e->stream;
+    const bool tran = r.tran && !r.tran_singular;
+    vector<u32> st(count), its(count, 0), step(count, esl::kStepOp);
+    HIPCHK(hipMemcpyAsync(st.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
+    if (tran) HIPCHK(hipMemcpyAsync(step.data(), e->tstep.p, count * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    float ms = 0;
+    const u32 rep0 = e->timing.repivots;
+    // 1. operating points: lanes that ran their transient count as finished here
+    vector<u32> st_op = st;
+    for (size_t l = 0; l < count; l++)
+        if (step[l] != esl::kStepOp) st_op[l] = esl::kConverged;
+    bool any_op = false;
+    for (size_t l = 0; l < count; l++) any_op |= st_op[l] == esl::kRepivot;
+    if (any_op) {
+        esl_status r1 = op_repivot(*e, r.a, count, st_op, its, r.lane64, ms);
+        if (r1 != ESL_OK) return r1;
+    }
+    for (size_t l = 0; l < count; l++)
+        if (step[l] == esl::kStepOp) st[l] = st_op[l];
+    if (tran) {
+        // 2. the transient of the points that converged in the rounds above
+        vector<u32> late;
+        for (size_t l = 0; l < count; l++)
+            if (step[l] == esl::kStepOp && st[l] == esl::kConverged) late.push_back((u32)l);
+        vector<esl::LaneResume> rs(count);
+        if (!late.empty()) {
+            HIPCHK(hipMemcpyAsync(rs.data(), e->resume.p, count * sizeof(esl::LaneResume), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            for (u32 l : late) {
+                rs[l] = esl::LaneResume{0, 0, r.ta.tran_tape, 0, 1e9, 0, 0, 0.0};
+                st[l] = esl::kTranRunning;
+                step[l] = 0;
+                its[l] = 0;
+            }
+            HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(e->status.p, st.data(), count * 4, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(e->tstep.p, step.data(), count * 4, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(e->iters.p, its.data(), count * 4, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(e->lane_ids.p, late.data(), late.size() * 4, hipMemcpyHostToDevice, s));
+            esl::TranArgs t = r.ta;
+            t.op.tapes = e->tape_table.as<esl::TapeProg>();
+            t.init = 1;
+            const u32 lds = tran_lds_bytes(c, t.op.tv_cap);
+            HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
+            HIPCHK(hipEventRecord(e->ev0, s));
+            HIPCHK(esl_launch_tran_steps(&t, (u32)late.size(), lds, s));
+            HIPCHK(hipEventRecord(e->ev1, s));
+            HIPCHK(hipEventSynchronize(e->ev1));
+            float m2 = 0;
+            HIPCHK(hipEventElapsedTime(&m2, e->ev0, e->ev1));
+            ms += m2;
+        } else {
+            // lanes that ran their transient keep their device statuses
+            HIPCHK(hipMemcpyAsync(e->status.p, st.data(), count * 4, hipMemcpyHostToDevice, s));
+        }
+        // 3. transient re-pivot rounds (one launch covers every step)
+        HIPCHK(hipMemcpyAsync(st.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        float m3 = 0;
+        esl::TranArgs t = r.ta;
+        t.op.tapes = e->tape_table.as<esl::TapeProg>();
+        esl_status r3 = tran_repivot(*e, t, count, r.ta.steps, 1, st, m3);
+        if (r3 != ESL_OK) return r3;
+        ms += m3;
+        HIPCHK(hipMemcpyAsync(step.data(), e->tstep.p, count * 4, hipMemcpyDeviceToHost, s));
+    }
+    // 4. results
+    vector<u32> dev_its(count);
+    HIPCHK(hipMemcpyAsync(it.x, e->x.p, count * n * 8, hipMemcpyDeviceToHost, s));
+    if (tran && it.tran->n_probes)
+        HIPCHK(hipMemcpyAsync(it.probe_out, e->tprobes.p, count * (size_t)it.tran->steps * it.tran->n_probes * 8,
+                              hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(dev_its.data(), e->iters.p, count * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(st.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (r.tran && it.tran->n_probes && !tran)
+        memset(it.probe_out, 0, count * (size_t)it.tran->steps * it.tran->n_probes * 8);
+    for (size_t l = 0; l < count; l++) {
+        u32 reason;
+        switch (st[l]) {
+        case esl::kConverged: reason = ESL_REASON_OK; break;
+        case esl::kSingular: reason = ESL_REASON_SINGULAR_MATRIX; break;
+        case esl::kNoConvergence: reason = ESL_REASON_NO_CONVERGENCE; break;
+        default: reason = ESL_REASON_OTHER; break;
+        }
+        u32 iters = st[l] == esl::kSingular && its[l] ? its[l] : dev_its[l];
+        if (r.tran) {
+            if (!tran) {
+                // the transient lead matrix is singular: as esl_tran_solve
+                if (reason == ESL_REASON_OK) reason = ESL_REASON_SINGULAR_MATRIX;
+                iters = 0;
+            } else if (step[l] == esl::kStepOp) {
+                iters = 0;   // the operating point failed; no transient ran
+            }
+        }
+        it.reasons[l] = reason;
+        it.iterations[l] = iters;
+    }
+    ms_out += ms;
+    repivots += e->timing.repivots - rep0;
+    return ESL_OK;
+}
+
+}  // namespace
+
+extern "C" void esl_default_batch_options(esl_batch_options *o)
+{
+    memset(o, 0, sizeof(*o));
+    o->struct_size = sizeof(*o);
+    o->tiling = ESL_TILING_AUTO;
+}
+
+extern "C" esl_status esl_batch_solve(const esl_batch_item *items, size_t n_items, const esl_batch_options *opt,
+                                      esl_batch_report *report)
+{
+    if (!items || n_items == 0 || n_items > 65536) return ESL_INVALID_ARGUMENT;
+    esl_batch_options o;
+    esl_default_batch_options(&o);
+    if (opt) {
+        if (opt->struct_size < sizeof(esl_batch_options)) return ESL_ABI_MISMATCH;
+        o = *opt;
+    }
+    if (report && report->struct_size < sizeof(esl_batch_report)) return ESL_ABI_MISMATCH;
+    if (o.tiling != ESL_TILING_AUTO && o.tiling != ESL_TILING_WAVE && o.tiling != ESL_TILING_BLOCK)
+        return ESL_INVALID_ARGUMENT;
+    g_last_error.clear();
+    const auto t_start = std::chrono::steady_clock::now();
+    vector<BatchItemRun> runs(n_items);
+    std::set<Engine *> seen;
+    for (size_t i = 0; i < n_items; i++) {
+        esl_status st = validate_batch_item(items[i], runs[i].e);
+        if (st != ESL_OK) return st;
+        if (!seen.insert(runs[i].e).second) return ESL_INVALID_ARGUMENT;   // one item per engine
+        runs[i].it = &items[i];
+    }
+    if (!device_present()) return ESL_DEVICE_UNAVAILABLE;
+    // 1. per item: uploads, lead tapes, arguments
+    for (auto &r : runs) {
+        esl_status st = batch_item_setup(r);
+        if (st != ESL_OK) return st;
+    }
+    // 2. launch shape: four-wavefront workgroups holding all the LDS they may
+    const u32 lds = (device_lds_max() - 256) / (8 * esl::kCrew) * (8 * esl::kCrew);
+    const u32 wave_doubles = lds / 8 / esl::kCrew;
+    HIPCHK(esl_kernel_attrs(lds));
+    int per_cu = 0;
+    HIPCHK(esl_batch_occupancy(lds, &per_cu));
+    const u32 cus = device_cu_count();
+    if (per_cu <= 0 || cus == 0) {
+        g_last_error = "the batch kernel does not fit on a compute unit";
+        return ESL_FAILED;
+    }
+    const u32 grid = cus * (u32)per_cu;
+    // 3. tiling per item: what fits, then the forced one, the tuned one, or both (tuning)
+    vector<esl::OpArgs> ops;
+    vector<esl::TranArgs> trans;
+    bool tuning = false;
+    for (auto &r : runs) {
+        if (r.lead_singular) continue;
+        const Compiled &c = r.e->c;
+        for (u32 k = 0; k < 2; k++) {
+            const u32 nw = k ? esl::kCrew : 1;
+            size_t need = (size_t)esl::lds_doubles(r.op_cap, c.n, nw);
+            if (r.tran && !r.tran_singular)
+                need = std::max<size_t>(need, esl::tran_doubles(r.tran_cap, c.n, (u32)c.cslot_pat.size(), nw));
+            r.need[k] = (u32)std::min<size_t>(need * 8, 0xffffffffu);
+        }
+        const bool fit_w = r.need[0] <= wave_doubles * 8, fit_b = r.need[1] <= lds;
+        if (!fit_w && !fit_b) {
+            g_last_error = "a batch point's working set exceeds the LDS of one workgroup";
+            return ESL_FAILED;
+        }
+        model_job_ns(r, r.est_ns);
+        if (r.e->bt_ns[0] > 0) r.est_ns[0] = r.e->bt_ns[0];
+        if (r.e->bt_ns[1] > 0) r.est_ns[1] = r.e->bt_ns[1];
+        if (!fit_w) r.tiling = 1;
+        else if (!fit_b) r.tiling = 0;
+        else if (o.tiling == ESL_TILING_WAVE) r.tiling = 0;
+        else if (o.tiling == ESL_TILING_BLOCK) r.tiling = 1;
+        else if (r.e->bt_tiling && !o.tune) r.tiling = r.e->bt_tiling == ESL_TILING_BLOCK ? 1 : 0;
+        else {
+            r.tiling = 2;
+            tuning = true;
+        }
+        r.a.tapes = r.e->tape_table.as<esl::TapeProg>();
+        r.op_idx = (u32)ops.size();
+        ops.push_back(r.a);
+        if (r.tran && !r.tran_singular) {
+            r.ta.op.tapes = r.e->tape_table.as<esl::TapeProg>();
+            r.tran_idx = (u32)trans.size();
+            trans.push_back(r.ta);
+        }
+    }
+    // 4. jobs and the schedule
+    vector<std::pair<esl::BatchJob, std::pair<u32, double>>> jobs;
+    for (auto &r : runs) {
+        if (r.lead_singular) continue;
+        for (size_t l = 0; l < r.it->count; l++) {
+            const u32 t = r.tiling == 2 ? (u32)(l & 1) : r.tiling;
+            jobs.push_back({esl::BatchJob{r.op_idx, r.tran_idx, (u32)l, 0}, {t, r.est_ns[t]}});
+        }
+    }
+    Sched S = schedule(jobs, grid);
+    vector<esl::BatchJob> bj, wj;
+    vector<u32> bptr(1, 0), wptr(1, 0);
+    for (auto &v : S.block) {
+        bj.insert(bj.end(), v.begin(), v.end());
+        bptr.push_back((u32)bj.size());
+    }
+    for (auto &v : S.wave) {
+        wj.insert(wj.end(), v.begin(), v.end());
+        wptr.push_back((u32)wj.size());
+    }
+    // every index the kernel follows, checked here
+    for (auto *v : {&bj, &wj})
+        for (auto &j : *v) {
+            if (j.op >= ops.size() || (j.tran != esl::kNoArgs && j.tran >= trans.size())) {
+                g_last_error = "batch schedule names an argument set that does not exist";
+                return ESL_FAILED;
+            }
+        }
+    // 5. upload and the one persistent launch (on the first item's stream)
+    hipStream_t s = runs[0].e->stream;
+    for (auto &r : runs) HIPCHK(hipStreamSynchronize(r.e->stream));
+    static DevBuf g_ops, g_trans, g_bj, g_wj, g_bptr, g_wptr, g_bt, g_wt;
+    HIPCHK(g_ops.ensure(std::max<size_t>(1, ops.size()) * sizeof(esl::OpArgs)));
+    HIPCHK(g_trans.ensure(std::max<size_t>(1, trans.size()) * sizeof(esl::TranArgs)));
+    HIPCHK(g_bj.ensure(std::max<size_t>(1, bj.size()) * sizeof(esl::BatchJob)));
+    HIPCHK(g_wj.ensure(std::max<size_t>(1, wj.size()) * sizeof(esl::BatchJob)));
+    HIPCHK(g_bptr.ensure(bptr.size() * 4));
+    HIPCHK(g_wptr.ensure(wptr.size() * 4));
+    HIPCHK(g_bt.ensure(std::max<size_t>(1, bj.size()) * 8));
+    HIPCHK(g_wt.ensure(std::max<size_t>(1, wj.size()) * 8));
+    HIPCHK(hipMemcpyAsync(g_ops.p, ops.data(), ops.size() * sizeof(esl::OpArgs), hipMemcpyHostToDevice, s));
+    if (!trans.empty())
+        HIPCHK(hipMemcpyAsync(g_trans.p, trans.data(), trans.size() * sizeof(esl::TranArgs), hipMemcpyHostToDevice, s));
+    if (!bj.empty()) HIPCHK(hipMemcpyAsync(g_bj.p, bj.data(), bj.size() * sizeof(esl::BatchJob), hipMemcpyHostToDevice, s));
+    if (!wj.empty()) HIPCHK(hipMemcpyAsync(g_wj.p, wj.data(), wj.size() * sizeof(esl::BatchJob), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(g_bptr.p, bptr.data(), bptr.size() * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(g_wptr.p, wptr.data(), wptr.size() * 4, hipMemcpyHostToDevice, s));
+    esl::BatchArgs B{};
+    B.ops = g_ops.as<esl::OpArgs>();
+    B.trans = g_trans.as<esl::TranArgs>();
+    B.block_jobs = g_bj.as<esl::BatchJob>();
+    B.block_ptr = g_bptr.as<u32>();
+    B.wave_jobs = g_wj.as<esl::BatchJob>();
+    B.wave_ptr = g_wptr.as<u32>();
+    B.wave_doubles = wave_doubles;
+    B.block_ticks = g_bt.as<uint64_t>();
+    B.wave_ticks = g_wt.as<uint64_t>();
+    const double setup_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    Engine *e0 = runs[0].e;
+    HIPCHK(hipEventRecord(e0->ev0, s));
+    HIPCHK(esl_launch_batch(&B, grid, lds, s));
+    HIPCHK(hipEventRecord(e0->ev1, s));
+    HIPCHK(hipEventSynchronize(e0->ev1));
+    float kms = 0;
+    HIPCHK(hipEventElapsedTime(&kms, e0->ev0, e0->ev1));
+    // 6. measured job times: the autotuner's input
+    {
+        vector<uint64_t> bt(bj.size()), wt(wj.size());
+        if (!bj.empty()) HIPCHK(hipMemcpy(bt.data(), g_bt.p, bj.size() * 8, hipMemcpyDeviceToHost));
+        if (!wj.empty()) HIPCHK(hipMemcpy(wt.data(), g_wt.p, wj.size() * 8, hipMemcpyDeviceToHost));
+        const double ns_per_tick = 1e6 / wall_clock_khz();
+        std::map<u32, std::pair<double, u32>> sums[2];   // op_idx -> (ns, jobs)
+        for (size_t j = 0; j < bj.size(); j++) {
+            auto &p = sums[1][bj[j].op];
+            p.first += bt[j] * ns_per_tick;
+            p.second++;
+        }
+        for (size_t j = 0; j < wj.size(); j++) {
+            auto &p = sums[0][wj[j].op];
+            p.first += wt[j] * ns_per_tick;
+            p.second++;
+        }
+        for (auto &r : runs) {
+            if (r.lead_singular) continue;
+            for (u32 k = 0; k < 2; k++) {
+                auto f = sums[k].find(r.op_idx);
+                if (f != sums[k].end() && f->second.second) {
+                    r.e->bt_ns[k] = f->second.first / f->second.second;
+                    r.est_ns[k] = r.e->bt_ns[k];
+                }
+            }
+        }
+        if (tuning) {
+            // per item: lane per wavefront unless lane per block shortens the
+            // predicted makespan of this batch (items by wavefront job time,
+            // longest first)
+            vector<size_t> cand;
+            for (size_t i = 0; i < runs.size(); i++)
+                if (!runs[i].lead_singular && runs[i].tiling == 2) {
+                    runs[i].tiling = 0;
+                    cand.push_back(i);
+                }
+            auto predict = [&]() {
+                vector<std::pair<esl::BatchJob, std::pair<u32, double>>> pj;
+                for (auto &r : runs) {
+                    if (r.lead_singular) continue;
+                    for (size_t l = 0; l < r.it->count; l++)
+                        pj.push_back({esl::BatchJob{r.op_idx, r.tran_idx, (u32)l, 0}, {r.tiling, r.est_ns[r.tiling]}});
+                }
+                return schedule(pj, grid).makespan_ns;
+            };
+            std::stable_sort(cand.begin(), cand.end(),
+                             [&](size_t x, size_t y) { return runs[x].est_ns[0] > runs[y].est_ns[0]; });
+            double best = predict();
+            for (size_t i : cand) {
+                runs[i].tiling = 1;
+                const double v = predict();
+                if (v < best) best = v;
+                else runs[i].tiling = 0;
+            }
+            for (size_t i : cand) runs[i].e->bt_tiling = runs[i].tiling ? ESL_TILING_BLOCK : ESL_TILING_WAVE;
+            S.makespan_ns = best;
+        }
+    }
+    // 7. re-pivots, late transients, results
+    float rep_ms = 0;
+    u32 repivots = 0;
+    for (auto &r : runs) {
+        if (r.lead_singular) {
+            const esl_batch_item &it = *r.it;
+            for (size_t l = 0; l < it.count; l++) {
+                it.reasons[l] = ESL_REASON_SINGULAR_MATRIX;
+                it.iterations[l] = 0;
+            }
+            memset(it.x, 0, it.count * r.e->c.n * 8);
+            continue;
+        }
+        esl_status st = batch_item_finish(r, rep_ms, repivots);
+        if (st != ESL_OK) return st;
+    }
+    if (report) {
+        const size_t ss = report->struct_size;
+        memset(report, 0, sizeof(*report));
+        report->struct_size = ss;
+        report->kernel_ms = kms;
+        report->repivot_ms = rep_ms;
+        report->setup_ms = setup_ms;
+        report->predicted_ms = S.makespan_ns * 1e-6;
+        report->grid = grid;
+        report->block_jobs = (u32)bj.size();
+        report->wave_jobs = (u32)wj.size();
+        report->repivots = repivots;
+        report->lds_bytes = lds;
+        report->total_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    }
     return ESL_OK;
 }
 

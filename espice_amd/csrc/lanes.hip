@@ -45,6 +45,56 @@ __device__ inline double wave_fmax(double v)
 
 __device__ inline bool wave_any(bool p) { return __any((int)p) != 0; }
 
+// A lane's own barrier: orders the accesses of its wavefront's threads to
+// LDS and memory (waits for outstanding accesses) without a workgroup
+// barrier, so wavefronts of one workgroup that run different lanes never
+// wait on each other.
+__device__ inline void wsync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// A lane-per-block lane's crew: wavefront 0 leads the lane; at each
+// assembly round it posts the round's first chunk c0, every wavefront k of
+// the crew evaluates chunk c0 + k into its own staging area, and the leader
+// then reduces the round's chunks in chunk order. Sums stay in instance
+// order, so the result does not depend on the crew's size.
+struct CrewCtl {
+    uint32_t cmd, c0, charges, x_off;
+    uint32_t st_off, pad;
+    double t;
+};
+__shared__ CrewCtl g_crew;
+constexpr uint32_t kCmdEval = 1, kCmdDone = 2;
+
+// Where a crew's helpers find the lane: x and the staging areas as offsets
+// from the lane's working-set base.
+struct Crew {
+    uint32_t nw, x_off, st_off;
+};
+
+__device__ inline void crew_post(const Crew &cr, uint32_t c0, bool charges, double t, uint32_t tid)
+{
+    if (tid == 0) {
+        g_crew.cmd = kCmdEval;
+        g_crew.c0 = c0;
+        g_crew.charges = charges ? 1u : 0u;
+        g_crew.x_off = cr.x_off;
+        g_crew.st_off = cr.st_off;
+        g_crew.t = t;
+    }
+    __syncthreads();
+}
+
+// The leader ends the lane: after this the followers have left crew_follow.
+__device__ inline void crew_done(uint32_t tid)
+{
+    if (tid == 0) g_crew.cmd = kCmdDone;
+    __syncthreads();
+    __syncthreads();
+}
+
 // Per-lane limiting state: MOS1 (vgs, vds, vbs, von) as [4][n_mos], then one
 // junction voltage per diode, then (vbe, vbc) per BJT.
 __device__ inline uint32_t lim_width(const DeckProg &dp) { return 4 * dp.n_mos + dp.n_dio + 2 * dp.n_bjt; }
@@ -157,6 +207,25 @@ __device__ inline void eval_chunk(const DeckProg &dp, const Chunk &ch, const dou
     }
 }
 
+// A crew wavefront other than the leader: evaluates its chunk of each round
+// the leader posts until the leader ends the lane. base: the lane's working
+// set (LDS, or the workgroup's slab for a spilled lane).
+__device__ inline void crew_follow(const DeckProg &dp, const double *live, double *lim, double *base, uint32_t wv,
+                                   uint32_t tid)
+{
+    for (;;) {
+        __syncthreads();
+        const CrewCtl c = g_crew;
+        if (c.cmd != kCmdEval) break;
+        const uint32_t k = c.c0 + wv;
+        if (k < dp.n_chunks)
+            eval_chunk(dp, dp.chunks[k], live, base + c.x_off, lim, dp.n_mos, base + c.st_off + wv * kStageDoubles,
+                       tid, c.t, c.charges != 0);
+        __syncthreads();
+    }
+    __syncthreads();
+}
+
 // Assembly into pattern slots (the Jacobian dump for a host re-pivot): the
 // deck's reduce tasks, one target per thread.
 __device__ void assemble_pattern(const DeckProg &dp, const double *live, const double *x, double *lim,
@@ -165,12 +234,12 @@ __device__ void assemble_pattern(const DeckProg &dp, const double *live, const d
     const uint32_t n = dp.n;
     for (uint32_t i = tid; i < dp.nnz; i += kWave) a[i] = 0.0;
     for (uint32_t i = tid; i < n; i += kWave) rhs[i] = 0.0;
-    __syncthreads();
+    wsync();
     for (uint32_t c = 0; c < dp.n_chunks; c++) {
         const Chunk ch = dp.chunks[c];
         if (ch.kind == kC) continue;   // no static stamps
         eval_chunk(dp, ch, live, x, lim, lim_stride, st, tid);
-        __syncthreads();
+        wsync();
         const uint32_t t1 = dp.chunks[c + 1].task_begin;
         for (uint32_t t = ch.task_begin + tid; t < t1; t += kWave) {
             const uint32_t tgt = dp.task_target[t];
@@ -181,13 +250,13 @@ __device__ void assemble_pattern(const DeckProg &dp, const double *live, const d
             for (uint32_t cc = dp.task_cell[t]; cc < c1; cc++) acc = acc + st[dp.cells[cc]];
             *d = acc;
         }
-        __syncthreads();
+        wsync();
     }
     if (tid == 0) {
         a[dp.diag0_slot] = a[dp.diag0_slot] + 1.0;
         rhs[0] = rhs[0] + x[0];
     }
-    __syncthreads();
+    wsync();
 }
 
 struct RedAcc {
@@ -218,41 +287,47 @@ __device__ inline void red_word(uint32_t w, RedAcc &r, double *tv, double *rhs, 
 
 // Assembly into the tape's factor slots: every chunk evaluates, then each
 // thread runs its reduce words (coalesced rows, 16 rows in flight), adding
-// each target's cells in instance order into a register.
+// each target's cells in instance order into a register. A crew of nw
+// wavefronts evaluates nw chunks per round.
 __device__ void assemble(const DeckProg &dp, const TapeProg &T, const double *live, const double *x, double *lim,
-                         uint32_t lim_stride, double *tv, double *rhs, double *st, uint32_t tid)
+                         uint32_t lim_stride, double *tv, double *rhs, double *st, uint32_t tid, const Crew &cr)
 {
     const uint32_t n = dp.n;
     for (uint32_t i = tid; i < T.n_tv; i += kWave) tv[i] = 0.0;
     for (uint32_t i = tid; i < n; i += kWave) rhs[i] = 0.0;
-    __syncthreads();
-    for (uint32_t c = 0; c < dp.n_chunks; c++) {
-        const Chunk ch = dp.chunks[c];
-        if (ch.kind == kC) continue;   // no static stamps
-        eval_chunk(dp, ch, live, x, lim, lim_stride, st, tid);
-        __syncthreads();
-        const uint32_t *rs = T.red + T.red_off[c] + tid;
-        const uint32_t len = T.red_len[c];
-        RedAcc r{-1, false, 0.0};
-        uint32_t q[kRing];
+    wsync();
+    for (uint32_t c0 = 0; c0 < dp.n_chunks; c0 += cr.nw) {
+        if (cr.nw > 1) crew_post(cr, c0, false, 0.0, tid);
+        eval_chunk(dp, dp.chunks[c0], live, x, lim, lim_stride, st, tid);   // capacitors stage nothing here
+        if (cr.nw > 1) __syncthreads();
+        else wsync();
+        const uint32_t c1 = c0 + cr.nw < dp.n_chunks ? c0 + cr.nw : dp.n_chunks;
+        for (uint32_t c = c0; c < c1; c++) {
+            if (dp.chunks[c].kind == kC) continue;   // no static stamps
+            const double *stc = st + (c - c0) * kStageDoubles;
+            const uint32_t *rs = T.red + T.red_off[c] + tid;
+            const uint32_t len = T.red_len[c];
+            RedAcc r{-1, false, 0.0};
+            uint32_t q[kRing];
 #pragma unroll
-        for (int k = 0; k < kRing; k++) q[k] = rs[k * kWave];
-        for (uint32_t w = 0; w < len; w += kRing) {
+            for (int k = 0; k < kRing; k++) q[k] = rs[k * kWave];
+            for (uint32_t w = 0; w < len; w += kRing) {
 #pragma unroll
-            for (int k = 0; k < kRing; k++) {
-                red_word(q[k], r, tv, rhs, st);
-                q[k] = rs[(w + kRing + k) * kWave];
+                for (int k = 0; k < kRing; k++) {
+                    red_word(q[k], r, tv, rhs, stc);
+                    q[k] = rs[(w + kRing + k) * kWave];
+                }
             }
+            red_flush(r, tv, rhs);
+            wsync();
         }
-        red_flush(r, tv, rhs);
-        __syncthreads();
     }
     if (tid == 0) {
         const uint32_t s0 = T.diag_tv[0];
         tv[s0] = tv[s0] + 1.0;
         rhs[0] = rhs[0] + x[0];
     }
-    __syncthreads();
+    wsync();
 }
 
 // ---- chain streams (refactor and solve) ----
@@ -343,7 +418,7 @@ __device__ bool refactor(const TapeProg &T, uint32_t n, R *tv, uint32_t tid)
 __device__ void solve_neg(const TapeProg &T, uint32_t n, double *tv, const double *rhs, double *y, uint32_t tid)
 {
     for (uint32_t r = tid; r < n; r += kWave) y[T.pinv[r]] = -rhs[r];
-    __syncthreads();
+    wsync();
     LuFlags f{false, 0, 0};
     run_rows<double>(T.sw, T.sw_rows, tv, 0.0, f, tid);
 }
@@ -369,14 +444,14 @@ __device__ bool refine_solve(const OpArgs &a, const TapeProg &T, uint32_t n, flo
     fm = wave_max_u64(fm);
     double prev = __builtin_inf();
     for (uint32_t s = 0; s < a.refine_max; s++) {
-        __syncthreads();
+        wsync();
         for (uint32_t i = tid; i < n; i += kWave) y32[T.pinv[i]] = (float)r[i];
-        __syncthreads();
+        wsync();
         LuFlags f{false, 0, 0};
         run_rows<float>(T.sw, T.sw_rows, tv32, 0.0, f, tid);
-        __syncthreads();
+        wsync();
         for (uint32_t i = tid; i < n; i += kWave) dx[i] = dx[i] + (double)y32[T.qinv[i]];
-        __syncthreads();
+        wsync();
         uint64_t rm = 0, am = 0;
         for (uint32_t i = tid; i < n; i += kWave) {
             double acc = -rhs[i], aa = 0.0;
@@ -434,7 +509,7 @@ __device__ inline void lane_init(const OpArgs &a, uint32_t lane, const double *l
     } else {
         for (uint32_t i = tid; i < n; i += kWave) x[i] = xg[i];
     }
-    __syncthreads();
+    wsync();
     if (a.mode == 2) return;
     if (a.mode == 0) {
         for (uint32_t j = tid; j < nm; j += kWave) {
@@ -452,7 +527,7 @@ __device__ inline void lane_init(const OpArgs &a, uint32_t lane, const double *l
     } else {
         seed_lim_at(dp, live, x, lim, tid);
     }
-    __syncthreads();
+    wsync();
 }
 
 // A lane's Jacobian (pattern order) and residual at its start or saved
@@ -477,23 +552,26 @@ __global__ __launch_bounds__(kWave) void op_dump(OpArgs a)
     for (uint32_t i = tid; i < n; i += kWave) a.f_out[(size_t)blockIdx.x * n + i] = rhs[i];
 }
 
-__global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
+// One lane's operating point, led by one wavefront (the body of op_newton
+// and of the batch kernel's .op jobs); nw > 1: a crew of nw wavefronts
+// shares the device evaluation. Returns the lane's status.
+__device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, double *base, uint32_t nw)
 {
-    extern __shared__ double sm[];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t lane = a.lane_ids[blockIdx.x];
+    const uint32_t tid = threadIdx.x % kWave;
     const DeckProg &dp = a.dp;
     const uint32_t n = dp.n, nm = dp.n_mos;
-    // LDS: [tv (tv_cap) | Y block | rhs | x]. The lane's factor slots start
-    // at tv and its solve vector follows at tv + n_tv (one index space for
-    // the row programs); the Y block holds y, sdiag and xold after the
-    // assembly and the stamp staging area st during it.
-    double *tv = sm;
+    // Working set: [tv (tv_cap) | Y block | rhs | x]. The lane's factor
+    // slots start at tv and its solve vector follows at tv + n_tv (one index
+    // space for the row programs); the Y block holds y, sdiag and xold after
+    // the assembly and the stamp staging areas during it.
+    const uint32_t yb = lds_y_block(n, nw);
+    double *tv = base;
     double *sdiag = tv + a.tv_cap + n;
     double *xold = sdiag + n;
     double *st = tv + a.tv_cap;
-    double *rhs = tv + a.tv_cap + lds_y_block(n);
+    double *rhs = tv + a.tv_cap + yb;
     double *x = rhs + n;
+    const Crew cr{nw, a.tv_cap + yb + n, a.tv_cap};
     const double *live = a.live + (size_t)lane * dp.n_live;
     double *lim = a.lim + (size_t)lane * lim_width(dp);
     lane_init(a, lane, live, x, lim, tid);
@@ -520,9 +598,9 @@ __global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
     // residual did not fall in the first iteration, or max_iter passed
     auto restart_nominal = [&](uint32_t at) {
         for (uint32_t i = tid; i < n; i += kWave) x[i] = a.x_nom[i];
-        __syncthreads();
+        wsync();
         seed_lim_at(dp, live, x, lim, tid);
-        __syncthreads();
+        wsync();
         k0 = at;
         warm = 2;
     };
@@ -543,7 +621,7 @@ __global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
             restart_nominal(iter);
         }
         lap(3);
-        assemble(dp, T, live, x, lim, nm, tv, rhs, st, tid);
+        assemble(dp, T, live, x, lim, nm, tv, rhs, st, tid, cr);
         if (warm == 1) {
             uint64_t fb = 0;
             for (uint32_t i = tid; i < n; i += kWave) {
@@ -575,7 +653,7 @@ __global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
             limit = !(1e3 * scale > 1e9) ? 1e9 : 1e3 * scale;
         }
         for (uint32_t i = tid; i < n; i += kWave) sdiag[i] = tv[T.diag_tv[i]];
-        __syncthreads();
+        wsync();
         lap(0);
         // FP32 lanes: keep the FP64 Jacobian for the residual, round the
         // factor slots to FP32 in place (the FP32 slots and y32 occupy the
@@ -589,7 +667,7 @@ __global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
                 const float v = (float)tv[sl];
                 tv32[sl] = v;
             }
-            __syncthreads();
+            wsync();
             if (refactor<float>(T, n, tv32, tid)) {
                 exit_status = (rs.fresh && iter == iter0) ? kToFp64 : kRepivot;
             } else {
@@ -624,7 +702,7 @@ __global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
             solve_neg(T, n, tv, rhs, y, tid);
         }
         lap(2);
-        __syncthreads();
+        wsync();
         uint64_t worst = 0;
         for (uint32_t i = tid; i < n; i += kWave) {
             const double xo = x[i];
@@ -640,7 +718,7 @@ __global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
             worst = sb > worst ? sb : worst;
         }
         const double scaled = bitsd(wave_max_u64(worst));
-        __syncthreads();
+        wsync();
         bool limited = false;
         for (uint32_t j = tid; j < nm; j += kWave) {
             const Mos m = mos_load(dp, j, live);
@@ -679,11 +757,11 @@ __global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
             iters = iter + 1;
             break;
         }
-        __syncthreads();
+        wsync();
     }
     if (status == kNoConvergence) iters = iter;
     lap(3);
-    __syncthreads();
+    wsync();
     if (a.fell_back && tid == 0) a.fell_back[lane] = warm == 2 ? 1 : 0;
     if (status != kRepivot && status != kToFp64)
         for (uint32_t i = tid; i < n; i += kWave) a.x[(size_t)lane * n + i] = x[i];
@@ -693,6 +771,13 @@ __global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
         if (a.prof)
             for (int p = 0; p < kProfPhases; p++) a.prof[(size_t)lane * kProfPhases + p] += pc[p];
     }
+    return status;
+}
+
+__global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
+{
+    extern __shared__ double sm[];
+    op_lane(a, a.lane_ids[blockIdx.x], sm, 1);
 }
 
 // ---- transient lanes ----
@@ -701,10 +786,12 @@ __global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
 // tape's slot map) and rhs, charge-Jacobian cells into cq, row charges into q
 // (src/device/eval.zig:782-922 with the charge planes of LaneSink), the
 // ground pin, then A = G + ag0 C and F += ag0 (q - q1) - ag1 i_prev
-// (lane_kernels.zig:194-215).
+// (lane_kernels.zig:194-215). q1 and i_prev are the lane's rows in global
+// memory. A crew of nw wavefronts evaluates nw chunks per round.
 __device__ void assemble_tran(const DeckProg &dp, const TapeProg &T, const double *live, const double *x, double *lim,
                               uint32_t nm, double *tv, double *rhs, double *q, double *cq, double *st, double t,
-                              double ag0, double ag1, const double *q1, const double *ip, uint32_t tid)
+                              double ag0, double ag1, const double *q1, const double *ip, uint32_t tid,
+                              const Crew &cr)
 {
     const uint32_t n = dp.n;
     for (uint32_t i = tid; i < T.n_tv; i += kWave) tv[i] = 0.0;
@@ -713,29 +800,34 @@ __device__ void assemble_tran(const DeckProg &dp, const TapeProg &T, const doubl
         q[i] = 0.0;
     }
     for (uint32_t k = tid; k < dp.n_cslot; k += kWave) cq[k] = 0.0;
-    __syncthreads();
-    for (uint32_t c = 0; c < dp.n_chunks; c++) {
-        const Chunk ch = dp.chunks[c];
-        eval_chunk(dp, ch, live, x, lim, nm, st, tid, t, true);
-        __syncthreads();
-        const uint32_t t1 = dp.chunks[c + 1].task_begin;
-        for (uint32_t k = ch.task_begin + tid; k < t1; k += kWave) {
-            const uint32_t tgt = dp.task_target[k];
-            const uint32_t idx = tgt & kTargetMask;
-            double *d = (tgt & kRhsBit) ? &rhs[idx] : (tgt & kQBit) ? &q[idx] : (tgt & kCBit) ? &cq[idx]
-                                                                                               : &tv[T.amap[tgt]];
-            double acc = *d;
-            const uint32_t c1 = dp.task_cell[k + 1];
-            for (uint32_t cc = dp.task_cell[k]; cc < c1; cc++) acc = acc + st[dp.cells[cc]];
-            *d = acc;
+    wsync();
+    for (uint32_t c0 = 0; c0 < dp.n_chunks; c0 += cr.nw) {
+        if (cr.nw > 1) crew_post(cr, c0, true, t, tid);
+        eval_chunk(dp, dp.chunks[c0], live, x, lim, nm, st, tid, t, true);
+        if (cr.nw > 1) __syncthreads();
+        else wsync();
+        const uint32_t c1 = c0 + cr.nw < dp.n_chunks ? c0 + cr.nw : dp.n_chunks;
+        for (uint32_t c = c0; c < c1; c++) {
+            const double *stc = st + (c - c0) * kStageDoubles;
+            const uint32_t t1 = dp.chunks[c + 1].task_begin;
+            for (uint32_t k = dp.chunks[c].task_begin + tid; k < t1; k += kWave) {
+                const uint32_t tgt = dp.task_target[k];
+                const uint32_t idx = tgt & kTargetMask;
+                double *d = (tgt & kRhsBit) ? &rhs[idx] : (tgt & kQBit) ? &q[idx] : (tgt & kCBit) ? &cq[idx]
+                                                                                                   : &tv[T.amap[tgt]];
+                double acc = *d;
+                const uint32_t cc1 = dp.task_cell[k + 1];
+                for (uint32_t cc = dp.task_cell[k]; cc < cc1; cc++) acc = acc + stc[dp.cells[cc]];
+                *d = acc;
+            }
+            wsync();
         }
-        __syncthreads();
     }
     if (tid == 0) {
         tv[T.amap[dp.diag0_slot]] = tv[T.amap[dp.diag0_slot]] + 1.0;
         rhs[0] = rhs[0] + x[0];
     }
-    __syncthreads();
+    wsync();
     for (uint32_t k = tid; k < dp.n_cslot; k += kWave) {
         const uint32_t sl = T.amap[dp.cslot_pat[k]];
         tv[sl] = tv[sl] + ag0 * cq[k];
@@ -745,7 +837,7 @@ __device__ void assemble_tran(const DeckProg &dp, const TapeProg &T, const doubl
         const double base = rhs[i] + d;
         rhs[i] = base - ag1 * ip[i];
     }
-    __syncthreads();
+    wsync();
 }
 
 // Runs steps [s0, s1) of each still-running lane: per step a Newton solve
@@ -755,35 +847,35 @@ __device__ void assemble_tran(const DeckProg &dp, const TapeProg &T, const doubl
 // and limiting state reset to the accepted point. A lane whose replay fails
 // leaves its Jacobian in a_out for a host re-pivot and resumes at the same
 // iterate.
-__global__ __launch_bounds__(kWave) void tran_steps(TranArgs ta)
+__device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, double *base, uint32_t nw)
 {
-    extern __shared__ double sm[];
     const OpArgs &a = ta.op;
-    const uint32_t tid = threadIdx.x;
-    const uint32_t lane = a.lane_ids[blockIdx.x];
+    const uint32_t tid = threadIdx.x % kWave;
     if (a.status[lane] != kTranRunning) return;
     uint32_t s = ta.step[lane];
     if (s >= ta.s1) return;
     if (s < ta.s0) s = ta.s0;
     const DeckProg &dp = a.dp;
     const uint32_t n = dp.n, nm = dp.n_mos;
-    double *tv = sm;
-    double *sdiag = tv + a.tv_cap + n;
-    double *xold = sdiag + n;
+    // Working set: [tv (tv_cap) | Y block (y, xold) | rhs | x | q | cq]
+    const uint32_t yb = tran_y_block(n, nw);
+    double *tv = base;
+    double *xold = tv + a.tv_cap + n;
     double *st = tv + a.tv_cap;
-    double *rhs = tv + a.tv_cap + lds_y_block(n);
+    double *rhs = tv + a.tv_cap + yb;
     double *x = rhs + n;
     double *q = x + n;
-    double *q1 = q + n;
-    double *ip = q1 + n;
-    double *cq = ip + n;
+    double *cq = q + n;
+    double *q1 = ta.q1 + (size_t)lane * n;   // charges and dynamic currents of the last accepted step
+    double *ip = ta.ip + (size_t)lane * n;
+    const Crew cr{nw, a.tv_cap + yb + n, a.tv_cap};
     const double *live = a.live + (size_t)lane * dp.n_live;
     double *lim = a.lim + (size_t)lane * lim_width(dp);
     LaneResume rs = a.resume[lane];
     const TapeProg T = a.tapes[rs.tape];
     double *y = tv + T.n_tv;
     for (uint32_t i = tid; i < n; i += kWave) x[i] = a.x[(size_t)lane * n + i];
-    __syncthreads();
+    wsync();
     if (ta.init) {
         // start at the operating point: limiting state at its own voltages,
         // charges from one evaluation there, no dynamic current
@@ -792,23 +884,18 @@ __global__ __launch_bounds__(kWave) void tran_steps(TranArgs ta)
             q1[i] = 0.0;
             ip[i] = 0.0;
         }
-        __syncthreads();
-        assemble_tran(dp, T, live, x, lim, nm, tv, rhs, q, cq, st, 0.0, 0.0, 0.0, q1, ip, tid);
+        wsync();
+        assemble_tran(dp, T, live, x, lim, nm, tv, rhs, q, cq, st, 0.0, 0.0, 0.0, q1, ip, tid, cr);
         for (uint32_t i = tid; i < n; i += kWave) q1[i] = q[i];
-        __syncthreads();
+        wsync();
         if (ta.dump) {
             // the transient lead point's first Jacobian (step 0, iteration 0)
-            assemble_tran(dp, T, live, x, lim, nm, tv, rhs, q, cq, st, ta.dt, 1.0 / ta.dt, 0.0, q1, ip, tid);
-            for (uint32_t p = tid; p < dp.nnz; p += kWave) a.a_out[(size_t)blockIdx.x * dp.nnz + p] = tv[T.amap[p]];
+            assemble_tran(dp, T, live, x, lim, nm, tv, rhs, q, cq, st, ta.dt, 1.0 / ta.dt, 0.0, q1, ip, tid, cr);
+            for (uint32_t p = tid; p < dp.nnz; p += kWave) a.a_out[p] = tv[T.amap[p]];   // one lane per dump launch
             return;
         }
-    } else {
-        for (uint32_t i = tid; i < n; i += kWave) {
-            q1[i] = ta.q1[(size_t)lane * n + i];
-            ip[i] = ta.ip[(size_t)lane * n + i];
-        }
     }
-    __syncthreads();
+    wsync();
     uint32_t iter0 = rs.iter;   // nonzero only when resuming inside a step
     double limit = rs.limit;
     uint32_t status = kTranRunning;
@@ -820,7 +907,7 @@ __global__ __launch_bounds__(kWave) void tran_steps(TranArgs ta)
         bool converged = false;
         uint32_t iter = iter0;
         for (; iter < a.max_iter; iter++) {
-            assemble_tran(dp, T, live, x, lim, nm, tv, rhs, q, cq, st, t, ag0, ag1, q1, ip, tid);
+            assemble_tran(dp, T, live, x, lim, nm, tv, rhs, q, cq, st, t, ag0, ag1, q1, ip, tid, cr);
             if (iter == 0) {
                 double gm = 1.0;
                 uint64_t xm = 0, fm = 0;
@@ -836,11 +923,10 @@ __global__ __launch_bounds__(kWave) void tran_steps(TranArgs ta)
                 const double scale = fmin(fmax(bitsd(xm), bitsd(fm) * gm), 1e18);
                 limit = !(1e3 * scale > 1e9) ? 1e9 : 1e3 * scale;
             }
-            for (uint32_t i = tid; i < n; i += kWave) sdiag[i] = tv[T.diag_tv[i]];
-            __syncthreads();
+            wsync();
             if (refactor<double>(T, n, tv, tid)) {
                 // the Jacobian again, for the host's re-pivot
-                assemble_tran(dp, T, live, x, lim, nm, tv, rhs, q, cq, st, t, ag0, ag1, q1, ip, tid);
+                assemble_tran(dp, T, live, x, lim, nm, tv, rhs, q, cq, st, t, ag0, ag1, q1, ip, tid, cr);
                 for (uint32_t p = tid; p < dp.nnz; p += kWave) a.a_out[(size_t)lane * dp.nnz + p] = tv[T.amap[p]];
                 if (tid == 0) {
                     LaneResume o = rs;
@@ -853,7 +939,7 @@ __global__ __launch_bounds__(kWave) void tran_steps(TranArgs ta)
                 break;
             }
             solve_neg(T, n, tv, rhs, y, tid);
-            __syncthreads();
+            wsync();
             uint64_t worst = 0;
             for (uint32_t i = tid; i < n; i += kWave) {
                 const double xo = x[i];
@@ -869,7 +955,7 @@ __global__ __launch_bounds__(kWave) void tran_steps(TranArgs ta)
                 worst = sb > worst ? sb : worst;
             }
             const double scaled = bitsd(wave_max_u64(worst));
-            __syncthreads();
+            wsync();
             bool limited = false;
             for (uint32_t j = tid; j < nm; j += kWave) {
                 const Mos m = mos_load(dp, j, live);
@@ -896,7 +982,7 @@ __global__ __launch_bounds__(kWave) void tran_steps(TranArgs ta)
                 }
                 ok = !wave_any(miss);
             }
-            __syncthreads();
+            wsync();
             if (ok) {
                 for (uint32_t i = tid; i < n; i += kWave) x[i] = xold[i];
                 converged = true;
@@ -916,7 +1002,7 @@ __global__ __launch_bounds__(kWave) void tran_steps(TranArgs ta)
         }
         // accept: q holds the charges of the converged point (the last
         // assembly ran at x_old)
-        __syncthreads();
+        wsync();
         for (uint32_t i = tid; i < n; i += kWave) {
             const double d = ag0 * (q[i] - q1[i]);
             ip[i] = d - ag1 * ip[i];
@@ -924,16 +1010,12 @@ __global__ __launch_bounds__(kWave) void tran_steps(TranArgs ta)
         }
         for (uint32_t k = tid; k < ta.n_probes; k += kWave)
             ta.probes[((size_t)lane * ta.steps + s) * ta.n_probes + k] = x[ta.probe_ids[k]];
-        __syncthreads();
+        wsync();
         seed_lim_at(dp, live, x, lim, tid);
-        __syncthreads();
+        wsync();
     }
     if (status == kTranRunning && s >= ta.steps) status = kConverged;
-    for (uint32_t i = tid; i < n; i += kWave) {
-        a.x[(size_t)lane * n + i] = x[i];
-        ta.q1[(size_t)lane * n + i] = q1[i];
-        ta.ip[(size_t)lane * n + i] = ip[i];
-    }
+    for (uint32_t i = tid; i < n; i += kWave) a.x[(size_t)lane * n + i] = x[i];
     if (tid == 0) {
         if (status != kRepivot) {
             LaneResume o = rs;
@@ -943,6 +1025,71 @@ __global__ __launch_bounds__(kWave) void tran_steps(TranArgs ta)
         ta.step[lane] = s;
         a.status[lane] = status;
         a.iterations[lane] = total_it;
+    }
+}
+
+__global__ __launch_bounds__(kWave) void tran_steps(TranArgs ta)
+{
+    extern __shared__ double sm[];
+    tran_lane(ta, ta.op.lane_ids[blockIdx.x], sm, 1);
+}
+
+// ---- the persistent batch ----
+
+// One job on the lane's leader (nw > 1: with a crew): the operating point,
+// then, for a transient job whose point converged, the hand-over to the
+// transient tape and the steps.
+__device__ __forceinline__ void run_job(const BatchArgs &b, const BatchJob &jb, double *base, uint32_t nw)
+{
+    const uint32_t tid = threadIdx.x % kWave;
+    const OpArgs &oa = b.ops[jb.op];
+    const uint32_t st = op_lane(oa, jb.lane, base, nw);
+    if (jb.tran == kNoArgs || st != kConverged) return;
+    const TranArgs &ta = b.trans[jb.tran];
+    if (tid == 0) {
+        LaneResume r{};
+        r.tape = ta.tran_tape;
+        r.limit = 1e9;
+        ta.op.resume[jb.lane] = r;
+        ta.op.status[jb.lane] = kTranRunning;
+        ta.op.iterations[jb.lane] = 0;
+        ta.step[jb.lane] = 0;
+    }
+    wsync();
+    tran_lane(ta, jb.lane, base, nw);
+}
+
+__device__ __forceinline__ void follow_job(const BatchArgs &b, const BatchJob &jb, double *base, uint32_t wv)
+{
+    const OpArgs &oa = b.ops[jb.op];
+    crew_follow(oa.dp, oa.live + (size_t)jb.lane * oa.dp.n_live, oa.lim + (size_t)jb.lane * lim_width(oa.dp), base,
+                wv, threadIdx.x % kWave);
+}
+
+__global__ __launch_bounds__(kWave * kCrew) void batch_run(BatchArgs b)
+{
+    extern __shared__ double sm[];
+    const uint32_t wv = threadIdx.x / kWave, tid = threadIdx.x % kWave;
+    const uint32_t g = blockIdx.x;
+    // lane per block: the whole workgroup on one lane at a time
+    for (uint32_t j = b.block_ptr[g]; j < b.block_ptr[g + 1]; j++) {
+        const BatchJob jb = b.block_jobs[j];
+        if (wv == 0) {
+            const uint64_t t0 = wall_clock64();
+            run_job(b, jb, sm, kCrew);
+            crew_done(tid);
+            if (b.block_ticks && tid == 0) b.block_ticks[j] = wall_clock64() - t0;
+        } else {
+            follow_job(b, jb, sm, wv);
+        }
+    }
+    // lane per wavefront: each wavefront on its own lanes in its share of the LDS
+    const uint32_t w = g * kCrew + wv;
+    double *mine = sm + (size_t)wv * b.wave_doubles;
+    for (uint32_t j = b.wave_ptr[w]; j < b.wave_ptr[w + 1]; j++) {
+        const uint64_t t0 = wall_clock64();
+        run_job(b, b.wave_jobs[j], mine, 1);
+        if (b.wave_ticks && tid == 0) b.wave_ticks[j] = wall_clock64() - t0;
     }
 }
 
@@ -959,10 +1106,10 @@ __global__ __launch_bounds__(kWave) void tape_replay(ReplayArgs r)
     double *y = tv + T.n_tv;
     double *rhs = y + n;
     for (uint32_t i = tid; i < T.n_tv; i += kWave) tv[i] = 0.0;
-    __syncthreads();
+    wsync();
     for (uint32_t p = tid; p < r.nnz; p += kWave) tv[T.amap[p]] = r.a[(size_t)lane * r.nnz + p];
     for (uint32_t i = tid; i < n; i += kWave) rhs[i] = r.rhs[(size_t)lane * n + i];
-    __syncthreads();
+    wsync();
     const bool bad = refactor<double>(T, n, tv, tid);
     solve_neg(T, n, tv, rhs, y, tid);
     if (r.lx)
@@ -1020,6 +1167,18 @@ extern "C" hipError_t esl_launch_tran_steps(const esl::TranArgs *ta, uint32_t gr
     return hipGetLastError();
 }
 
+extern "C" hipError_t esl_launch_batch(const esl::BatchArgs *b, uint32_t grid, uint32_t lds_bytes, hipStream_t s)
+{
+    hipLaunchKernelGGL(esl::batch_run, dim3(grid), dim3(esl::kWave * esl::kCrew), lds_bytes, s, *b);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t esl_batch_occupancy(uint32_t lds_bytes, int *blocks_per_cu)
+{
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, esl::batch_run, esl::kWave * esl::kCrew,
+                                                        lds_bytes);
+}
+
 extern "C" hipError_t esl_kernel_attrs(uint32_t lds_bytes)
 {
     hipError_t e = hipFuncSetAttribute((const void *)esl::op_newton,
@@ -1028,6 +1187,8 @@ extern "C" hipError_t esl_kernel_attrs(uint32_t lds_bytes)
     e = hipFuncSetAttribute((const void *)esl::op_dump, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void *)esl::tran_steps, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void *)esl::batch_run, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) return e;
     return hipFuncSetAttribute((const void *)esl::tape_replay, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds_bytes);

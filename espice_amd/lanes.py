@@ -24,7 +24,8 @@ FAILED, ABI_MISMATCH, INVALID_HANDLE, DEVICE_UNAVAILABLE = 6, 7, 8, 9
 REASON_OK, REASON_NO_CONVERGENCE, REASON_SINGULAR = 0, 1, 3
 FP64, FP32_REFINED = 0, 1
 START_COLD, START_GIVEN, START_LEARNED = 0, 1, 2
-TILING_AUTO, TILING_WAVE, TILING_THREAD = 0, 1, 2
+TILING_AUTO, TILING_WAVE, TILING_THREAD, TILING_BLOCK = 0, 1, 2, 3
+BATCH_OP, BATCH_TRAN = 0, 1
 
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "BUFFER_TOO_SMALL", 3: "OUT_OF_MEMORY",
                 4: "INVALID_QUERY", 5: "RESULT_UNAVAILABLE", 6: "FAILED", 7: "ABI_MISMATCH",
@@ -37,6 +38,7 @@ EXPORTS = [
     "esl_mlp_default_train_options", "esl_mlp_create", "esl_mlp_destroy", "esl_mlp_param_count",
     "esl_mlp_get_params", "esl_mlp_set_params", "esl_mlp_set_scaling", "esl_mlp_gradient", "esl_mlp_train",
     "esl_mlp_predict", "esl_engine_set_warm_start", "esl_default_tran_options", "esl_tran_solve",
+    "esl_default_batch_options", "esl_batch_solve", "esl_error_message",
 ]
 
 
@@ -86,6 +88,24 @@ class TranOptions(C.Structure):
     _fields_ = [("struct_size", C.c_size_t), ("tol", Tolerances), ("dt", C.c_double), ("steps", C.c_uint32),
                 ("steps_per_launch", C.c_uint32), ("probes", C.c_void_p), ("n_probes", C.c_uint32),
                 ("use_graph", C.c_uint32)]
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("engine", C.c_void_p), ("values", C.c_void_p), ("count", C.c_size_t), ("analysis", C.c_uint32),
+                ("reserved0", C.c_uint32), ("tol", Tolerances), ("tran", C.POINTER(TranOptions)),
+                ("x", C.c_void_p), ("probe_out", C.c_void_p), ("iterations", C.c_void_p), ("reasons", C.c_void_p)]
+
+
+class BatchOptions(C.Structure):
+    _fields_ = [("struct_size", C.c_size_t), ("tiling", C.c_uint32), ("tune", C.c_uint32),
+                ("reserved", C.c_uint32 * 6)]
+
+
+class BatchReport(C.Structure):
+    _fields_ = [("struct_size", C.c_size_t), ("kernel_ms", C.c_double), ("repivot_ms", C.c_double),
+                ("setup_ms", C.c_double), ("total_ms", C.c_double), ("predicted_ms", C.c_double),
+                ("grid", C.c_uint32), ("block_jobs", C.c_uint32), ("wave_jobs", C.c_uint32),
+                ("repivots", C.c_uint32), ("lds_bytes", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
 class MlpTrainOptions(C.Structure):
@@ -154,6 +174,11 @@ def load() -> C.CDLL:
     L.esl_tran_solve.argtypes = [vp, vp, C.c_size_t, C.POINTER(TranOptions), vp, vp, vp, vp]
     L.esl_engine_set_warm_start.restype = C.c_uint32
     L.esl_engine_set_warm_start.argtypes = [vp, vp, vp]
+    L.esl_error_message.restype = C.c_uint32
+    L.esl_error_message.argtypes = [C.c_char_p, C.c_size_t, C.c_void_p]
+    L.esl_default_batch_options.argtypes = [C.POINTER(BatchOptions)]
+    L.esl_batch_solve.restype = C.c_uint32
+    L.esl_batch_solve.argtypes = [C.POINTER(BatchItem), C.c_size_t, C.POINTER(BatchOptions), C.POINTER(BatchReport)]
     _lib = L
     return L
 
@@ -162,9 +187,17 @@ def _ptr(a: np.ndarray | None):
     return None if a is None else a.ctypes.data
 
 
+def last_error() -> str:
+    """The calling thread's last failure text from the library."""
+    buf = C.create_string_buffer(512)
+    load().esl_error_message(buf, 512, None)
+    return buf.value.decode(errors="replace")
+
+
 def _check(st: int, what: str):
     if st != OK:
-        raise EslError(st, what)
+        msg = last_error()
+        raise EslError(st, f"{what} ({msg})" if msg else what)
 
 
 def default_options(precision=FP64, start=START_COLD) -> OpOptions:
@@ -286,6 +319,61 @@ class Engine:
         t = Timing()
         _check(load().esl_last_timing(self.h, C.byref(t)), "esl_last_timing")
         return t
+
+
+def solve_batch(jobs, tiling=TILING_AUTO, tune=False):
+    """One persistent launch over several templates (esl_batch_solve).
+
+    jobs: list of dicts {"engine": Engine, "values": [count][n_live],
+    "analysis": BATCH_OP | BATCH_TRAN, and for BATCH_TRAN "dt", "steps",
+    optional "probes", "max_iter"; for BATCH_OP optional "tol"}. Returns
+    (results, report): per job a dict like Engine.solve_op / solve_tran."""
+    L = load()
+    n = len(jobs)
+    items = (BatchItem * n)()
+    keep, out = [], []
+    for k, j in enumerate(jobs):
+        eng = j["engine"]
+        values = np.ascontiguousarray(np.atleast_2d(np.asarray(j["values"], dtype=np.float64)))
+        count = values.shape[0]
+        it = items[k]
+        it.engine, it.values, it.count = eng.h, _ptr(values), count
+        it.analysis = j.get("analysis", BATCH_OP)
+        t = Tolerances()
+        o = OpOptions()
+        L.esl_default_op_options(C.byref(o))
+        t = j.get("tol", o.tol)
+        it.tol = t
+        res = {"x": np.zeros((count, eng.n)), "iterations": np.zeros(count, np.uint32),
+               "reasons": np.zeros(count, np.uint32)}
+        keep.append(values)
+        if it.analysis == BATCH_TRAN:
+            to = TranOptions()
+            L.esl_default_tran_options(C.byref(to))
+            to.dt, to.steps = j["dt"], j["steps"]
+            if j.get("max_iter") is not None:
+                to.tol.max_iter = j["max_iter"]
+            pr = np.ascontiguousarray(j.get("probes", ()), dtype=np.uint32)
+            if len(pr):
+                to.probes, to.n_probes = pr.ctypes.data, len(pr)
+            po = np.zeros((count, j["steps"], max(len(pr), 1)))
+            res["probes"] = po
+            res["_np"] = len(pr)
+            keep += [to, pr, po]
+            it.tran = C.pointer(to)
+            it.probe_out = _ptr(po) if len(pr) else None
+        it.x, it.iterations, it.reasons = _ptr(res["x"]), _ptr(res["iterations"]), _ptr(res["reasons"])
+        out.append(res)
+    bo = BatchOptions()
+    L.esl_default_batch_options(C.byref(bo))
+    bo.tiling, bo.tune = tiling, 1 if tune else 0
+    rep = BatchReport()
+    rep.struct_size = C.sizeof(BatchReport)
+    _check(L.esl_batch_solve(items, n, C.byref(bo), C.byref(rep)), "esl_batch_solve")
+    for res in out:
+        if "probes" in res:
+            res["probes"] = res["probes"][:, :, :res.pop("_np")]
+    return out, rep
 
 
 class Tape:

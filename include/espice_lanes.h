@@ -140,7 +140,9 @@ enum {
 enum {
     ESL_TILING_AUTO = 0,
     ESL_TILING_WAVE = 1,    /* one wavefront per lane, factors in LDS */
-    ESL_TILING_THREAD = 2   /* one thread per lane, state lane-major in HBM */
+    ESL_TILING_THREAD = 2,  /* one thread per lane, state lane-major in HBM */
+    ESL_TILING_BLOCK = 3    /* esl_batch_solve: one workgroup of 4 wavefronts per
+                               lane; device evaluation spread over the four */
 };
 /* esl_op_options.start */
 enum {
@@ -198,6 +200,11 @@ typedef struct {
 typedef struct esl_engine esl_engine;
 
 uint32_t esl_abi_version(void);
+/* The calling thread's last failure, NUL-terminated (empty when none);
+ * *required (may be NULL) receives the size with the terminator. Like
+ * espice_error_message (include/espice.h:239), per thread instead of per
+ * problem. */
+esl_status esl_error_message(char *buf, size_t capacity, size_t *required);
 void esl_default_tolerances(esl_tolerances *tol);
 void esl_default_op_options(esl_op_options *opt);
 
@@ -257,6 +264,67 @@ void esl_default_tran_options(esl_tran_options *opt);
 esl_status esl_tran_solve(esl_engine *e, const double *values, size_t count,
                           const esl_tran_options *opt, double *x, double *probe_out,
                           uint32_t *iterations, uint32_t *reasons);
+
+/* ---- Persistent uneven batches (no reference counterpart: the reference
+ * forms one batch per template and options, executor.zig:480-520,
+ * session.zig:562-590). Points of several templates, .op and transient
+ * mixed, run in ONE launch of a persistent kernel: the host deals every
+ * point (a job) to a workgroup or a wavefront ahead of time, longest first,
+ * so no job is claimed at run time. A job runs lane per wavefront (four
+ * lanes per workgroup, each in a quarter of the LDS) or lane per block (one
+ * lane on the workgroup: the device evaluation spread over its four
+ * wavefronts, the factor and solves in the first); the autotuner picks the
+ * tiling per template from measured job times and keeps it on the engine.
+ * Points whose replay fails are re-pivoted on the host and finish in
+ * further launches of the per-engine kernels. Results equal esl_op_solve /
+ * esl_tran_solve on the same points bit for bit (same lead tapes, same
+ * arithmetic). ---- */
+enum {
+    ESL_BATCH_OP = 0,     /* operating points (cold start, FP64) */
+    ESL_BATCH_TRAN = 1    /* operating points, then a fixed-step transient */
+};
+typedef struct {
+    esl_engine *engine;        /* each engine at most once per batch */
+    const double *values;      /* [count][n_live] */
+    size_t count;
+    uint32_t analysis;         /* ESL_BATCH_* */
+    uint32_t reserved0;
+    esl_tolerances tol;        /* the operating points' Newton, both analyses (esl_tran_solve
+                                  runs its points' with the transient's tolerances and at
+                                  least 100 iterations) */
+    const esl_tran_options *tran;  /* ESL_BATCH_TRAN: dt, steps, tol, probes
+                                      (steps_per_launch and use_graph unused) */
+    double *x;                 /* out [count][n] */
+    double *probe_out;         /* out, ESL_BATCH_TRAN with probes: [count][steps][n_probes] */
+    uint32_t *iterations;      /* out [count] (.op: Newton iterations; transient:
+                                  Newton iterations over all steps) */
+    uint32_t *reasons;         /* out [count] */
+} esl_batch_item;
+typedef struct {
+    size_t struct_size;        /* sizeof(esl_batch_options) */
+    uint32_t tiling;           /* ESL_TILING_AUTO (the autotuner), _WAVE or _BLOCK for every job */
+    uint32_t tune;             /* AUTO: 1 = measure both tilings on this batch
+                                  (the first batch of an engine always does) */
+    uint32_t reserved[6];
+} esl_batch_options;
+void esl_default_batch_options(esl_batch_options *opt);
+typedef struct {
+    size_t struct_size;        /* sizeof(esl_batch_report) */
+    double kernel_ms;          /* the persistent launch, HIP events */
+    double repivot_ms;         /* follow-up launches for re-pivoted points */
+    double setup_ms;           /* host: lead tapes, uploads, schedule */
+    double total_ms;           /* whole call */
+    double predicted_ms;       /* the scheduler's makespan estimate (0 before tuning) */
+    uint32_t grid;             /* workgroups of the persistent launch */
+    uint32_t block_jobs;       /* jobs run lane per block */
+    uint32_t wave_jobs;        /* jobs run lane per wavefront */
+    uint32_t repivots;         /* points re-pivoted on the host */
+    uint32_t lds_bytes;        /* dynamic LDS per workgroup */
+    uint32_t reserved[3];
+} esl_batch_report;
+/* report may be NULL. */
+esl_status esl_batch_solve(const esl_batch_item *items, size_t n_items,
+                           const esl_batch_options *opt, esl_batch_report *report);
 
 /* Device-time split of the last esl_op_solve, milliseconds. */
 typedef struct {
