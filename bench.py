@@ -65,6 +65,10 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tiling", type=int, default=0, help="0 autotune, 1 lane per wavefront, 2 lane per thread")
+    ap.add_argument("--workload", default="ota_array", choices=["ota_array", "mixed_batch"],
+                    help="ota_array: the headline; mixed_batch: 10,000 uneven instances of eight families "
+                         "in one persistent launch")
+    ap.add_argument("--instances", type=int, default=10000)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run one plain device copy of this many MiB (scales the HBM byte counters) and exit")
     return ap.parse_args()
@@ -123,6 +127,9 @@ def main():
         sys.exit(2)
     if args.calibrate_copy_mib:
         calibrate_copy(args.calibrate_copy_mib)
+        return
+    if args.workload == "mixed_batch":
+        run_mixed(args)
         return
     from espice_amd import circuits, lanes
 
@@ -308,6 +315,125 @@ def main():
         "parity": parity,
     }
     print(json.dumps(out))
+
+
+# Newton iteration limit of the mixed batch's operating points, per family:
+# long inverter chains need hundreds of plain-Newton iterations from a cold
+# start (about two per stage), the other families converge within 100.
+MIXED_OP_MAX_ITER = {"inverter_chain": 1000, "inverter_chain_tran": 1000}
+
+
+def run_mixed(args):
+    """The uneven batch: every instance of eight families, .op and 200-step
+    transients mixed, in one esl_batch_solve (one persistent launch, plus
+    follow-up launches only for re-pivoted points). The first call tunes the
+    tiling per template (untimed warmup)."""
+    from espice_amd import circuits, lanes
+    groups = circuits.mixed_batch(args.instances, seed=2024)
+    jobs, engines, meta = [], [], []
+    t0 = time.perf_counter()
+    for g in groups:
+        fam = g["build"]()
+        eng = lanes.Engine(fam.deck.arrays())
+        vals = fam.sample_at(2024, g["instances"])
+        tol = lanes.default_options().tol
+        tol.max_iter = MIXED_OP_MAX_ITER.get(g["family"], 100)
+        j = {"engine": eng, "values": vals, "tol": tol,
+             "analysis": lanes.BATCH_OP if g["analysis"] == "op" else lanes.BATCH_TRAN}
+        if g["analysis"] == "tran":
+            j.update(dt=g["dt"], steps=g["steps"], probes=[eng.n - 1])
+        jobs.append(j)
+        engines.append(eng)
+        meta.append((g, fam))
+    t_build = time.perf_counter() - t0
+    n_inst = sum(len(j["values"]) for j in jobs)
+    _, tune_rep = lanes.solve_batch(jobs, tiling=lanes.TILING_AUTO, tune=True)
+    for _ in range(max(args.warmup - 1, 0)):
+        lanes.solve_batch(jobs)
+    conv, wall, kern, rep_ms, res, rep = 0, 0.0, 0.0, 0.0, None, None
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        res, rep = lanes.solve_batch(jobs)
+        wall += time.perf_counter() - t1
+        kern += rep.kernel_ms
+        rep_ms += rep.repivot_ms
+        conv += sum(int((r["reasons"] == 0).sum()) for r in res)
+    fams = {}
+    for (g, fam), r in zip(meta, res):
+        f = fams.setdefault(g["family"], {"instances": 0, "converged": 0, "templates": 0, "iterations": 0})
+        f["instances"] += len(r["reasons"])
+        f["converged"] += int((r["reasons"] == 0).sum())
+        f["templates"] += 1
+        f["iterations"] += int(r["iterations"].sum())
+    for f in fams.values():
+        f["mean_iterations"] = round(f.pop("iterations") / max(f["instances"], 1), 2)
+    # parity: a few points per template against the oracle, bitwise
+    parity = {"checked": 0, "bitwise_equal": 0, "max_rel_err": 0.0}
+    cpu = None
+    if not args.no_cpu:
+        from oracle import orc
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        t_cpu, n_cpu, conv_cpu = 0.0, 0, 0
+        for (g, fam), j, r in zip(meta, jobs, res):
+            k = min(2, len(j["values"]))
+            vals = j["values"][:k]
+            _, _, q = j["engine"].pattern()
+            c = orc.Circuit(fam.deck.arrays())
+            tc = time.perf_counter()
+            if g["analysis"] == "op":
+                t = j["tol"]
+                ref = c.op(vals, q, threads=min(threads, k),
+                           tol=orc.Tol(t.reltol, t.abstol, t.vntol, t.residual_tol, t.max_iter, 0))
+            else:
+                t = j["tol"]
+                ref = orc.tran(c, vals, q, g["dt"], g["steps"], threads=min(threads, k),
+                               tol_op=orc.Tol(t.reltol, t.abstol, t.vntol, t.residual_tol, t.max_iter, 0))
+            t_cpu += time.perf_counter() - tc
+            n_cpu += k
+            conv_cpu += int((ref["reasons"] == 0).sum())
+            ok = (ref["reasons"] == 0) & (r["reasons"][:k] == 0)
+            parity["checked"] += int(ok.sum())
+            gx, rx = r["x"][:k][ok], ref["x"][ok]
+            parity["bitwise_equal"] += int(sum(np.array_equal(a.view(np.uint64), b.view(np.uint64))
+                                               for a, b in zip(gx, rx)))
+            if ok.any():
+                err = np.abs(gx - rx) / np.maximum(np.abs(rx), 1e-6)
+                parity["max_rel_err"] = max(parity["max_rel_err"], float(err.max()))
+        cpu = {"value": round(conv_cpu / t_cpu, 2), "unit": "points/s", "cores": 1, "kind": "port",
+               "sample": f"the first 2 instances of each of the {len(jobs)} templates ({n_cpu} points) on the "
+                         f"oracle, one thread per call, {t_cpu:.1f} s; not the batch's size mix"}
+    out = {
+        "metric": "converged sweep points/sec, 1 MI355X, uneven persistent batch",
+        "value": round(conv / wall, 1),
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded sizes and parameter draws)",
+        "config": {"workload": f"mixed_batch: {n_inst} instances, {len(jobs)} templates of 8 families, "
+                               f".op and {circuits.MIXED_STEPS}-step transients",
+                   "grid": rep.grid, "lds_bytes": rep.lds_bytes},
+        "kernel_ms_per_step": round(kern / args.steps, 3),
+        "repivot_ms_per_step": round(rep_ms / args.steps, 3),
+        "setup_ms": round(rep.setup_ms, 1),
+        "predicted_ms": round(rep.predicted_ms, 3),
+        "block_jobs": rep.block_jobs,
+        "wave_jobs": rep.wave_jobs,
+        "tuning_run": {"kernel_ms": round(tune_rep.kernel_ms, 3), "block_jobs": tune_rep.block_jobs,
+                       "wave_jobs": tune_rep.wave_jobs},
+        "build_s": round(t_build, 2),
+        "families": fams,
+        "parity": parity,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    for e in engines:
+        e.close()
 
 
 if __name__ == "__main__":

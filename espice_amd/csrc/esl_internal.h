@@ -231,6 +231,7 @@ enum : uint32_t { kTranRunning = 102 };
 struct TranArgs {
     OpArgs op;                  // deck, tapes, lane ids, live, x, lim, resume, status, iterations, tolerances
     double *q1, *ip;            // [N][n] charges of the last accepted step, dynamic currents
+    double *xp;                 // [N][n] the accepted state before the last one (the predictor's)
     uint32_t *step;             // [N] next step to run
     const uint32_t *probe_ids;  // [n_probes]
     double *probes;             // [N][steps][n_probes]

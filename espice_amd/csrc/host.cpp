@@ -1183,7 +1183,7 @@ struct Engine {
     esl_mlp *warm_net = nullptr;   // handle of the attached warm-start network
     u32 lead_id = 0;               // tape of the last solve's lead point
     DevBuf probe_ids, probe_out;
-    DevBuf tq1, tip, tstep, tprobes;   // transient state
+    DevBuf tq1, tip, txp, tstep, tprobes;   // transient state
     bool has_nominal = false;
     // lane-per-thread state, lane-major
     DevBuf live_t, tvg, rhsg, xg, xoldg, yg, sdiagg, limg;
@@ -2265,6 +2265,7 @@ extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t
     // 2. transient state
     HIPCHK(e->tq1.ensure(count * n * 8));
     HIPCHK(e->tip.ensure(count * n * 8));
+    HIPCHK(e->txp.ensure(count * n * 8));
     HIPCHK(e->tstep.ensure(count * 4));
     HIPCHK(e->tprobes.ensure(std::max<size_t>(1, count * (size_t)o.steps * o.n_probes) * 8));
     HIPCHK(e->probe_ids.ensure(std::max<u32>(1, o.n_probes) * 4));
@@ -2296,6 +2297,7 @@ extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t
     a.a_out = e->a_out.as<double>();
     ta.q1 = e->tq1.as<double>();
     ta.ip = e->tip.as<double>();
+    ta.xp = e->txp.as<double>();
     ta.step = e->tstep.as<u32>();
     ta.probe_ids = e->probe_ids.as<u32>();
     ta.probes = e->tprobes.as<double>();
@@ -2575,6 +2577,7 @@ esl_status batch_item_setup(BatchItemRun &r)
     const esl_tran_options &o = *it.tran;
     HIPCHK(e->tq1.ensure(count * n * 8));
     HIPCHK(e->tip.ensure(count * n * 8));
+    HIPCHK(e->txp.ensure(count * n * 8));
     HIPCHK(e->tstep.ensure(count * 4));
     HIPCHK(e->tprobes.ensure(std::max<size_t>(1, count * (size_t)o.steps * o.n_probes) * 8));
     HIPCHK(e->probe_ids.ensure(std::max<u32>(1, o.n_probes) * 4));
@@ -2593,6 +2596,7 @@ esl_status batch_item_setup(BatchItemRun &r)
     ta.op.prof = nullptr;
     ta.q1 = e->tq1.as<double>();
     ta.ip = e->tip.as<double>();
+    ta.xp = e->txp.as<double>();
     ta.step = e->tstep.as<u32>();
     ta.probe_ids = e->probe_ids.as<u32>();
     ta.probes = e->tprobes.as<double>();
@@ -2600,26 +2604,32 @@ esl_status batch_item_setup(BatchItemRun &r)
     ta.steps = o.steps;
     ta.n_probes = o.n_probes;
 
-    // the transient lead point
+    // the transient lead point: the first of the first eight points whose
+    // operating point converges (with re-pivot rounds, as esl_tran_solve)
     const u32 k = (u32)std::min<size_t>(count, 8);
     float ms = 0;
     esl_status r2 = launch_lpw(*e, r.a, k, 0, r.op_cap, ms);
     if (r2 != ESL_OK) return r2;
-    vector<u32> stk(k);
+    vector<u32> stk(k), itk(k, 0);
     HIPCHK(hipMemcpyAsync(stk.data(), e->status.p, k * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    r2 = op_repivot(*e, r.a, k, stk, itk, r.lane64, ms);
+    if (r2 != ESL_OK) return r2;
+    vector<esl::LaneResume> rk(k);
+    HIPCHK(hipMemcpyAsync(rk.data(), e->resume.p, k * sizeof(esl::LaneResume), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     u32 l0 = esl::kNoArgs;
     for (u32 l = 0; l < k; l++)
         if (stk[l] == esl::kConverged) { l0 = l; break; }
     if (l0 == esl::kNoArgs) {
-        g_last_error = "no operating point among a transient item's first eight points converged on its lead tape";
+        g_last_error = "no operating point among the first eight points of a transient item converged";
         return ESL_FAILED;
     }
     const u32 running = esl::kTranRunning, zero = 0;
     HIPCHK(hipMemcpyAsync(e->status.as<u32>() + l0, &running, 4, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(e->tstep.as<u32>() + l0, &zero, 4, hipMemcpyHostToDevice, s));
     u32 id = 0;
-    esl_status r3 = tran_lead_tape(*e, ta, l0, e->lead_id, id, r.tran_singular);
+    esl_status r3 = tran_lead_tape(*e, ta, l0, rk[l0].tape, id, r.tran_singular);
     if (r3 != ESL_OK) return r3;
     // the probe launches above wrote lanes < k: back to their cold start
     HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), k * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
@@ -2675,18 +2685,22 @@ esl_status batch_item_finish(BatchItemRun &r, float &ms_out, u32 &repivots)
             if (step[l] == esl::kStepOp && st[l] == esl::kConverged) late.push_back((u32)l);
         vector<esl::LaneResume> rs(count);
         if (!late.empty()) {
+            // the late lanes start their transient; every other lane keeps
+            // its device state (its transient ran in the persistent launch)
+            vector<u32> dit(count);
             HIPCHK(hipMemcpyAsync(rs.data(), e->resume.p, count * sizeof(esl::LaneResume), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(dit.data(), e->iters.p, count * 4, hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
             for (u32 l : late) {
                 rs[l] = esl::LaneResume{0, 0, r.ta.tran_tape, 0, 1e9, 0, 0, 0.0};
                 st[l] = esl::kTranRunning;
                 step[l] = 0;
-                its[l] = 0;
+                dit[l] = 0;
             }
             HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
             HIPCHK(hipMemcpyAsync(e->status.p, st.data(), count * 4, hipMemcpyHostToDevice, s));
             HIPCHK(hipMemcpyAsync(e->tstep.p, step.data(), count * 4, hipMemcpyHostToDevice, s));
-            HIPCHK(hipMemcpyAsync(e->iters.p, its.data(), count * 4, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(e->iters.p, dit.data(), count * 4, hipMemcpyHostToDevice, s));
             HIPCHK(hipMemcpyAsync(e->lane_ids.p, late.data(), late.size() * 4, hipMemcpyHostToDevice, s));
             esl::TranArgs t = r.ta;
             t.op.tapes = e->tape_table.as<esl::TapeProg>();
@@ -2797,10 +2811,11 @@ extern "C" esl_status esl_batch_solve(const esl_batch_item *items, size_t n_item
     int per_cu = 0;
     HIPCHK(esl_batch_occupancy(lds, &per_cu));
     const u32 cus = device_cu_count();
-    if (per_cu <= 0 || cus == 0) {
-        g_last_error = "the batch kernel does not fit on a compute unit";
+    if (cus == 0) {
+        g_last_error = "no compute units reported";
         return ESL_FAILED;
     }
+    per_cu = std::max(per_cu, 1);   // one workgroup holds all the LDS it asks for
     const u32 grid = cus * (u32)per_cu;
     // 3. tiling per item: what fits, then the forced one, the tuned one, or both (tuning)
     vector<esl::OpArgs> ops;

@@ -868,6 +868,7 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
     double *cq = q + n;
     double *q1 = ta.q1 + (size_t)lane * n;   // charges and dynamic currents of the last accepted step
     double *ip = ta.ip + (size_t)lane * n;
+    double *xp = ta.xp + (size_t)lane * n;   // the accepted state before the last one
     const Crew cr{nw, a.tv_cap + yb + n, a.tv_cap};
     const double *live = a.live + (size_t)lane * dp.n_live;
     double *lim = a.lim + (size_t)lane * lim_width(dp);
@@ -883,6 +884,7 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
         for (uint32_t i = tid; i < n; i += kWave) {
             q1[i] = 0.0;
             ip[i] = 0.0;
+            xp[i] = x[i];
         }
         wsync();
         assemble_tran(dp, T, live, x, lim, nm, tv, rhs, q, cq, st, 0.0, 0.0, 0.0, q1, ip, tid, cr);
@@ -1013,6 +1015,18 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
         wsync();
         seed_lim_at(dp, live, x, lim, tid);
         wsync();
+        // the next step's first iterate: the last two accepted points
+        // extrapolated by dt/dt_prev = 1 (MODEINITPRED, tran.zig:1131-1137;
+        // lane_device.zig:141-153), limited against the accepted one through
+        // the limiting state just seeded
+        if (s + 1 < ta.steps) {
+            for (uint32_t i = tid; i < n; i += kWave) {
+                const double cu = x[i];
+                x[i] = cu + (cu - xp[i]);
+                xp[i] = cu;
+            }
+            wsync();
+        }
     }
     if (status == kTranRunning && s >= ta.steps) status = kConverged;
     for (uint32_t i = tid; i < n; i += kWave) a.x[(size_t)lane * n + i] = x[i];

@@ -240,7 +240,10 @@ esl_status esl_op_solve(esl_engine *e, const double *values, size_t count,
  * (cold start, as esl_op_solve), then integrates `steps` fixed steps of dt:
  * step 0 backward Euler, later steps the trapezoid (xmu = 0.5: ag0 = 2/dt,
  * ag1 = 1, integrator.zig:40-53), each a Newton solve on the device with the
- * .op gate order and the residual gate off (lane_device.zig:74-98). The
+ * .op gate order and the residual gate off (lane_device.zig:74-98), starting
+ * from the last two accepted points extrapolated (x + (x - x_prev), the
+ * predictor of tran.zig:1131-1137 at a fixed step) and limited against the
+ * last accepted point. The
  * lanes share one tape from the first converged point's first transient
  * Jacobian; a lane whose replay fails is re-pivoted on the host and resumes.
  * The launches, each `steps_per_launch` steps, are captured in one hipGraph

@@ -1321,8 +1321,9 @@ static void tran_lane(const tjob_t *j, size_t l)
     lane_dev_init(c, j->values + l * c->d.n_live, &ld);
     double *x = calloc(n + 1, 8), *x_old = calloc(n + 1, 8), *dx = calloc(n + 1, 8);
     double *g = calloc(nnz + 1, 8), *cp = calloc(nnz + 1, 8), *rhs = calloc(n + 1, 8);
-    double *q = calloc(n + 1, 8), *q1 = calloc(n + 1, 8), *ip = calloc(n + 1, 8);
+    double *q = calloc(n + 1, 8), *q1 = calloc(n + 1, 8), *ip = calloc(n + 1, 8), *xp = calloc(n + 1, 8);
     memcpy(x, j->x_op + l * n, (size_t)n * 8);
+    memcpy(xp, x, (size_t)n * 8);
     seed_at(c, &ld, x);
     assemble_t(c, &ld, x, 0.0, 1, g, rhs, cp, q);
     memcpy(q1, q, (size_t)n * 8);
@@ -1383,12 +1384,21 @@ static void tran_lane(const tjob_t *j, size_t l)
         for (uint32_t k = 0; k < j->n_probes; k++)
             j->probe_out[((size_t)l * j->steps + s) * j->n_probes + k] = x[j->probes[k]];
         seed_at(c, &ld, x);
+        /* the next step's first iterate: the last two accepted points
+         * extrapolated by dt/dt_prev = 1 (MODEINITPRED, tran.zig:1131-1137,
+         * predict at tran.zig:37-41), limited against the accepted one */
+        if (s + 1 < j->steps)
+            for (uint32_t i = 0; i < n; i++) {
+                const double cu = x[i];
+                x[i] = cu + (cu - xp[i]);
+                xp[i] = cu;
+            }
     }
     memcpy(j->x + l * n, x, (size_t)n * 8);
     j->iterations[l] = total;
     j->reasons[l] = reason;
     orc_lu_free(lu);
-    free(x); free(x_old); free(dx); free(g); free(cp); free(rhs); free(q); free(q1); free(ip);
+    free(x); free(x_old); free(dx); free(g); free(cp); free(rhs); free(q); free(q1); free(ip); free(xp);
     lane_dev_free(&ld);
 }
 
