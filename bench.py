@@ -69,6 +69,7 @@ def parse():
                     help="ota_array: the headline; mixed_batch: 10,000 uneven instances of eight families "
                          "in one persistent launch")
     ap.add_argument("--instances", type=int, default=10000)
+    ap.add_argument("--families", default="", help="mixed_batch: comma-separated subset of the eight families")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run one plain device copy of this many MiB (scales the HBM byte counters) and exit")
     return ap.parse_args()
@@ -329,7 +330,8 @@ def run_mixed(args):
     follow-up launches only for re-pivoted points). The first call tunes the
     tiling per template (untimed warmup)."""
     from espice_amd import circuits, lanes
-    groups = circuits.mixed_batch(args.instances, seed=2024)
+    groups = circuits.mixed_batch(args.instances, seed=2024,
+                                  families=args.families.split(",") if args.families else None)
     jobs, engines, meta = [], [], []
     t0 = time.perf_counter()
     for g in groups:
@@ -347,7 +349,11 @@ def run_mixed(args):
         meta.append((g, fam))
     t_build = time.perf_counter() - t0
     n_inst = sum(len(j["values"]) for j in jobs)
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    log(f"mixed batch: {n_inst} instances, {len(jobs)} templates built in {t_build:.1f} s")
     _, tune_rep = lanes.solve_batch(jobs, tiling=lanes.TILING_AUTO, tune=True)
+    log(f"tuning batch: kernel {tune_rep.kernel_ms:.1f} ms, re-pivot launches {tune_rep.repivot_ms:.1f} ms, "
+        f"total {tune_rep.total_ms:.1f} ms")
     for _ in range(max(args.warmup - 1, 0)):
         lanes.solve_batch(jobs)
     conv, wall, kern, rep_ms, res, rep = 0, 0.0, 0.0, 0.0, None, None
@@ -358,6 +364,9 @@ def run_mixed(args):
         kern += rep.kernel_ms
         rep_ms += rep.repivot_ms
         conv += sum(int((r["reasons"] == 0).sum()) for r in res)
+        log(f"step: {time.perf_counter() - t1:.3f} s, kernel {rep.kernel_ms:.1f} ms, predicted {rep.predicted_ms:.1f} ms, "
+            f"re-pivot launches {rep.repivot_ms:.1f} ms in {rep.repivot_launches} rounds ({rep.repivots} re-pivots), "
+            f"setup {rep.setup_ms:.1f} ms")
     fams = {}
     for (g, fam), r in zip(meta, res):
         f = fams.setdefault(g["family"], {"instances": 0, "converged": 0, "templates": 0, "iterations": 0})
@@ -365,6 +374,8 @@ def run_mixed(args):
         f["converged"] += int((r["reasons"] == 0).sum())
         f["templates"] += 1
         f["iterations"] += int(r["iterations"].sum())
+    for (g, fam), j in zip(meta, jobs):
+        fams[g["family"]]["tapes"] = fams[g["family"]].get("tapes", 0) + int(j["engine"].info().tapes)
     for f in fams.values():
         f["mean_iterations"] = round(f.pop("iterations") / max(f["instances"], 1), 2)
     # parity: a few points per template against the oracle, bitwise
@@ -390,6 +401,7 @@ def run_mixed(args):
                                tol_op=orc.Tol(t.reltol, t.abstol, t.vntol, t.residual_tol, t.max_iter, 0))
             t_cpu += time.perf_counter() - tc
             n_cpu += k
+            log(f"oracle {g['family']} {g['size']}: {time.perf_counter() - tc:.2f} s")
             conv_cpu += int((ref["reasons"] == 0).sum())
             ok = (ref["reasons"] == 0) & (r["reasons"][:k] == 0)
             parity["checked"] += int(ok.sum())
@@ -420,6 +432,8 @@ def run_mixed(args):
                    "grid": rep.grid, "lds_bytes": rep.lds_bytes},
         "kernel_ms_per_step": round(kern / args.steps, 3),
         "repivot_ms_per_step": round(rep_ms / args.steps, 3),
+        "repivot_launches": rep.repivot_launches,
+        "repivots": rep.repivots,
         "setup_ms": round(rep.setup_ms, 1),
         "predicted_ms": round(rep.predicted_ms, 3),
         "block_jobs": rep.block_jobs,

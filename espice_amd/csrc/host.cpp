@@ -2430,7 +2430,10 @@ struct BatchItemRun {
     u32 need[2] = {0, 0};         // LDS bytes of one lane: [0] lane per wavefront, [1] lane per block
     u32 tiling = 0;               // 0 wave, 1 block for this batch; 2 both (tuning)
     double est_ns[2] = {0.0, 0.0};
-    u32 op_idx = 0, tran_idx = esl::kNoArgs;
+    u32 op_idx = 0, tran_idx = esl::kNoArgs;       // cold .op, transient from the operating point
+    u32 op_res_idx = 0, tran_res_idx = esl::kNoArgs;   // resumed after a re-pivot
+    u32 res_cap = 0;              // factor slots of the largest tape (resumed lanes)
+    vector<u32> its;              // iterations of points that ended singular in a re-pivot
 };
 
 // Model of one job's time before any measurement: the rows a Newton
@@ -2572,6 +2575,7 @@ esl_status batch_item_setup(BatchItemRun &r)
     r.a.mode = 0;
     r.a.dump = 0;
     r.a.tv_cap = r.op_cap;
+    HIPCHK(e->a_out.ensure(std::max<size_t>(1, count * (size_t)nnz) * 8));
     if (!r.tran) return ESL_OK;
 
     const esl_tran_options &o = *it.tran;
@@ -2648,93 +2652,23 @@ esl_status batch_item_setup(BatchItemRun &r)
     return ESL_OK;
 }
 
-// After the persistent launch: re-pivot rounds for operating points, the
-// transient of points whose operating point finished in those rounds, then
-// re-pivot rounds for transient steps; results to the caller.
-esl_status batch_item_finish(BatchItemRun &r, float &ms_out, u32 &repivots)
+// The results of one item after the last round: outputs to the caller and
+// the per-point reasons (a transient item's point whose operating point
+// failed reports that failure with 0 transient iterations).
+esl_status batch_item_results(BatchItemRun &r)
 {
     Engine *e = r.e;
     const esl_batch_item &it = *r.it;
-    const Compiled &c = e->c;
-    const u32 n = c.n;
+    const u32 n = e->c.n;
     const size_t count = it.count;
     hipStream_t s = e->stream;
     const bool tran = r.tran && !r.tran_singular;
-    vector<u32> st(count), its(count, 0), step(count, esl::kStepOp);
-    HIPCHK(hipMemcpyAsync(st.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
-    if (tran) HIPCHK(hipMemcpyAsync(step.data(), e->tstep.p, count * 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    float ms = 0;
-    const u32 rep0 = e->timing.repivots;
-    // 1. operating points: lanes that ran their transient count as finished here
-    vector<u32> st_op = st;
-    for (size_t l = 0; l < count; l++)
-        if (step[l] != esl::kStepOp) st_op[l] = esl::kConverged;
-    bool any_op = false;
-    for (size_t l = 0; l < count; l++) any_op |= st_op[l] == esl::kRepivot;
-    if (any_op) {
-        esl_status r1 = op_repivot(*e, r.a, count, st_op, its, r.lane64, ms);
-        if (r1 != ESL_OK) return r1;
-    }
-    for (size_t l = 0; l < count; l++)
-        if (step[l] == esl::kStepOp) st[l] = st_op[l];
-    if (tran) {
-        // 2. the transient of the points that converged in the rounds above
-        vector<u32> late;
-        for (size_t l = 0; l < count; l++)
-            if (step[l] == esl::kStepOp && st[l] == esl::kConverged) late.push_back((u32)l);
-        vector<esl::LaneResume> rs(count);
-        if (!late.empty()) {
-            // the late lanes start their transient; every other lane keeps
-            // its device state (its transient ran in the persistent launch)
-            vector<u32> dit(count);
-            HIPCHK(hipMemcpyAsync(rs.data(), e->resume.p, count * sizeof(esl::LaneResume), hipMemcpyDeviceToHost, s));
-            HIPCHK(hipMemcpyAsync(dit.data(), e->iters.p, count * 4, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            for (u32 l : late) {
-                rs[l] = esl::LaneResume{0, 0, r.ta.tran_tape, 0, 1e9, 0, 0, 0.0};
-                st[l] = esl::kTranRunning;
-                step[l] = 0;
-                dit[l] = 0;
-            }
-            HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
-            HIPCHK(hipMemcpyAsync(e->status.p, st.data(), count * 4, hipMemcpyHostToDevice, s));
-            HIPCHK(hipMemcpyAsync(e->tstep.p, step.data(), count * 4, hipMemcpyHostToDevice, s));
-            HIPCHK(hipMemcpyAsync(e->iters.p, dit.data(), count * 4, hipMemcpyHostToDevice, s));
-            HIPCHK(hipMemcpyAsync(e->lane_ids.p, late.data(), late.size() * 4, hipMemcpyHostToDevice, s));
-            esl::TranArgs t = r.ta;
-            t.op.tapes = e->tape_table.as<esl::TapeProg>();
-            t.init = 1;
-            const u32 lds = tran_lds_bytes(c, t.op.tv_cap);
-            HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
-            HIPCHK(hipEventRecord(e->ev0, s));
-            HIPCHK(esl_launch_tran_steps(&t, (u32)late.size(), lds, s));
-            HIPCHK(hipEventRecord(e->ev1, s));
-            HIPCHK(hipEventSynchronize(e->ev1));
-            float m2 = 0;
-            HIPCHK(hipEventElapsedTime(&m2, e->ev0, e->ev1));
-            ms += m2;
-        } else {
-            // lanes that ran their transient keep their device statuses
-            HIPCHK(hipMemcpyAsync(e->status.p, st.data(), count * 4, hipMemcpyHostToDevice, s));
-        }
-        // 3. transient re-pivot rounds (one launch covers every step)
-        HIPCHK(hipMemcpyAsync(st.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        float m3 = 0;
-        esl::TranArgs t = r.ta;
-        t.op.tapes = e->tape_table.as<esl::TapeProg>();
-        esl_status r3 = tran_repivot(*e, t, count, r.ta.steps, 1, st, m3);
-        if (r3 != ESL_OK) return r3;
-        ms += m3;
-        HIPCHK(hipMemcpyAsync(step.data(), e->tstep.p, count * 4, hipMemcpyDeviceToHost, s));
-    }
-    // 4. results
-    vector<u32> dev_its(count);
+    vector<u32> st(count), dev_its(count), step(count, esl::kStepOp);
     HIPCHK(hipMemcpyAsync(it.x, e->x.p, count * n * 8, hipMemcpyDeviceToHost, s));
     if (tran && it.tran->n_probes)
         HIPCHK(hipMemcpyAsync(it.probe_out, e->tprobes.p, count * (size_t)it.tran->steps * it.tran->n_probes * 8,
                               hipMemcpyDeviceToHost, s));
+    if (tran) HIPCHK(hipMemcpyAsync(step.data(), e->tstep.p, count * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(dev_its.data(), e->iters.p, count * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(st.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -2748,7 +2682,7 @@ esl_status batch_item_finish(BatchItemRun &r, float &ms_out, u32 &repivots)
         case esl::kNoConvergence: reason = ESL_REASON_NO_CONVERGENCE; break;
         default: reason = ESL_REASON_OTHER; break;
         }
-        u32 iters = st[l] == esl::kSingular && its[l] ? its[l] : dev_its[l];
+        u32 iters = st[l] == esl::kSingular && r.its[l] ? r.its[l] : dev_its[l];
         if (r.tran) {
             if (!tran) {
                 // the transient lead matrix is singular: as esl_tran_solve
@@ -2761,8 +2695,188 @@ esl_status batch_item_finish(BatchItemRun &r, float &ms_out, u32 &repivots)
         it.reasons[l] = reason;
         it.iterations[l] = iters;
     }
-    ms_out += ms;
-    repivots += e->timing.repivots - rep0;
+    return ESL_OK;
+}
+
+// One re-pivot pass over an item: every point that left the last launch for
+// a new pivot order gets a tape from the Jacobian it wrote at its iterate
+// (deduplicated by pivot sequence) and a job that resumes it (an operating
+// point, then its transient; or a transient at its step). Singular
+// matrices end the point.
+esl_status batch_item_repivot(BatchItemRun &r, vector<esl::BatchJob> &resume_jobs, u32 &repivots)
+{
+    Engine *e = r.e;
+    const Compiled &c = e->c;
+    const u32 n = c.n, nnz = c.nnz;
+    const size_t count = r.it->count;
+    hipStream_t s = e->stream;
+    const bool tran = r.tran && !r.tran_singular;
+    vector<u32> st(count), step(count, esl::kStepOp);
+    HIPCHK(hipMemcpyAsync(st.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
+    if (tran) HIPCHK(hipMemcpyAsync(step.data(), e->tstep.p, count * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    vector<u32> again;
+    for (size_t l = 0; l < count; l++)
+        if (st[l] == esl::kRepivot) again.push_back((u32)l);
+    if (again.empty()) return ESL_OK;
+    vector<esl::LaneResume> rs(count);
+    HIPCHK(hipMemcpyAsync(rs.data(), e->resume.p, count * sizeof(esl::LaneResume), hipMemcpyDeviceToHost, s));
+    vector<double> A(nnz);
+    for (u32 l : again) {
+        HIPCHK(hipMemcpyAsync(A.data(), e->a_out.as<double>() + (size_t)l * nnz, nnz * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        HostTape t;
+        repivots++;
+        const bool op_phase = step[l] == esl::kStepOp;
+        if (!gp_factor(n, c.col_ptr.data(), c.row_idx.data(), A.data(), c.q.data(), 1e-3, t)) {
+            st[l] = esl::kSingular;
+            if (op_phase) r.its[l] = rs[l].iter + 1;
+            continue;
+        }
+        u32 id;
+        esl_status s2 = add_tape(*e, t, id);
+        if (s2 != ESL_OK) return s2;
+        rs[l].tape = id;
+        if (op_phase) {
+            rs[l].fresh = 1;
+            resume_jobs.push_back(esl::BatchJob{r.op_res_idx, tran ? r.tran_idx : esl::kNoArgs, l, 0});
+        } else {
+            st[l] = esl::kTranRunning;
+            resume_jobs.push_back(esl::BatchJob{esl::kNoArgs, r.tran_res_idx, l, 0});
+        }
+    }
+    HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(e->status.p, st.data(), count * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return ESL_OK;
+}
+
+// The argument sets of every item for the next launch: .op from a cold
+// start and resumed, transient from the operating point and resumed (the
+// tape table and the largest tape may have changed since the last launch).
+void batch_args(vector<BatchItemRun> &runs, vector<esl::OpArgs> &ops, vector<esl::TranArgs> &trans)
+{
+    ops.clear();
+    trans.clear();
+    for (auto &r : runs) {
+        if (r.lead_singular) continue;
+        Engine *e = r.e;
+        u32 cap_all = 0;
+        for (auto *t : e->tapes) cap_all = std::max(cap_all, t->prog.n_tv);
+        r.res_cap = cap_all;
+        esl::OpArgs a0 = r.a;
+        a0.tapes = e->tape_table.as<esl::TapeProg>();
+        a0.mode = 0;
+        a0.dump = 0;
+        a0.tv_cap = r.op_cap;
+        a0.a_out = e->a_out.as<double>();
+        r.op_idx = (u32)ops.size();
+        ops.push_back(a0);
+        esl::OpArgs a2 = a0;
+        a2.mode = 2;
+        a2.tv_cap = cap_all;
+        r.op_res_idx = (u32)ops.size();
+        ops.push_back(a2);
+        r.tran_idx = r.tran_res_idx = esl::kNoArgs;
+        if (r.tran && !r.tran_singular) {
+            esl::TranArgs t1 = r.ta;
+            t1.op.tapes = e->tape_table.as<esl::TapeProg>();
+            t1.op.a_out = e->a_out.as<double>();
+            t1.op.tv_cap = r.tran_cap;
+            t1.init = 1;
+            r.tran_idx = (u32)trans.size();
+            trans.push_back(t1);
+            esl::TranArgs t2 = t1;
+            t2.op.tv_cap = cap_all;
+            t2.init = 0;
+            r.tran_res_idx = (u32)trans.size();
+            trans.push_back(t2);
+        }
+    }
+}
+
+// LDS bytes of a job in each tiling ([0] lane per wavefront, [1] lane per
+// block): its .op part on op_cap slots and its transient part on tran_cap
+// slots (0: absent).
+void job_need(const Compiled &c, u32 op_cap, u32 tran_cap, u32 need[2])
+{
+    for (u32 k = 0; k < 2; k++) {
+        const u32 nw = k ? esl::kCrew : 1;
+        size_t d = 0;
+        if (op_cap) d = std::max<size_t>(d, esl::lds_doubles(op_cap, c.n, nw));
+        if (tran_cap) d = std::max<size_t>(d, esl::tran_doubles(tran_cap, c.n, (u32)c.cslot_pat.size(), nw));
+        need[k] = (u32)std::min<size_t>(d * 8, 0xffffffffu);
+    }
+}
+
+// Uploads the argument sets and the dealt jobs and runs one persistent
+// launch; per-job durations come back in ticks when asked.
+struct BatchLaunch {
+    DevBuf ops, trans, bj, wj, bptr, wptr, bt, wt;
+};
+
+esl_status batch_launch(BatchLaunch &L, const vector<esl::OpArgs> &ops, const vector<esl::TranArgs> &trans,
+                        const Sched &S, u32 grid, u32 lds, bool ticks, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1,
+                        vector<esl::BatchJob> &bj, vector<esl::BatchJob> &wj, vector<uint64_t> &bt,
+                        vector<uint64_t> &wt, float &ms)
+{
+    bj.clear();
+    wj.clear();
+    vector<u32> bptr(1, 0), wptr(1, 0);
+    for (auto &v : S.block) {
+        bj.insert(bj.end(), v.begin(), v.end());
+        bptr.push_back((u32)bj.size());
+    }
+    for (auto &v : S.wave) {
+        wj.insert(wj.end(), v.begin(), v.end());
+        wptr.push_back((u32)wj.size());
+    }
+    // every index the kernel follows, checked here
+    for (auto *v : {&bj, &wj})
+        for (auto &j : *v) {
+            const bool op_ok = j.op == esl::kNoArgs ? j.tran != esl::kNoArgs : j.op < ops.size();
+            const bool tr_ok = j.tran == esl::kNoArgs || j.tran < trans.size();
+            if (!op_ok || !tr_ok) {
+                g_last_error = "batch schedule names an argument set that does not exist";
+                return ESL_FAILED;
+            }
+        }
+    HIPCHK(L.ops.ensure(std::max<size_t>(1, ops.size()) * sizeof(esl::OpArgs)));
+    HIPCHK(L.trans.ensure(std::max<size_t>(1, trans.size()) * sizeof(esl::TranArgs)));
+    HIPCHK(L.bj.ensure(std::max<size_t>(1, bj.size()) * sizeof(esl::BatchJob)));
+    HIPCHK(L.wj.ensure(std::max<size_t>(1, wj.size()) * sizeof(esl::BatchJob)));
+    HIPCHK(L.bptr.ensure(bptr.size() * 4));
+    HIPCHK(L.wptr.ensure(wptr.size() * 4));
+    HIPCHK(L.bt.ensure(std::max<size_t>(1, bj.size()) * 8));
+    HIPCHK(L.wt.ensure(std::max<size_t>(1, wj.size()) * 8));
+    HIPCHK(hipMemcpyAsync(L.ops.p, ops.data(), ops.size() * sizeof(esl::OpArgs), hipMemcpyHostToDevice, s));
+    if (!trans.empty())
+        HIPCHK(hipMemcpyAsync(L.trans.p, trans.data(), trans.size() * sizeof(esl::TranArgs), hipMemcpyHostToDevice, s));
+    if (!bj.empty()) HIPCHK(hipMemcpyAsync(L.bj.p, bj.data(), bj.size() * sizeof(esl::BatchJob), hipMemcpyHostToDevice, s));
+    if (!wj.empty()) HIPCHK(hipMemcpyAsync(L.wj.p, wj.data(), wj.size() * sizeof(esl::BatchJob), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(L.bptr.p, bptr.data(), bptr.size() * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(L.wptr.p, wptr.data(), wptr.size() * 4, hipMemcpyHostToDevice, s));
+    esl::BatchArgs B{};
+    B.ops = L.ops.as<esl::OpArgs>();
+    B.trans = L.trans.as<esl::TranArgs>();
+    B.block_jobs = L.bj.as<esl::BatchJob>();
+    B.block_ptr = L.bptr.as<u32>();
+    B.wave_jobs = L.wj.as<esl::BatchJob>();
+    B.wave_ptr = L.wptr.as<u32>();
+    B.wave_doubles = lds / 8 / esl::kCrew;
+    B.block_ticks = ticks ? L.bt.as<uint64_t>() : nullptr;
+    B.wave_ticks = ticks ? L.wt.as<uint64_t>() : nullptr;
+    HIPCHK(hipEventRecord(ev0, s));
+    HIPCHK(esl_launch_batch(&B, grid, lds, s));
+    HIPCHK(hipEventRecord(ev1, s));
+    HIPCHK(hipEventSynchronize(ev1));
+    HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+    if (ticks) {
+        bt.resize(bj.size());
+        wt.resize(wj.size());
+        if (!bj.empty()) HIPCHK(hipMemcpy(bt.data(), L.bt.p, bj.size() * 8, hipMemcpyDeviceToHost));
+        if (!wj.empty()) HIPCHK(hipMemcpy(wt.data(), L.wt.p, wj.size() * 8, hipMemcpyDeviceToHost));
+    }
     return ESL_OK;
 }
 
@@ -2803,10 +2917,11 @@ extern "C" esl_status esl_batch_solve(const esl_batch_item *items, size_t n_item
     for (auto &r : runs) {
         esl_status st = batch_item_setup(r);
         if (st != ESL_OK) return st;
+        r.its.assign(r.it->count, 0);
     }
     // 2. launch shape: four-wavefront workgroups holding all the LDS they may
     const u32 lds = (device_lds_max() - 256) / (8 * esl::kCrew) * (8 * esl::kCrew);
-    const u32 wave_doubles = lds / 8 / esl::kCrew;
+    const u32 wave_bytes = lds / esl::kCrew / 8 * 8;
     HIPCHK(esl_kernel_attrs(lds));
     int per_cu = 0;
     HIPCHK(esl_batch_occupancy(lds, &per_cu));
@@ -2818,20 +2933,11 @@ extern "C" esl_status esl_batch_solve(const esl_batch_item *items, size_t n_item
     per_cu = std::max(per_cu, 1);   // one workgroup holds all the LDS it asks for
     const u32 grid = cus * (u32)per_cu;
     // 3. tiling per item: what fits, then the forced one, the tuned one, or both (tuning)
-    vector<esl::OpArgs> ops;
-    vector<esl::TranArgs> trans;
     bool tuning = false;
     for (auto &r : runs) {
         if (r.lead_singular) continue;
-        const Compiled &c = r.e->c;
-        for (u32 k = 0; k < 2; k++) {
-            const u32 nw = k ? esl::kCrew : 1;
-            size_t need = (size_t)esl::lds_doubles(r.op_cap, c.n, nw);
-            if (r.tran && !r.tran_singular)
-                need = std::max<size_t>(need, esl::tran_doubles(r.tran_cap, c.n, (u32)c.cslot_pat.size(), nw));
-            r.need[k] = (u32)std::min<size_t>(need * 8, 0xffffffffu);
-        }
-        const bool fit_w = r.need[0] <= wave_doubles * 8, fit_b = r.need[1] <= lds;
+        job_need(r.e->c, r.op_cap, r.tran && !r.tran_singular ? r.tran_cap : 0, r.need);
+        const bool fit_w = r.need[0] <= wave_bytes, fit_b = r.need[1] <= lds;
         if (!fit_w && !fit_b) {
             g_last_error = "a batch point's working set exceeds the LDS of one workgroup";
             return ESL_FAILED;
@@ -2848,15 +2954,10 @@ extern "C" esl_status esl_batch_solve(const esl_batch_item *items, size_t n_item
             r.tiling = 2;
             tuning = true;
         }
-        r.a.tapes = r.e->tape_table.as<esl::TapeProg>();
-        r.op_idx = (u32)ops.size();
-        ops.push_back(r.a);
-        if (r.tran && !r.tran_singular) {
-            r.ta.op.tapes = r.e->tape_table.as<esl::TapeProg>();
-            r.tran_idx = (u32)trans.size();
-            trans.push_back(r.ta);
-        }
     }
+    vector<esl::OpArgs> ops;
+    vector<esl::TranArgs> trans;
+    batch_args(runs, ops, trans);
     // 4. jobs and the schedule
     vector<std::pair<esl::BatchJob, std::pair<u32, double>>> jobs;
     for (auto &r : runs) {
@@ -2867,67 +2968,22 @@ extern "C" esl_status esl_batch_solve(const esl_batch_item *items, size_t n_item
         }
     }
     Sched S = schedule(jobs, grid);
-    vector<esl::BatchJob> bj, wj;
-    vector<u32> bptr(1, 0), wptr(1, 0);
-    for (auto &v : S.block) {
-        bj.insert(bj.end(), v.begin(), v.end());
-        bptr.push_back((u32)bj.size());
-    }
-    for (auto &v : S.wave) {
-        wj.insert(wj.end(), v.begin(), v.end());
-        wptr.push_back((u32)wj.size());
-    }
-    // every index the kernel follows, checked here
-    for (auto *v : {&bj, &wj})
-        for (auto &j : *v) {
-            if (j.op >= ops.size() || (j.tran != esl::kNoArgs && j.tran >= trans.size())) {
-                g_last_error = "batch schedule names an argument set that does not exist";
-                return ESL_FAILED;
-            }
-        }
-    // 5. upload and the one persistent launch (on the first item's stream)
     hipStream_t s = runs[0].e->stream;
     for (auto &r : runs) HIPCHK(hipStreamSynchronize(r.e->stream));
-    static DevBuf g_ops, g_trans, g_bj, g_wj, g_bptr, g_wptr, g_bt, g_wt;
-    HIPCHK(g_ops.ensure(std::max<size_t>(1, ops.size()) * sizeof(esl::OpArgs)));
-    HIPCHK(g_trans.ensure(std::max<size_t>(1, trans.size()) * sizeof(esl::TranArgs)));
-    HIPCHK(g_bj.ensure(std::max<size_t>(1, bj.size()) * sizeof(esl::BatchJob)));
-    HIPCHK(g_wj.ensure(std::max<size_t>(1, wj.size()) * sizeof(esl::BatchJob)));
-    HIPCHK(g_bptr.ensure(bptr.size() * 4));
-    HIPCHK(g_wptr.ensure(wptr.size() * 4));
-    HIPCHK(g_bt.ensure(std::max<size_t>(1, bj.size()) * 8));
-    HIPCHK(g_wt.ensure(std::max<size_t>(1, wj.size()) * 8));
-    HIPCHK(hipMemcpyAsync(g_ops.p, ops.data(), ops.size() * sizeof(esl::OpArgs), hipMemcpyHostToDevice, s));
-    if (!trans.empty())
-        HIPCHK(hipMemcpyAsync(g_trans.p, trans.data(), trans.size() * sizeof(esl::TranArgs), hipMemcpyHostToDevice, s));
-    if (!bj.empty()) HIPCHK(hipMemcpyAsync(g_bj.p, bj.data(), bj.size() * sizeof(esl::BatchJob), hipMemcpyHostToDevice, s));
-    if (!wj.empty()) HIPCHK(hipMemcpyAsync(g_wj.p, wj.data(), wj.size() * sizeof(esl::BatchJob), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(g_bptr.p, bptr.data(), bptr.size() * 4, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(g_wptr.p, wptr.data(), wptr.size() * 4, hipMemcpyHostToDevice, s));
-    esl::BatchArgs B{};
-    B.ops = g_ops.as<esl::OpArgs>();
-    B.trans = g_trans.as<esl::TranArgs>();
-    B.block_jobs = g_bj.as<esl::BatchJob>();
-    B.block_ptr = g_bptr.as<u32>();
-    B.wave_jobs = g_wj.as<esl::BatchJob>();
-    B.wave_ptr = g_wptr.as<u32>();
-    B.wave_doubles = wave_doubles;
-    B.block_ticks = g_bt.as<uint64_t>();
-    B.wave_ticks = g_wt.as<uint64_t>();
     const double setup_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    static BatchLaunch g_launch;
     Engine *e0 = runs[0].e;
-    HIPCHK(hipEventRecord(e0->ev0, s));
-    HIPCHK(esl_launch_batch(&B, grid, lds, s));
-    HIPCHK(hipEventRecord(e0->ev1, s));
-    HIPCHK(hipEventSynchronize(e0->ev1));
+    vector<esl::BatchJob> bj, wj;
+    vector<uint64_t> bt, wt;
     float kms = 0;
-    HIPCHK(hipEventElapsedTime(&kms, e0->ev0, e0->ev1));
-    // 6. measured job times: the autotuner's input
     {
-        vector<uint64_t> bt(bj.size()), wt(wj.size());
-        if (!bj.empty()) HIPCHK(hipMemcpy(bt.data(), g_bt.p, bj.size() * 8, hipMemcpyDeviceToHost));
-        if (!wj.empty()) HIPCHK(hipMemcpy(wt.data(), g_wt.p, wj.size() * 8, hipMemcpyDeviceToHost));
+        esl_status st = batch_launch(g_launch, ops, trans, S, grid, lds, true, s, e0->ev0, e0->ev1, bj, wj, bt, wt, kms);
+        if (st != ESL_OK) return st;
+    }
+    const u32 n_block = (u32)bj.size(), n_wave = (u32)wj.size();
+    // 5. measured job times: the autotuner's input
+    {
         const double ns_per_tick = 1e6 / wall_clock_khz();
         std::map<u32, std::pair<double, u32>> sums[2];   // op_idx -> (ns, jobs)
         for (size_t j = 0; j < bj.size(); j++) {
@@ -2982,9 +3038,48 @@ extern "C" esl_status esl_batch_solve(const esl_batch_item *items, size_t n_item
             S.makespan_ns = best;
         }
     }
-    // 7. re-pivots, late transients, results
+    // 6. re-pivot rounds: every point that needs a new pivot order, over all
+    // items, resumes in one more persistent launch per round
     float rep_ms = 0;
-    u32 repivots = 0;
+    u32 repivots = 0, rounds = 0;
+    for (int round = 0; round < 1000; round++) {
+        vector<esl::BatchJob> resumed;
+        vector<size_t> owner;
+        for (size_t i = 0; i < runs.size(); i++) {
+            if (runs[i].lead_singular) continue;
+            const size_t before = resumed.size();
+            esl_status st = batch_item_repivot(runs[i], resumed, repivots);
+            if (st != ESL_OK) return st;
+            owner.insert(owner.end(), resumed.size() - before, i);
+        }
+        if (resumed.empty()) break;
+        batch_args(runs, ops, trans);   // tape tables grew
+        // the job indices were taken before batch_args renumbered nothing:
+        // argument sets keep their order, so the indices stay valid
+        vector<std::pair<esl::BatchJob, std::pair<u32, double>>> rj;
+        for (size_t k = 0; k < resumed.size(); k++) {
+            const BatchItemRun &r = runs[owner[k]];
+            const esl::BatchJob &jb = resumed[k];
+            u32 need[2];
+            const bool tr = jb.tran != esl::kNoArgs;
+            job_need(r.e->c, jb.op == esl::kNoArgs ? 0 : r.res_cap,
+                     tr ? (jb.op == esl::kNoArgs ? r.res_cap : r.tran_cap) : 0, need);
+            u32 t = r.tiling == 1 ? 1u : 0u;
+            if (t == 0 && need[0] > wave_bytes) t = 1;
+            if (t == 1 && need[1] > lds) {
+                g_last_error = "a re-pivoted point's working set exceeds the LDS of one workgroup";
+                return ESL_FAILED;
+            }
+            rj.push_back({jb, {t, r.est_ns[t]}});
+        }
+        Sched R = schedule(rj, grid);
+        float ms = 0;
+        esl_status st = batch_launch(g_launch, ops, trans, R, grid, lds, false, s, e0->ev0, e0->ev1, bj, wj, bt, wt, ms);
+        if (st != ESL_OK) return st;
+        rep_ms += ms;
+        rounds++;
+    }
+    // 7. results
     for (auto &r : runs) {
         if (r.lead_singular) {
             const esl_batch_item &it = *r.it;
@@ -2995,7 +3090,7 @@ extern "C" esl_status esl_batch_solve(const esl_batch_item *items, size_t n_item
             memset(it.x, 0, it.count * r.e->c.n * 8);
             continue;
         }
-        esl_status st = batch_item_finish(r, rep_ms, repivots);
+        esl_status st = batch_item_results(r);
         if (st != ESL_OK) return st;
     }
     if (report) {
@@ -3007,9 +3102,10 @@ extern "C" esl_status esl_batch_solve(const esl_batch_item *items, size_t n_item
         report->setup_ms = setup_ms;
         report->predicted_ms = S.makespan_ns * 1e-6;
         report->grid = grid;
-        report->block_jobs = (u32)bj.size();
-        report->wave_jobs = (u32)wj.size();
+        report->block_jobs = n_block;
+        report->wave_jobs = n_wave;
         report->repivots = repivots;
+        report->repivot_launches = rounds;
         report->lds_bytes = lds;
         report->total_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();

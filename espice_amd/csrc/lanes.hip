@@ -693,6 +693,13 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
                 o.f0 = f0;
                 a.resume[lane] = o;
             }
+            if (a.a_out && exit_status == kRepivot) {
+                // the Jacobian at this iterate (pattern order) for the host's
+                // re-pivot; the device evaluation reads x and the limiting
+                // state exactly as the failed assembly did
+                wsync();
+                assemble_pattern(dp, live, x, lim, nm, a.a_out + (size_t)lane * dp.nnz, rhs, st, tid);
+            }
             status = exit_status;
             iters = iter;
             break;
@@ -1056,6 +1063,11 @@ __global__ __launch_bounds__(kWave) void tran_steps(TranArgs ta)
 __device__ __forceinline__ void run_job(const BatchArgs &b, const BatchJob &jb, double *base, uint32_t nw)
 {
     const uint32_t tid = threadIdx.x % kWave;
+    if (jb.op == kNoArgs) {
+        // a transient resumed at its saved step and iterate
+        tran_lane(b.trans[jb.tran], jb.lane, base, nw);
+        return;
+    }
     const OpArgs &oa = b.ops[jb.op];
     const uint32_t st = op_lane(oa, jb.lane, base, nw);
     if (jb.tran == kNoArgs || st != kConverged) return;
@@ -1075,7 +1087,7 @@ __device__ __forceinline__ void run_job(const BatchArgs &b, const BatchJob &jb, 
 
 __device__ __forceinline__ void follow_job(const BatchArgs &b, const BatchJob &jb, double *base, uint32_t wv)
 {
-    const OpArgs &oa = b.ops[jb.op];
+    const OpArgs &oa = jb.op == kNoArgs ? b.trans[jb.tran].op : b.ops[jb.op];
     crew_follow(oa.dp, oa.live + (size_t)jb.lane * oa.dp.n_live, oa.lim + (size_t)jb.lane * lim_width(oa.dp), base,
                 wv, threadIdx.x % kWave);
 }

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define ESL_ABI_VERSION 2u
+#define ESL_ABI_VERSION 3u
 
 /* esl_status: the values of espice_status (include/espice.h:21-38). */
 typedef uint32_t esl_status;
@@ -278,8 +278,9 @@ esl_status esl_tran_solve(esl_engine *e, const double *values, size_t count,
  * lane on the workgroup: the device evaluation spread over its four
  * wavefronts, the factor and solves in the first); the autotuner picks the
  * tiling per template from measured job times and keeps it on the engine.
- * Points whose replay fails are re-pivoted on the host and finish in
- * further launches of the per-engine kernels. Results equal esl_op_solve /
+ * A point whose replay fails leaves its Jacobian; the host factors it into
+ * a new tape and the point resumes, with every other such point of the
+ * batch, in one further persistent launch per round. Results equal esl_op_solve /
  * esl_tran_solve on the same points bit for bit (same lead tapes, same
  * arithmetic). ---- */
 enum {
@@ -321,9 +322,10 @@ typedef struct {
     uint32_t grid;             /* workgroups of the persistent launch */
     uint32_t block_jobs;       /* jobs run lane per block */
     uint32_t wave_jobs;        /* jobs run lane per wavefront */
-    uint32_t repivots;         /* points re-pivoted on the host */
+    uint32_t repivots;         /* re-pivots on the host (a point may need several) */
     uint32_t lds_bytes;        /* dynamic LDS per workgroup */
-    uint32_t reserved[3];
+    uint32_t repivot_launches; /* persistent launches after the first (one per round) */
+    uint32_t reserved[2];
 } esl_batch_report;
 /* report may be NULL. */
 esl_status esl_batch_solve(const esl_batch_item *items, size_t n_items,

@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
 def test_abi_version_and_defaults():
     from espice_amd import lanes
     lib = lanes.load()
-    assert lib.esl_abi_version() == 2
+    assert lib.esl_abi_version() == 3
     o = lanes.default_options()
     assert o.struct_size == ctypes.sizeof(lanes.OpOptions)
     assert o.tol.reltol == 1e-3 and o.tol.abstol == 1e-12 and o.tol.vntol == 1e-6
@@ -55,3 +55,31 @@ def test_invalid_network_handles_are_rejected():
     assert lib.esl_mlp_destroy(ctypes.c_void_p(777)) == lanes.INVALID_HANDLE
     o = lanes.default_train_options()
     assert o.struct_size == ctypes.sizeof(lanes.MlpTrainOptions) and o.epochs > 0 and o.batch > 0
+
+
+def test_batch_defaults_and_argument_checks():
+    """Defaults of the batch options, and the checks that run before any
+    device work (no GPU needed)."""
+    from espice_amd import lanes
+    lib = lanes.load()
+    o = lanes.BatchOptions()
+    lib.esl_default_batch_options(ctypes.byref(o))
+    assert o.struct_size == ctypes.sizeof(lanes.BatchOptions) and o.tiling == lanes.TILING_AUTO and o.tune == 0
+    rep = lanes.BatchReport()
+    rep.struct_size = ctypes.sizeof(lanes.BatchReport)
+    assert lib.esl_batch_solve(None, 1, ctypes.byref(o), ctypes.byref(rep)) == lanes.INVALID_ARGUMENT
+    items = (lanes.BatchItem * 1)()
+    assert lib.esl_batch_solve(items, 0, ctypes.byref(o), ctypes.byref(rep)) == lanes.INVALID_ARGUMENT
+    bad = lanes.BatchOptions()
+    bad.struct_size = 8
+    assert lib.esl_batch_solve(items, 1, ctypes.byref(bad), ctypes.byref(rep)) == lanes.ABI_MISMATCH
+    # a null engine handle
+    assert lib.esl_batch_solve(items, 1, ctypes.byref(o), ctypes.byref(rep)) == lanes.INVALID_HANDLE
+
+
+def test_error_message_reports_the_last_failure():
+    from espice_amd import lanes
+    lib = lanes.load()
+    need = ctypes.c_size_t(0)
+    assert lib.esl_error_message(None, 0, ctypes.byref(need)) == lanes.OK and need.value >= 1
+    assert isinstance(lanes.last_error(), str)
