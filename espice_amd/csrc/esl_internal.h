@@ -92,6 +92,9 @@ struct DeckProg {
     const uint32_t *v_wkind; const double *v_v2, *v_td;
     // charge-Jacobian slots: the k-th is pattern slot cslot_pat[k]
     const uint32_t *cslot_pat;
+    // the pattern by columns ([n + 1], [nnz] rows) and the column order
+    // of the factors: a lane's own re-pivot reads them
+    const uint32_t *csc_ptr, *csc_row, *colq;
 };
 constexpr uint32_t kNoSlot = 0xffffffffu;
 
@@ -193,6 +196,23 @@ struct LaneResume {
     double f0;
 };
 
+// A lane's own LU (the device re-pivot, esl_privlu.h). cap == 0: off, a
+// lane whose replay fails leaves for the host.
+// Per-engine arrays, lane-strided; `cap` entries of L and of U per lane.
+struct PrivLU {
+    uint32_t cap, n, nnz, pad;
+    uint32_t *hdr;                                  // [N][4]: nnz_l, nnz_u, n_void_slots, 0
+    uint32_t *pinv, *flags, *flag, *topo, *stack, *pstack, *want, *want_of;   // [N][n]
+    uint32_t *lp, *up;                              // [N][n + 1]
+    uint32_t *prow, *void_slots;                    // [N][nnz]
+    uint32_t *li, *ui;                              // [N][cap]
+    double *lx, *ux;                                // [N][cap]
+    double *udiag, *w, *rscale;                     // [N][n]; w is all zero between calls
+    double *ap;                                     // [N][nnz] the lane's Jacobian, pattern order
+};
+
+constexpr uint32_t kPrivTape = 0xfffffffeu;   // LaneResume::tape of a lane on its own LU
+
 // Arguments of the Newton kernel (op_newton).
 struct OpArgs {
     DeckProg dp;
@@ -220,6 +240,8 @@ struct OpArgs {
     // optional: [N][kProfPhases] shader cycles per phase (assemble, refactor,
     // solve, update + limits + gates), summed over the lane's iterations
     unsigned long long *prof;
+    // the lanes' own LU for re-pivots on the device (FP64 lanes)
+    PrivLU priv;
 };
 constexpr int kProfPhases = 4;
 

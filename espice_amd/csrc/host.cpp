@@ -19,6 +19,7 @@
 #include <mutex>
 #include <queue>
 #include <set>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -1184,6 +1185,7 @@ struct Engine {
     u32 lead_id = 0;               // tape of the last solve's lead point
     DevBuf probe_ids, probe_out;
     DevBuf tq1, tip, txp, tstep, tprobes;   // transient state
+    DevBuf pl_u32, pl_f64;                  // the lanes' own LU (device re-pivot)
     bool has_nominal = false;
     // lane-per-thread state, lane-major
     DevBuf live_t, tvg, rhsg, xg, xoldg, yg, sdiagg, limg;
@@ -1227,6 +1229,7 @@ hipError_t upload_deck(Engine &e)
            o_cv = pk.add(c.c_val), o_cb = pk.add(c.c_bind), o_vw = pk.add(c.v_wkind), o_v2 = pk.add(c.v_v2),
            o_vtd = pk.add(c.v_td), o_csp = pk.add(c.cslot_pat), o_qt = pk.add(c.q_t), o_qp = pk.add(c.q_par),
            o_qbi = pk.add(c.q_bind_is), o_qbb = pk.add(c.q_bind_bf);
+    size_t o_ccp = pk.add(c.col_ptr), o_ccr = pk.add(c.row_idx), o_cq = pk.add(c.q);
     hipError_t err = e.deck_buf.ensure(pk.blob.size());
     if (err != hipSuccess) return err;
     err = hipMemcpy(e.deck_buf.p, pk.blob.data(), pk.blob.size(), hipMemcpyHostToDevice);
@@ -1284,7 +1287,56 @@ hipError_t upload_deck(Engine &e)
     dp.q_par = reinterpret_cast<const double *>(b + o_qp);
     dp.q_bind_is = reinterpret_cast<const int32_t *>(b + o_qbi);
     dp.q_bind_bf = reinterpret_cast<const int32_t *>(b + o_qbb);
+    dp.csc_ptr = reinterpret_cast<const u32 *>(b + o_ccp);
+    dp.csc_row = reinterpret_cast<const u32 *>(b + o_ccr);
+    dp.colq = reinterpret_cast<const u32 *>(b + o_cq);
     return hipSuccess;
+}
+
+// The lanes' own LU for `count` lanes (device re-pivot): index and value
+// arrays carved from two allocations, L and U each with room for four
+// times the lead tape's entries plus the pattern. w starts all zero.
+hipError_t priv_setup(Engine &e, size_t count, u32 nnz_l, u32 nnz_u, hipStream_t s, esl::PrivLU &P)
+{
+    const size_t n = e.c.n, nnz = e.c.nnz, N = count;
+    const size_t cap = 4 * ((size_t)nnz_l + nnz_u) + nnz + 64;
+    if (cap > 0xffffffffu) return hipErrorInvalidValue;
+    const size_t u32s = N * (4 + 8 * n + 2 * (n + 1) + 2 * nnz + 2 * cap);
+    const size_t f64s = N * (2 * cap + 3 * n + nnz);
+    hipError_t er = e.pl_u32.ensure(u32s * 4);
+    if (er != hipSuccess) return er;
+    er = e.pl_f64.ensure(f64s * 8);
+    if (er != hipSuccess) return er;
+    u32 *u = e.pl_u32.as<u32>();
+    double *f = e.pl_f64.as<double>();
+    P = esl::PrivLU{};
+    P.cap = (u32)cap;
+    P.n = (u32)n;
+    P.nnz = (u32)nnz;
+    auto take_u = [&](size_t per) { u32 *p = u; u += N * per; return p; };
+    auto take_f = [&](size_t per) { double *p = f; f += N * per; return p; };
+    P.hdr = take_u(4);
+    P.pinv = take_u(n);
+    P.flags = take_u(n);
+    P.flag = take_u(n);
+    P.topo = take_u(n);
+    P.stack = take_u(n);
+    P.pstack = take_u(n);
+    P.want = take_u(n);
+    P.want_of = take_u(n);
+    P.lp = take_u(n + 1);
+    P.up = take_u(n + 1);
+    P.prow = take_u(nnz);
+    P.void_slots = take_u(nnz);
+    P.li = take_u(cap);
+    P.ui = take_u(cap);
+    P.lx = take_f(cap);
+    P.ux = take_f(cap);
+    P.udiag = take_f(n);
+    P.w = take_f(n);
+    P.rscale = take_f(n);
+    P.ap = take_f(nnz);
+    return hipMemsetAsync(P.w, 0, N * n * 8, s);
 }
 
 u32 lds_bytes_for(u32 tv_cap, u32 n)
@@ -1474,29 +1526,46 @@ u32 device_lds_max()
     return cached;
 }
 
-// Registers a tape program (deduplicated by pivot sequence and flags).
-esl_status add_tape(Engine &e, const HostTape &t, u32 &id)
+// The host half of registering a tape: its row programs and reduce
+// streams, checked. No device calls, so it runs on worker threads.
+TapeHostProg *build_tape(const Engine &e, const HostTape &t, const char *&err)
 {
-    vector<u32> key = t.pinv;
-    key.insert(key.end(), t.flags.begin(), t.flags.end());
-    auto it = e.tape_index.find(key);
-    if (it != e.tape_index.end()) {
-        id = it->second;
-        return ESL_OK;
-    }
     auto *tp = new TapeHostProg();
     build_tape_prog(t, e.c.col_ptr, 1e-12, 1.0 / 1e-3, *tp);
     build_reduce_streams(e.c, *tp);
-    if (!validate_tape(*tp, e.c.n, e.c.nnz)) {
+    err = nullptr;
+    if (!validate_tape(*tp, e.c.n, e.c.nnz)) err = "tape program failed validation";
+    else if (lds_bytes_for(tp->prog.n_tv, e.c.n) > device_lds_max()) err = "lane state exceeds the LDS of one workgroup";
+    if (err) {
         delete tp;
-        g_last_error = "tape program failed validation";
-        return ESL_FAILED;
+        return nullptr;
     }
-    if (lds_bytes_for(tp->prog.n_tv, e.c.n) > device_lds_max()) {
-        delete tp;
-        g_last_error = "lane state exceeds the LDS of one workgroup";
-        return ESL_FAILED;
-    }
+    return tp;
+}
+
+vector<u32> tape_key(const HostTape &t)
+{
+    vector<u32> key = t.pinv;
+    key.insert(key.end(), t.flags.begin(), t.flags.end());
+    return key;
+}
+
+esl_status upload_tape_table(Engine &e)
+{
+    vector<esl::TapeProg> table;
+    for (auto *p : e.tapes) table.push_back(p->prog);
+    hipError_t er = e.tape_table.ensure(table.size() * sizeof(esl::TapeProg));
+    if (er != hipSuccess) return fail_hip(er, "tape table");
+    er = hipMemcpy(e.tape_table.p, table.data(), table.size() * sizeof(esl::TapeProg), hipMemcpyHostToDevice);
+    if (er != hipSuccess) return fail_hip(er, "tape table");
+    return ESL_OK;
+}
+
+// The device half: uploads the programs and gives the tape its id (the
+// table of tape descriptors is uploaded by the caller, once per batch of
+// new tapes).
+esl_status register_tape(Engine &e, vector<u32> key, TapeHostProg *tp, u32 &id)
+{
     hipError_t er = upload_tape_prog(*tp);
     if (er != hipSuccess) {
         delete tp;
@@ -1504,15 +1573,47 @@ esl_status add_tape(Engine &e, const HostTape &t, u32 &id)
     }
     id = (u32)e.tapes.size();
     e.tapes.push_back(tp);
-    e.tape_index[key] = id;
-    vector<esl::TapeProg> table;
-    for (auto *p : e.tapes) table.push_back(p->prog);
-    er = e.tape_table.ensure(table.size() * sizeof(esl::TapeProg));
-    if (er != hipSuccess) return fail_hip(er, "tape table");
-    er = hipMemcpy(e.tape_table.p, table.data(), table.size() * sizeof(esl::TapeProg), hipMemcpyHostToDevice);
-    if (er != hipSuccess) return fail_hip(er, "tape table");
+    e.tape_index[std::move(key)] = id;
     e.lds_max = std::max(e.lds_max, lds_bytes_for(tp->prog.n_tv, e.c.n));
     return ESL_OK;
+}
+
+// Registers a tape program (deduplicated by pivot sequence and flags).
+esl_status add_tape(Engine &e, const HostTape &t, u32 &id)
+{
+    vector<u32> key = tape_key(t);
+    auto it = e.tape_index.find(key);
+    if (it != e.tape_index.end()) {
+        id = it->second;
+        return ESL_OK;
+    }
+    const char *err = nullptr;
+    TapeHostProg *tp = build_tape(e, t, err);
+    if (!tp) {
+        g_last_error = err;
+        return ESL_FAILED;
+    }
+    esl_status st = register_tape(e, std::move(key), tp, id);
+    if (st != ESL_OK) return st;
+    return upload_tape_table(e);
+}
+
+// Runs f(k) for k in [0, count) on up to 16 host threads.
+template <typename F>
+void parallel_for(size_t count, F f)
+{
+    const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
+    const size_t nt = std::min<size_t>({count, hw, 16});
+    if (nt <= 1) {
+        for (size_t k = 0; k < count; k++) f(k);
+        return;
+    }
+    vector<std::thread> pool;
+    for (size_t t = 0; t < nt; t++)
+        pool.emplace_back([&, t]() {
+            for (size_t k = t; k < count; k += nt) f(k);
+        });
+    for (auto &th : pool) th.join();
 }
 
 void clear_tapes(Engine &e)
@@ -1850,6 +1951,10 @@ esl_status op_prepare(Engine &E, const double *values, size_t count, const esl_o
         e->lead_levels = e->tapes[id]->prog.n_levels;
         e->lead_lds = lds_bytes_for(e->tapes[id]->prog.n_tv, n);
     }
+    // FP64 solves re-pivot on the device, each lane on its own LU
+    a.priv = esl::PrivLU{};
+    if (o.precision == ESL_FP64 && !(o.flags & ESL_OP_HOST_REPIVOT))
+        HIPCHK(priv_setup(*e, count, e->lead_nl, e->lead_nu, s, a.priv));
     e->timing.lead_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_lead0).count();
     return ESL_OK;
@@ -1997,6 +2102,10 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
         return ESL_OK;
     }
     const u32 lead_cap = e->tapes[e->lead_id]->prog.n_tv;
+    // the main launch runs the plain kernel; the lanes' own LU, when set up,
+    // serves the lanes whose replay fails
+    const esl::PrivLU own = a.priv;
+    a.priv = esl::PrivLU{};
     vector<esl::LaneResume> rs(count);
     for (auto &r : rs) r = esl::LaneResume{0, 0, e->lead_id, 0, 1e9, 0, 0, 0.0};
     HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), count * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
@@ -2042,6 +2151,27 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
     e->timing.launches++;
     HIPCHK(hipMemcpyAsync(st.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    if (own.cap) {
+        // lanes whose replay failed resume in the kernel with their own LU:
+        // they re-pivot on the device, as often as they need
+        vector<u32> again;
+        for (size_t l = 0; l < count; l++)
+            if (st[l] == esl::kRepivot) again.push_back((u32)l);
+        if (!again.empty()) {
+            HIPCHK(hipMemcpyAsync(e->lane_ids.p, again.data(), again.size() * 4, hipMemcpyHostToDevice, s));
+            u32 cap = 0;
+            for (auto *t : e->tapes) cap = std::max(cap, t->prog.n_tv);
+            HostOpArgs ao = a;
+            ao.priv = own;
+            esl_status r = launch_lpw(*e, ao, (u32)again.size(), 2, cap, ms);
+            if (r != ESL_OK) return r;
+            kernel_ms += ms;
+            e->timing.launches++;
+            e->timing.device_repivot_lanes += (u32)again.size();
+            HIPCHK(hipMemcpyAsync(st.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+        }
+    }
     {
         esl_status r = op_repivot(*e, a, count, st, its, lane64, kernel_ms);
         if (r != ESL_OK) return r;
@@ -2304,6 +2434,7 @@ extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t
     ta.dt = o.dt;
     ta.steps = o.steps;
     ta.n_probes = o.n_probes;
+    HIPCHK(priv_setup(*e, count, e->lead_nl, e->lead_nu, s, a.priv));
 
     // 3. the transient lead tape: the first converged point's first
     // transient Jacobian (its resume record still names its .op tape)
@@ -2325,6 +2456,10 @@ extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t
         HIPCHK(hipMemcpyAsync(e->lane_ids.p, ids.data(), count * 4, hipMemcpyHostToDevice, s));
     }
     ta.op.tapes = e->tape_table.as<esl::TapeProg>();
+    // the step launches run the plain kernel; the lanes' own LU serves the
+    // lanes whose replay fails
+    const esl::PrivLU own = ta.op.priv;
+    ta.op.priv = esl::PrivLU{};
 
     // 4. the step launches: blocks of steps_per_launch, captured in a graph
     const u32 S = o.steps_per_launch, B = (o.steps + S - 1) / S;
@@ -2382,7 +2517,47 @@ extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t
         HIPCHK(hipEventElapsedTime(&kms, e->ev0, e->ev1));
     }
     e->timing.launches = B;
-    // 5. re-pivot rounds: factor each failing lane's dumped Jacobian, resume
+    // 5. lanes whose replay failed resume with their own LU (re-pivots on
+    // the device); what is left gets host re-pivot rounds
+    if (own.cap) {
+        HIPCHK(hipMemcpyAsync(stt.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
+        vector<u32> steps_at(count);
+        HIPCHK(hipMemcpyAsync(steps_at.data(), e->tstep.p, count * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        vector<u32> again;
+        u32 b0 = B;
+        for (size_t l = 0; l < count; l++)
+            if (stt[l] == esl::kRepivot) {
+                again.push_back((u32)l);
+                b0 = std::min(b0, steps_at[l] / S);
+            }
+        if (!again.empty()) {
+            for (u32 l : again) stt[l] = esl::kTranRunning;
+            HIPCHK(hipMemcpyAsync(e->status.p, stt.data(), count * 4, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(e->lane_ids.p, again.data(), again.size() * 4, hipMemcpyHostToDevice, s));
+            u32 cap = 0;
+            for (auto *t : e->tapes) cap = std::max(cap, t->prog.n_tv);
+            const u32 lds = tran_lds_bytes(c, cap);
+            if (lds > device_lds_max()) {
+                g_last_error = "transient lane state exceeds the LDS of one workgroup";
+                return ESL_FAILED;
+            }
+            HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
+            float ms = 0;
+            HIPCHK(hipEventRecord(e->ev0, s));
+            for (u32 b = b0; b < B; b++) {
+                esl::TranArgs t = block_args(b, cap, 0);
+                t.op.priv = own;
+                HIPCHK(esl_launch_tran_steps(&t, (u32)again.size(), lds, s));
+            }
+            HIPCHK(hipEventRecord(e->ev1, s));
+            HIPCHK(hipEventSynchronize(e->ev1));
+            HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+            kms += ms;
+            e->timing.launches += B - b0;
+            e->timing.device_repivot_lanes += (u32)again.size();
+        }
+    }
     {
         esl_status s2 = tran_repivot(*e, ta, count, S, B, stt, kms);
         if (s2 != ESL_OK) return s2;
@@ -2720,23 +2895,77 @@ esl_status batch_item_repivot(BatchItemRun &r, vector<esl::BatchJob> &resume_job
         if (st[l] == esl::kRepivot) again.push_back((u32)l);
     if (again.empty()) return ESL_OK;
     vector<esl::LaneResume> rs(count);
+    vector<double> A((size_t)count * nnz);
     HIPCHK(hipMemcpyAsync(rs.data(), e->resume.p, count * sizeof(esl::LaneResume), hipMemcpyDeviceToHost, s));
-    vector<double> A(nnz);
-    for (u32 l : again) {
-        HIPCHK(hipMemcpyAsync(A.data(), e->a_out.as<double>() + (size_t)l * nnz, nnz * 8, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        HostTape t;
+    HIPCHK(hipMemcpyAsync(A.data(), e->a_out.p, A.size() * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    // factor the failing points' matrices and build the programs of new
+    // pivot sequences on host threads; register them in point order
+    const size_t na = again.size();
+    vector<HostTape> ht(na);
+    vector<uint8_t> ok(na, 0);
+    parallel_for(na, [&](size_t k) {
+        ok[k] = gp_factor(n, c.col_ptr.data(), c.row_idx.data(), A.data() + (size_t)again[k] * nnz, c.q.data(), 1e-3,
+                          ht[k]) ? 1 : 0;
+    });
+    vector<vector<u32>> keys(na);
+    vector<int> first(na, -1);      // the first point of this pass with the same new pivot sequence
+    std::map<vector<u32>, size_t> fresh;
+    vector<size_t> build;
+    for (size_t k = 0; k < na; k++) {
+        if (!ok[k]) continue;
+        keys[k] = tape_key(ht[k]);
+        if (e->tape_index.count(keys[k])) continue;
+        auto f = fresh.find(keys[k]);
+        if (f != fresh.end()) {
+            first[k] = (int)f->second;
+        } else {
+            fresh[keys[k]] = k;
+            build.push_back(k);
+        }
+    }
+    vector<TapeHostProg *> progs(na, nullptr);
+    vector<const char *> errs(na, nullptr);
+    parallel_for(build.size(), [&](size_t b) { progs[build[b]] = build_tape(*e, ht[build[b]], errs[build[b]]); });
+    for (size_t b : build)
+        if (!progs[b]) {
+            for (size_t x : build) delete progs[x];
+            g_last_error = errs[b];
+            return ESL_FAILED;
+        }
+    vector<u32> ids(na, 0);
+    for (size_t k = 0; k < na; k++) {
+        if (!ok[k]) continue;
+        auto it = e->tape_index.find(keys[k]);
+        if (it != e->tape_index.end()) {
+            ids[k] = it->second;
+            continue;
+        }
+        if (first[k] >= 0) {
+            ids[k] = ids[(size_t)first[k]];
+            continue;
+        }
+        esl_status s2 = register_tape(*e, keys[k], progs[k], ids[k]);
+        progs[k] = nullptr;
+        if (s2 != ESL_OK) {
+            for (size_t x : build) delete progs[x];
+            return s2;
+        }
+    }
+    if (!build.empty()) {
+        esl_status s3 = upload_tape_table(*e);
+        if (s3 != ESL_OK) return s3;
+    }
+    for (size_t k = 0; k < na; k++) {
+        const u32 l = again[k];
         repivots++;
         const bool op_phase = step[l] == esl::kStepOp;
-        if (!gp_factor(n, c.col_ptr.data(), c.row_idx.data(), A.data(), c.q.data(), 1e-3, t)) {
+        if (!ok[k]) {
             st[l] = esl::kSingular;
             if (op_phase) r.its[l] = rs[l].iter + 1;
             continue;
         }
-        u32 id;
-        esl_status s2 = add_tape(*e, t, id);
-        if (s2 != ESL_OK) return s2;
-        rs[l].tape = id;
+        rs[l].tape = ids[k];
         if (op_phase) {
             rs[l].fresh = 1;
             resume_jobs.push_back(esl::BatchJob{r.op_res_idx, tran ? r.tran_idx : esl::kNoArgs, l, 0});

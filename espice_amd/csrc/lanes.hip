@@ -21,6 +21,7 @@
 
 #include "esl_device.h"
 #include "esl_internal.h"
+#include "esl_privlu.h"
 #include "eslmath.h"
 
 namespace esl {
@@ -555,6 +556,7 @@ __global__ __launch_bounds__(kWave) void op_dump(OpArgs a)
 // One lane's operating point, led by one wavefront (the body of op_newton
 // and of the batch kernel's .op jobs); nw > 1: a crew of nw wavefronts
 // shares the device evaluation. Returns the lane's status.
+template <bool kOwnLU>
 __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, double *base, uint32_t nw)
 {
     const uint32_t tid = threadIdx.x % kWave;
@@ -578,9 +580,14 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
 
 
     LaneResume rs = a.resume[lane];
-    const TapeProg T = a.tapes[rs.tape];
-    double *y = tv + T.n_tv;   // FP64 solve vector; in FP32 lanes the refined dx (row order)
     const bool fp32 = a.fp32 && !a.lane64[lane];
+    // FP64 lanes re-pivot on the device, on their own LU, when it is there
+    const bool can_priv = kOwnLU && a.priv.cap != 0 && !fp32;
+    bool priv = kOwnLU && rs.tape == kPrivTape;
+    const TapeProg T = a.tapes[priv ? 0 : rs.tape];   // unused while the lane is on its own LU
+    double *pap = a.priv.ap + (size_t)lane * dp.nnz;  // the lane's Jacobian for its own LU (pattern order)
+    // FP64 solve vector; in FP32 lanes and on the lane's own LU, dx in row order
+    double *y = priv ? tv + a.tv_cap : tv + T.n_tv;
     float *tv32 = reinterpret_cast<float *>(tv);
     double *a64 = a.a64 + (size_t)lane * dp.nnz;
     uint32_t iter0 = 0, k0 = 0, warm = a.mode == 3 ? 1u : 0u;
@@ -621,7 +628,8 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
             restart_nominal(iter);
         }
         lap(3);
-        assemble(dp, T, live, x, lim, nm, tv, rhs, st, tid, cr);
+        if (priv) assemble_pattern(dp, live, x, lim, nm, pap, rhs, st, tid);
+        else assemble(dp, T, live, x, lim, nm, tv, rhs, st, tid, cr);
         if (warm == 1) {
             uint64_t fb = 0;
             for (uint32_t i = tid; i < n; i += kWave) {
@@ -641,7 +649,7 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
             double gm = 1.0;
             uint64_t xm = 0, fm = 0;
             for (uint32_t i = tid; i < n; i += kWave) {
-                gm = fmax(gm, fabs(tv[T.diag_tv[i]]));
+                gm = fmax(gm, fabs(priv ? pap[dp.diag_slot[i]] : tv[T.diag_tv[i]]));
                 const uint64_t xb = dbits(fabs(x[i])), fb = dbits(fabs(rhs[i]));
                 xm = xb > xm ? xb : xm;
                 fm = fb > fm ? fb : fm;
@@ -652,7 +660,7 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
             const double scale = fmin(fmax(bitsd(xm), bitsd(fm) * gm), 1e18);
             limit = !(1e3 * scale > 1e9) ? 1e9 : 1e3 * scale;
         }
-        for (uint32_t i = tid; i < n; i += kWave) sdiag[i] = tv[T.diag_tv[i]];
+        for (uint32_t i = tid; i < n; i += kWave) sdiag[i] = priv ? pap[dp.diag_slot[i]] : tv[T.diag_tv[i]];
         wsync();
         lap(0);
         // FP32 lanes: keep the FP64 Jacobian for the residual, round the
@@ -675,8 +683,40 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
                 if (refine_solve(a, T, n, tv32, rhs, a64, y, xold, tid)) have_dx = true;
                 else exit_status = kToFp64;
             }
+        } else if (priv) {
+            // the lane's own LU: replay it, factor again when a peel test fails
+            int rc = kPlOk;
+            if (tid == 0) rc = priv_update_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq);
+            rc = __shfl(rc, 0, kWave);
+            wsync();
+            if (rc == kPlSingular) exit_status = kSingular;
+            else if (rc == kPlFull) exit_status = kRepivot;
         } else if (refactor<double>(T, n, tv, tid)) {
             exit_status = kRepivot;
+            if (can_priv) {
+                // re-pivot here: the Jacobian again in pattern order (the same
+                // sums as the failed assembly), factored by the lane
+                wsync();
+                assemble_pattern(dp, live, x, lim, nm, pap, rhs, st, tid);
+                int rc = kPlOk;
+                if (tid == 0) rc = priv_factor_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq);
+                rc = __shfl(rc, 0, kWave);
+                for (uint32_t i = tid; i < n; i += kWave) sdiag[i] = pap[dp.diag_slot[i]];
+                wsync();
+                if (rc == kPlOk) {
+                    exit_status = kConverged;
+                    priv = true;
+                    rs.tape = kPrivTape;
+                    y = tv + a.tv_cap;
+                } else if (rc == kPlSingular) {
+                    exit_status = kSingular;
+                }
+            }
+        }
+        if (exit_status == kSingular) {
+            status = kSingular;
+            iters = iter + 1;
+            break;
         }
         if (exit_status != kConverged) {
             lap(1);
@@ -704,7 +744,12 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
             iters = iter;
             break;
         }
-        if (!have_dx) {
+        if (!have_dx && priv) {
+            lap(1);
+            if (tid == 0) priv_solve_at(a.priv, lane, dp.colq, rhs, y);
+            wsync();
+            have_dx = true;
+        } else if (!have_dx) {
             lap(1);
             solve_neg(T, n, tv, rhs, y, tid);
         }
@@ -781,10 +826,19 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
     return status;
 }
 
+// The Newton kernel; op_newton_own adds the lanes' own LU (a lane whose
+// replay fails factors its Jacobian itself). The plain kernel keeps the
+// registers of the common path: lanes that fail there resume in the other.
 __global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
 {
     extern __shared__ double sm[];
-    op_lane(a, a.lane_ids[blockIdx.x], sm, 1);
+    op_lane<false>(a, a.lane_ids[blockIdx.x], sm, 1);
+}
+
+__global__ __launch_bounds__(kWave) void op_newton_own(OpArgs a)
+{
+    extern __shared__ double sm[];
+    op_lane<true>(a, a.lane_ids[blockIdx.x], sm, 1);
 }
 
 // ---- transient lanes ----
@@ -794,14 +848,15 @@ __global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
 // (src/device/eval.zig:782-922 with the charge planes of LaneSink), the
 // ground pin, then A = G + ag0 C and F += ag0 (q - q1) - ag1 i_prev
 // (lane_kernels.zig:194-215). q1 and i_prev are the lane's rows in global
-// memory. A crew of nw wavefronts evaluates nw chunks per round.
-__device__ void assemble_tran(const DeckProg &dp, const TapeProg &T, const double *live, const double *x, double *lim,
-                              uint32_t nm, double *tv, double *rhs, double *q, double *cq, double *st, double t,
-                              double ag0, double ag1, const double *q1, const double *ip, uint32_t tid,
-                              const Crew &cr)
+// memory. Jacobian entries go to tv[amap[slot]] (amap null: tv[slot], the
+// pattern order). A crew of nw wavefronts evaluates nw chunks per round.
+__device__ void assemble_tran(const DeckProg &dp, const uint32_t *amap, uint32_t n_slots, const double *live,
+                              const double *x, double *lim, uint32_t nm, double *tv, double *rhs, double *q,
+                              double *cq, double *st, double t, double ag0, double ag1, const double *q1,
+                              const double *ip, uint32_t tid, const Crew &cr)
 {
     const uint32_t n = dp.n;
-    for (uint32_t i = tid; i < T.n_tv; i += kWave) tv[i] = 0.0;
+    for (uint32_t i = tid; i < n_slots; i += kWave) tv[i] = 0.0;
     for (uint32_t i = tid; i < n; i += kWave) {
         rhs[i] = 0.0;
         q[i] = 0.0;
@@ -821,7 +876,7 @@ __device__ void assemble_tran(const DeckProg &dp, const TapeProg &T, const doubl
                 const uint32_t tgt = dp.task_target[k];
                 const uint32_t idx = tgt & kTargetMask;
                 double *d = (tgt & kRhsBit) ? &rhs[idx] : (tgt & kQBit) ? &q[idx] : (tgt & kCBit) ? &cq[idx]
-                                                                                                   : &tv[T.amap[tgt]];
+                                                                                                   : &tv[amap ? amap[tgt] : tgt];
                 double acc = *d;
                 const uint32_t cc1 = dp.task_cell[k + 1];
                 for (uint32_t cc = dp.task_cell[k]; cc < cc1; cc++) acc = acc + stc[dp.cells[cc]];
@@ -831,12 +886,13 @@ __device__ void assemble_tran(const DeckProg &dp, const TapeProg &T, const doubl
         }
     }
     if (tid == 0) {
-        tv[T.amap[dp.diag0_slot]] = tv[T.amap[dp.diag0_slot]] + 1.0;
+        const uint32_t d0 = amap ? amap[dp.diag0_slot] : dp.diag0_slot;
+        tv[d0] = tv[d0] + 1.0;
         rhs[0] = rhs[0] + x[0];
     }
     wsync();
     for (uint32_t k = tid; k < dp.n_cslot; k += kWave) {
-        const uint32_t sl = T.amap[dp.cslot_pat[k]];
+        const uint32_t sl = amap ? amap[dp.cslot_pat[k]] : dp.cslot_pat[k];
         tv[sl] = tv[sl] + ag0 * cq[k];
     }
     for (uint32_t i = tid; i < n; i += kWave) {
@@ -854,6 +910,7 @@ __device__ void assemble_tran(const DeckProg &dp, const TapeProg &T, const doubl
 // and limiting state reset to the accepted point. A lane whose replay fails
 // leaves its Jacobian in a_out for a host re-pivot and resumes at the same
 // iterate.
+template <bool kOwnLU>
 __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, double *base, uint32_t nw)
 {
     const OpArgs &a = ta.op;
@@ -880,8 +937,17 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
     const double *live = a.live + (size_t)lane * dp.n_live;
     double *lim = a.lim + (size_t)lane * lim_width(dp);
     LaneResume rs = a.resume[lane];
-    const TapeProg T = a.tapes[rs.tape];
-    double *y = tv + T.n_tv;
+    const bool can_priv = kOwnLU && a.priv.cap != 0;
+    bool priv = kOwnLU && rs.tape == kPrivTape;
+    const TapeProg T = a.tapes[priv ? 0 : rs.tape];   // unused while the lane is on its own LU
+    double *pap = a.priv.ap + (size_t)lane * dp.nnz;  // the lane's Jacobian for its own LU (pattern order)
+    double *y = priv ? tv + a.tv_cap : tv + T.n_tv;
+    // the companion assembly into the tape's slots, or in pattern order for
+    // the lane's own LU
+    auto assemble_step = [&](bool own, double tt, double g0, double g1) {
+        if (own) assemble_tran(dp, nullptr, dp.nnz, live, x, lim, nm, pap, rhs, q, cq, st, tt, g0, g1, q1, ip, tid, cr);
+        else assemble_tran(dp, T.amap, T.n_tv, live, x, lim, nm, tv, rhs, q, cq, st, tt, g0, g1, q1, ip, tid, cr);
+    };
     for (uint32_t i = tid; i < n; i += kWave) x[i] = a.x[(size_t)lane * n + i];
     wsync();
     if (ta.init) {
@@ -894,13 +960,14 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
             xp[i] = x[i];
         }
         wsync();
-        assemble_tran(dp, T, live, x, lim, nm, tv, rhs, q, cq, st, 0.0, 0.0, 0.0, q1, ip, tid, cr);
+        assemble_step(priv, 0.0, 0.0, 0.0);
         for (uint32_t i = tid; i < n; i += kWave) q1[i] = q[i];
         wsync();
         if (ta.dump) {
-            // the transient lead point's first Jacobian (step 0, iteration 0)
-            assemble_tran(dp, T, live, x, lim, nm, tv, rhs, q, cq, st, ta.dt, 1.0 / ta.dt, 0.0, q1, ip, tid, cr);
-            for (uint32_t p = tid; p < dp.nnz; p += kWave) a.a_out[p] = tv[T.amap[p]];   // one lane per dump launch
+            // the transient lead point's first Jacobian (step 0, iteration 0),
+            // pattern order, one lane per dump launch
+            assemble_tran(dp, nullptr, dp.nnz, live, x, lim, nm, a.a_out, rhs, q, cq, st, ta.dt, 1.0 / ta.dt, 0.0, q1,
+                          ip, tid, cr);
             return;
         }
     }
@@ -916,12 +983,12 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
         bool converged = false;
         uint32_t iter = iter0;
         for (; iter < a.max_iter; iter++) {
-            assemble_tran(dp, T, live, x, lim, nm, tv, rhs, q, cq, st, t, ag0, ag1, q1, ip, tid, cr);
+            assemble_step(priv, t, ag0, ag1);
             if (iter == 0) {
                 double gm = 1.0;
                 uint64_t xm = 0, fm = 0;
                 for (uint32_t i = tid; i < n; i += kWave) {
-                    gm = fmax(gm, fabs(tv[T.diag_tv[i]]));
+                    gm = fmax(gm, fabs(priv ? pap[dp.diag_slot[i]] : tv[T.diag_tv[i]]));
                     const uint64_t xb = dbits(fabs(x[i])), fb = dbits(fabs(rhs[i]));
                     xm = xb > xm ? xb : xm;
                     fm = fb > fm ? fb : fm;
@@ -933,10 +1000,42 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
                 limit = !(1e3 * scale > 1e9) ? 1e9 : 1e3 * scale;
             }
             wsync();
-            if (refactor<double>(T, n, tv, tid)) {
-                // the Jacobian again, for the host's re-pivot
-                assemble_tran(dp, T, live, x, lim, nm, tv, rhs, q, cq, st, t, ag0, ag1, q1, ip, tid, cr);
-                for (uint32_t p = tid; p < dp.nnz; p += kWave) a.a_out[(size_t)lane * dp.nnz + p] = tv[T.amap[p]];
+            // the replay, then on a failed peel test the lane's own factor
+            // (or, without room for it, the host's re-pivot)
+            int rc = kPlOk;
+            bool to_host = false;
+            if (priv) {
+                if (tid == 0) rc = priv_update_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq);
+                rc = __shfl(rc, 0, kWave);
+                wsync();
+                to_host = rc == kPlFull;
+            } else if (refactor<double>(T, n, tv, tid)) {
+                wsync();
+                assemble_step(can_priv, t, ag0, ag1);   // the Jacobian again, pattern order when own
+                if (can_priv) {
+                    if (tid == 0) rc = priv_factor_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq);
+                    rc = __shfl(rc, 0, kWave);
+                    wsync();
+                    if (rc == kPlOk) {
+                        priv = true;
+                        rs.tape = kPrivTape;
+                        y = tv + a.tv_cap;
+                    }
+                    to_host = rc == kPlFull;
+                } else {
+                    to_host = true;
+                }
+            }
+            if (rc == kPlSingular) {
+                status = kSingular;
+                break;
+            }
+            if (to_host) {
+                // the Jacobian for the host's re-pivot; resume at this iterate
+                if (priv || can_priv)
+                    for (uint32_t p = tid; p < dp.nnz; p += kWave) a.a_out[(size_t)lane * dp.nnz + p] = pap[p];
+                else
+                    for (uint32_t p = tid; p < dp.nnz; p += kWave) a.a_out[(size_t)lane * dp.nnz + p] = tv[T.amap[p]];
                 if (tid == 0) {
                     LaneResume o = rs;
                     o.iter = iter;
@@ -947,12 +1046,16 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
                 status = kRepivot;
                 break;
             }
-            solve_neg(T, n, tv, rhs, y, tid);
+            if (priv) {
+                if (tid == 0) priv_solve_at(a.priv, lane, dp.colq, rhs, y);
+            } else {
+                solve_neg(T, n, tv, rhs, y, tid);
+            }
             wsync();
             uint64_t worst = 0;
             for (uint32_t i = tid; i < n; i += kWave) {
                 const double xo = x[i];
-                const double d = y[T.qinv[i]];
+                const double d = priv ? y[i] : y[T.qinv[i]];
                 const double xn = xo + d;
                 xold[i] = xo;
                 x[i] = xn;
@@ -999,7 +1102,7 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
                 break;
             }
         }
-        if (status == kRepivot) {
+        if (status == kRepivot || status == kSingular) {
             total_it += iter - iter0;
             break;
         }
@@ -1052,7 +1155,13 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
 __global__ __launch_bounds__(kWave) void tran_steps(TranArgs ta)
 {
     extern __shared__ double sm[];
-    tran_lane(ta, ta.op.lane_ids[blockIdx.x], sm, 1);
+    tran_lane<false>(ta, ta.op.lane_ids[blockIdx.x], sm, 1);
+}
+
+__global__ __launch_bounds__(kWave) void tran_steps_own(TranArgs ta)
+{
+    extern __shared__ double sm[];
+    tran_lane<true>(ta, ta.op.lane_ids[blockIdx.x], sm, 1);
 }
 
 // ---- the persistent batch ----
@@ -1065,11 +1174,11 @@ __device__ __forceinline__ void run_job(const BatchArgs &b, const BatchJob &jb, 
     const uint32_t tid = threadIdx.x % kWave;
     if (jb.op == kNoArgs) {
         // a transient resumed at its saved step and iterate
-        tran_lane(b.trans[jb.tran], jb.lane, base, nw);
+        tran_lane<true>(b.trans[jb.tran], jb.lane, base, nw);
         return;
     }
     const OpArgs &oa = b.ops[jb.op];
-    const uint32_t st = op_lane(oa, jb.lane, base, nw);
+    const uint32_t st = op_lane<true>(oa, jb.lane, base, nw);
     if (jb.tran == kNoArgs || st != kConverged) return;
     const TranArgs &ta = b.trans[jb.tran];
     if (tid == 0) {
@@ -1082,7 +1191,7 @@ __device__ __forceinline__ void run_job(const BatchArgs &b, const BatchJob &jb, 
         ta.step[jb.lane] = 0;
     }
     wsync();
-    tran_lane(ta, jb.lane, base, nw);
+    tran_lane<true>(ta, jb.lane, base, nw);
 }
 
 __device__ __forceinline__ void follow_job(const BatchArgs &b, const BatchJob &jb, double *base, uint32_t wv)
@@ -1175,6 +1284,8 @@ extern "C" hipError_t esl_launch_op_newton(const esl::OpArgs *a, uint32_t grid, 
 {
     if (a->dump)
         hipLaunchKernelGGL(esl::op_dump, dim3(grid), dim3(esl::kWave), lds_bytes, s, *a);
+    else if (a->priv.cap)
+        hipLaunchKernelGGL(esl::op_newton_own, dim3(grid), dim3(esl::kWave), lds_bytes, s, *a);
     else
         hipLaunchKernelGGL(esl::op_newton, dim3(grid), dim3(esl::kWave), lds_bytes, s, *a);
     return hipGetLastError();
@@ -1189,7 +1300,10 @@ extern "C" hipError_t esl_launch_tape_replay(const esl::ReplayArgs *r, uint32_t 
 
 extern "C" hipError_t esl_launch_tran_steps(const esl::TranArgs *ta, uint32_t grid, uint32_t lds_bytes, hipStream_t s)
 {
-    hipLaunchKernelGGL(esl::tran_steps, dim3(grid), dim3(esl::kWave), lds_bytes, s, *ta);
+    if (ta->op.priv.cap)
+        hipLaunchKernelGGL(esl::tran_steps_own, dim3(grid), dim3(esl::kWave), lds_bytes, s, *ta);
+    else
+        hipLaunchKernelGGL(esl::tran_steps, dim3(grid), dim3(esl::kWave), lds_bytes, s, *ta);
     return hipGetLastError();
 }
 
@@ -1213,6 +1327,12 @@ extern "C" hipError_t esl_kernel_attrs(uint32_t lds_bytes)
     e = hipFuncSetAttribute((const void *)esl::op_dump, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void *)esl::tran_steps, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void *)esl::tran_steps_own, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds_bytes);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void *)esl::op_newton_own, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds_bytes);
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void *)esl::batch_run, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) return e;

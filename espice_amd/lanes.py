@@ -25,6 +25,7 @@ REASON_OK, REASON_NO_CONVERGENCE, REASON_SINGULAR = 0, 1, 3
 FP64, FP32_REFINED = 0, 1
 START_COLD, START_GIVEN, START_LEARNED = 0, 1, 2
 TILING_AUTO, TILING_WAVE, TILING_THREAD, TILING_BLOCK = 0, 1, 2, 3
+OP_HOST_REPIVOT = 1
 BATCH_OP, BATCH_TRAN = 0, 1
 
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "BUFFER_TOO_SMALL", 3: "OUT_OF_MEMORY",
@@ -66,7 +67,7 @@ class OpOptions(C.Structure):
     _fields_ = [("struct_size", C.c_size_t), ("tol", Tolerances), ("precision", C.c_uint32),
                 ("start", C.c_uint32), ("x0", C.c_void_p), ("refine_tol", C.c_double),
                 ("refine_max", C.c_uint32), ("tiling", C.c_uint32), ("probes", C.c_void_p),
-                ("n_probes", C.c_uint32), ("reserved", C.c_uint32)]
+                ("n_probes", C.c_uint32), ("flags", C.c_uint32)]
 
 
 class EngineInfo(C.Structure):
@@ -80,7 +81,7 @@ class EngineInfo(C.Structure):
 class Timing(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("launches", C.c_uint32),
                 ("repivots", C.c_uint32), ("lane_iterations", C.c_uint64), ("tiling", C.c_uint32),
-                ("fp64_lanes", C.c_uint32), ("warm_fallbacks", C.c_uint32), ("reserved", C.c_uint32),
+                ("fp64_lanes", C.c_uint32), ("warm_fallbacks", C.c_uint32), ("device_repivot_lanes", C.c_uint32),
                 ("warm_ms", C.c_double), ("lead_ms", C.c_double), ("download_ms", C.c_double)]
 
 
@@ -253,7 +254,7 @@ class Engine:
         return cp, ri[: info.nnz], q
 
     def solve_op(self, values, x0=None, precision=FP64, tol: Tolerances | None = None, tiling=TILING_AUTO,
-                 refine_tol=None, refine_max=None, learned=False, probes=None):
+                 refine_tol=None, refine_max=None, learned=False, probes=None, host_repivot=False):
         """Operating points of rows `values` [count][n_live]. Returns dict
         with x [count][n], iterations, reasons. x0 [count][n]: given starts;
         learned=True: starts from the attached network (set_warm_start);
@@ -263,6 +264,7 @@ class Engine:
         start = START_LEARNED if learned else (START_COLD if x0 is None else START_GIVEN)
         o = default_options(precision, start)
         o.tiling = tiling
+        o.flags = OP_HOST_REPIVOT if host_repivot else 0
         if refine_tol is not None:
             o.refine_tol = refine_tol
         if refine_max is not None:

@@ -176,8 +176,12 @@ typedef struct {
      * per-point measures, include/espice.h:338), instead of [count][n]. */
     const uint32_t *probes;
     uint32_t n_probes;
-    uint32_t reserved;
+    /* ESL_OP_HOST_REPIVOT: a lane whose replay fails leaves for the host,
+     * which factors its Jacobian into a new tape (the default for FP64 is
+     * the lane's own factor on the device; both give the same bits). */
+    uint32_t flags;
 } esl_op_options;
+enum { ESL_OP_HOST_REPIVOT = 1u };
 
 /* Sizes of a compiled engine. */
 typedef struct {
@@ -341,7 +345,7 @@ typedef struct {
     uint32_t tiling;       /* ESL_TILING_* the last solve ran (first launch) */
     uint32_t fp64_lanes;   /* FP32_REFINED: lanes that fell back to FP64 factors */
     uint32_t warm_fallbacks;  /* START_LEARNED: lanes restarted from the nominal point */
-    uint32_t reserved;
+    uint32_t device_repivot_lanes; /* lanes that resumed with their own LU on the device */
     double warm_ms;        /* START_LEARNED: network inference, HIP events */
     double lead_ms;        /* host: lead point's Jacobian, factor and tape program */
     double download_ms;    /* host: results to the caller */

@@ -106,6 +106,34 @@ def test_family_bitwise_against_oracle(lanes, oracle, family, lanes_n, tiling):
     eng.close()
 
 
+@pytest.mark.parametrize("stages", [16, 64])
+def test_device_and_host_repivots_give_the_oracles_bits(lanes, oracle, stages):
+    """Long inverter chains change pivot order many times on their way from
+    a cold start: the lanes' own factors on the device and the host's
+    re-pivot tapes must both reproduce the oracle (its replay, then a full
+    factor whenever a peel test fails) bit for bit."""
+    fam = circuits.inverter_chain(stages)
+    vals = fam.sample(77, 96)
+    eng = lanes.Engine(fam.deck.arrays())
+    _, _, q = eng.pattern()
+    t = lanes.default_options().tol
+    t.max_iter = 400
+    dev = eng.solve_op(vals, tiling=lanes.TILING_WAVE, tol=t)
+    dev_rep = eng.timing().repivots
+    host = eng.solve_op(vals, tiling=lanes.TILING_WAVE, tol=t, host_repivot=True)
+    host_rep = eng.timing().repivots
+    ref = oracle.Circuit(fam.deck.arrays()).op(vals, q, threads=8,
+                                                tol=oracle.Tol(t.reltol, t.abstol, t.vntol, t.residual_tol, 400, 0))
+    say(f"  {stages} stages: host re-pivots {host_rep}, device run's host re-pivots {dev_rep}, "
+        f"oracle re-pivots {int(ref['repivots'].sum())}, converged {(ref['reasons'] == 0).sum()}/96")
+    assert ref["repivots"].sum() > 0 and host_rep > 0 and dev_rep == 0
+    for got in (dev, host):
+        assert np.array_equal(got["reasons"], ref["reasons"])
+        assert np.array_equal(got["iterations"], ref["iterations"])
+        assert np.array_equal(bits(got["x"]), bits(ref["x"]))
+    eng.close()
+
+
 @pytest.mark.parametrize("tiling", [1, 2])
 def test_warm_start_bitwise_against_oracle(lanes, oracle, tiling):
     fam = circuits.ota_array()
