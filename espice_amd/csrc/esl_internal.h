@@ -181,7 +181,8 @@ struct TapeProg {
 // Per-lane results of the Newton kernel.
 enum LaneStatus : uint32_t {
     kConverged = 0, kNoConvergence = 1, kSingular = 3, kRepivot = 100,
-    kToFp64 = 101   // FP32 factors failed on a fresh tape, or refinement stalled
+    kToFp64 = 101,  // FP32 factors failed on a fresh tape, or refinement stalled
+    kNoRoom = 104   // a lane on its own LU met a launch whose LDS cannot hold its pattern
 };
 
 // Where a lane that left the kernel early resumes. fresh != 0: the host just
@@ -207,7 +208,7 @@ struct PrivLU {
     uint32_t *prow, *void_slots;                    // [N][nnz]
     uint32_t *li, *ui;                              // [N][cap]
     double *lx, *ux;                                // [N][cap]
-    double *udiag, *w, *rscale;                     // [N][n]; w is all zero between calls
+    double *udiag, *rscale;                         // [N][n]
     double *ap;                                     // [N][nnz] the lane's Jacobian, pattern order
 };
 

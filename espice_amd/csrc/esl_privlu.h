@@ -3,8 +3,8 @@
 // When a lane's replay of its shared tape fails the peel tests, the lane
 // factors its Jacobian itself with threshold partial pivoting (left-looking
 // Gilbert-Peierls, sparse_lu.zig:260-622; the host's gp_factor and the
-// oracle's orc_lu_factor state the same steps), keeps the factors and
-// pivot sequence in its own global-memory slice, and from then on replays
+// oracle's orc_lu_factor state the same steps), keeps the factors and pivot
+// sequence in its own slice of memory (the pattern in LDS), and from then on replays
 // them column by column (lane_kernels.zig:49-118 with the wide-multiplier
 // test), factoring again whenever the replay fails. Every operation runs in
 // the oracle's order on one thread, so the lane's numbers are bitwise those
@@ -16,14 +16,20 @@
 
 namespace esl {
 
+// One lane's view: the pattern of its factors (lp, up, li, ui) and the
+// dense column w live in the lane's LDS (the factor-slot area, unused while
+// the lane is on its own LU), with copies in global memory that carry them
+// from one launch to the next; values, pivots and work arrays stay global.
 struct PrivLane {
-    uint32_t n, nnz, cap;
-    uint32_t *hdr, *pinv, *flags, *flag, *topo, *stack, *pstack, *want, *want_of, *lp, *up, *prow, *void_slots, *li,
-        *ui;
+    uint32_t n, nnz, cap, lcap;   // cap: L and U entries in global memory; lcap: in LDS
+    uint32_t *hdr, *pinv, *flags, *flag, *topo, *stack, *pstack, *want, *want_of, *prow, *void_slots;
+    uint32_t *lp, *up, *li, *ui;        // LDS
+    uint32_t *glp, *gup, *gli, *gui;    // global copies
     double *lx, *ux, *udiag, *w, *rscale, *ap;
 };
 
-__device__ inline PrivLane priv_lane(const PrivLU &P, uint32_t lane)
+// lds: the lane's factor-slot area, lds_doubles long.
+__device__ inline PrivLane priv_lane(const PrivLU &P, uint32_t lane, double *lds, uint32_t lds_doubles)
 {
     const size_t n = P.n, n1 = P.n + 1, z = P.nnz, c = P.cap, l = lane;
     PrivLane L;
@@ -39,18 +45,25 @@ __device__ inline PrivLane priv_lane(const PrivLU &P, uint32_t lane)
     L.pstack = P.pstack + l * n;
     L.want = P.want + l * n;
     L.want_of = P.want_of + l * n;
-    L.lp = P.lp + l * n1;
-    L.up = P.up + l * n1;
+    L.glp = P.lp + l * n1;
+    L.gup = P.up + l * n1;
     L.prow = P.prow + l * z;
     L.void_slots = P.void_slots + l * z;
-    L.li = P.li + l * c;
-    L.ui = P.ui + l * c;
+    L.gli = P.li + l * c;
+    L.gui = P.ui + l * c;
     L.lx = P.lx + l * c;
     L.ux = P.ux + l * c;
     L.udiag = P.udiag + l * n;
-    L.w = P.w + l * n;
     L.rscale = P.rscale + l * n;
     L.ap = P.ap + l * z;
+    // LDS: [w (n doubles) | lp (n + 1) | up (n + 1) | li (lcap) | ui (lcap)] in u32 after w
+    L.w = lds;
+    const int64_t words = 2 * ((int64_t)lds_doubles - (int64_t)n) - 2 * (int64_t)n1;
+    L.lcap = words > 0 ? (uint32_t)(words / 2) : 0u;
+    L.lp = reinterpret_cast<uint32_t *>(lds + n);
+    L.up = L.lp + n1;
+    L.li = L.up + n1;
+    L.ui = L.li + L.lcap;
     return L;
 }
 
@@ -149,7 +162,8 @@ __device__ int priv_factor(const PrivLane &L, const uint32_t *col_ptr, const uin
                 sp--;
             }
         }
-        if (nnz_u + nt > L.cap || nnz_l + nt > L.cap) {
+        const uint32_t room = L.cap < L.lcap ? L.cap : L.lcap;
+        if (nnz_u + nt > room || nnz_l + nt > room) {
             for (uint32_t i = 0; i < n; i++) w[i] = 0.0;
             return kPlFull;
         }
@@ -236,7 +250,29 @@ __device__ int priv_factor(const PrivLane &L, const uint32_t *col_ptr, const uin
     L.hdr[0] = nnz_l;
     L.hdr[1] = nnz_u;
     L.hdr[2] = n_void;
+    // the pattern's global copy, for the lane's later launches
+    for (uint32_t k = 0; k <= n; k++) {
+        L.glp[k] = L.lp[k];
+        L.gup[k] = L.up[k];
+    }
+    for (uint32_t p = 0; p < nnz_l; p++) L.gli[p] = L.li[p];
+    for (uint32_t p = 0; p < nnz_u; p++) L.gui[p] = L.ui[p];
     return kPlOk;
+}
+
+// A launch's start on the lane's own LU: the pattern back into LDS (all
+// threads of the wavefront). False when it does not fit this launch's LDS.
+__device__ inline bool priv_stage(const PrivLane &L, uint32_t tid, uint32_t wave)
+{
+    const uint32_t nl = L.hdr[0], nu = L.hdr[1];
+    if (nl > L.lcap || nu > L.lcap) return false;
+    for (uint32_t k = tid; k <= L.n; k += wave) {
+        L.lp[k] = L.glp[k];
+        L.up[k] = L.gup[k];
+    }
+    for (uint32_t p = tid; p < nl; p += wave) L.li[p] = L.gli[p];
+    for (uint32_t p = tid; p < nu; p += wave) L.ui[p] = L.gui[p];
+    return true;
 }
 
 // The lane replay of its own factors on L.ap (one thread); true when a
@@ -346,31 +382,46 @@ __device__ void priv_solve_neg(const PrivLane &L, const uint32_t *q, const doubl
     for (uint32_t j = 0; j < n; j++) dx[q[j]] = y[j];
 }
 
-// Out-of-line entry points: the lane kernels call them rarely (a failed
-// replay) and keep their own registers free of these loops.
+// Entry points for the lane kernels (one thread). lds: the lane's factor-
+// slot area (lds_doubles long); its w is all zero on entry and on return.
 
 // The lane's full factor of its Jacobian (PrivLU::ap of the lane).
-__device__ __attribute__((noinline)) int priv_factor_at(PrivLU P, uint32_t lane, const uint32_t *col_ptr,
-                                                       const uint32_t *row_idx, const uint32_t *q)
+__device__ __forceinline__ int priv_factor_at(const PrivLU &P, uint32_t lane, const uint32_t *col_ptr,
+                                              const uint32_t *row_idx, const uint32_t *q, double *lds,
+                                              uint32_t lds_doubles)
 {
-    return priv_factor(priv_lane(P, lane), col_ptr, row_idx, q, 1e-3);
+    return priv_factor(priv_lane(P, lane, lds, lds_doubles), col_ptr, row_idx, q, 1e-3);
 }
 
 // The replay of the lane's own factors, and on a failed peel test its full
 // factor: kPlOk, kPlSingular or kPlFull.
-__device__ __attribute__((noinline)) int priv_update_at(PrivLU P, uint32_t lane, const uint32_t *col_ptr,
-                                                       const uint32_t *row_idx, const uint32_t *q)
+__device__ __forceinline__ int priv_update_at(const PrivLU &P, uint32_t lane, const uint32_t *col_ptr,
+                                              const uint32_t *row_idx, const uint32_t *q, double *lds,
+                                              uint32_t lds_doubles)
 {
-    const PrivLane L = priv_lane(P, lane);
+    const PrivLane L = priv_lane(P, lane, lds, lds_doubles);
     if (!priv_refactor(L, col_ptr, q, 1e-12, 1.0 / 1e-3)) return kPlOk;
     return priv_factor(L, col_ptr, row_idx, q, 1e-3);
 }
 
-__device__ __attribute__((noinline)) void priv_solve_at(PrivLU P, uint32_t lane, const uint32_t *q, const double *rhs,
-                                                       double *dx)
+// dx = -A^-1 rhs in unknown order; w serves as the solve's scratch and is
+// cleared again.
+__device__ __forceinline__ void priv_solve_at(const PrivLU &P, uint32_t lane, const uint32_t *q, const double *rhs,
+                                              double *dx, double *lds, uint32_t lds_doubles)
 {
-    const PrivLane L = priv_lane(P, lane);
-    priv_solve_neg(L, q, rhs, dx, L.rscale);
+    const PrivLane L = priv_lane(P, lane, lds, lds_doubles);
+    priv_solve_neg(L, q, rhs, dx, L.w);
+    for (uint32_t i = 0; i < L.n; i++) L.w[i] = 0.0;
+}
+
+// A launch's start for a lane already on its own LU: w cleared and the
+// pattern staged (all threads); false when it does not fit.
+__device__ __forceinline__ bool priv_enter(const PrivLU &P, uint32_t lane, double *lds, uint32_t lds_doubles,
+                                           uint32_t tid, uint32_t wave)
+{
+    const PrivLane L = priv_lane(P, lane, lds, lds_doubles);
+    for (uint32_t i = tid; i < L.n; i += wave) L.w[i] = 0.0;
+    return priv_stage(L, tid, wave);
 }
 
 }  // namespace esl

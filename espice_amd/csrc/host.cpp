@@ -1295,14 +1295,14 @@ hipError_t upload_deck(Engine &e)
 
 // The lanes' own LU for `count` lanes (device re-pivot): index and value
 // arrays carved from two allocations, L and U each with room for four
-// times the lead tape's entries plus the pattern. w starts all zero.
+// times the lead tape's entries plus the pattern.
 hipError_t priv_setup(Engine &e, size_t count, u32 nnz_l, u32 nnz_u, hipStream_t s, esl::PrivLU &P)
 {
     const size_t n = e.c.n, nnz = e.c.nnz, N = count;
     const size_t cap = 4 * ((size_t)nnz_l + nnz_u) + nnz + 64;
     if (cap > 0xffffffffu) return hipErrorInvalidValue;
     const size_t u32s = N * (4 + 8 * n + 2 * (n + 1) + 2 * nnz + 2 * cap);
-    const size_t f64s = N * (2 * cap + 3 * n + nnz);
+    const size_t f64s = N * (2 * cap + 2 * n + nnz);
     hipError_t er = e.pl_u32.ensure(u32s * 4);
     if (er != hipSuccess) return er;
     er = e.pl_f64.ensure(f64s * 8);
@@ -1333,10 +1333,10 @@ hipError_t priv_setup(Engine &e, size_t count, u32 nnz_l, u32 nnz_u, hipStream_t
     P.lx = take_f(cap);
     P.ux = take_f(cap);
     P.udiag = take_f(n);
-    P.w = take_f(n);
     P.rscale = take_f(n);
     P.ap = take_f(nnz);
-    return hipMemsetAsync(P.w, 0, N * n * 8, s);
+    (void)s;
+    return hipSuccess;
 }
 
 u32 lds_bytes_for(u32 tv_cap, u32 n)
@@ -2268,7 +2268,7 @@ esl_status tran_lead_tape(Engine &E, const esl::TranArgs &ta, u32 l0, u32 op_tap
     HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
     HIPCHK(esl_launch_tran_steps(&d, 1, lds, s));
     vector<double> A(nnz);
-    HIPCHK(hipMemcpyAsync(A.data(), e->a_out.p, nnz * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(A.data(), e->a_out.as<double>() + (size_t)l0 * nnz, nnz * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     HostTape t;
     if (!gp_factor(n, c.col_ptr.data(), c.row_idx.data(), A.data(), c.q.data(), 1e-3, t)) {
@@ -2609,6 +2609,7 @@ struct BatchItemRun {
     u32 op_res_idx = 0, tran_res_idx = esl::kNoArgs;   // resumed after a re-pivot
     u32 res_cap = 0;              // factor slots of the largest tape (resumed lanes)
     vector<u32> its;              // iterations of points that ended singular in a re-pivot
+    bool needs_lead = false;      // a transient item before its lead tape
 };
 
 // Model of one job's time before any measurement: the rows a Newton
@@ -2783,47 +2784,16 @@ esl_status batch_item_setup(BatchItemRun &r)
     ta.steps = o.steps;
     ta.n_probes = o.n_probes;
 
-    // the transient lead point: the first of the first eight points whose
-    // operating point converges (with re-pivot rounds, as esl_tran_solve)
-    const u32 k = (u32)std::min<size_t>(count, 8);
-    float ms = 0;
-    esl_status r2 = launch_lpw(*e, r.a, k, 0, r.op_cap, ms);
-    if (r2 != ESL_OK) return r2;
-    vector<u32> stk(k), itk(k, 0);
-    HIPCHK(hipMemcpyAsync(stk.data(), e->status.p, k * 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    r2 = op_repivot(*e, r.a, k, stk, itk, r.lane64, ms);
-    if (r2 != ESL_OK) return r2;
-    vector<esl::LaneResume> rk(k);
-    HIPCHK(hipMemcpyAsync(rk.data(), e->resume.p, k * sizeof(esl::LaneResume), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    u32 l0 = esl::kNoArgs;
-    for (u32 l = 0; l < k; l++)
-        if (stk[l] == esl::kConverged) { l0 = l; break; }
-    if (l0 == esl::kNoArgs) {
-        g_last_error = "no operating point among the first eight points of a transient item converged";
-        return ESL_FAILED;
-    }
-    const u32 running = esl::kTranRunning, zero = 0;
-    HIPCHK(hipMemcpyAsync(e->status.as<u32>() + l0, &running, 4, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(e->tstep.as<u32>() + l0, &zero, 4, hipMemcpyHostToDevice, s));
-    u32 id = 0;
-    esl_status r3 = tran_lead_tape(*e, ta, l0, rk[l0].tape, id, r.tran_singular);
-    if (r3 != ESL_OK) return r3;
-    // the probe launches above wrote lanes < k: back to their cold start
-    HIPCHK(hipMemcpyAsync(e->resume.p, rs.data(), k * sizeof(esl::LaneResume), hipMemcpyHostToDevice, s));
+    // the transient lead tape comes from the pre-pass (batch_tran_leads);
+    // until then the step arguments dump lead candidates
     HIPCHK(hipMemsetAsync(e->tstep.p, 0xff, count * 4, s));   // every lane before its transient (kStepOp)
-    vector<u32> ids(count);
-    for (size_t l = 0; l < count; l++) ids[l] = (u32)l;
-    HIPCHK(hipMemcpyAsync(e->lane_ids.p, ids.data(), count * 4, hipMemcpyHostToDevice, s));
-    if (r.tran_singular) return ESL_OK;
-    ta.tran_tape = id;
-    r.tran_cap = e->tapes[id]->prog.n_tv;
-    ta.op.tv_cap = r.tran_cap;
+    ta.tran_tape = e->lead_id;
+    ta.op.tv_cap = r.op_cap;
     ta.init = 1;
-    ta.dump = 0;
+    ta.dump = 1;
     ta.s0 = 0;
     ta.s1 = o.steps;
+    r.needs_lead = true;
     return ESL_OK;
 }
 
@@ -3109,6 +3079,89 @@ esl_status batch_launch(BatchLaunch &L, const vector<esl::OpArgs> &ops, const ve
     return ESL_OK;
 }
 
+// The transient lead tapes of every transient item at once: one launch
+// runs the operating points of each item's first eight points and, where
+// one converges, dumps its first transient Jacobian; the first such point
+// of an item (in point order) is its lead, factored on the host.
+esl_status batch_tran_leads(vector<BatchItemRun> &runs, u32 grid, u32 lds, u32 wave_bytes, BatchLaunch &L,
+                            hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, float &ms)
+{
+    vector<esl::OpArgs> ops;
+    vector<esl::TranArgs> trans;
+    vector<std::pair<esl::BatchJob, std::pair<u32, double>>> jobs;
+    for (auto &r : runs) {
+        if (!r.needs_lead || r.lead_singular) continue;
+        Engine *e = r.e;
+        esl::OpArgs a0 = r.a;
+        a0.tapes = e->tape_table.as<esl::TapeProg>();
+        a0.mode = 0;
+        a0.dump = 0;
+        a0.tv_cap = r.op_cap;
+        a0.a_out = e->a_out.as<double>();
+        const u32 oi = (u32)ops.size();
+        ops.push_back(a0);
+        esl::TranArgs t = r.ta;
+        t.op.tapes = e->tape_table.as<esl::TapeProg>();
+        t.op.a_out = e->a_out.as<double>();
+        const u32 ti = (u32)trans.size();
+        trans.push_back(t);
+        u32 need[2];
+        job_need(e->c, r.op_cap, r.op_cap, need);
+        u32 tl = need[0] <= wave_bytes ? 0u : 1u;
+        if (tl == 1 && need[1] > lds) {
+            g_last_error = "a batch point's working set exceeds the LDS of one workgroup";
+            return ESL_FAILED;
+        }
+        double est[2];
+        model_job_ns(r, est);
+        const u32 k = (u32)std::min<size_t>(r.it->count, 8);
+        for (u32 l = 0; l < k; l++) jobs.push_back({esl::BatchJob{oi, ti, l, 0}, {tl, est[tl]}});
+    }
+    if (jobs.empty()) return ESL_OK;
+    for (auto &r : runs) HIPCHK(hipStreamSynchronize(r.e->stream));
+    Sched S = schedule(jobs, grid);
+    vector<esl::BatchJob> bj, wj;
+    vector<uint64_t> bt, wt;
+    esl_status st = batch_launch(L, ops, trans, S, grid, lds, false, s, ev0, ev1, bj, wj, bt, wt, ms);
+    if (st != ESL_OK) return st;
+    for (auto &r : runs) {
+        if (!r.needs_lead || r.lead_singular) continue;
+        Engine *e = r.e;
+        const Compiled &c = e->c;
+        const u32 k = (u32)std::min<size_t>(r.it->count, 8);
+        vector<u32> stk(k);
+        HIPCHK(hipMemcpy(stk.data(), e->status.p, k * 4, hipMemcpyDeviceToHost));
+        u32 l0 = esl::kNoArgs;
+        for (u32 l = 0; l < k; l++)
+            if (stk[l] == esl::kTranRunning) { l0 = l; break; }
+        if (l0 == esl::kNoArgs) {
+            g_last_error = "no operating point among the first eight points of a transient item converged";
+            return ESL_FAILED;
+        }
+        vector<double> A(c.nnz);
+        HIPCHK(hipMemcpy(A.data(), e->a_out.as<double>() + (size_t)l0 * c.nnz, c.nnz * 8, hipMemcpyDeviceToHost));
+        // the pre-pass wrote these points' state: back to their cold start
+        vector<esl::LaneResume> rs(k, esl::LaneResume{0, 0, e->lead_id, 0, 1e9, 0, 0, 0.0});
+        HIPCHK(hipMemcpy(e->resume.p, rs.data(), k * sizeof(esl::LaneResume), hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(e->tstep.p, 0xff, r.it->count * 4));
+        r.needs_lead = false;
+        HostTape t;
+        if (!gp_factor(c.n, c.col_ptr.data(), c.row_idx.data(), A.data(), c.q.data(), 1e-3, t)) {
+            r.tran_singular = true;
+            continue;
+        }
+        u32 id;
+        esl_status s2 = add_tape(*e, t, id);
+        if (s2 != ESL_OK) return s2;
+        r.ta.tran_tape = id;
+        r.tran_cap = e->tapes[id]->prog.n_tv;
+        r.ta.op.tv_cap = r.tran_cap;
+        r.ta.init = 1;
+        r.ta.dump = 0;
+    }
+    return ESL_OK;
+}
+
 }  // namespace
 
 extern "C" void esl_default_batch_options(esl_batch_options *o)
@@ -3161,6 +3214,14 @@ extern "C" esl_status esl_batch_solve(const esl_batch_item *items, size_t n_item
     }
     per_cu = std::max(per_cu, 1);   // one workgroup holds all the LDS it asks for
     const u32 grid = cus * (u32)per_cu;
+    static BatchLaunch g_launch;
+    Engine *e0 = runs[0].e;
+    hipStream_t s = e0->stream;
+    float lead_ms = 0;
+    {
+        esl_status st = batch_tran_leads(runs, grid, lds, wave_bytes, g_launch, s, e0->ev0, e0->ev1, lead_ms);
+        if (st != ESL_OK) return st;
+    }
     // 3. tiling per item: what fits, then the forced one, the tuned one, or both (tuning)
     bool tuning = false;
     for (auto &r : runs) {
@@ -3197,12 +3258,9 @@ extern "C" esl_status esl_batch_solve(const esl_batch_item *items, size_t n_item
         }
     }
     Sched S = schedule(jobs, grid);
-    hipStream_t s = runs[0].e->stream;
     for (auto &r : runs) HIPCHK(hipStreamSynchronize(r.e->stream));
     const double setup_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
-    static BatchLaunch g_launch;
-    Engine *e0 = runs[0].e;
     vector<esl::BatchJob> bj, wj;
     vector<uint64_t> bt, wt;
     float kms = 0;
@@ -3329,6 +3387,7 @@ extern "C" esl_status esl_batch_solve(const esl_batch_item *items, size_t n_item
         report->kernel_ms = kms;
         report->repivot_ms = rep_ms;
         report->setup_ms = setup_ms;
+        report->lead_ms = lead_ms;
         report->predicted_ms = S.makespan_ns * 1e-6;
         report->grid = grid;
         report->block_jobs = n_block;

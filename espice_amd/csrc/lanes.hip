@@ -588,6 +588,18 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
     double *pap = a.priv.ap + (size_t)lane * dp.nnz;  // the lane's Jacobian for its own LU (pattern order)
     // FP64 solve vector; in FP32 lanes and on the lane's own LU, dx in row order
     double *y = priv ? tv + a.tv_cap : tv + T.n_tv;
+    bool staged = true;
+    if (priv) {
+        // own LU: its dense column and pattern live at tv
+        staged = priv_enter(a.priv, lane, tv, a.tv_cap, tid, kWave);
+        wsync();
+    }
+    if (!staged) {
+        // cannot happen with launches of one solve (they share tv_cap); the
+        // point fails visibly rather than continue on other pivots
+        if (tid == 0) a.status[lane] = kNoRoom;
+        return kNoRoom;
+    }
     float *tv32 = reinterpret_cast<float *>(tv);
     double *a64 = a.a64 + (size_t)lane * dp.nnz;
     uint32_t iter0 = 0, k0 = 0, warm = a.mode == 3 ? 1u : 0u;
@@ -686,7 +698,7 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
         } else if (priv) {
             // the lane's own LU: replay it, factor again when a peel test fails
             int rc = kPlOk;
-            if (tid == 0) rc = priv_update_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq);
+            if (tid == 0) rc = priv_update_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq, tv, a.tv_cap);
             rc = __shfl(rc, 0, kWave);
             wsync();
             if (rc == kPlSingular) exit_status = kSingular;
@@ -698,8 +710,10 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
                 // sums as the failed assembly), factored by the lane
                 wsync();
                 assemble_pattern(dp, live, x, lim, nm, pap, rhs, st, tid);
+                for (uint32_t i = tid; i < n; i += kWave) tv[i] = 0.0;   // the factor's dense column (LDS)
+                wsync();
                 int rc = kPlOk;
-                if (tid == 0) rc = priv_factor_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq);
+                if (tid == 0) rc = priv_factor_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq, tv, a.tv_cap);
                 rc = __shfl(rc, 0, kWave);
                 for (uint32_t i = tid; i < n; i += kWave) sdiag[i] = pap[dp.diag_slot[i]];
                 wsync();
@@ -746,7 +760,7 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
         }
         if (!have_dx && priv) {
             lap(1);
-            if (tid == 0) priv_solve_at(a.priv, lane, dp.colq, rhs, y);
+            if (tid == 0) priv_solve_at(a.priv, lane, dp.colq, rhs, y, tv, a.tv_cap);
             wsync();
             have_dx = true;
         } else if (!have_dx) {
@@ -942,6 +956,18 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
     const TapeProg T = a.tapes[priv ? 0 : rs.tape];   // unused while the lane is on its own LU
     double *pap = a.priv.ap + (size_t)lane * dp.nnz;  // the lane's Jacobian for its own LU (pattern order)
     double *y = priv ? tv + a.tv_cap : tv + T.n_tv;
+    bool staged = true;
+    if (priv) {
+        // own LU: its dense column and pattern live at tv
+        staged = priv_enter(a.priv, lane, tv, a.tv_cap, tid, kWave);
+        wsync();
+    }
+    if (!staged) {
+        // cannot happen with launches of one solve (they share tv_cap); the
+        // point fails visibly rather than continue on other pivots
+        if (tid == 0) a.status[lane] = kNoRoom;
+        return;
+    }
     // the companion assembly into the tape's slots, or in pattern order for
     // the lane's own LU
     auto assemble_step = [&](bool own, double tt, double g0, double g1) {
@@ -964,10 +990,10 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
         for (uint32_t i = tid; i < n; i += kWave) q1[i] = q[i];
         wsync();
         if (ta.dump) {
-            // the transient lead point's first Jacobian (step 0, iteration 0),
-            // pattern order, one lane per dump launch
-            assemble_tran(dp, nullptr, dp.nnz, live, x, lim, nm, a.a_out, rhs, q, cq, st, ta.dt, 1.0 / ta.dt, 0.0, q1,
-                          ip, tid, cr);
+            // the point's first transient Jacobian (step 0, iteration 0),
+            // pattern order, in its row of a_out: a transient lead candidate
+            assemble_tran(dp, nullptr, dp.nnz, live, x, lim, nm, a.a_out + (size_t)lane * dp.nnz, rhs, q, cq, st,
+                          ta.dt, 1.0 / ta.dt, 0.0, q1, ip, tid, cr);
             return;
         }
     }
@@ -1005,7 +1031,7 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
             int rc = kPlOk;
             bool to_host = false;
             if (priv) {
-                if (tid == 0) rc = priv_update_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq);
+                if (tid == 0) rc = priv_update_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq, tv, a.tv_cap);
                 rc = __shfl(rc, 0, kWave);
                 wsync();
                 to_host = rc == kPlFull;
@@ -1013,7 +1039,9 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
                 wsync();
                 assemble_step(can_priv, t, ag0, ag1);   // the Jacobian again, pattern order when own
                 if (can_priv) {
-                    if (tid == 0) rc = priv_factor_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq);
+                    for (uint32_t i = tid; i < n; i += kWave) tv[i] = 0.0;   // the factor's dense column (LDS)
+                    wsync();
+                    if (tid == 0) rc = priv_factor_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq, tv, a.tv_cap);
                     rc = __shfl(rc, 0, kWave);
                     wsync();
                     if (rc == kPlOk) {
@@ -1047,7 +1075,7 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
                 break;
             }
             if (priv) {
-                if (tid == 0) priv_solve_at(a.priv, lane, dp.colq, rhs, y);
+                if (tid == 0) priv_solve_at(a.priv, lane, dp.colq, rhs, y, tv, a.tv_cap);
             } else {
                 solve_neg(T, n, tv, rhs, y, tid);
             }

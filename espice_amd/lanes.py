@@ -105,6 +105,7 @@ class BatchOptions(C.Structure):
 class BatchReport(C.Structure):
     _fields_ = [("struct_size", C.c_size_t), ("kernel_ms", C.c_double), ("repivot_ms", C.c_double),
                 ("setup_ms", C.c_double), ("total_ms", C.c_double), ("predicted_ms", C.c_double),
+                ("lead_ms", C.c_double),
                 ("grid", C.c_uint32), ("block_jobs", C.c_uint32), ("wave_jobs", C.c_uint32),
                 ("repivots", C.c_uint32), ("lds_bytes", C.c_uint32), ("repivot_launches", C.c_uint32),
                 ("reserved", C.c_uint32 * 2)]

@@ -323,6 +323,7 @@ typedef struct {
     double setup_ms;           /* host: lead tapes, uploads, schedule */
     double total_ms;           /* whole call */
     double predicted_ms;       /* the scheduler's makespan estimate (0 before tuning) */
+    double lead_ms;            /* the transient lead pre-pass launch (part of setup_ms) */
     uint32_t grid;             /* workgroups of the persistent launch */
     uint32_t block_jobs;       /* jobs run lane per block */
     uint32_t wave_jobs;        /* jobs run lane per wavefront */

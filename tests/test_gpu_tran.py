@@ -94,3 +94,27 @@ def test_graph_and_launch_blocking_do_not_change_bits(lanes):
         assert np.array_equal(bits(a["probes"]), bits(r["probes"]))
         assert np.array_equal(a["iterations"], r["iterations"])
     eng.close()
+
+
+def test_switching_transient_repivots_on_the_device_bitwise(lanes, oracle):
+    """Inverter chains switching under a step input change pivot order
+    during the transient: lanes whose replay fails factor their own
+    Jacobian on the device and must still match the oracle (replay, then a
+    full factor when a peel test fails) bit for bit, probes included."""
+    fam = circuits.inverter_chain_tran(16)
+    eng = lanes.Engine(fam.deck.arrays())
+    _, _, q = eng.pattern()
+    vals = fam.sample(8, 48)
+    dt, steps = 2e-12, 60
+    probes = [eng.n - 1, 3]
+    r = eng.solve_tran(vals, dt, steps, probes=probes, steps_per_launch=13)
+    tm = eng.timing()
+    ref = oracle.tran(oracle.Circuit(fam.deck.arrays()), vals, q, dt, steps, probes=probes, threads=8)
+    say(f"  lanes on their own LU {tm.device_repivot_lanes}, host re-pivots {tm.repivots}")
+    assert (ref["reasons"] == 0).mean() > 0.9
+    assert np.array_equal(r["reasons"], ref["reasons"])
+    assert np.array_equal(r["iterations"], ref["iterations"])
+    ok = r["reasons"] == 0
+    assert np.array_equal(bits(r["x"][ok]), bits(ref["x"][ok]))
+    assert np.array_equal(bits(r["probes"][ok]), bits(ref["probes"][ok]))
+    eng.close()
