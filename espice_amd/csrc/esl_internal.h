@@ -313,6 +313,24 @@ struct LptArgs {
     unsigned long long *prof;   // optional [lanes][kProfPhases]
 };
 
+// Arguments of the lane-per-thread kernel for lanes whose working set is
+// beyond a workgroup's LDS (big_lanes): thread t runs lane
+// ta.op.lane_ids[t], .op (tran == 0: the OpArgs of ta, modes 0-2, dump) or
+// transient steps (tran == 1). Its working set is lane-major in ws,
+// element i at ws[i * N + t]: factor slots [tv_cap], staging cells
+// [kStageDoubles], then rhs, x, xold, y, sdiag, q [n each] and the
+// charge-Jacobian cells [n_cslot].
+struct BigArgs {
+    TranArgs ta;
+    double *ws;
+    uint32_t N, tran;
+    uint32_t tv_cap, pad;
+};
+__host__ __device__ inline size_t big_doubles(uint32_t tv_cap, uint32_t n, uint32_t n_cslot)
+{
+    return (size_t)tv_cap + kStageDoubles + 6 * (size_t)n + n_cslot;
+}
+
 // Arguments of the LU-only replay (tape_replay).
 struct ReplayArgs {
     TapeProg T;

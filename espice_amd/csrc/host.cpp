@@ -37,6 +37,7 @@ extern "C" hipError_t esl_launch_transpose_live(const double *in, double *out, u
 extern "C" hipError_t esl_launch_tran_steps(const void *ta, uint32_t grid, uint32_t lds_bytes, hipStream_t s);
 extern "C" hipError_t esl_launch_batch(const void *b, uint32_t grid, uint32_t lds_bytes, hipStream_t s);
 extern "C" hipError_t esl_batch_occupancy(uint32_t lds_bytes, int *blocks_per_cu);
+extern "C" hipError_t esl_launch_big(const void *b, hipStream_t s);
 extern "C" hipError_t esl_launch_gather(const double *x, uint32_t n, const uint32_t *ids, uint32_t k, uint32_t count,
                                         double *out, hipStream_t s);
 
@@ -463,6 +464,7 @@ struct TapeHostProg {
     RowProgram rch, sch;
     vector<u32> red, red_off, red_len, diag_tv;
     uint64_t flops = 0;
+    bool big = false;   // column replay only (lanes beyond a workgroup's LDS): no row programs or reduce streams
     DevBuf buf;
 };
 
@@ -569,14 +571,36 @@ void build_tape_prog(const HostTape &t, const vector<u32> &col_ptr, double growt
     levels_to_lists(flev, o.flev_ptr, o.flev_rows, nflev);
     levels_to_lists(blev, o.blev_ptr, o.blev_rows, nblev);
     o.void_slots = t.void_slots;
+    // factor slots: [U | diag | L | discard | void extras | 64 pads]
+    const u32 pad_base = next_slot;
+    esl::TapeProg &P = o.prog;
+    P.n_tv = pad_base + esl::kWave;
+    P.pad_slot = pad_base;
+    P.n_u = nu;
+    P.n_l = nl;
+    P.lx0 = lx0;
+    P.discard = discard;
+    P.n_levels = nlev;
+    P.n_flev = nflev;
+    P.n_blev = nblev;
+    P.n_void = (u32)o.void_slots.size();
+    P.growth = growth;
+    P.wide = wide;
+}
+
+// The row programs of a tape (the wavefront kernels' refactor and solves):
+// every operation of the column replay scheduled into rows of 64
+// independent words.
+void build_row_programs(const HostTape &t, TapeHostProg &o)
+{
+    const u32 n = t.n, nu = t.nnz_u;
+    const u32 lx0 = o.prog.lx0, pad_base = o.prog.pad_slot, n_tv = o.prog.n_tv;
+    const u32 yb = n_tv;   // y(j) sits at index n_tv + j
     // Refactor ops (the column replay of sparse_lu.zig:916-1024): every slot
     // starts from its assembled value and takes its multiply-subtracts in
     // stored topological order; an L slot ends with its pivot division. A
     // void column's pivot is set to 1 before the rows run and its L slots
     // stay at their assembled zero.
-    const u32 pad_base = next_slot;
-    const u32 n_tv = pad_base + esl::kWave;
-    const u32 yb = n_tv;   // y(j) sits at index n_tv + j
     {
         vector<vector<std::pair<u32, u32>>> terms(n_tv);   // (L slot, U slot) per target
         for (u32 k = 0; k < n; k++) {
@@ -637,19 +661,6 @@ void build_tape_prog(const HostTape &t, const vector<u32> &col_ptr, double growt
         g.schedule(pad_base, o.sch);
         o.prog.n_slev = o.sch.critical;
     }
-    esl::TapeProg &P = o.prog;
-    P.n_tv = n_tv;
-    P.pad_slot = pad_base;
-    P.n_u = nu;
-    P.n_l = nl;
-    P.lx0 = lx0;
-    P.discard = discard;
-    P.n_levels = nlev;
-    P.n_flev = nflev;
-    P.n_blev = nblev;
-    P.n_void = (u32)o.void_slots.size();
-    P.growth = growth;
-    P.wide = wide;
 }
 
 hipError_t upload_tape_prog(TapeHostProg &o)
@@ -1201,6 +1212,11 @@ struct Engine {
     bool profiling = false;
     vector<unsigned long long> prof_host;
     u32 lds_max = 0;
+    // lanes beyond a workgroup's LDS: every launch runs one thread per lane
+    // with its working set in the workspace bigws (lane-major)
+    bool big = false;
+    u32 lds_limit = 0;   // the device's LDS per workgroup
+    DevBuf bigws;
     uint64_t lead_flops = 0;
     u32 lead_levels = 0, lead_nl = 0, lead_nu = 0, lead_lds = 0;
 
@@ -1420,8 +1436,11 @@ bool validate_tape(const TapeHostProg &o, u32 n, u32 nnz)
     for (u32 v : o.urow_slot) if (v >= P.n_tv) return false;
     for (u32 v : o.void_slots) if (v >= nnz) return false;
     if (P.lx0 + P.n_l > P.n_tv || P.n_u + n > P.n_tv) return false;
-    if (P.n_tv + n >= 65536) return false;
     if (P.pad_slot + esl::kWave != P.n_tv) return false;
+    if ((o.big || !o.red_off.empty()) && o.diag_tv.size() != n) return false;
+    for (u32 v : o.diag_tv) if (v >= P.n_tv) return false;
+    if (o.big) return o.rch.words.empty() && o.sch.words.empty() && o.red.empty();
+    if (P.n_tv + n >= 65536) return false;
     auto rows_ok = [&](const RowProgram &rp, bool solve) {
         if (rp.rows % esl::kRing || rp.used > rp.rows) return false;
         if (rp.words.size() != (size_t)(rp.rows + esl::kRing) * esl::kWave) return false;
@@ -1442,7 +1461,7 @@ bool validate_tape(const TapeHostProg &o, u32 n, u32 nnz)
     };
     if (!rows_ok(o.rch, false) || !rows_ok(o.sch, true)) return false;
     if (!o.red_off.empty()) {
-        if (o.red_len.size() != o.red_off.size() || o.diag_tv.size() != n) return false;
+        if (o.red_len.size() != o.red_off.size()) return false;
         for (size_t ci = 0; ci < o.red_off.size(); ci++) {
             if ((size_t)o.red_off[ci] + (size_t)(o.red_len[ci] + esl::kRing) * esl::kWave > o.red.size()) return false;
             if (o.red_len[ci] % esl::kRing) return false;
@@ -1455,7 +1474,6 @@ bool validate_tape(const TapeHostProg &o, u32 n, u32 nnz)
                 return false;
             }
         }
-        for (u32 v : o.diag_tv) if (v >= P.n_tv) return false;
     }
     return true;
 }
@@ -1528,14 +1546,27 @@ u32 device_lds_max()
 
 // The host half of registering a tape: its row programs and reduce
 // streams, checked. No device calls, so it runs on worker threads.
+// A tape of an engine whose lanes run one per thread in HBM (big), or the
+// first tape of an engine that does not fit a workgroup's LDS (which makes
+// the engine big), keeps only the column replay.
 TapeHostProg *build_tape(const Engine &e, const HostTape &t, const char *&err)
 {
     auto *tp = new TapeHostProg();
     build_tape_prog(t, e.c.col_ptr, 1e-12, 1.0 / 1e-3, *tp);
-    build_reduce_streams(e.c, *tp);
+    const u32 n = e.c.n;
+    const bool fits = lds_bytes_for(tp->prog.n_tv, n) <= e.lds_limit && tp->prog.n_tv + n < 65536;
     err = nullptr;
-    if (!validate_tape(*tp, e.c.n, e.c.nnz)) err = "tape program failed validation";
-    else if (lds_bytes_for(tp->prog.n_tv, e.c.n) > device_lds_max()) err = "lane state exceeds the LDS of one workgroup";
+    if (e.big || (!fits && e.tapes.empty())) {
+        tp->big = true;
+        tp->diag_tv.assign(n, 0);
+        for (u32 i = 0; i < n; i++) tp->diag_tv[i] = tp->amap[e.c.diag[i]];
+    } else if (!fits) {
+        err = "lane state exceeds the LDS of one workgroup";
+    } else {
+        build_row_programs(t, *tp);
+        build_reduce_streams(e.c, *tp);
+    }
+    if (!err && !validate_tape(*tp, n, e.c.nnz)) err = "tape program failed validation";
     if (err) {
         delete tp;
         return nullptr;
@@ -1575,6 +1606,7 @@ esl_status register_tape(Engine &e, vector<u32> key, TapeHostProg *tp, u32 &id)
     e.tapes.push_back(tp);
     e.tape_index[std::move(key)] = id;
     e.lds_max = std::max(e.lds_max, lds_bytes_for(tp->prog.n_tv, e.c.n));
+    if (tp->big) e.big = true;
     return ESL_OK;
 }
 
@@ -1688,11 +1720,10 @@ esl_status esl_engine_create(const esl_deck *deck, esl_engine **out, char *diag,
         delete e;
         return ESL_FAILED;
     }
-    if (lds_bytes_for(e->c.nnz + 1, e->c.n) > device_lds_max()) {
-        put_diag(diag, cap, "lane state exceeds the LDS of one workgroup");
-        delete e;
-        return ESL_FAILED;
-    }
+    // a deck whose Jacobian alone exceeds a workgroup's LDS runs its lanes
+    // one per thread, working sets in HBM
+    e->lds_limit = device_lds_max();
+    e->big = lds_bytes_for(e->c.nnz + 1, e->c.n) > e->lds_limit;
     hipError_t er = upload_deck(*e);
     if (er == hipSuccess) er = hipStreamCreate(&e->stream);
     if (er == hipSuccess) er = hipEventCreate(&e->ev0);
@@ -1826,6 +1857,50 @@ esl_status launch_lpt(Engine &e, const HostOpArgs &a, u32 lanes, u32 mode, float
     return ESL_OK;
 }
 
+// The workspace of a lane-per-thread launch (big engines) over `lanes`
+// lanes with up to tv_cap factor slots each.
+esl_status big_room(Engine &e, size_t lanes, u32 tv_cap)
+{
+    const size_t per = esl::big_doubles(tv_cap, e.c.n, (u32)e.c.cslot_pat.size());
+    HIPCHK(e.bigws.ensure(lanes * per * 8));
+    return ESL_OK;
+}
+
+esl::BigArgs big_args(Engine &e, const esl::TranArgs &t, u32 lanes, u32 tran)
+{
+    esl::BigArgs b{};
+    b.ta = t;
+    b.ws = e.bigws.as<double>();
+    b.N = lanes;
+    b.tran = tran;
+    b.tv_cap = t.op.tv_cap;
+    return b;
+}
+
+// Launches the .op kernel over `grid` lanes (e.lane_ids; dump: op_dump):
+// one wavefront per lane with its working set in LDS, or for a big engine
+// one thread per lane with its working set in HBM.
+esl_status launch_op(Engine &e, const HostOpArgs &d, u32 grid, hipStream_t s)
+{
+    if (e.big) {
+        esl_status r = big_room(e, grid, d.tv_cap);
+        if (r != ESL_OK) return r;
+        esl::TranArgs t{};
+        t.op = d;
+        const esl::BigArgs b = big_args(e, t, grid, 0);
+        HIPCHK(esl_launch_big(&b, s));
+        return ESL_OK;
+    }
+    const u32 lds = lds_bytes_for(d.tv_cap, e.c.n);
+    if (lds > e.lds_limit) {
+        g_last_error = "lane state exceeds the LDS of one workgroup";
+        return ESL_FAILED;
+    }
+    HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
+    HIPCHK(esl_launch_op_newton(&d, grid, lds, s));
+    return ESL_OK;
+}
+
 // One wavefront-per-lane launch over `lanes` lanes listed in e.lane_ids.
 esl_status launch_lpw(Engine &e, HostOpArgs a, u32 lanes, u32 mode, u32 tv_cap, float &ms)
 {
@@ -1833,10 +1908,11 @@ esl_status launch_lpw(Engine &e, HostOpArgs a, u32 lanes, u32 mode, u32 tv_cap, 
     a.mode = mode;
     a.dump = 0;
     a.tv_cap = tv_cap;
-    const u32 lds = lds_bytes_for(tv_cap, e.c.n);
-    HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
     HIPCHK(hipEventRecord(e.ev0, e.stream));
-    HIPCHK(esl_launch_op_newton(&a, lanes, lds, e.stream));
+    {
+        esl_status r = launch_op(e, a, lanes, e.stream);
+        if (r != ESL_OK) return r;
+    }
     HIPCHK(hipEventRecord(e.ev1, e.stream));
     HIPCHK(hipEventSynchronize(e.ev1));
     HIPCHK(hipEventElapsedTime(&ms, e.ev0, e.ev1));
@@ -1930,9 +2006,8 @@ esl_status op_prepare(Engine &E, const double *values, size_t count, const esl_o
         d.tv_cap = nnz + 1;
         d.a_out = e->a_out.as<double>();
         d.f_out = e->f_out.as<double>();
-        const u32 lds = lds_bytes_for(nnz + 1, n);
-        HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
-        HIPCHK(esl_launch_op_newton(&d, 1, lds, s));
+        esl_status r = launch_op(*e, d, 1, s);
+        if (r != ESL_OK) return r;
         vector<double> a0(nnz);
         HIPCHK(hipMemcpyAsync(a0.data(), e->a_out.p, nnz * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
@@ -1953,7 +2028,7 @@ esl_status op_prepare(Engine &E, const double *values, size_t count, const esl_o
     }
     // FP64 solves re-pivot on the device, each lane on its own LU
     a.priv = esl::PrivLU{};
-    if (o.precision == ESL_FP64 && !(o.flags & ESL_OP_HOST_REPIVOT))
+    if (o.precision == ESL_FP64 && !(o.flags & ESL_OP_HOST_REPIVOT) && !e->big)
         HIPCHK(priv_setup(*e, count, e->lead_nl, e->lead_nu, s, a.priv));
     e->timing.lead_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_lead0).count();
@@ -2005,9 +2080,8 @@ esl_status op_repivot(Engine &E, const HostOpArgs &a, size_t count, vector<u32> 
         d.tv_cap = std::max(tv_cap, nnz + 1);
         d.a_out = e->a_out.as<double>();
         d.f_out = e->f_out.as<double>();
-        const u32 dl = lds_bytes_for(d.tv_cap, n);
-        HIPCHK(esl_kernel_attrs(std::max<u32>(dl, 65536)));
-        HIPCHK(esl_launch_op_newton(&d, (u32)again.size(), dl, s));
+        esl_status r = launch_op(*e, d, (u32)again.size(), s);
+        if (r != ESL_OK) return r;
         vector<double> A(again.size() * (size_t)nnz);
         HIPCHK(hipMemcpyAsync(A.data(), e->a_out.p, A.size() * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(rs.data(), e->resume.p, count * sizeof(esl::LaneResume), hipMemcpyDeviceToHost, s));
@@ -2119,7 +2193,15 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
     const bool wave_only = fp32 || net || c.n_dio > 0 || c.n_bjt > 0;
     if (wave_only && o.tiling == ESL_TILING_THREAD) return ESL_INVALID_ARGUMENT;
     u32 tiling = wave_only ? (u32)ESL_TILING_WAVE : o.tiling;
-    if (tiling == ESL_TILING_AUTO) {
+    if (e->big) {
+        // lanes beyond a workgroup's LDS: FP64 from a cold or given start,
+        // one thread per lane
+        if (fp32 || net) {
+            g_last_error = "FP32 factors and learned starts need lanes that fit a workgroup's LDS";
+            return ESL_INVALID_ARGUMENT;
+        }
+        tiling = ESL_TILING_WAVE;
+    } else if (tiling == ESL_TILING_AUTO) {
         if (e->tuned) {
             tiling = e->tuned;
         } else if (count >= 1024) {
@@ -2136,7 +2218,7 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
             tiling = ESL_TILING_WAVE;
         }
     }
-    e->timing.tiling = tiling;
+    e->timing.tiling = e->big ? (u32)ESL_TILING_THREAD : tiling;
 
     vector<u32> st(count), its(count);
     float kernel_ms = 0, ms = 0;
@@ -2238,6 +2320,31 @@ u32 tran_lds_bytes(const Compiled &c, u32 tv_cap, u32 nw = 1)
 {
     return (u32)((size_t)esl::tran_doubles(tv_cap, c.n, (u32)c.cslot_pat.size(), nw) * sizeof(double));
 }
+
+// Room for transient launches over `lanes` lanes with up to tv_cap factor
+// slots each: the kernels' LDS (checked, attributes set), or a big engine's
+// workspace. Runs before a graph capture of the launches.
+esl_status tran_room(Engine &e, size_t lanes, u32 tv_cap)
+{
+    if (e.big) return big_room(e, lanes, tv_cap);
+    const u32 lds = tran_lds_bytes(e.c, tv_cap);
+    if (lds > e.lds_limit) {
+        g_last_error = "transient lane state exceeds the LDS of one workgroup";
+        return ESL_FAILED;
+    }
+    HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
+    return ESL_OK;
+}
+
+// One transient launch over `grid` lanes (e.lane_ids), after tran_room.
+hipError_t launch_tran(Engine &e, const esl::TranArgs &t, u32 grid, hipStream_t s)
+{
+    if (e.big) {
+        const esl::BigArgs b = big_args(e, t, grid, 1);
+        return esl_launch_big(&b, s);
+    }
+    return esl_launch_tran_steps(&t, grid, tran_lds_bytes(e.c, t.op.tv_cap), s);
+}
 }  // namespace
 
 namespace {
@@ -2260,13 +2367,9 @@ esl_status tran_lead_tape(Engine &E, const esl::TranArgs &ta, u32 l0, u32 op_tap
     d.dump = 1;
     d.s0 = 0;
     d.s1 = ta.steps;
-    const u32 lds = tran_lds_bytes(c, d.op.tv_cap);
-    if (lds > device_lds_max()) {
-        g_last_error = "transient lane state exceeds the LDS of one workgroup";
-        return ESL_FAILED;
-    }
-    HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
-    HIPCHK(esl_launch_tran_steps(&d, 1, lds, s));
+    esl_status r = tran_room(*e, 1, d.op.tv_cap);
+    if (r != ESL_OK) return r;
+    HIPCHK(launch_tran(*e, d, 1, s));
     vector<double> A(nnz);
     HIPCHK(hipMemcpyAsync(A.data(), e->a_out.as<double>() + (size_t)l0 * nnz, nnz * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -2336,17 +2439,13 @@ esl_status tran_repivot(Engine &E, const esl::TranArgs &ta, size_t count, u32 S,
         HIPCHK(hipMemcpyAsync(e->lane_ids.p, next.data(), next.size() * 4, hipMemcpyHostToDevice, s));
         u32 cap = 0;
         for (u32 l : next) cap = std::max(cap, e->tapes[rs[l].tape]->prog.n_tv);
-        const u32 lds = tran_lds_bytes(c, cap);
-        if (lds > device_lds_max()) {
-            g_last_error = "transient lane state exceeds the LDS of one workgroup";
-            return ESL_FAILED;
-        }
-        HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
+        esl_status r = tran_room(*e, next.size(), cap);
+        if (r != ESL_OK) return r;
         float ms = 0;
         HIPCHK(hipEventRecord(e->ev0, s));
         for (u32 b = b0; b < B; b++) {
             esl::TranArgs t = block_args(b, cap, 0);
-            HIPCHK(esl_launch_tran_steps(&t, (u32)next.size(), lds, s));
+            HIPCHK(launch_tran(*e, t, (u32)next.size(), s));
         }
         HIPCHK(hipEventRecord(e->ev1, s));
         HIPCHK(hipEventSynchronize(e->ev1));
@@ -2434,7 +2533,7 @@ extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t
     ta.dt = o.dt;
     ta.steps = o.steps;
     ta.n_probes = o.n_probes;
-    HIPCHK(priv_setup(*e, count, e->lead_nl, e->lead_nu, s, a.priv));
+    if (!e->big) HIPCHK(priv_setup(*e, count, e->lead_nl, e->lead_nu, s, a.priv));
 
     // 3. the transient lead tape: the first converged point's first
     // transient Jacobian (its resume record still names its .op tape)
@@ -2476,12 +2575,8 @@ extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t
     float kms = 0;
     {
         const u32 cap = e->tapes[rs[0].tape]->prog.n_tv;
-        const u32 lds = tran_lds_bytes(c, cap);
-        if (lds > device_lds_max()) {
-            g_last_error = "transient lane state exceeds the LDS of one workgroup";
-            return ESL_FAILED;
-        }
-        HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
+        esl_status r = tran_room(*e, count, cap);
+        if (r != ESL_OK) return r;
         HIPCHK(hipEventRecord(e->ev0, s));
         if (o.use_graph) {
             hipGraph_t g = nullptr;
@@ -2490,7 +2585,7 @@ extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t
             hipError_t ce = hipSuccess;
             for (u32 b = 0; b < B && ce == hipSuccess; b++) {
                 esl::TranArgs t = block_args(b, cap, b == 0);
-                ce = esl_launch_tran_steps(&t, (u32)count, lds, s);
+                ce = launch_tran(*e, t, (u32)count, s);
             }
             hipError_t ee = hipStreamEndCapture(s, &g);
             if (ce != hipSuccess) return fail_hip(ce, "tran capture");
@@ -2509,7 +2604,7 @@ extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t
         } else {
             for (u32 b = 0; b < B; b++) {
                 esl::TranArgs t = block_args(b, cap, b == 0);
-                HIPCHK(esl_launch_tran_steps(&t, (u32)count, lds, s));
+                HIPCHK(launch_tran(*e, t, (u32)count, s));
             }
         }
         HIPCHK(hipEventRecord(e->ev1, s));
@@ -2711,6 +2806,10 @@ esl_status validate_batch_item(const esl_batch_item &it, Engine *&e)
     e = static_cast<Engine *>(lookup(it.engine, 1));
     if (!e) return ESL_INVALID_HANDLE;
     if (it.count == 0 || it.count > 0x7fffffff || !it.x || !it.iterations || !it.reasons) return ESL_INVALID_ARGUMENT;
+    if (e->big) {
+        g_last_error = "a batch point's working set exceeds the LDS of one workgroup";
+        return ESL_INVALID_ARGUMENT;
+    }
     if (e->c.n_live && !it.values) return ESL_INVALID_ARGUMENT;
     for (size_t i = 0; i < it.count * e->c.n_live; i++)
         if (!std::isfinite(it.values[i])) return ESL_INVALID_ARGUMENT;
@@ -3465,7 +3564,9 @@ esl_status esl_tape_create(uint32_t n, const uint32_t *col_ptr, const uint32_t *
         return ESL_FAILED;
     }
     build_tape_prog(t->t, t->col_ptr, 1e-12, 1.0 / pivot_tol, t->prog);
-    if (!validate_tape(t->prog, n, t->nnz) || (size_t)(t->prog.prog.n_tv + 2 * n) * 8 > device_lds_max()) {
+    const bool fits = (size_t)(t->prog.prog.n_tv + 2 * n) * 8 <= device_lds_max() && t->prog.prog.n_tv + n < 65536;
+    if (fits) build_row_programs(t->t, t->prog);
+    if (!fits || !validate_tape(t->prog, n, t->nnz)) {
         delete t;
         return ESL_FAILED;
     }
