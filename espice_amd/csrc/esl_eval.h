@@ -1,6 +1,8 @@
-// Device evaluation shared by the lane kernels: one instance of a chunk
-// into the staging cells st[e * 64 + t]. x and st are any indexable (a
-// pointer, or a lane-major view for the lane-per-thread kernels).
+// Device code shared by the lane kernels: wavefront reductions and the
+// lane's own barrier, device evaluation of one instance of a chunk into the
+// staging cells st[e * 64 + t] (x and st any indexable: a pointer, or a
+// lane-major view), the limiting-state seed and the row programs of the
+// refactor and solves.
 #pragma once
 #include <stdint.h>
 
@@ -8,6 +10,36 @@
 #include "esl_internal.h"
 
 namespace esl {
+
+__device__ inline uint64_t dbits(double d) { return __builtin_bit_cast(uint64_t, d); }
+__device__ inline double bitsd(uint64_t b) { return __builtin_bit_cast(double, b); }
+
+__device__ inline uint64_t wave_max_u64(uint64_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        uint64_t w = __shfl_xor(v, o, kWave);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+__device__ inline double wave_fmax(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+__device__ inline bool wave_any(bool p) { return __any((int)p) != 0; }
+
+// A lane's own barrier: orders the accesses of its wavefront's threads to
+// LDS and memory (waits for outstanding accesses) without a workgroup
+// barrier, so wavefronts of one workgroup that run different lanes never
+// wait on each other.
+__device__ inline void wsync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
 
 // A lane-major array seen as one lane's vector: element i at p[i * s].
 struct Strided {
@@ -129,5 +161,120 @@ __device__ inline void eval_chunk(const DeckProg &dp, const Chunk &ch, const dou
     }
 }
 
+// Limiting state of a start at x (no junction seeds): the previous terminal
+// voltages are x's own.
+__device__ inline void seed_lim_at(const DeckProg &dp, const double *live, const double *x, double *lim,
+                                   uint32_t tid)
+{
+    const uint32_t nm = dp.n_mos;
+    for (uint32_t j = tid; j < nm; j += kWave) {
+        const Mos m = mos_load(dp, j, live);
+        const uint32_t *t4 = dp.m_t + 4 * j;
+        const double xd = x[t4[0]], xgg = x[t4[1]], xs = x[t4[2]], xb = x[t4[3]];
+        lim[j] = m.tp * (xgg - xs);
+        lim[nm + j] = m.tp * (xd - xs);
+        lim[2 * nm + j] = m.tp * (xb - xs);
+        lim[3 * nm + j] = 0.0;
+    }
+    for (uint32_t j = tid; j < dp.n_dio; j += kWave) lim[4 * nm + j] = x[dp.d_t[2 * j]] - x[dp.d_t[2 * j + 1]];
+    for (uint32_t j = tid; j < dp.n_bjt; j += kWave) {
+        const uint32_t *t4 = dp.q_t + 4 * j;
+        const double tp = dp.q_par[QF_TYPE * dp.n_bjt + j];
+        lim[lim_bjt(dp) + 2 * j] = tp * (x[t4[1]] - x[t4[2]]);
+        lim[lim_bjt(dp) + 2 * j + 1] = tp * (x[t4[1]] - x[t4[0]]);
+    }
+}
+
+// A lane's start (OpArgs modes 0 cold, 1 given x, 2 resume at the saved
+// iterate) and, except on a resume, its limiting state.
+__device__ inline void lane_init(const OpArgs &a, uint32_t lane, const double *live, double *x, double *lim,
+                                 uint32_t tid)
+{
+    const DeckProg &dp = a.dp;
+    const uint32_t n = dp.n, nm = dp.n_mos;
+    const double *xg = a.x + (size_t)lane * n;
+    if (a.mode == 0) {
+        for (uint32_t i = tid; i < n; i += kWave) x[i] = 0.0;
+    } else {
+        for (uint32_t i = tid; i < n; i += kWave) x[i] = xg[i];
+    }
+    wsync();
+    if (a.mode == 2) return;
+    if (a.mode == 0) {
+        for (uint32_t j = tid; j < nm; j += kWave) {
+            const Mos m = mos_load(dp, j, live);
+            lim[j] = m.tp * m.vbit + m.gammax * sqrt(m.phit);
+            lim[nm + j] = 0.0;
+            lim[2 * nm + j] = -1.0;
+            lim[3 * nm + j] = 0.0;
+        }
+        for (uint32_t j = tid; j < dp.n_dio; j += kWave) lim[4 * nm + j] = dio_load(dp, j, live).vcrit;
+        for (uint32_t j = tid; j < dp.n_bjt; j += kWave) {
+            lim[lim_bjt(dp) + 2 * j] = bjt_load(dp, j, live).vcrit;
+            lim[lim_bjt(dp) + 2 * j + 1] = 0.0;
+        }
+    } else {
+        seed_lim_at(dp, live, x, lim, tid);
+    }
+    wsync();
+}
+
+// ---- row programs (refactor and solves) ----
+// The peel-test state a row program accumulates: a failed growth test,
+// non-finite quotients (bit patterns ORed) and the largest multiplier (bit
+// pattern max).
+struct LuFlags {
+    bool bad;
+    uint64_t nonfinite, worst;
+};
+
+// One row of a row program: every thread runs one op of the row's kind on
+// M = [factor slots | solve vector], in the factor precision R. Rows run in
+// order within the wavefront (LDS accesses of one wavefront complete in issue
+// order), so a row sees the results of every earlier row. The peel tests are
+// computed in FP64 on the values.
+template <typename R>
+__device__ inline void run_row(esl::uint2_t w, R *M, double growth, LuFlags &f)
+{
+    const uint32_t kind = __builtin_amdgcn_readfirstlane(esl::row_kind(w.y));
+    const uint32_t d = w.x & 0xffffu, b = w.x >> 16, c = w.y & 0xffffu;
+    if (kind == R_MAC) {
+        const R v = M[d], A = M[b], B = M[c];
+        M[d] = v - A * B;
+    } else if (kind == R_MACZ) {
+        const R v = M[d], A = M[b], B = M[c];
+        if (B != R(0)) M[d] = v - A * B;
+    } else if (kind == R_DIV) {
+        const R v = M[d], A = M[b];
+        const R q = v / A;
+        M[d] = q;
+        const uint32_t fl = esl::row_flags(w.y);
+        const bool u = fl & kFlagUnscaled;
+        const uint64_t nf = dbits((double)(q - q));
+        const bool check_growth = growth > 0;
+        f.bad |= u && check_growth && !(fabs((double)A) >= growth * fabs((double)v));
+        f.nonfinite |= ((u && !check_growth) || (fl & kFlagScaled)) ? nf : 0;
+        const uint64_t lb = u ? dbits(fabs((double)q)) : 0;
+        f.worst = lb > f.worst ? lb : f.worst;
+    }
+}
+
+// Runs a row program with kRing rows of words in flight.
+template <typename R>
+__device__ void run_rows(const esl::uint2_t *words, uint32_t rows, R *M, double growth, LuFlags &f, uint32_t tid)
+{
+    const esl::uint2_t *w = words + tid;
+    esl::uint2_t q[kRing];
+#pragma unroll
+    for (int k = 0; k < kRing; k++) q[k] = w[k * kWave];
+    for (uint32_t row = 0; row < rows; row += kRing) {
+#pragma unroll
+        for (int k = 0; k < kRing; k++) {
+            const esl::uint2_t cur = q[k];
+            q[k] = w[(row + kRing + k) * kWave];
+            run_row<R>(cur, M, growth, f);
+        }
+    }
+}
 
 }  // namespace esl

@@ -146,6 +146,16 @@ __host__ __device__ inline uint32_t tran_doubles(uint32_t tv_cap, uint32_t n, ui
     return tv_cap + tran_y_block(n, nw) + 3 * n + n_cslot;
 }
 
+// One block of a big tape's replay: a run of consecutive ops of the row
+// program order whose slots fit LDS. The block gathers its m slots
+// (blk_slots[slot_off ...]) into LDS positions 0..m-1, runs its rows (words
+// from row row0 of blk_words, indices into [m slots | 64 pads]), then
+// scatters the first w positions (the slots it writes) back.
+struct BigBlock {
+    uint32_t slot_off, m, w, rows;   // rows: a multiple of kRing
+    uint32_t row0, pad0, pad1, pad2;
+};
+
 struct TapeProg {
     uint32_t n_tv, n_u, n_l, lx0;   // tv = [U | diag | L | discard | void extras | 64 pads]
     uint32_t discard, n_levels, n_flev, n_blev;
@@ -176,6 +186,16 @@ struct TapeProg {
     const uint32_t *red_len;    // [n_chunks] rows of each chunk's stream (multiple of kRing)
     const uint32_t *diag_tv;    // [n] tv slot of each row's diagonal
     const uint32_t *li;         // [n_l] row (pivot step) of each L entry
+    // big tapes (lanes whose slots live in HBM): the replay in blocks of at
+    // most blk_m slots; the solves in chunks of columns, each chunk's L
+    // (forward) or U (backward) entries at most sol_cap: columns
+    // fch[c] .. fch[c + 1] and bch[c] .. bch[c + 1]
+    uint32_t n_blk, blk_m, n_fch, n_bch;
+    uint32_t sol_cap, pad5, pad6, pad7;
+    const BigBlock *blk;        // [n_blk]
+    const uint32_t *blk_slots;  // tv slots of each block's LDS positions
+    const uint2_t *blk_words;   // rows of all blocks, [row][64]
+    const uint32_t *fch, *bch;  // [n_fch + 1], [n_bch + 1]
 };
 
 // Per-lane results of the Newton kernel.
@@ -313,22 +333,35 @@ struct LptArgs {
     unsigned long long *prof;   // optional [lanes][kProfPhases]
 };
 
-// Arguments of the lane-per-thread kernel for lanes whose working set is
-// beyond a workgroup's LDS (big_lanes): thread t runs lane
-// ta.op.lane_ids[t], .op (tran == 0: the OpArgs of ta, modes 0-2, dump) or
-// transient steps (tran == 1). Its working set is lane-major in ws,
-// element i at ws[i * N + t]: factor slots [tv_cap], staging cells
-// [kStageDoubles], then rhs, x, xold, y, sdiag, q [n each] and the
-// charge-Jacobian cells [n_cslot].
+// Arguments of the kernel for lanes whose working set is beyond a
+// workgroup's LDS (big_lanes): one wavefront per lane, workgroup g runs lane
+// ta.op.lane_ids[g], .op (tran == 0: the OpArgs of ta, modes 0-2, dump) or
+// transient steps (tran == 1). The lane's vectors and factor slots live in
+// HBM at ws + g * ws_stride: tv [tv_cap], then rhs, x, xold, sdiag, q [n
+// each] and the charge-Jacobian cells [n_cslot]; LDS (lds doubles) holds
+// the staging cells, a block of the replay, or the solve vector and a chunk
+// of factor entries.
 struct BigArgs {
     TranArgs ta;
     double *ws;
+    uint64_t ws_stride;
     uint32_t N, tran;
-    uint32_t tv_cap, pad;
+    uint32_t tv_cap, lds;
 };
-__host__ __device__ inline size_t big_doubles(uint32_t tv_cap, uint32_t n, uint32_t n_cslot)
+__host__ __device__ inline size_t big_stride(uint32_t tv_cap, uint32_t n, uint32_t n_cslot)
 {
-    return (size_t)tv_cap + kStageDoubles + 6 * (size_t)n + n_cslot;
+    const size_t d = (size_t)tv_cap + 5 * (size_t)n + n_cslot;
+    return (d + 15) / 16 * 16;   // 128-byte aligned lanes
+}
+// the LDS of a big lane: the staging cells, a replay block of blk_m slots
+// and 64 pads, or the solve vector with a chunk of sol_cap entries and
+// their indices
+__host__ __device__ inline uint32_t big_lds_doubles(uint32_t n, uint32_t blk_m, uint32_t sol_cap)
+{
+    uint32_t d = blk_m + kWave;
+    const uint32_t s = n + sol_cap + (sol_cap + 1) / 2;
+    d = s > d ? s : d;
+    return kStageDoubles > d ? kStageDoubles : d;
 }
 
 // Arguments of the LU-only replay (tape_replay).

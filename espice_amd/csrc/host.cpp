@@ -38,6 +38,7 @@ extern "C" hipError_t esl_launch_tran_steps(const void *ta, uint32_t grid, uint3
 extern "C" hipError_t esl_launch_batch(const void *b, uint32_t grid, uint32_t lds_bytes, hipStream_t s);
 extern "C" hipError_t esl_batch_occupancy(uint32_t lds_bytes, int *blocks_per_cu);
 extern "C" hipError_t esl_launch_big(const void *b, hipStream_t s);
+extern "C" hipError_t esl_big_attrs(uint32_t lds_bytes);
 extern "C" hipError_t esl_launch_gather(const double *x, uint32_t n, const uint32_t *ids, uint32_t k, uint32_t count,
                                         double *out, hipStream_t s);
 
@@ -464,7 +465,12 @@ struct TapeHostProg {
     RowProgram rch, sch;
     vector<u32> red, red_off, red_len, diag_tv;
     uint64_t flops = 0;
-    bool big = false;   // column replay only (lanes beyond a workgroup's LDS): no row programs or reduce streams
+    // lanes beyond a workgroup's LDS: the replay in blocks and the solves in
+    // chunks instead of whole-tape row programs and reduce streams
+    bool big = false;
+    vector<esl::BigBlock> blk;
+    vector<u32> blk_slots, fch, bch;
+    vector<esl::uint2_t> blk_words;
     DevBuf buf;
 };
 
@@ -663,31 +669,168 @@ void build_row_programs(const HostTape &t, TapeHostProg &o)
     }
 }
 
+// The replay of a big tape (lanes whose factor slots live in HBM): the ops
+// of the refactor row program in the order build_row_programs adds them
+// (column by column; in a column each target's multiply-subtracts in the
+// order of the column replay, an L target then its pivot division), cut
+// into runs that touch at most kBigBlockSlots slots; each run is scheduled
+// into rows of its own over LDS positions (the slots it writes first). The
+// solves run in chunks of columns of at most sol_cap entries. False when a
+// flop's target lies outside its column.
+constexpr u32 kBigBlockSlots = 4032;
+constexpr u32 kBigSolveCap = 512;
+
+bool build_big_program(const HostTape &t, TapeHostProg &o, u32 block_slots = kBigBlockSlots)
+{
+    const u32 n = t.n, nu = t.nnz_u;
+    const esl::TapeProg &P = o.prog;
+    const u32 lx0 = P.lx0, discard = P.discard;
+    o.blk.clear();
+    o.blk_slots.clear();
+    o.blk_words.clear();
+    vector<SchedOp> run;                  // the current block's ops (tv slots)
+    vector<u32> stamp(P.n_tv, NONE);      // block id that touched a slot
+    vector<u32> local(P.n_tv, NONE);
+    u32 touched = 0, blk_id = 0, rows_total = 0;
+    auto close = [&]() {
+        if (run.empty()) return;
+        esl::BigBlock b{};
+        b.slot_off = (u32)o.blk_slots.size();
+        u32 m = 0;
+        for (const SchedOp &op : run)
+            if (local[op.d] == NONE) {
+                local[op.d] = m++;
+                o.blk_slots.push_back(op.d);
+            }
+        b.w = m;
+        for (const SchedOp &op : run)
+            for (u32 s : {op.b, op.c})
+                if (local[s] == NONE) {
+                    local[s] = m++;
+                    o.blk_slots.push_back(s);
+                }
+        b.m = m;
+        OpGraph g(m + esl::kWave);
+        for (const SchedOp &op : run) g.add(local[op.d], local[op.b], local[op.c], op.kind, op.flags);
+        RowProgram rp;
+        g.schedule(m, rp);
+        b.rows = rp.rows;
+        b.row0 = rows_total;
+        rows_total += rp.rows + esl::kRing;
+        o.blk_words.insert(o.blk_words.end(), rp.words.begin(), rp.words.end());
+        for (u32 k = b.slot_off; k < (u32)o.blk_slots.size(); k++) local[o.blk_slots[k]] = NONE;
+        o.blk.push_back(b);
+        run.clear();
+        touched = 0;
+        blk_id++;
+    };
+    auto push = [&](u32 d, u32 b, u32 c, u32 kind, u32 flags) {
+        // the slots the op adds to the block (d, b, c may repeat)
+        u32 add = 0;
+        const u32 sl[3] = {d, b, c};
+        for (int i = 0; i < 3; i++) {
+            bool seen = stamp[sl[i]] == blk_id;
+            for (int j = 0; j < i; j++) seen |= sl[j] == sl[i];
+            add += seen ? 0 : 1;
+        }
+        if (touched + add > block_slots) close();
+        for (u32 s : sl)
+            if (stamp[s] != blk_id) {
+                stamp[s] = blk_id;
+                touched++;
+            }
+        run.push_back({d, b, c, kind, flags});
+    };
+    vector<vector<std::pair<u32, u32>>> terms;
+    for (u32 k = 0; k < n; k++) {
+        const u32 u0 = t.up[k], u1 = t.up[k + 1], l0 = t.lp[k], l1 = t.lp[k + 1];
+        const u32 nuk = u1 - u0, nlk = l1 - l0;
+        const bool void_k = t.flags[k] & esl::kVoidBit;
+        // each target's terms (L slot, U slot) in the column replay's order
+        terms.assign(nuk + 1 + nlk, {});
+        u32 f = o.fptr[k];
+        for (u32 p = u0; p < u1; p++) {
+            const u32 i = t.ui[p];
+            for (u32 qq = t.lp[i]; qq < t.lp[i + 1]; qq++, f++) {
+                const u32 d = o.dst[f];
+                u32 pos;
+                if (d == discard) continue;
+                if (d >= u0 && d < u1) pos = d - u0;
+                else if (d == nu + k) pos = nuk;
+                else if (d >= lx0 + l0 && d < lx0 + l1) pos = nuk + 1 + (d - lx0 - l0);
+                else return false;
+                terms[pos].push_back({lx0 + qq, p});
+            }
+        }
+        for (u32 p = u0; p < u1; p++)
+            for (auto &e : terms[p - u0]) push(p, e.first, e.second, esl::R_MAC, 0);
+        if (void_k) continue;
+        const u32 piv = nu + k;
+        for (auto &e : terms[nuk]) push(piv, e.first, e.second, esl::R_MAC, 0);
+        const u32 fl = (t.flags[k] & esl::kScaledBit) ? esl::kFlagScaled : esl::kFlagUnscaled;
+        for (u32 qq = l0; qq < l1; qq++) {
+            for (auto &e : terms[nuk + 1 + (qq - l0)]) push(lx0 + qq, e.first, e.second, esl::R_MAC, 0);
+            push(lx0 + qq, piv, lx0 + qq, esl::R_DIV, fl);
+        }
+    }
+    close();
+    // solve chunks: forward over L entries, backward over U entries and pivots
+    u32 cap = kBigSolveCap;
+    for (u32 k = 0; k < n; k++) cap = std::max({cap, t.lp[k + 1] - t.lp[k], t.up[k + 1] - t.up[k] + 1});
+    o.fch.assign(1, 0);
+    o.bch.assign(1, 0);
+    u32 cf = 0, cb = 0;
+    for (u32 k = 0; k < n; k++) {
+        const u32 lf = t.lp[k + 1] - t.lp[k], lb = t.up[k + 1] - t.up[k] + 1;
+        if (cf + lf > cap) {
+            o.fch.push_back(k);
+            cf = 0;
+        }
+        if (cb + lb > cap) {
+            o.bch.push_back(k);
+            cb = 0;
+        }
+        cf += lf;
+        cb += lb;
+    }
+    o.fch.push_back(n);
+    o.bch.push_back(n);
+    esl::TapeProg &Q = o.prog;
+    Q.n_blk = (u32)o.blk.size();
+    Q.blk_m = 0;
+    for (auto &b : o.blk) Q.blk_m = std::max(Q.blk_m, b.m);
+    Q.n_fch = (u32)o.fch.size() - 1;
+    Q.n_bch = (u32)o.bch.size() - 1;
+    Q.sol_cap = cap;
+    return true;
+}
+
 hipError_t upload_tape_prog(TapeHostProg &o)
 {
     Packer pk;
-    size_t f[32];
-    int i = 0;
-    f[i++] = pk.add(o.amap); f[i++] = pk.add(o.pinv); f[i++] = pk.add(o.qinv); f[i++] = pk.add(o.flags);
-    f[i++] = pk.add(o.lev_ptr); f[i++] = pk.add(o.lev_cols); f[i++] = pk.add(o.up); f[i++] = pk.add(o.ui);
-    f[i++] = pk.add(o.lp); f[i++] = pk.add(o.fptr); f[i++] = pk.add(o.dst);
-    f[i++] = pk.add(o.flev_ptr); f[i++] = pk.add(o.flev_rows);
-    f[i++] = pk.add(o.lrow_ptr); f[i++] = pk.add(o.lrow_k); f[i++] = pk.add(o.lrow_slot);
-    f[i++] = pk.add(o.blev_ptr); f[i++] = pk.add(o.blev_rows);
-    f[i++] = pk.add(o.urow_ptr); f[i++] = pk.add(o.urow_k); f[i++] = pk.add(o.urow_slot);
-    f[i++] = pk.add(o.void_slots);
-    f[i++] = pk.add(o.rch.words);
-    f[i++] = pk.add(o.sch.words);
-    f[i++] = pk.add(o.red);
-    f[i++] = pk.add(o.red_off); f[i++] = pk.add(o.red_len); f[i++] = pk.add(o.diag_tv);
-    f[i++] = pk.add(o.li);
+    vector<size_t> f;   // blob offset of each array, in the order they are read back below
+    f.push_back(pk.add(o.amap)); f.push_back(pk.add(o.pinv)); f.push_back(pk.add(o.qinv)); f.push_back(pk.add(o.flags));
+    f.push_back(pk.add(o.lev_ptr)); f.push_back(pk.add(o.lev_cols)); f.push_back(pk.add(o.up)); f.push_back(pk.add(o.ui));
+    f.push_back(pk.add(o.lp)); f.push_back(pk.add(o.fptr)); f.push_back(pk.add(o.dst));
+    f.push_back(pk.add(o.flev_ptr)); f.push_back(pk.add(o.flev_rows));
+    f.push_back(pk.add(o.lrow_ptr)); f.push_back(pk.add(o.lrow_k)); f.push_back(pk.add(o.lrow_slot));
+    f.push_back(pk.add(o.blev_ptr)); f.push_back(pk.add(o.blev_rows));
+    f.push_back(pk.add(o.urow_ptr)); f.push_back(pk.add(o.urow_k)); f.push_back(pk.add(o.urow_slot));
+    f.push_back(pk.add(o.void_slots));
+    f.push_back(pk.add(o.rch.words));
+    f.push_back(pk.add(o.sch.words));
+    f.push_back(pk.add(o.red));
+    f.push_back(pk.add(o.red_off)); f.push_back(pk.add(o.red_len)); f.push_back(pk.add(o.diag_tv));
+    f.push_back(pk.add(o.li));
+    f.push_back(pk.add(o.blk)); f.push_back(pk.add(o.blk_slots)); f.push_back(pk.add(o.blk_words));
+    f.push_back(pk.add(o.fch)); f.push_back(pk.add(o.bch));
     hipError_t e = o.buf.ensure(pk.blob.size());
     if (e != hipSuccess) return e;
     e = hipMemcpy(o.buf.p, pk.blob.data(), pk.blob.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) return e;
-    auto P = [&](int k) { return reinterpret_cast<const u32 *>(o.buf.as<unsigned char>() + f[k]); };
+    auto P = [&](size_t k) { return reinterpret_cast<const u32 *>(o.buf.as<unsigned char>() + f.at(k)); };
     esl::TapeProg &T = o.prog;
-    i = 0;
+    size_t i = 0;
     T.amap = P(i++); T.pinv = P(i++); T.qinv = P(i++); T.flags = P(i++);
     T.lev_ptr = P(i++); T.lev_cols = P(i++); T.up = P(i++); T.ui = P(i++);
     T.lp = P(i++); T.fptr = P(i++); T.dst = P(i++);
@@ -705,6 +848,12 @@ hipError_t upload_tape_prog(TapeHostProg &o)
     T.red_len = P(i++);
     T.diag_tv = P(i++);
     T.li = P(i++);
+    T.blk = reinterpret_cast<const esl::BigBlock *>(P(i++));
+    T.blk_slots = P(i++);
+    T.blk_words = reinterpret_cast<const esl::uint2_t *>(P(i++));
+    T.fch = P(i++);
+    T.bch = P(i++);
+    if (i != f.size()) return hipErrorInvalidValue;
     return hipSuccess;
 }
 
@@ -1407,6 +1556,46 @@ void build_reduce_streams(const Compiled &c, TapeHostProg &o)
 // ---------------------------------------------------------------- validation
 // Every index a kernel reads is checked on the host before the first launch,
 // so a malformed program is an error here rather than a GPU memory fault.
+
+// A big tape's blocks and solve chunks.
+bool validate_big(const TapeHostProg &o, u32 n)
+{
+    const esl::TapeProg &P = o.prog;
+    if (P.n_blk != o.blk.size() || o.blk_words.size() % esl::kWave) return false;
+    const size_t total_rows = o.blk_words.size() / esl::kWave;
+    u32 bm = 0;
+    for (const esl::BigBlock &b : o.blk) {
+        if (b.m == 0 || b.w == 0 || b.w > b.m || b.m > kBigBlockSlots || b.m + esl::kWave > 65536) return false;
+        if ((size_t)b.slot_off + b.m > o.blk_slots.size()) return false;
+        if (b.rows % esl::kRing || (size_t)b.row0 + b.rows + esl::kRing > total_rows) return false;
+        bm = std::max(bm, b.m);
+        const u32 lim = b.m + esl::kWave;
+        for (size_t r = b.row0; r < (size_t)b.row0 + b.rows + esl::kRing; r++) {
+            const u32 kind = esl::row_kind(o.blk_words[r * esl::kWave].y);
+            if (kind >= esl::R_KINDS || kind == esl::R_MACZ) return false;
+            for (u32 t = 0; t < (u32)esl::kWave; t++) {
+                const esl::uint2_t w = o.blk_words[r * esl::kWave + t];
+                if (esl::row_kind(w.y) != kind || (w.y >> 22) != 0) return false;
+                if (kind == esl::R_NOP) continue;
+                const u32 d = w.x & 0xffff, bb = w.x >> 16, c = w.y & 0xffff;
+                if (d >= lim || bb >= lim || c >= lim) return false;
+            }
+        }
+    }
+    if (bm != P.blk_m) return false;
+    for (u32 v : o.blk_slots) if (v >= P.n_tv) return false;
+    auto chunks_ok = [&](const vector<u32> &ch, u32 count, bool fwd) {
+        if (ch.size() != (size_t)count + 1 || ch.front() != 0 || ch.back() != n) return false;
+        for (u32 c = 0; c < count; c++) {
+            if (ch[c] >= ch[c + 1]) return false;
+            size_t e = 0;
+            for (u32 k = ch[c]; k < ch[c + 1]; k++) e += fwd ? o.lp[k + 1] - o.lp[k] : o.up[k + 1] - o.up[k] + 1;
+            if (e > P.sol_cap) return false;
+        }
+        return true;
+    };
+    return chunks_ok(o.fch, P.n_fch, true) && chunks_ok(o.bch, P.n_bch, false);
+}
 bool validate_tape(const TapeHostProg &o, u32 n, u32 nnz)
 {
     const esl::TapeProg &P = o.prog;
@@ -1439,7 +1628,7 @@ bool validate_tape(const TapeHostProg &o, u32 n, u32 nnz)
     if (P.pad_slot + esl::kWave != P.n_tv) return false;
     if ((o.big || !o.red_off.empty()) && o.diag_tv.size() != n) return false;
     for (u32 v : o.diag_tv) if (v >= P.n_tv) return false;
-    if (o.big) return o.rch.words.empty() && o.sch.words.empty() && o.red.empty();
+    if (o.big) return o.rch.words.empty() && o.sch.words.empty() && o.red.empty() && validate_big(o, n);
     if (P.n_tv + n >= 65536) return false;
     auto rows_ok = [&](const RowProgram &rp, bool solve) {
         if (rp.rows % esl::kRing || rp.used > rp.rows) return false;
@@ -1560,6 +1749,9 @@ TapeHostProg *build_tape(const Engine &e, const HostTape &t, const char *&err)
         tp->big = true;
         tp->diag_tv.assign(n, 0);
         for (u32 i = 0; i < n; i++) tp->diag_tv[i] = tp->amap[e.c.diag[i]];
+        if (!build_big_program(t, *tp)) err = "tape program failed validation";
+        else if ((size_t)esl::big_lds_doubles(n, tp->prog.blk_m, tp->prog.sol_cap) * 8 > e.lds_limit)
+            err = "too many unknowns for a lane's solve vector in LDS";
     } else if (!fits) {
         err = "lane state exceeds the LDS of one workgroup";
     } else {
@@ -1796,6 +1988,104 @@ esl_status esl_debug_profile(esl_engine *h, int enable, unsigned long long *out,
     return ESL_OK;
 }
 
+// Development hook, not part of the drop-in: the replay and solves of a
+// big tape (lanes in HBM) run on the CPU exactly as the big-lane kernel runs
+// them (blocks of at most block_slots slots gathered, their rows applied in
+// order, written slots scattered; the solves over the column chunks), for
+// `lanes` matrices a [lanes][nnz] and right-hand sides rhs [lanes][n] on
+// the tape factored from lead_vals. Outputs per lane: L, U, pivots in the
+// tape's order, dx (row order) and the re-pivot flag; sizes: the block
+// count, nnz_l and nnz_u (lanes == 0: only these).
+esl_status esl_debug_big_replay(uint32_t n, const uint32_t *col_ptr, const uint32_t *row_idx,
+                                const double *lead_vals, const uint32_t *q, double pivot_tol, uint32_t block_slots,
+                                const double *a, const double *rhs, uint32_t lanes, double *lx, double *ux,
+                                double *udiag, double *dx, uint32_t *fail, uint32_t *sizes)
+{
+    if (!col_ptr || !row_idx || !lead_vals || !q || n == 0 || block_slots < 3 || block_slots > kBigBlockSlots)
+        return ESL_INVALID_ARGUMENT;
+    if (lanes && (!a || !rhs || !lx || !ux || !udiag || !dx || !fail)) return ESL_INVALID_ARGUMENT;
+    HostTape t;
+    if (!gp_factor(n, col_ptr, row_idx, lead_vals, q, pivot_tol, t)) return ESL_FAILED;
+    vector<u32> cp(col_ptr, col_ptr + n + 1);
+    TapeHostProg o;
+    build_tape_prog(t, cp, 1e-12, 1.0 / pivot_tol, o);
+    o.big = true;
+    if (!build_big_program(t, o, block_slots) || !validate_big(o, n)) return ESL_FAILED;
+    const esl::TapeProg &T = o.prog;
+    if (sizes) {
+        sizes[0] = T.n_blk;
+        sizes[1] = T.n_l;
+        sizes[2] = T.n_u;
+    }
+    const u32 nnz = cp[n];
+    for (u32 l = 0; l < lanes; l++) {
+        vector<double> tv(T.n_tv, 0.0), M(T.blk_m + esl::kWave, 0.0), y(n);
+        for (u32 p = 0; p < nnz; p++) tv[o.amap[p]] = a[(size_t)l * nnz + p];
+        bool bad = false;
+        uint64_t nonfinite = 0, worst = 0;
+        auto bits = [](double d) { uint64_t b; memcpy(&b, &d, 8); return b; };
+        auto dbl = [](uint64_t b) { double d; memcpy(&d, &b, 8); return d; };
+        for (u32 v : o.void_slots) bad |= tv[o.amap[v]] != 0.0;
+        for (u32 k = 0; k < n; k++)
+            if (o.flags[k] & esl::kVoidBit) tv[T.n_u + k] = 1.0;
+        for (const esl::BigBlock &B : o.blk) {
+            for (u32 j = 0; j < B.m; j++) M[j] = tv[o.blk_slots[B.slot_off + j]];
+            for (u32 r = B.row0; r < B.row0 + B.rows; r++)
+                for (u32 th = 0; th < (u32)esl::kWave; th++) {
+                    const esl::uint2_t w = o.blk_words[(size_t)r * esl::kWave + th];
+                    const u32 kind = esl::row_kind(w.y), d = w.x & 0xffff, b = w.x >> 16, c = w.y & 0xffff;
+                    if (kind == esl::R_MAC) {
+                        const double m = M[b] * M[c];
+                        M[d] = M[d] - m;
+                    } else if (kind == esl::R_DIV) {
+                        const double v = M[d], A = M[b], qq = v / A;
+                        M[d] = qq;
+                        const u32 fl = esl::row_flags(w.y);
+                        const bool un = fl & esl::kFlagUnscaled;
+                        bad |= un && T.growth > 0 && !(std::fabs(A) >= T.growth * std::fabs(v));
+                        nonfinite |= ((un && !(T.growth > 0)) || (fl & esl::kFlagScaled)) ? bits(qq - qq) : 0;
+                        const uint64_t lb = un ? bits(std::fabs(qq)) : 0;
+                        worst = lb > worst ? lb : worst;
+                    }
+                }
+            for (u32 j = 0; j < B.w; j++) tv[o.blk_slots[B.slot_off + j]] = M[j];
+        }
+        for (u32 p = 0; p < T.n_u; p++) nonfinite |= bits(tv[p] - tv[p]);
+        for (u32 k = 0; k < n; k++) {
+            if (o.flags[k] & esl::kVoidBit) continue;
+            const double d = tv[T.n_u + k];
+            bad |= d == 0.0 || !(std::fabs(d) < INFINITY);
+        }
+        fail[l] = (bad || nonfinite != 0 || !(dbl(worst) <= T.wide)) ? 1u : 0u;
+        for (u32 r = 0; r < n; r++) y[o.pinv[r]] = -rhs[(size_t)l * n + r];
+        for (u32 c = 0; c < T.n_fch; c++)
+            for (u32 k = o.fch[c]; k < o.fch[c + 1]; k++) {
+                const double yk = y[k];
+                for (u32 qq = o.lp[k]; qq < o.lp[k + 1]; qq++) {
+                    const double old = y[o.li[qq]], m = tv[T.lx0 + qq] * yk, sv = old - m;
+                    y[o.li[qq]] = yk != 0.0 ? sv : old;
+                }
+            }
+        for (u32 c = T.n_bch; c > 0;) {
+            c--;
+            for (u32 k = o.bch[c + 1]; k > o.bch[c];) {
+                k--;
+                const double zk = y[k] / tv[T.n_u + k];
+                y[k] = zk;
+                for (u32 p = o.up[k]; p < o.up[k + 1]; p++) {
+                    const double old = y[o.ui[p]], m = tv[p] * zk, sv = old - m;
+                    y[o.ui[p]] = zk != 0.0 ? sv : old;
+                }
+            }
+        }
+        for (u32 p = 0; p < T.n_u; p++) ux[(size_t)l * T.n_u + p] = tv[p];
+        for (u32 k = 0; k < n; k++) udiag[(size_t)l * n + k] = tv[T.n_u + k];
+        for (u32 qq = 0; qq < T.n_l; qq++) lx[(size_t)l * T.n_l + qq] = tv[T.lx0 + qq];
+        for (u32 i = 0; i < n; i++) dx[(size_t)l * n + i] = y[o.qinv[i]];
+    }
+    return ESL_OK;
+}
+
 esl_status esl_last_timing(esl_engine *h, esl_timing *out)
 {
     auto *e = static_cast<Engine *>(lookup(h, 1));
@@ -1857,12 +2147,31 @@ esl_status launch_lpt(Engine &e, const HostOpArgs &a, u32 lanes, u32 mode, float
     return ESL_OK;
 }
 
-// The workspace of a lane-per-thread launch (big engines) over `lanes`
-// lanes with up to tv_cap factor slots each.
+// The LDS of a big engine's lanes: the staging cells, its tapes' largest
+// replay block, or the solve vector with its largest solve chunk.
+u32 big_lds(const Engine &e)
+{
+    u32 bm = 0, sc = 0;
+    for (const TapeHostProg *t : e.tapes) {
+        bm = std::max(bm, t->prog.blk_m);
+        sc = std::max(sc, t->prog.sol_cap);
+    }
+    return esl::big_lds_doubles(e.c.n, bm, sc);
+}
+
+// The workspace of a big engine's launch over `lanes` lanes with up to
+// tv_cap factor slots each, and the kernel's LDS limit (set ahead of graph
+// capture).
 esl_status big_room(Engine &e, size_t lanes, u32 tv_cap)
 {
-    const size_t per = esl::big_doubles(tv_cap, e.c.n, (u32)e.c.cslot_pat.size());
-    HIPCHK(e.bigws.ensure(lanes * per * 8));
+    const size_t stride = esl::big_stride(tv_cap, e.c.n, (u32)e.c.cslot_pat.size());
+    HIPCHK(e.bigws.ensure(lanes * stride * 8));
+    const u32 lds = big_lds(e) * 8;
+    if (lds > e.lds_limit) {
+        g_last_error = "too many unknowns for a lane's solve vector in LDS";
+        return ESL_FAILED;
+    }
+    HIPCHK(esl_big_attrs(lds));
     return ESL_OK;
 }
 
@@ -1871,9 +2180,11 @@ esl::BigArgs big_args(Engine &e, const esl::TranArgs &t, u32 lanes, u32 tran)
     esl::BigArgs b{};
     b.ta = t;
     b.ws = e.bigws.as<double>();
+    b.ws_stride = esl::big_stride(t.op.tv_cap, e.c.n, (u32)e.c.cslot_pat.size());
     b.N = lanes;
     b.tran = tran;
     b.tv_cap = t.op.tv_cap;
+    b.lds = big_lds(e);
     return b;
 }
 

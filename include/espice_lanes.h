@@ -360,6 +360,25 @@ esl_status esl_last_timing(esl_engine *e, esl_timing *out);
 esl_status esl_debug_profile(esl_engine *e, int enable, unsigned long long *out,
                              size_t capacity);
 
+/* Development hook, not part of the drop-in: the refactor and solves of the
+ * kernel for lanes beyond a workgroup's LDS (the replay cut into blocks of
+ * at most block_slots slots, the solves in column chunks), run on the CPU
+ * for `lanes` matrices a [lanes][nnz] and right-hand sides rhs [lanes][n]
+ * on the tape factored from lead_vals with column order q. Writes per lane
+ * L [lanes][nnz_l], U [lanes][nnz_u], the pivots [lanes][n], dx [lanes][n]
+ * (row order), fail (1: the lane must re-pivot) and sizes[3] = {blocks,
+ * nnz_l, nnz_u} (lanes == 0: only sizes). Needs no device; lets the block
+ * programs be checked against the serial replay (lane_kernels.zig:49-143)
+ * on any host. */
+esl_status esl_debug_big_replay(uint32_t n, const uint32_t *col_ptr,
+                                const uint32_t *row_idx,
+                                const double *lead_vals, const uint32_t *q,
+                                double pivot_tol, uint32_t block_slots,
+                                const double *a, const double *rhs,
+                                uint32_t lanes, double *lx, double *ux,
+                                double *udiag, double *dx, uint32_t *fail,
+                                uint32_t *sizes);
+
 /* ---- The LU boundary alone (arp_lane_refactor + arp_lane_solve,
  * src/solver/lane_device.zig:26-38; bodies lane_kernels.zig:49-143). ----
  * A tape is built on the host from one matrix (the lead lane) by the
