@@ -41,6 +41,22 @@ __device__ inline void wsync()
     __builtin_amdgcn_wave_barrier();
 }
 
+// Explicit address spaces. A pointer the compiler cannot trace to LDS or to
+// a kernel argument (one read from memory, or passed through a struct) is
+// accessed with flat instructions, which count on both the LDS and the
+// memory counters: an LDS read then waits for every global load in flight.
+// These casts keep LDS accesses on ds_* and HBM accesses on global_*.
+template <typename T>
+__device__ inline __attribute__((address_space(1))) T *gptr(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+template <typename T>
+__device__ inline __attribute__((address_space(3))) T *lptr(T *p)
+{
+    return (__attribute__((address_space(3))) T *)p;
+}
+
 // A lane-major array seen as one lane's vector: element i at p[i * s].
 struct Strided {
     double *p;
@@ -233,8 +249,8 @@ struct LuFlags {
 // order within the wavefront (LDS accesses of one wavefront complete in issue
 // order), so a row sees the results of every earlier row. The peel tests are
 // computed in FP64 on the values.
-template <typename R>
-__device__ inline void run_row(esl::uint2_t w, R *M, double growth, LuFlags &f)
+template <typename R, typename MP>
+__device__ inline void run_row(esl::uint2_t w, MP M, double growth, LuFlags &f)
 {
     const uint32_t kind = __builtin_amdgcn_readfirstlane(esl::row_kind(w.y));
     const uint32_t d = w.x & 0xffffu, b = w.x >> 16, c = w.y & 0xffffu;
@@ -259,20 +275,29 @@ __device__ inline void run_row(esl::uint2_t w, R *M, double growth, LuFlags &f)
     }
 }
 
-// Runs a row program with kRing rows of words in flight.
-template <typename R>
-__device__ void run_rows(const esl::uint2_t *words, uint32_t rows, R *M, double growth, LuFlags &f, uint32_t tid)
+// Runs a row program with kRing rows of words in flight. M (R elements)
+// and the words may be address-space qualified pointers. The word loads are
+// pinned in row order (scheduling barriers) and each refills the register
+// its row just used, so the oldest load completes first and a row waits only
+// for its own word, never for the whole ring.
+template <typename R, typename WP, typename MP>
+__device__ void run_rows(WP words, uint32_t rows, MP M, double growth, LuFlags &f, uint32_t tid)
 {
-    const esl::uint2_t *w = words + tid;
+    const WP w = words + tid;
     esl::uint2_t q[kRing];
 #pragma unroll
-    for (int k = 0; k < kRing; k++) q[k] = w[k * kWave];
+    for (int k = 0; k < kRing; k++) {
+        q[k] = esl::uint2_t{w[k * kWave].x, w[k * kWave].y};
+        __builtin_amdgcn_sched_barrier(0);
+    }
     for (uint32_t row = 0; row < rows; row += kRing) {
 #pragma unroll
         for (int k = 0; k < kRing; k++) {
-            const esl::uint2_t cur = q[k];
-            q[k] = w[(row + kRing + k) * kWave];
-            run_row<R>(cur, M, growth, f);
+            // use the word, then refill its register (no register rotation)
+            run_row<R>(q[k], M, growth, f);
+            const size_t at = (size_t)(row + kRing + k) * kWave;
+            q[k] = esl::uint2_t{w[at].x, w[at].y};
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
 }

@@ -1995,7 +1995,7 @@ esl_status esl_debug_profile(esl_engine *h, int enable, unsigned long long *out,
 // `lanes` matrices a [lanes][nnz] and right-hand sides rhs [lanes][n] on
 // the tape factored from lead_vals. Outputs per lane: L, U, pivots in the
 // tape's order, dx (row order) and the re-pivot flag; sizes: the block
-// count, nnz_l and nnz_u (lanes == 0: only these).
+// count, nnz_l, nnz_u and the rows of all blocks (lanes == 0: only these).
 esl_status esl_debug_big_replay(uint32_t n, const uint32_t *col_ptr, const uint32_t *row_idx,
                                 const double *lead_vals, const uint32_t *q, double pivot_tol, uint32_t block_slots,
                                 const double *a, const double *rhs, uint32_t lanes, double *lx, double *ux,
@@ -2016,6 +2016,7 @@ esl_status esl_debug_big_replay(uint32_t n, const uint32_t *col_ptr, const uint3
         sizes[0] = T.n_blk;
         sizes[1] = T.n_l;
         sizes[2] = T.n_u;
+        sizes[3] = (u32)(o.blk_words.size() / esl::kWave);
     }
     const u32 nnz = cp[n];
     for (u32 l = 0; l < lanes; l++) {
@@ -2844,6 +2845,12 @@ extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t
     ta.dt = o.dt;
     ta.steps = o.steps;
     ta.n_probes = o.n_probes;
+    a.prof = nullptr;
+    if (e->profiling) {
+        HIPCHK(e->prof.ensure(count * esl::kProfPhases * 8));
+        HIPCHK(hipMemsetAsync(e->prof.p, 0, count * esl::kProfPhases * 8, s));
+        a.prof = e->prof.as<unsigned long long>();
+    }
     if (!e->big) HIPCHK(priv_setup(*e, count, e->lead_nl, e->lead_nu, s, a.priv));
 
     // 3. the transient lead tape: the first converged point's first
@@ -2985,6 +2992,10 @@ extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t
         default: reasons[l] = ESL_REASON_OTHER; break;
         }
         lane_it += iterations[l];
+    }
+    if (e->profiling) {
+        e->prof_host.assign(count * esl::kProfPhases, 0);
+        HIPCHK(hipMemcpy(e->prof_host.data(), e->prof.p, count * esl::kProfPhases * 8, hipMemcpyDeviceToHost));
     }
     e->timing.kernel_ms = op_ms + kms;
     e->timing.lane_iterations = lane_it;

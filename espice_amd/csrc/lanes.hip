@@ -165,17 +165,23 @@ __device__ void assemble(const DeckProg &dp, const TapeProg &T, const double *li
         for (uint32_t c = c0; c < c1; c++) {
             if (dp.chunks[c].kind == kC) continue;   // no static stamps
             const double *stc = st + (c - c0) * kStageDoubles;
-            const uint32_t *rs = T.red + T.red_off[c] + tid;
+            // the stream through a global pointer: its ring of loads must
+            // not hold up the LDS waits of the reduce (flat loads would)
+            const __attribute__((address_space(1))) uint32_t *rs = gptr(T.red) + T.red_off[c] + tid;
             const uint32_t len = T.red_len[c];
             RedAcc r{-1, false, 0.0};
             uint32_t q[kRing];
 #pragma unroll
-            for (int k = 0; k < kRing; k++) q[k] = rs[k * kWave];
+            for (int k = 0; k < kRing; k++) {
+                q[k] = rs[k * kWave];
+                __builtin_amdgcn_sched_barrier(0);
+            }
             for (uint32_t w = 0; w < len; w += kRing) {
 #pragma unroll
                 for (int k = 0; k < kRing; k++) {
                     red_word(q[k], r, tv, rhs, stc);
                     q[k] = rs[(w + kRing + k) * kWave];
+                    __builtin_amdgcn_sched_barrier(0);
                 }
             }
             red_flush(r, tv, rhs);
@@ -199,7 +205,7 @@ __device__ bool refactor(const TapeProg &T, uint32_t n, R *tv, uint32_t tid)
     for (uint32_t v = tid; v < T.n_void; v += kWave) f.bad |= tv[T.amap[T.void_slots[v]]] != R(0);
     for (uint32_t k = tid; k < n; k += kWave)
         if (T.flags[k] & kVoidBit) tv[T.n_u + k] = R(1);
-    run_rows<R>(T.rw, T.rw_rows, tv, T.growth, f, tid);
+    run_rows<R>(gptr(T.rw), T.rw_rows, tv, T.growth, f, tid);
     for (uint32_t p = tid; p < T.n_u; p += kWave) {
         const R u = tv[p];
         f.nonfinite |= dbits((double)(u - u));
@@ -221,7 +227,7 @@ __device__ void solve_neg(const TapeProg &T, uint32_t n, double *tv, const doubl
     for (uint32_t r = tid; r < n; r += kWave) y[T.pinv[r]] = -rhs[r];
     wsync();
     LuFlags f{false, 0, 0};
-    run_rows<double>(T.sw, T.sw_rows, tv, 0.0, f, tid);
+    run_rows<double>(gptr(T.sw), T.sw_rows, tv, 0.0, f, tid);
 }
 
 // ---- FP32 factors + FP64 refinement of one Newton solve: dx = -A^-1 F.
@@ -249,7 +255,7 @@ __device__ bool refine_solve(const OpArgs &a, const TapeProg &T, uint32_t n, flo
         for (uint32_t i = tid; i < n; i += kWave) y32[T.pinv[i]] = (float)r[i];
         wsync();
         LuFlags f{false, 0, 0};
-        run_rows<float>(T.sw, T.sw_rows, tv32, 0.0, f, tid);
+        run_rows<float>(gptr(T.sw), T.sw_rows, tv32, 0.0, f, tid);
         wsync();
         for (uint32_t i = tid; i < n; i += kWave) dx[i] = dx[i] + (double)y32[T.qinv[i]];
         wsync();

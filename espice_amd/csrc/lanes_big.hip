@@ -25,24 +25,33 @@
 
 namespace esl {
 
-// The lane's arrays in HBM and its LDS area.
+// Pointers with their address space explicit (gptr/lptr): global for the
+// lane's HBM arrays and the tape, LDS for the staging area.
+typedef __attribute__((address_space(1))) double gdbl;
+typedef __attribute__((address_space(1))) const double gcdbl;
+typedef __attribute__((address_space(1))) const uint32_t gcu32;
+typedef __attribute__((address_space(3))) double ldbl;
+typedef __attribute__((address_space(3))) uint32_t lu32;
+
+// The lane's arrays in HBM and its LDS area (the kernel's dynamic LDS,
+// passed down so that no function looks its address up).
 struct BigLane {
-    double *tv, *rhs, *x, *xold, *sdiag, *q, *cq;
-    double *lds;
+    gdbl *tv, *rhs, *x, *xold, *sdiag, *q, *cq;
+    ldbl *sm;
 };
 
-__device__ inline BigLane big_lane(const BigArgs &b, uint32_t g, double *lds)
+__device__ inline BigLane big_lane(const BigArgs &b, uint32_t g, ldbl *sm)
 {
     const uint32_t n = b.ta.op.dp.n;
     BigLane L;
-    L.tv = b.ws + (size_t)g * b.ws_stride;
+    L.tv = gptr(b.ws) + (size_t)g * b.ws_stride;
     L.rhs = L.tv + b.tv_cap;
     L.x = L.rhs + n;
     L.xold = L.x + n;
     L.sdiag = L.xold + n;
     L.q = L.sdiag + n;
     L.cq = L.q + n;
-    L.lds = lds;
+    L.sm = sm;
     return L;
 }
 
@@ -51,13 +60,17 @@ __device__ inline BigLane big_lane(const BigArgs &b, uint32_t g, double *lds)
 // charges too (row charges into q, charge-Jacobian cells into cq), then the
 // ground pin and the companion A = G + ag0 C, F += ag0 (q - q1) - ag1 i_prev
 // (assemble_tran's arithmetic); otherwise capacitors and charge targets are
-// skipped and only the ground pin follows (assemble_pattern's).
-__device__ void big_assemble(const DeckProg &dp, const uint32_t *amap, uint32_t n_slots, const double *live,
-                             const double *x, double *lim, double *M, const BigLane &L, bool tran, double t,
-                             double ag0, double ag1, const double *q1, const double *ip, uint32_t tid)
+// skipped and only the ground pin follows (assemble_pattern's). The stamp
+// cells of each chunk are staged in LDS.
+__device__ void big_assemble(const DeckProg &dp, const uint32_t *amap_, uint32_t n_slots, const double *live,
+                             gdbl *x, double *lim, gdbl *M, const BigLane &L, bool tran, double t, double ag0,
+                             double ag1, gcdbl *q1, gcdbl *ip, uint32_t tid)
 {
     const uint32_t n = dp.n;
-    double *st = L.lds;
+    ldbl *st = L.sm;
+    gcu32 *amap = amap_ ? gptr(amap_) : nullptr;
+    gcu32 *task_target = gptr(dp.task_target), *task_cell = gptr(dp.task_cell);
+    const __attribute__((address_space(1))) uint16_t *cells = gptr(dp.cells);
     for (uint32_t i = tid; i < n_slots; i += kWave) M[i] = 0.0;
     for (uint32_t i = tid; i < n; i += kWave) {
         L.rhs[i] = 0.0;
@@ -73,9 +86,9 @@ __device__ void big_assemble(const DeckProg &dp, const uint32_t *amap, uint32_t 
         wsync();
         const uint32_t t1 = dp.chunks[c + 1].task_begin;
         for (uint32_t k = ch.task_begin + tid; k < t1; k += kWave) {
-            const uint32_t tgt = dp.task_target[k];
+            const uint32_t tgt = task_target[k];
             const uint32_t idx = tgt & kTargetMask;
-            double *d;
+            gdbl *d;
             if (tgt & kRhsBit) {
                 d = &L.rhs[idx];
             } else if (tgt & (kCBit | kQBit)) {
@@ -85,8 +98,8 @@ __device__ void big_assemble(const DeckProg &dp, const uint32_t *amap, uint32_t 
                 d = &M[amap ? amap[tgt] : tgt];
             }
             double acc = *d;
-            const uint32_t cc1 = dp.task_cell[k + 1];
-            for (uint32_t cc = dp.task_cell[k]; cc < cc1; cc++) acc = acc + st[dp.cells[cc]];
+            const uint32_t cc1 = task_cell[k + 1];
+            for (uint32_t cc = task_cell[k]; cc < cc1; cc++) acc = acc + st[cells[cc]];
             *d = acc;
         }
         wsync();
@@ -98,8 +111,9 @@ __device__ void big_assemble(const DeckProg &dp, const uint32_t *amap, uint32_t 
     }
     wsync();
     if (!tran) return;
+    gcu32 *cslot_pat = gptr(dp.cslot_pat);
     for (uint32_t k = tid; k < dp.n_cslot; k += kWave) {
-        const uint32_t sl = amap ? amap[dp.cslot_pat[k]] : dp.cslot_pat[k];
+        const uint32_t sl = amap ? amap[cslot_pat[k]] : cslot_pat[k];
         M[sl] = M[sl] + ag0 * L.cq[k];
     }
     for (uint32_t i = tid; i < n; i += kWave) {
@@ -110,24 +124,98 @@ __device__ void big_assemble(const DeckProg &dp, const uint32_t *amap, uint32_t 
     wsync();
 }
 
-// The replay on the lane's slots in HBM: void pivots set to 1, the blocks
-// (gather, rows, scatter), then the peel tests of refactor(). True when the
-// lane must re-pivot.
-__device__ bool big_refactor(const TapeProg &T, uint32_t n, double *tv, double *M, uint32_t tid)
+// Batched moves between HBM and LDS: each thread issues kBatch independent
+// loads before it uses any, so a wavefront has up to 16 x 64 loads in
+// flight instead of waiting on each.
+constexpr uint32_t kBatch = 16;
+
+// dst[j] = src[idx[j]] for j < count (LDS from HBM through an index list).
+// Pipelined: the next batch's indices load while this batch's values do, so
+// a batch costs one memory latency, not two.
+__device__ inline void gather(ldbl *dst, gcdbl *src, gcu32 *idx, uint32_t count, uint32_t tid)
 {
+    uint32_t id[kBatch];
+#pragma unroll
+    for (uint32_t u = 0; u < kBatch; u++) {
+        const uint32_t j = u * kWave + tid;
+        id[u] = idx[j < count ? j : 0];
+    }
+    for (uint32_t j0 = 0; j0 < count; j0 += kBatch * kWave) {
+        double v[kBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; u++) v[u] = src[id[u]];
+        const uint32_t j1 = j0 + kBatch * kWave;
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; u++) {
+            const uint32_t j = j1 + u * kWave + tid;
+            id[u] = idx[j < count ? j : 0];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; u++) {
+            const uint32_t j = j0 + u * kWave + tid;
+            if (j < count) dst[j] = v[u];
+        }
+    }
+}
+
+// dst[idx[j]] = src[j] for j < count (HBM from LDS through an index list).
+__device__ inline void scatter(gdbl *dst, gcu32 *idx, const ldbl *src, uint32_t count, uint32_t tid)
+{
+    for (uint32_t j0 = 0; j0 < count; j0 += kBatch * kWave) {
+        uint32_t id[kBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; u++) {
+            const uint32_t j = j0 + u * kWave + tid;
+            id[u] = idx[j < count ? j : 0];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; u++) {
+            const uint32_t j = j0 + u * kWave + tid;
+            if (j < count) dst[id[u]] = src[j];
+        }
+    }
+}
+
+// dst[j] = src[j] for j < count (LDS from a contiguous HBM range, doubles
+// or 32-bit indices), with the same batching.
+template <typename D, typename S>
+__device__ inline void stage(D *dst, S *src, uint32_t count, uint32_t tid)
+{
+    for (uint32_t j0 = 0; j0 < count; j0 += kBatch * kWave) {
+        decltype(+src[0]) v[kBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; u++) {
+            const uint32_t j = j0 + u * kWave + tid;
+            v[u] = src[j < count ? j : 0];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; u++) {
+            const uint32_t j = j0 + u * kWave + tid;
+            if (j < count) dst[j] = v[u];
+        }
+    }
+}
+
+// The replay on the lane's slots in HBM: void pivots set to 1, the blocks
+// (gather into LDS, rows, scatter), then the peel tests of refactor(). True
+// when the lane must re-pivot.
+__device__ bool big_refactor(const TapeProg &T, uint32_t n, gdbl *tv, ldbl *M, uint32_t tid)
+{
+    gcu32 *flags = gptr(T.flags), *amap = gptr(T.amap), *void_slots = gptr(T.void_slots);
     LuFlags f{false, 0, 0};
-    for (uint32_t v = tid; v < T.n_void; v += kWave) f.bad |= tv[T.amap[T.void_slots[v]]] != 0.0;
+    for (uint32_t v = tid; v < T.n_void; v += kWave) f.bad |= tv[amap[void_slots[v]]] != 0.0;
     for (uint32_t k = tid; k < n; k += kWave)
-        if (T.flags[k] & kVoidBit) tv[T.n_u + k] = 1.0;
+        if (flags[k] & kVoidBit) tv[T.n_u + k] = 1.0;
     wsync();
+    const __attribute__((address_space(1))) BigBlock *blk = gptr(T.blk);
     for (uint32_t b = 0; b < T.n_blk; b++) {
-        const BigBlock B = T.blk[b];
-        const uint32_t *sl = T.blk_slots + B.slot_off;
-        for (uint32_t j = tid; j < B.m; j += kWave) M[j] = tv[sl[j]];
+        const uint32_t m = blk[b].m, w = blk[b].w, rows = blk[b].rows, row0 = blk[b].row0;
+        gcu32 *sl = gptr(T.blk_slots) + blk[b].slot_off;
+        gather(M, tv, sl, m, tid);
         wsync();
-        run_rows<double>(T.blk_words + (size_t)B.row0 * kWave, B.rows, M, T.growth, f, tid);
+        run_rows<double>(gptr(T.blk_words) + (size_t)row0 * kWave, rows, M, T.growth, f, tid);
         wsync();
-        for (uint32_t j = tid; j < B.w; j += kWave) tv[sl[j]] = M[j];
+        scatter(tv, sl, M, w, tid);
         wsync();
     }
     for (uint32_t p = tid; p < T.n_u; p += kWave) {
@@ -135,7 +223,7 @@ __device__ bool big_refactor(const TapeProg &T, uint32_t n, double *tv, double *
         f.nonfinite |= dbits(u - u);
     }
     for (uint32_t k = tid; k < n; k += kWave) {
-        if (T.flags[k] & kVoidBit) continue;
+        if (flags[k] & kVoidBit) continue;
         const double d = tv[T.n_u + k];
         f.bad |= d == 0.0 || !(fabs(d) < __builtin_inf());
     }
@@ -150,25 +238,24 @@ __device__ bool big_refactor(const TapeProg &T, uint32_t n, double *tv, double *
 // entries, rows and pivots are staged (lane_kernels.zig:123-143). Within a
 // column the updated rows are distinct; columns run in order (LDS accesses
 // of one wavefront complete in issue order).
-__device__ void big_solve(const TapeProg &T, uint32_t n, const double *tv, const double *rhs, double *y,
-                          uint32_t tid)
+__device__ void big_solve(const TapeProg &T, uint32_t n, gcdbl *tv, gcdbl *rhs, ldbl *y, uint32_t tid)
 {
-    double *sv = y + n;
-    uint32_t *si = reinterpret_cast<uint32_t *>(sv + T.sol_cap);
-    for (uint32_t r = tid; r < n; r += kWave) y[T.pinv[r]] = -rhs[r];
+    ldbl *sv = y + n;
+    lu32 *si = (lu32 *)(sv + T.sol_cap);
+    gcu32 *pinv = gptr(T.pinv), *lp = gptr(T.lp), *li = gptr(T.li), *up = gptr(T.up), *ui = gptr(T.ui);
+    gcu32 *fch = gptr(T.fch), *bch = gptr(T.bch);
+    for (uint32_t r = tid; r < n; r += kWave) y[pinv[r]] = -rhs[r];
     wsync();
     for (uint32_t c = 0; c < T.n_fch; c++) {
-        const uint32_t k0 = T.fch[c], k1 = T.fch[c + 1];
-        const uint32_t e0 = T.lp[k0], e1 = T.lp[k1];
-        for (uint32_t e = tid; e < e1 - e0; e += kWave) {
-            sv[e] = tv[T.lx0 + e0 + e];
-            si[e] = T.li[e0 + e];
-        }
+        const uint32_t k0 = fch[c], k1 = fch[c + 1];
+        const uint32_t e0 = lp[k0], e1 = lp[k1];
+        stage(sv, tv + T.lx0 + e0, e1 - e0, tid);
+        stage(si, li + e0, e1 - e0, tid);
         wsync();
         for (uint32_t k = k0; k < k1; k++) {
             const double yk = y[k];
-            const uint32_t q1 = T.lp[k + 1] - e0;
-            for (uint32_t q = T.lp[k] - e0 + tid; q < q1; q += kWave) {
+            const uint32_t q1 = lp[k + 1] - e0;
+            for (uint32_t q = lp[k] - e0 + tid; q < q1; q += kWave) {
                 const uint32_t r = si[q];
                 const double old = y[r];
                 const double m = sv[q] * yk;
@@ -180,20 +267,18 @@ __device__ void big_solve(const TapeProg &T, uint32_t n, const double *tv, const
     }
     for (uint32_t c = T.n_bch; c > 0;) {
         c--;
-        const uint32_t k0 = T.bch[c], k1 = T.bch[c + 1];
-        const uint32_t e0 = T.up[k0], e1 = T.up[k1], ne = e1 - e0;
-        for (uint32_t e = tid; e < ne; e += kWave) {
-            sv[e] = tv[e0 + e];
-            si[e] = T.ui[e0 + e];
-        }
-        for (uint32_t k = k0 + tid; k < k1; k += kWave) sv[ne + (k - k0)] = tv[T.n_u + k];
+        const uint32_t k0 = bch[c], k1 = bch[c + 1];
+        const uint32_t e0 = up[k0], e1 = up[k1], ne = e1 - e0;
+        stage(sv, tv + e0, ne, tid);
+        stage(si, ui + e0, ne, tid);
+        stage(sv + ne, tv + T.n_u + k0, k1 - k0, tid);
         wsync();
         for (uint32_t k = k1; k > k0;) {
             k--;
             const double zk = y[k] / sv[ne + (k - k0)];
             if (tid == 0) y[k] = zk;
-            const uint32_t p1 = T.up[k + 1] - e0;
-            for (uint32_t p = T.up[k] - e0 + tid; p < p1; p += kWave) {
+            const uint32_t p1 = up[k + 1] - e0;
+            for (uint32_t p = up[k] - e0 + tid; p < p1; p += kWave) {
                 const uint32_t i = si[p];
                 const double old = y[i];
                 const double m = sv[p] * zk;
@@ -206,13 +291,13 @@ __device__ void big_solve(const TapeProg &T, uint32_t n, const double *tv, const
 }
 
 // The plausible limit of a Newton run's first iterate (converger.zig:377-384).
-__device__ inline double big_limit_at(const TapeProg &T, uint32_t n, const double *tv, const double *x,
-                                      const double *rhs, uint32_t tid)
+__device__ inline double big_limit_at(const TapeProg &T, uint32_t n, gcdbl *tv, gcdbl *x, gcdbl *rhs, uint32_t tid)
 {
+    gcu32 *diag_tv = gptr(T.diag_tv);
     double gm = 1.0;
     uint64_t xm = 0, fm = 0;
     for (uint32_t i = tid; i < n; i += kWave) {
-        gm = fmax(gm, fabs(tv[T.diag_tv[i]]));
+        gm = fmax(gm, fabs(tv[diag_tv[i]]));
         const uint64_t xb = dbits(fabs(x[i])), fb = dbits(fabs(rhs[i]));
         xm = xb > xm ? xb : xm;
         fm = fb > fm ? fb : fm;
@@ -226,18 +311,20 @@ __device__ inline double big_limit_at(const TapeProg &T, uint32_t n, const doubl
 
 // x += dx (dx = y[qinv[i]]), x_old kept; returns the largest scaled update
 // (converger.zig:794-819).
-__device__ inline double big_update(const OpArgs &a, const TapeProg &T, const BigLane &L, const double *y,
-                                    uint32_t tid)
+__device__ inline double big_update(const OpArgs &a, const TapeProg &T, const BigLane &L, uint32_t tid)
 {
     const DeckProg &dp = a.dp;
+    const ldbl *y = L.sm;
+    gcu32 *qinv = gptr(T.qinv);
+    const __attribute__((address_space(1))) uint8_t *current_row = gptr(dp.current_row);
     uint64_t worst = 0;
     for (uint32_t i = tid; i < dp.n; i += kWave) {
         const double xo = L.x[i];
-        const double d = y[T.qinv[i]];
+        const double d = y[qinv[i]];
         const double xn = xo + d;
         L.xold[i] = xo;
         L.x[i] = xn;
-        const double tl = a.reltol * fmax(fabs(xn), fabs(xo)) + (dp.current_row[i] ? a.abstol : a.vntol);
+        const double tl = a.reltol * fmax(fabs(xn), fabs(xo)) + (current_row[i] ? a.abstol : a.vntol);
         const bool finite = fabs(xn) < __builtin_inf() && fabs(d) < __builtin_inf();
         const double qq = fabs(d) / tl;
         const double score = !finite ? __builtin_inf() : (qq == qq ? qq : 0.0);
@@ -248,7 +335,7 @@ __device__ inline double big_update(const OpArgs &a, const TapeProg &T, const Bi
 }
 
 // Limiting after an update (MOS1, diodes, BJTs): true when any limited.
-__device__ inline bool big_limit(const DeckProg &dp, const double *live, const double *x, double *lim, uint32_t tid)
+__device__ __forceinline__ bool big_limit(const DeckProg &dp, const double *live, gcdbl *x, double *lim, uint32_t tid)
 {
     const uint32_t nm = dp.n_mos;
     bool limited = false;
@@ -273,20 +360,22 @@ __device__ inline bool big_limit(const DeckProg &dp, const double *live, const d
 // One lane's operating point (op_lane's FP64 path: cold, given or resumed
 // start; dump: the Jacobian and residual at the start or saved iterate into
 // row g of a_out and f_out).
-__device__ void big_op_lane(const BigArgs &b, uint32_t g, uint32_t lane, double *lds)
+__device__ __forceinline__ void big_op_lane(const BigArgs &b, uint32_t g, uint32_t lane, ldbl *sm)
 {
     const OpArgs &a = b.ta.op;
     const uint32_t tid = threadIdx.x;
     const DeckProg &dp = a.dp;
     const uint32_t n = dp.n;
-    const BigLane L = big_lane(b, g, lds);
+    const BigLane L = big_lane(b, g, sm);
     const double *live = a.live + (size_t)lane * dp.n_live;
     double *lim = a.lim + (size_t)lane * lim_width(dp);
-    lane_init(a, lane, live, L.x, lim, tid);
+    gdbl *xg = gptr(a.x) + (size_t)lane * n;
+    lane_init(a, lane, live, (double *)L.x, lim, tid);
     if (a.dump) {
-        big_assemble(dp, nullptr, dp.nnz, live, L.x, lim, a.a_out + (size_t)g * dp.nnz, L, false, 0.0, 0.0, 0.0,
-                     nullptr, nullptr, tid);
-        for (uint32_t i = tid; i < n; i += kWave) a.f_out[(size_t)g * n + i] = L.rhs[i];
+        big_assemble(dp, nullptr, dp.nnz, live, L.x, lim, gptr(a.a_out) + (size_t)g * dp.nnz, L, false, 0.0, 0.0,
+                     0.0, nullptr, nullptr, tid);
+        gdbl *f = gptr(a.f_out) + (size_t)g * n;
+        for (uint32_t i = tid; i < n; i += kWave) f[i] = L.rhs[i];
         return;
     }
     const LaneResume rs = a.resume[lane];
@@ -301,14 +390,27 @@ __device__ void big_op_lane(const BigArgs &b, uint32_t g, uint32_t lane, double 
         k0 = rs.k0;
     }
     uint32_t status = kNoConvergence, iters = a.max_iter;
+    unsigned long long pc[kProfPhases] = {0, 0, 0, 0};
+    unsigned long long tc = clock64();
+    auto lap = [&](int ph) {
+        if (a.prof) {
+            const unsigned long long now = clock64();
+            pc[ph] += now - tc;
+            tc = now;
+        }
+    };
     uint32_t iter = iter0;
     for (; iter - k0 < a.max_iter; iter++) {
+        lap(3);
         big_assemble(dp, T.amap, T.n_tv, live, L.x, lim, L.tv, L, false, 0.0, 0.0, 0.0, nullptr, nullptr, tid);
         if (iter == k0) limit = big_limit_at(T, n, L.tv, L.x, L.rhs, tid);
         for (uint32_t i = tid; i < n; i += kWave) L.sdiag[i] = L.tv[T.diag_tv[i]];
         wsync();
-        if (big_refactor(T, n, L.tv, L.lds, tid)) {
-            for (uint32_t i = tid; i < n; i += kWave) a.x[(size_t)lane * n + i] = L.x[i];
+        lap(0);
+        const bool repivot = big_refactor(T, n, L.tv, L.sm, tid);
+        lap(1);
+        if (repivot) {
+            for (uint32_t i = tid; i < n; i += kWave) xg[i] = L.x[i];
             if (tid == 0) {
                 LaneResume o{};
                 o.iter = iter;
@@ -322,16 +424,17 @@ __device__ void big_op_lane(const BigArgs &b, uint32_t g, uint32_t lane, double 
                 // the Jacobian at this iterate (pattern order) for the host's
                 // re-pivot: the same sums as the failed assembly
                 wsync();
-                big_assemble(dp, nullptr, dp.nnz, live, L.x, lim, a.a_out + (size_t)lane * dp.nnz, L, false, 0.0,
-                             0.0, 0.0, nullptr, nullptr, tid);
+                big_assemble(dp, nullptr, dp.nnz, live, L.x, lim, gptr(a.a_out) + (size_t)lane * dp.nnz, L, false,
+                             0.0, 0.0, 0.0, nullptr, nullptr, tid);
             }
             status = kRepivot;
             iters = iter;
             break;
         }
-        big_solve(T, n, L.tv, L.rhs, L.lds, tid);
+        big_solve(T, n, L.tv, L.rhs, L.sm, tid);
         wsync();
-        const double scaled = big_update(a, T, L, L.lds, tid);
+        lap(2);
+        const double scaled = big_update(a, T, L, tid);
         wsync();
         const bool limited = big_limit(dp, live, L.x, lim, tid);
         bool ok = !limited && iter != k0 && scaled < 1.0;
@@ -360,10 +463,13 @@ __device__ void big_op_lane(const BigArgs &b, uint32_t g, uint32_t lane, double 
     if (status == kNoConvergence) iters = iter;
     wsync();
     if (status != kRepivot)
-        for (uint32_t i = tid; i < n; i += kWave) a.x[(size_t)lane * n + i] = L.x[i];
+        for (uint32_t i = tid; i < n; i += kWave) xg[i] = L.x[i];
+    lap(3);
     if (tid == 0) {
         a.status[lane] = status;
         a.iterations[lane] = iters;
+        if (a.prof)
+            for (int p = 0; p < kProfPhases; p++) a.prof[(size_t)lane * kProfPhases + p] += pc[p];
     }
 }
 
@@ -371,7 +477,7 @@ __device__ void big_op_lane(const BigArgs &b, uint32_t g, uint32_t lane, double 
 // Newton with the transient gates, acceptance, probes and predictor; a lane
 // whose replay fails leaves its Jacobian in its row of a_out and resumes at
 // the same iterate; dump: the first step's first Jacobian into that row).
-__device__ void big_tran_lane(const BigArgs &b, uint32_t g, uint32_t lane, double *lds)
+__device__ __forceinline__ void big_tran_lane(const BigArgs &b, uint32_t g, uint32_t lane, ldbl *sm)
 {
     const TranArgs &ta = b.ta;
     const OpArgs &a = ta.op;
@@ -382,21 +488,23 @@ __device__ void big_tran_lane(const BigArgs &b, uint32_t g, uint32_t lane, doubl
     if (s < ta.s0) s = ta.s0;
     const DeckProg &dp = a.dp;
     const uint32_t n = dp.n;
-    const BigLane L = big_lane(b, g, lds);
+    const BigLane L = big_lane(b, g, sm);
     const double *live = a.live + (size_t)lane * dp.n_live;
     double *lim = a.lim + (size_t)lane * lim_width(dp);
-    double *q1 = ta.q1 + (size_t)lane * n;
-    double *ip = ta.ip + (size_t)lane * n;
-    double *xp = ta.xp + (size_t)lane * n;
+    gdbl *q1 = gptr(ta.q1) + (size_t)lane * n;
+    gdbl *ip = gptr(ta.ip) + (size_t)lane * n;
+    gdbl *xp = gptr(ta.xp) + (size_t)lane * n;
+    gdbl *xg = gptr(a.x) + (size_t)lane * n;
     const LaneResume rs = a.resume[lane];
     const TapeProg T = a.tapes[rs.tape];
-    double *aout = a.a_out + (size_t)lane * dp.nnz;
-    for (uint32_t i = tid; i < n; i += kWave) L.x[i] = a.x[(size_t)lane * n + i];
+    gdbl *aout = gptr(a.a_out) + (size_t)lane * dp.nnz;
+    gcu32 *amap = gptr(T.amap);
+    for (uint32_t i = tid; i < n; i += kWave) L.x[i] = xg[i];
     wsync();
     if (ta.init) {
         // start at the operating point: limiting state at its own voltages,
         // charges from one evaluation there, no dynamic current
-        seed_lim_at(dp, live, L.x, lim, tid);
+        seed_lim_at(dp, live, (const double *)L.x, lim, tid);
         for (uint32_t i = tid; i < n; i += kWave) {
             q1[i] = 0.0;
             ip[i] = 0.0;
@@ -416,6 +524,15 @@ __device__ void big_tran_lane(const BigArgs &b, uint32_t g, uint32_t lane, doubl
     double limit = rs.limit;
     uint32_t status = kTranRunning;
     uint32_t total_it = a.iterations[lane];
+    unsigned long long pc[kProfPhases] = {0, 0, 0, 0};
+    unsigned long long tc = clock64();
+    auto lap = [&](int ph) {
+        if (a.prof) {
+            const unsigned long long now = clock64();
+            pc[ph] += now - tc;
+            tc = now;
+        }
+    };
     for (; s < ta.s1 && status == kTranRunning; s++) {
         const double t = (double)(s + 1) * ta.dt;
         const double ag0 = s == 0 ? 1.0 / ta.dt : 2.0 / ta.dt;
@@ -423,15 +540,19 @@ __device__ void big_tran_lane(const BigArgs &b, uint32_t g, uint32_t lane, doubl
         bool converged = false;
         uint32_t iter = iter0;
         for (; iter < a.max_iter; iter++) {
+            lap(3);
             big_assemble(dp, T.amap, T.n_tv, live, L.x, lim, L.tv, L, true, t, ag0, ag1, q1, ip, tid);
             if (iter == 0) limit = big_limit_at(T, n, L.tv, L.x, L.rhs, tid);
             wsync();
-            if (big_refactor(T, n, L.tv, L.lds, tid)) {
+            lap(0);
+            const bool repivot = big_refactor(T, n, L.tv, L.sm, tid);
+            lap(1);
+            if (repivot) {
                 // the Jacobian again, read out in pattern order for the
                 // host's re-pivot; resume at this iterate
                 wsync();
                 big_assemble(dp, T.amap, T.n_tv, live, L.x, lim, L.tv, L, true, t, ag0, ag1, q1, ip, tid);
-                for (uint32_t p = tid; p < dp.nnz; p += kWave) aout[p] = L.tv[T.amap[p]];
+                for (uint32_t p = tid; p < dp.nnz; p += kWave) aout[p] = L.tv[amap[p]];
                 if (tid == 0) {
                     LaneResume o = rs;
                     o.iter = iter;
@@ -442,9 +563,10 @@ __device__ void big_tran_lane(const BigArgs &b, uint32_t g, uint32_t lane, doubl
                 status = kRepivot;
                 break;
             }
-            big_solve(T, n, L.tv, L.rhs, L.lds, tid);
+            big_solve(T, n, L.tv, L.rhs, L.sm, tid);
             wsync();
-            const double scaled = big_update(a, T, L, L.lds, tid);
+            lap(2);
+            const double scaled = big_update(a, T, L, tid);
             wsync();
             const bool limited = big_limit(dp, live, L.x, lim, tid);
             bool ok = !limited && iter != 0 && scaled < 1.0;
@@ -482,7 +604,7 @@ __device__ void big_tran_lane(const BigArgs &b, uint32_t g, uint32_t lane, doubl
         for (uint32_t k = tid; k < ta.n_probes; k += kWave)
             ta.probes[((size_t)lane * ta.steps + s) * ta.n_probes + k] = L.x[ta.probe_ids[k]];
         wsync();
-        seed_lim_at(dp, live, L.x, lim, tid);
+        seed_lim_at(dp, live, (const double *)L.x, lim, tid);
         wsync();
         // the next step's first iterate: the last two accepted points
         // extrapolated (tran.zig:1131-1137; lane_device.zig:141-153)
@@ -495,8 +617,9 @@ __device__ void big_tran_lane(const BigArgs &b, uint32_t g, uint32_t lane, doubl
             wsync();
         }
     }
+    lap(3);
     if (status == kTranRunning && s >= ta.steps) status = kConverged;
-    for (uint32_t i = tid; i < n; i += kWave) a.x[(size_t)lane * n + i] = L.x[i];
+    for (uint32_t i = tid; i < n; i += kWave) xg[i] = L.x[i];
     if (tid == 0) {
         if (status != kRepivot) {
             LaneResume o = rs;
@@ -506,14 +629,17 @@ __device__ void big_tran_lane(const BigArgs &b, uint32_t g, uint32_t lane, doubl
         ta.step[lane] = s;
         a.status[lane] = status;
         a.iterations[lane] = total_it;
+        if (a.prof)
+            for (int p = 0; p < kProfPhases; p++) a.prof[(size_t)lane * kProfPhases + p] += pc[p];
     }
 }
 
 __global__ __launch_bounds__(kWave) void big_lanes(BigArgs b)
 {
-    extern __shared__ double sm[];
     const uint32_t g = blockIdx.x;
     const uint32_t lane = b.ta.op.lane_ids[g];
+    extern __shared__ double dyn[];
+    ldbl *sm = lptr(dyn);
     if (b.tran) big_tran_lane(b, g, lane, sm);
     else big_op_lane(b, g, lane, sm);
 }

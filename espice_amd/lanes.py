@@ -393,7 +393,7 @@ def debug_big_replay(col_ptr, row_idx, lead_vals, q, a, rhs, block_slots=4032, p
     a = np.ascontiguousarray(np.atleast_2d(a), dtype=np.float64)
     rhs = np.ascontiguousarray(np.atleast_2d(rhs), dtype=np.float64)
     n, lanes = len(cp) - 1, a.shape[0]
-    sizes = np.zeros(3, np.uint32)
+    sizes = np.zeros(4, np.uint32)
     args = (n, _ptr(cp), _ptr(ri), _ptr(lead), _ptr(qq), pivot_tol, block_slots)
     _check(L.esl_debug_big_replay(*args, None, None, 0, None, None, None, None, None, _ptr(sizes)), "big replay")
     nl, nu = int(sizes[1]), int(sizes[2])
@@ -402,7 +402,7 @@ def debug_big_replay(col_ptr, row_idx, lead_vals, q, a, rhs, block_slots=4032, p
     _check(L.esl_debug_big_replay(*args, _ptr(a), _ptr(rhs), lanes, _ptr(out["lx"]), _ptr(out["ux"]),
                                   _ptr(out["udiag"]), _ptr(out["dx"]), _ptr(out["fail"]), _ptr(sizes)),
            "big replay")
-    out["blocks"] = int(sizes[0])
+    out["blocks"], out["rows"] = int(sizes[0]), int(sizes[3])
     return out
 
 

@@ -366,8 +366,8 @@ esl_status esl_debug_profile(esl_engine *e, int enable, unsigned long long *out,
  * for `lanes` matrices a [lanes][nnz] and right-hand sides rhs [lanes][n]
  * on the tape factored from lead_vals with column order q. Writes per lane
  * L [lanes][nnz_l], U [lanes][nnz_u], the pivots [lanes][n], dx [lanes][n]
- * (row order), fail (1: the lane must re-pivot) and sizes[3] = {blocks,
- * nnz_l, nnz_u} (lanes == 0: only sizes). Needs no device; lets the block
+ * (row order), fail (1: the lane must re-pivot) and sizes[4] = {blocks,
+ * nnz_l, nnz_u, rows of all blocks} (lanes == 0: only sizes). Needs no device; lets the block
  * programs be checked against the serial replay (lane_kernels.zig:49-143)
  * on any host. */
 esl_status esl_debug_big_replay(uint32_t n, const uint32_t *col_ptr,
