@@ -275,11 +275,13 @@ __device__ inline void run_row(esl::uint2_t w, MP M, double growth, LuFlags &f)
     }
 }
 
-// Runs a row program with kRing rows of words in flight. M (R elements)
-// and the words may be address-space qualified pointers. The word loads are
-// pinned in row order (scheduling barriers) and each refills the register
-// its row just used, so the oldest load completes first and a row waits only
-// for its own word, never for the whole ring.
+// Runs a row program of `rows` rows, a multiple of kRing (its words
+// followed by at least kRing more rows, which are only loaded), with kRing
+// rows of words in flight. M
+// (R elements) and the words may be address-space qualified pointers. The
+// word loads are pinned in row order (scheduling barriers) and each refills
+// the register its row just used, so the oldest load completes first and a
+// row waits only for its own word, never for the whole ring.
 template <typename R, typename WP, typename MP>
 __device__ void run_rows(WP words, uint32_t rows, MP M, double growth, LuFlags &f, uint32_t tid)
 {

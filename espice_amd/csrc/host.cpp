@@ -673,16 +673,27 @@ void build_row_programs(const HostTape &t, TapeHostProg &o)
 // of the refactor row program in the order build_row_programs adds them
 // (column by column; in a column each target's multiply-subtracts in the
 // order of the column replay, an L target then its pivot division), cut
-// into runs that touch at most kBigBlockSlots slots; each run is scheduled
-// into rows of its own over LDS positions (the slots it writes first). The
-// solves run in chunks of columns of at most sol_cap entries. False when a
-// flop's target lies outside its column.
+// into runs that touch at most block_slots slots (by default as many as the
+// LDS the solves need anyway holds, at least kBigBlockSlots); each run is
+// scheduled into rows of its own over LDS positions (the slots it writes
+// first), padded to a multiple of kRing. The runs' rows are stored back to
+// back (a run's prefetches past its end load the next run's words), kRing
+// idle rows after the last. The solves run in chunks of columns of at most
+// sol_cap entries. False when a flop's target lies outside its column.
 constexpr u32 kBigBlockSlots = 4032;
+constexpr u32 kBigSlotLimit = 65536 - esl::kWave;   // 16-bit row-word indices
 constexpr u32 kBigSolveCap = 512;
 
-bool build_big_program(const HostTape &t, TapeHostProg &o, u32 block_slots = kBigBlockSlots)
+bool build_big_program(const HostTape &t, TapeHostProg &o, u32 block_slots = 0)
 {
     const u32 n = t.n, nu = t.nnz_u;
+    // solve chunks: forward over L entries, backward over U entries and pivots
+    u32 cap = kBigSolveCap;
+    for (u32 k = 0; k < n; k++) cap = std::max({cap, t.lp[k + 1] - t.lp[k], t.up[k + 1] - t.up[k] + 1});
+    if (block_slots == 0) {
+        const uint64_t solve = (uint64_t)n + cap + (cap + 1) / 2;
+        block_slots = (u32)std::min<uint64_t>(kBigSlotLimit, std::max<uint64_t>(kBigBlockSlots, solve - esl::kWave));
+    }
     const esl::TapeProg &P = o.prog;
     const u32 lx0 = P.lx0, discard = P.discard;
     o.blk.clear();
@@ -716,8 +727,8 @@ bool build_big_program(const HostTape &t, TapeHostProg &o, u32 block_slots = kBi
         g.schedule(m, rp);
         b.rows = rp.rows;
         b.row0 = rows_total;
-        rows_total += rp.rows + esl::kRing;
-        o.blk_words.insert(o.blk_words.end(), rp.words.begin(), rp.words.end());
+        rows_total += rp.rows;
+        o.blk_words.insert(o.blk_words.end(), rp.words.begin(), rp.words.begin() + (size_t)rp.rows * esl::kWave);
         for (u32 k = b.slot_off; k < (u32)o.blk_slots.size(); k++) local[o.blk_slots[k]] = NONE;
         o.blk.push_back(b);
         run.clear();
@@ -774,9 +785,9 @@ bool build_big_program(const HostTape &t, TapeHostProg &o, u32 block_slots = kBi
         }
     }
     close();
-    // solve chunks: forward over L entries, backward over U entries and pivots
-    u32 cap = kBigSolveCap;
-    for (u32 k = 0; k < n; k++) cap = std::max({cap, t.lp[k + 1] - t.lp[k], t.up[k + 1] - t.up[k] + 1});
+    // idle rows after the last block: the last rows' prefetches stay inside
+    // (a block's prefetches past its end read the next block's words)
+    o.blk_words.resize(o.blk_words.size() + (size_t)esl::kRing * esl::kWave, esl::uint2_t{0, 0});
     o.fch.assign(1, 0);
     o.bch.assign(1, 0);
     u32 cf = 0, cb = 0;
@@ -1565,12 +1576,12 @@ bool validate_big(const TapeHostProg &o, u32 n)
     const size_t total_rows = o.blk_words.size() / esl::kWave;
     u32 bm = 0;
     for (const esl::BigBlock &b : o.blk) {
-        if (b.m == 0 || b.w == 0 || b.w > b.m || b.m > kBigBlockSlots || b.m + esl::kWave > 65536) return false;
+        if (b.m == 0 || b.w == 0 || b.w > b.m || b.m > kBigSlotLimit) return false;
         if ((size_t)b.slot_off + b.m > o.blk_slots.size()) return false;
-        if (b.rows % esl::kRing || (size_t)b.row0 + b.rows + esl::kRing > total_rows) return false;
+        if (b.rows == 0 || b.rows % esl::kRing || (size_t)b.row0 + b.rows + esl::kRing > total_rows) return false;
         bm = std::max(bm, b.m);
         const u32 lim = b.m + esl::kWave;
-        for (size_t r = b.row0; r < (size_t)b.row0 + b.rows + esl::kRing; r++) {
+        for (size_t r = b.row0; r < (size_t)b.row0 + b.rows; r++) {
             const u32 kind = esl::row_kind(o.blk_words[r * esl::kWave].y);
             if (kind >= esl::R_KINDS || kind == esl::R_MACZ) return false;
             for (u32 t = 0; t < (u32)esl::kWave; t++) {
@@ -2001,7 +2012,8 @@ esl_status esl_debug_big_replay(uint32_t n, const uint32_t *col_ptr, const uint3
                                 const double *a, const double *rhs, uint32_t lanes, double *lx, double *ux,
                                 double *udiag, double *dx, uint32_t *fail, uint32_t *sizes)
 {
-    if (!col_ptr || !row_idx || !lead_vals || !q || n == 0 || block_slots < 3 || block_slots > kBigBlockSlots)
+    if (!col_ptr || !row_idx || !lead_vals || !q || n == 0 || (block_slots != 0 && block_slots < 3) ||
+        block_slots > kBigSlotLimit)
         return ESL_INVALID_ARGUMENT;
     if (lanes && (!a || !rhs || !lx || !ux || !udiag || !dx || !fail)) return ESL_INVALID_ARGUMENT;
     HostTape t;
