@@ -65,7 +65,10 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tiling", type=int, default=0, help="0 autotune, 1 lane per wavefront, 2 lane per thread")
-    ap.add_argument("--workload", default="ota_array", choices=["ota_array", "mixed_batch"],
+    ap.add_argument("--mesh-lanes", type=int, default=4096, help="rc_mesh: points")
+    ap.add_argument("--mesh-steps", type=int, default=1000, help="rc_mesh: trapezoid steps of 1 ps")
+    ap.add_argument("--mesh-steps-per-launch", type=int, default=10, help="rc_mesh: steps per kernel launch")
+    ap.add_argument("--workload", default="ota_array", choices=["ota_array", "mixed_batch", "rc_mesh"],
                     help="ota_array: the headline; mixed_batch: 10,000 uneven instances of eight families "
                          "in one persistent launch")
     ap.add_argument("--instances", type=int, default=10000)
@@ -128,6 +131,9 @@ def main():
         sys.exit(2)
     if args.calibrate_copy_mib:
         calibrate_copy(args.calibrate_copy_mib)
+        return
+    if args.workload == "rc_mesh":
+        run_mesh(args)
         return
     if args.workload == "mixed_batch":
         run_mixed(args)
@@ -322,6 +328,92 @@ def main():
 # long inverter chains need hundreds of plain-Newton iterations from a cold
 # start (about two per stage), the other families converge within 100.
 MIXED_OP_MAX_ITER = {"inverter_chain": 1000, "inverter_chain_tran": 1000}
+
+
+def run_mesh(args):
+    """The 64 x 64 RC mesh with diode clamps (4,099 unknowns), 4,096 points,
+    1,000 fixed trapezoid steps of 1 ps after each point's operating point:
+    lanes beyond a workgroup's LDS, one wavefront per lane with its factor
+    slots and vectors in HBM. One step of the benchmark = one esl_tran_solve
+    of every point (the operating points, the transient lead tape, every
+    step), host to host. The transient reports its progress every 100 steps
+    (the progress callback), which this run logs."""
+    from espice_amd import circuits, lanes
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    fam = circuits.rc_mesh(64)
+    d = fam.deck
+    eng = lanes.Engine(d.arrays())
+    probes = [d.node_index("n0_0"), d.node_index("n63_63"), d.node_index("n32_32")]
+    vals = fam.sample(2024, args.mesh_lanes)
+    steps, dt = args.mesh_steps, 1e-12
+    t0 = time.perf_counter()
+    eng.solve_tran(vals[:64], dt, 2, probes=probes)   # untimed: tapes, workspace
+    log(f"rc_mesh 64x64: n {eng.n}, warmup {time.perf_counter() - t0:.1f} s")
+    conv, wall, kern, lane_it, res = 0, 0.0, 0.0, 0, None
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        prog = lambda done, total: log(f"  {done}/{total} steps, {time.perf_counter() - t1:.1f} s") and False
+        res = eng.solve_tran(vals, dt, steps, probes=probes, steps_per_launch=args.mesh_steps_per_launch,
+                             progress=prog, progress_steps=100)
+        wall += time.perf_counter() - t1
+        tm = eng.timing()
+        kern += tm.kernel_ms
+        lane_it += int(tm.lane_iterations)
+        conv += int((res["reasons"] == 0).sum())
+        log(f"step: {time.perf_counter() - t1:.1f} s, kernel {tm.kernel_ms:.0f} ms, "
+            f"{res['iterations'].mean():.1f} transient iterations per point, repivots {tm.repivots}")
+    info = eng.info()
+    n, nnz, nl, nu = eng.n, eng.nnz, int(info.nnz_l), int(info.nnz_u)
+    # algorithmic HBM bytes per lane-iteration with the factors streamed:
+    # A, L and U written then read back by the solves, the pivots, and six
+    # n-vectors (x, x_old, dx, rhs, y, w)
+    per_it = 8 * (nnz + 2 * (nl + nu) + n + 6 * n)
+    achieved = per_it * lane_it / (kern * 1e-3) / 1e9
+    parity, cpu = None, None
+    if not args.no_cpu:
+        from oracle import orc
+        k = min(16, len(vals))
+        _, _, q = eng.pattern()
+        tc = time.perf_counter()
+        ref = orc.tran(orc.Circuit(d.arrays()), vals[:k], q, dt, steps, probes=probes, threads=k)
+        t_cpu = time.perf_counter() - tc
+        ok = (ref["reasons"] == 0) & (res["reasons"][:k] == 0)
+        gx, rx = res["probes"][:k][ok], ref["probes"][ok]
+        parity = {"checked": int(ok.sum()), "probe_rows_bitwise_equal": bool(np.array_equal(
+            gx.view(np.uint64), rx.view(np.uint64))), "x_bitwise_equal": bool(np.array_equal(
+            res["x"][:k][ok].view(np.uint64), ref["x"][ok].view(np.uint64))),
+            "iterations_equal": bool(np.array_equal(res["iterations"][:k], ref["iterations"]))}
+        cpu = {"value": round(int((ref["reasons"] == 0).sum()) / t_cpu, 3), "unit": "points/s", "cores": k,
+               "kind": "port", "sample": f"the first {k} points of the same workload, every step, on the oracle "
+                                         f"(one thread per point, {k} threads, {t_cpu:.1f} s)"}
+        log(f"oracle: {k} points in {t_cpu:.1f} s")
+    out = {
+        "metric": "converged sweep points/sec, 1 MI355X, 64x64 RC mesh 1000-step transient",
+        "value": round(conv / wall, 3),
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 1,
+        "ms_per_step": round(1e3 * wall / args.steps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded R, C and IS draws per point)",
+        "config": {"workload": f"rc_mesh 64x64 + 5% diode clamps, {len(vals)} points, {steps} trapezoid steps "
+                               f"of 1 ps", "n": n, "nnz": nnz, "nnz_l": nl, "nnz_u": nu,
+                   "flops_F": int(info.flops), "tiling": "lane per wavefront, factors and vectors in HBM, "
+                                                         "replay blocks staged in LDS"},
+        "kernel_ms_per_step": round(kern / args.steps, 1),
+        "transient_iterations_per_point": round(lane_it / max(conv, 1) / args.steps, 2),
+        "roofline": {"bound": "hbm", "kernel": "big_lanes", "achieved": round(achieved, 2), "peak": 8000.0,
+                     "unit": "GB/s", "frac": round(achieved / 8000.0, 5), "traffic": None,
+                     "bytes_per_lane_iteration": per_it},
+        "parity": parity,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    eng.close()
 
 
 def run_mixed(args):

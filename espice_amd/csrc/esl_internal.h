@@ -336,17 +336,20 @@ struct LptArgs {
 // Arguments of the kernel for lanes whose working set is beyond a
 // workgroup's LDS (big_lanes): one wavefront per lane, workgroup g runs lane
 // ta.op.lane_ids[g], .op (tran == 0: the OpArgs of ta, modes 0-2, dump) or
-// transient steps (tran == 1). The lane's vectors and factor slots live in
-// HBM at ws + g * ws_stride: tv [tv_cap], then rhs, x, xold, sdiag, q [n
+// transient steps (tran == 1); a workgroup holds `waves` lanes, one per
+// wavefront. The lane's vectors and factor slots live in HBM at
+// ws + g * ws_stride: tv [tv_cap], then rhs, x, xold, sdiag, q [n
 // each] and the charge-Jacobian cells [n_cslot]; LDS (lds doubles) holds
 // the staging cells, a block of the replay, or the solve vector and a chunk
 // of factor entries.
+constexpr uint32_t kBigWaves = 4;   // most lanes (wavefronts) per big-lane workgroup
 struct BigArgs {
     TranArgs ta;
     double *ws;
     uint64_t ws_stride;
     uint32_t N, tran;
-    uint32_t tv_cap, lds;
+    uint32_t tv_cap, lds;       // lds: doubles of one lane's LDS area
+    uint32_t waves, pad;        // lanes per workgroup (1..kBigWaves)
 };
 __host__ __device__ inline size_t big_stride(uint32_t tv_cap, uint32_t n, uint32_t n_cslot)
 {

@@ -2172,6 +2172,14 @@ u32 big_lds(const Engine &e)
     return esl::big_lds_doubles(e.c.n, bm, sc);
 }
 
+// Lanes per big-lane workgroup: as many as the LDS of one workgroup holds,
+// at most kBigWaves.
+u32 big_waves(const Engine &e)
+{
+    const u32 per = big_lds(e) * 8;
+    return std::max<u32>(1, std::min<u32>(esl::kBigWaves, e.lds_limit / per));
+}
+
 // The workspace of a big engine's launch over `lanes` lanes with up to
 // tv_cap factor slots each, and the kernel's LDS limit (set ahead of graph
 // capture).
@@ -2184,7 +2192,7 @@ esl_status big_room(Engine &e, size_t lanes, u32 tv_cap)
         g_last_error = "too many unknowns for a lane's solve vector in LDS";
         return ESL_FAILED;
     }
-    HIPCHK(esl_big_attrs(lds));
+    HIPCHK(esl_big_attrs(big_waves(e) * lds));
     return ESL_OK;
 }
 
@@ -2198,6 +2206,7 @@ esl::BigArgs big_args(Engine &e, const esl::TranArgs &t, u32 lanes, u32 tran)
     b.tran = tran;
     b.tv_cap = t.op.tv_cap;
     b.lds = big_lds(e);
+    b.waves = big_waves(e);
     return b;
 }
 
@@ -2907,13 +2916,20 @@ extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t
         const u32 cap = e->tapes[rs[0].tape]->prog.n_tv;
         esl_status r = tran_room(*e, count, cap);
         if (r != ESL_OK) return r;
-        HIPCHK(hipEventRecord(e->ev0, s));
-        if (o.use_graph) {
+        // the launch blocks [b0, b1): one graph, or launch by launch
+        auto run_blocks = [&](u32 b0, u32 b1) -> esl_status {
+            if (!o.use_graph) {
+                for (u32 b = b0; b < b1; b++) {
+                    esl::TranArgs t = block_args(b, cap, b == 0);
+                    HIPCHK(launch_tran(*e, t, (u32)count, s));
+                }
+                return ESL_OK;
+            }
             hipGraph_t g = nullptr;
             hipGraphExec_t ge = nullptr;
             HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             hipError_t ce = hipSuccess;
-            for (u32 b = 0; b < B && ce == hipSuccess; b++) {
+            for (u32 b = b0; b < b1 && ce == hipSuccess; b++) {
                 esl::TranArgs t = block_args(b, cap, b == 0);
                 ce = launch_tran(*e, t, (u32)count, s);
             }
@@ -2931,10 +2947,25 @@ extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t
             (void)hipGraphDestroy(g);
             HIPCHK(le);
             HIPCHK(se);
+            return ESL_OK;
+        };
+        HIPCHK(hipEventRecord(e->ev0, s));
+        if (!o.progress) {
+            esl_status r2 = run_blocks(0, B);
+            if (r2 != ESL_OK) return r2;
         } else {
-            for (u32 b = 0; b < B; b++) {
-                esl::TranArgs t = block_args(b, cap, b == 0);
-                HIPCHK(launch_tran(*e, t, (u32)count, s));
+            // groups of launch blocks, the caller told after each
+            const u32 every = o.progress_steps ? o.progress_steps : 100;
+            const u32 G = std::max<u32>(1, (every + S - 1) / S);
+            for (u32 b0 = 0; b0 < B; b0 += G) {
+                const u32 b1 = std::min(B, b0 + G);
+                esl_status r2 = run_blocks(b0, b1);
+                if (r2 != ESL_OK) return r2;
+                HIPCHK(hipStreamSynchronize(s));
+                if (o.progress(o.progress_ctx, std::min(o.steps, b1 * S), o.steps) != 0) {
+                    g_last_error = "transient cancelled by its progress callback";
+                    return ESL_FAILED;
+                }
             }
         }
         HIPCHK(hipEventRecord(e->ev1, s));

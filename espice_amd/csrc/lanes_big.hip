@@ -363,7 +363,7 @@ __device__ __forceinline__ bool big_limit(const DeckProg &dp, const double *live
 __device__ __forceinline__ void big_op_lane(const BigArgs &b, uint32_t g, uint32_t lane, ldbl *sm)
 {
     const OpArgs &a = b.ta.op;
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x % kWave;
     const DeckProg &dp = a.dp;
     const uint32_t n = dp.n;
     const BigLane L = big_lane(b, g, sm);
@@ -481,7 +481,7 @@ __device__ __forceinline__ void big_tran_lane(const BigArgs &b, uint32_t g, uint
 {
     const TranArgs &ta = b.ta;
     const OpArgs &a = ta.op;
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x % kWave;
     if (a.status[lane] != kTranRunning) return;
     uint32_t s = ta.step[lane];
     if (s >= ta.s1) return;
@@ -634,12 +634,19 @@ __device__ __forceinline__ void big_tran_lane(const BigArgs &b, uint32_t g, uint
     }
 }
 
-__global__ __launch_bounds__(kWave) void big_lanes(BigArgs b)
+// Workgroups of b.waves wavefronts, each wavefront one lane in its own
+// b.lds doubles of the LDS. The lanes of a workgroup start together and do
+// the same work per Newton iteration (the same tape), so they walk the
+// replay's row words nearly in step and share their fetches in the caches;
+// they never wait on one another.
+__global__ __launch_bounds__(kWave * kBigWaves) void big_lanes(BigArgs b)
 {
-    const uint32_t g = blockIdx.x;
+    const uint32_t w = threadIdx.x / kWave;
+    const uint32_t g = blockIdx.x * b.waves + w;
+    if (g >= b.N) return;
     const uint32_t lane = b.ta.op.lane_ids[g];
     extern __shared__ double dyn[];
-    ldbl *sm = lptr(dyn);
+    ldbl *sm = lptr(dyn) + (size_t)w * b.lds;
     if (b.tran) big_tran_lane(b, g, lane, sm);
     else big_op_lane(b, g, lane, sm);
 }
@@ -655,6 +662,8 @@ extern "C" hipError_t esl_big_attrs(uint32_t lds_bytes)
 
 extern "C" hipError_t esl_launch_big(const esl::BigArgs *b, hipStream_t s)
 {
-    hipLaunchKernelGGL(esl::big_lanes, dim3(b->N), dim3(esl::kWave), b->lds * (uint32_t)sizeof(double), s, *b);
+    const uint32_t grid = (b->N + b->waves - 1) / b->waves;
+    hipLaunchKernelGGL(esl::big_lanes, dim3(grid), dim3(esl::kWave * b->waves),
+                       b->waves * b->lds * (uint32_t)sizeof(double), s, *b);
     return hipGetLastError();
 }

@@ -85,10 +85,15 @@ class Timing(C.Structure):
                 ("warm_ms", C.c_double), ("lead_ms", C.c_double), ("download_ms", C.c_double)]
 
 
+# progress(ctx, steps_done, steps) -> nonzero cancels (esl_tran_options.progress)
+PROGRESS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.c_uint32)
+
+
 class TranOptions(C.Structure):
     _fields_ = [("struct_size", C.c_size_t), ("tol", Tolerances), ("dt", C.c_double), ("steps", C.c_uint32),
                 ("steps_per_launch", C.c_uint32), ("probes", C.c_void_p), ("n_probes", C.c_uint32),
-                ("use_graph", C.c_uint32)]
+                ("use_graph", C.c_uint32), ("progress", PROGRESS_FN), ("progress_ctx", C.c_void_p),
+                ("progress_steps", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class BatchItem(C.Structure):
@@ -290,10 +295,13 @@ class Engine:
                "esl_op_solve")
         return {"x": x, "iterations": it, "reasons": rs}
 
-    def solve_tran(self, values, dt, steps, probes=(), steps_per_launch=10, use_graph=True, max_iter=None):
+    def solve_tran(self, values, dt, steps, probes=(), steps_per_launch=10, use_graph=True, max_iter=None,
+                   progress=None, progress_steps=0):
         """Transient lanes: operating point, then `steps` fixed steps of dt.
         Returns x [count][n] (last state), probes [count][steps][n_probes],
-        iterations, reasons."""
+        iterations, reasons. progress(steps_done, steps), when given, is
+        called after each group of about progress_steps steps; a true return
+        cancels the solve (EslError)."""
         values = np.ascontiguousarray(np.atleast_2d(np.asarray(values, dtype=np.float64)))
         count = values.shape[0]
         o = TranOptions()
@@ -301,6 +309,9 @@ class Engine:
         o.dt, o.steps, o.steps_per_launch, o.use_graph = dt, steps, steps_per_launch, 1 if use_graph else 0
         if max_iter is not None:
             o.tol.max_iter = max_iter
+        if progress is not None:
+            cb = PROGRESS_FN(lambda ctx, done, total: 1 if progress(int(done), int(total)) else 0)
+            o.progress, o.progress_steps = cb, progress_steps
         pr = np.ascontiguousarray(probes, dtype=np.uint32)
         if len(pr):
             o.probes, o.n_probes = pr.ctypes.data, len(pr)

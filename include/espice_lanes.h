@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define ESL_ABI_VERSION 3u
+#define ESL_ABI_VERSION 4u
 
 /* esl_status: the values of espice_status (include/espice.h:21-38). */
 typedef uint32_t esl_status;
@@ -263,6 +263,17 @@ typedef struct {
     const uint32_t *probes;    /* unknowns recorded after every step */
     uint32_t n_probes;
     uint32_t use_graph;        /* 1 (default): capture the launches in a hipGraph */
+    /* Progress and cancellation (the reference ends a running batch on a
+     * cancelled query, tran.zig:1824-1826): when set, the launches run in
+     * groups of at least progress_steps steps (0: 100), each group captured
+     * in its own graph, and after each group progress(ctx, steps done,
+     * steps) is called on the calling thread; a nonzero return cancels the
+     * solve (ESL_FAILED, esl_error_message says so). Results do not depend
+     * on the grouping. */
+    int (*progress)(void *ctx, uint32_t steps_done, uint32_t steps);
+    void *progress_ctx;
+    uint32_t progress_steps;
+    uint32_t reserved;
 } esl_tran_options;
 void esl_default_tran_options(esl_tran_options *opt);
 /* values [count][n_live]; outputs x [count][n] (the last accepted state),
@@ -301,7 +312,8 @@ typedef struct {
                                   runs its points' with the transient's tolerances and at
                                   least 100 iterations) */
     const esl_tran_options *tran;  /* ESL_BATCH_TRAN: dt, steps, tol, probes
-                                      (steps_per_launch and use_graph unused) */
+                                      (steps_per_launch, use_graph and progress
+                                      unused) */
     double *x;                 /* out [count][n] */
     double *probe_out;         /* out, ESL_BATCH_TRAN with probes: [count][steps][n_probes] */
     uint32_t *iterations;      /* out [count] (.op: Newton iterations; transient:

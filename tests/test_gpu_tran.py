@@ -118,3 +118,26 @@ def test_switching_transient_repivots_on_the_device_bitwise(lanes, oracle):
     assert np.array_equal(bits(r["x"][ok]), bits(ref["x"][ok]))
     assert np.array_equal(bits(r["probes"][ok]), bits(ref["probes"][ok]))
     eng.close()
+
+
+def test_progress_groups_and_cancellation(lanes):
+    """The progress callback sees each group of steps; grouping the launches
+    (one graph per group) leaves the bits unchanged; a true return cancels."""
+    fam = circuits.rc_mesh(8, diode_frac=0.1)
+    vals = fam.sample(77, 32)
+    probes = [fam.deck.node_index("n7_7")]
+    eng = lanes.Engine(fam.deck.arrays())
+    ref = eng.solve_tran(vals, 1e-12, 25, probes=probes, steps_per_launch=5)
+    seen = []
+    got = eng.solve_tran(vals, 1e-12, 25, probes=probes, steps_per_launch=5,
+                         progress=lambda done, total: seen.append((done, total)), progress_steps=10)
+    assert seen == [(10, 25), (20, 25), (25, 25)]
+    assert np.array_equal(bits(got["probes"]), bits(ref["probes"]))
+    assert np.array_equal(got["iterations"], ref["iterations"])
+    with pytest.raises(lanes.EslError) as ex:
+        eng.solve_tran(vals, 1e-12, 25, probes=probes, steps_per_launch=5,
+                       progress=lambda done, total: done >= 10, progress_steps=10)
+    assert "cancelled" in str(ex.value)
+    again = eng.solve_tran(vals, 1e-12, 25, probes=probes, steps_per_launch=5)
+    assert np.array_equal(bits(again["probes"]), bits(ref["probes"]))
+    eng.close()

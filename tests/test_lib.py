@@ -31,11 +31,16 @@ def test_library_exports_every_declared_symbol():
 def test_abi_version_and_defaults():
     from espice_amd import lanes
     lib = lanes.load()
-    assert lib.esl_abi_version() == 3
+    assert lib.esl_abi_version() == 4
     o = lanes.default_options()
     assert o.struct_size == ctypes.sizeof(lanes.OpOptions)
     assert o.tol.reltol == 1e-3 and o.tol.abstol == 1e-12 and o.tol.vntol == 1e-6
     assert o.tol.residual_tol == 1e-9 and o.tol.max_iter == 100
+    t = lanes.TranOptions()
+    lib.esl_default_tran_options(ctypes.byref(t))
+    assert t.struct_size == ctypes.sizeof(lanes.TranOptions)
+    assert t.steps_per_launch == 10 and t.use_graph == 1 and t.tol.max_iter == 50
+    assert not t.progress and t.progress_ctx is None and t.progress_steps == 0
 
 
 def test_invalid_handles_are_rejected():
