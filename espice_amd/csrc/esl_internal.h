@@ -263,6 +263,14 @@ struct OpArgs {
     unsigned long long *prof;
     // the lanes' own LU for re-pivots on the device (FP64 lanes)
     PrivLU priv;
+    // optional [N]: the .op ladder's gmin rung, a conductance gmin[lane]
+    // from every node row to ground (A_ii += g, F_i += g x_i) after each
+    // assembly (Jacobian dumps included)
+    const double *gmin;
+    // 1: a lane that ends its run (converged, unconverged, singular) writes
+    // the LU it is on back to resume[lane].tape, so that a later run of the
+    // lane (the ladder's next rung) continues on the same pivots
+    uint32_t keep_tape, pad_kt;
 };
 constexpr int kProfPhases = 4;
 

@@ -119,6 +119,23 @@ __device__ void assemble_pattern(const DeckProg &dp, const double *live, const d
     wsync();
 }
 
+// The gmin rung's shunts (OpArgs::gmin) on an assembled Jacobian M whose
+// row-i diagonal is M[diag[i]]: A_ii += g, F_i += g x_i on every node row
+// (not row 0, the ground pin, nor branch rows), as the oracle's newton_run.
+__device__ inline void add_gmin(const OpArgs &a, uint32_t lane, const uint32_t *diag, double *M, double *rhs,
+                                const double *x, uint32_t tid)
+{
+    if (!a.gmin) return;
+    const double g = a.gmin[lane];
+    if (!(g > 0)) return;
+    for (uint32_t i = 1 + tid; i < a.dp.n; i += kWave)
+        if (!a.dp.current_row[i]) {
+            M[diag[i]] = M[diag[i]] + g;
+            rhs[i] = rhs[i] + g * x[i];
+        }
+    wsync();
+}
+
 struct RedAcc {
     int cur;
     bool rhs;
@@ -299,6 +316,7 @@ __global__ __launch_bounds__(kWave) void op_dump(OpArgs a)
     double *lim = a.lim + (size_t)lane * lim_width(dp);
     lane_init(a, lane, live, x, lim, tid);
     assemble_pattern(dp, live, x, lim, nm, tv, rhs, st, tid);
+    add_gmin(a, lane, dp.diag_slot, tv, rhs, x, tid);
     for (uint32_t p = tid; p < dp.nnz; p += kWave) a.a_out[(size_t)blockIdx.x * dp.nnz + p] = tv[p];
     for (uint32_t i = tid; i < n; i += kWave) a.f_out[(size_t)blockIdx.x * n + i] = rhs[i];
 }
@@ -390,8 +408,13 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
             restart_nominal(iter);
         }
         lap(3);
-        if (priv) assemble_pattern(dp, live, x, lim, nm, pap, rhs, st, tid);
-        else assemble(dp, T, live, x, lim, nm, tv, rhs, st, tid, cr);
+        if (priv) {
+            assemble_pattern(dp, live, x, lim, nm, pap, rhs, st, tid);
+            add_gmin(a, lane, dp.diag_slot, pap, rhs, x, tid);
+        } else {
+            assemble(dp, T, live, x, lim, nm, tv, rhs, st, tid, cr);
+            add_gmin(a, lane, T.diag_tv, tv, rhs, x, tid);
+        }
         if (warm == 1) {
             uint64_t fb = 0;
             for (uint32_t i = tid; i < n; i += kWave) {
@@ -460,6 +483,7 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
                 // sums as the failed assembly), factored by the lane
                 wsync();
                 assemble_pattern(dp, live, x, lim, nm, pap, rhs, st, tid);
+                add_gmin(a, lane, dp.diag_slot, pap, rhs, x, tid);
                 for (uint32_t i = tid; i < n; i += kWave) tv[i] = 0.0;   // the factor's dense column (LDS)
                 wsync();
                 int rc = kPlOk;
@@ -503,6 +527,7 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
                 // state exactly as the failed assembly did
                 wsync();
                 assemble_pattern(dp, live, x, lim, nm, a.a_out + (size_t)lane * dp.nnz, rhs, st, tid);
+                add_gmin(a, lane, dp.diag_slot, a.a_out + (size_t)lane * dp.nnz, rhs, x, tid);
             }
             status = exit_status;
             iters = iter;
@@ -584,6 +609,7 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
     if (tid == 0) {
         a.status[lane] = status;
         a.iterations[lane] = iters;
+        if (a.keep_tape && status != kRepivot && status != kToFp64) a.resume[lane].tape = rs.tape;
         if (a.prof)
             for (int p = 0; p < kProfPhases; p++) a.prof[(size_t)lane * kProfPhases + p] += pc[p];
     }

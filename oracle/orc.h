@@ -121,8 +121,11 @@ typedef struct {
 
 typedef struct {
     double reltol, abstol, vntol, residual_tol;
-    uint32_t max_iter, reserved;
+    uint32_t max_iter, reserved;   /* reserved: flags (ORC_TOL_GMIN_STEPPING) */
 } orc_tol;                         /* the layout of esl_tolerances */
+/* orc_op_solve*: FP64 cold or given starts that fail the plain rung run the
+ * .op ladder's gmin rung (orc_ckt.c gmin_ladder) */
+#define ORC_TOL_GMIN_STEPPING 1u
 
 typedef struct orc_ckt orc_ckt;
 /* NULL on a malformed deck (message in err). */

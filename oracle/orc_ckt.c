@@ -1033,6 +1033,144 @@ static int refine_solve(const orc_ckt *c, const orc_lu *lu, orc_lu32 *f, const d
     return 1;
 }
 
+/* ---- the .op ladder's gmin rung (dc/op.zig:160-222, ngspice cktop.c
+ * dynamic_gmin): after a failed plain rung, Newton runs with a conductance
+ * gmin from every node row to ground (A_ii += gmin, F_i += gmin x_i, the
+ * load of converger.zig:699-713 on the node diagonals), gmin descending from
+ * 1e-3 by an adaptive factor to 1e-12, each run at most itl2 = 50
+ * iterations; then a clean run without the shunts from the last point. A
+ * run that fails backs up toward the last good gmin by the 4th root of the
+ * factor and restarts from the last converged point (cold while none).
+ * The GPU host path runs this control, its rungs on the device. ---- */
+#define ORC_GMIN_TARGET 1e-12
+#define ORC_GMIN_START 1e-2
+#define ORC_ITL2 50u
+
+/* Scratch of one lane's FP64 Newton runs. */
+typedef struct {
+    double *x_old, *dx, *g, *rhs, *sdiag, *x_good;
+} newton_ws;
+
+/* One plain FP64 Newton run of a lane: cold (x = 0, cold limiting state,
+ * INITFIX) or from x as given (limiting state seeded at x), at most max_iter
+ * iterations, gmin shunts on the node diagonals when gmin > 0; the gates of
+ * solve_lane. Returns the reason (0 converged, 1 not converged, 3 singular)
+ * and the iterations in *its; re-pivots the lane's tape lu in place. */
+static uint32_t newton_run(const orc_ckt *c, lane_dev *ld, orc_lu *lu, const uint32_t *q, const orc_tol *tol,
+                           uint32_t max_iter, double gmin, int cold, double *x, newton_ws *w, uint32_t *its,
+                           uint32_t *repiv)
+{
+    const uint32_t n = c->n;
+    const double wide = 1.0 / 1e-3;
+    int init_fix;
+    if (cold) {
+        memset(x, 0, (size_t)n * 8);
+        seed_cold(c, ld);
+        init_fix = 1;
+    } else {
+        seed_at(c, ld, x);
+        init_fix = 0;
+    }
+    double limit = 1e9;
+    uint32_t iter = 0;
+    for (; iter < max_iter; iter++) {
+        assemble(c, ld, x, w->g, w->rhs);
+        if (gmin > 0)
+            for (uint32_t i = 1; i < n; i++)
+                if (!c->current_row[i]) {
+                    w->g[c->diag[i]] += gmin;
+                    w->rhs[i] += gmin * x[i];
+                }
+        if (iter == 0) limit = plausible_limit(c, x, w->rhs, w->g);
+        for (uint32_t i = 0; i < n; i++) w->sdiag[i] = w->g[c->diag[i]];
+        if (orc_lu_lane_refactor(lu, c->col_ptr, w->g, 1e-12, wide)) {
+            (*repiv)++;
+            if (orc_lu_factor(lu, c->col_ptr, c->row_idx, w->g, q, 1e-3) != 0) {
+                *its = iter + 1;
+                return 3;
+            }
+        }
+        orc_lu_solve_neg(lu, w->rhs, w->dx);
+        uint64_t worst = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            const double xo = x[i], d = w->dx[i], xn = xo + d;
+            w->x_old[i] = xo;
+            x[i] = xn;
+            const double t = tol->reltol * fmax(fabs(xn), fabs(xo)) + (c->current_row[i] ? tol->abstol : tol->vntol);
+            const int finite = fabs(xn) < INFINITY && fabs(d) < INFINITY;
+            const double qq = fabs(d) / t;
+            const double score = !finite ? INFINITY : (qq == qq ? qq : 0.0);
+            const uint64_t sb = db(score);
+            if (sb > worst) worst = sb;
+        }
+        const int limited = apply_limits(c, ld, x);
+        int ok = !limited && iter != 0 && bd(worst) < 1.0;
+        if (ok)
+            for (uint32_t i = 0; i < n; i++)
+                if (!(fabs(x[i]) <= limit)) { ok = 0; break; }
+        if (ok)
+            for (uint32_t i = 0; i < n; i++) {
+                const double s0 = fabs(w->sdiag[i]);
+                const double s = s0 == 0 ? 1.0 : s0;
+                const double rt = fmax(tol->residual_tol, 10.0 * s * (tol->reltol * fabs(x[i]) + tol->vntol));
+                if (!(fabs(w->rhs[i]) <= rt)) { ok = 0; break; }
+            }
+        if (ok && init_fix) {
+            init_fix = 0;
+        } else if (ok) {
+            memcpy(x, w->x_old, (size_t)n * 8);
+            *its = iter + 1;
+            return 0;
+        }
+    }
+    *its = iter;
+    return 1;
+}
+
+/* The gmin rung and the clean run after a failed plain rung of `plain`
+ * iterations; x is the lane's state (overwritten). Returns the reason (0
+ * or 1) and the lane's total iterations in *total. */
+static uint32_t gmin_ladder(const orc_ckt *c, lane_dev *ld, orc_lu *lu, const uint32_t *q, const orc_tol *tol,
+                            double *x, newton_ws *w, uint32_t plain, uint32_t *total, uint32_t *repiv)
+{
+    const uint32_t n = c->n;
+    double factor = 10.0, good = ORC_GMIN_START, gv = good / factor;
+    int have_good = 0, cold = 1;
+    uint32_t tot = plain, its = 0;
+    for (uint32_t solves = 0; solves < 100; solves++) {
+        const uint32_t r = newton_run(c, ld, lu, q, tol, ORC_ITL2, gv, cold, x, w, &its, repiv);
+        tot += its;
+        if (r == 0) {
+            if (gv <= ORC_GMIN_TARGET) {
+                const uint32_t rc = newton_run(c, ld, lu, q, tol, tol->max_iter, 0.0, 0, x, w, &its, repiv);
+                tot += its;
+                *total = tot;
+                return rc == 0 ? 0 : 1;
+            }
+            memcpy(w->x_good, x, (size_t)n * 8);
+            have_good = 1;
+            good = gv;
+            if (its <= ORC_ITL2 / 4) factor = fmin(factor * sqrt(factor), 10.0);
+            else if (its > 3 * ORC_ITL2 / 4) factor = fmax(sqrt(factor), 1.00005);
+            if (gv < factor * ORC_GMIN_TARGET) {
+                factor = gv / ORC_GMIN_TARGET;
+                gv = ORC_GMIN_TARGET;
+            } else {
+                gv /= factor;
+            }
+            cold = 0;
+        } else {
+            if (factor < 1.00005) break;
+            factor = sqrt(sqrt(factor));
+            gv = good / factor;
+            if (have_good) memcpy(x, w->x_good, (size_t)n * 8);
+            cold = !have_good;
+        }
+    }
+    *total = tot;
+    return 1;
+}
+
 /* one lane's Newton (converger.newtonLanes at W = 1 over a lane solver) */
 static void solve_lane(const job_t *j, size_t l)
 {
@@ -1189,6 +1327,11 @@ static void solve_lane(const job_t *j, size_t l)
         }
     }
     if (reason == 1) iters = iter;
+    if (reason != 0 && (tol->reserved & ORC_TOL_GMIN_STEPPING) && j->prec.precision == 0 && !j->prec.nominal_x) {
+        newton_ws w = {x_old, dx, g, rhs, sdiag, calloc(n + 1, 8)};
+        reason = gmin_ladder(c, &ld, lu, j->q, tol, x, &w, iters, &iters, &repiv);
+        free(w.x_good);
+    }
     memcpy(j->x + l * n, x, (size_t)n * 8);
     j->iterations[l] = iters;
     if (j->fell_back) j->fell_back[l] = warm == 2;
