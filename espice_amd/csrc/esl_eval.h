@@ -94,10 +94,12 @@ __device__ inline void mos_eval(const Mos &m, double xd, double xg, double xs, d
 }
 
 // ---- device evaluation of one chunk (one instance per thread) into the
-// staging cells st[e * 64 + t]
+// staging cells st[e * 64 + t]; src scales every independent source (the
+// .op ladder's source stepping, vsource.va:327-329, isource.va:326-328)
 template <typename X, typename S>
 __device__ inline void eval_chunk(const DeckProg &dp, const Chunk &ch, const double *live, X x, double *lim,
-                                  uint32_t lim_stride, S st, uint32_t tid, double t = 0.0, bool charges = false)
+                                  uint32_t lim_stride, S st, uint32_t tid, double t = 0.0, bool charges = false,
+                                  double src = 1.0)
 {
     if (tid >= ch.count) return;
     const uint32_t j = ch.first + tid;
@@ -131,7 +133,7 @@ __device__ inline void eval_chunk(const DeckProg &dp, const Chunk &ch, const dou
     case kV: {
         const uint32_t p = dp.v_t[3 * j], q = dp.v_t[3 * j + 1], br = dp.v_t[3 * j + 2];
         const double v1 = dp.v_bind[j] >= 0 ? live[dp.v_bind[j]] : dp.v_val[j];
-        const double dc = (dp.v_wkind[j] == 1 && t > dp.v_td[j]) ? dp.v_v2[j] : v1;
+        const double dc = ((dp.v_wkind[j] == 1 && t > dp.v_td[j]) ? dp.v_v2[j] : v1) * src;
         const double ib = x[br];
         st[0 * kWave + tid] = 1.0;
         st[1 * kWave + tid] = -1.0;
@@ -143,7 +145,7 @@ __device__ inline void eval_chunk(const DeckProg &dp, const Chunk &ch, const dou
         break;
     }
     case kI: {
-        const double dc = dp.i_bind[j] >= 0 ? live[dp.i_bind[j]] : dp.i_val[j];
+        const double dc = (dp.i_bind[j] >= 0 ? live[dp.i_bind[j]] : dp.i_val[j]) * src;
         st[0 * kWave + tid] = dc;
         st[1 * kWave + tid] = -dc;
         break;

@@ -271,6 +271,9 @@ struct OpArgs {
     // the LU it is on back to resume[lane].tape, so that a later run of the
     // lane (the ladder's next rung) continues on the same pivots
     uint32_t keep_tape, pad_kt;
+    // optional [N]: the .op ladder's source rung, every independent source
+    // scaled by src[lane] in the device evaluation (Jacobian dumps included)
+    const double *src;
 };
 constexpr int kProfPhases = 4;
 

@@ -1356,7 +1356,7 @@ struct Engine {
     u32 lead_id = 0;               // tape of the last solve's lead point
     DevBuf probe_ids, probe_out;
     DevBuf tq1, tip, txp, tstep, tprobes;   // transient state
-    DevBuf gminv, xgood;                     // the .op ladder's gmin rung: per-lane g, last converged x
+    DevBuf gminv, srcv, xgood;               // the .op ladder: per-lane g and source scale, last converged x
     DevBuf pl_u32, pl_f64;                  // the lanes' own LU (device re-pivot)
     bool has_nominal = false;
     // lane-per-thread state, lane-major
@@ -2453,26 +2453,32 @@ esl_status op_repivot(Engine &E, const HostOpArgs &a, size_t count, vector<u32> 
     return ESL_OK;
 }
 
-// The .op ladder's gmin rung for the lanes the plain rung left unconverged
-// or singular (control of dc/op.zig:160-222, the oracle's gmin_ladder): each
-// rung is a Newton run on the device (the plain kernels with per-lane g on
-// the node diagonals, re-pivots as in the plain rung), the control here.
-// Runs rounds of three launch groups (cold rungs, rungs from the last point,
-// clean runs); st and the device's status and iterations are final on
-// return (iterations summed over the plain rung and every run).
-esl_status op_gmin_ladder(Engine &E, HostOpArgs a, const esl::PrivLU &own, size_t count, const esl_op_options &o,
-                          vector<u32> &st, vector<u32> &its, vector<uint8_t> &lane64, float &kernel_ms)
+// The .op ladder for the lanes the plain rung left unconverged or singular
+// (dc/op.zig:160-288, the oracle's op_ladder): the gmin rung (Newton runs
+// with per-lane shunts g on the node diagonals, g stepped down from 1e-3 to
+// 1e-12, then a clean run), then the source rung (runs with every
+// independent source scaled by a per-lane lambda stepped from 0 to 1), then
+// a final run at full sources. Every run is a launch of the plain kernels
+// over the lanes at that point of their ladder (re-pivots as in the plain
+// rung); the control is here. A round launches three groups: cold runs and
+// runs from the last point (both at most itl2 = 50 iterations), and clean
+// and final runs (max_iter). st and the device's status and iterations are
+// final on return (iterations summed over the plain rung and every run).
+esl_status op_ladder(Engine &E, HostOpArgs a, const esl::PrivLU &own, size_t count, const esl_op_options &o,
+                     vector<u32> &st, vector<u32> &its, vector<uint8_t> &lane64, float &kernel_ms)
 {
     Engine *e = &E;
     const u32 n = e->c.n;
     hipStream_t s = e->stream;
     constexpr double kTarget = 1e-12, kStart = 1e-2;
     constexpr u32 kItl2 = 50;
+    enum : int { kGminRung, kGminClean, kSrcRung, kFinal, kDone };
     struct Lad {
-        double factor, good, gv;
+        double factor, good, gv;            // gmin rung
+        double lambda, lambda_good, delta;  // source rung
         bool have_good, cold;
         u32 total, solves;
-        int phase;   // 0 gmin rungs, 1 the clean run, 2 done
+        int phase;
     };
     vector<u32> dev_its(count);
     HIPCHK(hipMemcpyAsync(dev_its.data(), e->iters.p, count * 4, hipMemcpyDeviceToHost, s));
@@ -2482,23 +2488,31 @@ esl_status op_gmin_ladder(Engine &E, HostOpArgs a, const esl::PrivLU &own, size_
     for (size_t l = 0; l < count; l++) {
         if (st[l] != esl::kNoConvergence && st[l] != esl::kSingular) continue;
         const u32 plain = (st[l] == esl::kSingular && its[l]) ? its[l] : dev_its[l];
-        lad[l] = Lad{10.0, kStart, kStart / 10.0, false, true, plain, 0, 0};
+        lad[l] = Lad{10.0, kStart, kStart / 10.0, 0.0, -1.0, 0.25, false, true, plain, 0, kGminRung};
         ladder.push_back((u32)l);
     }
     if (ladder.empty()) return ESL_OK;
     HIPCHK(e->gminv.ensure(count * 8));
+    HIPCHK(e->srcv.ensure(count * 8));
     HIPCHK(e->xgood.ensure(count * (size_t)n * 8));
-    vector<double> gh(count, 0.0);
+    vector<double> gh(count, 0.0), sh(count, 1.0);
     double *xd = e->x.as<double>(), *xg = e->xgood.as<double>();
+    auto save_good = [&](u32 l) -> hipError_t {
+        return hipMemcpyAsync(xg + (size_t)l * n, xd + (size_t)l * n, n * 8, hipMemcpyDeviceToDevice, s);
+    };
+    auto back_to_good = [&](u32 l) -> hipError_t {
+        return hipMemcpyAsync(xd + (size_t)l * n, xg + (size_t)l * n, n * 8, hipMemcpyDeviceToDevice, s);
+    };
     // one launch group: Newton runs from mode's start, then host re-pivot
     // rounds for lanes that left for one
-    auto run = [&](const vector<u32> &ls, u32 mode, u32 max_iter, bool shunt) -> esl_status {
+    auto run = [&](const vector<u32> &ls, u32 mode, u32 max_iter) -> esl_status {
         if (ls.empty()) return ESL_OK;
         for (u32 l : ls) its[l] = 0;   // set again by a host re-pivot that finds the lane singular
         HIPCHK(hipMemcpyAsync(e->lane_ids.p, ls.data(), ls.size() * 4, hipMemcpyHostToDevice, s));
         HostOpArgs r = a;
         r.max_iter = max_iter;
-        r.gmin = shunt ? e->gminv.as<double>() : nullptr;
+        r.gmin = e->gminv.as<double>();
+        r.src = e->srcv.as<double>();
         r.priv = own;
         r.keep_tape = 1;
         u32 cap = 0;
@@ -2518,63 +2532,97 @@ esl_status op_gmin_ladder(Engine &E, HostOpArgs a, const esl::PrivLU &own, size_
         return ESL_OK;
     };
     for (;;) {
-        vector<u32> cold, warm, clean;
+        vector<u32> cold, warm, full;
         for (u32 l : ladder) {
-            if (lad[l].phase == 0) (lad[l].cold ? cold : warm).push_back(l);
-            else if (lad[l].phase == 1) clean.push_back(l);
+            const Lad &d = lad[l];
+            gh[l] = d.phase == kGminRung ? d.gv : 0.0;
+            sh[l] = d.phase == kSrcRung ? d.lambda : 1.0;
+            if (d.phase == kGminRung) (d.cold ? cold : warm).push_back(l);
+            else if (d.phase == kSrcRung) (d.lambda_good < 0.0 ? cold : warm).push_back(l);
+            else if (d.phase == kGminClean || d.phase == kFinal) full.push_back(l);
         }
-        if (cold.empty() && warm.empty() && clean.empty()) break;
-        for (u32 l : ladder) gh[l] = lad[l].phase == 0 ? lad[l].gv : 0.0;
+        if (cold.empty() && warm.empty() && full.empty()) break;
         HIPCHK(hipMemcpyAsync(e->gminv.p, gh.data(), count * 8, hipMemcpyHostToDevice, s));
-        esl_status rc = run(cold, 0, kItl2, true);
-        if (rc == ESL_OK) rc = run(warm, 1, kItl2, true);
-        if (rc == ESL_OK) rc = run(clean, 1, o.tol.max_iter, false);
+        HIPCHK(hipMemcpyAsync(e->srcv.p, sh.data(), count * 8, hipMemcpyHostToDevice, s));
+        esl_status rc = run(cold, 0, kItl2);
+        if (rc == ESL_OK) rc = run(warm, 1, kItl2);
+        if (rc == ESL_OK) rc = run(full, 1, o.tol.max_iter);
         if (rc != ESL_OK) return rc;
-        for (u32 l : clean) {
-            Lad &d = lad[l];
-            d.total += dev_its[l];
-            d.phase = 2;
-            if (st[l] != esl::kConverged) st[l] = esl::kNoConvergence;
-        }
-        for (const vector<u32> *g : {&cold, &warm})
+        for (const vector<u32> *g : {&cold, &warm, &full})
             for (u32 l : *g) {
                 Lad &d = lad[l];
                 const u32 it = st[l] == esl::kSingular && its[l] ? its[l] : dev_its[l];
+                const bool ok = st[l] == esl::kConverged;
                 d.total += it;
-                d.solves++;
-                if (st[l] == esl::kConverged) {
-                    if (d.gv <= kTarget) {
-                        d.phase = 1;   // the clean run, from this point
-                        continue;
-                    }
-                    HIPCHK(hipMemcpyAsync(xg + (size_t)l * n, xd + (size_t)l * n, n * 8, hipMemcpyDeviceToDevice, s));
-                    d.have_good = true;
-                    d.good = d.gv;
-                    if (it <= kItl2 / 4) d.factor = std::min(d.factor * std::sqrt(d.factor), 10.0);
-                    else if (it > 3 * kItl2 / 4) d.factor = std::max(std::sqrt(d.factor), 1.00005);
-                    if (d.gv < d.factor * kTarget) {
-                        d.factor = d.gv / kTarget;
-                        d.gv = kTarget;
+                switch (d.phase) {
+                case kGminRung:
+                    d.solves++;
+                    if (ok) {
+                        if (d.gv <= kTarget) {
+                            d.phase = kGminClean;   // the clean run, from this point
+                            break;
+                        }
+                        HIPCHK(save_good(l));
+                        d.have_good = true;
+                        d.good = d.gv;
+                        if (it <= kItl2 / 4) d.factor = std::min(d.factor * std::sqrt(d.factor), 10.0);
+                        else if (it > 3 * kItl2 / 4) d.factor = std::max(std::sqrt(d.factor), 1.00005);
+                        if (d.gv < d.factor * kTarget) {
+                            d.factor = d.gv / kTarget;
+                            d.gv = kTarget;
+                        } else {
+                            d.gv /= d.factor;
+                        }
+                        d.cold = false;
                     } else {
-                        d.gv /= d.factor;
+                        if (d.factor < 1.00005) {
+                            d.phase = kSrcRung;
+                            d.solves = 0;
+                            break;
+                        }
+                        d.factor = std::sqrt(std::sqrt(d.factor));
+                        d.gv = d.good / d.factor;
+                        if (d.have_good) HIPCHK(back_to_good(l));
+                        d.cold = !d.have_good;
                     }
-                    d.cold = false;
-                } else {
-                    if (d.factor < 1.00005) {
-                        d.phase = 2;
-                        st[l] = esl::kNoConvergence;
-                        continue;
+                    if (d.phase == kGminRung && d.solves >= 100) {
+                        d.phase = kSrcRung;
+                        d.solves = 0;
                     }
-                    d.factor = std::sqrt(std::sqrt(d.factor));
-                    d.gv = d.good / d.factor;
-                    if (d.have_good)
-                        HIPCHK(hipMemcpyAsync(xd + (size_t)l * n, xg + (size_t)l * n, n * 8, hipMemcpyDeviceToDevice,
-                                              s));
-                    d.cold = !d.have_good;
-                }
-                if (d.phase == 0 && d.solves >= 100) {
-                    d.phase = 2;
-                    st[l] = esl::kNoConvergence;
+                    break;
+                case kGminClean:
+                    if (ok) d.phase = kDone;
+                    else {
+                        d.phase = kSrcRung;
+                        d.solves = 0;
+                    }
+                    break;
+                case kSrcRung:
+                    d.solves++;
+                    if (ok) {
+                        if (d.lambda >= 1.0) {
+                            d.phase = kFinal;
+                            break;
+                        }
+                        d.lambda_good = d.lambda;
+                        HIPCHK(save_good(l));
+                        d.delta *= 1.5;
+                        d.lambda = std::min(d.lambda + d.delta, 1.0);
+                    } else {
+                        d.delta *= 0.5;
+                        if (d.delta < 1e-4 || d.lambda_good < 0.0) {
+                            d.phase = kFinal;   // from where the last run stopped
+                            break;
+                        }
+                        HIPCHK(back_to_good(l));
+                        d.lambda = std::min(d.lambda_good + d.delta, 1.0);
+                    }
+                    if (d.solves >= 100) d.phase = kFinal;
+                    break;
+                case kFinal:
+                    d.phase = kDone;
+                    if (!ok) st[l] = esl::kNoConvergence;
+                    break;
                 }
             }
     }
@@ -2716,7 +2764,7 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
             HostOpArgs ao = a;
             ao.priv = own;
             // the ladder continues each lane on the LU it ends on here
-            ao.keep_tape = (o.flags & ESL_OP_GMIN_STEPPING) ? 1u : 0u;
+            ao.keep_tape = (o.flags & ESL_OP_LADDER) ? 1u : 0u;
             esl_status r = launch_lpw(*e, ao, (u32)again.size(), 2, cap, ms);
             if (r != ESL_OK) return r;
             kernel_ms += ms;
@@ -2730,8 +2778,8 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
         esl_status r = op_repivot(*e, a, count, st, its, lane64, kernel_ms);
         if (r != ESL_OK) return r;
     }
-    if ((o.flags & ESL_OP_GMIN_STEPPING) && !fp32 && !net && !e->big) {
-        esl_status r = op_gmin_ladder(*e, a, own, count, o, st, its, lane64, kernel_ms);
+    if ((o.flags & ESL_OP_LADDER) && !fp32 && !net && !e->big) {
+        esl_status r = op_ladder(*e, a, own, count, o, st, its, lane64, kernel_ms);
         if (r != ESL_OK) return r;
     }
     const auto t_dl0 = std::chrono::steady_clock::now();

@@ -35,7 +35,7 @@ namespace esl {
 struct CrewCtl {
     uint32_t cmd, c0, charges, x_off;
     uint32_t st_off, pad;
-    double t;
+    double t, src;
 };
 __shared__ CrewCtl g_crew;
 constexpr uint32_t kCmdEval = 1, kCmdDone = 2;
@@ -46,7 +46,8 @@ struct Crew {
     uint32_t nw, x_off, st_off;
 };
 
-__device__ inline void crew_post(const Crew &cr, uint32_t c0, bool charges, double t, uint32_t tid)
+__device__ inline void crew_post(const Crew &cr, uint32_t c0, bool charges, double t, uint32_t tid,
+                                 double src = 1.0)
 {
     if (tid == 0) {
         g_crew.cmd = kCmdEval;
@@ -55,6 +56,7 @@ __device__ inline void crew_post(const Crew &cr, uint32_t c0, bool charges, doub
         g_crew.x_off = cr.x_off;
         g_crew.st_off = cr.st_off;
         g_crew.t = t;
+        g_crew.src = src;
     }
     __syncthreads();
 }
@@ -80,7 +82,7 @@ __device__ inline void crew_follow(const DeckProg &dp, const double *live, doubl
         const uint32_t k = c.c0 + wv;
         if (k < dp.n_chunks)
             eval_chunk(dp, dp.chunks[k], live, base + c.x_off, lim, dp.n_mos, base + c.st_off + wv * kStageDoubles,
-                       tid, c.t, c.charges != 0);
+                       tid, c.t, c.charges != 0, c.src);
         __syncthreads();
     }
     __syncthreads();
@@ -89,7 +91,8 @@ __device__ inline void crew_follow(const DeckProg &dp, const double *live, doubl
 // Assembly into pattern slots (the Jacobian dump for a host re-pivot): the
 // deck's reduce tasks, one target per thread.
 __device__ void assemble_pattern(const DeckProg &dp, const double *live, const double *x, double *lim,
-                                 uint32_t lim_stride, double *a, double *rhs, double *st, uint32_t tid)
+                                 uint32_t lim_stride, double *a, double *rhs, double *st, uint32_t tid,
+                                 double src = 1.0)
 {
     const uint32_t n = dp.n;
     for (uint32_t i = tid; i < dp.nnz; i += kWave) a[i] = 0.0;
@@ -98,7 +101,7 @@ __device__ void assemble_pattern(const DeckProg &dp, const double *live, const d
     for (uint32_t c = 0; c < dp.n_chunks; c++) {
         const Chunk ch = dp.chunks[c];
         if (ch.kind == kC) continue;   // no static stamps
-        eval_chunk(dp, ch, live, x, lim, lim_stride, st, tid);
+        eval_chunk(dp, ch, live, x, lim, lim_stride, st, tid, 0.0, false, src);
         wsync();
         const uint32_t t1 = dp.chunks[c + 1].task_begin;
         for (uint32_t t = ch.task_begin + tid; t < t1; t += kWave) {
@@ -167,15 +170,16 @@ __device__ inline void red_word(uint32_t w, RedAcc &r, double *tv, double *rhs, 
 // each target's cells in instance order into a register. A crew of nw
 // wavefronts evaluates nw chunks per round.
 __device__ void assemble(const DeckProg &dp, const TapeProg &T, const double *live, const double *x, double *lim,
-                         uint32_t lim_stride, double *tv, double *rhs, double *st, uint32_t tid, const Crew &cr)
+                         uint32_t lim_stride, double *tv, double *rhs, double *st, uint32_t tid, const Crew &cr,
+                         double src = 1.0)
 {
     const uint32_t n = dp.n;
     for (uint32_t i = tid; i < T.n_tv; i += kWave) tv[i] = 0.0;
     for (uint32_t i = tid; i < n; i += kWave) rhs[i] = 0.0;
     wsync();
     for (uint32_t c0 = 0; c0 < dp.n_chunks; c0 += cr.nw) {
-        if (cr.nw > 1) crew_post(cr, c0, false, 0.0, tid);
-        eval_chunk(dp, dp.chunks[c0], live, x, lim, lim_stride, st, tid);   // capacitors stage nothing here
+        if (cr.nw > 1) crew_post(cr, c0, false, 0.0, tid, src);
+        eval_chunk(dp, dp.chunks[c0], live, x, lim, lim_stride, st, tid, 0.0, false, src);   // capacitors stage nothing here
         if (cr.nw > 1) __syncthreads();
         else wsync();
         const uint32_t c1 = c0 + cr.nw < dp.n_chunks ? c0 + cr.nw : dp.n_chunks;
@@ -315,7 +319,7 @@ __global__ __launch_bounds__(kWave) void op_dump(OpArgs a)
     const double *live = a.live + (size_t)lane * dp.n_live;
     double *lim = a.lim + (size_t)lane * lim_width(dp);
     lane_init(a, lane, live, x, lim, tid);
-    assemble_pattern(dp, live, x, lim, nm, tv, rhs, st, tid);
+    assemble_pattern(dp, live, x, lim, nm, tv, rhs, st, tid, a.src ? a.src[lane] : 1.0);
     add_gmin(a, lane, dp.diag_slot, tv, rhs, x, tid);
     for (uint32_t p = tid; p < dp.nnz; p += kWave) a.a_out[(size_t)blockIdx.x * dp.nnz + p] = tv[p];
     for (uint32_t i = tid; i < n; i += kWave) a.f_out[(size_t)blockIdx.x * n + i] = rhs[i];
@@ -348,6 +352,7 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
 
 
     LaneResume rs = a.resume[lane];
+    const double src = a.src ? a.src[lane] : 1.0;   // the source rung's scale
     const bool fp32 = a.fp32 && !a.lane64[lane];
     // FP64 lanes re-pivot on the device, on their own LU, when it is there
     const bool can_priv = kOwnLU && a.priv.cap != 0 && !fp32;
@@ -409,10 +414,10 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
         }
         lap(3);
         if (priv) {
-            assemble_pattern(dp, live, x, lim, nm, pap, rhs, st, tid);
+            assemble_pattern(dp, live, x, lim, nm, pap, rhs, st, tid, src);
             add_gmin(a, lane, dp.diag_slot, pap, rhs, x, tid);
         } else {
-            assemble(dp, T, live, x, lim, nm, tv, rhs, st, tid, cr);
+            assemble(dp, T, live, x, lim, nm, tv, rhs, st, tid, cr, src);
             add_gmin(a, lane, T.diag_tv, tv, rhs, x, tid);
         }
         if (warm == 1) {
@@ -482,7 +487,7 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
                 // re-pivot here: the Jacobian again in pattern order (the same
                 // sums as the failed assembly), factored by the lane
                 wsync();
-                assemble_pattern(dp, live, x, lim, nm, pap, rhs, st, tid);
+                assemble_pattern(dp, live, x, lim, nm, pap, rhs, st, tid, src);
                 add_gmin(a, lane, dp.diag_slot, pap, rhs, x, tid);
                 for (uint32_t i = tid; i < n; i += kWave) tv[i] = 0.0;   // the factor's dense column (LDS)
                 wsync();
@@ -526,7 +531,7 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
                 // re-pivot; the device evaluation reads x and the limiting
                 // state exactly as the failed assembly did
                 wsync();
-                assemble_pattern(dp, live, x, lim, nm, a.a_out + (size_t)lane * dp.nnz, rhs, st, tid);
+                assemble_pattern(dp, live, x, lim, nm, a.a_out + (size_t)lane * dp.nnz, rhs, st, tid, src);
                 add_gmin(a, lane, dp.diag_slot, a.a_out + (size_t)lane * dp.nnz, rhs, x, tid);
             }
             status = exit_status;

@@ -25,7 +25,7 @@ REASON_OK, REASON_NO_CONVERGENCE, REASON_SINGULAR = 0, 1, 3
 FP64, FP32_REFINED = 0, 1
 START_COLD, START_GIVEN, START_LEARNED = 0, 1, 2
 TILING_AUTO, TILING_WAVE, TILING_THREAD, TILING_BLOCK = 0, 1, 2, 3
-OP_HOST_REPIVOT, OP_GMIN_STEPPING = 1, 2
+OP_HOST_REPIVOT, OP_LADDER = 1, 2
 BATCH_OP, BATCH_TRAN = 0, 1
 
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "BUFFER_TOO_SMALL", 3: "OUT_OF_MEMORY",
@@ -264,17 +264,19 @@ class Engine:
 
     def solve_op(self, values, x0=None, precision=FP64, tol: Tolerances | None = None, tiling=TILING_AUTO,
                  refine_tol=None, refine_max=None, learned=False, probes=None, host_repivot=False,
-                 gmin_stepping=False):
+                 ladder=False):
         """Operating points of rows `values` [count][n_live]. Returns dict
         with x [count][n], iterations, reasons. x0 [count][n]: given starts;
         learned=True: starts from the attached network (set_warm_start);
-        probes: unknown indices to return instead of all of x."""
+        probes: unknown indices to return instead of all of x; ladder=True:
+        points the plain rung leaves unconverged climb the .op ladder's gmin
+        and source-stepping rungs (FP64)."""
         values = np.ascontiguousarray(np.atleast_2d(np.asarray(values, dtype=np.float64)))
         count = values.shape[0]
         start = START_LEARNED if learned else (START_COLD if x0 is None else START_GIVEN)
         o = default_options(precision, start)
         o.tiling = tiling
-        o.flags = (OP_HOST_REPIVOT if host_repivot else 0) | (OP_GMIN_STEPPING if gmin_stepping else 0)
+        o.flags = (OP_HOST_REPIVOT if host_repivot else 0) | (OP_LADDER if ladder else 0)
         if refine_tol is not None:
             o.refine_tol = refine_tol
         if refine_max is not None:

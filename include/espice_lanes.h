@@ -179,16 +179,20 @@ typedef struct {
     /* ESL_OP_HOST_REPIVOT: a lane whose replay fails leaves for the host,
      * which factors its Jacobian into a new tape (the default for FP64 is
      * the lane's own factor on the device; both give the same bits).
-     * ESL_OP_GMIN_STEPPING: FP64 cold or given starts that the plain rung
-     * leaves unconverged (or singular) climb the .op ladder's gmin rung
-     * (dc/op.zig:160-222): Newton runs on the device with a conductance g
+     * ESL_OP_LADDER: FP64 cold or given starts that the plain rung leaves
+     * unconverged (or singular) climb the .op ladder (dc/op.zig:160-288),
+     * every run a Newton run on the device: the gmin rung (a conductance g
      * from every node row to ground, g from 1e-3 down to 1e-12 by an
-     * adaptive factor (at most 50 iterations each, a failed run backs up
-     * toward the last good g from the last converged point), then a clean
-     * run without it; the point's iterations are the sum over its runs. */
+     * adaptive factor, at most 50 iterations a run, a failed run backing up
+     * toward the last good g from the last converged point; then a clean
+     * run without it), and when that fails the source rung (every
+     * independent source scaled by lambda, lambda from 0 to 1 in steps that
+     * grow 1.5x after a converged run and halve after a failed one) and a
+     * final run at full sources. The point's iterations are the sum over
+     * its runs. */
     uint32_t flags;
 } esl_op_options;
-enum { ESL_OP_HOST_REPIVOT = 1u, ESL_OP_GMIN_STEPPING = 2u };
+enum { ESL_OP_HOST_REPIVOT = 1u, ESL_OP_LADDER = 2u };
 
 /* Sizes of a compiled engine. */
 typedef struct {

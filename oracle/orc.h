@@ -121,11 +121,16 @@ typedef struct {
 
 typedef struct {
     double reltol, abstol, vntol, residual_tol;
-    uint32_t max_iter, reserved;   /* reserved: flags (ORC_TOL_GMIN_STEPPING) */
+    uint32_t max_iter, reserved;   /* reserved: flags (ORC_TOL_LADDER ...) */
 } orc_tol;                         /* the layout of esl_tolerances */
-/* orc_op_solve*: FP64 cold or given starts that fail the plain rung run the
- * .op ladder's gmin rung (orc_ckt.c gmin_ladder) */
-#define ORC_TOL_GMIN_STEPPING 1u
+/* orc_op_solve*: FP64 cold or given starts that fail the plain rung climb
+ * the .op ladder's gmin and source-stepping rungs (orc_ckt.c op_ladder) */
+#define ORC_TOL_LADDER 1u
+/* for the tests: the ladder also after a converged plain rung, the ladder
+ * without its gmin rung, the ladder without its source rung */
+#define ORC_TOL_LADDER_ALWAYS 2u
+#define ORC_TOL_LADDER_NO_GMIN 4u
+#define ORC_TOL_LADDER_NO_SRC 8u
 
 typedef struct orc_ckt orc_ckt;
 /* NULL on a malformed deck (message in err). */
@@ -142,6 +147,9 @@ void orc_ckt_assemble_cold(orc_ckt *c, const double *live, double *g, double *rh
 /* Assembles at x given (no limiting: a warm start's iteration 0). */
 void orc_ckt_assemble_at(orc_ckt *c, const double *live, const double *x,
                          double *g, double *rhs);
+/* The same with every independent source scaled by src (the source rung). */
+void orc_ckt_assemble_scaled(orc_ckt *c, const double *live, const double *x, double src,
+                             double *g, double *rhs);
 
 /* Operating point of `count` points over `threads` host threads (0 = one).
  * q: the column order every lane's lead factor uses; lead tape from point 0's

@@ -80,6 +80,7 @@ def lib():
         L.orc_ckt_pattern.argtypes = [C.c_void_p, u32p, u32p, u32p, u8p]
         L.orc_ckt_assemble_cold.argtypes = [C.c_void_p, f64p, f64p, f64p]
         L.orc_ckt_assemble_at.argtypes = [C.c_void_p, f64p, f64p, f64p, f64p]
+        L.orc_ckt_assemble_scaled.argtypes = [C.c_void_p, f64p, f64p, C.c_double, f64p, f64p]
         L.orc_op_solve2.restype = C.c_int
         L.orc_op_solve2.argtypes = [C.c_void_p, f64p, C.c_size_t, C.POINTER(Tol), C.POINTER(Prec), u32p,
                                     C.c_void_p, C.c_int, f64p, u32p, u32p, u32p, u32p, u32p]
@@ -235,10 +236,15 @@ class Circuit:
         lib().orc_ckt_assemble_cold(self.h, _f64(live), g, r)
         return g, r
 
-    def assemble_at(self, live, x):
+    def assemble_at(self, live, x, src=None):
+        """Jacobian (pattern slots) and residual at x; src: every
+        independent source scaled by it."""
         g = np.zeros(self.nnz)
         r = np.zeros(self.n)
-        lib().orc_ckt_assemble_at(self.h, _f64(live), _f64(x), g, r)
+        if src is None:
+            lib().orc_ckt_assemble_at(self.h, _f64(live), _f64(x), g, r)
+        else:
+            lib().orc_ckt_assemble_scaled(self.h, _f64(live), _f64(x), float(src), g, r)
         return g, r
 
     def op(self, values, q, start_x=None, threads=1, tol=None, precision=0, refine_tol=1e-12, refine_max=8,

@@ -755,11 +755,16 @@ typedef struct {
     dio_prep *dp;       /* per instance (diodes only used) */
     bjt_prep *bp;       /* per instance (BJTs only used) */
     double *lim;        /* [n_inst][4]: MOS (vgs, vds, vbs, von), diode vd */
+    double src;         /* independent sources scaled by src (the .op ladder's
+                         * source rung; 1 everywhere else): vsource.va:327-329,
+                         * isource.va:326-328 multiply the output by
+                         * $simparam("sourceScaleFactor") */
 } lane_dev;
 
 static void lane_dev_init(const orc_ckt *c, const double *live, lane_dev *ld)
 {
     size_t nd = c->d.n_dev;
+    ld->src = 1.0;
     ld->p = malloc(nd * NPARAM * 8 + 8);
     memcpy(ld->p, c->params, nd * NPARAM * 8);
     for (uint32_t b = 0; b < c->d.n_bind; b++)
@@ -863,7 +868,7 @@ static void assemble_t(const orc_ckt *c, lane_dev *ld, const double *x, double t
         case K_V: {
             const double ib = x[in->br];
             /* waveform (ESL_V_WAVE 1: step to p[2] after p[3]) */
-            const double dc = (p[1] == 1.0 && t > p[3]) ? p[2] : p[0];
+            const double dc = ((p[1] == 1.0 && t > p[3]) ? p[2] : p[0]) * ld->src;
             rhs[ROW(in->t[0])] += ib;
             rhs[ROW(in->t[1])] += -ib;
             rhs[in->br] += (x[in->t[0]] - x[in->t[1]]) - dc;
@@ -873,10 +878,12 @@ static void assemble_t(const orc_ckt *c, lane_dev *ld, const double *x, double t
             g[SLOT(in->slot[3])] += -1.0;
             break;
         }
-        case K_I:
-            rhs[ROW(in->t[0])] += p[0];
-            rhs[ROW(in->t[1])] += -p[0];
+        case K_I: {
+            const double iv = p[0] * ld->src;
+            rhs[ROW(in->t[0])] += iv;
+            rhs[ROW(in->t[1])] += -iv;
             break;
+        }
         case K_D: {
             double o[12] = {0};
             dio1_eval(&ld->dp[i], x[in->t[0]], x[in->t[1]], ld->lim[(size_t)i * 4], charges, o);
@@ -948,8 +955,14 @@ void orc_ckt_assemble_cold(orc_ckt *c, const double *live, double *g, double *rh
 
 void orc_ckt_assemble_at(orc_ckt *c, const double *live, const double *x, double *g, double *rhs)
 {
+    orc_ckt_assemble_scaled(c, live, x, 1.0, g, rhs);
+}
+
+void orc_ckt_assemble_scaled(orc_ckt *c, const double *live, const double *x, double src, double *g, double *rhs)
+{
     lane_dev ld;
     lane_dev_init(c, live, &ld);
+    ld.src = src;
     seed_at(c, &ld, x);
     double *gg = malloc((size_t)(c->nnz + 1) * 8), *rr = malloc((size_t)(c->n + 1) * 8);
     assemble(c, &ld, x, gg, rr);
@@ -1127,25 +1140,43 @@ static uint32_t newton_run(const orc_ckt *c, lane_dev *ld, orc_lu *lu, const uin
     return 1;
 }
 
-/* The gmin rung and the clean run after a failed plain rung of `plain`
- * iterations; x is the lane's state (overwritten). Returns the reason (0
- * or 1) and the lane's total iterations in *total. */
-static uint32_t gmin_ladder(const orc_ckt *c, lane_dev *ld, orc_lu *lu, const uint32_t *q, const orc_tol *tol,
-                            double *x, newton_ws *w, uint32_t plain, uint32_t *total, uint32_t *repiv)
+/* The ladder after a failed plain rung of `plain` iterations (the oracle's
+ * tests can also run it after a converged one, without the gmin rung, or
+ * without the source rung and final run: ORC_TOL_LADDER_ALWAYS,
+ * ORC_TOL_LADDER_NO_GMIN, ORC_TOL_LADDER_NO_SRC); x is the
+ * lane's state (overwritten). Returns the reason (0 or 1) and the lane's
+ * total iterations in *total.
+ *
+ * Rung 2, dynamic gmin stepping (dc/op.zig:160-222): see above; a clean run
+ * that converges ends the ladder.
+ * Rung 3, source stepping (dc/op.zig:228-278): from a cold start, Newton
+ * runs of at most itl2 iterations with every independent source scaled by
+ * lambda, lambda from 0 by a step of 0.25 that grows 1.5x after each
+ * converged run and halves after a failed one (the run retried from the
+ * last converged lambda and point); it stops at lambda = 1 converged, when
+ * the step falls below 1e-4, after 100 runs, or when the cold lambda = 0
+ * run fails. Then a final run at full sources from where it stopped
+ * (dc/op.zig:279-288); that run's outcome is the point's. */
+static uint32_t op_ladder(const orc_ckt *c, lane_dev *ld, orc_lu *lu, const uint32_t *q, const orc_tol *tol,
+                          double *x, newton_ws *w, uint32_t plain, uint32_t *total, uint32_t *repiv)
 {
     const uint32_t n = c->n;
     double factor = 10.0, good = ORC_GMIN_START, gv = good / factor;
     int have_good = 0, cold = 1;
     uint32_t tot = plain, its = 0;
-    for (uint32_t solves = 0; solves < 100; solves++) {
+    const uint32_t gmin_runs = (tol->reserved & ORC_TOL_LADDER_NO_GMIN) ? 0u : 100u;
+    for (uint32_t solves = 0; solves < gmin_runs; solves++) {
         const uint32_t r = newton_run(c, ld, lu, q, tol, ORC_ITL2, gv, cold, x, w, &its, repiv);
         tot += its;
         if (r == 0) {
             if (gv <= ORC_GMIN_TARGET) {
                 const uint32_t rc = newton_run(c, ld, lu, q, tol, tol->max_iter, 0.0, 0, x, w, &its, repiv);
                 tot += its;
-                *total = tot;
-                return rc == 0 ? 0 : 1;
+                if (rc == 0) {
+                    *total = tot;
+                    return 0;
+                }
+                break;
             }
             memcpy(w->x_good, x, (size_t)n * 8);
             have_good = 1;
@@ -1167,8 +1198,35 @@ static uint32_t gmin_ladder(const orc_ckt *c, lane_dev *ld, orc_lu *lu, const ui
             cold = !have_good;
         }
     }
+    if (tol->reserved & ORC_TOL_LADDER_NO_SRC) {
+        *total = tot;
+        return 1;
+    }
+    /* rung 3: source stepping */
+    double lambda = 0.0, lambda_good = -1.0, delta = 0.25;
+    for (uint32_t solves = 0; solves < 100; solves++) {
+        ld->src = lambda;
+        const uint32_t r = newton_run(c, ld, lu, q, tol, ORC_ITL2, 0.0, lambda_good < 0.0, x, w, &its, repiv);
+        tot += its;
+        if (r == 0) {
+            if (lambda >= 1.0) break;
+            lambda_good = lambda;
+            memcpy(w->x_good, x, (size_t)n * 8);
+            delta *= 1.5;
+            lambda = fmin(lambda + delta, 1.0);
+        } else {
+            delta *= 0.5;
+            if (delta < 1e-4) break;
+            if (lambda_good < 0.0) break;
+            memcpy(x, w->x_good, (size_t)n * 8);
+            lambda = fmin(lambda_good + delta, 1.0);
+        }
+    }
+    ld->src = 1.0;
+    const uint32_t rf = newton_run(c, ld, lu, q, tol, tol->max_iter, 0.0, 0, x, w, &its, repiv);
+    tot += its;
     *total = tot;
-    return 1;
+    return rf == 0 ? 0 : 1;
 }
 
 /* one lane's Newton (converger.newtonLanes at W = 1 over a lane solver) */
@@ -1327,9 +1385,10 @@ static void solve_lane(const job_t *j, size_t l)
         }
     }
     if (reason == 1) iters = iter;
-    if (reason != 0 && (tol->reserved & ORC_TOL_GMIN_STEPPING) && j->prec.precision == 0 && !j->prec.nominal_x) {
+    const int ladder = (tol->reserved & ORC_TOL_LADDER) && (reason != 0 || (tol->reserved & ORC_TOL_LADDER_ALWAYS));
+    if (ladder && j->prec.precision == 0 && !j->prec.nominal_x) {
         newton_ws w = {x_old, dx, g, rhs, sdiag, calloc(n + 1, 8)};
-        reason = gmin_ladder(c, &ld, lu, j->q, tol, x, &w, iters, &iters, &repiv);
+        reason = op_ladder(c, &ld, lu, j->q, tol, x, &w, iters, &iters, &repiv);
         free(w.x_good);
     }
     memcpy(j->x + l * n, x, (size_t)n * 8);

@@ -288,21 +288,27 @@ def test_diffpair_array_fp64_and_fp32_bitwise_and_within_contract(lanes, oracle)
 
 
 @pytest.mark.parametrize("stages,host_repivot", [(64, False), (128, False), (128, True)])
-def test_gmin_ladder_matches_oracle_bitwise(lanes, oracle, stages, host_repivot):
+def test_ladder_matches_oracle_bitwise(lanes, oracle, stages, host_repivot):
     """Long inverter chains that plain Newton leaves unconverged climb the
-    .op ladder's gmin rung: every rung runs on the device, the control on
-    the host; the oracle's gmin_ladder must give the same bits, iterations
-    (summed over the runs) and reasons."""
+    .op ladder (the gmin rung, then source stepping and a final run): every
+    run on the device, the control on the host; the oracle's op_ladder must
+    give the same bits, iterations (summed over the runs) and reasons. Some
+    points of each case go past the gmin rung (the oracle without its source
+    rung ends them differently), so both rungs are compared."""
     fam = circuits.inverter_chain(stages)
     eng = lanes.Engine(fam.deck.arrays())
     _, _, q = eng.pattern()
     vals = fam.sample(5, 32)
     plain = eng.solve_op(vals, tiling=lanes.TILING_WAVE, host_repivot=host_repivot)
-    got = eng.solve_op(vals, tiling=lanes.TILING_WAVE, host_repivot=host_repivot, gmin_stepping=True)
-    ref = oracle.Circuit(fam.deck.arrays()).op(vals, q, threads=8, tol=oracle.Tol(1e-3, 1e-12, 1e-6, 1e-9, 100, 1))
+    got = eng.solve_op(vals, tiling=lanes.TILING_WAVE, host_repivot=host_repivot, ladder=True)
+    c = oracle.Circuit(fam.deck.arrays())
+    ref = c.op(vals, q, threads=8, tol=oracle.Tol(1e-3, 1e-12, 1e-6, 1e-9, 100, 1))
+    gmin_only = c.op(vals, q, threads=8, tol=oracle.Tol(1e-3, 1e-12, 1e-6, 1e-9, 100, 1 | 8))
+    past_gmin = ref["iterations"] != gmin_only["iterations"]
     tm = eng.timing()
     say(f"  chain {stages}: plain {int((plain['reasons'] == 0).sum())}/32, ladder {int((got['reasons'] == 0).sum())}/32, "
-        f"launches {tm.launches}")
+        f"past the gmin rung {int(past_gmin.sum())}, launches {tm.launches}")
+    assert past_gmin.any()
     same_x = [bool(np.array_equal(bits(a), bits(b))) for a, b in zip(got["x"], ref["x"])]
     for l in np.nonzero((got["iterations"] != ref["iterations"]) | ~np.array(same_x))[0]:
         say(f"  lane {l}: reasons {got['reasons'][l]}/{ref['reasons'][l]} iterations {got['iterations'][l]}/"
