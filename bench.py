@@ -89,21 +89,25 @@ def calibrate_copy(mib):
     print(json.dumps({"copy_bytes_read": mib << 20, "copy_bytes_written": mib << 20}))
 
 
-def measured_traffic(mode, points):
-    """HBM bytes per op_newton launch from the committed rocprofv3 counter
-    summary for this workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or
-    None when no summary covers it."""
-    if mode != "learned_fp32" or points != 16384:
-        return None
-    path = os.path.join(ROOT, "profiles", "op_newton_ota_array_64_learned_fp32_hbm_traffic.md")
+def profile_traffic(name):
+    """traffic_bytes_per_launch of a committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
     try:
-        with open(path) as f:
+        with open(os.path.join(ROOT, "profiles", name)) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return int(line.split(":", 1)[1])
     except OSError:
         return None
     return None
+
+
+def measured_traffic(mode, points):
+    """HBM bytes per op_newton launch from the committed counter summary for
+    this workload, or None when no summary covers it."""
+    if mode != "learned_fp32" or points != 16384:
+        return None
+    return profile_traffic("op_newton_ota_array_64_learned_fp32_hbm_traffic.md")
 
 
 def train_network(lanes, fam, eng, x_nom, rows, epochs):
@@ -387,6 +391,15 @@ def run_mesh(args):
                "kind": "port", "sample": f"the first {k} points of the same workload, every step, on the oracle "
                                          f"(one thread per point, {k} threads, {t_cpu:.1f} s)"}
         log(f"oracle: {k} points in {t_cpu:.1f} s")
+    # measured for launches of one step over 4096 points (two Newton
+    # iterations each); a launch here runs steps_per_launch steps
+    per_step = profile_traffic("big_lanes_rc_mesh_64_hbm_traffic.md")
+    traffic, traffic_note = None, None
+    if per_step is not None:
+        spl = min(args.mesh_steps_per_launch, steps)
+        traffic = int(per_step * spl * len(vals) / 4096)
+        traffic_note = (f"counters of one-step launches over 4096 points ({per_step} B), "
+                        f"times {spl} steps per launch and {len(vals)}/4096 points")
     out = {
         "metric": "converged sweep points/sec, 1 MI355X, 64x64 RC mesh 1000-step transient",
         "value": round(conv / wall, 3),
@@ -407,8 +420,8 @@ def run_mesh(args):
         "kernel_ms_per_step": round(kern / args.steps, 1),
         "transient_iterations_per_point": round(lane_it / max(conv, 1) / args.steps, 2),
         "roofline": {"bound": "hbm", "kernel": "big_lanes", "achieved": round(achieved, 2), "peak": 8000.0,
-                     "unit": "GB/s", "frac": round(achieved / 8000.0, 5), "traffic": None,
-                     "bytes_per_lane_iteration": per_it},
+                     "unit": "GB/s", "frac": round(achieved / 8000.0, 5), "traffic": traffic,
+                     "bytes_per_lane_iteration": per_it, "traffic_note": traffic_note},
         "parity": parity,
         "cpu_baseline": cpu,
     }
