@@ -68,9 +68,12 @@ def parse():
     ap.add_argument("--mesh-lanes", type=int, default=4096, help="rc_mesh: points")
     ap.add_argument("--mesh-steps", type=int, default=1000, help="rc_mesh: trapezoid steps of 1 ps")
     ap.add_argument("--mesh-steps-per-launch", type=int, default=10, help="rc_mesh: steps per kernel launch")
-    ap.add_argument("--workload", default="ota_array", choices=["ota_array", "mixed_batch", "rc_mesh"],
+    ap.add_argument("--workload", default="ota_array",
+                    choices=["ota_array", "mixed_batch", "rc_mesh", "diffpair_array"],
                     help="ota_array: the headline; mixed_batch: 10,000 uneven instances of eight families "
-                         "in one persistent launch")
+                         "in one persistent launch; diffpair_array: 2048 points of 64 Gummel-Poon pairs, "
+                         "FP32 factors + FP64 refinement against plain FP64")
+    ap.add_argument("--pair-points", type=int, default=2048, help="diffpair_array: points")
     ap.add_argument("--instances", type=int, default=10000)
     ap.add_argument("--families", default="", help="mixed_batch: comma-separated subset of the eight families")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -141,6 +144,9 @@ def main():
         return
     if args.workload == "mixed_batch":
         run_mixed(args)
+        return
+    if args.workload == "diffpair_array":
+        run_pairs(args)
         return
     from espice_amd import circuits, lanes
 
@@ -422,6 +428,104 @@ def run_mesh(args):
         "roofline": {"bound": "hbm", "kernel": "big_lanes", "achieved": round(achieved, 2), "peak": 8000.0,
                      "unit": "GB/s", "frac": round(achieved / 8000.0, 5), "traffic": traffic,
                      "bytes_per_lane_iteration": per_it, "traffic_note": traffic_note},
+        "parity": parity,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    eng.close()
+
+
+def run_pairs(args):
+    """64 Gummel-Poon differential pairs (128 BJTs, 453 unknowns), 2048
+    Monte Carlo points per step, FP32 factors with FP64 refinement against
+    plain FP64 lanes, both from a cold start."""
+    from espice_amd import circuits, lanes
+    fam = circuits.diffpair_array(64)
+    deck = fam.deck
+    n_pts = args.pair_points
+    vals = fam.sample(2032, n_pts)
+    eng = lanes.Engine(deck.arrays())
+    _, _, q = eng.pattern()
+    probes = np.array([deck.node_index(f"c{s}{k}") for k in range(64) for s in "pn"], dtype=np.uint32)
+
+    def timed(precision):
+        for _ in range(args.warmup):
+            eng.solve_op(vals, precision=precision, tiling=lanes.TILING_WAVE, probes=probes)
+        t_k, conv, lane_it, fb = 0.0, 0, 0, 0
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            r = eng.solve_op(vals, precision=precision, tiling=lanes.TILING_WAVE, probes=probes)
+            tm = eng.timing()
+            t_k += tm.kernel_ms
+            conv += int((r["reasons"] == 0).sum())
+            lane_it += int(tm.lane_iterations)
+            fb += int(tm.fp64_lanes)
+        wall = time.perf_counter() - t0
+        return {"points_per_s": round(conv / wall, 1), "ms_per_step": round(1e3 * wall / args.steps, 3),
+                "kernel_ms_per_step": round(t_k / args.steps, 3),
+                "newton_iterations_per_point": round(lane_it / (n_pts * args.steps), 3),
+                "fp64_fallback_rate": fb / (n_pts * args.steps), "lane_iterations": lane_it, "kernel_ms": t_k}
+
+    f64 = timed(lanes.FP64)
+    f32 = timed(lanes.FP32_REFINED)
+    a = eng.solve_op(vals, tiling=lanes.TILING_WAVE)
+    b = eng.solve_op(vals, precision=lanes.FP32_REFINED, tiling=lanes.TILING_WAVE)
+    err = float((np.abs(a["x"] - b["x"]) / np.maximum(np.abs(a["x"]), 1e-6)).max())
+    info = eng.info()
+    # LU and vector work per lane-iteration; the Gummel-Poon evaluation is
+    # not counted, so the figure is a lower bound
+    flops_iter = 2 * info.flops + info.nnz_l + 2 * (info.nnz_l + info.nnz_u) + 4 * info.n
+    achieved = flops_iter * f32["lane_iterations"] / (f32["kernel_ms"] * 1e-3) / 1e12
+    parity, cpu = None, None
+    if not args.no_cpu:
+        from oracle import orc
+        c = orc.Circuit(deck.arrays())
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        pick = np.arange(0, n_pts, 97)
+        r64 = c.op(vals[pick], q, threads=threads)
+        r32 = c.op(vals[pick], q, threads=threads, precision=1)
+        parity = {"sample": len(pick),
+                  "fp64_bitwise_equal_oracle": bool(np.array_equal(a["x"][pick].view(np.uint64),
+                                                                   r64["x"].view(np.uint64))),
+                  "fp32_refined_bitwise_equal_oracle_fp32": bool(np.array_equal(b["x"][pick].view(np.uint64),
+                                                                               r32["x"].view(np.uint64))),
+                  "max_rel_err_fp32_refined_vs_fp64": err}
+        # the sample solved again until at least a second of wall time
+        k = min(args.cpu_sample, n_pts)
+        t0 = time.perf_counter()
+        done, rounds = 0, 0
+        while rounds == 0 or time.perf_counter() - t0 < 1.0:
+            rc = c.op(vals[:k], q, threads=threads)
+            done += int((rc["reasons"] == 0).sum())
+            rounds += 1
+        t_cpu = time.perf_counter() - t0
+        cpu = {"value": round(done / t_cpu, 1), "unit": "points/s", "cores": threads,
+               "kind": "port", "sample": f"{k} points of the same workload solved {rounds} times, plain FP64 "
+                                         f"cold-start lanes of the oracle on {threads} host threads, "
+                                         f"{t_cpu:.2f} s wall"}
+    out = {
+        "metric": "converged sweep points/sec, 1 MI355X, 64 Gummel-Poon pairs .op, FP32 LU + FP64 refinement",
+        "value": f32["points_per_s"],
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": f32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded IS, BF and load draws per point)",
+        "config": {"workload": f"diffpair_array_64 .op Monte Carlo, {n_pts} points per step, cold start",
+                   "n": int(info.n), "nnz_l": int(info.nnz_l), "nnz_u": int(info.nnz_u), "flops_F": int(info.flops),
+                   "tiling": "lane-per-wavefront, factors in LDS"},
+        "fp32_refined": {k: v for k, v in f32.items() if k not in ("lane_iterations", "kernel_ms")},
+        "fp64": {k: v for k, v in f64.items() if k not in ("lane_iterations", "kernel_ms")},
+        "fp32_refined_over_fp64": round(f32["points_per_s"] / max(f64["points_per_s"], 1e-9), 3),
+        "roofline": {"bound": "mfma", "pipe": "vector ALU (FP64 vector peak 78.6 TFLOP/s; LU and vector flops "
+                     "only, the Gummel-Poon evaluation not counted)", "kernel": "op_newton",
+                     "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP64_TFLOPS, 5), "traffic": None},
         "parity": parity,
         "cpu_baseline": cpu,
     }
