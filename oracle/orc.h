@@ -2,7 +2,8 @@
  *
  * TEST INFRASTRUCTURE ONLY. Nothing in the espice_amd package links, loads
  * or calls this library; only tests/, __graft_entry__.smoke() and the CPU
- * baseline leg of bench.py do, and only as the checker.
+ * baseline leg of bench.py do, to check results against and to time the
+ * CPU path.
  *
  * Restates (file:line in OmarSiwy/ESPice 1.2.0):
  *   - SparseLu.factor / refactorColumns / solve (src/solver/sparse_lu.zig:

@@ -1,8 +1,8 @@
 """ctypes binding of the oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
-CPU-baseline leg of bench.py as the checker. The espice_amd package never
-imports it.
+CPU-baseline leg of bench.py, to check results against and to time the CPU
+path. The espice_amd package never imports it.
 """
 from __future__ import annotations
 

@@ -1,6 +1,6 @@
 """Writes tests/golden/ngspice_mos1.json from the reference's own fixtures.
 
-Run in the build container (the reference tree is not on the GPU box):
+Run where the reference tree is present (the tests only read the JSON):
     python tests/golden/make_golden.py
 
 Each case pairs a deck, transcribed by hand below from the fixture's `.sp`
