@@ -95,6 +95,10 @@ struct DeckProg {
     // the pattern by columns ([n + 1], [nnz] rows) and the column order
     // of the factors: a lane's own re-pivot reads them
     const uint32_t *csc_ptr, *csc_row, *colq;
+    // pattern slot of each Jacobian entry (kNoSlot for a ground entry):
+    // diodes [4 * n_dio] (a-a, a-k, k-a, k-k), Gummel-Poon [12 * n_bjt] in
+    // the entry order of their evaluation (the lane-per-thread kernel)
+    const uint32_t *d_slot, *q_slot;
 };
 constexpr uint32_t kNoSlot = 0xffffffffu;
 

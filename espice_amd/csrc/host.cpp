@@ -950,7 +950,7 @@ struct Compiled {
     vector<u32> r_t, v_t, i_t, m_t;
     vector<double> r_val, v_val, i_val, m_par;
     vector<int32_t> r_bind, v_bind, i_bind, m_bind_vto, m_bind_kp;
-    vector<u32> r_slot, v_slot, m_slot;
+    vector<u32> r_slot, v_slot, m_slot, d_slot, q_slot;
     vector<u32> csr_ptr, csr_col, csr_slot;   // the pattern by rows, columns ascending
     vector<u32> d_t, c_t, v_wkind, cslot_pat, q_t;
     vector<double> d_par, c_val, v_v2, v_td, q_par;
@@ -1211,10 +1211,12 @@ bool compile_deck(const esl_deck *d, Compiled &c, std::string &err)
             break;
         case esl::kD:
             c.d_t.push_back(in.t[0]); c.d_t.push_back(in.t[1]);
+            for (int e = 0; e < 4; e++) c.d_slot.push_back(in.slot[e]);
             c.d_bind_is.push_back(bd[ESL_D_IS]);
             break;
         case esl::kQ:
             for (int t = 0; t < 4; t++) c.q_t.push_back(in.t[t]);
+            for (int e = 0; e < 12; e++) c.q_slot.push_back(in.slot[e]);
             c.q_bind_is.push_back(bd[ESL_Q_IS]);
             c.q_bind_bf.push_back(bd[ESL_Q_BF]);
             break;
@@ -1407,6 +1409,7 @@ hipError_t upload_deck(Engine &e)
            o_vtd = pk.add(c.v_td), o_csp = pk.add(c.cslot_pat), o_qt = pk.add(c.q_t), o_qp = pk.add(c.q_par),
            o_qbi = pk.add(c.q_bind_is), o_qbb = pk.add(c.q_bind_bf);
     size_t o_ccp = pk.add(c.col_ptr), o_ccr = pk.add(c.row_idx), o_cq = pk.add(c.q);
+    size_t o_ds = pk.add(c.d_slot), o_qs = pk.add(c.q_slot);
     hipError_t err = e.deck_buf.ensure(pk.blob.size());
     if (err != hipSuccess) return err;
     err = hipMemcpy(e.deck_buf.p, pk.blob.data(), pk.blob.size(), hipMemcpyHostToDevice);
@@ -1443,6 +1446,8 @@ hipError_t upload_deck(Engine &e)
     dp.r_slot = reinterpret_cast<const u32 *>(b + o_rs);
     dp.v_slot = reinterpret_cast<const u32 *>(b + o_vs);
     dp.m_slot = reinterpret_cast<const u32 *>(b + o_ms);
+    dp.d_slot = reinterpret_cast<const u32 *>(b + o_ds);
+    dp.q_slot = reinterpret_cast<const u32 *>(b + o_qs);
     dp.csr_ptr = reinterpret_cast<const u32 *>(b + o_cp);
     dp.csr_col = reinterpret_cast<const u32 *>(b + o_cc);
     dp.csr_slot = reinterpret_cast<const u32 *>(b + o_cs);
@@ -1728,7 +1733,10 @@ bool validate_deck(const Compiled &c)
         for (u32 v : s) if (v != NONE && v >= nnz) return false;
         return true;
     };
-    if (!slots_ok(c.r_slot) || !slots_ok(c.v_slot) || !slots_ok(c.m_slot)) return false;
+    if (!slots_ok(c.r_slot) || !slots_ok(c.v_slot) || !slots_ok(c.m_slot) || !slots_ok(c.d_slot) ||
+        !slots_ok(c.q_slot))
+        return false;
+    if (c.d_slot.size() != 4 * (size_t)c.n_dio || c.q_slot.size() != 12 * (size_t)c.n_bjt) return false;
     return binds_ok(c.r_bind) && binds_ok(c.v_bind) && binds_ok(c.i_bind) && binds_ok(c.m_bind_vto) &&
            binds_ok(c.m_bind_kp) && binds_ok(c.d_bind_is) && binds_ok(c.c_bind) && binds_ok(c.q_bind_is) &&
            binds_ok(c.q_bind_bf);
@@ -1747,9 +1755,10 @@ u32 device_lds_max()
 
 // The host half of registering a tape: its row programs and reduce
 // streams, checked. No device calls, so it runs on worker threads.
-// A tape of an engine whose lanes run one per thread in HBM (big), or the
+// A tape of an engine whose lanes keep their factors in HBM (big), or the
 // first tape of an engine that does not fit a workgroup's LDS (which makes
-// the engine big), keeps only the column replay.
+// the engine big), gets replay blocks and solve chunks instead of whole-tape
+// row programs and reduce streams.
 TapeHostProg *build_tape(const Engine &e, const HostTape &t, const char *&err)
 {
     auto *tp = new TapeHostProg();
@@ -2126,7 +2135,7 @@ esl_status launch_lpt(Engine &e, const HostOpArgs &a, u32 lanes, u32 mode, float
     HIPCHK(e.xoldg.ensure(N * c.n * 8));
     HIPCHK(e.yg.ensure(N * c.n * 8));
     HIPCHK(e.sdiagg.ensure(N * c.n * 8));
-    HIPCHK(e.limg.ensure(std::max<size_t>(1, N * 4 * c.n_mos) * 8));
+    HIPCHK(e.limg.ensure(std::max<size_t>(1, N * (4 * c.n_mos + c.n_dio + 2 * c.n_bjt)) * 8));
     esl::LptArgs L{};
     L.dp = a.dp;
     L.T = tp.prog;
@@ -2707,13 +2716,13 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
     // tilings once per engine on the first batch of at least 1024 points
     // (kernel time of the first launch over every lane) and keeps the faster;
     // below that size a batch runs one wavefront per lane.
-    // the thread tiling covers R, V, I and MOS1 decks in FP64
-    const bool wave_only = fp32 || net || c.n_dio > 0 || c.n_bjt > 0;
+    // the thread tiling runs FP64 lanes from cold or given starts
+    const bool wave_only = fp32 || net;
     if (wave_only && o.tiling == ESL_TILING_THREAD) return ESL_INVALID_ARGUMENT;
     u32 tiling = wave_only ? (u32)ESL_TILING_WAVE : o.tiling;
     if (e->big) {
         // lanes beyond a workgroup's LDS: FP64 from a cold or given start,
-        // one thread per lane
+        // one wavefront per lane with the factors in HBM
         if (fp32 || net) {
             g_last_error = "FP32 factors and learned starts need lanes that fit a workgroup's LDS";
             return ESL_INVALID_ARGUMENT;
@@ -2736,7 +2745,7 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
             tiling = ESL_TILING_WAVE;
         }
     }
-    e->timing.tiling = e->big ? (u32)ESL_TILING_THREAD : tiling;
+    e->timing.tiling = tiling;
 
     vector<u32> st(count), its(count);
     float kernel_ms = 0, ms = 0;

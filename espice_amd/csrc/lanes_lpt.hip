@@ -54,6 +54,22 @@ __global__ __launch_bounds__(256) void op_newton_lpt(LptArgs a)
         }
         V(lim, 3 * nm + j) = 0.0;
     }
+    const uint32_t nd = dp.n_dio, nq = dp.n_bjt, lb = 4 * nm + nd;   // diode, BJT limiting state
+    for (uint32_t j = 0; j < nd; j++) {
+        const uint32_t pa = dp.d_t[2 * j], pk = dp.d_t[2 * j + 1];
+        V(lim, 4 * nm + j) = a.mode == 0 ? dio_load(dp, j, live, a.N).vcrit : V(x, pa) - V(x, pk);
+    }
+    for (uint32_t j = 0; j < nq; j++) {
+        const uint32_t *t4 = dp.q_t + 4 * j;
+        if (a.mode == 0) {
+            V(lim, lb + 2 * j) = bjt_load(dp, j, live, a.N).vcrit;
+            V(lim, lb + 2 * j + 1) = 0.0;
+        } else {
+            const double tp = dp.q_par[QF_TYPE * nq + j];
+            V(lim, lb + 2 * j) = tp * (V(x, t4[1]) - V(x, t4[2]));
+            V(lim, lb + 2 * j + 1) = tp * (V(x, t4[1]) - V(x, t4[0]));
+        }
+    }
 
     bool init_fix = a.mode == 0;
     double limit = 1e9;
@@ -107,6 +123,18 @@ __global__ __launch_bounds__(256) void op_newton_lpt(LptArgs a)
             if (p != 0) V(rhs, p) += dc;
             if (q != 0) V(rhs, q) += -dc;
         }
+        for (uint32_t j = 0; j < nd; j++) {
+            const Dio d = dio_load(dp, j, live, a.N);
+            const uint32_t pa = dp.d_t[2 * j], pk = dp.d_t[2 * j + 1];
+            double out[12];
+            dio_eval_v(d, V(x, pa), V(x, pk), V(lim, 4 * nm + j), false, out);
+            for (int e = 0; e < 4; e++) {
+                const uint32_t sl = dp.d_slot[4 * j + e];
+                if (sl != kNoSlot) V(tv, T.amap[sl]) += out[e];
+            }
+            if (pa != 0) V(rhs, pa) += out[4];
+            if (pk != 0) V(rhs, pk) += out[5];
+        }
         for (uint32_t j = 0; j < nm; j++) {
             const Mos m = mos_load(dp, j, live, a.N);
             const uint32_t *t4 = dp.m_t + 4 * j;
@@ -120,6 +148,19 @@ __global__ __launch_bounds__(256) void op_newton_lpt(LptArgs a)
                 const uint32_t sl = dp.m_slot[11 * j + e];
                 if (sl != kNoSlot) V(tv, T.amap[sl]) += out[e];
             }
+        }
+        for (uint32_t j = 0; j < nq; j++) {
+            const Bjt b = bjt_load(dp, j, live, a.N);
+            const uint32_t *t4 = dp.q_t + 4 * j;
+            double out[16];
+            bjt_eval_v(b, V(x, t4[0]), V(x, t4[1]), V(x, t4[2]), V(x, t4[3]), V(lim, lb + 2 * j),
+                       V(lim, lb + 2 * j + 1), out);
+            for (int e = 0; e < 12; e++) {
+                const uint32_t sl = dp.q_slot[12 * j + e];
+                if (sl != kNoSlot) V(tv, T.amap[sl]) += out[e];
+            }
+            for (int r = 0; r < 4; r++)
+                if (t4[r] != 0) V(rhs, t4[r]) += out[12 + r];
         }
         V(tv, T.diag_tv[0]) += 1.0;
         V(rhs, 0) += V(x, 0);
@@ -195,7 +236,8 @@ __global__ __launch_bounds__(256) void op_newton_lpt(LptArgs a)
         if (bad || nonfinite != 0 || !(lbitsd(worst) <= T.wide)) {
             // hand the lane to a host re-pivot, in the wavefront kernel's layout
             for (uint32_t i = 0; i < n; i++) a.x[(size_t)lane * n + i] = V(x, i);
-            for (uint32_t k = 0; k < 4 * nm; k++) a.lim[(size_t)lane * 4 * nm + k] = V(lim, k);
+            const uint32_t lw = lb + 2 * nq;   // the wavefront kernel's limiting-state width
+            for (uint32_t k = 0; k < lw; k++) a.lim[(size_t)lane * lw + k] = V(lim, k);
             LaneResume o;
             o.iter = iter;
             o.init_fix = init_fix ? 1u : 0u;
@@ -263,6 +305,20 @@ __global__ __launch_bounds__(256) void op_newton_lpt(LptArgs a)
             V(lim, j) = lgs;
             V(lim, nm + j) = lds;
             V(lim, 2 * nm + j) = lbs;
+        }
+        for (uint32_t j = 0; j < nd; j++) {
+            const Dio d = dio_load(dp, j, live, a.N);
+            double lv = V(lim, 4 * nm + j);
+            limited |= dio_limit_v(d, V(x, dp.d_t[2 * j]), V(x, dp.d_t[2 * j + 1]), lv);
+            V(lim, 4 * nm + j) = lv;
+        }
+        for (uint32_t j = 0; j < nq; j++) {
+            const Bjt b = bjt_load(dp, j, live, a.N);
+            const uint32_t *t4 = dp.q_t + 4 * j;
+            double lbe = V(lim, lb + 2 * j), lbc = V(lim, lb + 2 * j + 1);
+            limited |= bjt_limit_v(b, V(x, t4[0]), V(x, t4[1]), V(x, t4[2]), lbe, lbc);
+            V(lim, lb + 2 * j) = lbe;
+            V(lim, lb + 2 * j + 1) = lbc;
         }
         bool ok = !limited && iter != 0 && scaled < 1.0;
         if (ok) {

@@ -140,7 +140,8 @@ enum {
 enum {
     ESL_TILING_AUTO = 0,
     ESL_TILING_WAVE = 1,    /* one wavefront per lane, factors in LDS */
-    ESL_TILING_THREAD = 2,  /* one thread per lane, state lane-major in HBM */
+    ESL_TILING_THREAD = 2,  /* one thread per lane, state lane-major in HBM;
+                               FP64 from cold or given starts, every model */
     ESL_TILING_BLOCK = 3    /* esl_batch_solve: one workgroup of 4 wavefronts per
                                lane; device evaluation spread over the four */
 };

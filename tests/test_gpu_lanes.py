@@ -70,8 +70,6 @@ def test_tape_replay_bitwise_random_systems(lanes, oracle):
 def test_golden_cases_on_gpu(lanes, oracle, name, tiling):
     case = G.load()[name]
     deck, live, reader = G.build(case)
-    if tiling == 2 and any(k in (4, 6) for k in deck.kind):
-        pytest.skip("the lane-per-thread tiling covers R, C, V, I and MOS1 decks")
     say(f"golden {name} rows={live.shape[0]} tiling={tiling}")
     eng = lanes.Engine(deck.arrays())
     cp, ri, q = eng.pattern()
