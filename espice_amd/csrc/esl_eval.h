@@ -250,16 +250,25 @@ struct LuFlags {
 // M = [factor slots | solve vector], in the factor precision R. Rows run in
 // order within the wavefront (LDS accesses of one wavefront complete in issue
 // order), so a row sees the results of every earlier row. The peel tests are
-// computed in FP64 on the values.
-template <typename R, typename MP>
+// computed in FP64 on the values. kSolve: a solve program (R_MACZ and R_DIV
+// rows), whose R_MACZ and idle rows run one branch-free path: an idle row, or
+// an op whose multiplier is zero, writes its slot's own value back (a row
+// never holds two ops on one slot; idle rows touch slot 0 or pads). The
+// refactor's R_MAC rows keep the plain path, which measured faster there.
+template <typename R, bool kSolve, typename MP>
 __device__ inline void run_row(esl::uint2_t w, MP M, double growth, LuFlags &f)
 {
     const uint32_t kind = __builtin_amdgcn_readfirstlane(esl::row_kind(w.y));
     const uint32_t d = w.x & 0xffffu, b = w.x >> 16, c = w.y & 0xffffu;
-    if (kind == R_MAC) {
+    if (kSolve && kind != R_DIV) {
+        const R v = M[d], A = M[b], B = M[c];
+        const R p = v - A * B;
+        const bool keep = kind == R_NOP || B == R(0);
+        M[d] = keep ? v : p;
+    } else if (!kSolve && kind == R_MAC) {
         const R v = M[d], A = M[b], B = M[c];
         M[d] = v - A * B;
-    } else if (kind == R_MACZ) {
+    } else if (!kSolve && kind == R_MACZ) {
         const R v = M[d], A = M[b], B = M[c];
         if (B != R(0)) M[d] = v - A * B;
     } else if (kind == R_DIV) {
@@ -284,7 +293,7 @@ __device__ inline void run_row(esl::uint2_t w, MP M, double growth, LuFlags &f)
 // word loads are pinned in row order (scheduling barriers) and each refills
 // the register its row just used, so the oldest load completes first and a
 // row waits only for its own word, never for the whole ring.
-template <typename R, typename WP, typename MP>
+template <typename R, bool kSolve = false, typename WP, typename MP>
 __device__ void run_rows(WP words, uint32_t rows, MP M, double growth, LuFlags &f, uint32_t tid)
 {
     const WP w = words + tid;
@@ -298,7 +307,7 @@ __device__ void run_rows(WP words, uint32_t rows, MP M, double growth, LuFlags &
 #pragma unroll
         for (int k = 0; k < kRing; k++) {
             // use the word, then refill its register (no register rotation)
-            run_row<R>(q[k], M, growth, f);
+            run_row<R, kSolve>(q[k], M, growth, f);
             const size_t at = (size_t)(row + kRing + k) * kWave;
             q[k] = esl::uint2_t{w[at].x, w[at].y};
             __builtin_amdgcn_sched_barrier(0);

@@ -248,7 +248,7 @@ __device__ void solve_neg(const TapeProg &T, uint32_t n, double *tv, const doubl
     for (uint32_t r = tid; r < n; r += kWave) y[T.pinv[r]] = -rhs[r];
     wsync();
     LuFlags f{false, 0, 0};
-    run_rows<double>(gptr(T.sw), T.sw_rows, tv, 0.0, f, tid);
+    run_rows<double, true>(gptr(T.sw), T.sw_rows, tv, 0.0, f, tid);
 }
 
 // ---- FP32 factors + FP64 refinement of one Newton solve: dx = -A^-1 F.
@@ -276,7 +276,7 @@ __device__ bool refine_solve(const OpArgs &a, const TapeProg &T, uint32_t n, flo
         for (uint32_t i = tid; i < n; i += kWave) y32[T.pinv[i]] = (float)r[i];
         wsync();
         LuFlags f{false, 0, 0};
-        run_rows<float>(gptr(T.sw), T.sw_rows, tv32, 0.0, f, tid);
+        run_rows<float, true>(gptr(T.sw), T.sw_rows, tv32, 0.0, f, tid);
         wsync();
         for (uint32_t i = tid; i < n; i += kWave) dx[i] = dx[i] + (double)y32[T.qinv[i]];
         wsync();
