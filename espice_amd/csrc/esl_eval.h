@@ -266,15 +266,14 @@ __device__ inline void run_row(esl::uint2_t w, MP M, double growth, LuFlags &f)
         const bool keep = kind == R_NOP || B == R(0);
         M[d] = keep ? v : p;
     } else if (!kSolve && kind == R_MAC) {
+        // (refactor programs hold no R_MACZ rows: checked on the host)
         const R v = M[d], A = M[b], B = M[c];
         M[d] = v - A * B;
-    } else if (!kSolve && kind == R_MACZ) {
-        const R v = M[d], A = M[b], B = M[c];
-        if (B != R(0)) M[d] = v - A * B;
     } else if (kind == R_DIV) {
         const R v = M[d], A = M[b];
         const R q = v / A;
         M[d] = q;
+        if (kSolve) return;   // solve programs carry no peel tests (flags 0)
         const uint32_t fl = esl::row_flags(w.y);
         const bool u = fl & kFlagUnscaled;
         const uint64_t nf = dbits((double)(q - q));

@@ -1658,6 +1658,7 @@ bool validate_tape(const TapeHostProg &o, u32 n, u32 nnz)
             for (u32 t = 0; t < (u32)esl::kWave; t++) {
                 const esl::uint2_t w = rp.words[r * esl::kWave + t];
                 if (esl::row_kind(w.y) != kind || (w.y >> 22) != 0) return false;
+                if (solve && esl::row_flags(w.y) != 0) return false;   // no peel tests in the solves
                 if (kind == esl::R_NOP) continue;
                 const u32 d = w.x & 0xffff, b = w.x >> 16, c = w.y & 0xffff;
                 if (d >= lim || b >= lim || c >= lim) return false;
