@@ -46,7 +46,13 @@ sys.path.insert(0, ROOT)
 # FLOPs per device evaluation, counted from the restated equations
 # (exp/log/sqrt/div count 1 each): MOS1 eval with both junctions and the
 # limiter pass; V source; I source; resistor.
-E_EVAL = {"mos1": 148, "v": 6, "i": 2, "r": 7}
+# Gummel-Poon: five junction laws, the base charge qb with its square
+# root, the Early and knee terms, the limiter pass and the residual
+# corrections.
+E_EVAL = {"mos1": 148, "v": 6, "i": 2, "r": 7, "bjt": 190}
+# Roofline label of the lane kernels: their factors stay in LDS, so the
+# bound is FP64 vector issue (no MFMA instruction runs in them).
+BOUND_FP64 = "fp64_vector"
 PEAK_FP64_TFLOPS = 78.6
 HEADLINE = "learned_fp32"
 BASELINE_MODE = "fp64_cold"
@@ -69,11 +75,13 @@ def parse():
     ap.add_argument("--mesh-steps", type=int, default=1000, help="rc_mesh: trapezoid steps of 1 ps")
     ap.add_argument("--mesh-steps-per-launch", type=int, default=10, help="rc_mesh: steps per kernel launch")
     ap.add_argument("--workload", default="ota_array",
-                    choices=["ota_array", "mixed_batch", "rc_mesh", "diffpair_array"],
+                    choices=["ota_array", "mixed_batch", "rc_mesh", "diffpair_array", "inverter_chain"],
                     help="ota_array: the headline; mixed_batch: 10,000 uneven instances of eight families "
                          "in one persistent launch; diffpair_array: 2048 points of 64 Gummel-Poon pairs, "
-                         "FP32 factors + FP64 refinement against plain FP64")
+                         "FP32 factors + FP64 refinement against plain FP64; inverter_chain: 256 points of "
+                         "the 16-stage MOS1 chain, plain FP64")
     ap.add_argument("--pair-points", type=int, default=2048, help="diffpair_array: points")
+    ap.add_argument("--chain-points", type=int, default=256, help="inverter_chain: points")
     ap.add_argument("--instances", type=int, default=10000)
     ap.add_argument("--families", default="", help="mixed_batch: comma-separated subset of the eight families")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -147,6 +155,9 @@ def main():
         return
     if args.workload == "diffpair_array":
         run_pairs(args)
+        return
+    if args.workload == "inverter_chain":
+        run_chain(args)
         return
     from espice_amd import circuits, lanes
 
@@ -316,8 +327,8 @@ def main():
         },
         "speedup_vs_plain_fp64": (round(res["points_per_s"] / base["points_per_s"], 3) if base else None),
         "roofline": {
-            "bound": "mfma",
-            "pipe": "vector ALU (lane factors stay in LDS; peak = MI355X FP64 vector 78.6 TFLOP/s)",
+            "bound": BOUND_FP64,
+            "pipe": "FP64 vector ALU (lane factors stay in LDS; no MFMA; peak = MI355X FP64 vector 78.6 TFLOP/s)",
             "kernel": "op_newton",
             "achieved": round(res["tflops"], 4),
             "peak": PEAK_FP64_TFLOPS,
@@ -472,9 +483,12 @@ def run_pairs(args):
     b = eng.solve_op(vals, precision=lanes.FP32_REFINED, tiling=lanes.TILING_WAVE)
     err = float((np.abs(a["x"] - b["x"]) / np.maximum(np.abs(a["x"]), 1e-6)).max())
     info = eng.info()
-    # LU and vector work per lane-iteration; the Gummel-Poon evaluation is
-    # not counted, so the figure is a lower bound
-    flops_iter = 2 * info.flops + info.nnz_l + 2 * (info.nnz_l + info.nnz_u) + 4 * info.n
+    # LU, vector and device-evaluation work per lane-iteration (the
+    # refinement steps of FP32 lanes are not counted: a lower bound there)
+    kinds = list(deck.kind)
+    e_eval = (E_EVAL["bjt"] * kinds.count(6) + E_EVAL["r"] * kinds.count(0) + E_EVAL["v"] * kinds.count(2)
+              + E_EVAL["i"] * kinds.count(3))
+    flops_iter = 2 * info.flops + info.nnz_l + 2 * (info.nnz_l + info.nnz_u) + 4 * info.n + e_eval
     achieved = flops_iter * f32["lane_iterations"] / (f32["kernel_ms"] * 1e-3) / 1e12
     parity, cpu = None, None
     if not args.no_cpu:
@@ -522,10 +536,97 @@ def run_pairs(args):
         "fp32_refined": {k: v for k, v in f32.items() if k not in ("lane_iterations", "kernel_ms")},
         "fp64": {k: v for k, v in f64.items() if k not in ("lane_iterations", "kernel_ms")},
         "fp32_refined_over_fp64": round(f32["points_per_s"] / max(f64["points_per_s"], 1e-9), 3),
-        "roofline": {"bound": "mfma", "pipe": "vector ALU (FP64 vector peak 78.6 TFLOP/s; LU and vector flops "
-                     "only, the Gummel-Poon evaluation not counted)", "kernel": "op_newton",
+        "roofline": {"bound": BOUND_FP64, "pipe": "FP64 vector ALU (peak 78.6 TFLOP/s; LU, vector and "
+                     "Gummel-Poon evaluation flops, refinement steps not counted)", "kernel": "op_newton",
                      "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64_TFLOPS, 5), "traffic": None},
+        "parity": parity,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    eng.close()
+
+
+def run_chain(args):
+    """The 16-stage MOS1 inverter chain: plain FP64 Monte Carlo .op over
+    --chain-points points (256 by default), cold starts, the autotuned
+    tiling; the CPU lanes (the oracle on the host threads) beside it."""
+    from espice_amd import circuits, lanes
+    fam = circuits.inverter_chain(16)
+    deck = fam.deck
+    n_pts = args.chain_points
+    vals = fam.sample(2025, n_pts)
+    eng = lanes.Engine(deck.arrays())
+    _, _, q = eng.pattern()
+    probes = np.array([deck.node_index(f"s{k}") for k in range(1, 17)], dtype=np.uint32)
+    for _ in range(args.warmup):
+        eng.solve_op(vals, probes=probes)
+    t_k, conv, lane_it = 0.0, 0, 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = eng.solve_op(vals, probes=probes)
+        tm = eng.timing()
+        t_k += tm.kernel_ms
+        conv += int((r["reasons"] == 0).sum())
+        lane_it += int(tm.lane_iterations)
+    wall = time.perf_counter() - t0
+    tiling = int(eng.timing().tiling)
+    info = eng.info()
+    kinds = list(deck.kind)
+    e_eval = (E_EVAL["mos1"] * kinds.count(5) + E_EVAL["v"] * kinds.count(2) + E_EVAL["r"] * kinds.count(0)
+              + E_EVAL["i"] * kinds.count(3))
+    flops_iter = 2 * info.flops + info.nnz_l + 2 * (info.nnz_l + info.nnz_u) + 4 * info.n + e_eval
+    achieved = flops_iter * lane_it / (t_k * 1e-3) / 1e12
+    full = eng.solve_op(vals)
+    parity, cpu = None, None
+    if not args.no_cpu:
+        from oracle import orc
+        c = orc.Circuit(deck.arrays())
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        ref = c.op(vals, q, threads=threads)
+        err = np.abs(full["x"] - ref["x"]) / np.maximum(np.abs(ref["x"]), 1e-6)
+        parity = {"sample": n_pts,
+                  "bitwise_equal_fp64_oracle": bool(np.array_equal(full["x"].view(np.uint64),
+                                                                   ref["x"].view(np.uint64))),
+                  "iterations_equal": bool(np.array_equal(full["iterations"], ref["iterations"])),
+                  "max_rel_err_vs_fp64_oracle": float(err.max())}
+        t0 = time.perf_counter()
+        done, rounds = 0, 0
+        while rounds == 0 or time.perf_counter() - t0 < 2.0:
+            rc = c.op(vals, q, threads=threads)
+            done += int((rc["reasons"] == 0).sum())
+            rounds += 1
+        t_cpu = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        one = c.op(vals, q, threads=1)
+        t_one = time.perf_counter() - t1
+        cpu = {"value": round(done / t_cpu, 1), "unit": "points/s", "cores": threads, "kind": "port",
+               "sample": f"the {n_pts} points of the workload solved {rounds} times, plain FP64 cold-start "
+                         f"lanes of the oracle on {threads} host threads, {t_cpu:.2f} s wall",
+               "single_thread_points_per_s": round(float((one["reasons"] == 0).sum() / t_one), 1)}
+    out = {
+        "metric": "converged sweep points/sec, 1 MI355X, 16-stage MOS1 inverter chain .op, plain FP64 lanes",
+        "value": round(conv / wall, 1),
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded VTO, VDD and input-level draws per point)",
+        "config": {"workload": f"inverter_chain_16 .op Monte Carlo, {n_pts} points per step, cold start",
+                   "n": int(info.n), "nnz": int(info.nnz), "nnz_l": int(info.nnz_l), "nnz_u": int(info.nnz_u),
+                   "flops_F": int(info.flops),
+                   "tiling": "lane-per-wavefront, factors in LDS" if tiling == 1 else "lane-per-thread"},
+        "kernel_ms_per_step": round(t_k / args.steps, 4),
+        "newton_iterations_per_point": round(lane_it / (n_pts * args.steps), 3),
+        "roofline": {"bound": BOUND_FP64, "pipe": "FP64 vector ALU (peak 78.6 TFLOP/s)",
+                     "kernel": "op_newton" if tiling == 1 else "op_newton_lpt",
+                     "achieved": round(achieved, 5), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP64_TFLOPS, 6), "traffic": None},
         "parity": parity,
         "cpu_baseline": cpu,
     }

@@ -181,8 +181,11 @@ __device__ inline void eval_chunk(const DeckProg &dp, const Chunk &ch, const dou
 
 // Limiting state of a start at x (no junction seeds): the previous terminal
 // voltages are x's own.
+// keep_von: MOS fetlim thresholds stay the von of the last evaluation (an
+// accepted transient step: the predictor's limit pass runs against the
+// accepted point with the von its last load stored, mos1.va:293-297).
 __device__ inline void seed_lim_at(const DeckProg &dp, const double *live, const double *x, double *lim,
-                                   uint32_t tid)
+                                   uint32_t tid, bool keep_von = false)
 {
     const uint32_t nm = dp.n_mos;
     for (uint32_t j = tid; j < nm; j += kWave) {
@@ -192,7 +195,7 @@ __device__ inline void seed_lim_at(const DeckProg &dp, const double *live, const
         lim[j] = m.tp * (xgg - xs);
         lim[nm + j] = m.tp * (xd - xs);
         lim[2 * nm + j] = m.tp * (xb - xs);
-        lim[3 * nm + j] = 0.0;
+        if (!keep_von) lim[3 * nm + j] = 0.0;
     }
     for (uint32_t j = tid; j < dp.n_dio; j += kWave) lim[4 * nm + j] = x[dp.d_t[2 * j]] - x[dp.d_t[2 * j + 1]];
     for (uint32_t j = tid; j < dp.n_bjt; j += kWave) {
@@ -201,6 +204,32 @@ __device__ inline void seed_lim_at(const DeckProg &dp, const double *live, const
         lim[lim_bjt(dp) + 2 * j] = tp * (x[t4[1]] - x[t4[2]]);
         lim[lim_bjt(dp) + 2 * j + 1] = tp * (x[t4[1]] - x[t4[0]]);
     }
+}
+
+// One limit pass of a lane's devices at x against its limiting state (MOS1
+// fetlim/limvds/pnjlim, diode and BJT pnjlim, eval.zig limitRange): the
+// thread's part; true when any of its devices moved.
+template <typename X>
+__device__ inline bool limit_pass(const DeckProg &dp, const double *live, X x, double *lim, uint32_t tid)
+{
+    const uint32_t nm = dp.n_mos;
+    bool limited = false;
+    for (uint32_t j = tid; j < nm; j += kWave) {
+        const Mos m = mos_load(dp, j, live);
+        const uint32_t *t4 = dp.m_t + 4 * j;
+        limited |= mos_limit(m, x[t4[0]], x[t4[1]], x[t4[2]], x[t4[3]], lim + j, nm);
+    }
+    for (uint32_t j = tid; j < dp.n_dio; j += kWave) {
+        const Dio d = dio_load(dp, j, live);
+        limited |= dio_limit_v(d, x[dp.d_t[2 * j]], x[dp.d_t[2 * j + 1]], lim[4 * nm + j]);
+    }
+    for (uint32_t j = tid; j < dp.n_bjt; j += kWave) {
+        const Bjt b = bjt_load(dp, j, live);
+        const uint32_t *t4 = dp.q_t + 4 * j;
+        double *ql = lim + lim_bjt(dp) + 2 * j;
+        limited |= bjt_limit_v(b, x[t4[0]], x[t4[1]], x[t4[2]], ql[0], ql[1]);
+    }
+    return limited;
 }
 
 // A lane's start (OpArgs modes 0 cold, 1 given x, 2 resume at the saved

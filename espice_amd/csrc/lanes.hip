@@ -565,23 +565,7 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
         }
         const double scaled = bitsd(wave_max_u64(worst));
         wsync();
-        bool limited = false;
-        for (uint32_t j = tid; j < nm; j += kWave) {
-            const Mos m = mos_load(dp, j, live);
-            const uint32_t *t4 = dp.m_t + 4 * j;
-            limited |= mos_limit(m, x[t4[0]], x[t4[1]], x[t4[2]], x[t4[3]], lim + j, nm);
-        }
-        for (uint32_t j = tid; j < dp.n_dio; j += kWave) {
-            const Dio d = dio_load(dp, j, live);
-            limited |= dio_limit_v(d, x[dp.d_t[2 * j]], x[dp.d_t[2 * j + 1]], lim[4 * nm + j]);
-        }
-        for (uint32_t j = tid; j < dp.n_bjt; j += kWave) {
-            const Bjt b = bjt_load(dp, j, live);
-            const uint32_t *t4 = dp.q_t + 4 * j;
-            double *ql = lim + lim_bjt(dp) + 2 * j;
-            limited |= bjt_limit_v(b, x[t4[0]], x[t4[1]], x[t4[2]], ql[0], ql[1]);
-        }
-        limited = wave_any(limited);
+        const bool limited = wave_any(limit_pass(dp, live, x, lim, tid));
         bool ok = !limited && iter != k0 && scaled < 1.0;
         if (ok) {
             bool miss = false;
@@ -877,23 +861,7 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
             }
             const double scaled = bitsd(wave_max_u64(worst));
             wsync();
-            bool limited = false;
-            for (uint32_t j = tid; j < nm; j += kWave) {
-                const Mos m = mos_load(dp, j, live);
-                const uint32_t *t4 = dp.m_t + 4 * j;
-                limited |= mos_limit(m, x[t4[0]], x[t4[1]], x[t4[2]], x[t4[3]], lim + j, nm);
-            }
-            for (uint32_t j = tid; j < dp.n_dio; j += kWave) {
-                const Dio d = dio_load(dp, j, live);
-                limited |= dio_limit_v(d, x[dp.d_t[2 * j]], x[dp.d_t[2 * j + 1]], lim[4 * nm + j]);
-            }
-            for (uint32_t j = tid; j < dp.n_bjt; j += kWave) {
-                const Bjt b = bjt_load(dp, j, live);
-                const uint32_t *t4 = dp.q_t + 4 * j;
-                double *ql = lim + lim_bjt(dp) + 2 * j;
-                limited |= bjt_limit_v(b, x[t4[0]], x[t4[1]], x[t4[2]], ql[0], ql[1]);
-            }
-            limited = wave_any(limited);
+            const bool limited = wave_any(limit_pass(dp, live, x, lim, tid));
             bool ok = !limited && iter != 0 && scaled < 1.0;
             if (ok) {
                 bool miss = false;
@@ -932,18 +900,22 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
         for (uint32_t k = tid; k < ta.n_probes; k += kWave)
             ta.probes[((size_t)lane * ta.steps + s) * ta.n_probes + k] = x[ta.probe_ids[k]];
         wsync();
-        seed_lim_at(dp, live, x, lim, tid);
+        // the limiting state at the accepted point (fetlim thresholds from
+        // its last evaluation)
+        seed_lim_at(dp, live, x, lim, tid, true);
         wsync();
         // the next step's first iterate: the last two accepted points
         // extrapolated by dt/dt_prev = 1 (MODEINITPRED, tran.zig:1131-1137;
-        // lane_device.zig:141-153), limited against the accepted one through
-        // the limiting state just seeded
+        // lane_device.zig:141-153), then one limit pass of it against the
+        // accepted point (applyLimits(trial, cur), tran.zig:1138)
         if (s + 1 < ta.steps) {
             for (uint32_t i = tid; i < n; i += kWave) {
                 const double cu = x[i];
                 x[i] = cu + (cu - xp[i]);
                 xp[i] = cu;
             }
+            wsync();
+            limit_pass(dp, live, x, lim, tid);
             wsync();
         }
     }

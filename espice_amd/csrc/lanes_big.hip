@@ -337,24 +337,7 @@ __device__ inline double big_update(const OpArgs &a, const TapeProg &T, const Bi
 // Limiting after an update (MOS1, diodes, BJTs): true when any limited.
 __device__ __forceinline__ bool big_limit(const DeckProg &dp, const double *live, gcdbl *x, double *lim, uint32_t tid)
 {
-    const uint32_t nm = dp.n_mos;
-    bool limited = false;
-    for (uint32_t j = tid; j < nm; j += kWave) {
-        const Mos m = mos_load(dp, j, live);
-        const uint32_t *t4 = dp.m_t + 4 * j;
-        limited |= mos_limit(m, x[t4[0]], x[t4[1]], x[t4[2]], x[t4[3]], lim + j, nm);
-    }
-    for (uint32_t j = tid; j < dp.n_dio; j += kWave) {
-        const Dio d = dio_load(dp, j, live);
-        limited |= dio_limit_v(d, x[dp.d_t[2 * j]], x[dp.d_t[2 * j + 1]], lim[4 * nm + j]);
-    }
-    for (uint32_t j = tid; j < dp.n_bjt; j += kWave) {
-        const Bjt b = bjt_load(dp, j, live);
-        const uint32_t *t4 = dp.q_t + 4 * j;
-        double *ql = lim + lim_bjt(dp) + 2 * j;
-        limited |= bjt_limit_v(b, x[t4[0]], x[t4[1]], x[t4[2]], ql[0], ql[1]);
-    }
-    return wave_any(limited);
+    return wave_any(limit_pass(dp, live, x, lim, tid));
 }
 
 // One lane's operating point (op_lane's FP64 path: cold, given or resumed
@@ -604,16 +587,19 @@ __device__ __forceinline__ void big_tran_lane(const BigArgs &b, uint32_t g, uint
         for (uint32_t k = tid; k < ta.n_probes; k += kWave)
             ta.probes[((size_t)lane * ta.steps + s) * ta.n_probes + k] = L.x[ta.probe_ids[k]];
         wsync();
-        seed_lim_at(dp, live, (const double *)L.x, lim, tid);
+        seed_lim_at(dp, live, (const double *)L.x, lim, tid, true);
         wsync();
         // the next step's first iterate: the last two accepted points
-        // extrapolated (tran.zig:1131-1137; lane_device.zig:141-153)
+        // extrapolated (tran.zig:1131-1137; lane_device.zig:141-153), then
+        // one limit pass against the accepted point (tran.zig:1138)
         if (s + 1 < ta.steps) {
             for (uint32_t i = tid; i < n; i += kWave) {
                 const double cu = L.x[i];
                 L.x[i] = cu + (cu - xp[i]);
                 xp[i] = cu;
             }
+            wsync();
+            limit_pass(dp, live, (gcdbl *)L.x, lim, tid);
             wsync();
         }
     }

@@ -132,6 +132,9 @@ typedef struct {
 #define ORC_TOL_LADDER_ALWAYS 2u
 #define ORC_TOL_LADDER_NO_GMIN 4u
 #define ORC_TOL_LADDER_NO_SRC 8u
+/* for the tests: transient steps start from the accepted point instead of
+ * the predictor's extrapolation */
+#define ORC_TOL_NO_PREDICT 16u
 
 typedef struct orc_ckt orc_ckt;
 /* NULL on a malformed deck (message in err). */
@@ -148,6 +151,11 @@ void orc_ckt_assemble_cold(orc_ckt *c, const double *live, double *g, double *rh
 /* Assembles at x given (no limiting: a warm start's iteration 0). */
 void orc_ckt_assemble_at(orc_ckt *c, const double *live, const double *x,
                          double *g, double *rhs);
+/* Assembles at x (no limiting) with charges at source time t: g, rhs as
+ * orc_ckt_assemble_at, cp [nnz] the charge Jacobian dq/dx (the c plane), q [n]
+ * the row charges. */
+void orc_ckt_assemble_charges(orc_ckt *c, const double *live, const double *x, double t, double *g, double *rhs,
+                              double *cp, double *q);
 /* The same with every independent source scaled by src (the source rung). */
 void orc_ckt_assemble_scaled(orc_ckt *c, const double *live, const double *x, double src,
                              double *g, double *rhs);
@@ -197,4 +205,12 @@ int orc_tran_solve(orc_ckt *c, const double *values, size_t count, const orc_tol
                    const orc_tol *tol, double dt, uint32_t steps, const uint32_t *probes,
                    uint32_t n_probes, const uint32_t *q, int threads, double *x,
                    double *probe_out, uint32_t *iterations, uint32_t *reasons);
+/* orc_tran_solve with traces of each step's first iterate (tests): first_x
+ * [count][steps][n] the predicted x, first_lim [count][steps][n_inst][4] the
+ * limiting state after the predictor's limit pass (diode: [0] the limited
+ * junction voltage; MOS: vgs, vds, vbs, von); rows of step 0 stay as given */
+int orc_tran_solve_traced(orc_ckt *c, const double *values, size_t count, const orc_tol *tol_op,
+                          const orc_tol *tol, double dt, uint32_t steps, const uint32_t *probes,
+                          uint32_t n_probes, const uint32_t *q, int threads, double *x, double *probe_out,
+                          uint32_t *iterations, uint32_t *reasons, double *first_x, double *first_lim);
 #endif

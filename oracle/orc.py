@@ -90,6 +90,11 @@ def lib():
         L.orc_tran_solve.restype = C.c_int
         L.orc_tran_solve.argtypes = [C.c_void_p, f64p, C.c_size_t, C.POINTER(Tol), C.POINTER(Tol), C.c_double,
                                      C.c_uint32, C.c_void_p, C.c_uint32, u32p, C.c_int, f64p, C.c_void_p, u32p, u32p]
+        L.orc_ckt_assemble_charges.argtypes = [C.c_void_p, f64p, f64p, C.c_double, f64p, f64p, f64p, f64p]
+        L.orc_tran_solve_traced.restype = C.c_int
+        L.orc_tran_solve_traced.argtypes = [C.c_void_p, f64p, C.c_size_t, C.POINTER(Tol), C.POINTER(Tol),
+                                            C.c_double, C.c_uint32, C.c_void_p, C.c_uint32, u32p, C.c_int, f64p,
+                                            C.c_void_p, u32p, u32p, C.c_void_p, C.c_void_p]
         L.orc_op_solve.restype = C.c_int
         L.orc_op_solve.argtypes = [C.c_void_p, f64p, C.c_size_t, C.POINTER(Tol), u32p, C.c_void_p,
                                    C.c_int, f64p, u32p, u32p, u32p]
@@ -247,6 +252,16 @@ class Circuit:
             lib().orc_ckt_assemble_scaled(self.h, _f64(live), _f64(x), float(src), g, r)
         return g, r
 
+    def assemble_charges(self, live, x, t=0.0):
+        """Jacobian g, residual f, charge Jacobian c (the c plane, pattern
+        slots) and row charges q at x (no limiting), sources at time t."""
+        g = np.zeros(self.nnz)
+        r = np.zeros(self.n)
+        cp = np.zeros(self.nnz)
+        q = np.zeros(self.n)
+        lib().orc_ckt_assemble_charges(self.h, _f64(live), _f64(x), float(t), g, r, cp, q)
+        return g, r, cp, q
+
     def op(self, values, q, start_x=None, threads=1, tol=None, precision=0, refine_tol=1e-12, refine_max=8,
            nominal_x=None):
         """Operating points; precision 1 = FP32 factors + FP64 refinement;
@@ -268,8 +283,14 @@ class Circuit:
         return dict(x=x, iterations=it, reasons=rs, repivots=rp, fp64_lanes=fb, fell_back=wb)
 
 
-def tran(circuit, values, q, dt, steps, probes=(), threads=1, tol=None, tol_op=None):
-    """Transient lanes on the oracle (see orc_tran_solve)."""
+NO_PREDICT = 16   # ORC_TOL_NO_PREDICT: steps start at the accepted point
+
+
+def tran(circuit, values, q, dt, steps, probes=(), threads=1, tol=None, tol_op=None, trace=False):
+    """Transient lanes on the oracle (see orc_tran_solve). trace: also
+    returns each step's first iterate (first_x [count][steps][n]) and the
+    limiting state after the predictor's limit pass (first_lim
+    [count][steps][n_inst][4]); step 0's rows are zero."""
     values = _f64(np.atleast_2d(values))
     count = values.shape[0]
     t = Tol(1e-3, 1e-12, 1e-6, 1e-9, 50, 0) if tol is None else tol
@@ -279,10 +300,19 @@ def tran(circuit, values, q, dt, steps, probes=(), threads=1, tol=None, tol_op=N
     po = np.zeros((count, steps, max(len(pr), 1)))
     it = np.zeros(count, np.uint32)
     rs = np.zeros(count, np.uint32)
-    lib().orc_tran_solve(circuit.h, values, count, C.byref(to), C.byref(t), float(dt), int(steps),
-                         pr.ctypes.data if len(pr) else None, len(pr), _u32(q), int(threads), x,
-                         po.ctypes.data, it, rs)
-    return dict(x=x, probes=po[:, :, :len(pr)], iterations=it, reasons=rs)
+    fx = fl = None
+    if trace:
+        n_inst = int(len(circuit._keep["kind"]))
+        fx = np.zeros((count, steps, circuit.n))
+        fl = np.zeros((count, steps, n_inst, 4))
+    lib().orc_tran_solve_traced(circuit.h, values, count, C.byref(to), C.byref(t), float(dt), int(steps),
+                                pr.ctypes.data if len(pr) else None, len(pr), _u32(q), int(threads), x,
+                                po.ctypes.data, it, rs, None if fx is None else fx.ctypes.data,
+                                None if fl is None else fl.ctypes.data)
+    out = dict(x=x, probes=po[:, :, :len(pr)], iterations=it, reasons=rs)
+    if trace:
+        out.update(first_x=fx, first_lim=fl)
+    return out
 
 
 def natural_order(n):

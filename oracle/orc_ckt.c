@@ -800,8 +800,11 @@ static void seed_cold(const orc_ckt *c, lane_dev *ld)
     }
 }
 
-/* warm start: the limiting state is the start point's own branch voltages */
-static void seed_at(const orc_ckt *c, lane_dev *ld, const double *x)
+/* warm start: the limiting state is the start point's own branch voltages.
+ * keep_von: MOS fetlim thresholds stay the von the last evaluation stored
+ * (an accepted transient step: the predictor's limit pass runs against the
+ * accepted point with the von of its last load, mos1.va:293-297) */
+static void seed_at_von(const orc_ckt *c, lane_dev *ld, const double *x, int keep_von)
 {
     for (uint32_t i = 0; i < c->n_inst; i++) {
         const orc_inst *in = &c->inst[i];
@@ -817,9 +820,11 @@ static void seed_at(const orc_ckt *c, lane_dev *ld, const double *x)
         lim[0] = tp * (xg - xs);
         lim[1] = tp * (xd - xs);
         lim[2] = tp * (xb - xs);
-        lim[3] = 0.0;
+        if (!keep_von) lim[3] = 0.0;
     }
 }
+
+static void seed_at(const orc_ckt *c, lane_dev *ld, const double *x) { seed_at_von(c, ld, x, 0); }
 
 /* Static stamps into g (pattern slots) and rhs; with charges, the charge
  * Jacobian into cp (pattern slots) and the row charges into q. t is the
@@ -969,6 +974,23 @@ void orc_ckt_assemble_scaled(orc_ckt *c, const double *live, const double *x, do
     memcpy(g, gg, (size_t)c->nnz * 8);
     memcpy(rhs, rr, (size_t)c->n * 8);
     free(gg); free(rr);
+    lane_dev_free(&ld);
+}
+
+void orc_ckt_assemble_charges(orc_ckt *c, const double *live, const double *x, double t, double *g, double *rhs,
+                              double *cp, double *q)
+{
+    lane_dev ld;
+    lane_dev_init(c, live, &ld);
+    seed_at(c, &ld, x);
+    double *gg = malloc((size_t)(c->nnz + 1) * 8), *rr = malloc((size_t)(c->n + 1) * 8);
+    double *cc = malloc((size_t)(c->nnz + 1) * 8), *qq = malloc((size_t)(c->n + 1) * 8);
+    assemble_t(c, &ld, x, t, 1, gg, rr, cc, qq);
+    memcpy(g, gg, (size_t)c->nnz * 8);
+    memcpy(rhs, rr, (size_t)c->n * 8);
+    memcpy(cp, cc, (size_t)c->nnz * 8);
+    memcpy(q, qq, (size_t)c->n * 8);
+    free(gg); free(rr); free(cc); free(qq);
     lane_dev_free(&ld);
 }
 
@@ -1475,6 +1497,7 @@ typedef struct {
     const uint32_t *op_reason;
     double *x, *probe_out;
     uint32_t *iterations, *reasons;
+    double *first_x, *first_lim;   /* traces (tests): [count][steps][n], [count][steps][n_inst*4] */
 } tjob_t;
 
 /* A = G + ag0 C on the charge slots, F += ag0 (q - q1) - ag1 i_prev */
@@ -1585,16 +1608,28 @@ static void tran_lane(const tjob_t *j, size_t l)
         }
         for (uint32_t k = 0; k < j->n_probes; k++)
             j->probe_out[((size_t)l * j->steps + s) * j->n_probes + k] = x[j->probes[k]];
-        seed_at(c, &ld, x);
+        /* the limiting state at the accepted point, MOS fetlim thresholds
+         * from its last evaluation */
+        seed_at_von(c, &ld, x, 1);
         /* the next step's first iterate: the last two accepted points
          * extrapolated by dt/dt_prev = 1 (MODEINITPRED, tran.zig:1131-1137,
-         * predict at tran.zig:37-41), limited against the accepted one */
-        if (s + 1 < j->steps)
-            for (uint32_t i = 0; i < n; i++) {
-                const double cu = x[i];
-                x[i] = cu + (cu - xp[i]);
-                xp[i] = cu;
+         * predict at tran.zig:37-41), then one limit pass of the trial
+         * against the accepted point (applyLimits(trial, cur),
+         * tran.zig:1138; lane_device.zig:141-153 keeps x_old = cur for it) */
+        if (s + 1 < j->steps) {
+            if (!(tol->reserved & ORC_TOL_NO_PREDICT))
+                for (uint32_t i = 0; i < n; i++) {
+                    const double cu = x[i];
+                    x[i] = cu + (cu - xp[i]);
+                    xp[i] = cu;
+                }
+            apply_limits(c, &ld, x);
+            if (j->first_x) {
+                memcpy(j->first_x + ((size_t)l * j->steps + s + 1) * n, x, (size_t)n * 8);
+                memcpy(j->first_lim + ((size_t)l * j->steps + s + 1) * c->n_inst * 4, ld.lim,
+                       (size_t)c->n_inst * 4 * 8);
             }
+        }
     }
     memcpy(j->x + l * n, x, (size_t)n * 8);
     j->iterations[l] = total;
@@ -1614,6 +1649,15 @@ static void *run_tjob(void *arg)
 int orc_tran_solve(orc_ckt *c, const double *values, size_t count, const orc_tol *tol_op, const orc_tol *tol,
                    double dt, uint32_t steps, const uint32_t *probes, uint32_t n_probes, const uint32_t *q,
                    int threads, double *x, double *probe_out, uint32_t *iterations, uint32_t *reasons)
+{
+    return orc_tran_solve_traced(c, values, count, tol_op, tol, dt, steps, probes, n_probes, q, threads, x,
+                                 probe_out, iterations, reasons, NULL, NULL);
+}
+
+int orc_tran_solve_traced(orc_ckt *c, const double *values, size_t count, const orc_tol *tol_op,
+                          const orc_tol *tol, double dt, uint32_t steps, const uint32_t *probes,
+                          uint32_t n_probes, const uint32_t *q, int threads, double *x, double *probe_out,
+                          uint32_t *iterations, uint32_t *reasons, double *first_x, double *first_lim)
 {
     if (count == 0) return 0;
     const uint32_t n = c->n, nnz = c->nnz;
@@ -1668,7 +1712,7 @@ int orc_tran_solve(orc_ckt *c, const double *values, size_t count, const orc_tol
         for (int t = 0; t < threads; t++) {
             size_t k = per + ((size_t)t < rem ? 1 : 0);
             jobs[t] = (tjob_t){c, values, at, k, tol, dt, steps, n_probes, probes, lead, q, has_c, x_op, op_rs,
-                               x, probe_out, iterations, reasons};
+                               x, probe_out, iterations, reasons, first_x, first_lim};
             at += k;
         }
         if (threads == 1) run_tjob(&jobs[0]);

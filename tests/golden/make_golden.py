@@ -76,6 +76,11 @@ CASES = {
         ("M", "m1", ["drain", "gate", "0", "0"],
          {"type": -1, "vto": -0.7, "kp": 60e-6, "gamma": 0.4, "phi": 0.65, "lambda": 0.05}, {"w": 20e-6, "l": 1e-6})],
         [["vsd", "v(drain)", -1.0], ["vsg", "v(gate)", -1.0]]),
+    "device_mos1_subthreshold": ("dc/device_mos1_subthreshold", [
+        ("V", "vds", ["drain", "0"], 1.0), ("V", "vgs", ["gate", "0"], 0.0),
+        ("M", "m1", ["drain", "gate", "0", "0"],
+         {"type": 1, "vto": 0.7, "kp": 120e-6, "gamma": 0.4, "phi": 0.65, "lambda": 0.04}, {"w": 10e-6, "l": 2e-6})],
+        [["vds", "v(drain)", 1.0], ["vgs", "v(gate)", 1.0]]),
     "device_mos1_body_effect": ("dc/device_mos1_body_effect", [
         ("V", "vds", ["drain", "0"], 3.0), ("V", "vgs", ["gate", "0"], 0.0), ("V", "vbs", ["bulk", "0"], 0.0),
         ("M", "m1", ["drain", "gate", "0", "bulk"],
@@ -83,6 +88,18 @@ CASES = {
         [["vds", "v(drain)", 1.0], ["vgs", "v(gate)", 1.0], ["vbs", "v(bulk)", 1.0]]),
 }
 
+# dc/device_mos1_temp: `.dc Vgs 0 5 0.01 TEMP -40 125 55`, the outer sweep
+# over -40, 15, 70 and 125 degC. One case per temperature: the model at that
+# temperature (TOX and UO are given, but so is KP, which wins), and the
+# fixture's selected rows of that block (row index // 501).
+TEMP_ROWS = 501
+for k, t in enumerate((-40.0, 15.0, 70.0, 125.0)):
+    CASES[f"device_mos1_temp_{int(t)}".replace("-", "m")] = ("dc/device_mos1_temp", [
+        ("V", "vds", ["drain", "0"], 3.0), ("V", "vgs", ["gate", "0"], 0.0),
+        ("M", "m1", ["drain", "gate", "0", "0"],
+         {"type": 1, "vto": 0.7, "kp": 120e-6, "gamma": 0.4, "phi": 0.65, "lambda": 0.04, "tox": 40e-9,
+          "u0": 600.0, "temp": t}, {"w": 10e-6, "l": 1e-6})],
+        [["vds", "v(drain)", 1.0], ["vgs", "v(gate)", 1.0]], {"row_block": TEMP_ROWS, "block": k})
 
 QM = {"type": 1, "is": 1e-15, "bf": 100, "br": 2, "vaf": 100}
 QD = {"type": 1, "is": 1e-15, "bf": 200, "vaf": 100}
@@ -120,11 +137,16 @@ BJT_CASES["bench_analog_current_mirror"] = ("op/bench_analog_current_mirror", [
 
 def write(cases, path):
     out = {"source": "OmarSiwy/ESPice 1.2.0 tests/fixtures (ngspice-44.2 oracle values)", "cases": {}}
-    for name, (fixture, devices, live) in cases.items():
+    for name, (fixture, devices, live, *sel) in cases.items():
         exp = json.load(open(os.path.join(REF, fixture + ".expected.json")))
         net = open(os.path.join(REF, fixture + ".sp"), "rb").read()
         plot = exp["expect"]["plots"][0]
-        cols = {k: {"values": v["values"], "rtol": v.get("rtol", 1e-3), "atol": v.get("atol", 0.0)}
+        keep = None
+        if sel:   # the rows of one block of an outer sweep
+            idx = plot["row_indices"]
+            keep = [i for i, r in enumerate(idx) if r // sel[0]["row_block"] == sel[0]["block"]]
+        cols = {k: {"values": v["values"] if keep is None else [v["values"][i] for i in keep],
+                    "rtol": v.get("rtol", 1e-3), "atol": v.get("atol", 0.0)}
                 for k, v in plot["columns"].items()}
         out["cases"][name] = {
             "fixture": fixture,
