@@ -167,3 +167,48 @@ def test_nominal_fallback_when_the_guess_is_bad(lanes, oracle):
     assert np.array_equal(got["x"].view(np.uint64), ref["x"].view(np.uint64))
     eng.close()
     net.close()
+
+
+def test_headline_learned_fp32_full_size_bitwise_against_oracle(lanes, oracle):
+    """The headline mode at its workload: 16,384 OTA points on the
+    benchmark's seed, learned starts from a network trained on 50,000
+    operating points the engine solved, FP32 factors + FP64 refinement.
+    Every point is bitwise the oracle's FP32 mode from the same guesses
+    (iterations, reasons, FP64 fallbacks and nominal restarts included), and
+    within 1e-7 relative of the oracle's plain FP64 lanes from those
+    guesses (the contract)."""
+    fam = circuits.ota_array()
+    eng = lanes.Engine(fam.deck.arrays())
+    _, _, q = eng.pattern()
+    c = oracle.Circuit(fam.deck.arrays())
+    x_nom = eng.solve_op(fam.nominal()[None, :])["x"][0]
+    tv = fam.sample(7, 50000)
+    sol = eng.solve_op(tv, x0=np.repeat(x_nom[None, :], len(tv), 0), tiling=lanes.TILING_WAVE)
+    ok = sol["reasons"] == 0
+    say(f"training the warm start on {ok.sum()} operating points")
+    net = lanes.Mlp(tv.shape[1], eng.n - 1, n_hidden=256, n_hidden_layers=3, seed=7)
+    net.set_scaling(tv[ok].mean(0), tv[ok].std(0) + 1e-30, sol["x"][ok][:, 1:].mean(0),
+                    sol["x"][ok][:, 1:].std(0) + 1e-12)
+    losses = net.train(tv[ok], sol["x"][ok], epochs=30, batch=512)
+    say(f"  loss {losses[0]:.3e} -> {losses[-1]:.3e}")
+    eng.set_warm_start(net, x_nom)
+    vals = fam.sample(2026, 16384)
+    got = eng.solve_op(vals, learned=True, precision=lanes.FP32_REFINED)
+    tm = eng.timing()
+    x0 = net.predict(vals)
+    ref = c.op(vals, q, start_x=x0, threads=16, precision=1, nominal_x=x_nom)
+    say(f"  its/pt {got['iterations'].mean():.3f}; fp64 fallbacks {tm.fp64_lanes} (oracle "
+        f"{int(ref['fp64_lanes'].sum())}); nominal restarts {tm.warm_fallbacks} (oracle {int(ref['fell_back'].sum())})")
+    assert np.array_equal(got["reasons"], ref["reasons"])
+    assert np.array_equal(got["iterations"], ref["iterations"])
+    assert np.array_equal(got["x"].view(np.uint64), ref["x"].view(np.uint64))
+    assert tm.fp64_lanes == int(ref["fp64_lanes"].sum())
+    assert tm.warm_fallbacks == int(ref["fell_back"].sum())
+    r64 = c.op(vals, q, start_x=x0, threads=16, nominal_x=x_nom)
+    ok = (ref["reasons"] == 0) & (r64["reasons"] == 0)
+    err = np.abs(got["x"][ok] - r64["x"][ok]) / np.maximum(np.abs(r64["x"][ok]), 1e-6)
+    say(f"  max relative error against the FP64 lanes from the same guesses {err.max():.3e}")
+    assert ok.mean() > 0.999 and err.max() <= 1e-7
+    assert got["iterations"].mean() < 3.0
+    eng.close()
+    net.close()
