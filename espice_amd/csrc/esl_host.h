@@ -50,4 +50,10 @@ void *lookup(void *h, int type);
 void *release(void *h, int type);
 bool device_present();
 
+// The body of esl_tape_replay_noalloc: esl_tape_replay with its device
+// buffers carved from `stage` (stage_bytes of device memory); no allocation.
+esl_status tape_replay_staged(esl_tape *h, const double *a, const double *rhs, size_t lanes, double growth,
+                              double *lx, double *ux, double *udiag, double *dx, uint32_t *fail, void *stage,
+                              size_t stage_bytes);
+
 }  // namespace eslh

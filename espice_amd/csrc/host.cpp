@@ -4182,6 +4182,96 @@ esl_status esl_tape_export(esl_tape *h, uint32_t *pinv, uint32_t *lp, uint32_t *
     return ESL_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Device buffers of one replay: the staged inputs and the outputs.
+struct ReplayBufs {
+    double *a, *rhs, *lx, *ux, *udiag, *dx;
+    u32 *fail;
+};
+
+// The replay of esl_tape_replay through device buffers the caller of this
+// function provides: inputs copied in, one launch, outputs copied back, and
+// a wait for the tape's stream. Allocates nothing.
+esl_status replay_through(TapeObj *t, const double *a, const double *rhs, size_t lanes, double growth, double *lx,
+                          double *ux, double *udiag, double *dx, uint32_t *fail, const ReplayBufs &d)
+{
+    const u32 n = t->n, nnz = t->nnz;
+    const esl::TapeProg &P = t->prog.prog;
+    hipStream_t s = t->stream;
+    HIPCHK(hipMemcpyAsync(d.a, a, lanes * nnz * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d.rhs, rhs, lanes * n * 8, hipMemcpyHostToDevice, s));
+    HostReplayArgs r{};
+    r.T = P;
+    r.T.growth = growth;
+    r.n = n;
+    r.nnz = nnz;
+    r.a = d.a;
+    r.rhs = d.rhs;
+    r.lx = d.lx;
+    r.ux = d.ux;
+    r.udiag = d.udiag;
+    r.dx = d.dx;
+    r.fail = d.fail;
+    const u32 lds = (u32)((size_t)(P.n_tv + 2 * n) * 8);
+    HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
+    HIPCHK(esl_launch_tape_replay(&r, (u32)lanes, lds, s));
+    if (lx && P.n_l) HIPCHK(hipMemcpyAsync(lx, d.lx, lanes * P.n_l * 8, hipMemcpyDeviceToHost, s));
+    if (ux && P.n_u) HIPCHK(hipMemcpyAsync(ux, d.ux, lanes * P.n_u * 8, hipMemcpyDeviceToHost, s));
+    if (udiag) HIPCHK(hipMemcpyAsync(udiag, d.udiag, lanes * n * 8, hipMemcpyDeviceToHost, s));
+    if (dx) HIPCHK(hipMemcpyAsync(dx, d.dx, lanes * n * 8, hipMemcpyDeviceToHost, s));
+    if (fail) HIPCHK(hipMemcpyAsync(fail, d.fail, lanes * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return ESL_OK;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+}  // namespace
+
+namespace eslh {
+
+// esl_tape_replay_noalloc's body: the replay's buffers carved from `stage`
+// (device memory of stage_bytes, 256-byte aligned), each region 256-byte
+// aligned. Every region is written in full before it is read (inputs by
+// the copies, outputs by the kernel for every lane), so nothing needs
+// clearing between calls.
+esl_status tape_replay_staged(esl_tape *h, const double *a, const double *rhs, size_t lanes, double growth,
+                              double *lx, double *ux, double *udiag, double *dx, uint32_t *fail, void *stage,
+                              size_t stage_bytes)
+{
+    auto *t = static_cast<TapeObj *>(lookup(h, 2));
+    if (!t) return ESL_INVALID_HANDLE;
+    if (!a || !rhs || lanes == 0 || lanes > 0x7fffffff) return ESL_INVALID_ARGUMENT;
+    const size_t n = t->n, nnz = t->nnz;
+    const esl::TapeProg &P = t->prog.prog;
+    const size_t sizes[7] = {lanes * nnz * 8, lanes * n * 8, lanes * std::max<size_t>(P.n_l, 1) * 8,
+                             lanes * std::max<size_t>(P.n_u, 1) * 8, lanes * n * 8, lanes * n * 8, lanes * 4};
+    size_t off[7], need = 0;
+    for (int k = 0; k < 7; k++) {
+        off[k] = need;
+        need += align256(sizes[k]);
+    }
+    if (need > stage_bytes) {
+        g_last_error = "esl_tape_replay_noalloc: " + std::to_string(lanes) + " lanes need " + std::to_string(need) +
+                       " bytes of scratch; the library's fixed scratch holds " + std::to_string(stage_bytes) +
+                       " bytes (call again with fewer lanes)";
+        return ESL_BUFFER_TOO_SMALL;
+    }
+    char *b = static_cast<char *>(stage);
+    const ReplayBufs d{reinterpret_cast<double *>(b + off[0]), reinterpret_cast<double *>(b + off[1]),
+                       reinterpret_cast<double *>(b + off[2]), reinterpret_cast<double *>(b + off[3]),
+                       reinterpret_cast<double *>(b + off[4]), reinterpret_cast<double *>(b + off[5]),
+                       reinterpret_cast<u32 *>(b + off[6])};
+    return replay_through(t, a, rhs, lanes, growth, lx, ux, udiag, dx, fail, d);
+}
+
+}  // namespace eslh
+
+extern "C" {
+
 esl_status esl_tape_replay(esl_tape *h, const double *a, const double *rhs, size_t lanes, double growth, double *lx,
                            double *ux, double *udiag, double *dx, uint32_t *fail)
 {
@@ -4197,31 +4287,9 @@ esl_status esl_tape_replay(esl_tape *h, const double *a, const double *rhs, size
     HIPCHK(t->udiag.ensure(lanes * n * 8));
     HIPCHK(t->dx.ensure(lanes * n * 8));
     HIPCHK(t->fail.ensure(lanes * 4));
-    hipStream_t s = t->stream;
-    HIPCHK(hipMemcpyAsync(t->a.p, a, lanes * nnz * 8, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(t->rhs.p, rhs, lanes * n * 8, hipMemcpyHostToDevice, s));
-    HostReplayArgs r{};
-    r.T = P;
-    r.T.growth = growth;
-    r.n = n;
-    r.nnz = nnz;
-    r.a = t->a.as<double>();
-    r.rhs = t->rhs.as<double>();
-    r.lx = t->lx.as<double>();
-    r.ux = t->ux.as<double>();
-    r.udiag = t->udiag.as<double>();
-    r.dx = t->dx.as<double>();
-    r.fail = t->fail.as<u32>();
-    const u32 lds = (u32)((size_t)(P.n_tv + 2 * n) * 8);
-    HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
-    HIPCHK(esl_launch_tape_replay(&r, (u32)lanes, lds, s));
-    if (lx && P.n_l) HIPCHK(hipMemcpyAsync(lx, t->lx.p, lanes * P.n_l * 8, hipMemcpyDeviceToHost, s));
-    if (ux && P.n_u) HIPCHK(hipMemcpyAsync(ux, t->ux.p, lanes * P.n_u * 8, hipMemcpyDeviceToHost, s));
-    if (udiag) HIPCHK(hipMemcpyAsync(udiag, t->udiag.p, lanes * n * 8, hipMemcpyDeviceToHost, s));
-    if (dx) HIPCHK(hipMemcpyAsync(dx, t->dx.p, lanes * n * 8, hipMemcpyDeviceToHost, s));
-    if (fail) HIPCHK(hipMemcpyAsync(fail, t->fail.p, lanes * 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    return ESL_OK;
+    const ReplayBufs d{t->a.as<double>(), t->rhs.as<double>(), t->lx.as<double>(), t->ux.as<double>(),
+                       t->udiag.as<double>(), t->dx.as<double>(), t->fail.as<u32>()};
+    return replay_through(t, a, rhs, lanes, growth, lx, ux, udiag, dx, fail, d);
 }
 
 }  // extern "C"

@@ -36,6 +36,7 @@ EXPORTS = [
     "esl_abi_version", "esl_default_tolerances", "esl_default_op_options", "esl_engine_create",
     "esl_engine_destroy", "esl_engine_info_get", "esl_engine_pattern", "esl_op_solve", "esl_last_timing",
     "esl_tape_create", "esl_tape_destroy", "esl_tape_sizes", "esl_tape_export", "esl_tape_replay",
+    "esl_tape_replay_noalloc",
     "esl_mlp_default_train_options", "esl_mlp_create", "esl_mlp_destroy", "esl_mlp_param_count",
     "esl_mlp_get_params", "esl_mlp_set_params", "esl_mlp_set_scaling", "esl_mlp_gradient", "esl_mlp_train",
     "esl_mlp_predict", "esl_engine_set_warm_start", "esl_default_tran_options", "esl_tran_solve",
@@ -161,6 +162,8 @@ def load() -> C.CDLL:
     L.esl_tape_export.argtypes = [vp] * 8
     L.esl_tape_replay.restype = C.c_uint32
     L.esl_tape_replay.argtypes = [vp, vp, vp, C.c_size_t, C.c_double, vp, vp, vp, vp, vp]
+    L.esl_tape_replay_noalloc.restype = C.c_uint32
+    L.esl_tape_replay_noalloc.argtypes = [vp, vp, vp, C.c_size_t, C.c_double, vp, vp, vp, vp, vp]
     L.esl_mlp_default_train_options.argtypes = [C.POINTER(MlpTrainOptions)]
     L.esl_mlp_create.restype = C.c_uint32
     L.esl_mlp_create.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(vp)]
@@ -460,7 +463,10 @@ class Tape:
         out["prow"] = out["prow"][:nnz]
         return out
 
-    def replay(self, a, rhs, growth=1e-12) -> dict:
+    def replay(self, a, rhs, growth=1e-12, noalloc=False) -> dict:
+        """Replays the tape over a [lanes][nnz] and solves -rhs [lanes][n].
+        noalloc: through esl_tape_replay_noalloc (the library's fixed
+        scratch; no device allocation in the call)."""
         a = np.ascontiguousarray(a, dtype=np.float64)
         rhs = np.ascontiguousarray(rhs, dtype=np.float64)
         lanes = a.shape[0]
@@ -469,8 +475,9 @@ class Tape:
         ud = np.zeros((lanes, self.n))
         dx = np.zeros((lanes, self.n))
         fail = np.zeros(lanes, np.uint32)
-        _check(load().esl_tape_replay(self.h, _ptr(a), _ptr(rhs), lanes, growth, _ptr(lx), _ptr(ux), _ptr(ud),
-                                      _ptr(dx), _ptr(fail)), "esl_tape_replay")
+        fn = load().esl_tape_replay_noalloc if noalloc else load().esl_tape_replay
+        _check(fn(self.h, _ptr(a), _ptr(rhs), lanes, growth, _ptr(lx), _ptr(ux), _ptr(ud), _ptr(dx), _ptr(fail)),
+               "esl_tape_replay_noalloc" if noalloc else "esl_tape_replay")
         return {"lx": lx[:, : self.nnz_l], "ux": ux[:, : self.nnz_u], "udiag": ud, "dx": dx, "fail": fail}
 
 

@@ -436,6 +436,19 @@ esl_status esl_tape_replay(esl_tape *t, const double *a, const double *rhs,
                            size_t lanes, double growth, double *lx,
                            double *ux, double *udiag, double *dx,
                            uint32_t *fail);
+/* No counterpart in the reference: esl_tape_replay for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments and results (bitwise); the device staging of
+ * the inputs and outputs lives in one fixed 32 MiB array per GPU that the
+ * library holds until the process ends, and the call allocates nothing.
+ * A call whose lanes need more than that (lanes * (nnz + 3n + nnz_l + nnz_u)
+ * doubles plus lanes flags, each region rounded up to 256 bytes) returns
+ * ESL_BUFFER_TOO_SMALL with a message; split the lanes. Calls that overlap
+ * on one GPU share the array and race: serialize them. */
+esl_status esl_tape_replay_noalloc(esl_tape *t, const double *a,
+                                   const double *rhs, size_t lanes,
+                                   double growth, double *lx, double *ux,
+                                   double *udiag, double *dx, uint32_t *fail);
 
 /* ---- Learned warm start (no counterpart in the reference) ----
  * A multilayer perceptron n_in -> n_hidden (x n_hidden_layers, SiLU) -> n_out
