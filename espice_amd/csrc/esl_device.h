@@ -236,6 +236,9 @@ __device__ inline double mos_eval_v(const Mos &m, double xd, double xg, double x
 // recombination, knee currents or sidewall) ----
 struct Dio {
     double is, vte, vcrit, gmin, cjo, vj, mt, qexp, vfc, f1, inv_f2, f3, tt;
+    double bv, xbv, vtebrk;       // breakdown, armed when bv > 0
+    double ikf;                   // high-injection knee IKF * area, 0 off
+    double isr, vterec;           // recombination ISR(T) and NR vt, isr 0 off
 };
 
 __device__ inline Dio dio_load(const DeckProg &dp, uint32_t j, const double *live, uint32_t live_stride = 1)
@@ -258,11 +261,27 @@ __device__ inline Dio dio_load(const DeckProg &dp, uint32_t j, const double *liv
     d.inv_f2 = f[DF_INVF2 * nd + j];
     d.f3 = f[DF_F3 * nd + j];
     d.tt = f[DF_TT * nd + j];
+    // the breakdown knee (diotemp.c): where -is(T) exp(-(xbv + vd) / vtebrk)
+    // carries IBV at vd = -BV; it moves with a per-point IS
+    d.bv = f[DF_BV * nd + j];
+    d.vtebrk = f[DF_VTEBRK * nd + j];
+    d.xbv = d.bv;
+    const double ibv = f[DF_IBV * nd + j], vt = f[DF_VT * nd + j];
+    if (d.bv > 0.0 && ibv >= d.is * d.bv / vt) {
+        double xbv = d.bv - d.vtebrk * esl::log(1.0 + ibv / d.is);
+        xbv = d.bv - d.vtebrk * esl::log(ibv / d.is + 1.0 - xbv / vt);
+        d.xbv = xbv;
+    }
+    d.ikf = f[DF_IKF * nd + j];
+    d.isr = f[DF_ISR * nd + j];
+    d.vterec = f[DF_VTEREC * nd + j];
     return d;
 }
 
-// Junction current with the 80 vte tangent continuation and the cubic
-// reverse tail, plus gmin * vd; g = dI/dvd.
+// I = ihi + gmin vd (diode.va:264-314): the junction current with the 80
+// vte tangent and the cubic reverse tail, the ISR/NR recombination current
+// in forward bias, the breakdown branch past -xbv, the IKF knee roll-off;
+// g = dI/dvd.
 __device__ inline double dio_current(const Dio &d, double vd, double &g)
 {
     double id, gd;
@@ -278,8 +297,36 @@ __device__ inline double dio_current(const Dio &d, double vd, double &g)
         id = -d.is * (1.0 + w3);
         gd = 3.0 * d.is * w3 / vd;
     }
-    g = gd + d.gmin;
-    return id + d.gmin * vd;
+    double irec = 0.0, grec = 0.0;
+    if (d.isr > 0.0 && vd >= -3.0 * d.vte) {
+        const double ur = vd / d.vterec;
+        const double url = fmin(ur, 80.0);
+        const double er = esl::exp(url);
+        const double a = 1.0 - vd / d.vj;
+        const double gg = a * a + 0.005;
+        const double genfac = esl::exp((d.mt / 2.0) * esl::log(gg));
+        const double base = er * (1.0 + (ur - url)) - 1.0;
+        irec = d.isr * base * genfac;
+        const double dgen = genfac * (d.mt / 2.0) * (-2.0 * a / d.vj) / gg;
+        grec = d.isr * (er / d.vterec * genfac + base * dgen);
+    }
+    if (d.bv > 0.0 && vd < -3.0 * d.vte && vd < -d.xbv) {
+        const double wb = -(d.xbv + vd) / d.vtebrk;
+        const double wl = fmin(wb, 80.0);
+        const double e = esl::exp(wl);
+        id = -d.is * e * (1.0 + (wb - wl));
+        gd = d.is * e / d.vtebrk;
+    }
+    double ihi = id + irec, ghi = gd + grec;
+    if (d.ikf > 0.0 && ihi > 0.0) {
+        // ihi / (1 + s), s = sqrt(ihi / ikf): d/dihi = (1 + s/2) / (1 + s)^2
+        const double sq = sqrt(ihi / d.ikf);
+        const double den = 1.0 + sq;
+        ghi = ghi * ((1.0 + 0.5 * sq) / (den * den));
+        ihi = ihi / den;
+    }
+    g = ghi + d.gmin;
+    return ihi + d.gmin * vd;
 }
 
 // Junction charge (depletion, two-branch SPICE form) plus tt * I; c = dq/dvd.

@@ -49,6 +49,7 @@ enum BjtField : uint32_t {
 // Diode fields (prepared on the host): is_t = IS * DF_IS_SCALE.
 enum DioField : uint32_t {
     DF_IS = 0, DF_IS_SCALE, DF_VTE, DF_GMIN, DF_CJO, DF_VJ, DF_MT, DF_QEXP, DF_VFC, DF_F1, DF_INVF2, DF_F3, DF_TT,
+    DF_BV, DF_IBV, DF_VT, DF_VTEBRK, DF_IKF, DF_ISR, DF_VTEREC,
     DF_COUNT
 };
 
@@ -286,6 +287,30 @@ constexpr int kProfPhases = 4;
 // t = (s + 1) dt; step 0 is backward Euler (ag0 = 1/dt, ag1 = 0), later
 // steps the trapezoid with xmu = 0.5 (ag0 = 2/dt, ag1 = 1).
 enum : uint32_t { kTranRunning = 102 };
+
+// Per-lane step control of an adaptive transient lane (tran.zig Sim: the
+// time, the step, the last two accepted steps, the order, counts and the
+// breakpoint clamp).
+struct AdaptLane {
+    double t, dt, dt_old0, dt_old1, bp_next, bp_save_dt;   // bp_next: inf when none
+    uint32_t order, steps, rejects, bp_clamped;
+};
+
+// An adaptive transient (TranArgs::adaptive): the lanes choose their steps
+// (tran.zig:728-1086 for the trapezoid: orders 1 and 2, the LTE bound over
+// the row plane of charges, accept/reject, order promotion, step-source
+// breakpoints). q2, q3 and the accepted point xc per lane in HBM; the
+// accepted times and probe rows go to times [N][max_steps] and
+// TranArgs::probes [N][max_steps][n_probes].
+struct AdaptArgs {
+    AdaptLane *st;                 // [N]
+    double *q2, *q3, *xc;          // [N][n]
+    double *times;                 // [N][max_steps]
+    const double *bps;             // step-source edges after 0, ascending
+    double t_stop, dt_max, dt_min, trtol, chgtol, delmin, min_break;
+    uint32_t n_bps, max_steps, attempts, any_charge;   // attempts: per launch
+};
+
 struct TranArgs {
     OpArgs op;                  // deck, tapes, lane ids, live, x, lim, resume, status, iterations, tolerances
     double *q1, *ip;            // [N][n] charges of the last accepted step, dynamic currents
@@ -298,7 +323,8 @@ struct TranArgs {
     uint32_t init;              // 1: the first launch (q1 from the operating point, ip = 0)
     uint32_t dump;              // 1: only write the first step's first Jacobian to op.a_out
     uint32_t tran_tape;         // batch: the tape a lane's transient starts on
-    uint32_t pad;
+    uint32_t adaptive;          // 1: steps chosen per lane by `ad` (s0, s1, dt unused past the lead)
+    AdaptArgs ad;
 };
 
 // The persistent batch: lanes of several templates (engines) in one launch

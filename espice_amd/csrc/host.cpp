@@ -1052,6 +1052,19 @@ void dio_fields(const double *p, double *f)
     f[esl::DF_INVF2] = 1.0 / pw(1.0 - fc, 1.0 + mt);
     f[esl::DF_F3] = 1.0 - fc * (1.0 + mt);
     f[esl::DF_TT] = p[ESL_D_TT];
+    // breakdown (BV(T) = BV: TCV 0), NBV defaulting to N; the knee xbv is
+    // computed on the device from the point's IS
+    f[esl::DF_BV] = p[ESL_D_BV];
+    f[esl::DF_IBV] = p[ESL_D_IBV];
+    f[esl::DF_VT] = vt;
+    f[esl::DF_VTEBRK] = (p[ESL_D_NBV] > 0 ? p[ESL_D_NBV] : n) * vt;
+    f[esl::DF_IKF] = p[ESL_D_IKF] * area;
+    // ISR(T) rides NR and reuses XTI (TLEV 0)
+    const double nr = p[ESL_D_NR] > 0 ? p[ESL_D_NR] : 2.0;
+    f[esl::DF_VTEREC] = nr * vt;
+    f[esl::DF_ISR] = p[ESL_D_ISR] > 0
+                         ? p[ESL_D_ISR] * area * esl::exp((ratio - 1.0) * eg / (nr * vt) + (xti / nr) * esl::log(ratio))
+                         : 0.0;
 }
 
 u32 find_slot(const Compiled &c, u32 r, u32 col)
@@ -1358,6 +1371,7 @@ struct Engine {
     u32 lead_id = 0;               // tape of the last solve's lead point
     DevBuf probe_ids, probe_out;
     DevBuf tq1, tip, txp, tstep, tprobes;   // transient state
+    DevBuf tq2, tq3, txc, ttimes, tadapt, tbps;   // adaptive transient state
     DevBuf gminv, srcv, xgood;               // the .op ladder: per-lane g and source scale, last converged x
     DevBuf pl_u32, pl_f64;                  // the lanes' own LU (device re-pivot)
     bool has_nominal = false;
@@ -2992,17 +3006,105 @@ esl_status tran_repivot(Engine &E, const esl::TranArgs &ta, size_t count, u32 S,
 
 }  // namespace
 
-extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t count, const esl_tran_options *opt,
-                                     double *x, double *probe_out, uint32_t *iterations, uint32_t *reasons)
+namespace {
+
+// The adaptive mode of tran_run: the caller's options and its outputs.
+struct AdaptRun {
+    esl_tran_adaptive_options o;
+    double *times;
+    uint32_t *accepted, *rejected;
+};
+
+// The launches of an adaptive transient after its lead tape: every lane
+// still running gets `attempts_per_launch` attempts per launch, on the
+// kernel with the lanes' own LU (a failed replay re-pivots in place),
+// until no lane runs; then the results.
+esl_status tran_adaptive_rounds(Engine &E, const esl::TranArgs &ta, const esl::PrivLU &own, size_t count,
+                                AdaptRun &ar, float op_ms, std::chrono::steady_clock::time_point t_start, double *x,
+                                double *probe_out, uint32_t *iterations, uint32_t *reasons)
+{
+    Engine *e = &E;
+    hipStream_t s = e->stream;
+    const u32 n = e->c.n;
+    const esl_tran_adaptive_options &o = ar.o;
+    vector<u32> stt(count);
+    HIPCHK(hipMemcpyAsync(stt.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    u32 cap = 0;
+    for (auto *t : e->tapes) cap = std::max(cap, t->prog.n_tv);
+    float kms = 0;
+    u32 launches = 0;
+    bool first = true;
+    for (;;) {
+        vector<u32> run;
+        for (size_t l = 0; l < count; l++)
+            if (stt[l] == esl::kTranRunning) run.push_back((u32)l);
+        if (run.empty()) break;
+        HIPCHK(hipMemcpyAsync(e->lane_ids.p, run.data(), run.size() * 4, hipMemcpyHostToDevice, s));
+        esl::TranArgs t = ta;
+        t.op.tapes = e->tape_table.as<esl::TapeProg>();
+        t.op.tv_cap = cap;
+        t.op.priv = own;
+        t.init = first ? 1u : 0u;
+        t.dump = 0;
+        t.s0 = 0;
+        t.s1 = ta.steps;
+        esl_status r = tran_room(*e, run.size(), cap);
+        if (r != ESL_OK) return r;
+        float ms = 0;
+        HIPCHK(hipEventRecord(e->ev0, s));
+        HIPCHK(esl_launch_tran_steps(&t, (u32)run.size(), tran_lds_bytes(e->c, cap), s));
+        HIPCHK(hipEventRecord(e->ev1, s));
+        HIPCHK(hipEventSynchronize(e->ev1));
+        HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+        kms += ms;
+        launches++;
+        first = false;
+        HIPCHK(hipMemcpyAsync(stt.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    vector<esl::AdaptLane> st(count);
+    HIPCHK(hipMemcpyAsync(x, e->x.p, count * n * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(ar.times, e->ttimes.p, count * (size_t)o.max_steps * 8, hipMemcpyDeviceToHost, s));
+    if (o.n_probes)
+        HIPCHK(hipMemcpyAsync(probe_out, e->tprobes.p, count * (size_t)o.max_steps * o.n_probes * 8,
+                              hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(st.data(), e->tadapt.p, count * sizeof(esl::AdaptLane), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(iterations, e->iters.p, count * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    uint64_t lane_it = 0;
+    for (size_t l = 0; l < count; l++) {
+        if (reasons[l] != ESL_REASON_OK) {   // the operating point failed
+            iterations[l] = 0;
+            ar.accepted[l] = ar.rejected[l] = 0;
+            continue;
+        }
+        ar.accepted[l] = st[l].steps;
+        ar.rejected[l] = st[l].rejects;
+        switch (stt[l]) {
+        case esl::kConverged: reasons[l] = ESL_REASON_OK; break;
+        case esl::kSingular: reasons[l] = ESL_REASON_SINGULAR_MATRIX; break;
+        case esl::kNoConvergence: reasons[l] = ESL_REASON_NO_CONVERGENCE; break;
+        default: reasons[l] = ESL_REASON_OTHER; break;   // factors outgrew the lane's own LU room
+        }
+        lane_it += iterations[l];
+    }
+    e->timing.launches = launches;
+    e->timing.kernel_ms = op_ms + kms;
+    e->timing.lane_iterations = lane_it;
+    e->timing.total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    return ESL_OK;
+}
+
+// esl_tran_solve and esl_tran_adaptive_solve: the operating points, the
+// transient lead tape, then the step launches (fixed steps in blocks, or
+// adaptive lanes in rounds of attempts).
+esl_status tran_run(esl_engine *h, const double *values, size_t count, const esl_tran_options &o, AdaptRun *ar,
+                    double *x, double *probe_out, uint32_t *iterations, uint32_t *reasons)
 {
     auto *e = static_cast<Engine *>(lookup(h, 1));
     if (!e) return ESL_INVALID_HANDLE;
-    if (!opt || opt->struct_size < sizeof(esl_tran_options)) return ESL_ABI_MISMATCH;
-    const esl_tran_options o = *opt;
-    if (count == 0 || count > 0x7fffffff || !x || !iterations || !reasons) return ESL_INVALID_ARGUMENT;
-    if (!(o.dt > 0) || !std::isfinite(o.dt) || o.steps == 0 || o.steps_per_launch == 0 || o.tol.max_iter == 0)
-        return ESL_INVALID_ARGUMENT;
-    if (o.n_probes && (!o.probes || !probe_out)) return ESL_INVALID_ARGUMENT;
     const Compiled &c = e->c;
     const u32 n = c.n, nnz = c.nnz;
     for (u32 k = 0; k < o.n_probes; k++)
@@ -3034,6 +3136,13 @@ extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t
     HIPCHK(e->probe_ids.ensure(std::max<u32>(1, o.n_probes) * 4));
     if (o.n_probes) HIPCHK(hipMemcpyAsync(e->probe_ids.p, o.probes, o.n_probes * 4, hipMemcpyHostToDevice, s));
     HIPCHK(e->a_out.ensure(std::max<size_t>(1, count * (size_t)nnz) * 8));
+    if (ar) {
+        HIPCHK(e->tq2.ensure(count * n * 8));
+        HIPCHK(e->tq3.ensure(count * n * 8));
+        HIPCHK(e->txc.ensure(count * n * 8));
+        HIPCHK(e->ttimes.ensure(count * (size_t)ar->o.max_steps * 8));
+        HIPCHK(e->tadapt.ensure(count * sizeof(esl::AdaptLane)));
+    }
     HIPCHK(hipMemsetAsync(e->tstep.p, 0, count * 4, s));
     HIPCHK(hipMemsetAsync(e->iters.p, 0, count * 4, s));
     vector<u32> stt(count);
@@ -3067,6 +3176,47 @@ extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t
     ta.dt = o.dt;
     ta.steps = o.steps;
     ta.n_probes = o.n_probes;
+    vector<double> bps;
+    if (ar) {
+        // step-source edges after 0 (ESL_V_WAVE 1 at td > 0; td is not a
+        // live parameter, so every lane has the same edges)
+        for (size_t j = 0; j < e->c.v_wkind.size(); j++)
+            if (e->c.v_wkind[j] == 1 && e->c.v_td[j] > 0) bps.push_back(e->c.v_td[j]);
+        std::sort(bps.begin(), bps.end());
+        bps.erase(std::unique(bps.begin(), bps.end()), bps.end());
+        HIPCHK(e->tbps.ensure(std::max<size_t>(1, bps.size()) * 8));
+        if (!bps.empty()) HIPCHK(hipMemcpyAsync(e->tbps.p, bps.data(), bps.size() * 8, hipMemcpyHostToDevice, s));
+        const esl_tran_adaptive_options &ao = ar->o;
+        esl::AdaptArgs &ad = ta.ad;
+        ta.adaptive = 1;
+        ad.st = e->tadapt.as<esl::AdaptLane>();
+        ad.q2 = e->tq2.as<double>();
+        ad.q3 = e->tq3.as<double>();
+        ad.xc = e->txc.as<double>();
+        ad.times = e->ttimes.as<double>();
+        ad.bps = e->tbps.as<double>();
+        ad.n_bps = (u32)bps.size();
+        ad.t_stop = ao.t_stop;
+        ad.dt_max = ao.dt_max > 0 ? ao.dt_max : ao.t_stop / 50.0;
+        ad.dt_min = ao.dt_min;
+        ad.trtol = ao.trtol;
+        ad.chgtol = ao.chgtol;
+        ad.delmin = 1e-11 * ad.dt_max;
+        ad.min_break = 10.0 * ad.delmin;
+        ad.max_steps = ao.max_steps;
+        ad.attempts = ao.attempts_per_launch;
+        ad.any_charge = e->c.cslot_pat.empty() ? 0u : 1u;
+        // every lane starts at t = 0 on the first step, order 1
+        esl::AdaptLane a0{};
+        a0.t = 0.0;
+        a0.dt = ta.dt;
+        a0.dt_old0 = a0.dt_old1 = ad.dt_max;
+        a0.bp_next = std::numeric_limits<double>::infinity();
+        a0.bp_save_dt = 0.0;
+        a0.order = 1;
+        vector<esl::AdaptLane> st0(count, a0);
+        HIPCHK(hipMemcpyAsync(e->tadapt.p, st0.data(), count * sizeof(esl::AdaptLane), hipMemcpyHostToDevice, s));
+    }
     a.prof = nullptr;
     if (e->profiling) {
         HIPCHK(e->prof.ensure(count * esl::kProfPhases * 8));
@@ -3100,6 +3250,7 @@ extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t
     const esl::PrivLU own = ta.op.priv;
     ta.op.priv = esl::PrivLU{};
 
+    if (ar) return tran_adaptive_rounds(*e, ta, own, count, *ar, op_ms, t_start, x, probe_out, iterations, reasons);
     // 4. the step launches: blocks of steps_per_launch, captured in a graph
     const u32 S = o.steps_per_launch, B = (o.steps + S - 1) / S;
     auto block_args = [&](u32 b, u32 cap, u32 init) {
@@ -3248,6 +3399,79 @@ extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t
     return ESL_OK;
 }
 
+}  // namespace
+
+extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t count, const esl_tran_options *opt,
+                                     double *x, double *probe_out, uint32_t *iterations, uint32_t *reasons)
+{
+    if (!lookup(h, 1)) return ESL_INVALID_HANDLE;
+    if (!opt || opt->struct_size < sizeof(esl_tran_options)) return ESL_ABI_MISMATCH;
+    const esl_tran_options o = *opt;
+    if (count == 0 || count > 0x7fffffff || !x || !iterations || !reasons) return ESL_INVALID_ARGUMENT;
+    if (!(o.dt > 0) || !std::isfinite(o.dt) || o.steps == 0 || o.steps_per_launch == 0 || o.tol.max_iter == 0)
+        return ESL_INVALID_ARGUMENT;
+    if (o.n_probes && (!o.probes || !probe_out)) return ESL_INVALID_ARGUMENT;
+    return tran_run(h, values, count, o, nullptr, x, probe_out, iterations, reasons);
+}
+
+extern "C" void esl_default_tran_adaptive_options(esl_tran_adaptive_options *o)
+{
+    if (!o) return;
+    memset(o, 0, sizeof(*o));
+    o->struct_size = sizeof(*o);
+    esl_default_tolerances(&o->tol);
+    o->tol.max_iter = 100;
+    o->t_stop = 1e-9;
+    o->t_step = 1e-11;
+    o->dt_max = 0.0;
+    o->dt_min = 1e-18;
+    o->trtol = 7.0;
+    o->chgtol = 1e-14;
+    o->max_steps = 1000;
+    o->attempts_per_launch = 64;
+}
+
+extern "C" esl_status esl_tran_adaptive_solve(esl_engine *h, const double *values, size_t count,
+                                              const esl_tran_adaptive_options *opt, double *x, double *times,
+                                              double *probe_out, uint32_t *accepted, uint32_t *rejected,
+                                              uint32_t *iterations, uint32_t *reasons)
+{
+    auto *e = static_cast<Engine *>(lookup(h, 1));
+    if (!e) return ESL_INVALID_HANDLE;
+    if (!opt || opt->struct_size < sizeof(esl_tran_adaptive_options)) return ESL_ABI_MISMATCH;
+    const esl_tran_adaptive_options ao = *opt;
+    if (count == 0 || count > 0x7fffffff || !x || !times || !accepted || !rejected || !iterations || !reasons)
+        return ESL_INVALID_ARGUMENT;
+    if (!(ao.t_stop > 0) || !std::isfinite(ao.t_stop) || !(ao.t_step > 0) || !std::isfinite(ao.t_step) ||
+        !(ao.dt_max >= 0) || !std::isfinite(ao.dt_max) || !(ao.dt_min > 0) || !(ao.trtol > 0) ||
+        !(ao.chgtol >= 0) || ao.max_steps == 0 || ao.attempts_per_launch == 0 || ao.tol.max_iter == 0)
+        return ESL_INVALID_ARGUMENT;
+    if (ao.n_probes && (!ao.probes || !probe_out)) return ESL_INVALID_ARGUMENT;
+    if (e->big) {
+        g_last_error = "adaptive transients need lanes that fit a workgroup's LDS";
+        return ESL_INVALID_ARGUMENT;
+    }
+    // the first step (dctran.c:134, 578-586, the firsttime /10): it also
+    // sets the transient lead tape's first Jacobian
+    const double dt_max = ao.dt_max > 0 ? ao.dt_max : ao.t_stop / 50.0;
+    const double min_break = 10.0 * 1e-11 * dt_max;
+    double dt = std::min(std::min(ao.t_step, ao.t_stop / 100.0) / 10.0, dt_max);
+    double bp0 = std::numeric_limits<double>::infinity();
+    for (size_t j = 0; j < e->c.v_wkind.size(); j++)
+        if (e->c.v_wkind[j] == 1 && e->c.v_td[j] > 0 && e->c.v_td[j] >= min_break) bp0 = std::min(bp0, e->c.v_td[j]);
+    if (bp0 < std::numeric_limits<double>::infinity()) dt = std::min(dt, 0.1 * bp0);
+    dt /= 10.0;
+    esl_tran_options o;
+    esl_default_tran_options(&o);
+    o.tol = ao.tol;
+    o.dt = dt;
+    o.steps = ao.max_steps;
+    o.probes = ao.probes;
+    o.n_probes = ao.n_probes;
+    AdaptRun ar{ao, times, accepted, rejected};
+    return tran_run(h, values, count, o, &ar, x, probe_out, iterations, reasons);
+}
+
 // ---------------------------------------------------------------- batches
 namespace {
 
@@ -3380,7 +3604,9 @@ esl_status validate_batch_item(const esl_batch_item &it, Engine *&e)
     for (size_t i = 0; i < it.count * e->c.n_live; i++)
         if (!std::isfinite(it.values[i])) return ESL_INVALID_ARGUMENT;
     if (it.tol.max_iter == 0) return ESL_INVALID_ARGUMENT;
+    if (it.flags & ~(u32)ESL_OP_LADDER) return ESL_INVALID_ARGUMENT;
     if (it.analysis == ESL_BATCH_OP) return ESL_OK;
+    if (it.flags) return ESL_INVALID_ARGUMENT;   // the ladder is for .op items
     if (it.analysis != ESL_BATCH_TRAN || !it.tran) return ESL_INVALID_ARGUMENT;
     const esl_tran_options &o = *it.tran;
     if (o.struct_size < sizeof(esl_tran_options)) return ESL_ABI_MISMATCH;
@@ -4044,6 +4270,37 @@ extern "C" esl_status esl_batch_solve(const esl_batch_item *items, size_t n_item
         }
         esl_status st = batch_item_results(r);
         if (st != ESL_OK) return st;
+    }
+    // 8. the .op ladder: an item's points that the plain Newton left
+    // unconverged or singular are solved again by esl_op_solve with the
+    // ladder (its plain rung, then the gmin and source rungs); its result is
+    // theirs and their iterations add both solves
+    for (auto &r : runs) {
+        const esl_batch_item &it = *r.it;
+        if (!(it.flags & ESL_OP_LADDER) || it.analysis != ESL_BATCH_OP) continue;
+        vector<size_t> bad;
+        for (size_t l = 0; l < it.count; l++)
+            if (it.reasons[l] == ESL_REASON_NO_CONVERGENCE || it.reasons[l] == ESL_REASON_SINGULAR_MATRIX)
+                bad.push_back(l);
+        if (bad.empty()) continue;
+        const u32 n = r.e->c.n, nl = r.e->c.n_live;
+        vector<double> v(bad.size() * std::max<u32>(nl, 1)), x(bad.size() * n);
+        vector<u32> its(bad.size()), rs(bad.size());
+        for (size_t k = 0; k < bad.size(); k++)
+            std::copy(it.values + bad[k] * nl, it.values + (bad[k] + 1) * nl, v.begin() + k * nl);
+        esl_op_options oo;
+        esl_default_op_options(&oo);
+        oo.tol = it.tol;
+        oo.tiling = ESL_TILING_WAVE;
+        oo.flags = ESL_OP_LADDER;
+        esl_status st = esl_op_solve(it.engine, v.data(), bad.size(), &oo, x.data(), its.data(), rs.data());
+        if (st != ESL_OK) return st;
+        for (size_t k = 0; k < bad.size(); k++) {
+            const size_t l = bad[k];
+            std::copy(x.begin() + k * n, x.begin() + (k + 1) * n, it.x + l * n);
+            it.reasons[l] = rs[k];
+            it.iterations[l] += its[k];
+        }
     }
     if (report) {
         const size_t ss = report->struct_size;

@@ -88,6 +88,23 @@ __device__ inline void crew_follow(const DeckProg &dp, const double *live, doubl
     __syncthreads();
 }
 
+// acc plus the staged cells [c0, c1) in their order: the cells are read
+// four at a time ahead of the adds, so the chain of adds does not wait on
+// an LDS round trip per cell (the sums are the serial ones).
+__device__ inline double add_cells(double acc, const double *stc, const uint16_t *cells, uint32_t c0, uint32_t c1)
+{
+    uint32_t cc = c0;
+    for (; cc + 4 <= c1; cc += 4) {
+        const double a0 = stc[cells[cc]], a1 = stc[cells[cc + 1]], a2 = stc[cells[cc + 2]], a3 = stc[cells[cc + 3]];
+        acc = acc + a0;
+        acc = acc + a1;
+        acc = acc + a2;
+        acc = acc + a3;
+    }
+    for (; cc < c1; cc++) acc = acc + stc[cells[cc]];
+    return acc;
+}
+
 // Assembly into pattern slots (the Jacobian dump for a host re-pivot): the
 // deck's reduce tasks, one target per thread.
 __device__ void assemble_pattern(const DeckProg &dp, const double *live, const double *x, double *lim,
@@ -108,10 +125,7 @@ __device__ void assemble_pattern(const DeckProg &dp, const double *live, const d
             const uint32_t tgt = dp.task_target[t];
             if (tgt & (kCBit | kQBit)) continue;
             double *d = (tgt & kRhsBit) ? &rhs[tgt & ~kRhsBit] : &a[tgt];
-            double acc = *d;
-            const uint32_t c1 = dp.task_cell[t + 1];
-            for (uint32_t cc = dp.task_cell[t]; cc < c1; cc++) acc = acc + st[dp.cells[cc]];
-            *d = acc;
+            *d = add_cells(*d, st, dp.cells, dp.task_cell[t], dp.task_cell[t + 1]);
         }
         wsync();
     }
@@ -153,15 +167,31 @@ __device__ inline void red_flush(RedAcc &r, double *tv, double *rhs)
     }
 }
 
-__device__ inline void red_word(uint32_t w, RedAcc &r, double *tv, double *rhs, const double *st)
+// The value a reduce word reads: a target's running sum (its factor slot or
+// rhs row), a staged cell, or nothing (padding).
+__device__ inline double red_value(uint32_t w, const double *tv, const double *rhs, const double *st)
+{
+    if (w & kRedTarget) {
+        const uint32_t t = w & 0xfffffu;
+        return (w & kRedRhs) ? rhs[t] : tv[t];
+    }
+    return w < (uint32_t)(kWave * kStageEntries) ? st[w] : 0.0;
+}
+
+// Consumes word w with its value v, read ahead: a target word flushes the
+// previous target's sum and starts from the new target's running sum; a
+// cell word adds its cell. The sums are the instance-ordered ones of the
+// serial assembly; reading ahead is safe because a chunk gives each target
+// to one thread, once.
+__device__ inline void red_use(uint32_t w, double v, RedAcc &r, double *tv, double *rhs)
 {
     if (w & kRedTarget) {
         red_flush(r, tv, rhs);
         r.cur = (int)(w & 0xfffffu);
         r.rhs = (w & kRedRhs) != 0;
-        r.acc = r.rhs ? rhs[r.cur] : tv[r.cur];
+        r.acc = v;
     } else if (w < (uint32_t)(kWave * kStageEntries)) {
-        r.acc = r.acc + st[w];
+        r.acc = r.acc + v;
     }
 }
 
@@ -197,10 +227,20 @@ __device__ void assemble(const DeckProg &dp, const TapeProg &T, const double *li
                 q[k] = rs[k * kWave];
                 __builtin_amdgcn_sched_barrier(0);
             }
+            // values read kAhead words ahead of their use: a long target's
+            // chain of adds waits on the adds, not on an LDS round trip each
+            constexpr int kAhead = kRing / 2;
+            double va[kAhead];
+#pragma unroll
+            for (int k = 0; k < kAhead; k++) va[k] = red_value(q[k], tv, rhs, stc);
             for (uint32_t w = 0; w < len; w += kRing) {
 #pragma unroll
                 for (int k = 0; k < kRing; k++) {
-                    red_word(q[k], r, tv, rhs, stc);
+                    const double v = va[k % kAhead];
+                    // word w + k + kAhead: still in the ring (k < kAhead), or
+                    // the refill of the slot used kAhead words ago
+                    va[k % kAhead] = red_value(q[(k + kAhead) % kRing], tv, rhs, stc);
+                    red_use(q[k], v, r, tv, rhs);
                     q[k] = rs[(w + kRing + k) * kWave];
                     __builtin_amdgcn_sched_barrier(0);
                 }
@@ -656,10 +696,7 @@ __device__ void assemble_tran(const DeckProg &dp, const uint32_t *amap, uint32_t
                 const uint32_t idx = tgt & kTargetMask;
                 double *d = (tgt & kRhsBit) ? &rhs[idx] : (tgt & kQBit) ? &q[idx] : (tgt & kCBit) ? &cq[idx]
                                                                                                    : &tv[amap ? amap[tgt] : tgt];
-                double acc = *d;
-                const uint32_t cc1 = dp.task_cell[k + 1];
-                for (uint32_t cc = dp.task_cell[k]; cc < cc1; cc++) acc = acc + stc[dp.cells[cc]];
-                *d = acc;
+                *d = add_cells(*d, stc, dp.cells, dp.task_cell[k], dp.task_cell[k + 1]);
             }
             wsync();
         }
@@ -689,6 +726,62 @@ __device__ void assemble_tran(const DeckProg &dp, const uint32_t *amap, uint32_t
 // and limiting state reset to the accepted point. A lane whose replay fails
 // leaves its Jacobian in a_out for a host re-pivot and resumes at the same
 // iterate.
+// |a - b| within `ulps` units in the last place (same sign), a == b included.
+__device__ inline bool almost_equal_ulps(double a, double b, int64_t ulps)
+{
+    if (a == b) return true;
+    if ((a < 0) != (b < 0)) return false;
+    const int64_t ia = (int64_t)dbits(fabs(a)), ib = (int64_t)dbits(fabs(b));
+    return (ia > ib ? ia - ib : ib - ia) <= ulps;
+}
+
+// The first step-source edge at or after t_from, or inf.
+__device__ inline double next_break(const AdaptArgs &ad, double t_from)
+{
+    for (uint32_t k = 0; k < ad.n_bps; k++)
+        if (ad.bps[k] >= t_from) return ad.bps[k];
+    return __builtin_inf();
+}
+
+// The LTE bound of one step (integrator.stepBound, lane_kernels.zig:230-269)
+// over the row plane of charges: method 0 backward Euler (coefficient 0.5),
+// 1 the trapezoid (0.08333333333); cur_method the method whose dynamic
+// current the step used. q0 (LDS) the step's charges, q1..q3 the accepted
+// ring, ip the dynamic currents (HBM). NaN when any entry read is not
+// finite. Every thread returns the lane's bound.
+__device__ inline double step_bound(int method, int cur_method, const double *q0, const double *q1,
+                                   const double *q2, const double *q3, const double *ip, uint32_t n,
+                                   const AdaptArgs &ad, double reltol, double abstol, double dt, double dt1,
+                                   double dt2, double ag0, double ag1, uint32_t tid)
+{
+    const double coeff = method == 0 ? 0.5 : 0.08333333333;
+    const double sum01 = dt + dt1, sum12 = dt1 + dt2, sum012 = dt + (dt1 + dt2);
+    double vmin = __builtin_inf();
+    bool poison = false;
+    for (uint32_t j = tid; j < n; j += kWave) {
+        const double qc = q0[j], qp = q1[j], qp2 = q2[j], qp3 = q3[j], i_p = ip[j];
+        const double i_new = cur_method == 1 ? ag0 * (qc - qp) - ag1 * i_p : ag0 * (qc - qp);
+        const double volttol = abstol + reltol * fmax(fabs(i_new), fabs(i_p));
+        const double chargetol = reltol * fmax(fmax(fabs(qc), fabs(qp)), ad.chgtol) / dt;
+        const double tl = fmax(volttol, chargetol);
+        poison |= !(fabs(qc) < __builtin_inf()) || !(fabs(qp) < __builtin_inf()) ||
+                  !(fabs(qp2) < __builtin_inf()) || !(fabs(i_p) < __builtin_inf());
+        const double f12 = (qp - qp2) / dt1;
+        double dd = ((qc - qp) / dt - f12) / sum01;
+        if (method != 0) {
+            poison |= !(fabs(qp3) < __builtin_inf());
+            const double f123 = (f12 - (qp2 - qp3) / dt2) / sum12;
+            dd = (dd - f123) / sum012;
+        }
+        const double del = ad.trtol * tl / fmax(abstol, coeff * fabs(dd));
+        poison |= del != del;
+        vmin = fmin(vmin, del);
+    }
+    vmin = -wave_fmax(-vmin);
+    if (wave_any(poison)) return __builtin_nan("");
+    return method == 0 ? vmin : sqrt(vmin);
+}
+
 template <bool kOwnLU>
 __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, double *base, uint32_t nw)
 {
@@ -753,6 +846,16 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
         wsync();
         assemble_step(priv, 0.0, 0.0, 0.0);
         for (uint32_t i = tid; i < n; i += kWave) q1[i] = q[i];
+        if (ta.adaptive) {
+            // the charge ring starts full of the operating point's charges
+            double *q2 = ta.ad.q2 + (size_t)lane * n, *q3 = ta.ad.q3 + (size_t)lane * n;
+            double *xc = ta.ad.xc + (size_t)lane * n;
+            for (uint32_t i = tid; i < n; i += kWave) {
+                q2[i] = q[i];
+                q3[i] = q[i];
+                xc[i] = x[i];
+            }
+        }
         wsync();
         if (ta.dump) {
             // the point's first transient Jacobian (step 0, iteration 0),
@@ -767,12 +870,12 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
     double limit = rs.limit;
     uint32_t status = kTranRunning;
     uint32_t total_it = a.iterations[lane];
-    for (; s < ta.s1 && status == kTranRunning; s++) {
-        const double t = (double)(s + 1) * ta.dt;
-        const double ag0 = s == 0 ? 1.0 / ta.dt : 2.0 / ta.dt;
-        const double ag1 = s == 0 ? 0.0 : 1.0;
-        bool converged = false;
-        uint32_t iter = iter0;
+    // One step's Newton at source time t with companion coefficients ag0,
+    // ag1, from iteration `iter` on: true when it converged (x is then the
+    // published point, x_old of the converging iteration, and q its
+    // charges); a failed replay without room for the lane's own factor, or
+    // a singular matrix, sets status.
+    auto newton = [&](double t, double ag0, double ag1, uint32_t &iter) -> bool {
         for (; iter < a.max_iter; iter++) {
             assemble_step(priv, t, ag0, ag1);
             if (iter == 0) {
@@ -874,11 +977,19 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
             wsync();
             if (ok) {
                 for (uint32_t i = tid; i < n; i += kWave) x[i] = xold[i];
-                converged = true;
                 iter++;
-                break;
+                return true;
             }
         }
+        return false;
+    };
+    if (!ta.adaptive)
+    for (; s < ta.s1 && status == kTranRunning; s++) {
+        const double t = (double)(s + 1) * ta.dt;
+        const double ag0 = s == 0 ? 1.0 / ta.dt : 2.0 / ta.dt;
+        const double ag1 = s == 0 ? 0.0 : 1.0;
+        uint32_t iter = iter0;
+        const bool converged = newton(t, ag0, ag1, iter);
         if (status == kRepivot || status == kSingular) {
             total_it += iter - iter0;
             break;
@@ -919,7 +1030,132 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
             wsync();
         }
     }
-    if (status == kTranRunning && s >= ta.steps) status = kConverged;
+    if (ta.adaptive) {
+        // per-lane step control (tran.zig:728-1086, the trapezoid: orders
+        // 1 and 2), `attempts` attempts per launch
+        const AdaptArgs &ad = ta.ad;
+        AdaptLane A = ad.st[lane];
+        double *q2 = ad.q2 + (size_t)lane * n, *q3 = ad.q3 + (size_t)lane * n;
+        double *xc = ad.xc + (size_t)lane * n;
+        for (uint32_t att = 0; att < ad.attempts && status == kTranRunning; att++) {
+            if (!(A.t < ad.t_stop && !almost_equal_ulps(A.t, ad.t_stop, 100) && A.steps < ad.max_steps)) {
+                const bool reached = A.t >= ad.t_stop || almost_equal_ulps(A.t, ad.t_stop, 100);
+                status = reached ? kConverged : kNoConvergence;
+                break;
+            }
+            const int method = A.order == 1 ? 0 : 1;
+            const double ag0 = method == 0 ? 1.0 / A.dt : 1.0 / A.dt / (1.0 - 0.5);
+            const double ag1 = method == 0 ? 0.0 : 0.5 / (1.0 - 0.5);
+            if (iter0 == 0) {
+                // the first iterate: the accepted points extrapolated by
+                // dt/dt_prev (tran.zig:1131-1137), one limit pass against
+                // the accepted point (tran.zig:1138)
+                const double xfact = A.dt / A.dt_old0;
+                for (uint32_t i = tid; i < n; i += kWave) x[i] = xc[i];
+                wsync();
+                seed_lim_at(dp, live, x, lim, tid, true);
+                wsync();
+                for (uint32_t i = tid; i < n; i += kWave) x[i] = x[i] + xfact * (x[i] - xp[i]);
+                wsync();
+                limit_pass(dp, live, x, lim, tid);
+                wsync();
+            }
+            uint32_t iter = iter0;
+            const bool converged = newton(A.t + A.dt, ag0, ag1, iter);
+            total_it += iter - iter0;
+            iter0 = 0;
+            if (status == kRepivot || status == kSingular) break;
+            if (!converged) {
+                // Newton failed: dt / 8 at order 1 (dctran.c:815, 823)
+                A.rejects++;
+                A.order = 1;
+                A.dt = A.dt / 8.0;
+                if (A.dt < ad.dt_min) status = kNoConvergence;
+                continue;
+            }
+            double dt_next = A.steps == 0 ? A.dt : fmin(A.dt * 2.0, ad.dt_max);
+            if (ad.any_charge && A.steps > 0) {
+                const double del = step_bound(method, method, q, q1, q2, q3, ip, n, ad, a.reltol, a.abstol, A.dt,
+                                              A.dt_old0, A.dt_old1, ag0, ag1, tid);
+                if (!(del >= 0.9 * A.dt)) {
+                    A.rejects++;
+                    if (del != del) {
+                        A.order = 1;
+                        A.dt = A.dt / 8.0;
+                    } else {
+                        A.dt = del;
+                    }
+                    if (A.dt < ad.dt_min) status = kNoConvergence;
+                    continue;
+                }
+                dt_next = fmin(fmin(fmax(del, ad.dt_min), 2.0 * A.dt), ad.dt_max);
+            }
+            // accepted: order promotion (dctran.c:901-913)
+            if (ad.any_charge && A.steps > 0 && A.order == 1) {
+                const double trial = step_bound(1, method, q, q1, q2, q3, ip, n, ad, a.reltol, a.abstol, A.dt,
+                                                A.dt_old0, A.dt_old1, ag0, ag1, tid);
+                if (trial == trial) {
+                    const double nd2 = fmin(2.0 * A.dt, trial);
+                    if (nd2 > 1.05 * A.dt) A.order = 2;
+                    dt_next = fmin(fmax(nd2, ad.dt_min), ad.dt_max);
+                }
+            }
+            // the dynamic current under the method used, the ring, the points
+            wsync();
+            for (uint32_t i = tid; i < n; i += kWave) {
+                const double d = ag0 * (q[i] - q1[i]);
+                ip[i] = d - ag1 * ip[i];
+                q3[i] = q2[i];
+                q2[i] = q1[i];
+                q1[i] = q[i];
+                xp[i] = xc[i];
+                xc[i] = x[i];
+            }
+            A.dt_old1 = A.dt_old0;
+            A.dt_old0 = A.dt;
+            A.t = A.t + A.dt;
+            if (tid == 0) ad.times[(size_t)lane * ad.max_steps + A.steps] = A.t;
+            for (uint32_t k = tid; k < ta.n_probes; k += kWave)
+                ta.probes[((size_t)lane * ad.max_steps + A.steps) * ta.n_probes + k] = x[ta.probe_ids[k]];
+            A.steps++;
+            // landed on a breakpoint: order 1, resume at a tenth of the gap
+            // (spice3 dctran's rule); then clamp to the next one
+            if (A.bp_next < __builtin_inf()) {
+                const bool landed = A.bp_clamped ? fabs(A.t - A.bp_next) <= ad.min_break
+                                                 : (A.bp_next - A.t <= ad.delmin ||
+                                                    almost_equal_ulps(A.t, A.bp_next, 100));
+                if (landed) {
+                    A.order = 1;
+                    double shrink = A.bp_save_dt;
+                    const double nb = next_break(ad, A.t + ad.min_break);
+                    if (nb < __builtin_inf()) shrink = fmin(shrink, nb - A.t);
+                    dt_next = fmin(dt_next, 0.1 * shrink);
+                }
+            }
+            A.bp_next = next_break(ad, A.t + ad.min_break);
+            A.bp_clamped = 0;
+            if (A.bp_next < __builtin_inf() && A.bp_next - A.t < dt_next) {
+                A.bp_save_dt = dt_next;
+                dt_next = A.bp_next - A.t;
+                A.bp_clamped = 1;
+            }
+            A.dt = dt_next;
+            if (A.t + A.dt > ad.t_stop) A.dt = ad.t_stop - A.t;
+            wsync();
+        }
+        if (status == kTranRunning &&
+            !(A.t < ad.t_stop && !almost_equal_ulps(A.t, ad.t_stop, 100) && A.steps < ad.max_steps))
+            status = (A.t >= ad.t_stop || almost_equal_ulps(A.t, ad.t_stop, 100)) ? kConverged : kNoConvergence;
+        if (tid == 0) ad.st[lane] = A;
+        // a lane that ends publishes its last accepted point; one that
+        // runs on resumes from it (the next attempt predicts from xc)
+        if (status != kTranRunning) {
+            wsync();
+            for (uint32_t i = tid; i < n; i += kWave) x[i] = xc[i];
+        }
+        wsync();
+    }
+    if (!ta.adaptive && status == kTranRunning && s >= ta.steps) status = kConverged;
     for (uint32_t i = tid; i < n; i += kWave) a.x[(size_t)lane * n + i] = x[i];
     if (tid == 0) {
         if (status != kRepivot) {

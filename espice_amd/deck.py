@@ -34,9 +34,11 @@ M_DEFAULTS = {
 MG_TOX, MG_KP, MG_PHI, MG_GAMMA = 1, 2, 4, 8
 # Diode level 1 slots (ESL_D_*), models/diode.va (RS = 0 subset).
 D_SLOTS = {"is": 0, "n": 1, "cjo": 2, "vj": 3, "m": 4, "tt": 5, "fc": 6, "area": 7, "gmin": 8,
-           "temp": 9, "tnom": 10, "eg": 11, "xti": 12}
+           "temp": 9, "tnom": 10, "eg": 11, "xti": 12, "bv": 13, "ibv": 14, "nbv": 15, "ikf": 16,
+           "isr": 17, "nr": 18}
 D_DEFAULTS = {"is": 1e-14, "n": 1.0, "cjo": 0.0, "vj": 1.0, "m": 0.5, "tt": 0.0, "fc": 0.5, "area": 1.0,
-              "gmin": 1e-12, "temp": 27.0, "tnom": 27.0, "eg": 1.11, "xti": 3.0}
+              "gmin": 1e-12, "temp": 27.0, "tnom": 27.0, "eg": 1.11, "xti": 3.0, "bv": 0.0, "ibv": 1e-3,
+              "nbv": 0.0, "ikf": 0.0, "isr": 0.0, "nr": 2.0}
 # Gummel-Poon slots (ESL_Q_*), models/gummel_poon.va (RB = RC = RE = 0).
 Q_SLOTS = {"type": 0, "is": 1, "bf": 2, "nf": 3, "vaf": 4, "ikf": 5, "ise": 6, "ne": 7, "br": 8, "nr": 9,
            "var": 10, "ikr": 11, "isc": 12, "nc": 13, "nkf": 14, "iss": 15, "ns": 16, "area": 17, "m": 18,
@@ -127,12 +129,24 @@ class Deck:
         return self._add(name, BJT, (c, b, e, s), a)
 
     def diode(self, name, a_node, k_node, model: dict | None = None, **inst):
+        """Junction diode (models/diode.va). A series RS > 0 becomes a
+        resistor RS / AREA from the anode to an internal node `name#a`
+        that the junction hangs from (diode.va's rsb branch; RS(T) = RS
+        with TRS = 0)."""
         vals = dict(D_DEFAULTS)
+        rs = 0.0
         for k, v in {**(model or {}), **inst}.items():
             k = k.lower()
+            if k == "rs":
+                rs = float(v)
+                continue
             if k not in D_SLOTS:
                 raise ValueError(f"unsupported diode parameter {k}")
             vals[k] = float(v)
+        if rs > 0:
+            inner = f"{name}#a"
+            self.resistor(f"{name}#rs", a_node, inner, max(rs / vals["area"], 1e-9))
+            a_node = inner
         a = np.zeros(NPARAM)
         for k, slot in D_SLOTS.items():
             a[slot] = vals[k]

@@ -40,7 +40,8 @@ EXPORTS = [
     "esl_mlp_default_train_options", "esl_mlp_create", "esl_mlp_destroy", "esl_mlp_param_count",
     "esl_mlp_get_params", "esl_mlp_set_params", "esl_mlp_set_scaling", "esl_mlp_gradient", "esl_mlp_train",
     "esl_mlp_predict", "esl_engine_set_warm_start", "esl_default_tran_options", "esl_tran_solve",
-    "esl_default_batch_options", "esl_batch_solve", "esl_error_message",
+    "esl_default_batch_options", "esl_batch_solve", "esl_error_message", "esl_default_tran_adaptive_options",
+    "esl_tran_adaptive_solve",
 ]
 
 
@@ -97,9 +98,16 @@ class TranOptions(C.Structure):
                 ("progress_steps", C.c_uint32), ("reserved", C.c_uint32)]
 
 
+class TranAdaptiveOptions(C.Structure):
+    _fields_ = [("struct_size", C.c_size_t), ("tol", Tolerances), ("t_stop", C.c_double), ("t_step", C.c_double),
+                ("dt_max", C.c_double), ("dt_min", C.c_double), ("trtol", C.c_double), ("chgtol", C.c_double),
+                ("max_steps", C.c_uint32), ("attempts_per_launch", C.c_uint32), ("probes", C.c_void_p),
+                ("n_probes", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 class BatchItem(C.Structure):
     _fields_ = [("engine", C.c_void_p), ("values", C.c_void_p), ("count", C.c_size_t), ("analysis", C.c_uint32),
-                ("reserved0", C.c_uint32), ("tol", Tolerances), ("tran", C.POINTER(TranOptions)),
+                ("flags", C.c_uint32), ("tol", Tolerances), ("tran", C.POINTER(TranOptions)),
                 ("x", C.c_void_p), ("probe_out", C.c_void_p), ("iterations", C.c_void_p), ("reasons", C.c_void_p)]
 
 
@@ -186,6 +194,10 @@ def load() -> C.CDLL:
     L.esl_default_tran_options.argtypes = [C.POINTER(TranOptions)]
     L.esl_tran_solve.restype = C.c_uint32
     L.esl_tran_solve.argtypes = [vp, vp, C.c_size_t, C.POINTER(TranOptions), vp, vp, vp, vp]
+    L.esl_default_tran_adaptive_options.argtypes = [C.POINTER(TranAdaptiveOptions)]
+    L.esl_tran_adaptive_solve.restype = C.c_uint32
+    L.esl_tran_adaptive_solve.argtypes = [vp, vp, C.c_size_t, C.POINTER(TranAdaptiveOptions), vp, vp, vp, vp, vp,
+                                          vp, vp]
     L.esl_engine_set_warm_start.restype = C.c_uint32
     L.esl_engine_set_warm_start.argtypes = [vp, vp, vp]
     L.esl_error_message.restype = C.c_uint32
@@ -329,6 +341,37 @@ class Engine:
                                      _ptr(po) if len(pr) else None, _ptr(it), _ptr(rs)), "esl_tran_solve")
         return {"x": x, "probes": po[:, :, :len(pr)], "iterations": it, "reasons": rs}
 
+    def solve_tran_adaptive(self, values, t_stop, t_step, probes=(), dt_max=0.0, dt_min=1e-18, trtol=7.0,
+                            chgtol=1e-14, max_steps=1000, attempts_per_launch=64, max_iter=None):
+        """Adaptive transient lanes (esl_tran_adaptive_solve): each point's
+        operating point, then steps the lane chooses up to t_stop. Returns
+        x [count][n] (last accepted state), times [count][max_steps],
+        probes [count][max_steps][n_probes] (rows past `accepted` unused),
+        accepted, rejected, iterations, reasons."""
+        values = np.ascontiguousarray(np.atleast_2d(np.asarray(values, dtype=np.float64)))
+        count = values.shape[0]
+        o = TranAdaptiveOptions()
+        load().esl_default_tran_adaptive_options(C.byref(o))
+        o.t_stop, o.t_step, o.dt_max, o.dt_min = t_stop, t_step, dt_max, dt_min
+        o.trtol, o.chgtol, o.max_steps, o.attempts_per_launch = trtol, chgtol, max_steps, attempts_per_launch
+        if max_iter is not None:
+            o.tol.max_iter = max_iter
+        pr = np.ascontiguousarray(probes, dtype=np.uint32)
+        if len(pr):
+            o.probes, o.n_probes = pr.ctypes.data, len(pr)
+        x = np.zeros((count, self.n))
+        tm = np.zeros((count, max_steps))
+        po = np.zeros((count, max_steps, max(len(pr), 1)))
+        acc = np.zeros(count, np.uint32)
+        rej = np.zeros(count, np.uint32)
+        it = np.zeros(count, np.uint32)
+        rs = np.zeros(count, np.uint32)
+        _check(load().esl_tran_adaptive_solve(self.h, _ptr(values), count, C.byref(o), _ptr(x), _ptr(tm),
+                                              _ptr(po) if len(pr) else None, _ptr(acc), _ptr(rej), _ptr(it),
+                                              _ptr(rs)), "esl_tran_adaptive_solve")
+        return {"x": x, "times": tm, "probes": po[:, :, :len(pr)], "accepted": acc, "rejected": rej,
+                "iterations": it, "reasons": rs}
+
     def set_warm_start(self, net: "Mlp | None", x_nominal=None):
         """Attaches the warm-start network and the nominal operating point."""
         if net is None:
@@ -350,7 +393,8 @@ def solve_batch(jobs, tiling=TILING_AUTO, tune=False):
 
     jobs: list of dicts {"engine": Engine, "values": [count][n_live],
     "analysis": BATCH_OP | BATCH_TRAN, and for BATCH_TRAN "dt", "steps",
-    optional "probes", "max_iter"; for BATCH_OP optional "tol"}. Returns
+    optional "probes", "max_iter"; for BATCH_OP optional "tol" and
+    "ladder" (True: unconverged points climb the .op ladder)}. Returns
     (results, report): per job a dict like Engine.solve_op / solve_tran."""
     L = load()
     n = len(jobs)
@@ -363,6 +407,7 @@ def solve_batch(jobs, tiling=TILING_AUTO, tune=False):
         it = items[k]
         it.engine, it.values, it.count = eng.h, _ptr(values), count
         it.analysis = j.get("analysis", BATCH_OP)
+        it.flags = OP_LADDER if j.get("ladder") else 0
         t = Tolerances()
         o = OpOptions()
         L.esl_default_op_options(C.byref(o))

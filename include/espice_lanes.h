@@ -87,6 +87,11 @@ enum {
     ESL_D_IS = 0, ESL_D_N = 1, ESL_D_CJO = 2, ESL_D_VJ = 3, ESL_D_M = 4,
     ESL_D_TT = 5, ESL_D_FC = 6, ESL_D_AREA = 7, ESL_D_GMIN = 8,
     ESL_D_TEMP = 9, ESL_D_TNOM = 10, ESL_D_EG = 11, ESL_D_XTI = 12,
+    /* breakdown BV (0: off), IBV, NBV (0: N); high-injection knee IKF (0:
+     * off); recombination ISR (0: off), NR. A series RS is its own
+     * ESL_DEV_R to an internal anode node (the builder adds it). */
+    ESL_D_BV = 13, ESL_D_IBV = 14, ESL_D_NBV = 15, ESL_D_IKF = 16,
+    ESL_D_ISR = 17, ESL_D_NR = 18,
     /* V source waveform (transient): slot 0 is the value at t <= td (and
      * the operating point's); ESL_V_WAVE 1 = step to ESL_V_V2 after
      * ESL_V_TD seconds. */
@@ -295,6 +300,49 @@ esl_status esl_tran_solve(esl_engine *e, const double *values, size_t count,
                           const esl_tran_options *opt, double *x, double *probe_out,
                           uint32_t *iterations, uint32_t *reasons);
 
+/* ---- Adaptive transients: step control per lane on the device. Replaces
+ * the host step control of tran.runGpuLanes and its round calls
+ * (src/analysis/tran/tran.zig:728-1086, 1654-1820; GpuLanes.predictRound /
+ * probe / apply, src/analysis/gpu_lanes.zig:699-713,890-969) for the
+ * trapezoid method: each lane runs Sim.begin/Sim.end itself -- the
+ * predictor, Newton, the LTE bound of integrator.stepBound
+ * (src/solver/lane_kernels.zig:230-269, ngspice CKTterr coefficients, over
+ * the row plane of charges), rejection below 0.9 dt, growth at most 2x and
+ * at most dt_max, order 1 -> 2 promotion, dt / 8 at order 1 after a failed
+ * Newton, landing on step-source edges. The first dt is
+ * min(min(t_step, t_stop/100)/10, dt_max), cut to a tenth of the first
+ * edge, /10 (dctran.c's first step). Lanes run `attempts_per_launch`
+ * attempts per launch until every lane reaches t_stop or fails. A lane
+ * whose replay fails re-pivots on the device; one whose factors outgrow
+ * that room ends with ESL_REASON_OTHER. Not for engines whose lanes exceed
+ * a workgroup's LDS. ---- */
+typedef struct {
+    size_t struct_size;        /* sizeof(esl_tran_adaptive_options) */
+    esl_tolerances tol;        /* each step's Newton (max_iter default 100) */
+    double t_stop;             /* s */
+    double t_step;             /* the print step: sets the first dt */
+    double dt_max;             /* 0: t_stop / 50 (ngspice's tmax) */
+    double dt_min;             /* a dt below it ends the lane (1e-18) */
+    double trtol, chgtol;      /* 7, 1e-14 (core/numerics.zig:377-404) */
+    uint32_t max_steps;        /* accepted steps per lane: rows of times and probe_out */
+    uint32_t attempts_per_launch;  /* default 64 */
+    const uint32_t *probes;    /* unknowns recorded at every accepted point */
+    uint32_t n_probes;
+    uint32_t reserved;
+} esl_tran_adaptive_options;
+void esl_default_tran_adaptive_options(esl_tran_adaptive_options *opt);
+/* values [count][n_live]. Outputs x [count][n] (the last accepted state),
+ * times [count][max_steps] and probe_out [count][max_steps][n_probes] (the
+ * accepted points; rows past accepted[l] are untouched; probe_out may be
+ * NULL when n_probes = 0), accepted / rejected [count] (steps),
+ * iterations [count] (Newton iterations of every attempt), reasons [count]
+ * (NO_CONVERGENCE: dt fell below dt_min or max_steps ran out). */
+esl_status esl_tran_adaptive_solve(esl_engine *e, const double *values, size_t count,
+                                   const esl_tran_adaptive_options *opt, double *x,
+                                   double *times, double *probe_out, uint32_t *accepted,
+                                   uint32_t *rejected, uint32_t *iterations,
+                                   uint32_t *reasons);
+
 /* ---- Persistent uneven batches (no reference counterpart: the reference
  * forms one batch per template and options, executor.zig:480-520,
  * session.zig:562-590). Points of several templates, .op and transient
@@ -319,7 +367,13 @@ typedef struct {
     const double *values;      /* [count][n_live] */
     size_t count;
     uint32_t analysis;         /* ESL_BATCH_* */
-    uint32_t reserved0;
+    uint32_t flags;            /* ESL_BATCH_OP: ESL_OP_LADDER sends the points the
+                                  batch's plain Newton leaves unconverged or singular
+                                  up the .op ladder (dc/op.zig:160-288): they are
+                                  solved again by esl_op_solve with ESL_OP_LADDER
+                                  (its plain rung, then the gmin and source rungs),
+                                  whose result is theirs; iterations add both solves.
+                                  0: no ladder. Other bits must be 0. */
     esl_tolerances tol;        /* the operating points' Newton, both analyses (esl_tran_solve
                                   runs its points' with the transient's tolerances and at
                                   least 100 iterations) */

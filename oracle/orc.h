@@ -213,4 +213,25 @@ int orc_tran_solve_traced(orc_ckt *c, const double *values, size_t count, const 
                           const orc_tol *tol, double dt, uint32_t steps, const uint32_t *probes,
                           uint32_t n_probes, const uint32_t *q, int threads, double *x, double *probe_out,
                           uint32_t *iterations, uint32_t *reasons, double *first_x, double *first_lim);
+/* Adaptive transient lanes (tran.zig Sim.begin/end, 728-1086, trapezoid
+ * method): per lane the operating point (tol_op), then steps whose dt the
+ * lane chooses: the first dt = min(min(t_step, t_stop/100)/10, dt_max),
+ * cut to a tenth of the first step-source edge, /10; Newton failure -> dt/8
+ * at order 1; the LTE bound of lane_kernels.zig:230-269 (ngspice CKTterr,
+ * trapezoid coefficient 0.08333333333, BE 0.5, over the row plane of
+ * charges) rejects below 0.9 dt and sets the next dt (growth at most 2x,
+ * at most dt_max); order 1 -> 2 when the trapezoid's bound beats 1.05 dt;
+ * landing on a step-source edge drops to order 1. times [count][max_steps]
+ * and probe_out [count][max_steps][n_probes] get the accepted points;
+ * accepted / rejected count steps; reasons 1 when dt fell below dt_min or
+ * max_steps ran out. Parity pinned by tests (RC closed forms, the
+ * trapezoid's error law, the step-size law of the bound). */
+typedef struct {
+    double t_stop, t_step, dt_max, dt_min, trtol, chgtol;
+    uint32_t max_steps, reserved;
+} orc_tran_adapt;
+int orc_tran_adaptive(orc_ckt *c, const double *values, size_t count, const orc_tol *tol_op, const orc_tol *tol,
+                      const orc_tran_adapt *o, const uint32_t *probes, uint32_t n_probes, const uint32_t *q,
+                      int threads, double *x, double *times, double *probe_out, uint32_t *accepted,
+                      uint32_t *rejected, uint32_t *iterations, uint32_t *reasons);
 #endif

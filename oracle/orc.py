@@ -37,6 +37,11 @@ class Tol(C.Structure):
     ]
 
 
+class TranAdapt(C.Structure):
+    _fields_ = [("t_stop", C.c_double), ("t_step", C.c_double), ("dt_max", C.c_double), ("dt_min", C.c_double),
+                ("trtol", C.c_double), ("chgtol", C.c_double), ("max_steps", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 class Prec(C.Structure):
     _fields_ = [("precision", C.c_uint32), ("refine_max", C.c_uint32), ("refine_tol", C.c_double),
                 ("nominal_x", C.c_void_p)]
@@ -95,6 +100,10 @@ def lib():
         L.orc_tran_solve_traced.argtypes = [C.c_void_p, f64p, C.c_size_t, C.POINTER(Tol), C.POINTER(Tol),
                                             C.c_double, C.c_uint32, C.c_void_p, C.c_uint32, u32p, C.c_int, f64p,
                                             C.c_void_p, u32p, u32p, C.c_void_p, C.c_void_p]
+        L.orc_tran_adaptive.restype = C.c_int
+        L.orc_tran_adaptive.argtypes = [C.c_void_p, f64p, C.c_size_t, C.POINTER(Tol), C.POINTER(Tol),
+                                        C.POINTER(TranAdapt), C.c_void_p, C.c_uint32, u32p, C.c_int, f64p,
+                                        C.c_void_p, C.c_void_p, u32p, u32p, u32p, u32p]
         L.orc_op_solve.restype = C.c_int
         L.orc_op_solve.argtypes = [C.c_void_p, f64p, C.c_size_t, C.POINTER(Tol), u32p, C.c_void_p,
                                    C.c_int, f64p, u32p, u32p, u32p]
@@ -317,3 +326,30 @@ def tran(circuit, values, q, dt, steps, probes=(), threads=1, tol=None, tol_op=N
 
 def natural_order(n):
     return np.arange(n, dtype=np.uint32)
+
+
+def tran_adaptive(circuit, values, q, t_stop, t_step, dt_max=None, dt_min=1e-18, trtol=7.0, chgtol=1e-14,
+                  max_steps=4000, probes=(), threads=1, tol=None, tol_op=None):
+    """Adaptive transient lanes on the oracle (orc_tran_adaptive). dt_max
+    defaults to t_stop / 50 (ngspice's tmax). Returns x (last accepted
+    state), times [count][max_steps], probes [count][max_steps][n_probes]
+    (rows beyond `accepted` are zero), accepted, rejected, iterations,
+    reasons."""
+    values = _f64(np.atleast_2d(values))
+    count = values.shape[0]
+    t = Tol(1e-3, 1e-12, 1e-6, 1e-9, 100, 0) if tol is None else tol
+    to = Tol(1e-3, 1e-12, 1e-6, 1e-9, 100, 0) if tol_op is None else tol_op
+    o = TranAdapt(float(t_stop), float(t_step), float(t_stop / 50.0 if dt_max is None else dt_max), float(dt_min),
+                  float(trtol), float(chgtol), int(max_steps), 0)
+    pr = np.ascontiguousarray(probes, dtype=np.uint32)
+    x = np.zeros((count, circuit.n))
+    tm = np.zeros((count, max_steps))
+    po = np.zeros((count, max_steps, max(len(pr), 1)))
+    acc = np.zeros(count, np.uint32)
+    rej = np.zeros(count, np.uint32)
+    it = np.zeros(count, np.uint32)
+    rs = np.zeros(count, np.uint32)
+    lib().orc_tran_adaptive(circuit.h, values, count, C.byref(to), C.byref(t), C.byref(o),
+                            pr.ctypes.data if len(pr) else None, len(pr), _u32(q), int(threads), x,
+                            tm.ctypes.data, po.ctypes.data, acc, rej, it, rs)
+    return dict(x=x, times=tm, probes=po[:, :, :len(pr)], accepted=acc, rejected=rej, iterations=it, reasons=rs)

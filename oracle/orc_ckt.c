@@ -207,6 +207,9 @@ static double limvds(double vnew, double vold)
  * recombination, knee currents or sidewall; TLEV 0 / TLEVC 0) ---- */
 typedef struct {
     double is, vte, vcrit, gmin, cjo, vj, mt, qexp, vfc, f1, inv_f2, f3, tt;
+    double bv, xbv, vtebrk;       /* breakdown: armed when bv > 0 */
+    double ikf;                   /* high-injection knee IKF * area, 0 off */
+    double isr, vterec;           /* recombination current ISR(T), NR vt; isr 0 off */
 } dio_prep;
 
 static double pw(double a, double b) { return orc_exp(b * orc_log(a)); }
@@ -247,9 +250,29 @@ static void dio1_prep(const double *p, dio_prep *o)
     o->inv_f2 = 1.0 / pw(1.0 - fc, 1.0 + mt);
     o->f3 = 1.0 - fc * (1.0 + mt);
     o->tt = p[5];
+    /* breakdown (diotemp.c): the knee xbv where -is(T) exp(-(xbv + vd) /
+     * vtebrk) carries IBV at vd = -BV; NBV defaults to N (diosetup.c);
+     * TCV = 0, so BV(T) = BV */
+    const double bv = p[13], ibv = p[14], nbv = p[15] > 0 ? p[15] : n;
+    o->bv = bv;
+    o->vtebrk = nbv * vt;
+    o->xbv = bv;
+    if (bv > 0.0 && ibv >= o->is * bv / vt) {
+        double xbv = bv - o->vtebrk * orc_log(1.0 + ibv / o->is);
+        xbv = bv - o->vtebrk * orc_log(ibv / o->is + 1.0 - xbv / vt);
+        o->xbv = xbv;
+    }
+    o->ikf = p[16] * area;
+    /* ISR(T) rides NR and reuses XTI (TLEV 0) */
+    const double nr = p[18] > 0 ? p[18] : 2.0;
+    o->vterec = nr * vt;
+    o->isr = p[17] > 0 ? p[17] * area * orc_exp((ratio - 1.0) * eg / o->vterec + (xti / nr) * orc_log(ratio)) : 0.0;
 }
 
-/* I = id + gmin vd with the 80 vte tangent and the cubic reverse tail */
+/* I = ihi + gmin vd (diode.va:264-314): the junction current with the 80 vte
+ * tangent and the cubic reverse tail, the ISR/NR recombination current in
+ * forward bias, the breakdown branch past -xbv, the IKF knee roll-off;
+ * g = dI/dvd */
 static double dio1_current(const dio_prep *d, double vd, double *g)
 {
     double id, gd;
@@ -265,8 +288,37 @@ static double dio1_current(const dio_prep *d, double vd, double *g)
         id = -d->is * (1.0 + w3);
         gd = 3.0 * d->is * w3 / vd;
     }
-    *g = gd + d->gmin;
-    return id + d->gmin * vd;
+    double irec = 0.0, grec = 0.0;
+    if (d->isr > 0.0 && vd >= -3.0 * d->vte) {
+        const double ur = vd / d->vterec;
+        const double url = fmin(ur, 80.0);
+        const double er = orc_exp(url);
+        const double a = 1.0 - vd / d->vj;
+        const double gg = a * a + 0.005;
+        const double genfac = orc_exp((d->mt / 2.0) * orc_log(gg));
+        const double base = er * (1.0 + (ur - url)) - 1.0;
+        irec = d->isr * base * genfac;
+        /* d genfac / dvd = genfac (mt / 2) (-2 a / vj) / gg */
+        const double dgen = genfac * (d->mt / 2.0) * (-2.0 * a / d->vj) / gg;
+        grec = d->isr * (er / d->vterec * genfac + base * dgen);
+    }
+    if (d->bv > 0.0 && vd < -3.0 * d->vte && vd < -d->xbv) {
+        const double wb = -(d->xbv + vd) / d->vtebrk;
+        const double wl = fmin(wb, 80.0);
+        const double e = orc_exp(wl);
+        id = -d->is * e * (1.0 + (wb - wl));
+        gd = d->is * e / d->vtebrk;
+    }
+    double ihi = id + irec, ghi = gd + grec;
+    if (d->ikf > 0.0 && ihi > 0.0) {
+        /* ihi / (1 + s), s = sqrt(ihi / ikf): d/dihi = (1 + s/2) / (1 + s)^2 */
+        const double sq = sqrt(ihi / d->ikf);
+        const double den = 1.0 + sq;
+        ghi = ghi * ((1.0 + 0.5 * sq) / (den * den));
+        ihi = ihi / den;
+    }
+    *g = ghi + d->gmin;
+    return ihi + d->gmin * vd;
 }
 
 /* depletion charge (two-branch form) + tt I; *c = dq/dvd */
@@ -1531,6 +1583,61 @@ static orc_lu *lu_copy(const orc_lu *s, uint32_t n, uint32_t nnz)
     return lu;
 }
 
+/* One transient step's Newton (the .op gate order with the residual gate
+ * off: only a non-finite residual refuses) at source time t with companion
+ * coefficients ag0, ag1: the replay, a re-pivot on a failed peel test,
+ * dx = -A^-1 F, the delta and plausible gates and the limit pass. On
+ * return x holds the published point (x_old of the converging iteration)
+ * and q its charges; *iters counts the iterations. Returns 0 converged, 1 no
+ * convergence within tol->max_iter, 3 singular. */
+static uint32_t tran_newton(const orc_ckt *c, lane_dev *ld, orc_lu *lu, const uint32_t *qcol, const orc_tol *tol,
+                            const uint8_t *has_c, double t, double ag0, double ag1, double *x, double *x_old,
+                            double *dx, double *g, double *cp, double *rhs, double *q, const double *q1,
+                            const double *ip, uint32_t *iters)
+{
+    const uint32_t n = c->n;
+    const double wide = 1.0 / 1e-3;
+    double limit = 1e9;
+    uint32_t iter = 0;
+    for (; iter < tol->max_iter; iter++) {
+        assemble_t(c, ld, x, t, 1, g, rhs, cp, q);
+        companion(c, has_c, ag0, ag1, g, cp, rhs, q, q1, ip);
+        if (iter == 0) limit = plausible_limit(c, x, rhs, g);
+        if (orc_lu_lane_refactor(lu, c->col_ptr, g, 1e-12, wide)) {
+            if (orc_lu_factor(lu, c->col_ptr, c->row_idx, g, qcol, 1e-3) != 0) {
+                *iters = iter;
+                return 3;
+            }
+        }
+        orc_lu_solve_neg(lu, rhs, dx);
+        uint64_t worst = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            const double xo = x[i], d = dx[i], xn = xo + d;
+            x_old[i] = xo;
+            x[i] = xn;
+            const double tl = tol->reltol * fmax(fabs(xn), fabs(xo)) + (c->current_row[i] ? tol->abstol : tol->vntol);
+            const int finite = fabs(xn) < INFINITY && fabs(d) < INFINITY;
+            const double qq = fabs(d) / tl;
+            const double score = !finite ? INFINITY : (qq == qq ? qq : 0.0);
+            const uint64_t sb = db(score);
+            if (sb > worst) worst = sb;
+        }
+        const double scaled = bd(worst);
+        const int limited = apply_limits(c, ld, x);
+        int ok = !limited && iter != 0 && scaled < 1.0;
+        if (ok)
+            for (uint32_t i = 0; i < n; i++)
+                if (!(fabs(x[i]) <= limit) || !(fabs(rhs[i]) <= INFINITY)) { ok = 0; break; }
+        if (ok) {
+            memcpy(x, x_old, (size_t)n * 8);
+            *iters = iter + 1;
+            return 0;
+        }
+    }
+    *iters = iter;
+    return 1;
+}
+
 static void tran_lane(const tjob_t *j, size_t l)
 {
     const orc_ckt *c = j->c;
@@ -1553,54 +1660,16 @@ static void tran_lane(const tjob_t *j, size_t l)
     assemble_t(c, &ld, x, 0.0, 1, g, rhs, cp, q);
     memcpy(q1, q, (size_t)n * 8);
     orc_lu *lu = lu_copy(j->lead, n, nnz);
-    const double wide = 1.0 / 1e-3;
     uint32_t reason = 0, total = 0;
     for (uint32_t s = 0; s < j->steps && reason == 0; s++) {
         const double t = (double)(s + 1) * j->dt;
         const double ag0 = s == 0 ? 1.0 / j->dt : 2.0 / j->dt;
         const double ag1 = s == 0 ? 0.0 : 1.0;
-        double limit = 1e9;
-        int converged = 0;
         uint32_t iter = 0;
-        for (; iter < tol->max_iter; iter++) {
-            assemble_t(c, &ld, x, t, 1, g, rhs, cp, q);
-            companion(c, j->has_c, ag0, ag1, g, cp, rhs, q, q1, ip);
-            if (iter == 0) limit = plausible_limit(c, x, rhs, g);
-            if (orc_lu_lane_refactor(lu, c->col_ptr, g, 1e-12, wide)) {
-                if (orc_lu_factor(lu, c->col_ptr, c->row_idx, g, j->q, 1e-3) != 0) {
-                    reason = 3;
-                    break;
-                }
-            }
-            orc_lu_solve_neg(lu, rhs, dx);
-            uint64_t worst = 0;
-            for (uint32_t i = 0; i < n; i++) {
-                const double xo = x[i], d = dx[i], xn = xo + d;
-                x_old[i] = xo;
-                x[i] = xn;
-                const double tl = tol->reltol * fmax(fabs(xn), fabs(xo)) + (c->current_row[i] ? tol->abstol : tol->vntol);
-                const int finite = fabs(xn) < INFINITY && fabs(d) < INFINITY;
-                const double qq = fabs(d) / tl;
-                const double score = !finite ? INFINITY : (qq == qq ? qq : 0.0);
-                const uint64_t sb = db(score);
-                if (sb > worst) worst = sb;
-            }
-            const double scaled = bd(worst);
-            const int limited = apply_limits(c, &ld, x);
-            int ok = !limited && iter != 0 && scaled < 1.0;
-            if (ok)
-                for (uint32_t i = 0; i < n; i++)
-                    if (!(fabs(x[i]) <= limit) || !(fabs(rhs[i]) <= INFINITY)) { ok = 0; break; }
-            if (ok) {
-                memcpy(x, x_old, (size_t)n * 8);
-                converged = 1;
-                iter++;
-                break;
-            }
-        }
+        reason = tran_newton(c, &ld, lu, j->q, tol, j->has_c, t, ag0, ag1, x, x_old, dx, g, cp, rhs, q, q1, ip,
+                             &iter);
         total += iter;
         if (reason) break;
-        if (!converged) { reason = 1; break; }
         for (uint32_t i = 0; i < n; i++) {
             const double d = ag0 * (q[i] - q1[i]);
             ip[i] = d - ag1 * ip[i];
@@ -1637,6 +1706,239 @@ static void tran_lane(const tjob_t *j, size_t l)
     orc_lu_free(lu);
     free(x); free(x_old); free(dx); free(g); free(cp); free(rhs); free(q); free(q1); free(ip); free(xp);
     lane_dev_free(&ld);
+}
+
+/* ---- adaptive transient lanes: per-lane step control (tran.zig Sim.begin
+ * and Sim.end, 728-1086, for the trapezoid method: orders 1 (backward
+ * Euler) and 2 (trapezoid), the LTE bound of integrator.stepBound /
+ * lane_kernels.zig:230-269 over the row plane, accept/reject, order
+ * promotion, breakpoints of step sources; no device state flips, VPI or
+ * delay lines, which the lane decks do not have) ---- */
+
+/* stepBound (lane_kernels.zig:230-269): method 0 BE, 1 trap; cur_method the
+ * method of the step being judged (its dynamic current) */
+static double step_bound(int method, int cur_method, const double *q0, const double *q1, const double *q2,
+                         const double *q3, const double *ip, uint32_t len, const orc_tran_adapt *o,
+                         const orc_tol *tol, double dt, double dt1, double dt2, double ag0, double ag1)
+{
+    const double coeff = method == 0 ? 0.5 : 0.08333333333;
+    const double sum01 = dt + dt1, sum12 = dt1 + dt2, sum012 = dt + (dt1 + dt2);
+    double vmin = INFINITY;
+    int poison = 0;
+    for (uint32_t j = 0; j < len; j++) {
+        const double qc = q0[j], qp = q1[j], qp2 = q2[j], qp3 = q3[j], i_p = ip[j];
+        const double i_new = cur_method == 1 ? ag0 * (qc - qp) - ag1 * i_p : ag0 * (qc - qp);
+        const double volttol = tol->abstol + tol->reltol * fmax(fabs(i_new), fabs(i_p));
+        const double chargetol = tol->reltol * fmax(fmax(fabs(qc), fabs(qp)), o->chgtol) / dt;
+        const double tl = fmax(volttol, chargetol);
+        poison |= !isfinite(qc) || !isfinite(qp) || !isfinite(qp2) || !isfinite(i_p);
+        const double f12 = (qp - qp2) / dt1;
+        double dd = ((qc - qp) / dt - f12) / sum01;
+        if (method != 0) {
+            poison |= !isfinite(qp3);
+            const double f123 = (f12 - (qp2 - qp3) / dt2) / sum12;
+            dd = (dd - f123) / sum012;
+        }
+        const double del = o->trtol * tl / fmax(tol->abstol, coeff * fabs(dd));
+        poison |= del != del;
+        vmin = fmin(vmin, del);
+    }
+    if (poison) return NAN;
+    return method == 0 ? vmin : sqrt(vmin);
+}
+
+/* the first step-source edge at or after t_from (ESL_V_WAVE 1 at td > 0),
+ * or INFINITY */
+static double next_break(const orc_ckt *c, const lane_dev *ld, double t_from)
+{
+    double bp = INFINITY;
+    for (uint32_t i = 0; i < c->n_inst; i++) {
+        const orc_inst *in = &c->inst[i];
+        if (in->kind != K_V) continue;
+        const double *p = ld->p + (size_t)in->dev * NPARAM;
+        if (p[1] == 1.0 && p[3] > 0 && p[3] >= t_from && p[3] < bp) bp = p[3];
+    }
+    return bp;
+}
+
+static int almost_equal_ulps(double a, double b, int64_t ulps)
+{
+    if (a == b) return 1;
+    if ((a < 0) != (b < 0)) return 0;
+    const int64_t ia = (int64_t)db(fabs(a)), ib = (int64_t)db(fabs(b));
+    const int64_t d = ia > ib ? ia - ib : ib - ia;
+    return d <= ulps;
+}
+
+typedef struct {
+    orc_ckt *c;
+    const double *values;
+    size_t first, count;
+    const orc_tol *tol;
+    const orc_tran_adapt *o;
+    uint32_t n_probes;
+    const uint32_t *probes;
+    const orc_lu *lead;
+    const uint32_t *q;
+    const uint8_t *has_c;
+    const double *x_op;
+    const uint32_t *op_reason;
+    int any_charge;
+    double *x, *times, *probe_out;
+    uint32_t *accepted, *rejected, *iterations, *reasons;
+} tajob_t;
+
+static void tran_adapt_lane(const tajob_t *j, size_t l)
+{
+    const orc_ckt *c = j->c;
+    const orc_tran_adapt *o = j->o;
+    const orc_tol *tol = j->tol;
+    const uint32_t n = c->n, nnz = c->nnz;
+    j->accepted[l] = 0;
+    j->rejected[l] = 0;
+    if (j->op_reason[l] != 0) {
+        memcpy(j->x + l * n, j->x_op + l * n, (size_t)n * 8);
+        j->iterations[l] = 0;
+        j->reasons[l] = j->op_reason[l];
+        return;
+    }
+    lane_dev ld;
+    lane_dev_init(c, j->values + l * c->d.n_live, &ld);
+    double *x = calloc(n + 1, 8), *x_old = calloc(n + 1, 8), *dx = calloc(n + 1, 8);
+    double *g = calloc(nnz + 1, 8), *cp = calloc(nnz + 1, 8), *rhs = calloc(n + 1, 8);
+    double *q = calloc(n + 1, 8), *ip = calloc(n + 1, 8);
+    double *q1 = calloc(n + 1, 8), *q2 = calloc(n + 1, 8), *q3 = calloc(n + 1, 8);
+    double *xc = calloc(n + 1, 8), *xp = calloc(n + 1, 8);
+    memcpy(xc, j->x_op + l * n, (size_t)n * 8);
+    memcpy(xp, xc, (size_t)n * 8);
+    memcpy(x, xc, (size_t)n * 8);
+    seed_at(c, &ld, x);
+    /* the charge rings start full of the operating point's charges */
+    assemble_t(c, &ld, x, 0.0, 1, g, rhs, cp, q);
+    memcpy(q1, q, (size_t)n * 8);
+    memcpy(q2, q, (size_t)n * 8);
+    memcpy(q3, q, (size_t)n * 8);
+    orc_lu *lu = lu_copy(j->lead, n, nnz);
+    const double dt_max = o->dt_max, delmin = 1e-11 * dt_max, min_break = 10.0 * delmin;
+    /* the first step (dctran.c:134, 578-586, firsttime /10) */
+    double dt = fmin(fmin(o->t_step, o->t_stop / 100.0) / 10.0, dt_max);
+    const double bp0 = next_break(c, &ld, min_break);
+    if (bp0 < INFINITY) dt = fmin(dt, 0.1 * bp0);
+    dt /= 10.0;
+    double dt_old0 = dt_max, dt_old1 = dt_max, t = 0.0;
+    double bp_next = INFINITY, bp_save_dt = 0.0;
+    int bp_clamped = 0, order = 1;
+    uint32_t steps = 0, rejects = 0, total = 0, reason = 0;
+    for (;;) {
+        if (!(t < o->t_stop && !almost_equal_ulps(t, o->t_stop, 100) && steps < o->max_steps)) break;
+        const int method = order == 1 ? 0 : 1;
+        const double ag0 = method == 0 ? 1.0 / dt : 1.0 / dt / (1.0 - 0.5);
+        const double ag1 = method == 0 ? 0.0 : 0.5 / (1.0 - 0.5);
+        /* the first iterate: the accepted points extrapolated by dt/dt_prev,
+         * one limit pass against the accepted point */
+        const double xfact = dt / dt_old0;
+        seed_at_von(c, &ld, xc, 1);
+        for (uint32_t i = 0; i < n; i++) x[i] = xc[i] + xfact * (xc[i] - xp[i]);
+        apply_limits(c, &ld, x);
+        uint32_t iters = 0;
+        const uint32_t nr = tran_newton(c, &ld, lu, j->q, tol, j->has_c, t + dt, ag0, ag1, x, x_old, dx, g, cp, rhs,
+                                        q, q1, ip, &iters);
+        total += iters;
+        if (nr == 3) { reason = 3; break; }
+        if (nr != 0) {
+            /* Newton failed: dt / 8 at order 1 (dctran.c:815, 823) */
+            rejects++;
+            order = 1;
+            dt = dt / 8.0;
+            if (dt < o->dt_min) { reason = 1; break; }
+            continue;
+        }
+        double dt_next = steps == 0 ? dt : fmin(dt * 2.0, dt_max);
+        if (j->any_charge && steps > 0) {
+            const double del = step_bound(method, method, q, q1, q2, q3, ip, n, o, tol, dt, dt_old0, dt_old1, ag0, ag1);
+            if (!(del >= 0.9 * dt)) {
+                rejects++;
+                if (del != del) {
+                    order = 1;
+                    dt = dt / 8.0;
+                } else {
+                    dt = del;
+                }
+                if (dt < o->dt_min) { reason = 1; break; }
+                continue;
+            }
+            dt_next = fmin(fmin(fmax(del, o->dt_min), 2.0 * dt), dt_max);
+        }
+        /* accepted: order promotion (dctran.c:901-913) */
+        if (j->any_charge && steps > 0 && order == 1) {
+            const double trial = step_bound(1, method, q, q1, q2, q3, ip, n, o, tol, dt, dt_old0, dt_old1, ag0, ag1);
+            if (trial == trial) {
+                const double nd2 = fmin(2.0 * dt, trial);
+                if (nd2 > 1.05 * dt) order = 2;
+                dt_next = fmin(fmax(nd2, o->dt_min), dt_max);
+            }
+        }
+        /* the dynamic current under the method used, then the ring */
+        for (uint32_t i = 0; i < n; i++) {
+            const double d = ag0 * (q[i] - q1[i]);
+            ip[i] = d - ag1 * ip[i];
+        }
+        memcpy(q3, q2, (size_t)n * 8);
+        memcpy(q2, q1, (size_t)n * 8);
+        memcpy(q1, q, (size_t)n * 8);
+        dt_old1 = dt_old0;
+        dt_old0 = dt;
+        memcpy(xp, xc, (size_t)n * 8);
+        memcpy(xc, x, (size_t)n * 8);
+        t = t + dt;
+        if (j->times) j->times[(size_t)l * o->max_steps + steps] = t;
+        for (uint32_t k = 0; k < j->n_probes; k++)
+            j->probe_out[((size_t)l * o->max_steps + steps) * j->n_probes + k] = xc[j->probes[k]];
+        steps++;
+        /* landed on a breakpoint: back to order 1, resume at a tenth of the
+         * gap (spice3 dctran's rule) */
+        if (bp_next < INFINITY) {
+            const int landed = bp_clamped ? fabs(t - bp_next) <= min_break
+                                          : (bp_next - t <= delmin || almost_equal_ulps(t, bp_next, 100));
+            if (landed) {
+                order = 1;
+                double shrink = bp_save_dt;
+                const double nb = next_break(c, &ld, t + min_break);
+                if (nb < INFINITY) shrink = fmin(shrink, nb - t);
+                dt_next = fmin(dt_next, 0.1 * shrink);
+            }
+            bp_next = INFINITY;
+        }
+        bp_next = next_break(c, &ld, t + min_break);
+        bp_clamped = 0;
+        if (bp_next < INFINITY) {
+            const double to_bp = bp_next - t;
+            if (to_bp < dt_next) {
+                bp_save_dt = dt_next;
+                dt_next = to_bp;
+                bp_clamped = 1;
+            }
+        }
+        dt = dt_next;
+        if (t + dt > o->t_stop) dt = o->t_stop - t;
+    }
+    if (reason == 0 && !(t >= o->t_stop || almost_equal_ulps(t, o->t_stop, 100))) reason = 1;   /* max_steps */
+    memcpy(j->x + l * n, xc, (size_t)n * 8);
+    j->iterations[l] = total;
+    j->reasons[l] = reason;
+    j->accepted[l] = steps;
+    j->rejected[l] = rejects;
+    orc_lu_free(lu);
+    free(x); free(x_old); free(dx); free(g); free(cp); free(rhs); free(q); free(ip);
+    free(q1); free(q2); free(q3); free(xc); free(xp);
+    lane_dev_free(&ld);
+}
+
+static void *run_tajob(void *arg)
+{
+    tajob_t *j = arg;
+    for (size_t l = j->first; l < j->first + j->count; l++) tran_adapt_lane(j, l);
+    return NULL;
 }
 
 static void *run_tjob(void *arg)
@@ -1718,6 +2020,88 @@ int orc_tran_solve_traced(orc_ckt *c, const double *values, size_t count, const 
         if (threads == 1) run_tjob(&jobs[0]);
         else {
             for (int t = 0; t < threads; t++) pthread_create(&th[t], NULL, run_tjob, &jobs[t]);
+            for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+        }
+        free(jobs); free(th);
+        orc_lu_free(lead);
+    }
+    free(x_op); free(op_it); free(op_rs); free(has_c);
+    return rc;
+}
+
+int orc_tran_adaptive(orc_ckt *c, const double *values, size_t count, const orc_tol *tol_op, const orc_tol *tol,
+                      const orc_tran_adapt *o, const uint32_t *probes, uint32_t n_probes, const uint32_t *q,
+                      int threads, double *x, double *times, double *probe_out, uint32_t *accepted,
+                      uint32_t *rejected, uint32_t *iterations, uint32_t *reasons)
+{
+    if (count == 0) return 0;
+    const uint32_t n = c->n, nnz = c->nnz;
+    double *x_op = calloc(count * n + 1, 8);
+    uint32_t *op_it = calloc(count + 1, 4), *op_rs = calloc(count + 1, 4);
+    orc_op_solve(c, values, count, tol_op, q, NULL, threads, x_op, op_it, op_rs, NULL);
+    size_t l0 = count;
+    for (size_t l = 0; l < count; l++)
+        if (op_rs[l] == 0) { l0 = l; break; }
+    uint8_t *has_c = calloc(nnz + 1, 1);
+    int any_charge = 0;
+    for (uint32_t i = 0; i < c->n_inst; i++) {
+        const orc_inst *in = &c->inst[i];
+        if (in->kind != K_C && in->kind != K_D) continue;
+        for (int e = 0; e < 4; e++)
+            if (in->slot[e] != TRASH) has_c[in->slot[e]] = 1;
+    }
+    for (uint32_t p = 0; p < nnz; p++) any_charge |= has_c[p];
+    int rc = 0;
+    orc_lu *lead = NULL;
+    if (l0 < count) {
+        /* the transient lead tape: point l0's first transient Jacobian (the
+         * first step's dt, backward Euler) */
+        lane_dev ld;
+        lane_dev_init(c, values + l0 * c->d.n_live, &ld);
+        const double dt_max = o->dt_max, min_break = 10.0 * 1e-11 * dt_max;
+        double dt = fmin(fmin(o->t_step, o->t_stop / 100.0) / 10.0, dt_max);
+        const double bp0 = next_break(c, &ld, min_break);
+        if (bp0 < INFINITY) dt = fmin(dt, 0.1 * bp0);
+        dt /= 10.0;
+        double *xx = malloc((size_t)(n + 1) * 8), *g = calloc(nnz + 1, 8), *cp = calloc(nnz + 1, 8);
+        double *rhs = calloc(n + 1, 8), *qq = calloc(n + 1, 8), *q1 = calloc(n + 1, 8), *ip = calloc(n + 1, 8);
+        memcpy(xx, x_op + l0 * n, (size_t)n * 8);
+        seed_at(c, &ld, xx);
+        assemble_t(c, &ld, xx, 0.0, 1, g, rhs, cp, qq);
+        memcpy(q1, qq, (size_t)n * 8);
+        assemble_t(c, &ld, xx, dt, 1, g, rhs, cp, qq);
+        companion(c, has_c, 1.0 / dt, 0.0, g, cp, rhs, qq, q1, ip);
+        lead = orc_lu_new(n, nnz);
+        if (orc_lu_factor(lead, c->col_ptr, c->row_idx, g, q, 1e-3) != 0) {
+            orc_lu_free(lead);
+            lead = NULL;
+        }
+        free(xx); free(g); free(cp); free(rhs); free(qq); free(q1); free(ip);
+        lane_dev_free(&ld);
+    }
+    if (!lead) {
+        for (size_t l = 0; l < count; l++) {
+            memcpy(x + l * n, x_op + l * n, (size_t)n * 8);
+            iterations[l] = 0;
+            accepted[l] = rejected[l] = 0;
+            reasons[l] = op_rs[l] != 0 ? op_rs[l] : 3;
+        }
+        rc = -1;
+    } else {
+        if (threads < 1) threads = 1;
+        if ((size_t)threads > count) threads = (int)count;
+        tajob_t *jobs = calloc(threads, sizeof(tajob_t));
+        pthread_t *th = calloc(threads, sizeof(pthread_t));
+        size_t per = count / threads, rem = count % threads, at = 0;
+        for (int t = 0; t < threads; t++) {
+            size_t k = per + ((size_t)t < rem ? 1 : 0);
+            jobs[t] = (tajob_t){c, values, at, k, tol, o, n_probes, probes, lead, q, has_c, x_op, op_rs, any_charge,
+                                x, times, probe_out, accepted, rejected, iterations, reasons};
+            at += k;
+        }
+        if (threads == 1) run_tajob(&jobs[0]);
+        else {
+            for (int t = 0; t < threads; t++) pthread_create(&th[t], NULL, run_tajob, &jobs[t]);
             for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
         }
         free(jobs); free(th);

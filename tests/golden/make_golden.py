@@ -18,6 +18,7 @@ import os
 REF = "/root/reference/tests/fixtures"
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ngspice_mos1.json")
 OUT_BJT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ngspice_bjt.json")
+OUT_DIODE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ngspice_diode.json")
 
 NM_REF = {"type": 1, "vto": 0.7, "kp": 200e-6, "gamma": 0.4, "phi": 0.6, "lambda": 0.02,
           "cgso": 100e-12, "cgdo": 100e-12}
@@ -135,6 +136,33 @@ BJT_CASES["bench_analog_current_mirror"] = ("op/bench_analog_current_mirror", [
     ("Q", "q2", ["out", "ref", "0", "0"], {"type": 1, "is": 1e-16, "bf": 200})], [])
 
 
+DIODE_CASES = {
+    "device_diode_iv_sweep": ("dc/device_diode_iv_sweep", [
+        ("V", "v1", ["anode", "0"], 0.0),
+        ("D", "d1", ["anode", "0"], {"is": 1e-14, "n": 1, "rs": 10, "bv": 100, "ibv": 1e-10})],
+        [["v1", "v(anode)", 1.0]]),
+    "device_diode_breakdown": ("dc/device_diode_breakdown", [
+        ("V", "v1", ["anode", "0"], 0.0), ("R", "r1", ["anode", "cathode"], 100.0),
+        ("D", "d1", ["cathode", "0"], {"is": 1e-14, "n": 1, "bv": 15, "ibv": 1e-3, "rs": 1})],
+        [["v1", "v(anode)", 1.0]]),
+    "device_diode_high_injection": ("dc/device_diode_high_injection", [
+        ("V", "v1", ["anode", "0"], 0.0),
+        ("D", "d1", ["anode", "0"], {"is": 1e-14, "n": 1, "rs": 50, "ikf": 10e-3})],
+        [["v1", "v(anode)", 1.0]]),
+    "device_diode_recombination": ("dc/device_diode_recombination", [
+        ("V", "v1", ["anode", "0"], 0.0),
+        ("D", "d1", ["anode", "0"], {"is": 1e-14, "n": 1, "isr": 1e-10, "nr": 2, "rs": 5})],
+        [["v1", "v(anode)", 1.0]]),
+}
+# dc/device_diode_temp: `.dc V1 0 0.9 0.005 TEMP -40 125 55`, the outer sweep
+# over -40, 15, 70 and 125 degC (181 rows each): one case per temperature.
+for k, t in enumerate((-40.0, 15.0, 70.0, 125.0)):
+    DIODE_CASES[f"device_diode_temp_{int(t)}".replace("-", "m")] = ("dc/device_diode_temp", [
+        ("V", "v1", ["anode", "0"], 0.0),
+        ("D", "d1", ["anode", "0"], {"is": 1e-14, "n": 1.05, "rs": 5, "xti": 3, "eg": 1.11, "temp": t})],
+        [["v1", "v(anode)", 1.0]], {"row_block": 181, "block": k})
+
+
 def write(cases, path):
     out = {"source": "OmarSiwy/ESPice 1.2.0 tests/fixtures (ngspice-44.2 oracle values)", "cases": {}}
     for name, (fixture, devices, live, *sel) in cases.items():
@@ -165,6 +193,7 @@ def write(cases, path):
 def main():
     write(CASES, OUT)
     write(BJT_CASES, OUT_BJT)
+    write(DIODE_CASES, OUT_DIODE)
 
 
 if __name__ == "__main__":

@@ -10,7 +10,8 @@ import numpy as np
 from espice_amd.deck import Deck
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-GOLDEN = [os.path.join(HERE, "ngspice_mos1.json"), os.path.join(HERE, "ngspice_bjt.json")]
+GOLDEN = [os.path.join(HERE, "ngspice_mos1.json"), os.path.join(HERE, "ngspice_bjt.json"),
+          os.path.join(HERE, "ngspice_diode.json")]
 
 
 def load():
@@ -37,6 +38,8 @@ def build(case: dict):
             d.mos1(name, *nodes, dev[3], **dev[4])
         elif kind == "Q":
             d.bjt(name, *nodes, dev[3])
+        elif kind == "D":
+            d.diode(name, *nodes, dev[3])
     for src, _col, _sign in case["live"]:
         d.bind(src, 0, src)
     cols = case["columns"]

@@ -173,3 +173,94 @@ def test_batch_rejects_bad_items(lanes):
     with pytest.raises(lanes.EslError):
         lanes.solve_batch([{"engine": eng, "values": vals, "analysis": lanes.BATCH_TRAN, "dt": -1.0, "steps": 3}])
     eng.close()
+
+
+def test_full_mixed_batch_at_its_workload(lanes, oracle):
+    """The uneven batch at its workload: 10,000 instances of the eight
+    families (inverter chains of 8-256 stages, RC ladders of 16-2048 nodes,
+    Gummel-Poon pair arrays of 4-64 pairs, MOS1 mirror trees of depth 2-8,
+    each as .op and as a 200-step transient; the largest template of each
+    family included) in one esl_batch_solve. Per template the first two
+    instances are bitwise the oracle; every converged .op point satisfies
+    KCL (its residual, from the oracle's assembly, is below the residual
+    gate); nearly every point converges."""
+    groups = circuits.mixed_batch(10000, seed=2024)
+    jobs, engines, meta = [], [], []
+    for g in groups:
+        fam = g["build"]()
+        eng = lanes.Engine(fam.deck.arrays())
+        vals = fam.sample_at(2024, g["instances"])
+        tol = lanes.default_options().tol
+        tol.max_iter = 1000 if g["family"].startswith("inverter_chain") else 100
+        j = {"engine": eng, "values": vals, "tol": tol,
+             "analysis": lanes.BATCH_OP if g["analysis"] == "op" else lanes.BATCH_TRAN,
+             "ladder": g["analysis"] == "op"}
+        if g["analysis"] == "tran":
+            j.update(dt=g["dt"], steps=g["steps"], probes=[eng.n - 1])
+        jobs.append(j)
+        engines.append(eng)
+        meta.append((g, fam))
+    largest = {}
+    for g, _ in meta:
+        largest[g["family"]] = max(largest.get(g["family"], 0), g["size"])
+    grid_max = {f[0]: max(f[2]) for f in circuits.MIXED_FAMILIES}
+    assert largest == grid_max, (largest, grid_max)
+    say(f"full mixed batch: {sum(len(j['values']) for j in jobs)} instances over {len(jobs)} templates")
+    res, rep = lanes.solve_batch(jobs, tiling=lanes.TILING_AUTO)
+    say(f"  kernel {rep.kernel_ms:.1f} ms, block jobs {rep.block_jobs}, wave jobs {rep.wave_jobs}, "
+        f"re-pivots {rep.repivots}")
+    n_all = sum(len(r["reasons"]) for r in res)
+    n_ok = sum(int((r["reasons"] == 0).sum()) for r in res)
+    say(f"  converged {n_ok} of {n_all}")
+    for (g, _), r in zip(meta, res):
+        bad = int((r["reasons"] != 0).sum())
+        if bad:
+            say(f"    {g['family']} {g['size']} ({g['analysis']}): {bad} of {len(r['reasons'])} unconverged")
+    assert n_all == 10000 and n_ok >= 0.995 * n_all
+    checked, laddered = 0, 0
+    for (g, fam), j, r in zip(meta, jobs, res):
+        c = oracle.Circuit(fam.deck.arrays())
+        _, _, q = j["engine"].pattern()
+        t = j["tol"]
+        tol = oracle.Tol(t.reltol, t.abstol, t.vntol, t.residual_tol, t.max_iter, 0)
+        k = min(2, len(j["values"]))
+        if g["analysis"] == "op":
+            ref = c.op(j["values"][:k], q, threads=2, tol=tol)
+        else:
+            ref = oracle.tran(c, j["values"][:k], q, g["dt"], g["steps"], probes=[j["engine"].n - 1],
+                              threads=2, tol_op=tol)
+        assert np.array_equal(r["reasons"][:k], ref["reasons"]), (g["family"], g["size"])
+        assert np.array_equal(r["iterations"][:k], ref["iterations"]), (g["family"], g["size"])
+        assert np.array_equal(bits(r["x"][:k]), bits(ref["x"])), (g["family"], g["size"])
+        if g["analysis"] == "tran":
+            assert np.array_equal(bits(r["probes"][:k]), bits(ref["probes"])), (g["family"], g["size"])
+        else:
+            conv = np.nonzero(r["reasons"] == 0)[0][:8]
+            for l in conv:
+                gj, f = c.assemble_at(j["values"][l], r["x"][l])
+                x = r["x"][l]
+                cp, ri, dg, _ = c.pattern()
+                s = np.abs(gj[dg])
+                s[s == 0] = 1.0
+                # the residual gate at x (converger.zig:336-346), with a
+                # factor 2 for the gate's x being the update's end point
+                gate = 2 * np.maximum(1e-9, 10 * s * (1e-3 * np.abs(x) + 1e-6))
+                assert (np.abs(f) <= gate).all(), (g["family"], g["size"], int(l))
+        checked += k
+        if g["analysis"] == "op" and (r["iterations"] > t.max_iter).any():
+            # points that climbed the ladder: the plain rung as the batch ran
+            # it (the item's lead tape), then the ladder solve of every such
+            # point of the item (its lead tape from the first of them)
+            plain = c.op(j["values"], q, threads=16, tol=tol)
+            bad = np.nonzero(plain["reasons"] != 0)[0]
+            lad = c.op(j["values"][bad], q, threads=16,
+                       tol=oracle.Tol(t.reltol, t.abstol, t.vntol, t.residual_tol, t.max_iter, 1))
+            say(f"  {g['family']} {g['size']}: {len(bad)} points on the ladder, "
+                f"{int((lad['reasons'] == 0).sum())} converge")
+            assert np.array_equal(r["reasons"][bad], lad["reasons"])
+            assert np.array_equal(r["iterations"][bad], plain["iterations"][bad] + lad["iterations"])
+            assert np.array_equal(bits(r["x"][bad]), bits(lad["x"]))
+            laddered += len(bad)
+    say(f"  {checked} points bitwise the oracle; {laddered} ladder points bitwise the oracle's ladder")
+    for e in engines:
+        e.close()

@@ -141,3 +141,68 @@ def test_progress_groups_and_cancellation(lanes):
     again = eng.solve_tran(vals, 1e-12, 25, probes=probes, steps_per_launch=5)
     assert np.array_equal(bits(again["probes"]), bits(ref["probes"]))
     eng.close()
+
+
+def adaptive_cases():
+    """(family, points, t_stop, t_step, dt_max, probes of the deck)"""
+    mesh = circuits.rc_mesh(8, diode_frac=0.15)
+    chain = circuits.inverter_chain_tran(12)
+    ladder = circuits.rc_ladder(40, tran=True, clamp_every=8)
+    pairs = circuits.diffpair_array_tran(4)
+    return [
+        ("rc_mesh_8", mesh, 48, 200e-12, 2e-12, 50e-12, ["n0_0", "n7_7", "n4_4"]),
+        ("inverter_chain_tran_12", chain, 40, 300e-12, 5e-12, 75e-12, None),
+        ("rc_ladder_tran_40", ladder, 40, 2e-9, 20e-12, 0.5e-9, None),
+        ("diffpair_array_tran_4", pairs, 24, 1e-9, 20e-12, 0.25e-9, None),
+    ]
+
+
+@pytest.mark.parametrize("case", [c[0] for c in adaptive_cases()])
+def test_adaptive_transient_bitwise_against_oracle(lanes, oracle, case):
+    """Per-lane step control on the device (esl_tran_adaptive_solve): every
+    lane's accepted times, probe rows, final state, accepted and rejected
+    steps, Newton iterations and reasons are bitwise the oracle's
+    (orc_tran_adaptive) on the same points, with several attempts per
+    launch so lanes stop and resume between launches."""
+    name, fam, count, t_stop, t_step, dt_max, pnames = [c for c in adaptive_cases() if c[0] == case][0]
+    d = fam.deck
+    eng = lanes.Engine(d.arrays())
+    _, _, q = eng.pattern()
+    probes = [d.node_index(p) for p in pnames] if pnames else [eng.n - 1, 1]
+    vals = fam.sample(77, count)
+    say(f"adaptive {name}: {count} points, t_stop {t_stop:g}")
+    got = eng.solve_tran_adaptive(vals, t_stop, t_step, probes=probes, dt_max=dt_max, max_steps=3000,
+                                  attempts_per_launch=16)
+    ref = oracle.tran_adaptive(oracle.Circuit(d.arrays()), vals, q, t_stop, t_step,
+                               dt_max=(dt_max if dt_max > 0 else None), max_steps=3000, probes=probes, threads=8)
+    say(f"  accepted {got['accepted'].mean():.1f} rejected {got['rejected'].mean():.2f} its "
+        f"{got['iterations'].mean():.1f} ok {(got['reasons'] == 0).sum()}; launches {eng.timing().launches}")
+    assert (ref["reasons"] == 0).mean() > 0.9
+    for k in ("reasons", "accepted", "rejected", "iterations"):
+        assert np.array_equal(got[k], ref[k]), k
+    assert np.array_equal(bits(got["x"]), bits(ref["x"]))
+    for l in range(count):
+        a = int(ref["accepted"][l])
+        assert np.array_equal(bits(got["times"][l, :a]), bits(ref["times"][l, :a])), l
+        assert np.array_equal(bits(got["probes"][l, :a]), bits(ref["probes"][l, :a])), l
+    eng.close()
+
+
+def test_adaptive_breakpoint_landing_bitwise_against_oracle(lanes, oracle):
+    """A step source at td > 0: the lanes land on the edge, drop to order 1
+    and restart with a tenth of the step, bitwise as the oracle."""
+    from test_oracle_adaptive import one_rc
+    d = one_rc(td=2.5e-9)
+    eng = lanes.Engine(d.arrays())
+    _, _, q = eng.pattern()
+    probes = [d.node_index("n0")]
+    got = eng.solve_tran_adaptive(np.zeros((3, 0)), 8e-9, 1e-10, probes=probes, dt_max=1e-9, attempts_per_launch=5)
+    ref = oracle.tran_adaptive(oracle.Circuit(d.arrays()), np.zeros((3, 0)), q, 8e-9, 1e-10, dt_max=1e-9,
+                               max_steps=1000, probes=probes)
+    for k in ("reasons", "accepted", "rejected", "iterations"):
+        assert np.array_equal(got[k], ref[k]), k
+    a = int(ref["accepted"][0])
+    assert np.array_equal(bits(got["times"][:, :a]), bits(ref["times"][:, :a]))
+    assert np.array_equal(bits(got["probes"][:, :a]), bits(ref["probes"][:, :a]))
+    assert np.min(np.abs(got["times"][0, :a] - 2.5e-9)) <= 1e-19
+    eng.close()

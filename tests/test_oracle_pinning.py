@@ -275,3 +275,30 @@ def test_draws_are_independent_across_keys():
     # a point's draw depends only on its key, not on the batch around it
     idx = np.array([5, 99_999, 17, 40_000], dtype=np.uint64)
     assert np.array_equal(circuits.normal(7, idx, 0), a[idx.astype(np.int64)])
+
+
+@pytest.mark.parametrize("model,volts", [
+    ({"is": 1e-14, "bv": 15.0, "ibv": 1e-3}, (-20.0, -16.0, -15.2, -14.0, -1.0, 0.3, 0.7)),
+    ({"is": 1e-14, "ikf": 10e-3}, (0.2, 0.6, 0.75, 0.9, 1.1)),
+    ({"is": 1e-14, "isr": 1e-10, "nr": 2.0, "m": 0.4, "vj": 0.8}, (-0.05, 0.1, 0.3, 0.5, 0.7)),
+    ({"is": 1e-14, "bv": 100.0, "ibv": 1e-10, "ikf": 1e-3, "isr": 1e-11}, (-101.0, -99.0, -1.0, 0.4, 0.8)),
+])
+def test_diode_breakdown_knee_recombination_jacobian(oracle, model, volts):
+    """The breakdown branch, the IKF knee and the ISR recombination current
+    (diode.va:264-314): the Jacobian equals central finite differences of
+    the residual on either side of each branch's threshold."""
+    d = one_diode(model)
+    c = oracle.Circuit(d.arrays())
+    cp, ri, _, _ = c.pattern()
+    a = d.node_index("a")
+    diag = [p for p in range(cp[a], cp[a + 1]) if ri[p] == a][0]
+    for v in volts:
+        x = np.zeros(c.n)
+        x[a] = v
+        g, _ = c.assemble_at(d.nominal(), x)
+        h = 1e-7 * max(1.0, abs(v))
+        xp, xm = x.copy(), x.copy()
+        xp[a] += h
+        xm[a] -= h
+        fd = (c.assemble_at(d.nominal(), xp)[1][a] - c.assemble_at(d.nominal(), xm)[1][a]) / (2 * h)
+        assert abs(fd - g[diag]) <= 1e-5 * abs(g[diag]) + 1e-12, (v, fd, g[diag])
