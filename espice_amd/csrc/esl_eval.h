@@ -289,6 +289,7 @@ __device__ inline void run_row(esl::uint2_t w, MP M, double growth, LuFlags &f)
 {
     const uint32_t kind = __builtin_amdgcn_readfirstlane(esl::row_kind(w.y));
     const uint32_t d = w.x & 0xffffu, b = w.x >> 16, c = w.y & 0xffffu;
+    if (kind == R_CHAIN) return;   // run after its block of rows (run_chain)
     if (kSolve && kind != R_DIV) {
         const R v = M[d], A = M[b], B = M[c];
         const R p = v - A * B;
@@ -314,15 +315,72 @@ __device__ inline void run_row(esl::uint2_t w, MP M, double growth, LuFlags &f)
     }
 }
 
+// One chain row (R_CHAIN word w): the thread's run of len terms on target
+// d, from the segment's term words, with d in a register. A block's kChainBlk
+// operand pairs are read together before its multiply-subtracts, and the
+// words of the block after next load while a block runs (the segment is
+// padded with kChainAhead blocks). Each term rounds as the R_MAC (refactor)
+// or R_MACZ (solve: a zero multiplier leaves the target) row of the same
+// term would; terms past len are skipped. Kept lean on registers: the
+// kernels that run it sit at the register budget of two wavefronts per
+// SIMD, and a spill there costs every phase of the Newton loop.
+template <typename R, bool kSolve, typename MP>
+__device__ void run_chain(esl::uint2_t w, const esl::uint2_t *segs, const uint32_t *cw, MP M, uint32_t tid)
+{
+    const uint32_t d = w.x & 0xffffu, len = w.x >> 16;
+    const esl::uint2_t sg = segs[__builtin_amdgcn_readfirstlane(w.y & 0xffffu)];
+    const uint32_t blocks = __builtin_amdgcn_readfirstlane(sg.y);
+    const __attribute__((address_space(1))) esl::uint4_t *p =
+        reinterpret_cast<const __attribute__((address_space(1))) esl::uint4_t *>(gptr(cw) + sg.x) + tid * 2;
+    auto ld = [&](uint32_t i) {
+        const __attribute__((address_space(1))) esl::uint4_t &v = p[i];
+        return esl::uint4_t{v.x, v.y, v.z, v.w};
+    };
+    constexpr uint32_t S = kWave * 2;
+    R acc = M[d];
+    auto block = [&](const esl::uint4_t &u0, const esl::uint4_t &u1, uint32_t k) {
+        const uint32_t ws[kChainBlk] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+        R A[kChainBlk], B[kChainBlk];
+#pragma unroll
+        for (int e = 0; e < kChainBlk; e++) {
+            A[e] = M[ws[e] & 0xffffu];
+            B[e] = M[ws[e] >> 16];
+        }
+#pragma unroll
+        for (int e = 0; e < kChainBlk; e++) {
+            const R t = A[e] * B[e];
+            const R nv = acc - t;
+            bool use = k * kChainBlk + e < len;
+            if (kSolve) use = use && B[e] != R(0);
+            acc = use ? nv : acc;
+        }
+    };
+    esl::uint4_t a0 = ld(0), a1 = ld(1), b0 = ld(S), b1 = ld(S + 1);
+    for (uint32_t k = 0; k < blocks; k += 2) {
+        block(a0, a1, k);
+        a0 = ld((k + 2) * S);
+        a1 = ld((k + 2) * S + 1);
+        if (k + 1 < blocks) {
+            block(b0, b1, k + 1);
+            b0 = ld((k + 3) * S);
+            b1 = ld((k + 3) * S + 1);
+        }
+    }
+    M[d] = acc;
+}
+
 // Runs a row program of `rows` rows, a multiple of kRing (its words
 // followed by at least kRing more rows, which are only loaded), with kRing
 // rows of words in flight. M
 // (R elements) and the words may be address-space qualified pointers. The
 // word loads are pinned in row order (scheduling barriers) and each refills
 // the register its row just used, so the oldest load completes first and a
-// row waits only for its own word, never for the whole ring.
+// row waits only for its own word, never for the whole ring. A chain row,
+// always a block's last row, runs after the block (segs, cw: the program's
+// chain segments and terms; null for programs without chain rows).
 template <typename R, bool kSolve = false, typename WP, typename MP>
-__device__ void run_rows(WP words, uint32_t rows, MP M, double growth, LuFlags &f, uint32_t tid)
+__device__ void run_rows(WP words, uint32_t rows, MP M, double growth, LuFlags &f, uint32_t tid,
+                         const esl::uint2_t *segs = nullptr, const uint32_t *cw = nullptr)
 {
     const WP w = words + tid;
     esl::uint2_t q[kRing];
@@ -332,6 +390,7 @@ __device__ void run_rows(WP words, uint32_t rows, MP M, double growth, LuFlags &
         __builtin_amdgcn_sched_barrier(0);
     }
     for (uint32_t row = 0; row < rows; row += kRing) {
+        const esl::uint2_t last = q[kRing - 1];
 #pragma unroll
         for (int k = 0; k < kRing; k++) {
             // use the word, then refill its register (no register rotation)
@@ -340,6 +399,8 @@ __device__ void run_rows(WP words, uint32_t rows, MP M, double growth, LuFlags &
             q[k] = esl::uint2_t{w[at].x, w[at].y};
             __builtin_amdgcn_sched_barrier(0);
         }
+        if (segs && __builtin_amdgcn_readfirstlane(esl::row_kind(last.y)) == R_CHAIN)
+            run_chain<R, kSolve>(last, segs, cw, M, tid);
     }
 }
 

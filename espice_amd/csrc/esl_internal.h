@@ -23,6 +23,9 @@ namespace esl {
 struct uint2_t {
     uint32_t x, y;
 };
+struct alignas(16) uint4_t {
+    uint32_t x, y, z, w;
+};
 
 constexpr int kWave = 64;
 constexpr int kStageEntries = 16;   // most cells one instance stages (Gummel-Poon: 12 + 4)
@@ -100,8 +103,15 @@ struct DeckProg {
     // diodes [4 * n_dio] (a-a, a-k, k-a, k-k), Gummel-Poon [12 * n_bjt] in
     // the entry order of their evaluation (the lane-per-thread kernel)
     const uint32_t *d_slot, *q_slot;
+    // the refinement residual's stream [res_steps + kRing][64] and the
+    // words it holds (the length of a lane's packed FP64 Jacobian)
+    uint32_t res_steps, res_total;
+    const uint32_t *res_w;
 };
 constexpr uint32_t kNoSlot = 0xffffffffu;
+// Residual-stream words: kResRow | i starts row i, kResRow | kResPad pads,
+// anything else is an entry's column.
+constexpr uint32_t kResRow = 0x80000000u, kResPad = 0x40000000u;
 
 // Words of the refactor and solve row programs, uint2 each:
 // x = d | (b << 16), y = c | (kind << 16) | (flags << 20); d, b, c are
@@ -110,27 +120,43 @@ constexpr uint32_t kNoSlot = 0xffffffffu;
 // as one branch-free wavefront step; rows run in order, and a word's operands
 // were final in an earlier row. Idle threads of a row work on a private pad
 // slot.
+// A chain row (R_CHAIN) hands each thread a run of multiply-subtracts on one
+// target, all of whose operands were final before the row: the thread keeps
+// the target in a register and takes the run's terms in their order, its
+// operand reads issued a block ahead, so the run costs a few cycles a term
+// instead of an LDS round trip a row. Its word: x = d | (len << 16), y = seg
+// | (R_CHAIN << 16), seg indexing the program's chain segments (word offset,
+// blocks); the segment's terms are u32 words b | (c << 16) laid out
+// [blocks + kChainAhead][64][kChainBlk] (terms past a thread's len are
+// skipped). Chain rows sit only at the last position of a kRing block of
+// rows, so the row loop runs them between blocks.
 enum RowKind : uint32_t {
     R_NOP = 0,    // whole row idle (stream padding)
     R_MAC = 1,    // M[d] = M[d] - M[b] * M[c]
     R_MACZ = 2,   // M[d] = M[c] != 0 ? M[d] - M[b] * M[c] : M[d]
     R_DIV = 3,    // M[d] = M[d] / M[b]
-    R_KINDS = 4
+    R_CHAIN = 4,  // a run of R_MAC (refactor) or R_MACZ (solve) terms per thread
+    R_KINDS = 5
 };
+constexpr int kChainBlk = 8;     // terms of a chain block (one thread's 32 bytes)
+constexpr int kChainAhead = 2;   // blocks of term words in flight (and padded at a segment's end)
 constexpr uint32_t kFlagUnscaled = 1;   // R_DIV on an unscaled column: growth + wide tests
 constexpr uint32_t kFlagScaled = 2;     // R_DIV on a scaled column: non-finite test
 inline constexpr uint32_t row_kind(uint32_t y) { return (y >> 16) & 0xfu; }
 inline constexpr uint32_t row_flags(uint32_t y) { return (y >> 20) & 0x3u; }
 // Reduce-stream words: a target word sets the accumulator's slot (bit 30:
 // residual row, else factor slot); a cell word adds one staged value.
-constexpr uint32_t kRedTarget = 0x80000000u, kRedRhs = 0x40000000u, kRedNop = 0x3fffffffu;
+constexpr uint32_t kRedTarget = 0x80000000u, kRedRhs = 0x40000000u;
+// A reduce stream's padding word: a target word for the lane's scratch
+// double at rhs + 2n (after x), which nothing else reads.
+__host__ __device__ inline uint32_t red_pad(uint32_t n) { return kRedTarget | kRedRhs | (2 * n); }
 constexpr int kRing = 16;      // rows in flight; streams carry kRing pad rows
 
-// Doubles of a lane's working set (LDS, or global memory for a spilled
-// lane): tv_cap factor slots, the Y block (y, sdiag, xold after the
-// assembly; during it the stamp staging areas, one per wavefront of the
-// lane's crew), rhs and x. A transient lane's Y block holds y and xold,
-// and q and the charge-Jacobian cells follow x.
+// Doubles of a lane's working set in LDS: tv_cap factor slots, the Y block
+// (y, sdiag, xold after the assembly; during it the stamp staging areas,
+// one per wavefront of the lane's crew), rhs, x, and a scratch pair that
+// the assembly reduce's padding words write (rhs + 2n). A transient lane's
+// Y block holds y and xold, and q and the charge-Jacobian cells follow x.
 constexpr uint32_t kStageDoubles = kWave * kStageEntries;
 __host__ __device__ inline uint32_t lds_y_block(uint32_t n, uint32_t nw = 1)
 {
@@ -139,7 +165,7 @@ __host__ __device__ inline uint32_t lds_y_block(uint32_t n, uint32_t nw = 1)
 }
 __host__ __device__ inline uint32_t lds_doubles(uint32_t tv_cap, uint32_t n, uint32_t nw = 1)
 {
-    return tv_cap + lds_y_block(n, nw) + 2 * n;
+    return tv_cap + lds_y_block(n, nw) + 2 * n + 2;
 }
 __host__ __device__ inline uint32_t tran_y_block(uint32_t n, uint32_t nw = 1)
 {
@@ -201,6 +227,13 @@ struct TapeProg {
     const uint32_t *blk_slots;  // tv slots of each block's LDS positions
     const uint2_t *blk_words;   // rows of all blocks, [row][64]
     const uint32_t *fch, *bch;  // [n_fch + 1], [n_bch + 1]
+    // chain segments of the refactor and solve programs: (word offset into
+    // the terms, blocks) each, and their terms
+    const uint2_t *rseg, *sseg;
+    const uint32_t *rcw, *scw;
+    // [res_total]: the factor slot each position of a lane's packed FP64
+    // Jacobian copies (a pad slot for a row start, whose value is unused)
+    const uint32_t *rsrc;
 };
 
 // Per-lane results of the Newton kernel.
@@ -258,11 +291,12 @@ struct OpArgs {
     const double *x_nom;        // [n] mode 3: the nominal operating point
     uint8_t *fell_back;         // [N] mode 3: 1 where the lane restarted from x_nom
     // FP32 factors with FP64 refinement: lanes with lane64[lane] == 0 factor
-    // in FP32; a64 [N][nnz] holds their FP64 Jacobian for the residual
+    // in FP32; a64 [N][res_total + 64] holds their FP64 Jacobian for the
+    // residual, packed in the residual stream's order
     uint32_t fp32, refine_max;
     double refine_tol;
     const uint8_t *lane64;      // [N]
-    double *a64;                // [N][nnz]
+    double *a64;                // [N][dp.res_total + 64]
     // optional: [N][kProfPhases] shader cycles per phase (assemble, refactor,
     // solve, update + limits + gates), summed over the lane's iterations
     unsigned long long *prof;

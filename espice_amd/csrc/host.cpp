@@ -32,6 +32,7 @@
 extern "C" hipError_t esl_launch_op_newton(const void *a, uint32_t grid, uint32_t lds_bytes, hipStream_t s);
 extern "C" hipError_t esl_launch_tape_replay(const void *r, uint32_t lanes, uint32_t lds_bytes, hipStream_t s);
 extern "C" hipError_t esl_kernel_attrs(uint32_t lds_bytes);
+extern "C" hipError_t esl_kernel_static_lds(uint32_t *bytes);
 extern "C" hipError_t esl_launch_op_newton_lpt(const void *a, hipStream_t s);
 extern "C" hipError_t esl_launch_transpose_live(const double *in, double *out, uint32_t N, uint32_t P, hipStream_t s);
 extern "C" hipError_t esl_launch_tran_steps(const void *ta, uint32_t grid, uint32_t lds_bytes, hipStream_t s);
@@ -358,6 +359,15 @@ vector<u32> min_degree(u32 n, const vector<u32> &col_ptr, const vector<u32> &row
 // longest path first. Each target's ops stay in their serial order, so every
 // slot sees exactly the sequence of roundings the column-by-column replay
 // makes.
+// A run of at least min_chain consecutive multiply-subtracts on one target
+// (a column's terms into one slot, or one row's terms of the forward solve)
+// is scheduled as one chain: it becomes ready when every operand of every
+// term is final, and one thread of a chain row takes its terms in order with
+// the target in a register. The runs are contiguous in op order and nothing
+// reads the target between them, so the chain sees the same operands and
+// rounds the same way as the ops one by one. Chain rows are placed at the
+// last position of a kRing block of rows; the positions before it take other
+// ready ops, or idle rows when there are none.
 struct SchedOp {
     u32 d, b, c, kind, flags;
 };
@@ -366,8 +376,14 @@ struct RowProgram {
     vector<esl::uint2_t> words;   // [rows + kRing][64]
     u32 rows = 0;                  // multiple of kRing
     u32 used = 0;                  // rows before padding
-    u32 critical = 0;              // longest dependency chain, in ops
+    u32 critical = 0;              // longest dependency chain, in rows (a chain counts its blocks)
+    vector<esl::uint2_t> seg;      // chain segments: (term word offset, blocks)
+    vector<u32> cw;                // chain terms, per segment [blocks + kChainAhead][64][kChainBlk]
+    u32 chains = 0, chain_terms = 0, chain_blocks = 0;
 };
+
+// Shortest run of terms on one target scheduled as a chain (0: none).
+constexpr u32 kMinChain = 8;
 
 class OpGraph {
   public:
@@ -394,46 +410,151 @@ class OpGraph {
     u32 last(u32 slot) const { return last_[slot]; }
     size_t size() const { return ops_.size(); }
 
-    void schedule(u32 pad_base, RowProgram &out) const
+    // pad_base: 64 private pad slots, one per thread
+    void schedule(u32 pad_base, RowProgram &out, u32 min_chain = 0) const
     {
         const size_t m = ops_.size();
-        vector<u32> height(m, 1);
-        for (size_t i = m; i > 0;) {
-            i--;
-            for (u32 s : succ_[i]) height[i] = std::max(height[i], height[s] + 1);
+        // 1. chains: runs of consecutive multiply-subtracts on one target
+        vector<u32> grp(m);
+        vector<std::pair<u32, u32>> chains;   // first and last op of each run
+        auto chainable = [&](size_t k, const SchedOp &head) {
+            const SchedOp &p = ops_[k];
+            return (p.kind == esl::R_MAC || p.kind == esl::R_MACZ) && p.kind == head.kind && p.d == head.d &&
+                   p.b != p.d && p.c != p.d;
+        };
+        for (size_t i = 0; i < m;) {
+            size_t j = i;
+            const bool ok = min_chain && chainable(i, ops_[i]);
+            if (ok)
+                while (j + 1 < m && chainable(j + 1, ops_[i])) j++;
+            if (ok && j - i + 1 >= min_chain && j - i + 1 < 65536) {
+                for (size_t k = i; k <= j; k++) grp[k] = (u32)(m + chains.size());
+                chains.push_back({(u32)i, (u32)j});
+            } else {
+                for (size_t k = i; k <= j; k++) grp[k] = (u32)k;
+            }
+            i = j + 1;
+        }
+        // 2. the graph with each run contracted to one node
+        const size_t N = m + chains.size();
+        auto live = [&](size_t v) { return v >= m || grp[v] == v; };
+        auto kind_of = [&](size_t v) { return v >= m ? (u32)esl::R_CHAIN : ops_[v].kind; };
+        auto len_of = [&](size_t v) { return v >= m ? chains[v - m].second - chains[v - m].first + 1 : 1u; };
+        vector<vector<u32>> succ(N);
+        for (size_t u = 0; u < m; u++)
+            for (u32 sv : succ_[u])
+                if (grp[sv] != grp[u]) succ[grp[u]].push_back(grp[sv]);
+        vector<u32> indeg(N, 0);
+        for (auto &v : succ) {
+            std::sort(v.begin(), v.end());
+            v.erase(std::unique(v.begin(), v.end()), v.end());
+            for (u32 x : v) indeg[x]++;
+        }
+        // 3. longest path to the end (a chain weighs 1 + its blocks)
+        vector<u32> order;
+        {
+            vector<u32> deg = indeg;
+            for (size_t v = 0; v < N; v++)
+                if (live(v) && deg[v] == 0) order.push_back((u32)v);
+            for (size_t h = 0; h < order.size(); h++)
+                for (u32 x : succ[order[h]])
+                    if (--deg[x] == 0) order.push_back(x);
+        }
+        vector<u32> height(N, 0);
+        for (size_t h = order.size(); h > 0;) {
+            const u32 v = order[--h];
+            u32 best = 0;
+            for (u32 x : succ[v]) best = std::max(best, height[x]);
+            height[v] = best + (kind_of(v) == esl::R_CHAIN ? 1 + (len_of(v) + esl::kChainBlk - 1) / esl::kChainBlk : 1);
         }
         u32 crit = 0;
-        for (u32 h : height) crit = std::max(crit, h);
-        vector<u32> indeg = indeg_;
+        for (u32 v : order) crit = std::max(crit, height[v]);
+        // 4. list scheduling
         auto cmp = [&](u32 a, u32 b) { return height[a] != height[b] ? height[a] < height[b] : a > b; };
         vector<std::priority_queue<u32, vector<u32>, decltype(cmp)>> ready;
         for (u32 k = 0; k < esl::R_KINDS; k++) ready.emplace_back(cmp);
-        for (u32 i = 0; i < m; i++)
-            if (indeg[i] == 0) ready[ops_[i].kind].push(i);
+        for (u32 v : order)
+            if (indeg[v] == 0) ready[kind_of(v)].push(v);
         vector<vector<u32>> rows;
+        vector<u32> row_kind;
         size_t done = 0;
-        while (done < m) {
+        const size_t total = order.size();
+        while (done < total) {
+            const bool chain_slot = rows.size() % esl::kRing == (size_t)esl::kRing - 1;
             u32 kind = esl::R_KINDS;
-            for (u32 k = 1; k < esl::R_KINDS; k++) {
-                if (ready[k].empty()) continue;
-                if (kind == esl::R_KINDS || cmp(ready[kind].top(), ready[k].top())) kind = k;
+            if (chain_slot && !ready[esl::R_CHAIN].empty()) {
+                kind = esl::R_CHAIN;
+            } else {
+                for (u32 k = 1; k < esl::R_CHAIN; k++) {
+                    if (ready[k].empty()) continue;
+                    if (kind == esl::R_KINDS || cmp(ready[kind].top(), ready[k].top())) kind = k;
+                }
             }
             vector<u32> row;
+            if (kind == esl::R_KINDS) {
+                // only chains are ready: idle rows up to the chain position
+                if (ready[esl::R_CHAIN].empty()) return;   // a cycle cannot happen; stop rather than loop
+                rows.push_back(row);
+                row_kind.push_back(esl::R_NOP);
+                continue;
+            }
             while (!ready[kind].empty() && row.size() < (size_t)esl::kWave) {
                 row.push_back(ready[kind].top());
                 ready[kind].pop();
             }
             done += row.size();
-            for (u32 i : row)
-                for (u32 s : succ_[i])
-                    if (--indeg[s] == 0) ready[ops_[s].kind].push(s);
+            for (u32 v : row)
+                for (u32 x : succ[v])
+                    if (--indeg[x] == 0) ready[kind_of(x)].push(x);
             rows.push_back(std::move(row));
+            row_kind.push_back(kind);
         }
+        // 5. words
         const u32 used = (u32)rows.size();
         const u32 padded = (used + esl::kRing - 1) / esl::kRing * esl::kRing;
         out.words.assign((size_t)(padded + esl::kRing) * esl::kWave, esl::uint2_t{0, 0});
+        out.seg.clear();
+        out.cw.clear();
+        out.chains = (u32)chains.size();
+        out.chain_terms = 0;
+        out.chain_blocks = 0;
         for (u32 r = 0; r < used; r++) {
-            const u32 kind = ops_[rows[r][0]].kind;
+            const u32 kind = row_kind[r];
+            if (kind == esl::R_CHAIN) {
+                u32 lmax = 0;
+                for (u32 v : rows[r]) lmax = std::max(lmax, len_of(v));
+                const u32 blocks = (lmax + esl::kChainBlk - 1) / esl::kChainBlk;
+                const u32 seg = (u32)out.seg.size();
+                out.seg.push_back(esl::uint2_t{(u32)out.cw.size(), blocks});
+                const size_t base = out.cw.size();
+                out.cw.resize(base + (size_t)(blocks + esl::kChainAhead) * esl::kWave * esl::kChainBlk);
+                for (u32 t = 0; t < (u32)esl::kWave; t++) {
+                    const u32 pad = pad_base + t;
+                    u32 d = pad, len = 0;
+                    const SchedOp *first = nullptr;
+                    if (t < rows[r].size()) {
+                        const auto &ch = chains[rows[r][t] - m];
+                        first = &ops_[ch.first];
+                        d = first->d;
+                        len = ch.second - ch.first + 1;
+                        out.chain_terms += len;
+                    }
+                    for (u32 k = 0; k < blocks + esl::kChainAhead; k++)
+                        for (u32 e = 0; e < (u32)esl::kChainBlk; e++) {
+                            const u32 step = k * esl::kChainBlk + e;
+                            u32 w = pad | (pad << 16);
+                            if (step < len) {
+                                const SchedOp &o = ops_[chains[rows[r][t] - m].first + step];
+                                w = o.b | (o.c << 16);
+                            }
+                            out.cw[base + ((size_t)k * esl::kWave + t) * esl::kChainBlk + e] = w;
+                        }
+                    out.words[(size_t)r * esl::kWave + t] =
+                        esl::uint2_t{d | (len << 16), seg | ((u32)esl::R_CHAIN << 16)};
+                }
+                out.chain_blocks += blocks;
+                continue;
+            }
             for (u32 t = 0; t < (u32)esl::kWave; t++) {
                 const u32 pad = pad_base + t;
                 esl::uint2_t w{pad | (pad << 16), pad | (kind << 16)};
@@ -464,6 +585,7 @@ struct TapeHostProg {
     vector<u32> blev_ptr, blev_rows, urow_ptr, urow_k, urow_slot, void_slots;
     RowProgram rch, sch;
     vector<u32> red, red_off, red_len, diag_tv;
+    vector<u32> rsrc;   // the factor slot of each position of the packed FP64 Jacobian
     uint64_t flops = 0;
     // lanes beyond a workgroup's LDS: the replay in blocks and the solves in
     // chunks instead of whole-tape row programs and reduce streams
@@ -597,7 +719,7 @@ void build_tape_prog(const HostTape &t, const vector<u32> &col_ptr, double growt
 // The row programs of a tape (the wavefront kernels' refactor and solves):
 // every operation of the column replay scheduled into rows of 64
 // independent words.
-void build_row_programs(const HostTape &t, TapeHostProg &o)
+void build_row_programs(const HostTape &t, TapeHostProg &o, u32 min_chain = kMinChain)
 {
     const u32 n = t.n, nu = t.nnz_u;
     const u32 lx0 = o.prog.lx0, pad_base = o.prog.pad_slot, n_tv = o.prog.n_tv;
@@ -632,7 +754,7 @@ void build_row_programs(const HostTape &t, TapeHostProg &o)
                 g.add(lx0 + qq, piv, lx0 + qq, esl::R_DIV, fl);
             }
         }
-        g.schedule(pad_base, o.rch);
+        g.schedule(pad_base, o.rch, min_chain);
         o.prog.n_rlev = o.rch.critical;
     }
     // Solve ops (lane_kernels.zig:123-143 by rows): y row r takes the
@@ -664,7 +786,7 @@ void build_row_programs(const HostTape &t, TapeHostProg &o)
             }
             g.add(yb + i, nu + i, yb + i, esl::R_DIV, 0, first ? readers[i] : vector<u32>{});
         }
-        g.schedule(pad_base, o.sch);
+        g.schedule(pad_base, o.sch, min_chain);
         o.prog.n_slev = o.sch.critical;
     }
 }
@@ -835,6 +957,9 @@ hipError_t upload_tape_prog(TapeHostProg &o)
     f.push_back(pk.add(o.li));
     f.push_back(pk.add(o.blk)); f.push_back(pk.add(o.blk_slots)); f.push_back(pk.add(o.blk_words));
     f.push_back(pk.add(o.fch)); f.push_back(pk.add(o.bch));
+    f.push_back(pk.add(o.rch.seg)); f.push_back(pk.add(o.sch.seg));
+    f.push_back(pk.add(o.rch.cw)); f.push_back(pk.add(o.sch.cw));
+    f.push_back(pk.add(o.rsrc));
     hipError_t e = o.buf.ensure(pk.blob.size());
     if (e != hipSuccess) return e;
     e = hipMemcpy(o.buf.p, pk.blob.data(), pk.blob.size(), hipMemcpyHostToDevice);
@@ -864,6 +989,11 @@ hipError_t upload_tape_prog(TapeHostProg &o)
     T.blk_words = reinterpret_cast<const esl::uint2_t *>(P(i++));
     T.fch = P(i++);
     T.bch = P(i++);
+    T.rseg = reinterpret_cast<const esl::uint2_t *>(P(i++));
+    T.sseg = reinterpret_cast<const esl::uint2_t *>(P(i++));
+    T.rcw = P(i++);
+    T.scw = P(i++);
+    T.rsrc = P(i++);
     if (i != f.size()) return hipErrorInvalidValue;
     return hipSuccess;
 }
@@ -952,6 +1082,8 @@ struct Compiled {
     vector<int32_t> r_bind, v_bind, i_bind, m_bind_vto, m_bind_kp;
     vector<u32> r_slot, v_slot, m_slot, d_slot, q_slot;
     vector<u32> csr_ptr, csr_col, csr_slot;   // the pattern by rows, columns ascending
+    vector<u32> res_w, res_p;                 // the refinement residual's stream (build_res_stream)
+    u32 res_steps = 0, res_total = 0;
     vector<u32> d_t, c_t, v_wkind, cslot_pat, q_t;
     vector<double> d_par, c_val, v_v2, v_td, q_par;
     vector<int32_t> d_bind_is, c_bind, q_bind_is, q_bind_bf;
@@ -973,6 +1105,54 @@ void build_csr(Compiled &c)
             c.csr_col[e] = j;
             c.csr_slot[e] = p;
         }
+}
+
+// The refinement residual r = -F - A dx as a stream: each row (its start
+// word, then its entries in ascending column order, the order of the
+// serial residual) goes whole to one of the 64 threads, longest rows first
+// to the least loaded thread; the threads are then numbered by their word
+// count, longest first, so the threads still busy at any step are a prefix.
+// res_w [res_steps + kRing][64]: kResRow | i starts row i, j an entry in
+// column j, kResRow | kResPad pads. The FP64 Jacobian a lane refines with
+// is stored in the order of the words (row starts included), packed: step
+// s's busy threads t at a64 index off_s + t, off_s the words before step s
+// (res_total in all); res_p gives each of those positions its pattern slot
+// (kNoSlot for a row start).
+void build_res_stream(Compiled &c)
+{
+    const u32 n = c.n;
+    vector<u32> rows(n);
+    for (u32 i = 0; i < n; i++) rows[i] = i;
+    auto len = [&](u32 i) { return 1 + c.csr_ptr[i + 1] - c.csr_ptr[i]; };
+    std::stable_sort(rows.begin(), rows.end(), [&](u32 a, u32 b) { return len(a) > len(b); });
+    vector<vector<u32>> words(esl::kWave), slots(esl::kWave);
+    for (u32 i : rows) {
+        size_t best = 0;
+        for (size_t t = 1; t < words.size(); t++)
+            if (words[t].size() < words[best].size()) best = t;
+        words[best].push_back(esl::kResRow | i);
+        slots[best].push_back(esl::kNoSlot);
+        for (u32 e = c.csr_ptr[i]; e < c.csr_ptr[i + 1]; e++) {
+            words[best].push_back(c.csr_col[e]);
+            slots[best].push_back(c.csr_slot[e]);
+        }
+    }
+    vector<u32> order(esl::kWave);
+    for (u32 t = 0; t < (u32)esl::kWave; t++) order[t] = t;
+    std::stable_sort(order.begin(), order.end(), [&](u32 a, u32 b) { return words[a].size() > words[b].size(); });
+    const u32 L = (u32)words[order[0]].size();
+    const u32 steps = (L + esl::kRing - 1) / esl::kRing * esl::kRing;
+    c.res_steps = steps;
+    c.res_w.assign((size_t)(steps + esl::kRing) * esl::kWave, esl::kResRow | esl::kResPad);
+    c.res_p.clear();
+    for (u32 st = 0; st < L; st++)
+        for (u32 t = 0; t < (u32)esl::kWave; t++) {
+            const auto &w = words[order[t]];
+            if (st >= w.size()) break;
+            c.res_w[(size_t)st * esl::kWave + t] = w[st];
+            c.res_p.push_back(slots[order[t]][st]);
+        }
+    c.res_total = (u32)c.res_p.size();
 }
 
 // Gummel-Poon prep (models/gummel_poon.va:158-242, bjttemp.c): the fields
@@ -1417,7 +1597,18 @@ hipError_t upload_deck(Engine &e)
            o_ib = pk.add(c.i_bind), o_mt = pk.add(c.m_t), o_mp = pk.add(c.m_par), o_mbv = pk.add(c.m_bind_vto),
            o_mbk = pk.add(c.m_bind_kp), o_rs = pk.add(c.r_slot), o_vs = pk.add(c.v_slot), o_ms = pk.add(c.m_slot);
     build_csr(c);
+    build_res_stream(c);
+    // the residual stream's words index rows and columns of the pattern,
+    // and its packed Jacobian holds every entry and row start once
+    if (c.res_total != c.nnz + c.n || c.res_steps % esl::kRing ||
+        c.res_w.size() != (size_t)(c.res_steps + esl::kRing) * esl::kWave)
+        return hipErrorInvalidValue;
+    for (u32 w : c.res_w)
+        if (w != (esl::kResRow | esl::kResPad) && (w & 0x3fffffffu) >= c.n) return hipErrorInvalidValue;
+    for (u32 p : c.res_p)
+        if (p != esl::kNoSlot && p >= c.nnz) return hipErrorInvalidValue;
     size_t o_cp = pk.add(c.csr_ptr), o_cc = pk.add(c.csr_col), o_cs = pk.add(c.csr_slot);
+    size_t o_rw = pk.add(c.res_w);
     size_t o_dt = pk.add(c.d_t), o_dp = pk.add(c.d_par), o_db = pk.add(c.d_bind_is), o_ct = pk.add(c.c_t),
            o_cv = pk.add(c.c_val), o_cb = pk.add(c.c_bind), o_vw = pk.add(c.v_wkind), o_v2 = pk.add(c.v_v2),
            o_vtd = pk.add(c.v_td), o_csp = pk.add(c.cslot_pat), o_qt = pk.add(c.q_t), o_qp = pk.add(c.q_par),
@@ -1465,6 +1656,9 @@ hipError_t upload_deck(Engine &e)
     dp.csr_ptr = reinterpret_cast<const u32 *>(b + o_cp);
     dp.csr_col = reinterpret_cast<const u32 *>(b + o_cc);
     dp.csr_slot = reinterpret_cast<const u32 *>(b + o_cs);
+    dp.res_w = reinterpret_cast<const u32 *>(b + o_rw);
+    dp.res_steps = c.res_steps;
+    dp.res_total = c.res_total;
     dp.n_dio = c.n_dio;
     dp.n_cap = c.n_cap;
     dp.n_cslot = (u32)c.cslot_pat.size();
@@ -1576,7 +1770,7 @@ void build_reduce_streams(const Compiled &c, TapeHostProg &o)
         o.red_off.push_back((u32)o.red.size());
         o.red_len.push_back((u32)L);
         const size_t base = o.red.size();
-        o.red.resize(base + (L + esl::kRing) * esl::kWave, esl::kRedNop);
+        o.red.resize(base + (L + esl::kRing) * esl::kWave, esl::red_pad(c.n));
         for (size_t t = 0; t < words.size(); t++)
             for (size_t w = 0; w < words[t].size(); w++) o.red[base + w * esl::kWave + t] = words[t][w];
     }
@@ -1603,7 +1797,7 @@ bool validate_big(const TapeHostProg &o, u32 n)
         const u32 lim = b.m + esl::kWave;
         for (size_t r = b.row0; r < (size_t)b.row0 + b.rows; r++) {
             const u32 kind = esl::row_kind(o.blk_words[r * esl::kWave].y);
-            if (kind >= esl::R_KINDS || kind == esl::R_MACZ) return false;
+            if (kind >= esl::R_KINDS || kind == esl::R_MACZ || kind == esl::R_CHAIN) return false;
             for (u32 t = 0; t < (u32)esl::kWave; t++) {
                 const esl::uint2_t w = o.blk_words[r * esl::kWave + t];
                 if (esl::row_kind(w.y) != kind || (w.y >> 22) != 0) return false;
@@ -1665,15 +1859,31 @@ bool validate_tape(const TapeHostProg &o, u32 n, u32 nnz)
         if (rp.rows % esl::kRing || rp.used > rp.rows) return false;
         if (rp.words.size() != (size_t)(rp.rows + esl::kRing) * esl::kWave) return false;
         const u32 lim = solve ? P.n_tv + n : P.n_tv;
+        size_t cw_end = 0;
+        for (size_t si = 0; si < rp.seg.size(); si++) {
+            const esl::uint2_t sg = rp.seg[si];
+            if (sg.x != cw_end || sg.y == 0) return false;
+            cw_end += (size_t)(sg.y + esl::kChainAhead) * esl::kWave * esl::kChainBlk;
+        }
+        if (cw_end != rp.cw.size()) return false;
+        for (u32 w : rp.cw)
+            if ((w & 0xffff) >= lim || (w >> 16) >= lim) return false;
         for (size_t r = 0; r < rp.rows + (size_t)esl::kRing; r++) {
             const u32 kind = esl::row_kind(rp.words[r * esl::kWave].y);
             if (kind >= esl::R_KINDS || (kind == esl::R_MAC && solve)) return false;
             if (kind == esl::R_MACZ && !solve) return false;
+            // chain rows only at the last position of a block of kRing rows
+            if (kind == esl::R_CHAIN && r % esl::kRing != (size_t)esl::kRing - 1) return false;
             for (u32 t = 0; t < (u32)esl::kWave; t++) {
                 const esl::uint2_t w = rp.words[r * esl::kWave + t];
                 if (esl::row_kind(w.y) != kind || (w.y >> 22) != 0) return false;
                 if (solve && esl::row_flags(w.y) != 0) return false;   // no peel tests in the solves
                 if (kind == esl::R_NOP) continue;
+                if (kind == esl::R_CHAIN) {
+                    const u32 d = w.x & 0xffff, len = w.x >> 16, seg = w.y & 0xffff;
+                    if (d >= lim || seg >= rp.seg.size() || len > rp.seg[seg].y * (u32)esl::kChainBlk) return false;
+                    continue;
+                }
                 const u32 d = w.x & 0xffff, b = w.x >> 16, c = w.y & 0xffff;
                 if (d >= lim || b >= lim || c >= lim) return false;
             }
@@ -1681,17 +1891,20 @@ bool validate_tape(const TapeHostProg &o, u32 n, u32 nnz)
         return true;
     };
     if (!rows_ok(o.rch, false) || !rows_ok(o.sch, true)) return false;
+    for (u32 v : o.rsrc) if (v >= P.n_tv) return false;
     if (!o.red_off.empty()) {
         if (o.red_len.size() != o.red_off.size()) return false;
         for (size_t ci = 0; ci < o.red_off.size(); ci++) {
             if ((size_t)o.red_off[ci] + (size_t)(o.red_len[ci] + esl::kRing) * esl::kWave > o.red.size()) return false;
             if (o.red_len[ci] % esl::kRing) return false;
         }
+        if (2 * n > 0xfffff) return false;
         for (u32 w : o.red) {
             if (w & esl::kRedTarget) {
                 const u32 idx = w & 0xfffff;
-                if ((w & esl::kRedRhs) ? idx >= n : idx >= P.n_tv) return false;
-            } else if (w != esl::kRedNop && w >= (u32)(esl::kWave * esl::kStageEntries)) {
+                if ((w & esl::kRedRhs) ? (idx >= n && w != esl::red_pad(n)) : idx >= P.n_tv) return false;
+                if ((w & ~(esl::kRedTarget | esl::kRedRhs)) > 0xfffff) return false;
+            } else if (w >= (u32)(esl::kWave * esl::kStageEntries)) {
                 return false;
             }
         }
@@ -1757,6 +1970,8 @@ bool validate_deck(const Compiled &c)
            binds_ok(c.q_bind_bf);
 }
 
+// The dynamic LDS a lane kernel's workgroup may hold: the device's LDS per
+// workgroup less the largest static LDS the lane kernels declare.
 u32 device_lds_max()
 {
     static u32 cached = 0;
@@ -1764,7 +1979,9 @@ u32 device_lds_max()
     int v = 0, dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 65536;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0) v = 65536;
-    cached = (u32)v;
+    uint32_t stat = 0;
+    if (esl_kernel_static_lds(&stat) != hipSuccess) stat = 1024;
+    cached = (u32)v - std::min<u32>((u32)v / 2, (stat + 255) / 256 * 256);
     return cached;
 }
 
@@ -1793,6 +2010,9 @@ TapeHostProg *build_tape(const Engine &e, const HostTape &t, const char *&err)
     } else {
         build_row_programs(t, *tp);
         build_reduce_streams(e.c, *tp);
+        tp->rsrc.resize(e.c.res_total);
+        for (u32 k = 0; k < e.c.res_total; k++)
+            tp->rsrc[k] = e.c.res_p[k] == esl::kNoSlot ? tp->prog.pad_slot : tp->amap[e.c.res_p[k]];
     }
     if (!err && !validate_tape(*tp, n, e.c.nnz)) err = "tape program failed validation";
     if (err) {
@@ -2124,6 +2344,144 @@ esl_status esl_debug_big_replay(uint32_t n, const uint32_t *col_ptr, const uint3
     return ESL_OK;
 }
 
+// Development hook, not part of the drop-in: the wavefront kernels' refactor
+// and solve row programs (chain rows included, runs of at least min_chain
+// terms) run on the CPU row by row as a wavefront runs them, for `lanes`
+// matrices a [lanes][nnz] and right-hand sides rhs [lanes][n] on the tape
+// factored from lead_vals. Every row is also checked for independence: no
+// two of its ops (or chains) write one slot, and none reads a slot another
+// writes. Outputs as esl_debug_big_replay; sizes[10] = {refactor rows,
+// nnz_l, nnz_u, solve rows, refactor chains, solve chains, refactor chain
+// blocks, solve chain blocks, refactor and solve critical paths}.
+esl_status esl_debug_row_replay(uint32_t n, const uint32_t *col_ptr, const uint32_t *row_idx,
+                                const double *lead_vals, const uint32_t *q, double pivot_tol, uint32_t min_chain,
+                                const double *a, const double *rhs, uint32_t lanes, double *lx, double *ux,
+                                double *udiag, double *dx, uint32_t *fail, uint32_t *sizes)
+{
+    if (!col_ptr || !row_idx || !lead_vals || !q || n == 0 || !sizes) return ESL_INVALID_ARGUMENT;
+    if (lanes && (!a || !rhs || !lx || !ux || !udiag || !dx || !fail)) return ESL_INVALID_ARGUMENT;
+    HostTape t;
+    if (!gp_factor(n, col_ptr, row_idx, lead_vals, q, pivot_tol, t)) return ESL_FAILED;
+    vector<u32> cp(col_ptr, col_ptr + n + 1);
+    TapeHostProg o;
+    build_tape_prog(t, cp, 1e-12, 1.0 / pivot_tol, o);
+    if (o.prog.n_tv + n >= 65536) return ESL_FAILED;
+    build_row_programs(t, o, min_chain);
+    if (!validate_tape(o, n, cp[n])) return ESL_FAILED;
+    const esl::TapeProg &T = o.prog;
+    sizes[0] = o.rch.used;
+    sizes[1] = T.n_l;
+    sizes[2] = T.n_u;
+    sizes[3] = o.sch.used;
+    sizes[4] = o.rch.chains;
+    sizes[5] = o.sch.chains;
+    sizes[6] = o.rch.chain_blocks;
+    sizes[7] = o.sch.chain_blocks;
+    sizes[8] = o.rch.critical;
+    sizes[9] = o.sch.critical;
+    auto bits = [](double d) { uint64_t b; memcpy(&b, &d, 8); return b; };
+    auto dbl = [](uint64_t b) { double d; memcpy(&d, &b, 8); return d; };
+    // a row program over M; false when a row's ops are not independent
+    auto run = [&](const RowProgram &rp, vector<double> &M, bool solve, bool &bad, uint64_t &nonfinite,
+                   uint64_t &worst) {
+        vector<u32> wrote(M.size(), NONE);
+        for (u32 r = 0; r < rp.rows; r++) {
+            const u32 kind = esl::row_kind(rp.words[(size_t)r * esl::kWave].y);
+            if (kind == esl::R_NOP) continue;
+            // independence: no two ops of the row write one slot, and no op
+            // reads (besides its own target) a slot the row writes
+            vector<std::pair<u32, vector<u32>>> ops;
+            for (u32 th = 0; th < (u32)esl::kWave; th++) {
+                const esl::uint2_t w = rp.words[(size_t)r * esl::kWave + th];
+                const u32 d = w.x & 0xffff;
+                vector<u32> rd;
+                if (kind == esl::R_CHAIN) {
+                    const esl::uint2_t sg = rp.seg[w.y & 0xffff];
+                    const u32 len = w.x >> 16;
+                    for (u32 k = 0; k < len; k++) {
+                        const u32 cw = rp.cw[sg.x + ((size_t)(k / esl::kChainBlk) * esl::kWave + th) * esl::kChainBlk +
+                                             k % esl::kChainBlk];
+                        rd.push_back(cw & 0xffff);
+                        rd.push_back(cw >> 16);
+                    }
+                } else {
+                    rd = {w.x >> 16, w.y & 0xffff};
+                }
+                if (wrote[d] == r) return false;
+                wrote[d] = r;
+                ops.push_back({d, std::move(rd)});
+            }
+            for (auto &op : ops)
+                for (u32 x : op.second)
+                    if (x != op.first && wrote[x] == r) return false;
+            for (u32 th = 0; th < (u32)esl::kWave; th++) {
+                const esl::uint2_t w = rp.words[(size_t)r * esl::kWave + th];
+                const u32 d = w.x & 0xffff, b = w.x >> 16, c = w.y & 0xffff;
+                if (kind == esl::R_CHAIN) {
+                    const esl::uint2_t sg = rp.seg[c];
+                    const u32 len = w.x >> 16;
+                    // every term of the segment as the kernel runs them;
+                    // those past len are skipped
+                    double acc = M[d];
+                    for (u32 k = 0; k < sg.y * (u32)esl::kChainBlk; k++) {
+                        const u32 cw = rp.cw[sg.x + ((size_t)(k / esl::kChainBlk) * esl::kWave + th) * esl::kChainBlk +
+                                             k % esl::kChainBlk];
+                        const double A = M[cw & 0xffff], B = M[cw >> 16];
+                        const double tt = A * B, nv = acc - tt;
+                        const bool use = k < len && (!solve || B != 0.0);
+                        acc = use ? nv : acc;
+                    }
+                    M[d] = acc;
+                } else if (kind == esl::R_MAC) {
+                    const double m = M[b] * M[c];
+                    M[d] = M[d] - m;
+                } else if (kind == esl::R_MACZ) {
+                    const double v = M[d], A = M[b], B = M[c], pv = v - A * B;
+                    M[d] = B == 0.0 ? v : pv;
+                } else if (kind == esl::R_DIV) {
+                    const double v = M[d], A = M[b], qq = v / A;
+                    M[d] = qq;
+                    if (solve) continue;
+                    const u32 fl = esl::row_flags(w.y);
+                    const bool un = fl & esl::kFlagUnscaled;
+                    bad |= un && T.growth > 0 && !(std::fabs(A) >= T.growth * std::fabs(v));
+                    nonfinite |= ((un && !(T.growth > 0)) || (fl & esl::kFlagScaled)) ? bits(qq - qq) : 0;
+                    const uint64_t lb = un ? bits(std::fabs(qq)) : 0;
+                    worst = lb > worst ? lb : worst;
+                }
+            }
+        }
+        return true;
+    };
+    const u32 nnz = cp[n];
+    for (u32 l = 0; l < lanes; l++) {
+        vector<double> M(T.n_tv + n, 0.0);
+        for (u32 p = 0; p < nnz; p++) M[o.amap[p]] = a[(size_t)l * nnz + p];
+        bool bad = false;
+        uint64_t nonfinite = 0, worst = 0;
+        for (u32 v : o.void_slots) bad |= M[o.amap[v]] != 0.0;
+        for (u32 k = 0; k < n; k++)
+            if (o.flags[k] & esl::kVoidBit) M[T.n_u + k] = 1.0;
+        if (!run(o.rch, M, false, bad, nonfinite, worst)) return ESL_FAILED;
+        for (u32 p = 0; p < T.n_u; p++) nonfinite |= bits(M[p] - M[p]);
+        for (u32 k = 0; k < n; k++) {
+            if (o.flags[k] & esl::kVoidBit) continue;
+            const double d = M[T.n_u + k];
+            bad |= d == 0.0 || !(std::fabs(d) < INFINITY);
+        }
+        fail[l] = (bad || nonfinite != 0 || !(dbl(worst) <= T.wide)) ? 1u : 0u;
+        for (u32 p = 0; p < T.n_u; p++) ux[(size_t)l * T.n_u + p] = M[p];
+        for (u32 k = 0; k < n; k++) udiag[(size_t)l * n + k] = M[T.n_u + k];
+        for (u32 qq = 0; qq < T.n_l; qq++) lx[(size_t)l * T.n_l + qq] = M[T.lx0 + qq];
+        for (u32 r = 0; r < n; r++) M[T.n_tv + o.pinv[r]] = -rhs[(size_t)l * n + r];
+        bool b2 = false;
+        uint64_t n2 = 0, w2 = 0;
+        if (!run(o.sch, M, true, b2, n2, w2)) return ESL_FAILED;
+        for (u32 i = 0; i < n; i++) dx[(size_t)l * n + i] = M[T.n_tv + o.qinv[i]];
+    }
+    return ESL_OK;
+}
+
 esl_status esl_last_timing(esl_engine *h, esl_timing *out)
 {
     auto *e = static_cast<Engine *>(lookup(h, 1));
@@ -2299,7 +2657,7 @@ esl_status op_prepare(Engine &E, const double *values, size_t count, const esl_o
     const bool fp32 = o.precision == ESL_FP32_REFINED;
     lane64.assign(count, 0);
     HIPCHK(e->lane64.ensure(count));
-    if (fp32) HIPCHK(e->a64.ensure(count * (size_t)nnz * 8));
+    if (fp32) HIPCHK(e->a64.ensure(count * ((size_t)e->c.res_total + esl::kWave) * 8));
     hipStream_t s = e->stream;
     HIPCHK(hipMemcpyAsync(e->lane64.p, lane64.data(), count, hipMemcpyHostToDevice, s));
     if (P) HIPCHK(hipMemcpyAsync(e->live.p, values, count * P * 8, hipMemcpyHostToDevice, s));

@@ -153,46 +153,14 @@ __device__ inline void add_gmin(const OpArgs &a, uint32_t lane, const uint32_t *
     wsync();
 }
 
-struct RedAcc {
-    int cur;
-    bool rhs;
-    double acc;
-};
-
-__device__ inline void red_flush(RedAcc &r, double *tv, double *rhs)
+// The LDS double a reduce word reads, without a branch: a target word its
+// factor slot or rhs row (padding words: the lane's scratch double at
+// rhs + 2n), a cell word its staged cell. Branch-free, so the reads of the
+// words ahead stay in flight together.
+__device__ inline double *red_src(uint32_t w, double *tv, double *rhs, double *stc)
 {
-    if (r.cur >= 0) {
-        if (r.rhs) rhs[r.cur] = r.acc;
-        else tv[r.cur] = r.acc;
-    }
-}
-
-// The value a reduce word reads: a target's running sum (its factor slot or
-// rhs row), a staged cell, or nothing (padding).
-__device__ inline double red_value(uint32_t w, const double *tv, const double *rhs, const double *st)
-{
-    if (w & kRedTarget) {
-        const uint32_t t = w & 0xfffffu;
-        return (w & kRedRhs) ? rhs[t] : tv[t];
-    }
-    return w < (uint32_t)(kWave * kStageEntries) ? st[w] : 0.0;
-}
-
-// Consumes word w with its value v, read ahead: a target word flushes the
-// previous target's sum and starts from the new target's running sum; a
-// cell word adds its cell. The sums are the instance-ordered ones of the
-// serial assembly; reading ahead is safe because a chunk gives each target
-// to one thread, once.
-__device__ inline void red_use(uint32_t w, double v, RedAcc &r, double *tv, double *rhs)
-{
-    if (w & kRedTarget) {
-        red_flush(r, tv, rhs);
-        r.cur = (int)(w & 0xfffffu);
-        r.rhs = (w & kRedRhs) != 0;
-        r.acc = v;
-    } else if (w < (uint32_t)(kWave * kStageEntries)) {
-        r.acc = r.acc + v;
-    }
+    double *tb = (w & kRedRhs) ? rhs : tv;
+    return ((int32_t)w < 0 ? tb : stc) + (w & 0xfffffu);
 }
 
 // Assembly into the tape's factor slots: every chunk evaluates, then each
@@ -215,37 +183,52 @@ __device__ void assemble(const DeckProg &dp, const TapeProg &T, const double *li
         const uint32_t c1 = c0 + cr.nw < dp.n_chunks ? c0 + cr.nw : dp.n_chunks;
         for (uint32_t c = c0; c < c1; c++) {
             if (dp.chunks[c].kind == kC) continue;   // no static stamps
-            const double *stc = st + (c - c0) * kStageDoubles;
+            double *stc = st + (c - c0) * kStageDoubles;
             // the stream through a global pointer: its ring of loads must
             // not hold up the LDS waits of the reduce (flat loads would)
             const __attribute__((address_space(1))) uint32_t *rs = gptr(T.red) + T.red_off[c] + tid;
             const uint32_t len = T.red_len[c];
-            RedAcc r{-1, false, 0.0};
             uint32_t q[kRing];
 #pragma unroll
             for (int k = 0; k < kRing; k++) {
                 q[k] = rs[k * kWave];
                 __builtin_amdgcn_sched_barrier(0);
             }
-            // values read kAhead words ahead of their use: a long target's
-            // chain of adds waits on the adds, not on an LDS round trip each
+            // Each word's value is read kAhead words before its use, so a
+            // long target's chain of adds waits on the adds, not on an LDS
+            // round trip each. The running sum is stored to the current
+            // target at every word (its final value at the switch to the
+            // next target): a chunk gives each target to one thread, once,
+            // so no other read sees the partial sums. The sums are the
+            // instance-ordered ones of the serial assembly.
             constexpr int kAhead = kRing / 2;
             double va[kAhead];
+            double *pa[kAhead];
 #pragma unroll
-            for (int k = 0; k < kAhead; k++) va[k] = red_value(q[k], tv, rhs, stc);
+            for (int k = 0; k < kAhead; k++) {
+                pa[k] = red_src(q[k], tv, rhs, stc);
+                va[k] = *pa[k];
+            }
+            double *cur = rhs + 2 * n;   // the scratch double
+            double acc = 0.0;
             for (uint32_t w = 0; w < len; w += kRing) {
 #pragma unroll
                 for (int k = 0; k < kRing; k++) {
                     const double v = va[k % kAhead];
+                    double *p = pa[k % kAhead];
+                    const bool tgt = (int32_t)q[k] < 0;
+                    *cur = acc;
+                    cur = tgt ? p : cur;
+                    acc = tgt ? v : acc + v;
                     // word w + k + kAhead: still in the ring (k < kAhead), or
                     // the refill of the slot used kAhead words ago
-                    va[k % kAhead] = red_value(q[(k + kAhead) % kRing], tv, rhs, stc);
-                    red_use(q[k], v, r, tv, rhs);
+                    pa[k % kAhead] = red_src(q[(k + kAhead) % kRing], tv, rhs, stc);
+                    va[k % kAhead] = *pa[k % kAhead];
                     q[k] = rs[(w + kRing + k) * kWave];
                     __builtin_amdgcn_sched_barrier(0);
                 }
             }
-            red_flush(r, tv, rhs);
+            *cur = acc;
             wsync();
         }
     }
@@ -266,7 +249,7 @@ __device__ bool refactor(const TapeProg &T, uint32_t n, R *tv, uint32_t tid)
     for (uint32_t v = tid; v < T.n_void; v += kWave) f.bad |= tv[T.amap[T.void_slots[v]]] != R(0);
     for (uint32_t k = tid; k < n; k += kWave)
         if (T.flags[k] & kVoidBit) tv[T.n_u + k] = R(1);
-    run_rows<R>(gptr(T.rw), T.rw_rows, tv, T.growth, f, tid);
+    run_rows<R>(gptr(T.rw), T.rw_rows, tv, T.growth, f, tid, T.rseg, T.rcw);
     for (uint32_t p = tid; p < T.n_u; p += kWave) {
         const R u = tv[p];
         f.nonfinite |= dbits((double)(u - u));
@@ -288,17 +271,18 @@ __device__ void solve_neg(const TapeProg &T, uint32_t n, double *tv, const doubl
     for (uint32_t r = tid; r < n; r += kWave) y[T.pinv[r]] = -rhs[r];
     wsync();
     LuFlags f{false, 0, 0};
-    run_rows<double, true>(gptr(T.sw), T.sw_rows, tv, 0.0, f, tid);
+    run_rows<double, true>(gptr(T.sw), T.sw_rows, tv, 0.0, f, tid, T.sseg, T.scw);
 }
 
 // ---- FP32 factors + FP64 refinement of one Newton solve: dx = -A^-1 F.
 // tv32 = [FP32 factor slots | y32]; r64 and dx64 are FP64 vectors; the FP64
-// Jacobian a64 is in pattern order. r = -F - A dx row by row over the
-// pattern in ascending column order. Stops when max|r| <= refine_tol *
-// max(max|F|, max_i sum_j |A_ij dx_j|); a step that does not halve max|r|,
-// or refine_max steps without success, is a stall. ----
+// Jacobian a64 is packed in the residual stream's order (dp.res_w). r = -F -
+// A dx row by row over the pattern in ascending column order. Stops when
+// max|r| <= refine_tol * max(max|F|, max_i sum_j |A_ij dx_j|); a step that
+// does not halve max|r|, or refine_max steps without success, is a stall.
+// scratch: an LDS double the stream's padding rows read and write. ----
 __device__ bool refine_solve(const OpArgs &a, const TapeProg &T, uint32_t n, float *tv32, const double *rhs,
-                             const double *a64, double *dx, double *r, uint32_t tid)
+                             const double *a64, double *dx, double *r, double *scratch, uint32_t tid)
 {
     const DeckProg &dp = a.dp;
     float *y32 = tv32 + T.n_tv;
@@ -309,6 +293,7 @@ __device__ bool refine_solve(const OpArgs &a, const TapeProg &T, uint32_t n, flo
         const uint64_t b = dbits(fabs(rhs[i]));
         fm = b > fm ? b : fm;
     }
+    if (tid == 0) *scratch = 0.0;
     fm = wave_max_u64(fm);
     double prev = __builtin_inf();
     for (uint32_t s = 0; s < a.refine_max; s++) {
@@ -316,23 +301,72 @@ __device__ bool refine_solve(const OpArgs &a, const TapeProg &T, uint32_t n, flo
         for (uint32_t i = tid; i < n; i += kWave) y32[T.pinv[i]] = (float)r[i];
         wsync();
         LuFlags f{false, 0, 0};
-        run_rows<float, true>(gptr(T.sw), T.sw_rows, tv32, 0.0, f, tid);
+        run_rows<float, true>(gptr(T.sw), T.sw_rows, tv32, 0.0, f, tid, T.sseg, T.scw);
         wsync();
         for (uint32_t i = tid; i < n; i += kWave) dx[i] = dx[i] + (double)y32[T.qinv[i]];
         wsync();
+        // r = -F - A dx over the residual stream: a row-start word flushes
+        // the previous row's sum into its r slot (the running sum is stored
+        // at every word; a row belongs to one thread) and starts from -F_i;
+        // an entry subtracts A_ij dx_j. The words' LDS values and the packed
+        // Jacobian's values are read kAhead words ahead of their use.
         uint64_t rm = 0, am = 0;
-        for (uint32_t i = tid; i < n; i += kWave) {
-            double acc = -rhs[i], aa = 0.0;
-            for (uint32_t e = dp.csr_ptr[i]; e < dp.csr_ptr[i + 1]; e++) {
-                const double t = a64[dp.csr_slot[e]] * dx[dp.csr_col[e]];
-                acc = acc - t;
-                aa = aa + fabs(t);
+        {
+            const __attribute__((address_space(1))) uint32_t *rw = gptr(dp.res_w) + tid;
+            const __attribute__((address_space(1))) double *ag = gptr(a64) + tid;
+            auto src = [&](uint32_t w) -> const double * {
+                const double *row = (w & kResPad) ? scratch : rhs + (w & 0x3fffffffu);
+                return (int32_t)w < 0 ? row : dx + w;
+            };
+            uint32_t q[kRing];
+#pragma unroll
+            for (int k = 0; k < kRing; k++) {
+                q[k] = rw[k * kWave];
+                __builtin_amdgcn_sched_barrier(0);
             }
-            r[i] = acc;
+            constexpr int kAhead = kRing / 2;
+            double va[kAhead], ga[kAhead];
+            uint32_t off = 0;   // packed index of the step read ahead, less tid
+#pragma unroll
+            for (int k = 0; k < kAhead; k++) {
+                va[k] = *src(q[k]);
+                ga[k] = ag[off];
+                off += (uint32_t)__popcll(__ballot(q[k] != (kResRow | kResPad)));
+            }
+            double *cur = scratch;
+            double acc = 0.0, aa = 0.0;
+            for (uint32_t st = 0; st < dp.res_steps; st += kRing) {
+#pragma unroll
+                for (int k = 0; k < kRing; k++) {
+                    const uint32_t w = q[k];
+                    const double v = va[k % kAhead], g = ga[k % kAhead];
+                    const bool row = (int32_t)w < 0;
+                    const double t = g * v;
+                    *cur = acc;
+                    const uint64_t rb = dbits(fabs(acc)), ab = dbits(aa);
+                    rm = row && rb > rm ? rb : rm;
+                    am = row && ab > am ? ab : am;
+                    double *next = (w & kResPad) ? scratch : r + (w & 0x3fffffffu);
+                    cur = row ? next : cur;
+                    const double nacc = acc - t, naa = aa + fabs(t);
+                    acc = row ? -v : nacc;
+                    aa = row ? 0.0 : naa;
+                    // step st + k + kAhead: still in the ring, or the refill
+                    // of the slot used kAhead steps ago
+                    const uint32_t wa = q[(k + kAhead) % kRing];
+                    va[k % kAhead] = *src(wa);
+                    ga[k % kAhead] = ag[off];
+                    off += (uint32_t)__popcll(__ballot(wa != (kResRow | kResPad)));
+                    q[k] = rw[(st + kRing + k) * kWave];
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            *cur = acc;
             const uint64_t rb = dbits(fabs(acc)), ab = dbits(aa);
             rm = rb > rm ? rb : rm;
             am = ab > am ? ab : am;
         }
+        wsync();
         const double rmax = bitsd(wave_max_u64(rm));
         const double target = a.refine_tol * fmax(bitsd(fm), bitsd(wave_max_u64(am)));
         if (rmax <= target) return true;
@@ -414,7 +448,7 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
         return kNoRoom;
     }
     float *tv32 = reinterpret_cast<float *>(tv);
-    double *a64 = a.a64 + (size_t)lane * dp.nnz;
+    double *a64 = a.a64 + (size_t)lane * (dp.res_total + kWave);
     uint32_t iter0 = 0, k0 = 0, warm = a.mode == 3 ? 1u : 0u;
     bool init_fix = a.mode == 0;
     double limit = 1e9, f0 = 0.0;
@@ -499,7 +533,7 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
         uint32_t exit_status = kConverged;
         bool have_dx = false;
         if (fp32) {
-            for (uint32_t p = tid; p < dp.nnz; p += kWave) a64[p] = tv[T.amap[p]];
+            for (uint32_t k = tid; k < dp.res_total; k += kWave) a64[k] = tv[T.rsrc[k]];
             __threadfence_block();   // the refinement reads a64 back from other threads
             for (uint32_t sl = tid; sl < T.n_tv; sl += kWave) {
                 const float v = (float)tv[sl];
@@ -510,7 +544,7 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
                 exit_status = (rs.fresh && iter == iter0) ? kToFp64 : kRepivot;
             } else {
                 lap(1);
-                if (refine_solve(a, T, n, tv32, rhs, a64, y, xold, tid)) have_dx = true;
+                if (refine_solve(a, T, n, tv32, rhs, a64, y, xold, rhs + 2 * n, tid)) have_dx = true;
                 else exit_status = kToFp64;
             }
         } else if (priv) {
@@ -648,7 +682,7 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
 // The Newton kernel; op_newton_own adds the lanes' own LU (a lane whose
 // replay fails factors its Jacobian itself). The plain kernel keeps the
 // registers of the common path: lanes that fail there resume in the other.
-__global__ __launch_bounds__(kWave) void op_newton(OpArgs a)
+__global__ __launch_bounds__(kWave, 2) void op_newton(OpArgs a)
 {
     extern __shared__ double sm[];
     op_lane<false>(a, a.lane_ids[blockIdx.x], sm, 1);
@@ -1169,13 +1203,13 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
     }
 }
 
-__global__ __launch_bounds__(kWave) void tran_steps(TranArgs ta)
+__global__ __launch_bounds__(kWave, 2) void tran_steps(TranArgs ta)
 {
     extern __shared__ double sm[];
     tran_lane<false>(ta, ta.op.lane_ids[blockIdx.x], sm, 1);
 }
 
-__global__ __launch_bounds__(kWave) void tran_steps_own(TranArgs ta)
+__global__ __launch_bounds__(kWave, 2) void tran_steps_own(TranArgs ta)
 {
     extern __shared__ double sm[];
     tran_lane<true>(ta, ta.op.lane_ids[blockIdx.x], sm, 1);
@@ -1334,6 +1368,26 @@ extern "C" hipError_t esl_batch_occupancy(uint32_t lds_bytes, int *blocks_per_cu
 {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, esl::batch_run, esl::kWave * esl::kCrew,
                                                         lds_bytes);
+}
+
+// The largest static LDS (the crew's control block) any lane kernel
+// declares: a workgroup's dynamic LDS is the device's per-workgroup LDS
+// less this.
+extern "C" hipError_t esl_kernel_static_lds(uint32_t *bytes)
+{
+    const void *fns[] = {(const void *)esl::op_newton,     (const void *)esl::op_dump,
+                         (const void *)esl::tran_steps,    (const void *)esl::tran_steps_own,
+                         (const void *)esl::op_newton_own, (const void *)esl::batch_run,
+                         (const void *)esl::tape_replay};
+    uint32_t m = 0;
+    for (const void *f : fns) {
+        hipFuncAttributes a{};
+        const hipError_t e = hipFuncGetAttributes(&a, f);
+        if (e != hipSuccess) return e;
+        m = a.sharedSizeBytes > m ? (uint32_t)a.sharedSizeBytes : m;
+    }
+    *bytes = m;
+    return hipSuccess;
 }
 
 extern "C" hipError_t esl_kernel_attrs(uint32_t lds_bytes)

@@ -161,6 +161,9 @@ def load() -> C.CDLL:
     L.esl_debug_big_replay.argtypes = [C.c_uint32, vp, vp, vp, vp, C.c_double, C.c_uint32, vp, vp, C.c_uint32,
                                        vp, vp, vp, vp, vp, vp]
     L.esl_debug_big_replay.restype = C.c_int
+    L.esl_debug_row_replay.argtypes = [C.c_uint32, vp, vp, vp, vp, C.c_double, C.c_uint32, vp, vp, C.c_uint32,
+                                       vp, vp, vp, vp, vp, vp]
+    L.esl_debug_row_replay.restype = C.c_int
     L.esl_tape_create.argtypes = [C.c_uint32, vp, vp, vp, vp, C.c_double, C.POINTER(vp)]
     L.esl_tape_destroy.restype = C.c_uint32
     L.esl_tape_destroy.argtypes = [vp]
@@ -465,6 +468,33 @@ def debug_big_replay(col_ptr, row_idx, lead_vals, q, a, rhs, block_slots=4032, p
                                   _ptr(out["udiag"]), _ptr(out["dx"]), _ptr(out["fail"]), _ptr(sizes)),
            "big replay")
     out["blocks"], out["rows"] = int(sizes[0]), int(sizes[3])
+    return out
+
+
+def debug_row_replay(col_ptr, row_idx, lead_vals, q, a, rhs, min_chain=8, pivot_tol=1e-3):
+    """The wavefront kernels' refactor and solve row programs (chain rows
+    for runs of at least min_chain terms) run on the CPU
+    (esl_debug_row_replay) for a [lanes][nnz], rhs [lanes][n], each row
+    checked for independent ops: a development check of the row programs,
+    no device needed."""
+    L = load()
+    cp, ri, qq = (np.ascontiguousarray(v, dtype=np.uint32) for v in (col_ptr, row_idx, q))
+    lead = np.ascontiguousarray(lead_vals, dtype=np.float64)
+    a = np.ascontiguousarray(np.atleast_2d(a), dtype=np.float64)
+    rhs = np.ascontiguousarray(np.atleast_2d(rhs), dtype=np.float64)
+    n, lanes = len(cp) - 1, a.shape[0]
+    sizes = np.zeros(10, np.uint32)
+    args = (n, _ptr(cp), _ptr(ri), _ptr(lead), _ptr(qq), pivot_tol, min_chain)
+    _check(L.esl_debug_row_replay(*args, None, None, 0, None, None, None, None, None, _ptr(sizes)), "row replay")
+    nl, nu = int(sizes[1]), int(sizes[2])
+    out = dict(lx=np.zeros((lanes, nl)), ux=np.zeros((lanes, nu)), udiag=np.zeros((lanes, n)),
+               dx=np.zeros((lanes, n)), fail=np.zeros(lanes, np.uint32))
+    _check(L.esl_debug_row_replay(*args, _ptr(a), _ptr(rhs), lanes, _ptr(out["lx"]), _ptr(out["ux"]),
+                                  _ptr(out["udiag"]), _ptr(out["dx"]), _ptr(out["fail"]), _ptr(sizes)),
+           "row replay")
+    keys = ("refactor_rows", None, None, "solve_rows", "refactor_chains", "solve_chains", "refactor_chain_blocks",
+            "solve_chain_blocks", "refactor_critical", "solve_critical")
+    out.update({k: int(v) for k, v in zip(keys, sizes) if k})
     return out
 
 

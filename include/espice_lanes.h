@@ -457,6 +457,25 @@ esl_status esl_debug_big_replay(uint32_t n, const uint32_t *col_ptr,
                                 double *udiag, double *dx, uint32_t *fail,
                                 uint32_t *sizes);
 
+/* Development hook, not part of the drop-in: the refactor and solve row
+ * programs of the wavefront kernels (chain rows for runs of at least
+ * min_chain multiply-subtracts on one target; 0: none), run on the CPU row by
+ * row for `lanes` matrices a [lanes][nnz] and right-hand sides rhs
+ * [lanes][n] on the tape factored from lead_vals with column order q, each
+ * row checked for independent ops. Outputs as esl_debug_big_replay;
+ * sizes[10] = {refactor rows, nnz_l, nnz_u, solve rows, refactor chains,
+ * solve chains, refactor chain blocks, solve chain blocks, refactor and
+ * solve critical paths} (lanes == 0: only sizes). ESL_FAILED when a row's
+ * ops are not independent. Needs no device. */
+esl_status esl_debug_row_replay(uint32_t n, const uint32_t *col_ptr,
+                                const uint32_t *row_idx,
+                                const double *lead_vals, const uint32_t *q,
+                                double pivot_tol, uint32_t min_chain,
+                                const double *a, const double *rhs,
+                                uint32_t lanes, double *lx, double *ux,
+                                double *udiag, double *dx, uint32_t *fail,
+                                uint32_t *sizes);
+
 /* ---- The LU boundary alone (arp_lane_refactor + arp_lane_solve,
  * src/solver/lane_device.zig:26-38; bodies lane_kernels.zig:49-143). ----
  * A tape is built on the host from one matrix (the lead lane) by the
