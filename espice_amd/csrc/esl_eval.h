@@ -369,6 +369,55 @@ __device__ void run_chain(esl::uint2_t w, const esl::uint2_t *segs, const uint32
     M[d] = acc;
 }
 
+// A broadcast chain row (kChainBcast): its chains one after another on the
+// whole wavefront. Per round of 64 terms thread k loads term k, forms its
+// product (zero for a skipped R_MACZ term and for padding) and writes it to
+// pad slot k (the pads are free between rows); then every thread subtracts
+// the 64 products from the target in term order, reading them at one
+// address for all threads, eight reads ahead of the subtractions, so each
+// target rounds exactly as its ops one by one. Headers and segments are read
+// through the scalar cache; the term words of up to four rounds load
+// together.
+template <typename R, bool kSolve, typename MP>
+__device__ void run_bchain(esl::uint2_t w, const esl::uint2_t *segs, const uint32_t *cw, MP M, uint32_t pad,
+                           uint32_t tid)
+{
+    const __attribute__((address_space(4))) uint32_t *sc =
+        (const __attribute__((address_space(4))) uint32_t *)cw;
+    const __attribute__((address_space(4))) esl::uint2_t *ss =
+        (const __attribute__((address_space(4))) esl::uint2_t *)segs;
+    const uint32_t seg = __builtin_amdgcn_readfirstlane(w.y & 0xffffu);
+    const uint32_t h0 = ss[seg].x, nc = ss[seg].y;
+    const __attribute__((address_space(1))) uint32_t *g = gptr(cw) + tid;
+    for (uint32_t j = 0; j < nc; j++) {
+        const uint32_t d = sc[h0 + 4 * j], len = sc[h0 + 4 * j + 1], off = sc[h0 + 4 * j + 2];
+        R acc = M[d];
+        for (uint32_t r0 = 0; r0 < len; r0 += 4 * kWave) {
+            uint32_t tw[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) tw[q] = r0 + q * kWave < len ? g[off + r0 + q * kWave] : 0u;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t at = r0 + q * kWave;
+                if (at >= len) break;
+                const R A = M[tw[q] & 0xffffu], B = M[tw[q] >> 16];
+                const R p = A * B;
+                const bool skip = at + tid >= len || (kSolve && B == R(0));
+                M[pad + tid] = skip ? R(0) : p;
+                const uint32_t groups = ((len - at < (uint32_t)kWave ? len - at : (uint32_t)kWave) + 7u) >> 3;
+                for (uint32_t k8 = 0; k8 < groups; k8++) {
+                    R v[8];
+#pragma unroll
+                    for (int e = 0; e < 8; e++) v[e] = M[pad + k8 * 8 + e];
+#pragma unroll
+                    for (int e = 0; e < 8; e++) acc = acc - v[e];
+                }
+            }
+        }
+        M[d] = acc;
+    }
+}
+
 // Runs a row program of `rows` rows, a multiple of kRing (its words
 // followed by at least kRing more rows, which are only loaded), with kRing
 // rows of words in flight. M
@@ -379,7 +428,7 @@ __device__ void run_chain(esl::uint2_t w, const esl::uint2_t *segs, const uint32
 // always a block's last row, runs after the block (segs, cw: the program's
 // chain segments and terms; null for programs without chain rows).
 template <typename R, bool kSolve = false, typename WP, typename MP>
-__device__ void run_rows(WP words, uint32_t rows, MP M, double growth, LuFlags &f, uint32_t tid,
+__device__ void run_rows(WP words, uint32_t rows, MP M, double growth, LuFlags &f, uint32_t tid, uint32_t pad,
                          const esl::uint2_t *segs = nullptr, const uint32_t *cw = nullptr)
 {
     const WP w = words + tid;
@@ -399,8 +448,12 @@ __device__ void run_rows(WP words, uint32_t rows, MP M, double growth, LuFlags &
             q[k] = esl::uint2_t{w[at].x, w[at].y};
             __builtin_amdgcn_sched_barrier(0);
         }
-        if (segs && __builtin_amdgcn_readfirstlane(esl::row_kind(last.y)) == R_CHAIN)
-            run_chain<R, kSolve>(last, segs, cw, M, tid);
+        if (segs && __builtin_amdgcn_readfirstlane(esl::row_kind(last.y)) == R_CHAIN) {
+            if (__builtin_amdgcn_readfirstlane(last.y) & kChainBcast)
+                run_bchain<R, kSolve>(last, segs, cw, M, pad, tid);
+            else
+                run_chain<R, kSolve>(last, segs, cw, M, tid);
+        }
     }
 }
 

@@ -104,9 +104,16 @@ struct DeckProg {
     // the entry order of their evaluation (the lane-per-thread kernel)
     const uint32_t *d_slot, *q_slot;
     // the refinement residual's stream [res_steps + kRing][64] and the
-    // words it holds (the length of a lane's packed FP64 Jacobian)
+    // length of a lane's packed FP64 Jacobian (stream words, then the
+    // chains' entries, each chain padded to 64)
     uint32_t res_steps, res_total;
     const uint32_t *res_w;
+    // the residual's chain rows, run after the stream by the whole
+    // wavefront: {row, entries, first column word, first a64 position},
+    // and their columns (each chain padded to 64 words)
+    const uint4_t *res_ch;
+    const uint32_t *res_cc;
+    uint32_t n_res_ch, pad_rc;
 };
 constexpr uint32_t kNoSlot = 0xffffffffu;
 // Residual-stream words: kResRow | i starts row i, kResRow | kResPad pads,
@@ -138,6 +145,14 @@ enum RowKind : uint32_t {
     R_CHAIN = 4,  // a run of R_MAC (refactor) or R_MACZ (solve) terms per thread
     R_KINDS = 5
 };
+// A chain row may instead run its chains one after another on the whole
+// wavefront (flag kChainBcast in y): its segment (header word offset,
+// chains) points at headers {d, terms, term word offset, 0} and each chain's
+// terms b | (c << 16), padded to a multiple of 64 words. Thread k loads term
+// k of a round of 64 and forms its product; the multiply-subtracts then run
+// in order on the broadcast products, so the target rounds as the ops one by
+// one (a skipped R_MACZ term subtracts +0, which leaves any value unchanged).
+constexpr uint32_t kChainBcast = 1u << 22;
 constexpr int kChainBlk = 8;     // terms of a chain block (one thread's 32 bytes)
 constexpr int kChainAhead = 2;   // blocks of term words in flight (and padded at a segment's end)
 constexpr uint32_t kFlagUnscaled = 1;   // R_DIV on an unscaled column: growth + wide tests
@@ -151,6 +166,9 @@ constexpr uint32_t kRedTarget = 0x80000000u, kRedRhs = 0x40000000u;
 // double at rhs + 2n (after x), which nothing else reads.
 __host__ __device__ inline uint32_t red_pad(uint32_t n) { return kRedTarget | kRedRhs | (2 * n); }
 constexpr int kRing = 16;      // rows in flight; streams carry kRing pad rows
+// 16-bit cell indices of a chunk's reduce chains: they fill the 64 pad
+// slots of the factor area
+constexpr uint32_t kRedChainCells = 256;
 
 // Doubles of a lane's working set in LDS: tv_cap factor slots, the Y block
 // (y, sdiag, xold after the assembly; during it the stamp staging areas,
@@ -234,6 +252,14 @@ struct TapeProg {
     // [res_total]: the factor slot each position of a lane's packed FP64
     // Jacobian copies (a pad slot for a row start, whose value is unused)
     const uint32_t *rsrc;
+    // assembly reduce chains: the longest targets of each chunk, one per
+    // thread; headers {target word as in the reduce stream, cells | index
+    // offset << 16} from rc_off[c] (rc_count[c] of them), the chunk's
+    // kRedChainCells 16-bit cell indices from word rc_coff[c], staged in
+    // the pad slots while the chains run, rc_blocks[c] blocks of kChainBlk
+    const uint2_t *rc_head;
+    const uint32_t *rc_cells;
+    const uint32_t *rc_off, *rc_count, *rc_coff, *rc_blocks;
 };
 
 // Per-lane results of the Newton kernel.

@@ -384,6 +384,11 @@ struct RowProgram {
 
 // Shortest run of terms on one target scheduled as a chain (0: none).
 constexpr u32 kMinChain = 8;
+// A chain row's cost on the device, in cycles (measured on the OTA array):
+// threads in parallel, per block of kChainBlk terms of the longest chain
+// (each block waits for its term words), or the whole wavefront chain by
+// chain, per chain and per term.
+constexpr double kParBlock = 860.0, kBcastSetup = 250.0, kBcastTerm = 16.0;
 
 class OpGraph {
   public:
@@ -522,8 +527,42 @@ class OpGraph {
             const u32 kind = row_kind[r];
             if (kind == esl::R_CHAIN) {
                 u32 lmax = 0;
-                for (u32 v : rows[r]) lmax = std::max(lmax, len_of(v));
+                double bcost = 0.0;
+                for (u32 v : rows[r]) {
+                    lmax = std::max(lmax, len_of(v));
+                    bcost += kBcastSetup + kBcastTerm * len_of(v);
+                }
                 const u32 blocks = (lmax + esl::kChainBlk - 1) / esl::kChainBlk;
+                if (bcost < kParBlock * blocks) {
+                    // the wavefront runs the row's chains one after another:
+                    // headers (target, terms, term offset) and each chain's
+                    // terms padded to a multiple of 64 words
+                    const u32 seg = (u32)out.seg.size();
+                    const u32 nc = (u32)rows[r].size();
+                    const size_t hoff = out.cw.size();
+                    out.seg.push_back(esl::uint2_t{(u32)hoff, nc});
+                    out.cw.resize(hoff + (size_t)(4 * nc + esl::kWave - 1) / esl::kWave * esl::kWave, 0u);
+                    for (u32 j = 0; j < nc; j++) {
+                        const auto &ch = chains[rows[r][j] - m];
+                        const u32 len = ch.second - ch.first + 1;
+                        const size_t toff = out.cw.size();
+                        out.cw.resize(toff + (size_t)(len + esl::kWave - 1) / esl::kWave * esl::kWave,
+                                      pad_base | (pad_base << 16));
+                        for (u32 k = 0; k < len; k++) {
+                            const SchedOp &o = ops_[ch.first + k];
+                            out.cw[toff + k] = o.b | (o.c << 16);
+                        }
+                        out.cw[hoff + 4 * j] = ops_[ch.first].d;
+                        out.cw[hoff + 4 * j + 1] = len;
+                        out.cw[hoff + 4 * j + 2] = (u32)toff;
+                        out.chain_terms += len;
+                        out.chain_blocks += (len + esl::kChainBlk - 1) / esl::kChainBlk;
+                    }
+                    for (u32 t = 0; t < (u32)esl::kWave; t++)
+                        out.words[(size_t)r * esl::kWave + t] =
+                            esl::uint2_t{pad_base + t, seg | ((u32)esl::R_CHAIN << 16) | esl::kChainBcast};
+                    continue;
+                }
                 const u32 seg = (u32)out.seg.size();
                 out.seg.push_back(esl::uint2_t{(u32)out.cw.size(), blocks});
                 const size_t base = out.cw.size();
@@ -585,6 +624,11 @@ struct TapeHostProg {
     vector<u32> blev_ptr, blev_rows, urow_ptr, urow_k, urow_slot, void_slots;
     RowProgram rch, sch;
     vector<u32> red, red_off, red_len, diag_tv;
+    // assembly reduce chains: headers {target word, cells | index offset <<
+    // 16}, per chunk with chains kRedChainCells 16-bit cell indices, and per
+    // chunk its first header, chains, first index word and blocks
+    vector<esl::uint2_t> rc_head;
+    vector<u32> rc_cells, rc_off, rc_count, rc_coff, rc_blocks;
     vector<u32> rsrc;   // the factor slot of each position of the packed FP64 Jacobian
     uint64_t flops = 0;
     // lanes beyond a workgroup's LDS: the replay in blocks and the solves in
@@ -960,6 +1004,9 @@ hipError_t upload_tape_prog(TapeHostProg &o)
     f.push_back(pk.add(o.rch.seg)); f.push_back(pk.add(o.sch.seg));
     f.push_back(pk.add(o.rch.cw)); f.push_back(pk.add(o.sch.cw));
     f.push_back(pk.add(o.rsrc));
+    f.push_back(pk.add(o.rc_head)); f.push_back(pk.add(o.rc_cells));
+    f.push_back(pk.add(o.rc_off)); f.push_back(pk.add(o.rc_count));
+    f.push_back(pk.add(o.rc_coff)); f.push_back(pk.add(o.rc_blocks));
     hipError_t e = o.buf.ensure(pk.blob.size());
     if (e != hipSuccess) return e;
     e = hipMemcpy(o.buf.p, pk.blob.data(), pk.blob.size(), hipMemcpyHostToDevice);
@@ -994,6 +1041,12 @@ hipError_t upload_tape_prog(TapeHostProg &o)
     T.rcw = P(i++);
     T.scw = P(i++);
     T.rsrc = P(i++);
+    T.rc_head = reinterpret_cast<const esl::uint2_t *>(P(i++));
+    T.rc_cells = P(i++);
+    T.rc_off = P(i++);
+    T.rc_count = P(i++);
+    T.rc_coff = P(i++);
+    T.rc_blocks = P(i++);
     if (i != f.size()) return hipErrorInvalidValue;
     return hipSuccess;
 }
@@ -1084,6 +1137,8 @@ struct Compiled {
     vector<u32> csr_ptr, csr_col, csr_slot;   // the pattern by rows, columns ascending
     vector<u32> res_w, res_p;                 // the refinement residual's stream (build_res_stream)
     u32 res_steps = 0, res_total = 0;
+    vector<esl::uint4_t> res_ch;              // the residual's chain rows {row, entries, column word, a64 position}
+    vector<u32> res_cc;                       // their columns, each chain padded to 64 words
     vector<u32> d_t, c_t, v_wkind, cslot_pat, q_t;
     vector<double> d_par, c_val, v_v2, v_td, q_par;
     vector<int32_t> d_bind_is, c_bind, q_bind_is, q_bind_bf;
@@ -1115,18 +1170,51 @@ void build_csr(Compiled &c)
 // res_w [res_steps + kRing][64]: kResRow | i starts row i, j an entry in
 // column j, kResRow | kResPad pads. The FP64 Jacobian a lane refines with
 // is stored in the order of the words (row starts included), packed: step
-// s's busy threads t at a64 index off_s + t, off_s the words before step s
-// (res_total in all); res_p gives each of those positions its pattern slot
-// (kNoSlot for a row start).
+// s's busy threads t at a64 index off_s + t, off_s the words before step s;
+// res_p gives each of those positions its pattern slot (kNoSlot for a row
+// start or padding). A row far longer than the average (a supply rail's)
+// would set the stream's length by itself: the longest rows instead become
+// chains that the whole wavefront runs after the stream (their products in
+// parallel, the subtractions in column order), their entries after the
+// stream's in a64, each chain padded to 64 positions (res_total in all).
+// Chains need the 64 pad slots of the FP64 factor area free while the
+// factors are FP32, which holds when the pattern has n + 64 entries or
+// more (the FP32 factors and solve vector then end below the pads).
 void build_res_stream(Compiled &c)
 {
+    constexpr double kStepCost = 280.0, kChainSetup = 1000.0, kTermCost = 13.0;
     const u32 n = c.n;
     vector<u32> rows(n);
     for (u32 i = 0; i < n; i++) rows[i] = i;
     auto len = [&](u32 i) { return 1 + c.csr_ptr[i + 1] - c.csr_ptr[i]; };
     std::stable_sort(rows.begin(), rows.end(), [&](u32 a, u32 b) { return len(a) > len(b); });
+    auto longest = [&](size_t first) {
+        vector<size_t> load(esl::kWave, 0);
+        for (size_t k = first; k < rows.size(); k++) {
+            size_t best = 0;
+            for (size_t t = 1; t < load.size(); t++)
+                if (load[t] < load[best]) best = t;
+            load[best] += len(rows[k]);
+        }
+        size_t L = 0;
+        for (size_t v : load) L = std::max(L, v);
+        return (L + esl::kRing - 1) / esl::kRing * esl::kRing;
+    };
+    size_t n_ch = 0;
+    if (c.nnz >= n + esl::kWave) {
+        double best = kStepCost * (double)longest(0), chain = 0.0;
+        for (size_t k = 1; k < rows.size() && k <= esl::kWave && len(rows[k - 1]) > 2 * esl::kRing; k++) {
+            chain += kChainSetup + kTermCost * (double)len(rows[k - 1]);
+            const double cost = kStepCost * (double)longest(k) + chain;
+            if (cost < best) {
+                best = cost;
+                n_ch = k;
+            }
+        }
+    }
     vector<vector<u32>> words(esl::kWave), slots(esl::kWave);
-    for (u32 i : rows) {
+    for (size_t k = n_ch; k < rows.size(); k++) {
+        const u32 i = rows[k];
         size_t best = 0;
         for (size_t t = 1; t < words.size(); t++)
             if (words[t].size() < words[best].size()) best = t;
@@ -1152,6 +1240,18 @@ void build_res_stream(Compiled &c)
             c.res_w[(size_t)st * esl::kWave + t] = w[st];
             c.res_p.push_back(slots[order[t]][st]);
         }
+    c.res_ch.clear();
+    c.res_cc.clear();
+    for (size_t k = 0; k < n_ch; k++) {
+        const u32 i = rows[k], m = len(i) - 1;
+        const u32 padded = (m + esl::kWave - 1) / esl::kWave * esl::kWave;
+        c.res_ch.push_back(esl::uint4_t{i, m, (u32)c.res_cc.size(), (u32)c.res_p.size()});
+        for (u32 e = 0; e < padded; e++) {
+            const bool real = e < m;
+            c.res_cc.push_back(real ? c.csr_col[c.csr_ptr[i] + e] : 0u);
+            c.res_p.push_back(real ? c.csr_slot[c.csr_ptr[i] + e] : esl::kNoSlot);
+        }
+    }
     c.res_total = (u32)c.res_p.size();
 }
 
@@ -1600,15 +1700,40 @@ hipError_t upload_deck(Engine &e)
     build_res_stream(c);
     // the residual stream's words index rows and columns of the pattern,
     // and its packed Jacobian holds every entry and row start once
-    if (c.res_total != c.nnz + c.n || c.res_steps % esl::kRing ||
-        c.res_w.size() != (size_t)(c.res_steps + esl::kRing) * esl::kWave)
+    {
+        // every row and entry once: in the stream (row starts included) or a chain
+        size_t chain_rows = 0, chain_pos = 0;
+        u32 cc_end = 0;
+        const u32 stream_total = c.res_ch.empty() ? c.res_total : c.res_ch.front().w;
+        for (const esl::uint4_t &h : c.res_ch) {
+            const u32 padded = (h.y + esl::kWave - 1) / esl::kWave * esl::kWave;
+            if (h.x >= c.n || h.y == 0 || h.y != c.csr_ptr[h.x + 1] - c.csr_ptr[h.x] || h.z != cc_end ||
+                h.w != stream_total + chain_pos)
+                return hipErrorInvalidValue;
+            cc_end += padded;
+            chain_pos += padded;
+            chain_rows++;
+        }
+        if (!c.res_ch.empty() && c.nnz < c.n + esl::kWave) return hipErrorInvalidValue;
+        if (cc_end != c.res_cc.size() || stream_total + chain_pos != c.res_total) return hipErrorInvalidValue;
+        for (u32 col : c.res_cc)
+            if (col >= c.n) return hipErrorInvalidValue;
+        size_t stream_entries = 0;
+        for (u32 w : c.res_w)
+            if (w != (esl::kResRow | esl::kResPad)) stream_entries++;
+        size_t chain_entries = 0;
+        for (const esl::uint4_t &h : c.res_ch) chain_entries += h.y;
+        if (stream_entries != stream_total || stream_entries + chain_entries + chain_rows != c.nnz + c.n)
+            return hipErrorInvalidValue;
+    }
+    if (c.res_steps % esl::kRing || c.res_w.size() != (size_t)(c.res_steps + esl::kRing) * esl::kWave)
         return hipErrorInvalidValue;
     for (u32 w : c.res_w)
         if (w != (esl::kResRow | esl::kResPad) && (w & 0x3fffffffu) >= c.n) return hipErrorInvalidValue;
     for (u32 p : c.res_p)
         if (p != esl::kNoSlot && p >= c.nnz) return hipErrorInvalidValue;
     size_t o_cp = pk.add(c.csr_ptr), o_cc = pk.add(c.csr_col), o_cs = pk.add(c.csr_slot);
-    size_t o_rw = pk.add(c.res_w);
+    size_t o_rw = pk.add(c.res_w), o_rch = pk.add(c.res_ch), o_rcc = pk.add(c.res_cc);
     size_t o_dt = pk.add(c.d_t), o_dp = pk.add(c.d_par), o_db = pk.add(c.d_bind_is), o_ct = pk.add(c.c_t),
            o_cv = pk.add(c.c_val), o_cb = pk.add(c.c_bind), o_vw = pk.add(c.v_wkind), o_v2 = pk.add(c.v_v2),
            o_vtd = pk.add(c.v_td), o_csp = pk.add(c.cslot_pat), o_qt = pk.add(c.q_t), o_qp = pk.add(c.q_par),
@@ -1659,6 +1784,9 @@ hipError_t upload_deck(Engine &e)
     dp.res_w = reinterpret_cast<const u32 *>(b + o_rw);
     dp.res_steps = c.res_steps;
     dp.res_total = c.res_total;
+    dp.res_ch = reinterpret_cast<const esl::uint4_t *>(b + o_rch);
+    dp.res_cc = reinterpret_cast<const u32 *>(b + o_rcc);
+    dp.n_res_ch = (u32)c.res_ch.size();
     dp.n_dio = c.n_dio;
     dp.n_cap = c.n_cap;
     dp.n_cslot = (u32)c.cslot_pat.size();
@@ -1739,12 +1867,28 @@ u32 lds_bytes_for(u32 tv_cap, u32 n)
 // Assembly reduce streams for one tape: each chunk's reduce tasks with their
 // targets mapped to factor slots, dealt to the 64 threads longest first, each
 // thread's words laid out row by row ([row][64]) so a wavefront reads one
-// coalesced row per step.
+// coalesced row per step. A step costs the whole wavefront whatever its
+// threads hold, so a target with far more cells than the chunk's average
+// (a supply rail's diagonal and residual row, which collect cells from
+// every device on the rail) would set the stream's length by itself; the
+// longest tasks of a chunk instead become chains, one per thread, which add
+// their cells in order with the target in a register, eight cells a block,
+// their 16-bit cell indices staged in the tape's 64 pad slots (unused
+// during the assembly; room for kRedChainCells indices). The split is
+// chosen per chunk by a cost model of the two loops (cycles per stream step
+// and per chain block, measured).
 void build_reduce_streams(const Compiled &c, TapeHostProg &o)
 {
+    constexpr double kStepCost = 130.0, kChainSetup = 300.0, kBlockCost = 200.0;
     o.red.clear();
     o.red_off.clear();
     o.red_len.clear();
+    o.rc_head.clear();
+    o.rc_cells.clear();
+    o.rc_off.clear();
+    o.rc_count.clear();
+    o.rc_coff.clear();
+    o.rc_blocks.clear();
     const size_t nch = c.chunks.size() - 1;
     for (size_t ci = 0; ci < nch; ci++) {
         const u32 t0 = c.chunks[ci].task_begin, t1 = c.chunks[ci + 1].task_begin;
@@ -1753,26 +1897,80 @@ void build_reduce_streams(const Compiled &c, TapeHostProg &o)
             if (!(c.task_target[t] & (esl::kCBit | esl::kQBit))) order.push_back(t);
         auto len = [&](u32 t) { return 1 + c.task_cell[t + 1] - c.task_cell[t]; };
         std::stable_sort(order.begin(), order.end(), [&](u32 a, u32 b) { return len(a) > len(b); });
-        vector<vector<u32>> words(esl::kWave);
-        for (u32 t : order) {
-            size_t best = 0;
-            for (size_t w = 1; w < words.size(); w++)
-                if (words[w].size() < words[best].size()) best = w;
+        auto word_of = [&](u32 t) {
             const u32 tg = c.task_target[t];
-            const u32 word = (tg & esl::kRhsBit) ? (esl::kRedTarget | esl::kRedRhs | (tg & ~esl::kRhsBit))
-                                                 : (esl::kRedTarget | o.amap[tg]);
-            words[best].push_back(word);
-            for (u32 cc = c.task_cell[t]; cc < c.task_cell[t + 1]; cc++) words[best].push_back(c.cells[cc]);
+            return (tg & esl::kRhsBit) ? (esl::kRedTarget | esl::kRedRhs | (tg & ~esl::kRhsBit))
+                                       : (esl::kRedTarget | o.amap[tg]);
+        };
+        // the stream rows of the tasks from `first` on, dealt longest first
+        // to the least loaded thread
+        auto deal = [&](size_t first, vector<vector<u32>> *words) {
+            vector<size_t> load(esl::kWave, 0);
+            for (size_t k = first; k < order.size(); k++) {
+                const u32 t = order[k];
+                size_t best = 0;
+                for (size_t w = 1; w < load.size(); w++)
+                    if (load[w] < load[best]) best = w;
+                load[best] += len(t);
+                if (words) {
+                    (*words)[best].push_back(word_of(t));
+                    for (u32 cc = c.task_cell[t]; cc < c.task_cell[t + 1]; cc++) (*words)[best].push_back(c.cells[cc]);
+                }
+            }
+            size_t L = 0;
+            for (size_t v : load) L = std::max(L, v);
+            return (L + esl::kRing - 1) / esl::kRing * esl::kRing;
+        };
+        // chains: the k longest tasks whose indices fit the pads; a task of
+        // few cells never pays
+        auto aligned = [](u32 cells) { return (cells + esl::kChainBlk - 1) / esl::kChainBlk * esl::kChainBlk; };
+        size_t best_k = 0;
+        double best_cost = kStepCost * (double)deal(0, nullptr);
+        // every chain thread reads the longest chain's blocks from its own
+        // offset on: the last chain's offset plus those must fit the pads
+        u32 at_last = 0;
+        for (size_t k = 1; k <= std::min<size_t>(order.size(), esl::kWave) && len(order[k - 1]) > 2 * esl::kRing;
+             k++) {
+            if (k > 1) at_last += aligned(len(order[k - 2]) - 1);
+            if (at_last + aligned(len(order[0]) - 1) > esl::kRedChainCells) break;
+            const double blocks = (double)aligned(len(order[0]) - 1) / esl::kChainBlk;
+            const double cost = kStepCost * (double)deal(k, nullptr) + kChainSetup + kBlockCost * blocks;
+            if (cost < best_cost) {
+                best_cost = cost;
+                best_k = k;
+            }
         }
-        size_t L = 0;
-        for (auto &w : words) L = std::max(L, w.size());
-        L = (L + esl::kRing - 1) / esl::kRing * esl::kRing;
+        vector<vector<u32>> words(esl::kWave);
+        const size_t L = deal(best_k, &words);
         o.red_off.push_back((u32)o.red.size());
         o.red_len.push_back((u32)L);
         const size_t base = o.red.size();
         o.red.resize(base + (L + esl::kRing) * esl::kWave, esl::red_pad(c.n));
         for (size_t t = 0; t < words.size(); t++)
             for (size_t w = 0; w < words[t].size(); w++) o.red[base + w * esl::kWave + t] = words[t][w];
+        // the chains' headers {target word, cells | index offset << 16}, one
+        // per thread, and their indices as kRedChainCells 16-bit values
+        // (two per word; unused ones 0, a valid staging index)
+        o.rc_off.push_back((u32)o.rc_head.size());
+        o.rc_count.push_back((u32)best_k);
+        o.rc_coff.push_back((u32)o.rc_cells.size());
+        u32 blocks = 0;
+        if (best_k) {
+            const size_t cb = o.rc_cells.size();
+            o.rc_cells.resize(cb + esl::kRedChainCells / 2, 0u);
+            u32 at = 0;
+            for (size_t k = 0; k < best_k; k++) {
+                const u32 task = order[k], cells = len(task) - 1;
+                o.rc_head.push_back(esl::uint2_t{word_of(task), cells | (at << 16)});
+                for (u32 j = 0; j < cells; j++) {
+                    const u32 cell = c.cells[c.task_cell[task] + j];
+                    o.rc_cells[cb + (at + j) / 2] |= ((at + j) % 2) ? (cell << 16) : cell;
+                }
+                at += aligned(cells);
+                blocks = std::max(blocks, aligned(cells) / esl::kChainBlk);
+            }
+        }
+        o.rc_blocks.push_back(blocks);
     }
     o.diag_tv.assign(c.n, 0);
     for (u32 i = 0; i < c.n; i++) o.diag_tv[i] = o.amap[c.diag[i]];
@@ -1859,26 +2057,58 @@ bool validate_tape(const TapeHostProg &o, u32 n, u32 nnz)
         if (rp.rows % esl::kRing || rp.used > rp.rows) return false;
         if (rp.words.size() != (size_t)(rp.rows + esl::kRing) * esl::kWave) return false;
         const u32 lim = solve ? P.n_tv + n : P.n_tv;
+        // each segment belongs to one chain row, parallel (0) or broadcast (1)
+        vector<int> seg_mode(rp.seg.size(), -1);
+        for (size_t r = 0; r < rp.rows + (size_t)esl::kRing; r++) {
+            const esl::uint2_t w = rp.words[r * esl::kWave];
+            if (esl::row_kind(w.y) != esl::R_CHAIN) continue;
+            const u32 seg = w.y & 0xffff;
+            if (seg >= rp.seg.size() || seg_mode[seg] != -1) return false;
+            seg_mode[seg] = (w.y & esl::kChainBcast) ? 1 : 0;
+        }
+        auto term_ok = [&](u32 w) { return (w & 0xffff) < lim && (w >> 16) < lim; };
         size_t cw_end = 0;
         for (size_t si = 0; si < rp.seg.size(); si++) {
             const esl::uint2_t sg = rp.seg[si];
-            if (sg.x != cw_end || sg.y == 0) return false;
-            cw_end += (size_t)(sg.y + esl::kChainAhead) * esl::kWave * esl::kChainBlk;
+            if (sg.x != cw_end || sg.y == 0 || seg_mode[si] < 0) return false;
+            if (seg_mode[si] == 0) {
+                cw_end += (size_t)(sg.y + esl::kChainAhead) * esl::kWave * esl::kChainBlk;
+                if (cw_end > rp.cw.size()) return false;
+                for (size_t k = sg.x; k < cw_end; k++)
+                    if (!term_ok(rp.cw[k])) return false;
+                continue;
+            }
+            cw_end += (size_t)(4 * sg.y + esl::kWave - 1) / esl::kWave * esl::kWave;
+            if (cw_end > rp.cw.size()) return false;
+            for (u32 j = 0; j < sg.y; j++) {
+                const u32 d = rp.cw[sg.x + 4 * j], len = rp.cw[sg.x + 4 * j + 1], off = rp.cw[sg.x + 4 * j + 2];
+                if (d >= lim || len == 0 || len > 0xffff || off != cw_end) return false;
+                cw_end += (size_t)(len + esl::kWave - 1) / esl::kWave * esl::kWave;
+                if (cw_end > rp.cw.size()) return false;
+                for (size_t k = off; k < cw_end; k++)
+                    if (!term_ok(rp.cw[k])) return false;
+            }
         }
         if (cw_end != rp.cw.size()) return false;
-        for (u32 w : rp.cw)
-            if ((w & 0xffff) >= lim || (w >> 16) >= lim) return false;
         for (size_t r = 0; r < rp.rows + (size_t)esl::kRing; r++) {
             const u32 kind = esl::row_kind(rp.words[r * esl::kWave].y);
+            const u32 bflag = rp.words[r * esl::kWave].y & esl::kChainBcast;
             if (kind >= esl::R_KINDS || (kind == esl::R_MAC && solve)) return false;
             if (kind == esl::R_MACZ && !solve) return false;
+            if (bflag && kind != esl::R_CHAIN) return false;
             // chain rows only at the last position of a block of kRing rows
             if (kind == esl::R_CHAIN && r % esl::kRing != (size_t)esl::kRing - 1) return false;
             for (u32 t = 0; t < (u32)esl::kWave; t++) {
                 const esl::uint2_t w = rp.words[r * esl::kWave + t];
-                if (esl::row_kind(w.y) != kind || (w.y >> 22) != 0) return false;
+                if (esl::row_kind(w.y) != kind || ((w.y & ~esl::kChainBcast) >> 22) != 0 ||
+                    (w.y & esl::kChainBcast) != bflag)
+                    return false;
                 if (solve && esl::row_flags(w.y) != 0) return false;   // no peel tests in the solves
                 if (kind == esl::R_NOP) continue;
+                if (kind == esl::R_CHAIN && bflag) {
+                    if ((w.x & 0xffff) >= lim) return false;
+                    continue;
+                }
                 if (kind == esl::R_CHAIN) {
                     const u32 d = w.x & 0xffff, len = w.x >> 16, seg = w.y & 0xffff;
                     if (d >= lim || seg >= rp.seg.size() || len > rp.seg[seg].y * (u32)esl::kChainBlk) return false;
@@ -1899,6 +2129,35 @@ bool validate_tape(const TapeHostProg &o, u32 n, u32 nnz)
             if (o.red_len[ci] % esl::kRing) return false;
         }
         if (2 * n > 0xfffff) return false;
+        auto target_ok = [&](u32 w) {
+            const u32 idx = w & 0xfffff;
+            if ((w & esl::kRedRhs) ? (idx >= n && w != esl::red_pad(n)) : idx >= P.n_tv) return false;
+            return (w & ~(esl::kRedTarget | esl::kRedRhs)) <= 0xfffff;
+        };
+        const size_t nch = o.red_off.size();
+        if (o.rc_off.size() != nch || o.rc_count.size() != nch || o.rc_coff.size() != nch ||
+            o.rc_blocks.size() != nch)
+            return false;
+        for (size_t ci = 0; ci < nch; ci++) {
+            const u32 nc = o.rc_count[ci];
+            if (nc == 0) continue;
+            if (nc > (u32)esl::kWave || (size_t)o.rc_off[ci] + nc > o.rc_head.size() ||
+                (size_t)o.rc_coff[ci] + esl::kRedChainCells / 2 > o.rc_cells.size())
+                return false;
+            for (u32 j = 0; j < nc; j++) {
+                const esl::uint2_t h = o.rc_head[o.rc_off[ci] + j];
+                const u32 cells = h.y & 0xffff, at = h.y >> 16;
+                if (!(h.x & esl::kRedTarget) || !target_ok(h.x) || cells == 0 || at % esl::kChainBlk ||
+                    at + o.rc_blocks[ci] * esl::kChainBlk > esl::kRedChainCells ||
+                    cells > o.rc_blocks[ci] * (u32)esl::kChainBlk)
+                    return false;
+            }
+        }
+        if (2 * esl::kRedChainCells > esl::kWave * sizeof(double)) return false;
+        for (u32 w : o.rc_cells)
+            if ((w & 0xffff) >= (u32)(esl::kWave * esl::kStageEntries) ||
+                (w >> 16) >= (u32)(esl::kWave * esl::kStageEntries))
+                return false;
         for (u32 w : o.red) {
             if (w & esl::kRedTarget) {
                 const u32 idx = w & 0xfffff;
@@ -2015,6 +2274,9 @@ TapeHostProg *build_tape(const Engine &e, const HostTape &t, const char *&err)
             tp->rsrc[k] = e.c.res_p[k] == esl::kNoSlot ? tp->prog.pad_slot : tp->amap[e.c.res_p[k]];
     }
     if (!err && !validate_tape(*tp, n, e.c.nnz)) err = "tape program failed validation";
+    // the residual's chain rows use the FP64 pads while the factors are FP32
+    if (!err && !tp->big && !e.c.res_ch.empty() && 2 * (size_t)(tp->prog.n_tv - esl::kWave) < (size_t)tp->prog.n_tv + n)
+        err = "tape program failed validation";
     if (err) {
         delete tp;
         return nullptr;
@@ -2391,6 +2653,38 @@ esl_status esl_debug_row_replay(uint32_t n, const uint32_t *col_ptr, const uint3
             // independence: no two ops of the row write one slot, and no op
             // reads (besides its own target) a slot the row writes
             vector<std::pair<u32, vector<u32>>> ops;
+            const esl::uint2_t w0 = rp.words[(size_t)r * esl::kWave];
+            if (kind == esl::R_CHAIN && (w0.y & esl::kChainBcast)) {
+                // the wavefront's chains one after another, each term's
+                // product formed first, a skipped R_MACZ term as -(+0)
+                const esl::uint2_t sg = rp.seg[w0.y & 0xffff];
+                for (u32 j = 0; j < sg.y; j++) {
+                    const u32 d = rp.cw[sg.x + 4 * j], len = rp.cw[sg.x + 4 * j + 1], off = rp.cw[sg.x + 4 * j + 2];
+                    vector<u32> rd;
+                    for (u32 k = 0; k < len; k++) {
+                        rd.push_back(rp.cw[off + k] & 0xffff);
+                        rd.push_back(rp.cw[off + k] >> 16);
+                    }
+                    if (wrote[d] == r) return false;
+                    wrote[d] = r;
+                    ops.push_back({d, std::move(rd)});
+                }
+                for (auto &op : ops)
+                    for (u32 x : op.second)
+                        if (x != op.first && wrote[x] == r) return false;
+                for (u32 j = 0; j < sg.y; j++) {
+                    const u32 d = rp.cw[sg.x + 4 * j], len = rp.cw[sg.x + 4 * j + 1], off = rp.cw[sg.x + 4 * j + 2];
+                    double acc = M[d];
+                    for (u32 k = 0; k < len; k++) {
+                        const double A = M[rp.cw[off + k] & 0xffff], B = M[rp.cw[off + k] >> 16];
+                        double tt = A * B;
+                        if (solve && B == 0.0) tt = 0.0;
+                        acc = acc - tt;
+                    }
+                    M[d] = acc;
+                }
+                continue;
+            }
             for (u32 th = 0; th < (u32)esl::kWave; th++) {
                 const esl::uint2_t w = rp.words[(size_t)r * esl::kWave + th];
                 const u32 d = w.x & 0xffff;

@@ -163,6 +163,45 @@ __device__ inline double *red_src(uint32_t w, double *tv, double *rhs, double *s
     return ((int32_t)w < 0 ? tb : stc) + (w & 0xfffffu);
 }
 
+// The reduce chains of a chunk: thread k < chains adds its chain's cells
+// to its target in their order, the target in a register, eight cells a
+// block; the sums are the ones the reduce stream would form. A block's
+// indices are read while the block before adds; cells past the chain's end
+// read as -0, which leaves any sum unchanged, so only the adds are serial. The chunk's
+// 16-bit cell indices sit in the pad slots (pw); hd: the thread's header
+// (threads without a chain add nothing to the scratch double), nb: blocks
+// of the longest chain.
+__device__ inline void red_chains(esl::uint2_t hd, uint32_t nb, const uint32_t *pw, double *tv, double *rhs,
+                                  double *stc)
+{
+    const uint32_t len = hd.y & 0xffffu;
+    const __attribute__((address_space(3))) uint32_t *ip = lptr(pw) + (hd.y >> 17);
+    const __attribute__((address_space(3))) double *sv = lptr(stc);
+    double *tgt = red_src(hd.x, tv, rhs, stc);
+    auto idx = [&](uint32_t b, uint32_t (&ws)[4]) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) ws[q] = ip[4 * b + q];
+    };
+    auto vals = [&](uint32_t b, const uint32_t (&ws)[4], double (&v)[kChainBlk]) {
+#pragma unroll
+        for (int e = 0; e < kChainBlk; e++) {
+            const double u = sv[(ws[e >> 1] >> ((e & 1) * 16)) & 0xffffu];
+            v[e] = b * kChainBlk + e < len ? u : -0.0;
+        }
+    };
+    uint32_t w[4];
+    idx(0, w);
+    double acc = *tgt;
+    for (uint32_t b = 0; b < nb; b++) {
+        double v[kChainBlk];
+        vals(b, w, v);
+        if (b + 1 < nb) idx(b + 1, w);
+#pragma unroll
+        for (int e = 0; e < kChainBlk; e++) acc = acc + v[e];
+    }
+    *tgt = acc;
+}
+
 // Assembly into the tape's factor slots: every chunk evaluates, then each
 // thread runs its reduce words (coalesced rows, 16 rows in flight), adding
 // each target's cells in instance order into a register. A crew of nw
@@ -188,6 +227,19 @@ __device__ void assemble(const DeckProg &dp, const TapeProg &T, const double *li
             // not hold up the LDS waits of the reduce (flat loads would)
             const __attribute__((address_space(1))) uint32_t *rs = gptr(T.red) + T.red_off[c] + tid;
             const uint32_t len = T.red_len[c];
+            // the chains' headers and cell indices load now, ahead of use
+            const uint32_t nc = T.rc_count[c];
+            uint32_t tw0 = 0u, tw1 = 0u;
+            esl::uint2_t hd{red_pad(n), 0u};
+            if (nc) {
+                const __attribute__((address_space(1))) uint32_t *cg = gptr(T.rc_cells) + T.rc_coff[c] + tid;
+                tw0 = cg[0];
+                tw1 = cg[kWave];
+                if (tid < nc) {
+                    const __attribute__((address_space(1))) esl::uint2_t &h = gptr(T.rc_head)[T.rc_off[c] + tid];
+                    hd = esl::uint2_t{h.x, h.y};
+                }
+            }
             uint32_t q[kRing];
 #pragma unroll
             for (int k = 0; k < kRing; k++) {
@@ -229,6 +281,14 @@ __device__ void assemble(const DeckProg &dp, const TapeProg &T, const double *li
                 }
             }
             *cur = acc;
+            if (nc) {
+                // the pad slots take the indices (the stream above reads no pad)
+                uint32_t *pw = reinterpret_cast<uint32_t *>(tv + T.pad_slot);
+                pw[tid] = tw0;
+                pw[kWave + tid] = tw1;
+                wsync();
+                red_chains(hd, T.rc_blocks[c], pw, tv, rhs, stc);
+            }
             wsync();
         }
     }
@@ -249,7 +309,7 @@ __device__ bool refactor(const TapeProg &T, uint32_t n, R *tv, uint32_t tid)
     for (uint32_t v = tid; v < T.n_void; v += kWave) f.bad |= tv[T.amap[T.void_slots[v]]] != R(0);
     for (uint32_t k = tid; k < n; k += kWave)
         if (T.flags[k] & kVoidBit) tv[T.n_u + k] = R(1);
-    run_rows<R>(gptr(T.rw), T.rw_rows, tv, T.growth, f, tid, T.rseg, T.rcw);
+    run_rows<R>(gptr(T.rw), T.rw_rows, tv, T.growth, f, tid, T.pad_slot, T.rseg, T.rcw);
     for (uint32_t p = tid; p < T.n_u; p += kWave) {
         const R u = tv[p];
         f.nonfinite |= dbits((double)(u - u));
@@ -271,13 +331,14 @@ __device__ void solve_neg(const TapeProg &T, uint32_t n, double *tv, const doubl
     for (uint32_t r = tid; r < n; r += kWave) y[T.pinv[r]] = -rhs[r];
     wsync();
     LuFlags f{false, 0, 0};
-    run_rows<double, true>(gptr(T.sw), T.sw_rows, tv, 0.0, f, tid, T.sseg, T.scw);
+    run_rows<double, true>(gptr(T.sw), T.sw_rows, tv, 0.0, f, tid, T.pad_slot, T.sseg, T.scw);
 }
 
 // ---- FP32 factors + FP64 refinement of one Newton solve: dx = -A^-1 F.
 // tv32 = [FP32 factor slots | y32]; r64 and dx64 are FP64 vectors; the FP64
-// Jacobian a64 is packed in the residual stream's order (dp.res_w). r = -F -
-// A dx row by row over the pattern in ascending column order. Stops when
+// Jacobian a64 is packed in the residual stream's order (dp.res_w, then the
+// chain rows' entries). r = -F - A dx row by row over the pattern in
+// ascending column order. Stops when
 // max|r| <= refine_tol * max(max|F|, max_i sum_j |A_ij dx_j|); a step that
 // does not halve max|r|, or refine_max steps without success, is a stall.
 // scratch: an LDS double the stream's padding rows read and write. ----
@@ -301,7 +362,7 @@ __device__ bool refine_solve(const OpArgs &a, const TapeProg &T, uint32_t n, flo
         for (uint32_t i = tid; i < n; i += kWave) y32[T.pinv[i]] = (float)r[i];
         wsync();
         LuFlags f{false, 0, 0};
-        run_rows<float, true>(gptr(T.sw), T.sw_rows, tv32, 0.0, f, tid, T.sseg, T.scw);
+        run_rows<float, true>(gptr(T.sw), T.sw_rows, tv32, 0.0, f, tid, T.pad_slot, T.sseg, T.scw);
         wsync();
         for (uint32_t i = tid; i < n; i += kWave) dx[i] = dx[i] + (double)y32[T.qinv[i]];
         wsync();
@@ -365,6 +426,41 @@ __device__ bool refine_solve(const OpArgs &a, const TapeProg &T, uint32_t n, flo
             const uint64_t rb = dbits(fabs(acc)), ab = dbits(aa);
             rm = rb > rm ? rb : rm;
             am = ab > am ? ab : am;
+        }
+        // the chain rows: thread k forms the product of entry k of a round
+        // of 64 and writes it to pad k of the FP64 factor area (above the
+        // FP32 factors), then every thread subtracts the round's products
+        // in column order (padding subtracts +0, which changes nothing)
+        if (dp.n_res_ch) {
+            double *pads = reinterpret_cast<double *>(tv32) + T.pad_slot;
+            const __attribute__((address_space(4))) esl::uint4_t *hs =
+                (const __attribute__((address_space(4))) esl::uint4_t *)dp.res_ch;
+            wsync();
+            for (uint32_t ch = 0; ch < dp.n_res_ch; ch++) {
+                const uint32_t row = hs[ch].x, m = hs[ch].y, cw = hs[ch].z, aw = hs[ch].w;
+                double acc = -rhs[row], aa = 0.0;
+                for (uint32_t r0 = 0; r0 < m; r0 += kWave) {
+                    const uint32_t col = gptr(dp.res_cc)[cw + r0 + tid];
+                    const double g = gptr(a64)[aw + r0 + tid];
+                    const double t = g * dx[col];
+                    pads[tid] = r0 + tid < m ? t : 0.0;
+                    const uint32_t groups = ((m - r0 < (uint32_t)kWave ? m - r0 : (uint32_t)kWave) + 7u) >> 3;
+                    for (uint32_t k8 = 0; k8 < groups; k8++) {
+                        double v[8];
+#pragma unroll
+                        for (int e = 0; e < 8; e++) v[e] = pads[k8 * 8 + e];
+#pragma unroll
+                        for (int e = 0; e < 8; e++) {
+                            acc = acc - v[e];
+                            aa = aa + fabs(v[e]);
+                        }
+                    }
+                }
+                r[row] = acc;
+                const uint64_t rb = dbits(fabs(acc)), ab = dbits(aa);
+                rm = rb > rm ? rb : rm;
+                am = ab > am ? ab : am;
+            }
         }
         wsync();
         const double rmax = bitsd(wave_max_u64(rm));

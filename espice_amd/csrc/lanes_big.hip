@@ -213,7 +213,7 @@ __device__ bool big_refactor(const TapeProg &T, uint32_t n, gdbl *tv, ldbl *M, u
         gcu32 *sl = gptr(T.blk_slots) + blk[b].slot_off;
         gather(M, tv, sl, m, tid);
         wsync();
-        run_rows<double>(gptr(T.blk_words) + (size_t)row0 * kWave, rows, M, T.growth, f, tid);
+        run_rows<double>(gptr(T.blk_words) + (size_t)row0 * kWave, rows, M, T.growth, f, tid, 0u);
         wsync();
         scatter(tv, sl, M, w, tid);
         wsync();
