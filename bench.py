@@ -64,7 +64,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--points", type=int, default=16384)
-    ap.add_argument("--modes", default="fp64_cold,fp64_nominal,fp32_cold,learned_fp32")
+    ap.add_argument("--modes", default="fp64_cold,fp64_nominal,fp32_cold,learned_fp64,learned_fp32")
     ap.add_argument("--train-points", type=int, default=50000)
     ap.add_argument("--train-epochs", type=int, default=30)
     ap.add_argument("--cpu-sample", type=int, default=16384)

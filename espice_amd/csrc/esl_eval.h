@@ -14,19 +14,29 @@ namespace esl {
 __device__ inline uint64_t dbits(double d) { return __builtin_bit_cast(uint64_t, d); }
 __device__ inline double bitsd(uint64_t b) { return __builtin_bit_cast(double, b); }
 
+// Uniform values the compiler cannot see as uniform (a lane reduction, a
+// load from a lane-indexed array) pass through readfirstlane, so branches
+// on them are scalar branches and they live in scalar registers.
+__device__ inline uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ inline uint64_t uni(uint64_t v)
+{
+    return ((uint64_t)uni((uint32_t)(v >> 32)) << 32) | uni((uint32_t)v);
+}
+__device__ inline double uni(double v) { return __builtin_bit_cast(double, uni(__builtin_bit_cast(uint64_t, v))); }
+
 __device__ inline uint64_t wave_max_u64(uint64_t v)
 {
     for (int o = 32; o > 0; o >>= 1) {
         uint64_t w = __shfl_xor(v, o, kWave);
         v = w > v ? w : v;
     }
-    return v;
+    return uni(v);
 }
 
 __device__ inline double wave_fmax(double v)
 {
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
-    return v;
+    return uni(v);
 }
 
 __device__ inline bool wave_any(bool p) { return __any((int)p) != 0; }

@@ -27,6 +27,21 @@
 
 namespace esl {
 
+// A lane's resume record, uniform across its wavefront.
+__device__ inline LaneResume uni_resume(const LaneResume &g)
+{
+    LaneResume r;
+    r.iter = uni(g.iter);
+    r.init_fix = uni(g.init_fix);
+    r.tape = uni(g.tape);
+    r.fresh = uni(g.fresh);
+    r.limit = uni(g.limit);
+    r.k0 = uni(g.k0);
+    r.warm = uni(g.warm);
+    r.f0 = uni(g.f0);
+    return r;
+}
+
 // A lane-per-block lane's crew: wavefront 0 leads the lane; at each
 // assembly round it posts the round's first chunk c0, every wavefront k of
 // the crew evaluates chunk c0 + k into its own staging area, and the leader
@@ -166,8 +181,9 @@ __device__ inline double *red_src(uint32_t w, double *tv, double *rhs, double *s
 // The reduce chains of a chunk: thread k < chains adds its chain's cells
 // to its target in their order, the target in a register, eight cells a
 // block; the sums are the ones the reduce stream would form. A block's
-// indices are read while the block before adds; cells past the chain's end
-// read as -0, which leaves any sum unchanged, so only the adds are serial. The chunk's
+// values are read while the block before adds, its indices a block earlier
+// still; cells past the chain's end read as -0, which leaves any sum
+// unchanged, so only the adds are serial. The chunk's
 // 16-bit cell indices sit in the pad slots (pw); hd: the thread's header
 // (threads without a chain add nothing to the scratch double), nb: blocks
 // of the longest chain.
@@ -190,14 +206,21 @@ __device__ inline void red_chains(esl::uint2_t hd, uint32_t nb, const uint32_t *
         }
     };
     uint32_t w[4];
+    double v[kChainBlk];
     idx(0, w);
+    vals(0, w, v);
+    if (nb > 1) idx(1, w);
     double acc = *tgt;
     for (uint32_t b = 0; b < nb; b++) {
-        double v[kChainBlk];
-        vals(b, w, v);
-        if (b + 1 < nb) idx(b + 1, w);
+        double u[kChainBlk];
 #pragma unroll
-        for (int e = 0; e < kChainBlk; e++) acc = acc + v[e];
+        for (int e = 0; e < kChainBlk; e++) u[e] = v[e];
+        if (b + 1 < nb) {
+            vals(b + 1, w, v);
+            if (b + 2 < nb) idx(b + 2, w);
+        }
+#pragma unroll
+        for (int e = 0; e < kChainBlk; e++) acc = acc + u[e];
     }
     *tgt = acc;
 }
@@ -521,9 +544,9 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
     lane_init(a, lane, live, x, lim, tid);
 
 
-    LaneResume rs = a.resume[lane];
-    const double src = a.src ? a.src[lane] : 1.0;   // the source rung's scale
-    const bool fp32 = a.fp32 && !a.lane64[lane];
+    LaneResume rs = uni_resume(a.resume[lane]);
+    const double src = a.src ? uni(a.src[lane]) : 1.0;   // the source rung's scale
+    const bool fp32 = a.fp32 && !uni((uint32_t)a.lane64[lane]);
     // FP64 lanes re-pivot on the device, on their own LU, when it is there
     const bool can_priv = kOwnLU && a.priv.cap != 0 && !fp32;
     bool priv = kOwnLU && rs.tape == kPrivTape;
@@ -647,7 +670,7 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
             // the lane's own LU: replay it, factor again when a peel test fails
             int rc = kPlOk;
             if (tid == 0) rc = priv_update_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq, tv, a.tv_cap);
-            rc = __shfl(rc, 0, kWave);
+            rc = (int)uni((uint32_t)rc);
             wsync();
             if (rc == kPlSingular) exit_status = kSingular;
             else if (rc == kPlFull) exit_status = kRepivot;
@@ -663,7 +686,7 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
                 wsync();
                 int rc = kPlOk;
                 if (tid == 0) rc = priv_factor_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq, tv, a.tv_cap);
-                rc = __shfl(rc, 0, kWave);
+                rc = (int)uni((uint32_t)rc);
                 for (uint32_t i = tid; i < n; i += kWave) sdiag[i] = pap[dp.diag_slot[i]];
                 wsync();
                 if (rc == kPlOk) {
@@ -781,13 +804,13 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
 __global__ __launch_bounds__(kWave, 2) void op_newton(OpArgs a)
 {
     extern __shared__ double sm[];
-    op_lane<false>(a, a.lane_ids[blockIdx.x], sm, 1);
+    op_lane<false>(a, uni(a.lane_ids[blockIdx.x]), sm, 1);
 }
 
 __global__ __launch_bounds__(kWave) void op_newton_own(OpArgs a)
 {
     extern __shared__ double sm[];
-    op_lane<true>(a, a.lane_ids[blockIdx.x], sm, 1);
+    op_lane<true>(a, uni(a.lane_ids[blockIdx.x]), sm, 1);
 }
 
 // ---- transient lanes ----
@@ -938,7 +961,7 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
     const Crew cr{nw, a.tv_cap + yb + n, a.tv_cap};
     const double *live = a.live + (size_t)lane * dp.n_live;
     double *lim = a.lim + (size_t)lane * lim_width(dp);
-    LaneResume rs = a.resume[lane];
+    LaneResume rs = uni_resume(a.resume[lane]);
     const bool can_priv = kOwnLU && a.priv.cap != 0;
     bool priv = kOwnLU && rs.tape == kPrivTape;
     const TapeProg T = a.tapes[priv ? 0 : rs.tape];   // unused while the lane is on its own LU
@@ -1030,7 +1053,7 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
             bool to_host = false;
             if (priv) {
                 if (tid == 0) rc = priv_update_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq, tv, a.tv_cap);
-                rc = __shfl(rc, 0, kWave);
+                rc = (int)uni((uint32_t)rc);
                 wsync();
                 to_host = rc == kPlFull;
             } else if (refactor<double>(T, n, tv, tid)) {
@@ -1040,7 +1063,7 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
                     for (uint32_t i = tid; i < n; i += kWave) tv[i] = 0.0;   // the factor's dense column (LDS)
                     wsync();
                     if (tid == 0) rc = priv_factor_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq, tv, a.tv_cap);
-                    rc = __shfl(rc, 0, kWave);
+                    rc = (int)uni((uint32_t)rc);
                     wsync();
                     if (rc == kPlOk) {
                         priv = true;
@@ -1302,16 +1325,27 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
 __global__ __launch_bounds__(kWave, 2) void tran_steps(TranArgs ta)
 {
     extern __shared__ double sm[];
-    tran_lane<false>(ta, ta.op.lane_ids[blockIdx.x], sm, 1);
+    tran_lane<false>(ta, uni(ta.op.lane_ids[blockIdx.x]), sm, 1);
 }
 
 __global__ __launch_bounds__(kWave, 2) void tran_steps_own(TranArgs ta)
 {
     extern __shared__ double sm[];
-    tran_lane<true>(ta, ta.op.lane_ids[blockIdx.x], sm, 1);
+    tran_lane<true>(ta, uni(ta.op.lane_ids[blockIdx.x]), sm, 1);
 }
 
 // ---- the persistent batch ----
+
+// A batch job, uniform across its wavefront.
+__device__ inline BatchJob uni_job(const BatchJob &g)
+{
+    BatchJob r;
+    r.op = uni(g.op);
+    r.tran = uni(g.tran);
+    r.lane = uni(g.lane);
+    r.flags = uni(g.flags);
+    return r;
+}
 
 // One job on the lane's leader (nw > 1: with a crew): the operating point,
 // then, for a transient job whose point converged, the hand-over to the
@@ -1355,7 +1389,7 @@ __global__ __launch_bounds__(kWave * kCrew) void batch_run(BatchArgs b)
     const uint32_t g = blockIdx.x;
     // lane per block: the whole workgroup on one lane at a time
     for (uint32_t j = b.block_ptr[g]; j < b.block_ptr[g + 1]; j++) {
-        const BatchJob jb = b.block_jobs[j];
+        const BatchJob jb = uni_job(b.block_jobs[j]);
         if (wv == 0) {
             const uint64_t t0 = wall_clock64();
             run_job(b, jb, sm, kCrew);
@@ -1370,7 +1404,7 @@ __global__ __launch_bounds__(kWave * kCrew) void batch_run(BatchArgs b)
     double *mine = sm + (size_t)wv * b.wave_doubles;
     for (uint32_t j = b.wave_ptr[w]; j < b.wave_ptr[w + 1]; j++) {
         const uint64_t t0 = wall_clock64();
-        run_job(b, b.wave_jobs[j], mine, 1);
+        run_job(b, uni_job(b.wave_jobs[j]), mine, 1);
         if (b.wave_ticks && tid == 0) b.wave_ticks[j] = wall_clock64() - t0;
     }
 }
