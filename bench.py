@@ -348,7 +348,14 @@ def main():
 # Newton iteration limit of the mixed batch's operating points, per family:
 # long inverter chains need hundreds of plain-Newton iterations from a cold
 # start (about two per stage), the other families converge within 100.
+# the uneven batch's operating points (.op items and the transients' initial
+# points) run a plain Newton budget, the reference's itl1 (100,
+# converger.zig:87) except for the long inverter chains, whose switching
+# stages need up to about 1,000 plain iterations (the reference's fourth
+# rung, JFNK, which would take many of them, is not built), and climb the
+# .op ladder on the device when that fails (dc/op.zig:160-288)
 MIXED_OP_MAX_ITER = {"inverter_chain": 1000, "inverter_chain_tran": 1000}
+ORC_LADDER = 1   # the oracle's Tol flag for the .op ladder
 
 
 def run_mesh(args):
@@ -650,11 +657,11 @@ def run_mixed(args):
         vals = fam.sample_at(2024, g["instances"])
         tol = lanes.default_options().tol
         tol.max_iter = MIXED_OP_MAX_ITER.get(g["family"], 100)
-        # .op points the plain Newton leaves unconverged climb the ladder
-        # (dc/op.zig:160-288), as the reference's .op always does
+        # operating points the plain Newton leaves unconverged climb the
+        # ladder (dc/op.zig:160-288), as the reference's .op always does
         j = {"engine": eng, "values": vals, "tol": tol,
              "analysis": lanes.BATCH_OP if g["analysis"] == "op" else lanes.BATCH_TRAN,
-             "ladder": g["analysis"] == "op"}
+             "ladder": True}
         if g["analysis"] == "tran":
             j.update(dt=g["dt"], steps=g["steps"], probes=[eng.n - 1])
         jobs.append(j)
@@ -706,12 +713,12 @@ def run_mixed(args):
             if g["analysis"] == "op":
                 t = j["tol"]
                 ref = c.op(vals, q, threads=1,
-                           tol=orc.Tol(t.reltol, t.abstol, t.vntol, t.residual_tol, t.max_iter, 0))
+                           tol=orc.Tol(t.reltol, t.abstol, t.vntol, t.residual_tol, t.max_iter, ORC_LADDER))
 
             else:
                 t = j["tol"]
                 ref = orc.tran(c, vals, q, g["dt"], g["steps"], threads=1,
-                               tol_op=orc.Tol(t.reltol, t.abstol, t.vntol, t.residual_tol, t.max_iter, 0))
+                               tol_op=orc.Tol(t.reltol, t.abstol, t.vntol, t.residual_tol, t.max_iter, ORC_LADDER))
             dt_t = time.perf_counter() - tc
             t_cpu += dt_t
             est_s += dt_t / k * len(j["values"])   # this template's share of the whole batch
@@ -735,8 +742,10 @@ def run_mixed(args):
         for (g, fam), j in zip(meta, jobs):
             idx = np.arange(0, len(j["values"]), 8)
             t = j["tol"]
+            flags = ORC_LADDER
             for i in idx:
-                tasks.append((g, fam, j, int(i), orc.Tol(t.reltol, t.abstol, t.vntol, t.residual_tol, t.max_iter, 0)))
+                tasks.append((g, fam, j, int(i),
+                              orc.Tol(t.reltol, t.abstol, t.vntol, t.residual_tol, t.max_iter, flags)))
 
         def one(task):
             g, fam, j, i, tol = task

@@ -244,35 +244,41 @@ __device__ inline bool limit_pass(const DeckProg &dp, const double *live, X x, d
 
 // A lane's start (OpArgs modes 0 cold, 1 given x, 2 resume at the saved
 // iterate) and, except on a resume, its limiting state.
+// A cold start: x = 0 and the limiting state's cold seeds (the initial
+// junction and threshold voltages).
+__device__ inline void lane_cold(const DeckProg &dp, const double *live, double *x, double *lim, uint32_t tid)
+{
+    const uint32_t n = dp.n, nm = dp.n_mos;
+    for (uint32_t i = tid; i < n; i += kWave) x[i] = 0.0;
+    for (uint32_t j = tid; j < nm; j += kWave) {
+        const Mos m = mos_load(dp, j, live);
+        lim[j] = m.tp * m.vbit + m.gammax * sqrt(m.phit);
+        lim[nm + j] = 0.0;
+        lim[2 * nm + j] = -1.0;
+        lim[3 * nm + j] = 0.0;
+    }
+    for (uint32_t j = tid; j < dp.n_dio; j += kWave) lim[4 * nm + j] = dio_load(dp, j, live).vcrit;
+    for (uint32_t j = tid; j < dp.n_bjt; j += kWave) {
+        lim[lim_bjt(dp) + 2 * j] = bjt_load(dp, j, live).vcrit;
+        lim[lim_bjt(dp) + 2 * j + 1] = 0.0;
+    }
+    wsync();
+}
+
 __device__ inline void lane_init(const OpArgs &a, uint32_t lane, const double *live, double *x, double *lim,
                                  uint32_t tid)
 {
     const DeckProg &dp = a.dp;
-    const uint32_t n = dp.n, nm = dp.n_mos;
-    const double *xg = a.x + (size_t)lane * n;
+    const uint32_t n = dp.n;
     if (a.mode == 0) {
-        for (uint32_t i = tid; i < n; i += kWave) x[i] = 0.0;
-    } else {
-        for (uint32_t i = tid; i < n; i += kWave) x[i] = xg[i];
+        lane_cold(dp, live, x, lim, tid);
+        return;
     }
+    const double *xg = a.x + (size_t)lane * n;
+    for (uint32_t i = tid; i < n; i += kWave) x[i] = xg[i];
     wsync();
     if (a.mode == 2) return;
-    if (a.mode == 0) {
-        for (uint32_t j = tid; j < nm; j += kWave) {
-            const Mos m = mos_load(dp, j, live);
-            lim[j] = m.tp * m.vbit + m.gammax * sqrt(m.phit);
-            lim[nm + j] = 0.0;
-            lim[2 * nm + j] = -1.0;
-            lim[3 * nm + j] = 0.0;
-        }
-        for (uint32_t j = tid; j < dp.n_dio; j += kWave) lim[4 * nm + j] = dio_load(dp, j, live).vcrit;
-        for (uint32_t j = tid; j < dp.n_bjt; j += kWave) {
-            lim[lim_bjt(dp) + 2 * j] = bjt_load(dp, j, live).vcrit;
-            lim[lim_bjt(dp) + 2 * j + 1] = 0.0;
-        }
-    } else {
-        seed_lim_at(dp, live, x, lim, tid);
-    }
+    seed_lim_at(dp, live, x, lim, tid);
     wsync();
 }
 

@@ -299,6 +299,26 @@ struct PrivLU {
 constexpr uint32_t kPrivTape = 0xfffffffeu;   // LaneResume::tape of a lane on its own LU
 
 // Arguments of the Newton kernel (op_newton).
+// A lane's .op ladder (dc/op.zig:160-288): the gmin rung (g stepped down
+// from kLadGminStart / 10 to the target by a factor that grows after easy
+// runs and shrinks after hard or failed ones, failed runs restarting from
+// the last converged point; then a clean run without the shunts), the
+// source rung (lambda from 0 by a step that grows 1.5x and halves on
+// failure) and a final run at full sources. Runs of the stepping rungs have
+// at most kLadItl2 iterations, the clean and final runs the plain budget.
+enum LadderPhase : uint32_t { kLadPlain = 0, kLadGmin, kLadGminClean, kLadSrc, kLadFinal, kLadDone };
+constexpr double kLadGminTarget = 1e-12, kLadGminStart = 1e-2;
+constexpr uint32_t kLadItl2 = 50;
+// flags: a good point saved, the next gmin run starts cold, and the last
+// run's factor met a singular matrix (the next run factors afresh instead of
+// replaying, as direct.zig:268-298 leaves no factorization after one)
+constexpr uint32_t kLadHaveGood = 1, kLadCold = 2, kLadNeedFactor = 4;
+struct LadderState {
+    double factor, good, gv;            // gmin rung
+    double lambda, lambda_good, delta;  // source rung
+    uint32_t phase, total, solves, flags;
+};
+
 struct OpArgs {
     DeckProg dp;
     const TapeProg *tapes;
@@ -339,6 +359,13 @@ struct OpArgs {
     // optional [N]: the .op ladder's source rung, every independent source
     // scaled by src[lane] in the device evaluation (Jacobian dumps included)
     const double *src;
+    // optional: the .op ladder on the device (FP64 lanes). A lane whose
+    // plain run ends unconverged or singular climbs the gmin rung, the
+    // source rung and the final run of dc/op.zig:160-288 in the same launch,
+    // its control state in lad[lane] across exits and resumes, its last
+    // converged point in xgood [N][n]
+    LadderState *lad;
+    double *xgood;
 };
 constexpr int kProfPhases = 4;
 

@@ -397,10 +397,11 @@ __device__ __forceinline__ int priv_factor_at(const PrivLU &P, uint32_t lane, co
 // factor: kPlOk, kPlSingular or kPlFull.
 __device__ __forceinline__ int priv_update_at(const PrivLU &P, uint32_t lane, const uint32_t *col_ptr,
                                               const uint32_t *row_idx, const uint32_t *q, double *lds,
-                                              uint32_t lds_doubles)
+                                              uint32_t lds_doubles, bool fresh = false)
 {
+    // fresh: the last factor met a singular matrix and left no usable LU
     const PrivLane L = priv_lane(P, lane, lds, lds_doubles);
-    if (!priv_refactor(L, col_ptr, q, 1e-12, 1.0 / 1e-3)) return kPlOk;
+    if (!fresh && !priv_refactor(L, col_ptr, q, 1e-12, 1.0 / 1e-3)) return kPlOk;
     return priv_factor(L, col_ptr, row_idx, q, 1e-3);
 }
 

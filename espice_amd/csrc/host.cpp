@@ -1652,7 +1652,7 @@ struct Engine {
     DevBuf probe_ids, probe_out;
     DevBuf tq1, tip, txp, tstep, tprobes;   // transient state
     DevBuf tq2, tq3, txc, ttimes, tadapt, tbps;   // adaptive transient state
-    DevBuf gminv, srcv, xgood;               // the .op ladder: per-lane g and source scale, last converged x
+    DevBuf lad, xgood;                       // the .op ladder on the device: per-lane control, last converged x
     DevBuf pl_u32, pl_f64;                  // the lanes' own LU (device re-pivot)
     bool has_nominal = false;
     // lane-per-thread state, lane-major
@@ -3103,6 +3103,8 @@ esl_status op_repivot(Engine &E, const HostOpArgs &a, size_t count, vector<u32> 
             if (!gp_factor(n, c.col_ptr.data(), c.row_idx.data(), A.data() + k * nnz, c.q.data(), 1e-3, t)) {
                 st[l] = esl::kSingular;
                 its[l] = rs[l].iter + 1;
+                // a lane on the device ladder goes on with its next run
+                if (a.lad) next.push_back(l);
                 continue;
             }
             u32 id;
@@ -3126,190 +3128,6 @@ esl_status op_repivot(Engine &E, const HostOpArgs &a, size_t count, vector<u32> 
         HIPCHK(hipMemcpyAsync(st.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
     }
-    return ESL_OK;
-}
-
-// The .op ladder for the lanes the plain rung left unconverged or singular
-// (dc/op.zig:160-288, the oracle's op_ladder): the gmin rung (Newton runs
-// with per-lane shunts g on the node diagonals, g stepped down from 1e-3 to
-// 1e-12, then a clean run), then the source rung (runs with every
-// independent source scaled by a per-lane lambda stepped from 0 to 1), then
-// a final run at full sources. Every run is a launch of the plain kernels
-// over the lanes at that point of their ladder (re-pivots as in the plain
-// rung); the control is here. A round launches three groups: cold runs and
-// runs from the last point (both at most itl2 = 50 iterations), and clean
-// and final runs (max_iter). st and the device's status and iterations are
-// final on return (iterations summed over the plain rung and every run).
-esl_status op_ladder(Engine &E, HostOpArgs a, const esl::PrivLU &own, size_t count, const esl_op_options &o,
-                     vector<u32> &st, vector<u32> &its, vector<uint8_t> &lane64, float &kernel_ms)
-{
-    Engine *e = &E;
-    const u32 n = e->c.n;
-    hipStream_t s = e->stream;
-    constexpr double kTarget = 1e-12, kStart = 1e-2;
-    constexpr u32 kItl2 = 50;
-    enum : int { kGminRung, kGminClean, kSrcRung, kFinal, kDone };
-    struct Lad {
-        double factor, good, gv;            // gmin rung
-        double lambda, lambda_good, delta;  // source rung
-        bool have_good, cold;
-        u32 total, solves;
-        int phase;
-    };
-    vector<u32> dev_its(count);
-    HIPCHK(hipMemcpyAsync(dev_its.data(), e->iters.p, count * 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    vector<Lad> lad(count);
-    vector<u32> ladder;
-    for (size_t l = 0; l < count; l++) {
-        if (st[l] != esl::kNoConvergence && st[l] != esl::kSingular) continue;
-        const u32 plain = (st[l] == esl::kSingular && its[l]) ? its[l] : dev_its[l];
-        lad[l] = Lad{10.0, kStart, kStart / 10.0, 0.0, -1.0, 0.25, false, true, plain, 0, kGminRung};
-        ladder.push_back((u32)l);
-    }
-    if (ladder.empty()) return ESL_OK;
-    HIPCHK(e->gminv.ensure(count * 8));
-    HIPCHK(e->srcv.ensure(count * 8));
-    HIPCHK(e->xgood.ensure(count * (size_t)n * 8));
-    vector<double> gh(count, 0.0), sh(count, 1.0);
-    double *xd = e->x.as<double>(), *xg = e->xgood.as<double>();
-    auto save_good = [&](u32 l) -> hipError_t {
-        return hipMemcpyAsync(xg + (size_t)l * n, xd + (size_t)l * n, n * 8, hipMemcpyDeviceToDevice, s);
-    };
-    auto back_to_good = [&](u32 l) -> hipError_t {
-        return hipMemcpyAsync(xd + (size_t)l * n, xg + (size_t)l * n, n * 8, hipMemcpyDeviceToDevice, s);
-    };
-    // one launch group: Newton runs from mode's start, then host re-pivot
-    // rounds for lanes that left for one
-    auto run = [&](const vector<u32> &ls, u32 mode, u32 max_iter) -> esl_status {
-        if (ls.empty()) return ESL_OK;
-        for (u32 l : ls) its[l] = 0;   // set again by a host re-pivot that finds the lane singular
-        HIPCHK(hipMemcpyAsync(e->lane_ids.p, ls.data(), ls.size() * 4, hipMemcpyHostToDevice, s));
-        HostOpArgs r = a;
-        r.max_iter = max_iter;
-        r.gmin = e->gminv.as<double>();
-        r.src = e->srcv.as<double>();
-        r.priv = own;
-        r.keep_tape = 1;
-        u32 cap = 0;
-        for (auto *t : e->tapes) cap = std::max(cap, t->prog.n_tv);
-        float ms = 0;
-        esl_status rc = launch_lpw(*e, r, (u32)ls.size(), mode, cap, ms);
-        if (rc != ESL_OK) return rc;
-        kernel_ms += ms;
-        e->timing.launches++;
-        HIPCHK(hipMemcpyAsync(st.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        rc = op_repivot(*e, r, count, st, its, lane64, kernel_ms);
-        if (rc != ESL_OK) return rc;
-        HIPCHK(hipMemcpyAsync(dev_its.data(), e->iters.p, count * 4, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(st.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        return ESL_OK;
-    };
-    for (;;) {
-        vector<u32> cold, warm, full;
-        for (u32 l : ladder) {
-            const Lad &d = lad[l];
-            gh[l] = d.phase == kGminRung ? d.gv : 0.0;
-            sh[l] = d.phase == kSrcRung ? d.lambda : 1.0;
-            if (d.phase == kGminRung) (d.cold ? cold : warm).push_back(l);
-            else if (d.phase == kSrcRung) (d.lambda_good < 0.0 ? cold : warm).push_back(l);
-            else if (d.phase == kGminClean || d.phase == kFinal) full.push_back(l);
-        }
-        if (cold.empty() && warm.empty() && full.empty()) break;
-        HIPCHK(hipMemcpyAsync(e->gminv.p, gh.data(), count * 8, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(e->srcv.p, sh.data(), count * 8, hipMemcpyHostToDevice, s));
-        esl_status rc = run(cold, 0, kItl2);
-        if (rc == ESL_OK) rc = run(warm, 1, kItl2);
-        if (rc == ESL_OK) rc = run(full, 1, o.tol.max_iter);
-        if (rc != ESL_OK) return rc;
-        for (const vector<u32> *g : {&cold, &warm, &full})
-            for (u32 l : *g) {
-                Lad &d = lad[l];
-                const u32 it = st[l] == esl::kSingular && its[l] ? its[l] : dev_its[l];
-                const bool ok = st[l] == esl::kConverged;
-                d.total += it;
-                switch (d.phase) {
-                case kGminRung:
-                    d.solves++;
-                    if (ok) {
-                        if (d.gv <= kTarget) {
-                            d.phase = kGminClean;   // the clean run, from this point
-                            break;
-                        }
-                        HIPCHK(save_good(l));
-                        d.have_good = true;
-                        d.good = d.gv;
-                        if (it <= kItl2 / 4) d.factor = std::min(d.factor * std::sqrt(d.factor), 10.0);
-                        else if (it > 3 * kItl2 / 4) d.factor = std::max(std::sqrt(d.factor), 1.00005);
-                        if (d.gv < d.factor * kTarget) {
-                            d.factor = d.gv / kTarget;
-                            d.gv = kTarget;
-                        } else {
-                            d.gv /= d.factor;
-                        }
-                        d.cold = false;
-                    } else {
-                        if (d.factor < 1.00005) {
-                            d.phase = kSrcRung;
-                            d.solves = 0;
-                            break;
-                        }
-                        d.factor = std::sqrt(std::sqrt(d.factor));
-                        d.gv = d.good / d.factor;
-                        if (d.have_good) HIPCHK(back_to_good(l));
-                        d.cold = !d.have_good;
-                    }
-                    if (d.phase == kGminRung && d.solves >= 100) {
-                        d.phase = kSrcRung;
-                        d.solves = 0;
-                    }
-                    break;
-                case kGminClean:
-                    if (ok) d.phase = kDone;
-                    else {
-                        d.phase = kSrcRung;
-                        d.solves = 0;
-                    }
-                    break;
-                case kSrcRung:
-                    d.solves++;
-                    if (ok) {
-                        if (d.lambda >= 1.0) {
-                            d.phase = kFinal;
-                            break;
-                        }
-                        d.lambda_good = d.lambda;
-                        HIPCHK(save_good(l));
-                        d.delta *= 1.5;
-                        d.lambda = std::min(d.lambda + d.delta, 1.0);
-                    } else {
-                        d.delta *= 0.5;
-                        if (d.delta < 1e-4 || d.lambda_good < 0.0) {
-                            d.phase = kFinal;   // from where the last run stopped
-                            break;
-                        }
-                        HIPCHK(back_to_good(l));
-                        d.lambda = std::min(d.lambda_good + d.delta, 1.0);
-                    }
-                    if (d.solves >= 100) d.phase = kFinal;
-                    break;
-                case kFinal:
-                    d.phase = kDone;
-                    if (!ok) st[l] = esl::kNoConvergence;
-                    break;
-                }
-            }
-    }
-    // the ladder's lanes: final status and the iterations of all their runs
-    for (u32 l : ladder) {
-        dev_its[l] = lad[l].total;
-        its[l] = lad[l].total;
-    }
-    HIPCHK(hipMemcpyAsync(e->iters.p, dev_its.data(), count * 4, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(e->status.p, st.data(), count * 4, hipMemcpyHostToDevice, s));
-    HIPCHK(hipStreamSynchronize(s));
     return ESL_OK;
 }
 
@@ -3384,8 +3202,16 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
     // (kernel time of the first launch over every lane) and keeps the faster;
     // below that size a batch runs one wavefront per lane.
     // the thread tiling runs FP64 lanes from cold or given starts
-    const bool wave_only = fp32 || net;
+    // the .op ladder runs on the device in the wavefront kernels (FP64 lanes)
+    const bool ladder = (o.flags & ESL_OP_LADDER) && !fp32 && !e->big;
+    const bool wave_only = fp32 || net || ladder;
     if (wave_only && o.tiling == ESL_TILING_THREAD) return ESL_INVALID_ARGUMENT;
+    if (ladder) {
+        HIPCHK(e->lad.ensure(count * sizeof(esl::LadderState)));
+        HIPCHK(e->xgood.ensure(count * (size_t)n * 8));
+        a.lad = e->lad.as<esl::LadderState>();
+        a.xgood = e->xgood.as<double>();
+    }
     u32 tiling = wave_only ? (u32)ESL_TILING_WAVE : o.tiling;
     if (e->big) {
         // lanes beyond a workgroup's LDS: FP64 from a cold or given start,
@@ -3439,8 +3265,6 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
             for (auto *t : e->tapes) cap = std::max(cap, t->prog.n_tv);
             HostOpArgs ao = a;
             ao.priv = own;
-            // the ladder continues each lane on the LU it ends on here
-            ao.keep_tape = (o.flags & ESL_OP_LADDER) ? 1u : 0u;
             esl_status r = launch_lpw(*e, ao, (u32)again.size(), 2, cap, ms);
             if (r != ESL_OK) return r;
             kernel_ms += ms;
@@ -3452,10 +3276,6 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
     }
     {
         esl_status r = op_repivot(*e, a, count, st, its, lane64, kernel_ms);
-        if (r != ESL_OK) return r;
-    }
-    if ((o.flags & ESL_OP_LADDER) && !fp32 && !net && !e->big) {
-        esl_status r = op_ladder(*e, a, own, count, o, st, its, lane64, kernel_ms);
         if (r != ESL_OK) return r;
     }
     const auto t_dl0 = std::chrono::steady_clock::now();
@@ -4258,7 +4078,6 @@ esl_status validate_batch_item(const esl_batch_item &it, Engine *&e)
     if (it.tol.max_iter == 0) return ESL_INVALID_ARGUMENT;
     if (it.flags & ~(u32)ESL_OP_LADDER) return ESL_INVALID_ARGUMENT;
     if (it.analysis == ESL_BATCH_OP) return ESL_OK;
-    if (it.flags) return ESL_INVALID_ARGUMENT;   // the ladder is for .op items
     if (it.analysis != ESL_BATCH_TRAN || !it.tran) return ESL_INVALID_ARGUMENT;
     const esl_tran_options &o = *it.tran;
     if (o.struct_size < sizeof(esl_tran_options)) return ESL_ABI_MISMATCH;
@@ -4294,6 +4113,13 @@ esl_status batch_item_setup(BatchItemRun &r)
     r.a.mode = 0;
     r.a.dump = 0;
     r.a.tv_cap = r.op_cap;
+    if (it.flags & ESL_OP_LADDER) {
+        // the .op ladder on the device, in the operating point's job
+        HIPCHK(e->lad.ensure(count * sizeof(esl::LadderState)));
+        HIPCHK(e->xgood.ensure(count * (size_t)n * 8));
+        r.a.lad = e->lad.as<esl::LadderState>();
+        r.a.xgood = e->xgood.as<double>();
+    }
     HIPCHK(e->a_out.ensure(std::max<size_t>(1, count * (size_t)nnz) * 8));
     if (!r.tran) return ESL_OK;
 
@@ -4922,37 +4748,6 @@ extern "C" esl_status esl_batch_solve(const esl_batch_item *items, size_t n_item
         }
         esl_status st = batch_item_results(r);
         if (st != ESL_OK) return st;
-    }
-    // 8. the .op ladder: an item's points that the plain Newton left
-    // unconverged or singular are solved again by esl_op_solve with the
-    // ladder (its plain rung, then the gmin and source rungs); its result is
-    // theirs and their iterations add both solves
-    for (auto &r : runs) {
-        const esl_batch_item &it = *r.it;
-        if (!(it.flags & ESL_OP_LADDER) || it.analysis != ESL_BATCH_OP) continue;
-        vector<size_t> bad;
-        for (size_t l = 0; l < it.count; l++)
-            if (it.reasons[l] == ESL_REASON_NO_CONVERGENCE || it.reasons[l] == ESL_REASON_SINGULAR_MATRIX)
-                bad.push_back(l);
-        if (bad.empty()) continue;
-        const u32 n = r.e->c.n, nl = r.e->c.n_live;
-        vector<double> v(bad.size() * std::max<u32>(nl, 1)), x(bad.size() * n);
-        vector<u32> its(bad.size()), rs(bad.size());
-        for (size_t k = 0; k < bad.size(); k++)
-            std::copy(it.values + bad[k] * nl, it.values + (bad[k] + 1) * nl, v.begin() + k * nl);
-        esl_op_options oo;
-        esl_default_op_options(&oo);
-        oo.tol = it.tol;
-        oo.tiling = ESL_TILING_WAVE;
-        oo.flags = ESL_OP_LADDER;
-        esl_status st = esl_op_solve(it.engine, v.data(), bad.size(), &oo, x.data(), its.data(), rs.data());
-        if (st != ESL_OK) return st;
-        for (size_t k = 0; k < bad.size(); k++) {
-            const size_t l = bad[k];
-            std::copy(x.begin() + k * n, x.begin() + (k + 1) * n, it.x + l * n);
-            it.reasons[l] = rs[k];
-            it.iterations[l] += its[k];
-        }
     }
     if (report) {
         const size_t ss = report->struct_size;

@@ -151,17 +151,15 @@ __device__ void assemble_pattern(const DeckProg &dp, const double *live, const d
     wsync();
 }
 
-// The gmin rung's shunts (OpArgs::gmin) on an assembled Jacobian M whose
+// The gmin rung's shunts (conductance g) on an assembled Jacobian M whose
 // row-i diagonal is M[diag[i]]: A_ii += g, F_i += g x_i on every node row
 // (not row 0, the ground pin, nor branch rows), as the oracle's newton_run.
-__device__ inline void add_gmin(const OpArgs &a, uint32_t lane, const uint32_t *diag, double *M, double *rhs,
+__device__ inline void add_gmin(double g, const DeckProg &dp, const uint32_t *diag, double *M, double *rhs,
                                 const double *x, uint32_t tid)
 {
-    if (!a.gmin) return;
-    const double g = a.gmin[lane];
     if (!(g > 0)) return;
-    for (uint32_t i = 1 + tid; i < a.dp.n; i += kWave)
-        if (!a.dp.current_row[i]) {
+    for (uint32_t i = 1 + tid; i < dp.n; i += kWave)
+        if (!dp.current_row[i]) {
             M[diag[i]] = M[diag[i]] + g;
             rhs[i] = rhs[i] + g * x[i];
         }
@@ -495,6 +493,147 @@ __device__ bool refine_solve(const OpArgs &a, const TapeProg &T, uint32_t n, flo
     return false;
 }
 
+// ---- the .op ladder's control on the device (dc/op.zig:160-288, the
+// oracle's op_ladder): the state of a lane between its runs ----
+__device__ inline LadderState uni_ladder(const LadderState &g)
+{
+    LadderState r;
+    r.factor = uni(g.factor);
+    r.good = uni(g.good);
+    r.gv = uni(g.gv);
+    r.lambda = uni(g.lambda);
+    r.lambda_good = uni(g.lambda_good);
+    r.delta = uni(g.delta);
+    r.phase = uni(g.phase);
+    r.total = uni(g.total);
+    r.solves = uni(g.solves);
+    r.flags = uni(g.flags);
+    return r;
+}
+
+// The run a phase makes: budget, the gmin rung's shunt and the source scale.
+__device__ inline void ladder_run(const LadderState &L, uint32_t max_iter, double &g, double &lam, uint32_t &budget)
+{
+    if (L.phase == kLadPlain) return;
+    g = L.phase == kLadGmin ? L.gv : 0.0;
+    lam = L.phase == kLadSrc ? L.lambda : 1.0;
+    budget = (L.phase == kLadGmin || L.phase == kLadSrc) ? kLadItl2 : max_iter;
+}
+
+// What follows a run that ended with status and its iterations: flags of
+// kLadEnd (the lane's result is status, its iterations L.total for a lane
+// that climbed), kLadSaveGood (the run's point becomes the last good one),
+// kLadFromCold / kLadFromGood (the next run's start; neither: from the
+// run's point). status becomes the final run's outcome.
+constexpr uint32_t kLadEnd = 1, kLadSaveGood = 2, kLadFromCold = 4, kLadFromGood = 8;
+__device__ inline uint32_t ladder_next(LadderState &L, uint32_t &status, uint32_t &iters)
+{
+    const bool ok = status == kConverged;
+    if (L.phase == kLadPlain) {
+        if (ok) return kLadEnd;
+        L.factor = 10.0;
+        L.good = kLadGminStart;
+        L.gv = kLadGminStart / 10.0;
+        L.lambda = 0.0;
+        L.lambda_good = -1.0;
+        L.delta = 0.25;
+        L.phase = kLadGmin;
+        L.total = iters;
+        L.solves = 0;
+        L.flags = kLadCold | (status == kSingular ? kLadNeedFactor : 0u);
+        return kLadFromCold;
+    }
+    L.total += iters;
+    uint32_t act = 0;
+    switch (L.phase) {
+    case kLadGmin:
+        L.solves++;
+        if (ok) {
+            if (L.gv <= kLadGminTarget) {
+                L.phase = kLadGminClean;   // the clean run, from this point
+                break;
+            }
+            act |= kLadSaveGood;
+            L.flags |= kLadHaveGood;
+            L.good = L.gv;
+            if (iters <= kLadItl2 / 4) L.factor = fmin(L.factor * sqrt(L.factor), 10.0);
+            else if (iters > 3 * kLadItl2 / 4) L.factor = fmax(sqrt(L.factor), 1.00005);
+            if (L.gv < L.factor * kLadGminTarget) {
+                L.factor = L.gv / kLadGminTarget;
+                L.gv = kLadGminTarget;
+            } else {
+                L.gv = L.gv / L.factor;
+            }
+            L.flags &= ~kLadCold;
+        } else {
+            if (L.factor < 1.00005) {
+                L.phase = kLadSrc;
+                L.solves = 0;
+                break;
+            }
+            L.factor = sqrt(sqrt(L.factor));
+            L.gv = L.good / L.factor;
+            if (L.flags & kLadHaveGood) {
+                act |= kLadFromGood;
+                L.flags &= ~kLadCold;
+            } else {
+                L.flags |= kLadCold;
+            }
+        }
+        if (L.phase == kLadGmin && L.solves >= 100) {
+            L.phase = kLadSrc;
+            L.solves = 0;
+        }
+        break;
+    case kLadGminClean:
+        if (ok) {
+            L.phase = kLadDone;
+        } else {
+            L.phase = kLadSrc;
+            L.solves = 0;
+        }
+        break;
+    case kLadSrc:
+        L.solves++;
+        if (ok) {
+            if (L.lambda >= 1.0) {
+                L.phase = kLadFinal;
+                break;
+            }
+            L.lambda_good = L.lambda;
+            act |= kLadSaveGood;
+            L.delta = L.delta * 1.5;
+            L.lambda = fmin(L.lambda + L.delta, 1.0);
+        } else {
+            L.delta = L.delta * 0.5;
+            if (L.delta < 1e-4 || L.lambda_good < 0.0) {
+                L.phase = kLadFinal;   // from where the last run stopped
+                break;
+            }
+            act |= kLadFromGood;
+            L.lambda = fmin(L.lambda_good + L.delta, 1.0);
+        }
+        if (L.solves >= 100) L.phase = kLadFinal;
+        break;
+    case kLadFinal:
+        L.phase = kLadDone;
+        if (!ok) status = kNoConvergence;
+        break;
+    default:
+        break;
+    }
+    if (L.phase == kLadDone) {
+        iters = L.total;
+        return act | kLadEnd;
+    }
+    if (status == kSingular) L.flags |= kLadNeedFactor;
+    // a run of the gmin rung starts cold until one converged; a source-rung
+    // run starts cold until a lambda converged
+    if ((L.phase == kLadGmin && (L.flags & kLadCold)) || (L.phase == kLadSrc && L.lambda_good < 0.0))
+        act |= kLadFromCold;
+    return act;
+}
+
 // A lane's Jacobian (pattern order) and residual at its start or saved
 // iterate: the lead point's matrix for the host factor, or a failing lane's
 // for its re-pivot.
@@ -512,8 +651,12 @@ __global__ __launch_bounds__(kWave) void op_dump(OpArgs a)
     const double *live = a.live + (size_t)lane * dp.n_live;
     double *lim = a.lim + (size_t)lane * lim_width(dp);
     lane_init(a, lane, live, x, lim, tid);
-    assemble_pattern(dp, live, x, lim, nm, tv, rhs, st, tid, a.src ? a.src[lane] : 1.0);
-    add_gmin(a, lane, dp.diag_slot, tv, rhs, x, tid);
+    // a lane on the device ladder: its run's shunt and source scale
+    double g = a.gmin ? a.gmin[lane] : 0.0, lam = a.src ? a.src[lane] : 1.0;
+    uint32_t budget = a.max_iter;
+    if (a.lad) ladder_run(uni_ladder(a.lad[lane]), a.max_iter, g, lam, budget);
+    assemble_pattern(dp, live, x, lim, nm, tv, rhs, st, tid, lam);
+    add_gmin(g, dp, dp.diag_slot, tv, rhs, x, tid);
     for (uint32_t p = tid; p < dp.nnz; p += kWave) a.a_out[(size_t)blockIdx.x * dp.nnz + p] = tv[p];
     for (uint32_t i = tid; i < n; i += kWave) a.f_out[(size_t)blockIdx.x * n + i] = rhs[i];
 }
@@ -589,7 +732,21 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
         k0 = at;
         warm = 2;
     };
-    uint32_t status = kNoConvergence, iters = a.max_iter;
+    // the .op ladder (FP64 lanes, when on): a run's budget, shunt and source
+    // scale follow the ladder's phase; the plain run is the first
+    const bool ladder = a.lad != nullptr && !fp32;
+    LadderState L{};
+    double g_run = a.gmin ? uni(a.gmin[lane]) : 0.0, lam = src;
+    uint32_t budget = a.max_iter;
+    if (ladder) {
+        if (a.mode == 2) L = uni_ladder(a.lad[lane]);
+        else L.phase = kLadPlain;
+        ladder_run(L, a.max_iter, g_run, lam, budget);
+    }
+    uint32_t status = kNoConvergence, iters = budget;
+    // a lane the host found singular in its re-pivot resumes only to go on
+    // with its ladder: its run ended singular at the saved iterate
+    bool host_singular = ladder && a.mode == 2 && uni(a.status[lane]) == kSingular;
     unsigned long long pc[kProfPhases] = {0, 0, 0, 0};
     unsigned long long tc = clock64();
     auto lap = [&](int ph) {
@@ -600,18 +757,25 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
         }
     };
     uint32_t iter = iter0;
+    for (;;) {   // the runs of the lane's ladder (one run without it)
     for (;; iter++) {
-        if (iter - k0 >= a.max_iter) {
+        if (host_singular) {
+            host_singular = false;
+            status = kSingular;
+            iters = iter + 1;
+            break;
+        }
+        if (iter - k0 >= budget) {
             if (warm != 1) break;
             restart_nominal(iter);
         }
         lap(3);
         if (priv) {
-            assemble_pattern(dp, live, x, lim, nm, pap, rhs, st, tid, src);
-            add_gmin(a, lane, dp.diag_slot, pap, rhs, x, tid);
+            assemble_pattern(dp, live, x, lim, nm, pap, rhs, st, tid, lam);
+            add_gmin(g_run, dp, dp.diag_slot, pap, rhs, x, tid);
         } else {
-            assemble(dp, T, live, x, lim, nm, tv, rhs, st, tid, cr, src);
-            add_gmin(a, lane, T.diag_tv, tv, rhs, x, tid);
+            assemble(dp, T, live, x, lim, nm, tv, rhs, st, tid, cr, lam);
+            add_gmin(g_run, dp, T.diag_tv, tv, rhs, x, tid);
         }
         if (warm == 1) {
             uint64_t fb = 0;
@@ -669,19 +833,23 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
         } else if (priv) {
             // the lane's own LU: replay it, factor again when a peel test fails
             int rc = kPlOk;
-            if (tid == 0) rc = priv_update_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq, tv, a.tv_cap);
+            const bool fresh = (L.flags & kLadNeedFactor) != 0;
+            L.flags &= ~kLadNeedFactor;
+            if (tid == 0) rc = priv_update_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq, tv, a.tv_cap, fresh);
             rc = (int)uni((uint32_t)rc);
             wsync();
             if (rc == kPlSingular) exit_status = kSingular;
             else if (rc == kPlFull) exit_status = kRepivot;
-        } else if (refactor<double>(T, n, tv, tid)) {
+        } else if ((L.flags & kLadNeedFactor) || refactor<double>(T, n, tv, tid)) {
+            // (a lane whose last factor met a singular matrix factors afresh)
+            L.flags &= ~kLadNeedFactor;
             exit_status = kRepivot;
             if (can_priv) {
                 // re-pivot here: the Jacobian again in pattern order (the same
                 // sums as the failed assembly), factored by the lane
                 wsync();
-                assemble_pattern(dp, live, x, lim, nm, pap, rhs, st, tid, src);
-                add_gmin(a, lane, dp.diag_slot, pap, rhs, x, tid);
+                assemble_pattern(dp, live, x, lim, nm, pap, rhs, st, tid, lam);
+                add_gmin(g_run, dp, dp.diag_slot, pap, rhs, x, tid);
                 for (uint32_t i = tid; i < n; i += kWave) tv[i] = 0.0;   // the factor's dense column (LDS)
                 wsync();
                 int rc = kPlOk;
@@ -724,8 +892,8 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
                 // re-pivot; the device evaluation reads x and the limiting
                 // state exactly as the failed assembly did
                 wsync();
-                assemble_pattern(dp, live, x, lim, nm, a.a_out + (size_t)lane * dp.nnz, rhs, st, tid, src);
-                add_gmin(a, lane, dp.diag_slot, a.a_out + (size_t)lane * dp.nnz, rhs, x, tid);
+                assemble_pattern(dp, live, x, lim, nm, a.a_out + (size_t)lane * dp.nnz, rhs, st, tid, lam);
+                add_gmin(g_run, dp, dp.diag_slot, a.a_out + (size_t)lane * dp.nnz, rhs, x, tid);
             }
             status = exit_status;
             iters = iter;
@@ -783,6 +951,34 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
         wsync();
     }
     if (status == kNoConvergence) iters = iter;
+    if (!ladder || (status != kConverged && status != kNoConvergence && status != kSingular)) break;
+    // the run ended: the ladder's next run, or the lane's end
+    wsync();
+    const uint32_t act = ladder_next(L, status, iters);
+    if (act & kLadSaveGood)
+        for (uint32_t i = tid; i < n; i += kWave) a.xgood[(size_t)lane * n + i] = x[i];
+    if (act & kLadEnd) break;
+    if (act & kLadFromGood) {
+        wsync();
+        for (uint32_t i = tid; i < n; i += kWave) x[i] = a.xgood[(size_t)lane * n + i];
+    }
+    wsync();
+    if (act & kLadFromCold) {
+        lane_cold(dp, live, x, lim, tid);
+    } else {
+        seed_lim_at(dp, live, x, lim, tid);
+        wsync();
+    }
+    ladder_run(L, a.max_iter, g_run, lam, budget);
+    status = kNoConvergence;
+    iters = budget;
+    iter = 0;
+    k0 = 0;
+    warm = 0;
+    init_fix = (act & kLadFromCold) != 0;
+    limit = 1e9;
+    f0 = 0.0;
+    }
     lap(3);
     wsync();
     if (a.fell_back && tid == 0) a.fell_back[lane] = warm == 2 ? 1 : 0;
@@ -791,6 +987,7 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
     if (tid == 0) {
         a.status[lane] = status;
         a.iterations[lane] = iters;
+        if (ladder) a.lad[lane] = L;   // a lane that leaves mid-ladder resumes its rung
         if (a.keep_tape && status != kRepivot && status != kToFp64) a.resume[lane].tape = rs.tape;
         if (a.prof)
             for (int p = 0; p < kProfPhases; p++) a.prof[(size_t)lane * kProfPhases + p] += pc[p];

@@ -195,7 +195,9 @@ typedef struct {
      * independent source scaled by lambda, lambda from 0 to 1 in steps that
      * grow 1.5x after a converged run and halve after a failed one) and a
      * final run at full sources. The point's iterations are the sum over
-     * its runs. */
+     * its runs. Each lane climbs its own ladder inside the same launch (the
+     * rung control runs on the device), so hard points cost only their own
+     * runs; lanes fitting a workgroup's LDS (wavefront tiling). */
     uint32_t flags;
 } esl_op_options;
 enum { ESL_OP_HOST_REPIVOT = 1u, ESL_OP_LADDER = 2u };
@@ -367,12 +369,13 @@ typedef struct {
     const double *values;      /* [count][n_live] */
     size_t count;
     uint32_t analysis;         /* ESL_BATCH_* */
-    uint32_t flags;            /* ESL_BATCH_OP: ESL_OP_LADDER sends the points the
-                                  batch's plain Newton leaves unconverged or singular
-                                  up the .op ladder (dc/op.zig:160-288): they are
-                                  solved again by esl_op_solve with ESL_OP_LADDER
-                                  (its plain rung, then the gmin and source rungs),
-                                  whose result is theirs; iterations add both solves.
+    uint32_t flags;            /* ESL_OP_LADDER: the operating points the plain
+                                  Newton leaves unconverged or singular climb the
+                                  .op ladder (dc/op.zig:160-288) in their job of the
+                                  persistent launch (the gmin and source rungs, as
+                                  esl_op_solve's ESL_OP_LADDER), in both analyses (a
+                                  transient's points start their steps from the
+                                  ladder's result); iterations sum every run.
                                   0: no ladder. Other bits must be 0. */
     esl_tolerances tol;        /* the operating points' Newton, both analyses (esl_tran_solve
                                   runs its points' with the transient's tolerances and at

@@ -1170,7 +1170,9 @@ static uint32_t newton_run(const orc_ckt *c, lane_dev *ld, orc_lu *lu, const uin
                 }
         if (iter == 0) limit = plausible_limit(c, x, w->rhs, w->g);
         for (uint32_t i = 0; i < n; i++) w->sdiag[i] = w->g[c->diag[i]];
-        if (orc_lu_lane_refactor(lu, c->col_ptr, w->g, 1e-12, wide)) {
+        /* a factor that met a singular matrix leaves no usable LU: the next
+         * run factors afresh instead of replaying (direct.zig:268-298) */
+        if (!lu->factored || orc_lu_lane_refactor(lu, c->col_ptr, w->g, 1e-12, wide)) {
             (*repiv)++;
             if (orc_lu_factor(lu, c->col_ptr, c->row_idx, w->g, q, 1e-3) != 0) {
                 *its = iter + 1;

@@ -302,7 +302,7 @@ def test_diffpair_array_fp64_and_fp32_bitwise_and_within_contract(lanes, oracle)
 def test_ladder_matches_oracle_bitwise(lanes, oracle, stages, host_repivot):
     """Long inverter chains that plain Newton leaves unconverged climb the
     .op ladder (the gmin rung, then source stepping and a final run): every
-    run on the device, the control on the host; the oracle's op_ladder must
+    run and the rung control on the device; the oracle's op_ladder must
     give the same bits, iterations (summed over the runs) and reasons. Some
     points of each case go past the gmin rung (the oracle without its source
     rung ends them differently), so both rungs are compared."""
