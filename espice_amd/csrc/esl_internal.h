@@ -18,6 +18,8 @@
 #pragma once
 #include <stdint.h>
 
+#include "../../include/espice_lanes.h"
+
 namespace esl {
 
 struct uint2_t {
@@ -398,6 +400,7 @@ struct AdaptArgs {
     uint32_t n_bps, max_steps, attempts, any_charge;   // attempts: per launch
 };
 
+struct MeasState;
 struct TranArgs {
     OpArgs op;                  // deck, tapes, lane ids, live, x, lim, resume, status, iterations, tolerances
     double *q1, *ip;            // [N][n] charges of the last accepted step, dynamic currents
@@ -412,6 +415,16 @@ struct TranArgs {
     uint32_t tran_tape;         // batch: the tape a lane's transient starts on
     uint32_t adaptive;          // 1: steps chosen per lane by `ad` (s0, s1, dt unused past the lead)
     AdaptArgs ad;
+    // .meas cards fed as fixed steps are accepted (esl_meas.h): the cards,
+    // per lane and card its state mst [N][n_meas], per lane the rows fed
+    // (mrows [N], 0 until the operating point's row), the values at the
+    // lane's end meas_out [N][n_meas]; meas_stop: a lane whose cards all
+    // resolved stops stepping
+    const esl_meas_card *meas;
+    MeasState *mst;
+    uint32_t *mrows;
+    double *meas_out;
+    uint32_t n_meas, meas_stop;
 };
 
 // The persistent batch: lanes of several templates (engines) in one launch

@@ -25,6 +25,7 @@
 
 #include "../../include/espice_lanes.h"
 #include "esl_internal.h"
+#include "esl_meas.h"
 #include "eslmath.h"
 #include "esl_host.h"
 #include "esl_warm.h"
@@ -1651,6 +1652,7 @@ struct Engine {
     u32 lead_id = 0;               // tape of the last solve's lead point
     DevBuf probe_ids, probe_out;
     DevBuf tq1, tip, txp, tstep, tprobes;   // transient state
+    DevBuf meas_cards, meas_state, meas_rows, meas_vals;   // transient .meas cards
     DevBuf tq2, tq3, txc, ttimes, tadapt, tbps;   // adaptive transient state
     DevBuf lad, xgood;                       // the .op ladder on the device: per-lane control, last converged x
     DevBuf pl_u32, pl_f64;                  // the lanes' own LU (device re-pivot)
@@ -3322,6 +3324,8 @@ extern "C" esl_status esl_op_solve(esl_engine *h, const double *values, size_t c
     return ESL_OK;
 }
 
+static_assert(sizeof(esl_meas_clause) == 72 && sizeof(esl_meas_card) == 160, "esl_meas_card layout");
+
 extern "C" void esl_default_tran_options(esl_tran_options *o)
 {
     if (!o) return;
@@ -3648,6 +3652,20 @@ esl_status tran_run(esl_engine *h, const double *values, size_t count, const esl
     ta.dt = o.dt;
     ta.steps = o.steps;
     ta.n_probes = o.n_probes;
+    if (o.n_meas) {
+        HIPCHK(e->meas_cards.ensure(o.n_meas * sizeof(esl_meas_card)));
+        HIPCHK(hipMemcpyAsync(e->meas_cards.p, o.meas, o.n_meas * sizeof(esl_meas_card), hipMemcpyHostToDevice, s));
+        HIPCHK(e->meas_state.ensure(count * o.n_meas * sizeof(esl::MeasState)));
+        HIPCHK(e->meas_rows.ensure(count * 4));
+        HIPCHK(hipMemsetAsync(e->meas_rows.p, 0, count * 4, s));
+        HIPCHK(e->meas_vals.ensure(count * o.n_meas * 8));
+        ta.meas = e->meas_cards.as<esl_meas_card>();
+        ta.mst = e->meas_state.as<esl::MeasState>();
+        ta.mrows = e->meas_rows.as<u32>();
+        ta.meas_out = e->meas_vals.as<double>();
+        ta.n_meas = o.n_meas;
+        ta.meas_stop = o.meas_stop ? 1u : 0u;
+    }
     vector<double> bps;
     if (ar) {
         // step-source edges after 0 (ESL_V_WAVE 1 at td > 0; td is not a
@@ -3848,7 +3866,19 @@ esl_status tran_run(esl_engine *h, const double *values, size_t count, const esl
         HIPCHK(hipMemcpyAsync(probe_out, e->tprobes.p, count * (size_t)o.steps * o.n_probes * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(iterations, e->iters.p, count * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(stt.data(), e->status.p, count * 4, hipMemcpyDeviceToHost, s));
+    if (o.n_meas) {
+        // points whose operating point failed never fed a row: NaN
+        HIPCHK(hipMemcpyAsync(o.meas_out, e->meas_vals.p, count * o.n_meas * 8, hipMemcpyDeviceToHost, s));
+    }
+    if (o.steps_done) HIPCHK(hipMemcpyAsync(o.steps_done, e->tstep.p, count * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    if (o.n_meas)
+        for (size_t l = 0; l < count; l++)
+            if (reasons[l] != ESL_REASON_OK)
+                for (u32 k = 0; k < o.n_meas; k++) o.meas_out[l * o.n_meas + k] = std::nan("");
+    if (o.steps_done)
+        for (size_t l = 0; l < count; l++)
+            if (reasons[l] != ESL_REASON_OK) o.steps_done[l] = 0;
     uint64_t lane_it = 0;
     for (size_t l = 0; l < count; l++) {
         if (reasons[l] != ESL_REASON_OK) { iterations[l] = 0; continue; }
@@ -3883,6 +3913,16 @@ extern "C" esl_status esl_tran_solve(esl_engine *h, const double *values, size_t
     if (!(o.dt > 0) || !std::isfinite(o.dt) || o.steps == 0 || o.steps_per_launch == 0 || o.tol.max_iter == 0)
         return ESL_INVALID_ARGUMENT;
     if (o.n_probes && (!o.probes || !probe_out)) return ESL_INVALID_ARGUMENT;
+    if (o.n_meas) {
+        const u32 n = static_cast<Engine *>(lookup(h, 1))->c.n;
+        if (o.n_meas > ESL_MEAS_MAX_CARDS || !o.meas || !o.meas_out) return ESL_INVALID_ARGUMENT;
+        for (u32 k = 0; k < o.n_meas; k++) {
+            const esl_meas_card &m = o.meas[k];
+            if (m.func < ESL_MEAS_WHEN || m.func > ESL_MEAS_RMS) return ESL_INVALID_ARGUMENT;
+            for (const esl_meas_clause *cl : {&m.a, &m.b})
+                if (cl->node >= n || (cl->has_level_node && cl->level_node >= n)) return ESL_INVALID_ARGUMENT;
+        }
+    }
     return tran_run(h, values, count, o, nullptr, x, probe_out, iterations, reasons);
 }
 

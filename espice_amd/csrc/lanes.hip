@@ -22,6 +22,7 @@
 #include "esl_device.h"
 #include "esl_eval.h"
 #include "esl_internal.h"
+#include "esl_meas.h"
 #include "esl_privlu.h"
 #include "eslmath.h"
 
@@ -1132,6 +1133,33 @@ __device__ inline double step_bound(int method, int cur_method, const double *q0
     return method == 0 ? vmin : sqrt(vmin);
 }
 
+// A row of the lane's .meas cards (t, the unknowns x in LDS): thread k
+// feeds card k (its state in HBM; the first row initialises it). True when
+// every card has resolved.
+__device__ inline bool meas_row(const TranArgs &ta, uint32_t lane, double t, const double *x, uint32_t tid)
+{
+    const bool first = uni(ta.mrows[lane]) == 0;
+    bool open = false;
+    for (uint32_t k = tid; k < ta.n_meas; k += kWave) {
+        const esl_meas_card &c = ta.meas[k];
+        MeasState &st = ta.mst[(size_t)lane * ta.n_meas + k];
+        if (first) meas_init(c, st);
+        open |= !meas_feed(c, st, t, x);
+    }
+    const bool all = !wave_any(open);
+    if (tid == 0) ta.mrows[lane] = ta.mrows[lane] + 1;
+    return all;
+}
+
+// The lane's measures at its end: meas_out [lane][k].
+__device__ inline void meas_finish(const TranArgs &ta, uint32_t lane, uint32_t tid)
+{
+    const bool fed = uni(ta.mrows[lane]) != 0;
+    for (uint32_t k = tid; k < ta.n_meas; k += kWave)
+        ta.meas_out[(size_t)lane * ta.n_meas + k] =
+            fed ? meas_value(ta.meas[k], ta.mst[(size_t)lane * ta.n_meas + k]) : __builtin_nan("");
+}
+
 template <bool kOwnLU>
 __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, double *base, uint32_t nw)
 {
@@ -1219,6 +1247,13 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
     uint32_t iter0 = rs.iter;   // nonzero only when resuming inside a step
     double limit = rs.limit;
     uint32_t status = kTranRunning;
+    // the measures read the operating point's row first (t = 0); a lane
+    // whose cards all resolve there takes no step when it stops early
+    if (ta.n_meas && !ta.adaptive && uni(ta.mrows[lane]) == 0) {
+        const bool all = meas_row(ta, lane, 0.0, x, tid);
+        if (all && ta.meas_stop) status = kConverged;
+        wsync();
+    }
     uint32_t total_it = a.iterations[lane];
     // One step's Newton at source time t with companion coefficients ag0,
     // ag1, from iteration `iter` on: true when it converged (x is then the
@@ -1361,6 +1396,9 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
         for (uint32_t k = tid; k < ta.n_probes; k += kWave)
             ta.probes[((size_t)lane * ta.steps + s) * ta.n_probes + k] = x[ta.probe_ids[k]];
         wsync();
+        // the measures read the accepted row; with every card resolved the
+        // lane may stop (CHANGELOGS.md:13-14)
+        if (ta.n_meas && meas_row(ta, lane, t, x, tid) && ta.meas_stop) status = kConverged;
         // the limiting state at the accepted point (fetlim thresholds from
         // its last evaluation)
         seed_lim_at(dp, live, x, lim, tid, true);
@@ -1369,7 +1407,7 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
         // extrapolated by dt/dt_prev = 1 (MODEINITPRED, tran.zig:1131-1137;
         // lane_device.zig:141-153), then one limit pass of it against the
         // accepted point (applyLimits(trial, cur), tran.zig:1138)
-        if (s + 1 < ta.steps) {
+        if (s + 1 < ta.steps && status == kTranRunning) {
             for (uint32_t i = tid; i < n; i += kWave) {
                 const double cu = x[i];
                 x[i] = cu + (cu - xp[i]);
@@ -1506,6 +1544,7 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
         wsync();
     }
     if (!ta.adaptive && status == kTranRunning && s >= ta.steps) status = kConverged;
+    if (ta.n_meas && status != kTranRunning && status != kRepivot) meas_finish(ta, lane, tid);
     for (uint32_t i = tid; i < n; i += kWave) a.x[(size_t)lane * n + i] = x[i];
     if (tid == 0) {
         if (status != kRepivot) {

@@ -95,7 +95,41 @@ class TranOptions(C.Structure):
     _fields_ = [("struct_size", C.c_size_t), ("tol", Tolerances), ("dt", C.c_double), ("steps", C.c_uint32),
                 ("steps_per_launch", C.c_uint32), ("probes", C.c_void_p), ("n_probes", C.c_uint32),
                 ("use_graph", C.c_uint32), ("progress", PROGRESS_FN), ("progress_ctx", C.c_void_p),
-                ("progress_steps", C.c_uint32), ("reserved", C.c_uint32)]
+                ("progress_steps", C.c_uint32), ("reserved", C.c_uint32), ("meas", C.c_void_p),
+                ("n_meas", C.c_uint32), ("meas_stop", C.c_uint32), ("meas_out", C.c_void_p),
+                ("steps_done", C.c_void_p)]
+
+
+# .meas cards (esl_meas_card): the subset of src/output/measure.zig that the
+# transient lanes measure as they step
+MEAS = {"when": 1, "trig_targ": 2, "find": 3, "avg": 4, "min": 5, "max": 6, "pp": 7, "integ": 8, "rms": 9}
+MEAS_UNSET, MEAS_LAST, MEAS_NO_AT = -1, -2, 1e99
+
+
+class MeasClause(C.Structure):
+    _fields_ = [("node", C.c_uint32), ("level_node", C.c_uint32), ("has_level_node", C.c_uint32),
+                ("reserved", C.c_uint32), ("val", C.c_double), ("td", C.c_double), ("frm", C.c_double),
+                ("to", C.c_double), ("at", C.c_double), ("rise", C.c_int32), ("fall", C.c_int32),
+                ("cross", C.c_int32), ("reserved2", C.c_int32)]
+
+
+class MeasCard(C.Structure):
+    _fields_ = [("func", C.c_uint32), ("reserved", C.c_uint32 * 3), ("a", MeasClause), ("b", MeasClause)]
+
+
+def meas_clause(node=0, val=1e99, td=0.0, frm=0.0, to=0.0, at=MEAS_NO_AT, rise=MEAS_UNSET, fall=MEAS_UNSET,
+                cross=MEAS_UNSET, level_node=None) -> MeasClause:
+    """A clause with the reference's defaults (core/deck.zig:311-340)."""
+    return MeasClause(node, 0 if level_node is None else level_node, 0 if level_node is None else 1, 0, val, td,
+                      frm, to, at, rise, fall, cross, 0)
+
+
+def meas_card(func: str, a: MeasClause, b: MeasClause | None = None) -> MeasCard:
+    c = MeasCard()
+    c.func = MEAS[func]
+    c.a = a
+    c.b = b if b is not None else meas_clause()
+    return c
 
 
 class TranAdaptiveOptions(C.Structure):
@@ -317,12 +351,15 @@ class Engine:
         return {"x": x, "iterations": it, "reasons": rs}
 
     def solve_tran(self, values, dt, steps, probes=(), steps_per_launch=10, use_graph=True, max_iter=None,
-                   progress=None, progress_steps=0):
+                   progress=None, progress_steps=0, meas=None, meas_stop=False):
         """Transient lanes: operating point, then `steps` fixed steps of dt.
         Returns x [count][n] (last state), probes [count][steps][n_probes],
         iterations, reasons. progress(steps_done, steps), when given, is
         called after each group of about progress_steps steps; a true return
-        cancels the solve (EslError)."""
+        cancels the solve (EslError). meas: MeasCard list measured on the
+        device as the steps are accepted ("meas" [count][n_meas],
+        "steps_done" [count]); meas_stop: a point stops once its cards
+        resolve."""
         values = np.ascontiguousarray(np.atleast_2d(np.asarray(values, dtype=np.float64)))
         count = values.shape[0]
         o = TranOptions()
@@ -336,13 +373,22 @@ class Engine:
         pr = np.ascontiguousarray(probes, dtype=np.uint32)
         if len(pr):
             o.probes, o.n_probes = pr.ctypes.data, len(pr)
+        cards, mo, sd = None, None, np.zeros(count, np.uint32)
+        if meas:
+            cards = (MeasCard * len(meas))(*meas)
+            mo = np.zeros((count, len(meas)))
+            o.meas, o.n_meas, o.meas_stop = C.cast(cards, C.c_void_p), len(meas), 1 if meas_stop else 0
+            o.meas_out, o.steps_done = mo.ctypes.data, sd.ctypes.data
         x = np.zeros((count, self.n))
         po = np.zeros((count, steps, max(len(pr), 1)))
         it = np.zeros(count, np.uint32)
         rs = np.zeros(count, np.uint32)
         _check(load().esl_tran_solve(self.h, _ptr(values), count, C.byref(o), _ptr(x),
                                      _ptr(po) if len(pr) else None, _ptr(it), _ptr(rs)), "esl_tran_solve")
-        return {"x": x, "probes": po[:, :, :len(pr)], "iterations": it, "reasons": rs}
+        out = {"x": x, "probes": po[:, :, :len(pr)], "iterations": it, "reasons": rs}
+        if meas:
+            out.update(meas=mo, steps_done=sd)
+        return out
 
     def solve_tran_adaptive(self, values, t_stop, t_step, probes=(), dt_max=0.0, dt_min=1e-18, trtol=7.0,
                             chgtol=1e-14, max_steps=1000, attempts_per_launch=64, max_iter=None):

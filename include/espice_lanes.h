@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define ESL_ABI_VERSION 4u
+#define ESL_ABI_VERSION 5u
 
 /* esl_status: the values of espice_status (include/espice.h:21-38). */
 typedef uint32_t esl_status;
@@ -273,6 +273,31 @@ esl_status esl_op_solve(esl_engine *e, const double *values, size_t count,
  * and replayed. A step that does not converge in max_iter iterations ends
  * the lane (ESL_REASON_NO_CONVERGENCE): fixed-step lanes do not cut the
  * step. ---- */
+/* A .meas card of the lanes' subset (the cards of src/output/measure.zig's
+ * evaluate, :575-660, that read one transient vector: core.Measure with its
+ * clauses, core/deck.zig:311-340): WHEN, TRIG/TARG, FIND (at AT=, or at
+ * WHEN b), AVG, MIN, MAX, PP, INTEG, RMS. A clause names an unknown (node);
+ * WHEN v(a)=v(b) names the level's unknown in level_node with
+ * has_level_node = 1. Counts rise/fall/cross: ESL_MEAS_UNSET, ESL_MEAS_LAST
+ * or k > 0; at: ESL_MEAS_NO_AT when unset; val 1e99 when unset (a clause
+ * without VAL never fires); td, from, to: 0 when unset. Card b: TARG, or the
+ * WHEN of FIND a WHEN b. */
+enum { ESL_MEAS_WHEN = 1, ESL_MEAS_TRIG_TARG, ESL_MEAS_FIND, ESL_MEAS_AVG, ESL_MEAS_MIN, ESL_MEAS_MAX,
+       ESL_MEAS_PP, ESL_MEAS_INTEG, ESL_MEAS_RMS };
+#define ESL_MEAS_UNSET (-1)
+#define ESL_MEAS_LAST (-2)
+#define ESL_MEAS_NO_AT 1e99
+#define ESL_MEAS_MAX_CARDS 64
+typedef struct {
+    uint32_t node, level_node, has_level_node, reserved;
+    double val, td, from, to, at;
+    int32_t rise, fall, cross, reserved2;
+} esl_meas_clause;
+typedef struct {
+    uint32_t func, reserved[3];
+    esl_meas_clause a, b;
+} esl_meas_card;
+
 typedef struct {
     size_t struct_size;        /* sizeof(esl_tran_options) */
     esl_tolerances tol;        /* per step (max_iter default 50) */
@@ -293,6 +318,20 @@ typedef struct {
     void *progress_ctx;
     uint32_t progress_steps;
     uint32_t reserved;
+    /* .meas cards measured on the device as each point accepts its steps,
+     * from its operating point's row (t = 0) on, as the reference's Stream
+     * measures a running transient (src/output/measure.zig:160-330):
+     * meas_out [count][n_meas] receives each card's value (NaN where its
+     * event never came); at most ESL_MEAS_MAX_CARDS cards. meas_stop = 1: a
+     * point stops stepping once every card has resolved (its event found,
+     * or past its TO= / AT=), the early stop of CHANGELOGS.md:13-14;
+     * steps_done [count] (may be NULL) receives each point's steps, and its
+     * probe rows past them are left untouched. */
+    const esl_meas_card *meas;
+    uint32_t n_meas;
+    uint32_t meas_stop;
+    double *meas_out;
+    uint32_t *steps_done;
 } esl_tran_options;
 void esl_default_tran_options(esl_tran_options *opt);
 /* values [count][n_live]; outputs x [count][n] (the last accepted state),

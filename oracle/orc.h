@@ -234,4 +234,53 @@ int orc_tran_adaptive(orc_ckt *c, const double *values, size_t count, const orc_
                       const orc_tran_adapt *o, const uint32_t *probes, uint32_t n_probes, const uint32_t *q,
                       int threads, double *x, double *times, double *probe_out, uint32_t *accepted,
                       uint32_t *rejected, uint32_t *iterations, uint32_t *reasons);
+/* .meas cards over a transient's rows (orc_meas.c): the subset WHEN,
+ * TRIG/TARG, FIND (AT= or WHEN), AVG, MIN, MAX, PP, INTEG, RMS, each a pass
+ * of src/output/measure.zig fed one row at a time as its Stream does.
+ * Counts: ORC_MEAS_UNSET, ORC_MEAS_LAST or k > 0; at = 1e99 when unset. */
+enum { ORC_MEAS_WHEN = 1, ORC_MEAS_TRIG_TARG, ORC_MEAS_FIND, ORC_MEAS_AVG, ORC_MEAS_MIN, ORC_MEAS_MAX,
+       ORC_MEAS_PP, ORC_MEAS_INTEG, ORC_MEAS_RMS };
+#define ORC_MEAS_UNSET (-1)
+#define ORC_MEAS_LAST (-2)
+#define ORC_MEAS_MAX_CARDS 64
+typedef struct {
+    uint32_t node, level_node, has_level_node, pad;   /* the vector; WHEN v=v2's level vector */
+    double val, td, from, to, at;
+    int32_t rise, fall, cross, pad2;
+} orc_meas_clause;
+typedef struct {
+    uint32_t func, pad[3];
+    orc_meas_clause a, b;   /* b: TARG, or FIND's WHEN */
+} orc_meas_card;
+typedef struct {
+    double td, last, prev, prev2, prev_x, time;
+    int32_t rise, fall, cross, want_rise, want_fall, want_cross, above, pending, first, found;
+    uint32_t rows, pad;
+} orc_meas_cross;
+typedef struct {
+    double px, pv, value;
+    uint32_t rows, pad;
+} orc_meas_at;
+typedef struct {
+    double m, m_at, xv, pv, px, span;
+    int32_t first, pad;
+} orc_meas_ext;
+typedef struct {
+    double px, py, p[4][2], first_x, cut_x, sum[3];
+    uint32_t rows, np, count;
+    int32_t inside, cut, pad;
+} orc_meas_integ;
+typedef struct {
+    orc_meas_cross cr[2];
+    orc_meas_at at;
+    orc_meas_ext ext[2];
+    orc_meas_integ integ;
+    double ring[3][2], find_at;
+    uint32_t ring_len, over[2], resolved;
+} orc_meas_state;
+void orc_meas_init(const orc_meas_card *c, orc_meas_state *s);
+int orc_meas_feed(const orc_meas_card *c, orc_meas_state *s, double t, const double *x);
+double orc_meas_value(const orc_meas_card *c, orc_meas_state *s);
+size_t orc_meas_rows(const orc_meas_card *cards, uint32_t n_cards, const double *t, const double *x, size_t rows,
+                     size_t stride, int early, double *out);
 #endif

@@ -75,6 +75,9 @@ def lib():
                                            C.c_size_t, C.c_double, f64p, f64p, f64p, f64p, u32p]
         L.orc_lu_solve_export.restype = C.c_int
         L.orc_lu_solve_export.argtypes = [C.c_uint32, u32p, u32p, f64p, u32p, C.c_double, f64p, f64p]
+        L.orc_meas_rows.restype = C.c_size_t
+        L.orc_meas_rows.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t,
+                                    C.c_int, C.c_void_p]
         L.orc_ckt_new.restype = C.c_void_p
         L.orc_ckt_new.argtypes = [C.POINTER(Deck), C.c_char_p, C.c_size_t]
         L.orc_ckt_free.argtypes = [C.c_void_p]
@@ -322,6 +325,51 @@ def tran(circuit, values, q, dt, steps, probes=(), threads=1, tol=None, tol_op=N
     if trace:
         out.update(first_x=fx, first_lim=fl)
     return out
+
+
+# ---- .meas cards (orc_meas.c) ----
+MEAS_FUNCS = {"when": 1, "trig_targ": 2, "find": 3, "avg": 4, "min": 5, "max": 6, "pp": 7, "integ": 8, "rms": 9}
+MEAS_UNSET, MEAS_LAST, MEAS_NO_AT = -1, -2, 1e99
+
+
+class MeasClause(C.Structure):
+    _fields_ = [("node", C.c_uint32), ("level_node", C.c_uint32), ("has_level_node", C.c_uint32),
+                ("pad", C.c_uint32), ("val", C.c_double), ("td", C.c_double), ("frm", C.c_double),
+                ("to", C.c_double), ("at", C.c_double), ("rise", C.c_int32), ("fall", C.c_int32),
+                ("cross", C.c_int32), ("pad2", C.c_int32)]
+
+
+class MeasCard(C.Structure):
+    _fields_ = [("func", C.c_uint32), ("pad", C.c_uint32 * 3), ("a", MeasClause), ("b", MeasClause)]
+
+
+def meas_clause(node=0, val=1e99, td=0.0, frm=0.0, to=0.0, at=MEAS_NO_AT, rise=MEAS_UNSET, fall=MEAS_UNSET,
+                cross=MEAS_UNSET, level_node=None):
+    """One clause with the reference's defaults (core/deck.zig:311-340)."""
+    return MeasClause(node, 0 if level_node is None else level_node, 0 if level_node is None else 1, 0, val, td, frm,
+                      to, at, rise, fall, cross, 0)
+
+
+def meas_card(func, a, b=None):
+    c = MeasCard()
+    c.func = MEAS_FUNCS[func]
+    c.a = a
+    c.b = b if b is not None else meas_clause()
+    return c
+
+
+def meas_rows(cards, t, x, early=False):
+    """Each card over the rows t[r], x[r][node], fed one row at a time;
+    returns (values, rows read: all, or up to the row that resolved every
+    card when early)."""
+    L = lib()
+    arr = (MeasCard * len(cards))(*cards)
+    t = _f64(t)
+    x = _f64(np.atleast_2d(x))
+    out = np.zeros(len(cards))
+    rows = L.orc_meas_rows(arr, len(cards), t.ctypes.data, x.ctypes.data, len(t), x.shape[1], int(early),
+                           out.ctypes.data)
+    return out, int(rows)
 
 
 def natural_order(n):

@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
 def test_abi_version_and_defaults():
     from espice_amd import lanes
     lib = lanes.load()
-    assert lib.esl_abi_version() == 4
+    assert lib.esl_abi_version() == 5
     o = lanes.default_options()
     assert o.struct_size == ctypes.sizeof(lanes.OpOptions)
     assert o.tol.reltol == 1e-3 and o.tol.abstol == 1e-12 and o.tol.vntol == 1e-6
@@ -41,6 +41,9 @@ def test_abi_version_and_defaults():
     assert t.struct_size == ctypes.sizeof(lanes.TranOptions)
     assert t.steps_per_launch == 10 and t.use_graph == 1 and t.tol.max_iter == 50
     assert not t.progress and t.progress_ctx is None and t.progress_steps == 0
+    assert t.n_meas == 0 and not t.meas and t.meas_stop == 0 and not t.meas_out and not t.steps_done
+    # the .meas card has the size the library asserts for esl_meas_card
+    assert ctypes.sizeof(lanes.MeasCard) == 160 and ctypes.sizeof(lanes.MeasClause) == 72
 
 
 def test_invalid_handles_are_rejected():
