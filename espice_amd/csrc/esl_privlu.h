@@ -12,6 +12,7 @@
 #pragma once
 #include <stdint.h>
 
+#include "esl_eval.h"
 #include "esl_internal.h"
 
 namespace esl {
@@ -275,148 +276,250 @@ __device__ inline bool priv_stage(const PrivLane &L, uint32_t tid, uint32_t wave
     return true;
 }
 
-// The lane replay of its own factors on L.ap (one thread); true when a
-// peel test fails (the lane then factors again).
-__device__ bool priv_refactor(const PrivLane &L, const uint32_t *col_ptr, const uint32_t *q, double growth_limit,
-                              double wide)
+// The lane's factor values (lx, ux, udiag) live in the LDS past the
+// lane's working set when nnz_l + nnz_u + n doubles fit there (`tail`,
+// tail_n doubles), and in global memory otherwise. Uniform.
+__device__ inline bool priv_resident(PrivLane &L, double *tail, uint32_t tail_n)
+{
+    const uint32_t nl = uni(L.hdr[0]), nu = uni(L.hdr[1]);
+    if ((uint64_t)nl + nu + L.n > tail_n) return false;
+    L.lx = tail;
+    L.ux = tail + nl;
+    L.udiag = tail + nl + nu;
+    return true;
+}
+
+template <bool kRes>
+__device__ inline auto pl_vals(double *p)
+{
+    if constexpr (kRes) return lptr(p);
+    else return gptr(p);
+}
+
+// The lane replay of its own factors on L.ap (lane_kernels.zig:49-118 with
+// the wide-multiplier test), spread over the wavefront: column k's scatter,
+// each U entry's update of the dense column (the targets of one L column
+// are distinct rows, so its terms go to different threads while the U
+// entries stay in their order), and the division of column k by its pivot
+// run one term per thread; every slot takes the serial replay's operations
+// in its order, so the factors are bitwise the serial replay's. LDS
+// operations of one wavefront complete in issue order, which carries each
+// step's writes to the next step's reads. True when a peel test fails (the
+// lane then factors again).
+template <bool kRes>
+__device__ bool priv_replay(const PrivLane &L, const uint32_t *col_ptr_, const uint32_t *q_, double growth_limit,
+                            double wide, uint32_t tid)
 {
     const uint32_t n = L.n;
-    const double *vals = L.ap;
-    double *w = L.w;
+    const auto vals = gptr(L.ap);
+    const auto prow = gptr(L.prow);
+    const auto flags = gptr(L.flags);
+    const auto col_ptr = gptr(col_ptr_);
+    const auto q = gptr(q_);
+    const auto w = lptr(L.w);
+    const auto lp = lptr(L.lp), up = lptr(L.up), li = lptr(L.li), ui = lptr(L.ui);
+    const auto lx = pl_vals<kRes>(L.lx), ux = pl_vals<kRes>(L.ux), ud = pl_vals<kRes>(L.udiag);
+    const uint32_t n_void = uni(L.hdr[2]);
+    bool nz = false;
+    for (uint32_t v = tid; v < n_void; v += kWave) nz = nz || vals[gptr(L.void_slots)[v]] != 0;
+    if (wave_any(nz)) return true;
+    uint64_t nonfinite = 0, worst = 0;
     bool bad = false;
-    uint64_t nonfinite = 0;
-    const uint32_t n_void = L.hdr[2];
-    for (uint32_t v = 0; v < n_void; v++)
-        if (vals[L.void_slots[v]] != 0) return true;
+    uint32_t u0 = uni(up[0]);
     for (uint32_t k = 0; k < n; k++) {
-        const uint32_t c = q[k];
-        for (uint32_t p = col_ptr[c]; p < col_ptr[c + 1]; p++) w[L.prow[p]] = vals[p];
-        for (uint32_t p = L.up[k]; p < L.up[k + 1]; p++) {
-            const uint32_t i = L.ui[p];
-            const double uki = w[i];
-            w[i] = 0;
-            L.ux[p] = uki;
-            nonfinite |= pl_bits(uki - uki);
-            for (uint32_t pl = L.lp[i]; pl < L.lp[i + 1]; pl++) {
-                const double t = L.lx[pl] * uki;
-                w[L.li[pl]] = w[L.li[pl]] - t;
+        const uint32_t c = uni(q[k]);
+        const uint32_t p0 = uni(col_ptr[c]), p1 = uni(col_ptr[c + 1]);
+        for (uint32_t p = p0 + tid; p < p1; p += kWave) w[prow[p]] = vals[p];
+        const uint32_t u1 = uni(up[k + 1]);
+        for (uint32_t p = u0; p < u1; p++) {
+            const uint32_t i = uni(ui[p]);
+            const double uki = uni(w[i]);
+            const uint32_t l0 = uni(lp[i]), l1 = uni(lp[i + 1]);
+            nonfinite |= dbits(uki - uki);
+            for (uint32_t pl = l0 + tid; pl < l1; pl += kWave) {
+                const uint32_t r = li[pl];
+                const double t = lx[pl] * uki;
+                w[r] = w[r] - t;
+            }
+            // row i is no target of its own L column
+            if (tid == 0) {
+                w[i] = 0;
+                ux[p] = uki;
             }
         }
-        if (L.flags[k] & kVoidBit) {
-            L.udiag[k] = 1;
-            for (uint32_t p = L.lp[k]; p < L.lp[k + 1]; p++) {
-                L.lx[p] = 0;
-                w[L.li[p]] = 0;
+        const uint32_t fk = uni(flags[k]);
+        const uint32_t l0 = uni(lp[k]), l1 = uni(lp[k + 1]);
+        if (fk & kVoidBit) {
+            if (tid == 0) {
+                ud[k] = 1;
+                for (uint32_t p = l0; p < l1; p++) {
+                    lx[p] = 0;
+                    w[li[p]] = 0;
+                }
+                w[k] = 0;
+                for (uint32_t p = u0; p < u1; p++) {
+                    const uint32_t i = ui[p];
+                    for (uint32_t pl = lp[i]; pl < lp[i + 1]; pl++) w[li[pl]] = 0;
+                }
             }
-            w[k] = 0;
-            for (uint32_t p = L.up[k]; p < L.up[k + 1]; p++) {
-                const uint32_t i = L.ui[p];
-                for (uint32_t pl = L.lp[i]; pl < L.lp[i + 1]; pl++) w[L.li[pl]] = 0;
-            }
+            u0 = u1;
             continue;
         }
-        double d = w[k];
-        w[k] = 0;
+        u0 = u1;
+        double d = uni(w[k]);
         const bool dead = d == 0 || !(fabs(d) < __builtin_inf());
         bad = bad || dead;
         if (dead) d = 1;
-        L.udiag[k] = d;
-        if (growth_limit > 0 && !(L.flags[k] & kScaledBit)) {
-            int64_t cbits = (int64_t)pl_bits(fabs(d));
-            for (uint32_t p = L.lp[k]; p < L.lp[k + 1]; p++) {
-                const uint32_t r = L.li[p];
+        if (tid == 0) {
+            w[k] = 0;
+            ud[k] = d;
+        }
+        const bool scaled = (fk & kScaledBit) != 0;
+        if (growth_limit > 0 && !scaled) {
+            uint64_t cb = 0;
+            for (uint32_t pl = l0 + tid; pl < l1; pl += kWave) {
+                const uint32_t r = li[pl];
                 const double v = w[r];
                 w[r] = 0;
-                const int64_t vb = (int64_t)pl_bits(fabs(v));
-                if (vb > cbits) cbits = vb;
-                L.lx[p] = v / d;
+                const uint64_t vb = dbits(fabs(v));
+                cb = vb > cb ? vb : cb;
+                const double lv = v / d;
+                lx[pl] = lv;
+                const uint64_t lb = dbits(fabs(lv));
+                worst = lb > worst ? lb : worst;
             }
-            const double cmax = pl_double((uint64_t)cbits);
-            bad = bad || !(fabs(d) >= growth_limit * cmax);
+            cb = wave_max_u64(cb);
+            const uint64_t db = dbits(fabs(d));
+            cb = db > cb ? db : cb;
+            bad = bad || !(fabs(d) >= growth_limit * bitsd(cb));
         } else {
-            for (uint32_t p = L.lp[k]; p < L.lp[k + 1]; p++) {
-                const uint32_t r = L.li[p];
+            for (uint32_t pl = l0 + tid; pl < l1; pl += kWave) {
+                const uint32_t r = li[pl];
                 const double lv = w[r] / d;
-                nonfinite |= pl_bits(lv - lv);
-                L.lx[p] = lv;
+                nonfinite |= dbits(lv - lv);
+                lx[pl] = lv;
                 w[r] = 0;
+                const uint64_t lb = dbits(fabs(lv));
+                if (!scaled) worst = lb > worst ? lb : worst;
             }
         }
     }
-    if (nonfinite != 0) return true;
-    uint64_t worst = 0;
-    for (uint32_t k = 0; k < n; k++) {
-        if (L.flags[k] & kScaledBit) continue;
-        for (uint32_t p = L.lp[k]; p < L.lp[k + 1]; p++) {
-            const uint64_t b = pl_bits(fabs(L.lx[p]));
-            worst = b > worst ? b : worst;
-        }
-    }
-    return bad || !(pl_double(worst) <= wide);
+    if (wave_any(nonfinite != 0)) return true;
+    worst = wave_max_u64(worst);
+    return bad || !(bitsd(worst) <= wide);
 }
 
-// dx = -A^-1 rhs from the lane's factors (one thread); dx in unknown
-// order. y: n doubles of scratch.
-__device__ void priv_solve_neg(const PrivLane &L, const uint32_t *q, const double *rhs, double *dx, double *y)
+// dx = -A^-1 rhs from the lane's factors, spread over the wavefront as the
+// replay (the terms of one column's update go to different threads, the
+// columns stay in their order); dx in unknown order. rhs and dx are in LDS;
+// w serves as the work vector and is all zero again on return.
+template <bool kRes>
+__device__ void priv_solve(const PrivLane &L, const uint32_t *q_, const double *rhs_, double *dx_, uint32_t tid)
 {
     const uint32_t n = L.n;
-    for (uint32_t r = 0; r < n; r++) y[L.pinv[r]] = -rhs[r];
+    const auto y = lptr(L.w);
+    const auto rhs = lptr(rhs_);
+    const auto dx = lptr(dx_);
+    const auto pinv = gptr(L.pinv);
+    const auto q = gptr(q_);
+    const auto lp = lptr(L.lp), up = lptr(L.up), li = lptr(L.li), ui = lptr(L.ui);
+    const auto lx = pl_vals<kRes>(L.lx), ux = pl_vals<kRes>(L.ux), ud = pl_vals<kRes>(L.udiag);
+    for (uint32_t r = tid; r < n; r += kWave) y[pinv[r]] = -rhs[r];
+    uint32_t l0 = uni(lp[0]);
     for (uint32_t k = 0; k < n; k++) {
-        const double yk = y[k];
+        const uint32_t l1 = uni(lp[k + 1]);
+        const double yk = uni(y[k]);
         if (yk != 0)
-            for (uint32_t p = L.lp[k]; p < L.lp[k + 1]; p++) {
-                const double t = L.lx[p] * yk;
-                y[L.li[p]] = y[L.li[p]] - t;
+            for (uint32_t p = l0 + tid; p < l1; p += kWave) {
+                const uint32_t r = li[p];
+                const double t = lx[p] * yk;
+                y[r] = y[r] - t;
             }
+        l0 = l1;
     }
+    uint32_t u1 = uni(up[n]);
     for (uint32_t k = n; k > 0;) {
         k--;
-        const double zk = y[k] / L.udiag[k];
-        y[k] = zk;
+        const uint32_t u0 = uni(up[k]);
+        const double zk = uni(y[k]) / uni(ud[k]);
+        if (tid == 0) y[k] = zk;
         if (zk != 0)
-            for (uint32_t p = L.up[k]; p < L.up[k + 1]; p++) {
-                const double t = L.ux[p] * zk;
-                y[L.ui[p]] = y[L.ui[p]] - t;
+            for (uint32_t p = u0 + tid; p < u1; p += kWave) {
+                const uint32_t r = ui[p];
+                const double t = ux[p] * zk;
+                y[r] = y[r] - t;
             }
+        u1 = u0;
     }
-    for (uint32_t j = 0; j < n; j++) dx[q[j]] = y[j];
+    for (uint32_t j = tid; j < n; j += kWave) {
+        dx[q[j]] = y[j];
+        y[j] = 0.0;
+    }
 }
 
-// Entry points for the lane kernels (one thread). lds: the lane's factor-
-// slot area (lds_doubles long); its w is all zero on entry and on return.
+// A fresh factor's values into the LDS tail when they fit there (all
+// threads; the factor wrote them to global memory).
+__device__ inline void priv_load(const PrivLane &G, double *tail, uint32_t tail_n, uint32_t tid)
+{
+    PrivLane L = G;
+    if (!priv_resident(L, tail, tail_n)) return;
+    const uint32_t nl = uni(G.hdr[0]), nu = uni(G.hdr[1]);
+    for (uint32_t p = tid; p < nl; p += kWave) L.lx[p] = G.lx[p];
+    for (uint32_t p = tid; p < nu; p += kWave) L.ux[p] = G.ux[p];
+    for (uint32_t k = tid; k < G.n; k += kWave) L.udiag[k] = G.udiag[k];
+}
 
-// The lane's full factor of its Jacobian (PrivLU::ap of the lane).
+// Entry points for the lane kernels, called by every thread of the lane's
+// wavefront. lds: the lane's factor-slot area (lds_doubles long); its w is
+// all zero on entry and on return. tail: LDS past the lane's working set
+// (tail_n doubles) for the factor values.
+
+// The lane's full factor of its Jacobian (PrivLU::ap of the lane) on one
+// thread: kPlOk, kPlSingular or kPlFull (uniform).
 __device__ __forceinline__ int priv_factor_at(const PrivLU &P, uint32_t lane, const uint32_t *col_ptr,
                                               const uint32_t *row_idx, const uint32_t *q, double *lds,
-                                              uint32_t lds_doubles)
+                                              uint32_t lds_doubles, double *tail, uint32_t tail_n, uint32_t tid)
 {
-    return priv_factor(priv_lane(P, lane, lds, lds_doubles), col_ptr, row_idx, q, 1e-3);
+    const PrivLane L = priv_lane(P, lane, lds, lds_doubles);
+    int rc = kPlOk;
+    if (tid == 0) rc = priv_factor(L, col_ptr, row_idx, q, 1e-3);
+    rc = (int)uni((uint32_t)rc);
+    wsync();
+    if (rc == kPlOk) priv_load(L, tail, tail_n, tid);
+    return rc;
 }
 
 // The replay of the lane's own factors, and on a failed peel test its full
-// factor: kPlOk, kPlSingular or kPlFull.
+// factor: kPlOk, kPlSingular or kPlFull (uniform). fresh: the last factor
+// met a singular matrix and left no usable LU.
 __device__ __forceinline__ int priv_update_at(const PrivLU &P, uint32_t lane, const uint32_t *col_ptr,
                                               const uint32_t *row_idx, const uint32_t *q, double *lds,
-                                              uint32_t lds_doubles, bool fresh = false)
+                                              uint32_t lds_doubles, double *tail, uint32_t tail_n, uint32_t tid,
+                                              bool fresh = false)
 {
-    // fresh: the last factor met a singular matrix and left no usable LU
-    const PrivLane L = priv_lane(P, lane, lds, lds_doubles);
-    if (!fresh && !priv_refactor(L, col_ptr, q, 1e-12, 1.0 / 1e-3)) return kPlOk;
-    return priv_factor(L, col_ptr, row_idx, q, 1e-3);
+    if (!fresh) {
+        PrivLane L = priv_lane(P, lane, lds, lds_doubles);
+        const bool failed = priv_resident(L, tail, tail_n) ? priv_replay<true>(L, col_ptr, q, 1e-12, 1.0 / 1e-3, tid)
+                                                           : priv_replay<false>(L, col_ptr, q, 1e-12, 1.0 / 1e-3, tid);
+        if (!failed) return kPlOk;
+        wsync();
+    }
+    return priv_factor_at(P, lane, col_ptr, row_idx, q, lds, lds_doubles, tail, tail_n, tid);
 }
 
-// dx = -A^-1 rhs in unknown order; w serves as the solve's scratch and is
-// cleared again.
+// dx = -A^-1 rhs in unknown order (rhs and dx in LDS).
 __device__ __forceinline__ void priv_solve_at(const PrivLU &P, uint32_t lane, const uint32_t *q, const double *rhs,
-                                              double *dx, double *lds, uint32_t lds_doubles)
+                                              double *dx, double *lds, uint32_t lds_doubles, double *tail,
+                                              uint32_t tail_n, uint32_t tid)
 {
-    const PrivLane L = priv_lane(P, lane, lds, lds_doubles);
-    priv_solve_neg(L, q, rhs, dx, L.w);
-    for (uint32_t i = 0; i < L.n; i++) L.w[i] = 0.0;
+    PrivLane L = priv_lane(P, lane, lds, lds_doubles);
+    if (priv_resident(L, tail, tail_n)) priv_solve<true>(L, q, rhs, dx, tid);
+    else priv_solve<false>(L, q, rhs, dx, tid);
 }
 
 // A launch's start for a lane already on its own LU: w cleared and the
-// pattern staged (all threads); false when it does not fit.
+// pattern staged (all threads); false when it does not fit. The values are
+// the next replay's (a replay always precedes the next solve).
 __device__ __forceinline__ bool priv_enter(const PrivLU &P, uint32_t lane, double *lds, uint32_t lds_doubles,
                                            uint32_t tid, uint32_t wave)
 {

@@ -666,7 +666,8 @@ __global__ __launch_bounds__(kWave) void op_dump(OpArgs a)
 // and of the batch kernel's .op jobs); nw > 1: a crew of nw wavefronts
 // shares the device evaluation. Returns the lane's status.
 template <bool kOwnLU>
-__device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, double *base, uint32_t nw)
+__device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, double *base, uint32_t nw,
+                                            uint32_t avail)
 {
     const uint32_t tid = threadIdx.x % kWave;
     const DeckProg &dp = a.dp;
@@ -683,6 +684,10 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
     double *rhs = tv + a.tv_cap + yb;
     double *x = rhs + n;
     const Crew cr{nw, a.tv_cap + yb + n, a.tv_cap};
+    // LDS past the working set (avail doubles from base): the own LU's values
+    const uint32_t used = lds_doubles(a.tv_cap, n, nw);
+    double *tail = base + used;
+    const uint32_t tail_n = avail > used ? avail - used : 0u;
     const double *live = a.live + (size_t)lane * dp.n_live;
     double *lim = a.lim + (size_t)lane * lim_width(dp);
     lane_init(a, lane, live, x, lim, tid);
@@ -836,8 +841,8 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
             int rc = kPlOk;
             const bool fresh = (L.flags & kLadNeedFactor) != 0;
             L.flags &= ~kLadNeedFactor;
-            if (tid == 0) rc = priv_update_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq, tv, a.tv_cap, fresh);
-            rc = (int)uni((uint32_t)rc);
+            rc = priv_update_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq, tv, a.tv_cap, tail, tail_n, tid,
+                                fresh);
             wsync();
             if (rc == kPlSingular) exit_status = kSingular;
             else if (rc == kPlFull) exit_status = kRepivot;
@@ -853,9 +858,8 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
                 add_gmin(g_run, dp, dp.diag_slot, pap, rhs, x, tid);
                 for (uint32_t i = tid; i < n; i += kWave) tv[i] = 0.0;   // the factor's dense column (LDS)
                 wsync();
-                int rc = kPlOk;
-                if (tid == 0) rc = priv_factor_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq, tv, a.tv_cap);
-                rc = (int)uni((uint32_t)rc);
+                const int rc =
+                    priv_factor_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq, tv, a.tv_cap, tail, tail_n, tid);
                 for (uint32_t i = tid; i < n; i += kWave) sdiag[i] = pap[dp.diag_slot[i]];
                 wsync();
                 if (rc == kPlOk) {
@@ -902,7 +906,7 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
         }
         if (!have_dx && priv) {
             lap(1);
-            if (tid == 0) priv_solve_at(a.priv, lane, dp.colq, rhs, y, tv, a.tv_cap);
+            priv_solve_at(a.priv, lane, dp.colq, rhs, y, tv, a.tv_cap, tail, tail_n, tid);
             wsync();
             have_dx = true;
         } else if (!have_dx) {
@@ -1002,13 +1006,13 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
 __global__ __launch_bounds__(kWave, 2) void op_newton(OpArgs a)
 {
     extern __shared__ double sm[];
-    op_lane<false>(a, uni(a.lane_ids[blockIdx.x]), sm, 1);
+    op_lane<false>(a, uni(a.lane_ids[blockIdx.x]), sm, 1, 0);
 }
 
 __global__ __launch_bounds__(kWave) void op_newton_own(OpArgs a)
 {
     extern __shared__ double sm[];
-    op_lane<true>(a, uni(a.lane_ids[blockIdx.x]), sm, 1);
+    op_lane<true>(a, uni(a.lane_ids[blockIdx.x]), sm, 1, 0);
 }
 
 // ---- transient lanes ----
@@ -1161,7 +1165,8 @@ __device__ inline void meas_finish(const TranArgs &ta, uint32_t lane, uint32_t t
 }
 
 template <bool kOwnLU>
-__device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, double *base, uint32_t nw)
+__device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, double *base, uint32_t nw,
+                                          uint32_t avail)
 {
     const OpArgs &a = ta.op;
     const uint32_t tid = threadIdx.x % kWave;
@@ -1184,6 +1189,10 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
     double *ip = ta.ip + (size_t)lane * n;
     double *xp = ta.xp + (size_t)lane * n;   // the accepted state before the last one
     const Crew cr{nw, a.tv_cap + yb + n, a.tv_cap};
+    // LDS past the working set (avail doubles from base): the own LU's values
+    const uint32_t used = tran_doubles(a.tv_cap, n, dp.n_cslot, nw);
+    double *tail = base + used;
+    const uint32_t tail_n = avail > used ? avail - used : 0u;
     const double *live = a.live + (size_t)lane * dp.n_live;
     double *lim = a.lim + (size_t)lane * lim_width(dp);
     LaneResume rs = uni_resume(a.resume[lane]);
@@ -1284,8 +1293,7 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
             int rc = kPlOk;
             bool to_host = false;
             if (priv) {
-                if (tid == 0) rc = priv_update_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq, tv, a.tv_cap);
-                rc = (int)uni((uint32_t)rc);
+                rc = priv_update_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq, tv, a.tv_cap, tail, tail_n, tid);
                 wsync();
                 to_host = rc == kPlFull;
             } else if (refactor<double>(T, n, tv, tid)) {
@@ -1294,8 +1302,8 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
                 if (can_priv) {
                     for (uint32_t i = tid; i < n; i += kWave) tv[i] = 0.0;   // the factor's dense column (LDS)
                     wsync();
-                    if (tid == 0) rc = priv_factor_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq, tv, a.tv_cap);
-                    rc = (int)uni((uint32_t)rc);
+                    rc = priv_factor_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq, tv, a.tv_cap, tail, tail_n,
+                                        tid);
                     wsync();
                     if (rc == kPlOk) {
                         priv = true;
@@ -1328,7 +1336,7 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
                 break;
             }
             if (priv) {
-                if (tid == 0) priv_solve_at(a.priv, lane, dp.colq, rhs, y, tv, a.tv_cap);
+                priv_solve_at(a.priv, lane, dp.colq, rhs, y, tv, a.tv_cap, tail, tail_n, tid);
             } else {
                 solve_neg(T, n, tv, rhs, y, tid);
             }
@@ -1561,13 +1569,13 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
 __global__ __launch_bounds__(kWave, 2) void tran_steps(TranArgs ta)
 {
     extern __shared__ double sm[];
-    tran_lane<false>(ta, uni(ta.op.lane_ids[blockIdx.x]), sm, 1);
+    tran_lane<false>(ta, uni(ta.op.lane_ids[blockIdx.x]), sm, 1, 0);
 }
 
 __global__ __launch_bounds__(kWave, 2) void tran_steps_own(TranArgs ta)
 {
     extern __shared__ double sm[];
-    tran_lane<true>(ta, uni(ta.op.lane_ids[blockIdx.x]), sm, 1);
+    tran_lane<true>(ta, uni(ta.op.lane_ids[blockIdx.x]), sm, 1, 0);
 }
 
 // ---- the persistent batch ----
@@ -1586,16 +1594,17 @@ __device__ inline BatchJob uni_job(const BatchJob &g)
 // One job on the lane's leader (nw > 1: with a crew): the operating point,
 // then, for a transient job whose point converged, the hand-over to the
 // transient tape and the steps.
-__device__ __forceinline__ void run_job(const BatchArgs &b, const BatchJob &jb, double *base, uint32_t nw)
+__device__ __forceinline__ void run_job(const BatchArgs &b, const BatchJob &jb, double *base, uint32_t nw,
+                                        uint32_t avail)
 {
     const uint32_t tid = threadIdx.x % kWave;
     if (jb.op == kNoArgs) {
         // a transient resumed at its saved step and iterate
-        tran_lane<true>(b.trans[jb.tran], jb.lane, base, nw);
+        tran_lane<true>(b.trans[jb.tran], jb.lane, base, nw, avail);
         return;
     }
     const OpArgs &oa = b.ops[jb.op];
-    const uint32_t st = op_lane<true>(oa, jb.lane, base, nw);
+    const uint32_t st = op_lane<true>(oa, jb.lane, base, nw, avail);
     if (jb.tran == kNoArgs || st != kConverged) return;
     const TranArgs &ta = b.trans[jb.tran];
     if (tid == 0) {
@@ -1608,7 +1617,7 @@ __device__ __forceinline__ void run_job(const BatchArgs &b, const BatchJob &jb, 
         ta.step[jb.lane] = 0;
     }
     wsync();
-    tran_lane<true>(ta, jb.lane, base, nw);
+    tran_lane<true>(ta, jb.lane, base, nw, avail);
 }
 
 __device__ __forceinline__ void follow_job(const BatchArgs &b, const BatchJob &jb, double *base, uint32_t wv)
@@ -1628,7 +1637,7 @@ __global__ __launch_bounds__(kWave * kCrew) void batch_run(BatchArgs b)
         const BatchJob jb = uni_job(b.block_jobs[j]);
         if (wv == 0) {
             const uint64_t t0 = wall_clock64();
-            run_job(b, jb, sm, kCrew);
+            run_job(b, jb, sm, kCrew, kCrew * b.wave_doubles);
             crew_done(tid);
             if (b.block_ticks && tid == 0) b.block_ticks[j] = wall_clock64() - t0;
         } else {
@@ -1640,7 +1649,7 @@ __global__ __launch_bounds__(kWave * kCrew) void batch_run(BatchArgs b)
     double *mine = sm + (size_t)wv * b.wave_doubles;
     for (uint32_t j = b.wave_ptr[w]; j < b.wave_ptr[w + 1]; j++) {
         const uint64_t t0 = wall_clock64();
-        run_job(b, uni_job(b.wave_jobs[j]), mine, 1);
+        run_job(b, uni_job(b.wave_jobs[j]), mine, 1, b.wave_doubles);
         if (b.wave_ticks && tid == 0) b.wave_ticks[j] = wall_clock64() - t0;
     }
 }
