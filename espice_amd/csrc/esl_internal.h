@@ -346,7 +346,8 @@ struct OpArgs {
     const uint8_t *lane64;      // [N]
     double *a64;                // [N][dp.res_total + 64]
     // optional: [N][kProfPhases] shader cycles per phase (assemble, refactor,
-    // solve, update + limits + gates), summed over the lane's iterations
+    // solve, update + limits + gates, the own LU's full factors), summed over
+    // the lane's iterations
     unsigned long long *prof;
     // the lanes' own LU for re-pivots on the device (FP64 lanes)
     PrivLU priv;
@@ -357,7 +358,11 @@ struct OpArgs {
     // 1: a lane that ends its run (converged, unconverged, singular) writes
     // the LU it is on back to resume[lane].tape, so that a later run of the
     // lane (the ladder's next rung) continues on the same pivots
-    uint32_t keep_tape, pad_kt;
+    uint32_t keep_tape;
+    // LDS doubles of one lane in the standalone kernels (set at launch): a
+    // launch that leaves LDS idle gives the lanes' own LU room past the
+    // working set
+    uint32_t lane_lds;
     // optional [N]: the .op ladder's source rung, every independent source
     // scaled by src[lane] in the device evaluation (Jacobian dumps included)
     const double *src;
@@ -369,7 +374,7 @@ struct OpArgs {
     LadderState *lad;
     double *xgood;
 };
-constexpr int kProfPhases = 4;
+constexpr int kProfPhases = 5;
 
 // Transient lanes (tran_steps): fixed-step integration, one wavefront per
 // lane, state between launches in HBM. Step s (0-based) ends at
@@ -430,12 +435,11 @@ struct TranArgs {
 // The persistent batch: lanes of several templates (engines) in one launch
 // of workgroups of kCrew wavefronts. A job is one lane: its operating point
 // (op >= 0 names the .op arguments), then, when tran names transient
-// arguments, its transient from that point. Workgroup g first runs its
-// block jobs (block_ptr[g] .. block_ptr[g + 1]) one at a time, each on the
-// whole workgroup (lane per block: wavefront 0 leads, the others evaluate
-// devices), then each wavefront w runs its own wave jobs (wave_ptr[g * kCrew
-// + w] ...) in a kCrew-th of the LDS (lane per wavefront). The host deals
-// the jobs; nothing is claimed at run time.
+// arguments, its transient from that point. Each workgroup claims block
+// jobs one at a time while any is left, each on the whole workgroup (lane
+// per block: wavefront 0 leads, the others evaluate devices), then each
+// of its wavefronts claims wave jobs, each in a kCrew-th of the LDS (lane
+// per wavefront). The host orders both lists longest predicted first.
 constexpr uint32_t kCrew = 4;
 constexpr uint32_t kNoArgs = 0xffffffffu;
 struct BatchJob {
@@ -444,10 +448,10 @@ struct BatchJob {
 struct BatchArgs {
     const OpArgs *ops;          // [.op argument sets]
     const TranArgs *trans;      // [transient argument sets]
-    const BatchJob *block_jobs;
-    const uint32_t *block_ptr;  // [grid + 1]
-    const BatchJob *wave_jobs;
-    const uint32_t *wave_ptr;   // [grid * kCrew + 1]
+    const BatchJob *block_jobs; // [n_block], longest predicted first
+    const BatchJob *wave_jobs;  // [n_wave], longest predicted first
+    uint32_t n_block, n_wave;
+    uint32_t *next;             // [2] claim counters (block, wave), zero at launch
     uint32_t wave_doubles;      // LDS doubles of one wavefront in the wave phase
     uint32_t pad;
     uint64_t *block_ticks, *wave_ticks;   // optional: each job's duration (wall-clock ticks)

@@ -4,11 +4,14 @@
 // factors its Jacobian itself with threshold partial pivoting (left-looking
 // Gilbert-Peierls, sparse_lu.zig:260-622; the host's gp_factor and the
 // oracle's orc_lu_factor state the same steps), keeps the factors and pivot
-// sequence in its own slice of memory (the pattern in LDS), and from then on replays
-// them column by column (lane_kernels.zig:49-118 with the wide-multiplier
-// test), factoring again whenever the replay fails. Every operation runs in
-// the oracle's order on one thread, so the lane's numbers are bitwise those
-// of the host re-pivot path (a host factor, then replays of that tape).
+// sequence in its own slice of memory (the pattern in LDS, the values in LDS
+// where they fit), and from then on replays them column by column
+// (lane_kernels.zig:49-118 with the wide-multiplier test), factoring again
+// whenever the replay fails. The factor's reach runs on one thread; its
+// numeric steps, the replay and the solves spread each column's terms over
+// the wavefront. Every slot takes the oracle's operations in the oracle's
+// order, so the lane's numbers are bitwise those of the host re-pivot path
+// (a host factor, then replays of that tape).
 #pragma once
 #include <stdint.h>
 
@@ -27,6 +30,7 @@ struct PrivLane {
     uint32_t *lp, *up, *li, *ui;        // LDS
     uint32_t *glp, *gup, *gli, *gui;    // global copies
     double *lx, *ux, *udiag, *w, *rscale, *ap;
+    uint32_t *kp, *fl;                  // LDS, values resident: column ranges and flags in pivot order
 };
 
 // lds: the lane's factor-slot area, lds_doubles long.
@@ -75,11 +79,12 @@ constexpr uint32_t kPlNone = 0xffffffffu;
 constexpr int kPlOk = 0, kPlSingular = -1, kPlFull = -2;
 
 // voidUnknown (sparse_lu.zig:847-871): column c and row c are structurally
-// present but hold only zeros, and c has no pivot yet.
-__device__ inline bool pl_void_unknown(const PrivLane &L, const uint32_t *col_ptr, const uint32_t *row_idx,
-                                       const double *vals, uint32_t c)
+// present but hold only zeros, and c has no pivot yet (pinv: the factor's
+// row pivots so far).
+__device__ inline bool pl_void_unknown_p(const PrivLane &L, const uint32_t *pinv, const uint32_t *col_ptr,
+                                         const uint32_t *row_idx, const double *vals, uint32_t c)
 {
-    if (L.pinv[c] != kPlNone) return false;
+    if (pinv[c] != kPlNone) return false;
     for (uint32_t p = col_ptr[c]; p < col_ptr[c + 1]; p++)
         if (vals[p] != 0) return false;
     for (uint32_t j = 0; j < L.n; j++) {
@@ -90,174 +95,249 @@ __device__ inline bool pl_void_unknown(const PrivLane &L, const uint32_t *col_pt
     return true;
 }
 
-// The full factor of L.ap with threshold `pivot_tol` (one thread). kPlFull:
-// the factors outgrew the lane's capacity (w is left all zero).
-__device__ int priv_factor(const PrivLane &L, const uint32_t *col_ptr, const uint32_t *row_idx, const uint32_t *q,
-                           double pivot_tol)
+// The full factor of L.ap with threshold `pivot_tol` (sparse_lu.zig:260-622,
+// the steps the host's gp_factor and the oracle's orc_lu_factor state),
+// spread over the wavefront:
+// the reach of each column through the graph of L runs on thread 0, the
+// triangular solve takes the reached columns in reverse finish order with
+// each column's L terms on different threads, the pivot search reduces over
+// the wavefront (largest magnitude, the first in finish order on ties, as
+// the serial scan keeps it), and the new L column is written in finish
+// order. ws: 5n words of LDS for the reach (pinv, marks, stacks, finish
+// order), or null to use the lane's global arrays. kPlFull: the factors
+// outgrew the lane's capacity (w is left all zero).
+__device__ int priv_factor_w(const PrivLane &L, const uint32_t *col_ptr, const uint32_t *row_idx, const uint32_t *q,
+                             double pivot_tol, uint32_t *ws, uint32_t tid)
 {
     const uint32_t n = L.n, nnz = L.nnz;
     const double *vals = L.ap;
-    double *w = L.w;
-    for (uint32_t i = 0; i < n; i++) {
-        L.pinv[i] = kPlNone;
+    const auto w = lptr(L.w);
+    uint32_t *pinv = ws ? ws : L.pinv;
+    uint32_t *flag = ws ? ws + n : L.flag;
+    uint32_t *stack = ws ? ws + 2 * n : L.stack;
+    uint32_t *pstack = ws ? ws + 3 * n : L.pstack;
+    uint32_t *topo = ws ? ws + 4 * n : L.topo;
+    uint64_t *rsb = reinterpret_cast<uint64_t *>(L.rscale);
+    const auto lp = lptr(L.lp), up = lptr(L.up), li = lptr(L.li), ui = lptr(L.ui);
+    for (uint32_t i = tid; i < n; i += kWave) {
+        pinv[i] = kPlNone;
         L.flags[i] = 0;
-        L.flag[i] = 0;
-        L.rscale[i] = 0.0;
-    }
-    uint32_t nnz_l = 0, nnz_u = 0, n_void = 0;
-    bool has_void = false;
-    uint64_t amaxb = 0;
-    for (uint32_t p = 0; p < nnz; p++) {
-        const uint64_t bb = pl_bits(fabs(vals[p]));
-        amaxb = bb > amaxb ? bb : amaxb;
-    }
-    if (!(pl_double(amaxb) <= 1.7976931348623157e308)) return kPlSingular;
-    for (uint32_t j = 0; j < n; j++)
-        for (uint32_t p = col_ptr[j]; p < col_ptr[j + 1]; p++)
-            L.rscale[row_idx[p]] = fmax(L.rscale[row_idx[p]], fabs(vals[p]));
-    for (uint32_t i = 0; i < n; i++) {
-        const double r = L.rscale[i];
-        L.rscale[i] = (r > 0 && r < __builtin_inf()) ? 1.0 / r : 1.0;
-    }
-    uint64_t nonfinite = 0;
-    for (uint32_t i = 0; i < n; i++) {
+        flag[i] = 0;
+        rsb[i] = 0;
         L.want[i] = i;
         L.want_of[i] = i;
     }
+    uint64_t amaxb = 0;
+    for (uint32_t p = tid; p < nnz; p += kWave) {
+        const uint64_t bb = pl_bits(fabs(vals[p]));
+        amaxb = bb > amaxb ? bb : amaxb;
+    }
+    amaxb = wave_max_u64(amaxb);
+    if (!(pl_double(amaxb) <= 1.7976931348623157e308)) return kPlSingular;
+    wsync();
+    // row scales: every entry finite here, so the largest magnitude is the
+    // largest bit pattern
+    for (uint32_t p = tid; p < nnz; p += kWave)
+        atomicMax(reinterpret_cast<unsigned long long *>(rsb + row_idx[p]), (unsigned long long)pl_bits(fabs(vals[p])));
+    wsync();
+    for (uint32_t i = tid; i < n; i += kWave) {
+        const double r = pl_double(rsb[i]);
+        L.rscale[i] = (r > 0 && r < __builtin_inf()) ? 1.0 / r : 1.0;
+    }
+    wsync();
+    uint32_t nnz_l = 0, nnz_u = 0;
+    bool has_void = false;
+    uint64_t nonfinite = 0;
+    const uint32_t room = L.cap < L.lcap ? L.cap : L.lcap;
     for (uint32_t k = 0; k < n; k++) {
-        const uint32_t c = q[k];
-        L.lp[k] = nnz_l;
-        L.up[k] = nnz_u;
+        const uint32_t c = uni(q[k]);
         const uint32_t mark = k + 1;
-        // reach of A[:, c] through the graph of L (rows still original)
         uint32_t nt = 0;
-        for (uint32_t p = col_ptr[c]; p < col_ptr[c + 1]; p++) {
-            uint32_t r = row_idx[p];
-            if (L.flag[r] == mark) continue;
-            uint32_t sp = 0;
-            L.stack[0] = r;
-            L.pstack[0] = L.pinv[r] == kPlNone ? 0 : L.lp[L.pinv[r]];
-            L.flag[r] = mark;
-            for (;;) {
-                r = L.stack[sp];
-                const uint32_t kc = L.pinv[r];
-                const uint32_t end = kc == kPlNone ? 0 : L.lp[kc + 1];
-                uint32_t pos = L.pstack[sp];
-                bool descended = false;
-                while (pos < end) {
-                    const uint32_t child = L.li[pos];
-                    pos++;
-                    if (L.flag[child] != mark) {
-                        L.pstack[sp] = pos;
-                        L.flag[child] = mark;
-                        sp++;
-                        L.stack[sp] = child;
-                        L.pstack[sp] = L.pinv[child] == kPlNone ? 0 : L.lp[L.pinv[child]];
-                        descended = true;
-                        break;
+        if (tid == 0) {
+            lp[k] = nnz_l;
+            up[k] = nnz_u;
+            // reach of A[:, c] through the graph of L (rows still original)
+            for (uint32_t p = col_ptr[c]; p < col_ptr[c + 1]; p++) {
+                uint32_t r = row_idx[p];
+                if (flag[r] == mark) continue;
+                uint32_t sp = 0;
+                stack[0] = r;
+                pstack[0] = pinv[r] == kPlNone ? 0 : lp[pinv[r]];
+                flag[r] = mark;
+                for (;;) {
+                    r = stack[sp];
+                    const uint32_t kc = pinv[r];
+                    const uint32_t end = kc == kPlNone ? 0 : lp[kc + 1];
+                    uint32_t pos = pstack[sp];
+                    bool descended = false;
+                    while (pos < end) {
+                        const uint32_t child = li[pos];
+                        pos++;
+                        if (flag[child] != mark) {
+                            pstack[sp] = pos;
+                            flag[child] = mark;
+                            sp++;
+                            stack[sp] = child;
+                            pstack[sp] = pinv[child] == kPlNone ? 0 : lp[pinv[child]];
+                            descended = true;
+                            break;
+                        }
                     }
+                    if (descended) continue;
+                    topo[nt++] = r;
+                    if (sp == 0) break;
+                    sp--;
                 }
-                if (descended) continue;
-                L.topo[nt++] = r;
-                if (sp == 0) break;
-                sp--;
             }
         }
-        const uint32_t room = L.cap < L.lcap ? L.cap : L.lcap;
+        nt = uni(nt);
+        wsync();
         if (nnz_u + nt > room || nnz_l + nt > room) {
-            for (uint32_t i = 0; i < n; i++) w[i] = 0.0;
+            for (uint32_t i = tid; i < n; i += kWave) w[i] = 0.0;
             return kPlFull;
         }
-        for (uint32_t p = col_ptr[c]; p < col_ptr[c + 1]; p++) w[row_idx[p]] = vals[p];
+        {
+            const uint32_t p0 = uni(col_ptr[c]), p1 = uni(col_ptr[c + 1]);
+            for (uint32_t p = p0 + tid; p < p1; p += kWave) w[row_idx[p]] = vals[p];
+        }
         // triangular solve in reverse finish order
         for (uint32_t idx = nt; idx > 0;) {
             idx--;
-            const uint32_t r = L.topo[idx], kc = L.pinv[r];
+            const uint32_t r = uni(topo[idx]), kc = uni(pinv[r]);
             if (kc == kPlNone) continue;
-            const double ukr = w[r];
-            L.ui[nnz_u] = kc;
-            L.ux[nnz_u] = ukr;
+            const double ukr = uni(w[r]);
+            if (tid == 0) {
+                ui[nnz_u] = kc;
+                L.ux[nnz_u] = ukr;
+            }
             nnz_u++;
-            for (uint32_t pl = L.lp[kc]; pl < L.lp[kc + 1]; pl++) {
+            const uint32_t l0 = uni(lp[kc]), l1 = uni(lp[kc + 1]);
+            for (uint32_t pl = l0 + tid; pl < l1; pl += kWave) {
+                const uint32_t t_r = li[pl];
                 const double t = L.lx[pl] * ukr;
-                w[L.li[pl]] = w[L.li[pl]] - t;
+                w[t_r] = w[t_r] - t;
             }
         }
+        // pivot search: the largest magnitude, the first in finish order on
+        // ties (a strictly larger one replaces it in the serial scan)
         double amax = 0, smax = 0;
-        uint32_t piv = kPlNone;
-        for (uint32_t s = 0; s < nt; s++) {
-            const uint32_t r = L.topo[s];
-            if (L.pinv[r] != kPlNone) continue;
+        uint32_t best_s = kPlNone;
+        for (uint32_t s0 = tid; s0 < nt; s0 += kWave) {
+            const uint32_t r = topo[s0];
+            if (pinv[r] != kPlNone) continue;
             const double a = fabs(w[r]);
             if (a > amax) {
                 amax = a;
-                piv = r;
+                best_s = s0;
             }
             smax = fmax(smax, a * L.rscale[r]);
         }
+        for (int o = 32; o > 0; o >>= 1) {
+            const double oa = __shfl_xor(amax, o, kWave);
+            const uint32_t os = (uint32_t)__shfl_xor((int)best_s, o, kWave);
+            if (oa > amax || (oa == amax && os < best_s)) {
+                amax = oa;
+                best_s = os;
+            }
+            smax = fmax(smax, __shfl_xor(smax, o, kWave));
+        }
+        amax = uni(amax);
+        smax = uni(smax);
+        best_s = uni(best_s);
+        uint32_t piv = (best_s == kPlNone || !(amax > 0)) ? kPlNone : uni(topo[best_s]);
         if (piv == kPlNone || amax == 0 || !(amax < __builtin_inf())) {
-            const bool is_void = pl_void_unknown(L, col_ptr, row_idx, vals, c);
-            for (uint32_t s = 0; s < nt; s++) w[L.topo[s]] = 0;
+            int is_void = 0;
+            if (tid == 0) is_void = pl_void_unknown_p(L, pinv, col_ptr, row_idx, vals, c) ? 1 : 0;
+            is_void = (int)uni((uint32_t)is_void);
+            for (uint32_t s0 = tid; s0 < nt; s0 += kWave) w[topo[s0]] = 0;
             if (!is_void) return kPlSingular;
-            L.flags[k] |= kVoidBit;
             has_void = true;
-            L.udiag[k] = 1;
-            L.pinv[c] = k;
+            if (tid == 0) {
+                L.flags[k] |= kVoidBit;
+                L.udiag[k] = 1;
+                pinv[c] = k;
+            }
+            wsync();
             continue;
         }
-        const uint32_t pr = L.want[c];
-        if (L.pinv[pr] == kPlNone) {
-            const double dmag = fabs(w[pr]);
-            if (dmag > 0 && dmag >= pivot_tol * amax) {
-                piv = pr;
-            } else if (dmag > 0 && dmag * L.rscale[pr] >= pivot_tol * smax) {
-                piv = pr;
-                L.flags[k] |= kScaledBit;
+        if (tid == 0) {
+            const uint32_t pr = L.want[c];
+            if (pinv[pr] == kPlNone) {
+                const double dmag = fabs(w[pr]);
+                if (dmag > 0 && dmag >= pivot_tol * amax) {
+                    piv = pr;
+                } else if (dmag > 0 && dmag * L.rscale[pr] >= pivot_tol * smax) {
+                    piv = pr;
+                    L.flags[k] |= kScaledBit;
+                }
             }
+            if (piv != pr) {
+                const uint32_t c2 = L.want_of[piv];
+                L.want[c2] = pr;
+                L.want_of[pr] = c2;
+            }
+            L.udiag[k] = w[piv];
+            pinv[piv] = k;
         }
-        if (piv != pr) {
-            const uint32_t c2 = L.want_of[piv];
-            L.want[c2] = pr;
-            L.want_of[pr] = c2;
-        }
-        const double d = w[piv];
-        L.udiag[k] = d;
-        L.pinv[piv] = k;
-        for (uint32_t s = 0; s < nt; s++) {
-            const uint32_t r = L.topo[s];
-            const double v = w[r];
-            nonfinite |= pl_bits(v - v);
-            if (L.pinv[r] == kPlNone) {
+        piv = uni(piv);
+        const double d = uni(w[piv]);
+        wsync();
+        // the L column in finish order
+        for (uint32_t b = 0; b < nt; b += kWave) {
+            const uint32_t s0 = b + tid;
+            bool in_l = false;
+            double v = 0;
+            uint32_t r = 0;
+            if (s0 < nt) {
+                r = topo[s0];
+                v = w[r];
+                nonfinite |= pl_bits(v - v);
+                in_l = pinv[r] == kPlNone;
+            }
+            const uint64_t mask = __ballot(in_l);
+            const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << tid) - 1ull));
+            if (in_l) {
                 const double l = v / d;
                 nonfinite |= pl_bits(l - l);
-                L.li[nnz_l] = r;
-                L.lx[nnz_l] = l;
-                nnz_l++;
+                li[nnz_l + rank] = r;
+                L.lx[nnz_l + rank] = l;
             }
-            w[r] = 0;
+            if (s0 < nt) w[r] = 0;
+            nnz_l += (uint32_t)__popcll(mask);
         }
+        wsync();
     }
-    L.lp[n] = nnz_l;
-    L.up[n] = nnz_u;
-    if (nonfinite != 0) return kPlSingular;
-    for (uint32_t p = 0; p < nnz_l; p++) L.li[p] = L.pinv[L.li[p]];
-    for (uint32_t p = 0; p < nnz; p++) L.prow[p] = L.pinv[row_idx[p]];
-    if (has_void)
+    if (tid == 0) {
+        lp[n] = nnz_l;
+        up[n] = nnz_u;
+    }
+    if (wave_any(nonfinite != 0)) return kPlSingular;
+    wsync();
+    for (uint32_t p = tid; p < nnz_l; p += kWave) li[p] = pinv[li[p]];
+    for (uint32_t p = tid; p < nnz; p += kWave) L.prow[p] = pinv[row_idx[p]];
+    if (ws)
+        for (uint32_t i = tid; i < n; i += kWave) L.pinv[i] = pinv[i];
+    wsync();
+    uint32_t n_void = 0;
+    if (has_void && tid == 0)
         for (uint32_t j = 0; j < n; j++) {
-            const uint32_t c = q[j];
-            for (uint32_t p = col_ptr[c]; p < col_ptr[c + 1]; p++)
+            const uint32_t cj = q[j];
+            for (uint32_t p = col_ptr[cj]; p < col_ptr[cj + 1]; p++)
                 if ((L.flags[j] & kVoidBit) || (L.flags[L.prow[p]] & kVoidBit)) L.void_slots[n_void++] = p;
         }
-    L.hdr[0] = nnz_l;
-    L.hdr[1] = nnz_u;
-    L.hdr[2] = n_void;
-    // the pattern's global copy, for the lane's later launches
-    for (uint32_t k = 0; k <= n; k++) {
-        L.glp[k] = L.lp[k];
-        L.gup[k] = L.up[k];
+    if (tid == 0) {
+        L.hdr[0] = nnz_l;
+        L.hdr[1] = nnz_u;
+        L.hdr[2] = n_void;
     }
-    for (uint32_t p = 0; p < nnz_l; p++) L.gli[p] = L.li[p];
-    for (uint32_t p = 0; p < nnz_u; p++) L.gui[p] = L.ui[p];
+    // the pattern's global copy, for the lane's later launches
+    for (uint32_t k = tid; k <= n; k += kWave) {
+        L.glp[k] = lp[k];
+        L.gup[k] = up[k];
+    }
+    for (uint32_t p = tid; p < nnz_l; p += kWave) L.gli[p] = li[p];
+    for (uint32_t p = tid; p < nnz_u; p += kWave) L.gui[p] = ui[p];
+    wsync();
     return kPlOk;
 }
 
@@ -277,16 +357,31 @@ __device__ inline bool priv_stage(const PrivLane &L, uint32_t tid, uint32_t wave
 }
 
 // The lane's factor values (lx, ux, udiag) live in the LDS past the
-// lane's working set when nnz_l + nnz_u + n doubles fit there (`tail`,
-// tail_n doubles), and in global memory otherwise. Uniform.
+// lane's working set (`tail`, tail_n doubles) when they fit there with the
+// column ranges of A in pivot order (kp[2k], kp[2k + 1]: column q[k]'s
+// span of the pattern) and the column flags, and in global memory
+// otherwise. Uniform.
 __device__ inline bool priv_resident(PrivLane &L, double *tail, uint32_t tail_n)
 {
     const uint32_t nl = uni(L.hdr[0]), nu = uni(L.hdr[1]);
-    if ((uint64_t)nl + nu + L.n > tail_n) return false;
+    if ((uint64_t)nl + nu + 2 * L.n + (L.n + 1) / 2 > tail_n) return false;
     L.lx = tail;
     L.ux = tail + nl;
     L.udiag = tail + nl + nu;
+    L.kp = reinterpret_cast<uint32_t *>(tail + nl + nu + L.n);
+    L.fl = L.kp + 2 * L.n;
     return true;
+}
+
+// The column ranges and flags of a resident lane (all threads).
+__device__ inline void priv_stage_static(const PrivLane &L, const uint32_t *col_ptr, const uint32_t *q, uint32_t tid)
+{
+    for (uint32_t k = tid; k < L.n; k += kWave) {
+        const uint32_t c = q[k];
+        L.kp[2 * k] = col_ptr[c];
+        L.kp[2 * k + 1] = col_ptr[c + 1];
+        L.fl[k] = L.flags[k];
+    }
 }
 
 template <bool kRes>
@@ -304,8 +399,9 @@ __device__ inline auto pl_vals(double *p)
 // run one term per thread; every slot takes the serial replay's operations
 // in its order, so the factors are bitwise the serial replay's. LDS
 // operations of one wavefront complete in issue order, which carries each
-// step's writes to the next step's reads. True when a peel test fails (the
-// lane then factors again).
+// step's writes to the next step's reads. A resident lane reads the next
+// column's terms of A while it works on this one. True when a peel test
+// fails (the lane then factors again).
 template <bool kRes>
 __device__ bool priv_replay(const PrivLane &L, const uint32_t *col_ptr_, const uint32_t *q_, double growth_limit,
                             double wide, uint32_t tid)
@@ -319,24 +415,84 @@ __device__ bool priv_replay(const PrivLane &L, const uint32_t *col_ptr_, const u
     const auto w = lptr(L.w);
     const auto lp = lptr(L.lp), up = lptr(L.up), li = lptr(L.li), ui = lptr(L.ui);
     const auto lx = pl_vals<kRes>(L.lx), ux = pl_vals<kRes>(L.ux), ud = pl_vals<kRes>(L.udiag);
+    const auto kp = lptr(L.kp), fl = lptr(L.fl);
     const uint32_t n_void = uni(L.hdr[2]);
     bool nz = false;
     for (uint32_t v = tid; v < n_void; v += kWave) nz = nz || vals[gptr(L.void_slots)[v]] != 0;
     if (wave_any(nz)) return true;
     uint64_t nonfinite = 0, worst = 0;
-    bool bad = false;
+    bool bad = false, growth_bad = false;
     uint32_t u0 = uni(up[0]);
+    // resident: the next column's span and this thread's first term of it
+    uint32_t np0 = 0, np1 = 0, sr = 0;
+    double sv = 0;
+    if constexpr (kRes) {
+        np0 = uni(kp[0]);
+        np1 = uni(kp[1]);
+        if (np0 + tid < np1) {
+            sv = vals[np0 + tid];
+            sr = prow[np0 + tid];
+        }
+    }
     for (uint32_t k = 0; k < n; k++) {
-        const uint32_t c = uni(q[k]);
-        const uint32_t p0 = uni(col_ptr[c]), p1 = uni(col_ptr[c + 1]);
-        for (uint32_t p = p0 + tid; p < p1; p += kWave) w[prow[p]] = vals[p];
+        uint32_t p0, p1, fk;
+        // this column's U entries (thread t: the t-th, its L column's span)
+        // and its own L rows, read together before any of them is needed
         const uint32_t u1 = uni(up[k + 1]);
-        for (uint32_t p = u0; p < u1; p++) {
-            const uint32_t i = uni(ui[p]);
+        const uint32_t l0 = uni(lp[k]), l1 = uni(lp[k + 1]);
+        const uint32_t m = u1 - u0;
+        uint32_t ei = 0, ea = 0, eb = 0;
+        if (tid < m) {
+            ei = ui[u0 + tid];
+            ea = lp[ei];
+            eb = lp[ei + 1];
+        }
+        const bool own = l0 + tid < l1;
+        const uint32_t kr = own ? li[l0 + tid] : 0u;
+        if constexpr (kRes) {
+            p0 = np0;
+            p1 = np1;
+            const double cv = sv;
+            const uint32_t cr = sr;
+            if (k + 1 < n) {
+                np0 = uni(kp[2 * k + 2]);
+                np1 = uni(kp[2 * k + 3]);
+                if (np0 + tid < np1) {
+                    sv = vals[np0 + tid];
+                    sr = prow[np0 + tid];
+                }
+            }
+            if (p0 + tid < p1) w[cr] = cv;
+            for (uint32_t p = p0 + kWave + tid; p < p1; p += kWave) w[prow[p]] = vals[p];
+            fk = uni(fl[k]);
+        } else {
+            const uint32_t c = uni(q[k]);
+            p0 = uni(col_ptr[c]);
+            p1 = uni(col_ptr[c + 1]);
+            for (uint32_t p = p0 + tid; p < p1; p += kWave) w[prow[p]] = vals[p];
+            fk = uni(flags[k]);
+        }
+        for (uint32_t j = 0; j < m; j++) {
+            uint32_t i, a0, a1;
+            if (j < kWave) {
+                i = (uint32_t)__builtin_amdgcn_readlane((int)ei, (int)j);
+                a0 = (uint32_t)__builtin_amdgcn_readlane((int)ea, (int)j);
+                a1 = (uint32_t)__builtin_amdgcn_readlane((int)eb, (int)j);
+            } else {
+                i = uni(ui[u0 + j]);
+                a0 = uni(lp[i]);
+                a1 = uni(lp[i + 1]);
+            }
+            const bool on = a0 + tid < a1;
+            const uint32_t r0 = on ? li[a0 + tid] : 0u;
+            const double m0 = on ? lx[a0 + tid] : 0.0;
             const double uki = uni(w[i]);
-            const uint32_t l0 = uni(lp[i]), l1 = uni(lp[i + 1]);
             nonfinite |= dbits(uki - uki);
-            for (uint32_t pl = l0 + tid; pl < l1; pl += kWave) {
+            if (on) {
+                const double t = m0 * uki;
+                w[r0] = w[r0] - t;
+            }
+            for (uint32_t pl = a0 + kWave + tid; pl < a1; pl += kWave) {
                 const uint32_t r = li[pl];
                 const double t = lx[pl] * uki;
                 w[r] = w[r] - t;
@@ -344,11 +500,9 @@ __device__ bool priv_replay(const PrivLane &L, const uint32_t *col_ptr_, const u
             // row i is no target of its own L column
             if (tid == 0) {
                 w[i] = 0;
-                ux[p] = uki;
+                ux[u0 + j] = uki;
             }
         }
-        const uint32_t fk = uni(flags[k]);
-        const uint32_t l0 = uni(lp[k]), l1 = uni(lp[k + 1]);
         if (fk & kVoidBit) {
             if (tid == 0) {
                 ud[k] = 1;
@@ -366,6 +520,7 @@ __device__ bool priv_replay(const PrivLane &L, const uint32_t *col_ptr_, const u
             continue;
         }
         u0 = u1;
+        const double v0 = own ? w[kr] : 0.0;
         double d = uni(w[k]);
         const bool dead = d == 0 || !(fabs(d) < __builtin_inf());
         bad = bad || dead;
@@ -376,8 +531,21 @@ __device__ bool priv_replay(const PrivLane &L, const uint32_t *col_ptr_, const u
         }
         const bool scaled = (fk & kScaledBit) != 0;
         if (growth_limit > 0 && !scaled) {
-            uint64_t cb = 0;
-            for (uint32_t pl = l0 + tid; pl < l1; pl += kWave) {
+            // the growth test on the column's largest magnitude: each thread
+            // tests its own terms (|d| >= g m holds for the largest m exactly
+            // when it holds for every term's)
+            const uint64_t db = dbits(fabs(d));
+            uint64_t cb = db;
+            if (own) {
+                w[kr] = 0;
+                const uint64_t vb = dbits(fabs(v0));
+                cb = vb > cb ? vb : cb;
+                const double lv = v0 / d;
+                lx[l0 + tid] = lv;
+                const uint64_t lb = dbits(fabs(lv));
+                worst = lb > worst ? lb : worst;
+            }
+            for (uint32_t pl = l0 + kWave + tid; pl < l1; pl += kWave) {
                 const uint32_t r = li[pl];
                 const double v = w[r];
                 w[r] = 0;
@@ -388,12 +556,17 @@ __device__ bool priv_replay(const PrivLane &L, const uint32_t *col_ptr_, const u
                 const uint64_t lb = dbits(fabs(lv));
                 worst = lb > worst ? lb : worst;
             }
-            cb = wave_max_u64(cb);
-            const uint64_t db = dbits(fabs(d));
-            cb = db > cb ? db : cb;
-            bad = bad || !(fabs(d) >= growth_limit * bitsd(cb));
+            growth_bad = growth_bad || !(fabs(d) >= growth_limit * bitsd(cb));
         } else {
-            for (uint32_t pl = l0 + tid; pl < l1; pl += kWave) {
+            if (own) {
+                const double lv = v0 / d;
+                nonfinite |= dbits(lv - lv);
+                lx[l0 + tid] = lv;
+                w[kr] = 0;
+                const uint64_t lb = dbits(fabs(lv));
+                if (!scaled) worst = lb > worst ? lb : worst;
+            }
+            for (uint32_t pl = l0 + kWave + tid; pl < l1; pl += kWave) {
                 const uint32_t r = li[pl];
                 const double lv = w[r] / d;
                 nonfinite |= dbits(lv - lv);
@@ -406,7 +579,7 @@ __device__ bool priv_replay(const PrivLane &L, const uint32_t *col_ptr_, const u
     }
     if (wave_any(nonfinite != 0)) return true;
     worst = wave_max_u64(worst);
-    return bad || !(bitsd(worst) <= wide);
+    return bad || wave_any(growth_bad) || !(bitsd(worst) <= wide);
 }
 
 // dx = -A^-1 rhs from the lane's factors, spread over the wavefront as the
@@ -425,31 +598,60 @@ __device__ void priv_solve(const PrivLane &L, const uint32_t *q_, const double *
     const auto lp = lptr(L.lp), up = lptr(L.up), li = lptr(L.li), ui = lptr(L.ui);
     const auto lx = pl_vals<kRes>(L.lx), ux = pl_vals<kRes>(L.ux), ud = pl_vals<kRes>(L.udiag);
     for (uint32_t r = tid; r < n; r += kWave) y[pinv[r]] = -rhs[r];
-    uint32_t l0 = uni(lp[0]);
+    // each column's first wavefront of terms is read before its pivot entry
+    // is known, so the two reads overlap
+    uint32_t sa = 0, sb = 0;   // thread t: lp[k0 + t], lp[k0 + t + 1] of the current 64 columns
     for (uint32_t k = 0; k < n; k++) {
-        const uint32_t l1 = uni(lp[k + 1]);
+        if ((k & (kWave - 1)) == 0 && k + tid < n) {
+            sa = lp[k + tid];
+            sb = lp[k + tid + 1];
+        }
+        const uint32_t l0 = (uint32_t)__builtin_amdgcn_readlane((int)sa, (int)(k & (kWave - 1)));
+        const uint32_t l1 = (uint32_t)__builtin_amdgcn_readlane((int)sb, (int)(k & (kWave - 1)));
+        const bool on = l0 + tid < l1;
+        const uint32_t r0 = on ? li[l0 + tid] : 0u;
+        const double m0 = on ? lx[l0 + tid] : 0.0;
         const double yk = uni(y[k]);
-        if (yk != 0)
-            for (uint32_t p = l0 + tid; p < l1; p += kWave) {
+        if (yk != 0) {
+            if (on) {
+                const double t = m0 * yk;
+                y[r0] = y[r0] - t;
+            }
+            for (uint32_t p = l0 + kWave + tid; p < l1; p += kWave) {
                 const uint32_t r = li[p];
                 const double t = lx[p] * yk;
                 y[r] = y[r] - t;
             }
-        l0 = l1;
+        }
     }
-    uint32_t u1 = uni(up[n]);
     for (uint32_t k = n; k > 0;) {
         k--;
-        const uint32_t u0 = uni(up[k]);
+        // thread t: up[k0 + t], up[k0 + t + 1] of the 64 columns ending at k
+        const uint32_t k0 = k & ~(kWave - 1);
+        if (k == n - 1 || (k & (kWave - 1)) == kWave - 1) {
+            if (k0 + tid <= k) {
+                sa = up[k0 + tid];
+                sb = up[k0 + tid + 1];
+            }
+        }
+        const uint32_t u0 = (uint32_t)__builtin_amdgcn_readlane((int)sa, (int)(k - k0));
+        const uint32_t u1 = (uint32_t)__builtin_amdgcn_readlane((int)sb, (int)(k - k0));
+        const bool on = u0 + tid < u1;
+        const uint32_t r0 = on ? ui[u0 + tid] : 0u;
+        const double m0 = on ? ux[u0 + tid] : 0.0;
         const double zk = uni(y[k]) / uni(ud[k]);
         if (tid == 0) y[k] = zk;
-        if (zk != 0)
-            for (uint32_t p = u0 + tid; p < u1; p += kWave) {
+        if (zk != 0) {
+            if (on) {
+                const double t = m0 * zk;
+                y[r0] = y[r0] - t;
+            }
+            for (uint32_t p = u0 + kWave + tid; p < u1; p += kWave) {
                 const uint32_t r = ui[p];
                 const double t = ux[p] * zk;
                 y[r] = y[r] - t;
             }
-        u1 = u0;
+        }
     }
     for (uint32_t j = tid; j < n; j += kWave) {
         dx[q[j]] = y[j];
@@ -459,10 +661,12 @@ __device__ void priv_solve(const PrivLane &L, const uint32_t *q_, const double *
 
 // A fresh factor's values into the LDS tail when they fit there (all
 // threads; the factor wrote them to global memory).
-__device__ inline void priv_load(const PrivLane &G, double *tail, uint32_t tail_n, uint32_t tid)
+__device__ inline void priv_load(const PrivLane &G, const uint32_t *col_ptr, const uint32_t *q, double *tail,
+                                 uint32_t tail_n, uint32_t tid)
 {
     PrivLane L = G;
     if (!priv_resident(L, tail, tail_n)) return;
+    priv_stage_static(L, col_ptr, q, tid);
     const uint32_t nl = uni(G.hdr[0]), nu = uni(G.hdr[1]);
     for (uint32_t p = tid; p < nl; p += kWave) L.lx[p] = G.lx[p];
     for (uint32_t p = tid; p < nu; p += kWave) L.ux[p] = G.ux[p];
@@ -481,29 +685,36 @@ __device__ __forceinline__ int priv_factor_at(const PrivLU &P, uint32_t lane, co
                                               uint32_t lds_doubles, double *tail, uint32_t tail_n, uint32_t tid)
 {
     const PrivLane L = priv_lane(P, lane, lds, lds_doubles);
-    int rc = kPlOk;
-    if (tid == 0) rc = priv_factor(L, col_ptr, row_idx, q, 1e-3);
-    rc = (int)uni((uint32_t)rc);
+    // the reach's work arrays in the LDS tail when they fit (its values are
+    // loaded after the factor)
+    uint32_t *ws = 5 * (uint64_t)L.n <= 2 * (uint64_t)tail_n ? reinterpret_cast<uint32_t *>(tail) : nullptr;
+    const int rc = priv_factor_w(L, col_ptr, row_idx, q, 1e-3, ws, tid);
     wsync();
-    if (rc == kPlOk) priv_load(L, tail, tail_n, tid);
+    if (rc == kPlOk) priv_load(L, col_ptr, q, tail, tail_n, tid);
     return rc;
 }
 
 // The replay of the lane's own factors, and on a failed peel test its full
 // factor: kPlOk, kPlSingular or kPlFull (uniform). fresh: the last factor
 // met a singular matrix and left no usable LU.
+// The replay alone: true when a peel test fails (uniform).
+__device__ __forceinline__ bool priv_replay_at(const PrivLU &P, uint32_t lane, const uint32_t *col_ptr,
+                                               const uint32_t *q, double *lds, uint32_t lds_doubles, double *tail,
+                                               uint32_t tail_n, uint32_t tid)
+{
+    PrivLane L = priv_lane(P, lane, lds, lds_doubles);
+    const bool failed = priv_resident(L, tail, tail_n) ? priv_replay<true>(L, col_ptr, q, 1e-12, 1.0 / 1e-3, tid)
+                                                       : priv_replay<false>(L, col_ptr, q, 1e-12, 1.0 / 1e-3, tid);
+    wsync();
+    return failed;
+}
+
 __device__ __forceinline__ int priv_update_at(const PrivLU &P, uint32_t lane, const uint32_t *col_ptr,
                                               const uint32_t *row_idx, const uint32_t *q, double *lds,
                                               uint32_t lds_doubles, double *tail, uint32_t tail_n, uint32_t tid,
                                               bool fresh = false)
 {
-    if (!fresh) {
-        PrivLane L = priv_lane(P, lane, lds, lds_doubles);
-        const bool failed = priv_resident(L, tail, tail_n) ? priv_replay<true>(L, col_ptr, q, 1e-12, 1.0 / 1e-3, tid)
-                                                           : priv_replay<false>(L, col_ptr, q, 1e-12, 1.0 / 1e-3, tid);
-        if (!failed) return kPlOk;
-        wsync();
-    }
+    if (!fresh && !priv_replay_at(P, lane, col_ptr, q, lds, lds_doubles, tail, tail_n, tid)) return kPlOk;
     return priv_factor_at(P, lane, col_ptr, row_idx, q, lds, lds_doubles, tail, tail_n, tid);
 }
 
@@ -517,15 +728,19 @@ __device__ __forceinline__ void priv_solve_at(const PrivLU &P, uint32_t lane, co
     else priv_solve<false>(L, q, rhs, dx, tid);
 }
 
-// A launch's start for a lane already on its own LU: w cleared and the
-// pattern staged (all threads); false when it does not fit. The values are
-// the next replay's (a replay always precedes the next solve).
-__device__ __forceinline__ bool priv_enter(const PrivLU &P, uint32_t lane, double *lds, uint32_t lds_doubles,
-                                           uint32_t tid, uint32_t wave)
+// A launch's start for a lane already on its own LU: w cleared, the
+// pattern staged, and for a resident lane its column ranges and flags (all
+// threads); false when it does not fit. The values are the next replay's
+// (a replay always precedes the next solve).
+__device__ __forceinline__ bool priv_enter(const PrivLU &P, uint32_t lane, const uint32_t *col_ptr,
+                                           const uint32_t *q, double *lds, uint32_t lds_doubles, double *tail,
+                                           uint32_t tail_n, uint32_t tid)
 {
-    const PrivLane L = priv_lane(P, lane, lds, lds_doubles);
-    for (uint32_t i = tid; i < L.n; i += wave) L.w[i] = 0.0;
-    return priv_stage(L, tid, wave);
+    PrivLane L = priv_lane(P, lane, lds, lds_doubles);
+    for (uint32_t i = tid; i < L.n; i += kWave) L.w[i] = 0.0;
+    if (!priv_stage(L, tid, kWave)) return false;
+    if (priv_resident(L, tail, tail_n)) priv_stage_static(L, col_ptr, q, tid);
+    return true;
 }
 
 }  // namespace esl

@@ -1864,6 +1864,23 @@ u32 lds_bytes_for(u32 tv_cap, u32 n)
     return (u32)((size_t)esl::lds_doubles(tv_cap, n) * sizeof(double));
 }
 
+u32 device_cu_count();
+
+// LDS of one lane in a launch of the own-LU kernels over `grid` lanes: when
+// the launch leaves LDS idle (few lanes per CU), room past the working set
+// for the lanes' own factor values (twice the lead factor's entries, the
+// diagonal, the column ranges and flags), so that their replays and solves
+// run out of LDS; otherwise the working set alone.
+u32 own_lds_bytes(u32 lds, u32 grid, u32 n, u32 nnz_l, u32 nnz_u)
+{
+    const u32 cu = device_cu_count();
+    if (cu == 0 || grid == 0) return lds;
+    const size_t per_cu = (grid + cu - 1) / cu;
+    const size_t want = lds + 8 * (2 * ((size_t)nnz_l + nnz_u) + 3 * (size_t)n + 16);
+    const size_t room = std::min<size_t>(65536, 160 * 1024 / per_cu);
+    return want <= room ? (u32)want : lds;
+}
+
 
 
 // Assembly reduce streams for one tape: each chunk's reduce tasks with their
@@ -2909,7 +2926,8 @@ esl_status launch_op(Engine &e, const HostOpArgs &d, u32 grid, hipStream_t s)
         return ESL_FAILED;
     }
     HIPCHK(esl_kernel_attrs(std::max<u32>(lds, 65536)));
-    HIPCHK(esl_launch_op_newton(&d, grid, lds, s));
+    const u32 lds_run = d.priv.cap ? std::max(lds, own_lds_bytes(lds, grid, e.c.n, e.lead_nl, e.lead_nu)) : lds;
+    HIPCHK(esl_launch_op_newton(&d, grid, lds_run, s));
     return ESL_OK;
 }
 
@@ -3367,7 +3385,9 @@ hipError_t launch_tran(Engine &e, const esl::TranArgs &t, u32 grid, hipStream_t 
         const esl::BigArgs b = big_args(e, t, grid, 1);
         return esl_launch_big(&b, s);
     }
-    return esl_launch_tran_steps(&t, grid, tran_lds_bytes(e.c, t.op.tv_cap), s);
+    const u32 lds = tran_lds_bytes(e.c, t.op.tv_cap);
+    const u32 lds_run = t.op.priv.cap ? std::max(lds, own_lds_bytes(lds, grid, e.c.n, e.lead_nl, e.lead_nu)) : lds;
+    return esl_launch_tran_steps(&t, grid, lds_run, s);
 }
 }  // namespace
 
@@ -3529,7 +3549,13 @@ esl_status tran_adaptive_rounds(Engine &E, const esl::TranArgs &ta, const esl::P
         if (r != ESL_OK) return r;
         float ms = 0;
         HIPCHK(hipEventRecord(e->ev0, s));
-        HIPCHK(esl_launch_tran_steps(&t, (u32)run.size(), tran_lds_bytes(e->c, cap), s));
+        {
+            const u32 lds = tran_lds_bytes(e->c, cap);
+            const u32 lds_run = t.op.priv.cap
+                                    ? std::max(lds, own_lds_bytes(lds, (u32)run.size(), n, e->lead_nl, e->lead_nu))
+                                    : lds;
+            HIPCHK(esl_launch_tran_steps(&t, (u32)run.size(), lds_run, s));
+        }
         HIPCHK(hipEventRecord(e->ev1, s));
         HIPCHK(hipEventSynchronize(e->ev1));
         HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
@@ -3846,7 +3872,8 @@ esl_status tran_run(esl_engine *h, const double *values, size_t count, const esl
             for (u32 b = b0; b < B; b++) {
                 esl::TranArgs t = block_args(b, cap, 0);
                 t.op.priv = own;
-                HIPCHK(esl_launch_tran_steps(&t, (u32)again.size(), lds, s));
+                const u32 lds_run = std::max(lds, own_lds_bytes(lds, (u32)again.size(), n, e->lead_nl, e->lead_nu));
+                HIPCHK(esl_launch_tran_steps(&t, (u32)again.size(), lds_run, s));
             }
             HIPCHK(hipEventRecord(e->ev1, s));
             HIPCHK(hipEventSynchronize(e->ev1));
@@ -4034,19 +4061,19 @@ void model_job_ns(const BatchItemRun &r, double out[2])
 }
 
 struct Sched {
-    vector<vector<esl::BatchJob>> block, wave;   // [grid], [grid * kCrew]
+    vector<esl::BatchJob> block, wave;   // each tiling's jobs, longest predicted first
     double makespan_ns = 0;
 };
 
-// Deals the jobs longest first: lane-per-block jobs to the workgroup whose
-// block phase ends first, then lane-per-wavefront jobs to the wavefront
-// that is free first (its workgroup's block phase plus its own jobs).
+// Orders the jobs longest first, as the kernel claims them, and predicts
+// the makespan of that claiming: lane-per-block jobs go to the workgroup
+// whose block phase ends first, then lane-per-wavefront jobs to the
+// wavefront that is free first (its workgroup's block phase plus its own
+// jobs).
 Sched schedule(const vector<std::pair<esl::BatchJob, std::pair<u32, double>>> &jobs, u32 grid)
 {
     // jobs: (job, (tiling 0 wave / 1 block, predicted ns))
     Sched S;
-    S.block.assign(grid, {});
-    S.wave.assign((size_t)grid * esl::kCrew, {});
     vector<size_t> order(jobs.size());
     for (size_t i = 0; i < order.size(); i++) order[i] = i;
     std::stable_sort(order.begin(), order.end(),
@@ -4059,7 +4086,7 @@ Sched schedule(const vector<std::pair<esl::BatchJob, std::pair<u32, double>>> &j
         if (jobs[i].second.first != 1) continue;
         QE top = qb.top();
         qb.pop();
-        S.block[top.second].push_back(jobs[i].first);
+        S.block.push_back(jobs[i].first);
         top.first += jobs[i].second.second;
         bt[top.second] = top.first;
         qb.push(top);
@@ -4070,7 +4097,7 @@ Sched schedule(const vector<std::pair<esl::BatchJob, std::pair<u32, double>>> &j
         if (jobs[i].second.first != 0) continue;
         QE top = qw.top();
         qw.pop();
-        S.wave[top.second].push_back(jobs[i].first);
+        S.wave.push_back(jobs[i].first);
         top.first += jobs[i].second.second;
         qw.push(top);
     }
@@ -4420,7 +4447,7 @@ void job_need(const Compiled &c, u32 op_cap, u32 tran_cap, u32 need[2])
 // Uploads the argument sets and the dealt jobs and runs one persistent
 // launch; per-job durations come back in ticks when asked.
 struct BatchLaunch {
-    DevBuf ops, trans, bj, wj, bptr, wptr, bt, wt;
+    DevBuf ops, trans, bj, wj, next, bt, wt;
 };
 
 esl_status batch_launch(BatchLaunch &L, const vector<esl::OpArgs> &ops, const vector<esl::TranArgs> &trans,
@@ -4430,15 +4457,8 @@ esl_status batch_launch(BatchLaunch &L, const vector<esl::OpArgs> &ops, const ve
 {
     bj.clear();
     wj.clear();
-    vector<u32> bptr(1, 0), wptr(1, 0);
-    for (auto &v : S.block) {
-        bj.insert(bj.end(), v.begin(), v.end());
-        bptr.push_back((u32)bj.size());
-    }
-    for (auto &v : S.wave) {
-        wj.insert(wj.end(), v.begin(), v.end());
-        wptr.push_back((u32)wj.size());
-    }
+    bj = S.block;
+    wj = S.wave;
     // every index the kernel follows, checked here
     for (auto *v : {&bj, &wj})
         for (auto &j : *v) {
@@ -4453,8 +4473,7 @@ esl_status batch_launch(BatchLaunch &L, const vector<esl::OpArgs> &ops, const ve
     HIPCHK(L.trans.ensure(std::max<size_t>(1, trans.size()) * sizeof(esl::TranArgs)));
     HIPCHK(L.bj.ensure(std::max<size_t>(1, bj.size()) * sizeof(esl::BatchJob)));
     HIPCHK(L.wj.ensure(std::max<size_t>(1, wj.size()) * sizeof(esl::BatchJob)));
-    HIPCHK(L.bptr.ensure(bptr.size() * 4));
-    HIPCHK(L.wptr.ensure(wptr.size() * 4));
+    HIPCHK(L.next.ensure(2 * 4));
     HIPCHK(L.bt.ensure(std::max<size_t>(1, bj.size()) * 8));
     HIPCHK(L.wt.ensure(std::max<size_t>(1, wj.size()) * 8));
     HIPCHK(hipMemcpyAsync(L.ops.p, ops.data(), ops.size() * sizeof(esl::OpArgs), hipMemcpyHostToDevice, s));
@@ -4462,15 +4481,15 @@ esl_status batch_launch(BatchLaunch &L, const vector<esl::OpArgs> &ops, const ve
         HIPCHK(hipMemcpyAsync(L.trans.p, trans.data(), trans.size() * sizeof(esl::TranArgs), hipMemcpyHostToDevice, s));
     if (!bj.empty()) HIPCHK(hipMemcpyAsync(L.bj.p, bj.data(), bj.size() * sizeof(esl::BatchJob), hipMemcpyHostToDevice, s));
     if (!wj.empty()) HIPCHK(hipMemcpyAsync(L.wj.p, wj.data(), wj.size() * sizeof(esl::BatchJob), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(L.bptr.p, bptr.data(), bptr.size() * 4, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(L.wptr.p, wptr.data(), wptr.size() * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemsetAsync(L.next.p, 0, 2 * 4, s));
     esl::BatchArgs B{};
     B.ops = L.ops.as<esl::OpArgs>();
     B.trans = L.trans.as<esl::TranArgs>();
     B.block_jobs = L.bj.as<esl::BatchJob>();
-    B.block_ptr = L.bptr.as<u32>();
+    B.n_block = (u32)bj.size();
     B.wave_jobs = L.wj.as<esl::BatchJob>();
-    B.wave_ptr = L.wptr.as<u32>();
+    B.n_wave = (u32)wj.size();
+    B.next = L.next.as<u32>();
     B.wave_doubles = lds / 8 / esl::kCrew;
     B.block_ticks = ticks ? L.bt.as<uint64_t>() : nullptr;
     B.wave_ticks = ticks ? L.wt.as<uint64_t>() : nullptr;

@@ -706,7 +706,7 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
     bool staged = true;
     if (priv) {
         // own LU: its dense column and pattern live at tv
-        staged = priv_enter(a.priv, lane, tv, a.tv_cap, tid, kWave);
+        staged = priv_enter(a.priv, lane, dp.csc_ptr, dp.colq, tv, a.tv_cap, tail, tail_n, tid);
         wsync();
     }
     if (!staged) {
@@ -753,7 +753,7 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
     // a lane the host found singular in its re-pivot resumes only to go on
     // with its ladder: its run ended singular at the saved iterate
     bool host_singular = ladder && a.mode == 2 && uni(a.status[lane]) == kSingular;
-    unsigned long long pc[kProfPhases] = {0, 0, 0, 0};
+    unsigned long long pc[kProfPhases] = {};
     unsigned long long tc = clock64();
     auto lap = [&](int ph) {
         if (a.prof) {
@@ -841,8 +841,13 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
             int rc = kPlOk;
             const bool fresh = (L.flags & kLadNeedFactor) != 0;
             L.flags &= ~kLadNeedFactor;
-            rc = priv_update_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq, tv, a.tv_cap, tail, tail_n, tid,
-                                fresh);
+            const bool failed =
+                fresh || priv_replay_at(a.priv, lane, dp.csc_ptr, dp.colq, tv, a.tv_cap, tail, tail_n, tid);
+            lap(1);
+            if (failed) {
+                rc = priv_factor_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq, tv, a.tv_cap, tail, tail_n, tid);
+                lap(4);
+            }
             wsync();
             if (rc == kPlSingular) exit_status = kSingular;
             else if (rc == kPlFull) exit_status = kRepivot;
@@ -1012,7 +1017,7 @@ __global__ __launch_bounds__(kWave, 2) void op_newton(OpArgs a)
 __global__ __launch_bounds__(kWave) void op_newton_own(OpArgs a)
 {
     extern __shared__ double sm[];
-    op_lane<true>(a, uni(a.lane_ids[blockIdx.x]), sm, 1, 0);
+    op_lane<true>(a, uni(a.lane_ids[blockIdx.x]), sm, 1, a.lane_lds);
 }
 
 // ---- transient lanes ----
@@ -1204,7 +1209,7 @@ __device__ __forceinline__ void tran_lane(const TranArgs &ta, uint32_t lane, dou
     bool staged = true;
     if (priv) {
         // own LU: its dense column and pattern live at tv
-        staged = priv_enter(a.priv, lane, tv, a.tv_cap, tid, kWave);
+        staged = priv_enter(a.priv, lane, dp.csc_ptr, dp.colq, tv, a.tv_cap, tail, tail_n, tid);
         wsync();
     }
     if (!staged) {
@@ -1575,7 +1580,7 @@ __global__ __launch_bounds__(kWave, 2) void tran_steps(TranArgs ta)
 __global__ __launch_bounds__(kWave, 2) void tran_steps_own(TranArgs ta)
 {
     extern __shared__ double sm[];
-    tran_lane<true>(ta, uni(ta.op.lane_ids[blockIdx.x]), sm, 1, 0);
+    tran_lane<true>(ta, uni(ta.op.lane_ids[blockIdx.x]), sm, 1, ta.op.lane_lds);
 }
 
 // ---- the persistent batch ----
@@ -1627,13 +1632,22 @@ __device__ __forceinline__ void follow_job(const BatchArgs &b, const BatchJob &j
                 wv, threadIdx.x % kWave);
 }
 
+// Jobs are claimed at run time, longest predicted first: a workgroup takes
+// the next lane-per-block job while any is left, then each of its
+// wavefronts takes the next lane-per-wavefront job, so a job that runs
+// longer than predicted delays only the slot that holds it.
 __global__ __launch_bounds__(kWave * kCrew) void batch_run(BatchArgs b)
 {
     extern __shared__ double sm[];
+    __shared__ uint32_t claim;
     const uint32_t wv = threadIdx.x / kWave, tid = threadIdx.x % kWave;
-    const uint32_t g = blockIdx.x;
     // lane per block: the whole workgroup on one lane at a time
-    for (uint32_t j = b.block_ptr[g]; j < b.block_ptr[g + 1]; j++) {
+    for (;;) {
+        if (threadIdx.x == 0) claim = atomicAdd(b.next, 1u);
+        __syncthreads();
+        const uint32_t j = claim;
+        __syncthreads();
+        if (j >= b.n_block) break;
         const BatchJob jb = uni_job(b.block_jobs[j]);
         if (wv == 0) {
             const uint64_t t0 = wall_clock64();
@@ -1645,9 +1659,12 @@ __global__ __launch_bounds__(kWave * kCrew) void batch_run(BatchArgs b)
         }
     }
     // lane per wavefront: each wavefront on its own lanes in its share of the LDS
-    const uint32_t w = g * kCrew + wv;
     double *mine = sm + (size_t)wv * b.wave_doubles;
-    for (uint32_t j = b.wave_ptr[w]; j < b.wave_ptr[w + 1]; j++) {
+    for (;;) {
+        uint32_t j = 0;
+        if (tid == 0) j = atomicAdd(b.next + 1, 1u);
+        j = uni(j);
+        if (j >= b.n_wave) break;
         const uint64_t t0 = wall_clock64();
         run_job(b, uni_job(b.wave_jobs[j]), mine, 1, b.wave_doubles);
         if (b.wave_ticks && tid == 0) b.wave_ticks[j] = wall_clock64() - t0;
@@ -1710,8 +1727,11 @@ extern "C" hipError_t esl_launch_op_newton(const esl::OpArgs *a, uint32_t grid, 
 {
     if (a->dump)
         hipLaunchKernelGGL(esl::op_dump, dim3(grid), dim3(esl::kWave), lds_bytes, s, *a);
-    else if (a->priv.cap)
-        hipLaunchKernelGGL(esl::op_newton_own, dim3(grid), dim3(esl::kWave), lds_bytes, s, *a);
+    else if (a->priv.cap) {
+        esl::OpArgs b = *a;
+        b.lane_lds = lds_bytes / 8;
+        hipLaunchKernelGGL(esl::op_newton_own, dim3(grid), dim3(esl::kWave), lds_bytes, s, b);
+    }
     else
         hipLaunchKernelGGL(esl::op_newton, dim3(grid), dim3(esl::kWave), lds_bytes, s, *a);
     return hipGetLastError();
@@ -1726,8 +1746,11 @@ extern "C" hipError_t esl_launch_tape_replay(const esl::ReplayArgs *r, uint32_t 
 
 extern "C" hipError_t esl_launch_tran_steps(const esl::TranArgs *ta, uint32_t grid, uint32_t lds_bytes, hipStream_t s)
 {
-    if (ta->op.priv.cap)
-        hipLaunchKernelGGL(esl::tran_steps_own, dim3(grid), dim3(esl::kWave), lds_bytes, s, *ta);
+    if (ta->op.priv.cap) {
+        esl::TranArgs b = *ta;
+        b.op.lane_lds = lds_bytes / 8;
+        hipLaunchKernelGGL(esl::tran_steps_own, dim3(grid), dim3(esl::kWave), lds_bytes, s, b);
+    }
     else
         hipLaunchKernelGGL(esl::tran_steps, dim3(grid), dim3(esl::kWave), lds_bytes, s, *ta);
     return hipGetLastError();

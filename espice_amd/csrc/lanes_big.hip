@@ -373,7 +373,7 @@ __device__ __forceinline__ void big_op_lane(const BigArgs &b, uint32_t g, uint32
         k0 = rs.k0;
     }
     uint32_t status = kNoConvergence, iters = a.max_iter;
-    unsigned long long pc[kProfPhases] = {0, 0, 0, 0};
+    unsigned long long pc[kProfPhases] = {};
     unsigned long long tc = clock64();
     auto lap = [&](int ph) {
         if (a.prof) {
@@ -507,7 +507,7 @@ __device__ __forceinline__ void big_tran_lane(const BigArgs &b, uint32_t g, uint
     double limit = rs.limit;
     uint32_t status = kTranRunning;
     uint32_t total_it = a.iterations[lane];
-    unsigned long long pc[kProfPhases] = {0, 0, 0, 0};
+    unsigned long long pc[kProfPhases] = {};
     unsigned long long tc = clock64();
     auto lap = [&](int ph) {
         if (a.prof) {

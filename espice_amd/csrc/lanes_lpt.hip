@@ -74,7 +74,7 @@ __global__ __launch_bounds__(256) void op_newton_lpt(LptArgs a)
     bool init_fix = a.mode == 0;
     double limit = 1e9;
     uint32_t status = kNoConvergence, iters = a.max_iter;
-    unsigned long long pc[kProfPhases] = {0, 0, 0, 0};
+    unsigned long long pc[kProfPhases] = {};
     unsigned long long tc = clock64();
     auto lap = [&](int ph) {
         if (a.prof) {

@@ -387,9 +387,9 @@ esl_status esl_tran_adaptive_solve(esl_engine *e, const double *values, size_t c
 /* ---- Persistent uneven batches (no reference counterpart: the reference
  * forms one batch per template and options, executor.zig:480-520,
  * session.zig:562-590). Points of several templates, .op and transient
- * mixed, run in ONE launch of a persistent kernel: the host deals every
- * point (a job) to a workgroup or a wavefront ahead of time, longest first,
- * so no job is claimed at run time. A job runs lane per wavefront (four
+ * mixed, run in ONE launch of a persistent kernel: the host orders every
+ * point (a job) longest predicted first, and workgroups and wavefronts
+ * claim the next job as they finish one. A job runs lane per wavefront (four
  * lanes per workgroup, each in a quarter of the LDS) or lane per block (one
  * lane on the workgroup: the device evaluation spread over its four
  * wavefronts, the factor and solves in the first); the autotuner picks the
@@ -474,9 +474,10 @@ typedef struct {
 esl_status esl_last_timing(esl_engine *e, esl_timing *out);
 
 /* Development hook (no counterpart in the reference): enable != 0 makes the
- * following esl_op_solve calls count shader cycles per lane in four phases
- * (assembly, refactor, solve, update + limiting + gates); out receives the
- * last solve's counters [count][4] when capacity allows. */
+ * following esl_op_solve calls count shader cycles per lane in five phases
+ * (assembly, refactor, solve, update + limiting + gates, the full factors of
+ * lanes on their own LU); out receives the last solve's counters [count][5]
+ * when capacity allows. */
 esl_status esl_debug_profile(esl_engine *e, int enable, unsigned long long *out,
                              size_t capacity);
 
