@@ -689,7 +689,9 @@ def run_mixed(args):
             f"setup {rep.setup_ms:.1f} ms")
     fams = {}
     for (g, fam), r in zip(meta, res):
-        f = fams.setdefault(g["family"], {"instances": 0, "converged": 0, "templates": 0, "iterations": 0})
+        f = fams.setdefault(g["family"], {"instances": 0, "converged": 0, "templates": 0, "iterations": 0,
+                                          "max_iterations": 0})
+        f["max_iterations"] = max(f["max_iterations"], int(r["iterations"].max()))
         f["instances"] += len(r["reasons"])
         f["converged"] += int((r["reasons"] == 0).sum())
         f["templates"] += 1

@@ -449,9 +449,9 @@ struct BatchArgs {
     const OpArgs *ops;          // [.op argument sets]
     const TranArgs *trans;      // [transient argument sets]
     const BatchJob *block_jobs; // [n_block], longest predicted first
-    const BatchJob *wave_jobs;  // [n_wave], longest predicted first
-    uint32_t n_block, n_wave;
-    uint32_t *next;             // [2] claim counters (block, wave), zero at launch
+    const BatchJob *wave_jobs;  // [n_wave], longest predicted first; the first n_lead run before the block jobs
+    uint32_t n_block, n_wave, n_lead, pad0;
+    uint32_t *next;             // [3] claim counters (block, wave, lead): 0, n_lead, 0 at launch
     uint32_t wave_doubles;      // LDS doubles of one wavefront in the wave phase
     uint32_t pad;
     uint64_t *block_ticks, *wave_ticks;   // optional: each job's duration (wall-clock ticks)

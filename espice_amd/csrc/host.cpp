@@ -1664,6 +1664,7 @@ struct Engine {
     // 0 before tuning) and the measured mean job time per tiling (ns)
     u32 bt_tiling = 0;
     double bt_ns[2] = {0.0, 0.0};
+    double bt_max_ns[2] = {0.0, 0.0};   // the slowest measured job per tiling
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     esl_timing timing{};
@@ -4062,6 +4063,7 @@ void model_job_ns(const BatchItemRun &r, double out[2])
 
 struct Sched {
     vector<esl::BatchJob> block, wave;   // each tiling's jobs, longest predicted first
+    u32 n_lead = 0;                      // wave jobs that run before the block jobs
     double makespan_ns = 0;
 };
 
@@ -4070,14 +4072,16 @@ struct Sched {
 // whose block phase ends first, then lane-per-wavefront jobs to the
 // wavefront that is free first (its workgroup's block phase plus its own
 // jobs).
-Sched schedule(const vector<std::pair<esl::BatchJob, std::pair<u32, double>>> &jobs, u32 grid)
+Sched schedule(const vector<std::pair<esl::BatchJob, std::pair<u32, double>>> &jobs, u32 grid,
+               const vector<double> *keys = nullptr)
 {
-    // jobs: (job, (tiling 0 wave / 1 block, predicted ns))
+    // jobs: (job, (tiling 0 wave / 1 block, predicted ns)); keys: the claim
+    // order's key per job (default the predicted time), largest first
     Sched S;
     vector<size_t> order(jobs.size());
     for (size_t i = 0; i < order.size(); i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](size_t x, size_t y) { return jobs[x].second.second > jobs[y].second.second; });
+    auto key = [&](size_t i) { return keys ? (*keys)[i] : jobs[i].second.second; };
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return key(x) > key(y); });
     vector<double> bt(grid, 0.0);
     using QE = std::pair<double, u32>;
     std::priority_queue<QE, vector<QE>, std::greater<QE>> qb;
@@ -4106,6 +4110,15 @@ Sched schedule(const vector<std::pair<esl::BatchJob, std::pair<u32, double>>> &j
         qw.pop();
     }
     for (double v : bt) S.makespan_ns = std::max(S.makespan_ns, v);
+    // lead jobs: with measured keys, the wave jobs whose key reaches half
+    // the predicted makespan start before the block phase (at most one per
+    // wavefront), so that a slow point does not begin after it
+    if (keys)
+        for (size_t i : order) {
+            if (jobs[i].second.first != 0) continue;
+            if (S.n_lead >= (size_t)grid * esl::kCrew || !((*keys)[i] >= 0.5 * S.makespan_ns)) break;
+            S.n_lead++;
+        }
     return S;
 }
 
@@ -4473,7 +4486,7 @@ esl_status batch_launch(BatchLaunch &L, const vector<esl::OpArgs> &ops, const ve
     HIPCHK(L.trans.ensure(std::max<size_t>(1, trans.size()) * sizeof(esl::TranArgs)));
     HIPCHK(L.bj.ensure(std::max<size_t>(1, bj.size()) * sizeof(esl::BatchJob)));
     HIPCHK(L.wj.ensure(std::max<size_t>(1, wj.size()) * sizeof(esl::BatchJob)));
-    HIPCHK(L.next.ensure(2 * 4));
+    HIPCHK(L.next.ensure(3 * 4));
     HIPCHK(L.bt.ensure(std::max<size_t>(1, bj.size()) * 8));
     HIPCHK(L.wt.ensure(std::max<size_t>(1, wj.size()) * 8));
     HIPCHK(hipMemcpyAsync(L.ops.p, ops.data(), ops.size() * sizeof(esl::OpArgs), hipMemcpyHostToDevice, s));
@@ -4481,7 +4494,11 @@ esl_status batch_launch(BatchLaunch &L, const vector<esl::OpArgs> &ops, const ve
         HIPCHK(hipMemcpyAsync(L.trans.p, trans.data(), trans.size() * sizeof(esl::TranArgs), hipMemcpyHostToDevice, s));
     if (!bj.empty()) HIPCHK(hipMemcpyAsync(L.bj.p, bj.data(), bj.size() * sizeof(esl::BatchJob), hipMemcpyHostToDevice, s));
     if (!wj.empty()) HIPCHK(hipMemcpyAsync(L.wj.p, wj.data(), wj.size() * sizeof(esl::BatchJob), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemsetAsync(L.next.p, 0, 2 * 4, s));
+    {
+        const u32 next0[3] = {0, S.n_lead, 0};
+        HIPCHK(hipMemcpyAsync(L.next.p, next0, sizeof(next0), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));   // next0 lives on this stack
+    }
     esl::BatchArgs B{};
     B.ops = L.ops.as<esl::OpArgs>();
     B.trans = L.trans.as<esl::TranArgs>();
@@ -4489,6 +4506,7 @@ esl_status batch_launch(BatchLaunch &L, const vector<esl::OpArgs> &ops, const ve
     B.n_block = (u32)bj.size();
     B.wave_jobs = L.wj.as<esl::BatchJob>();
     B.n_wave = (u32)wj.size();
+    B.n_lead = S.n_lead;
     B.next = L.next.as<u32>();
     B.wave_doubles = lds / 8 / esl::kCrew;
     B.block_ticks = ticks ? L.bt.as<uint64_t>() : nullptr;
@@ -4677,15 +4695,20 @@ extern "C" esl_status esl_batch_solve(const esl_batch_item *items, size_t n_item
     vector<esl::TranArgs> trans;
     batch_args(runs, ops, trans);
     // 4. jobs and the schedule
+    // claim order: items whose slowest measured point ran longest first
+    // (a template's mean hides the few points that climb the .op ladder or
+    // re-pivot often), then by predicted time
     vector<std::pair<esl::BatchJob, std::pair<u32, double>>> jobs;
+    vector<double> keys;
     for (auto &r : runs) {
         if (r.lead_singular) continue;
         for (size_t l = 0; l < r.it->count; l++) {
             const u32 t = r.tiling == 2 ? (u32)(l & 1) : r.tiling;
             jobs.push_back({esl::BatchJob{r.op_idx, r.tran_idx, (u32)l, 0}, {t, r.est_ns[t]}});
+            keys.push_back(std::max(r.est_ns[t], r.e->bt_max_ns[t]));
         }
     }
-    Sched S = schedule(jobs, grid);
+    Sched S = schedule(jobs, grid, &keys);
     for (auto &r : runs) HIPCHK(hipStreamSynchronize(r.e->stream));
     const double setup_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
@@ -4701,15 +4724,20 @@ extern "C" esl_status esl_batch_solve(const esl_batch_item *items, size_t n_item
     {
         const double ns_per_tick = 1e6 / wall_clock_khz();
         std::map<u32, std::pair<double, u32>> sums[2];   // op_idx -> (ns, jobs)
+        std::map<u32, double> slowest[2];                 // op_idx -> the slowest job's ns
         for (size_t j = 0; j < bj.size(); j++) {
             auto &p = sums[1][bj[j].op];
             p.first += bt[j] * ns_per_tick;
             p.second++;
+            double &m = slowest[1][bj[j].op];
+            m = std::max(m, bt[j] * ns_per_tick);
         }
         for (size_t j = 0; j < wj.size(); j++) {
             auto &p = sums[0][wj[j].op];
             p.first += wt[j] * ns_per_tick;
             p.second++;
+            double &m = slowest[0][wj[j].op];
+            m = std::max(m, wt[j] * ns_per_tick);
         }
         for (auto &r : runs) {
             if (r.lead_singular) continue;
@@ -4717,6 +4745,7 @@ extern "C" esl_status esl_batch_solve(const esl_batch_item *items, size_t n_item
                 auto f = sums[k].find(r.op_idx);
                 if (f != sums[k].end() && f->second.second) {
                     r.e->bt_ns[k] = f->second.first / f->second.second;
+                    r.e->bt_max_ns[k] = slowest[k][r.op_idx];
                     r.est_ns[k] = r.e->bt_ns[k];
                 }
             }

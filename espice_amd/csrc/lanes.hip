@@ -1632,15 +1632,28 @@ __device__ __forceinline__ void follow_job(const BatchArgs &b, const BatchJob &j
                 wv, threadIdx.x % kWave);
 }
 
-// Jobs are claimed at run time, longest predicted first: a workgroup takes
-// the next lane-per-block job while any is left, then each of its
-// wavefronts takes the next lane-per-wavefront job, so a job that runs
-// longer than predicted delays only the slot that holds it.
+// Jobs are claimed at run time, longest predicted first: the wavefronts
+// first take the lead jobs (lane-per-wavefront jobs of items whose slowest
+// point outlasts the rest of the batch), then a workgroup takes the next
+// lane-per-block job while any is left, then each of its wavefronts takes
+// the next lane-per-wavefront job, so a job that runs longer than
+// predicted delays only the slot that holds it.
 __global__ __launch_bounds__(kWave * kCrew) void batch_run(BatchArgs b)
 {
     extern __shared__ double sm[];
     __shared__ uint32_t claim;
     const uint32_t wv = threadIdx.x / kWave, tid = threadIdx.x % kWave;
+    double *mine = sm + (size_t)wv * b.wave_doubles;
+    for (;;) {
+        uint32_t j = 0;
+        if (tid == 0) j = atomicAdd(b.next + 2, 1u);
+        j = uni(j);
+        if (j >= b.n_lead) break;
+        const uint64_t t0 = wall_clock64();
+        run_job(b, uni_job(b.wave_jobs[j]), mine, 1, b.wave_doubles);
+        if (b.wave_ticks && tid == 0) b.wave_ticks[j] = wall_clock64() - t0;
+    }
+    __syncthreads();
     // lane per block: the whole workgroup on one lane at a time
     for (;;) {
         if (threadIdx.x == 0) claim = atomicAdd(b.next, 1u);
@@ -1659,7 +1672,6 @@ __global__ __launch_bounds__(kWave * kCrew) void batch_run(BatchArgs b)
         }
     }
     // lane per wavefront: each wavefront on its own lanes in its share of the LDS
-    double *mine = sm + (size_t)wv * b.wave_doubles;
     for (;;) {
         uint32_t j = 0;
         if (tid == 0) j = atomicAdd(b.next + 1, 1u);
