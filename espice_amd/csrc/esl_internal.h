@@ -347,7 +347,8 @@ struct OpArgs {
     double *a64;                // [N][dp.res_total + 64]
     // optional: [N][kProfPhases] shader cycles per phase (assemble, refactor,
     // solve, update + limits + gates, the own LU's full factors), summed over
-    // the lane's iterations
+    // the lane's iterations, and the count of own-LU replays whose values
+    // were in LDS
     unsigned long long *prof;
     // the lanes' own LU for re-pivots on the device (FP64 lanes)
     PrivLU priv;
@@ -374,7 +375,7 @@ struct OpArgs {
     LadderState *lad;
     double *xgood;
 };
-constexpr int kProfPhases = 5;
+constexpr int kProfPhases = 6;
 
 // Transient lanes (tran_steps): fixed-step integration, one wavefront per
 // lane, state between launches in HBM. Step s (0-based) ends at

@@ -30,7 +30,8 @@ struct PrivLane {
     uint32_t *lp, *up, *li, *ui;        // LDS
     uint32_t *glp, *gup, *gli, *gui;    // global copies
     double *lx, *ux, *udiag, *w, *rscale, *ap;
-    uint32_t *kp, *fl;                  // LDS, values resident: column ranges and flags in pivot order
+    uint32_t *cr;                       // LDS, values resident: column records [n + 1][4]
+    uint32_t *ue;                       // LDS, values resident: U-entry records [nnz_u][2]
 };
 
 // lds: the lane's factor-slot area, lds_doubles long.
@@ -357,31 +358,257 @@ __device__ inline bool priv_stage(const PrivLane &L, uint32_t tid, uint32_t wave
 }
 
 // The lane's factor values (lx, ux, udiag) live in the LDS past the
-// lane's working set (`tail`, tail_n doubles) when they fit there with the
-// column ranges of A in pivot order (kp[2k], kp[2k + 1]: column q[k]'s
-// span of the pattern) and the column flags, and in global memory
-// otherwise. Uniform.
+// lane's working set (`tail`, tail_n doubles) when they fit there with two
+// record arrays the replay reads ahead of its chain: per column k (and a
+// sentinel at n) {up[k], lp[k], A's column span p0, p1 | flags << 30}, and
+// per U entry {i | (L column i's length) << 16, lp[i]}; otherwise in
+// global memory. Uniform.
 __device__ inline bool priv_resident(PrivLane &L, double *tail, uint32_t tail_n)
 {
-    const uint32_t nl = uni(L.hdr[0]), nu = uni(L.hdr[1]);
-    if ((uint64_t)nl + nu + 2 * L.n + (L.n + 1) / 2 > tail_n) return false;
+    const uint32_t nl = uni(L.hdr[0]), nu = uni(L.hdr[1]), n = L.n;
+    if (n >= 65536 || L.nnz >= (1u << 30)) return false;
+    uint64_t o = (uint64_t)nl + nu + n;
+    if (reinterpret_cast<uintptr_t>(tail + o) & 15) o++;
+    if (o + 2 * ((uint64_t)n + 1) + nu > tail_n) return false;
     L.lx = tail;
     L.ux = tail + nl;
     L.udiag = tail + nl + nu;
-    L.kp = reinterpret_cast<uint32_t *>(tail + nl + nu + L.n);
-    L.fl = L.kp + 2 * L.n;
+    L.cr = reinterpret_cast<uint32_t *>(tail + o);
+    L.ue = reinterpret_cast<uint32_t *>(tail + o + 2 * ((uint64_t)n + 1));
     return true;
 }
 
-// The column ranges and flags of a resident lane (all threads).
+// The record arrays of a resident lane (all threads; the pattern is staged).
 __device__ inline void priv_stage_static(const PrivLane &L, const uint32_t *col_ptr, const uint32_t *q, uint32_t tid)
 {
-    for (uint32_t k = tid; k < L.n; k += kWave) {
-        const uint32_t c = q[k];
-        L.kp[2 * k] = col_ptr[c];
-        L.kp[2 * k + 1] = col_ptr[c + 1];
-        L.fl[k] = L.flags[k];
+    const uint32_t n = L.n, nu = uni(L.hdr[1]);
+    for (uint32_t k = tid; k <= n; k += kWave) {
+        uint32_t *r = L.cr + 4 * (size_t)k;
+        r[0] = L.up[k];
+        r[1] = L.lp[k];
+        if (k < n) {
+            const uint32_t c = q[k];
+            r[2] = col_ptr[c];
+            r[3] = col_ptr[c + 1] | (L.flags[k] << 30);
+        } else {
+            r[2] = r[3] = 0;
+        }
     }
+    for (uint32_t p = tid; p < nu; p += kWave) {
+        const uint32_t i = L.ui[p], a0 = L.lp[i];
+        L.ue[2 * (size_t)p] = i | ((L.lp[i + 1] - a0) << 16);
+        L.ue[2 * (size_t)p + 1] = a0;
+    }
+}
+
+// The replay of a resident lane (priv_replay's steps in its order): the
+// records of the next column, its U entries and its first L rows, and its
+// terms of A, are read while this column runs, and each U entry's first
+// terms while the entry before it runs, so an entry costs one LDS round
+// trip (its pivot-row value and its targets together). True when a peel
+// test fails.
+__device__ bool priv_replay_res(const PrivLane &L, double growth_limit, double wide, uint32_t tid)
+{
+    const uint32_t n = L.n, nl = uni(L.hdr[0]), nu = uni(L.hdr[1]);
+    const auto vals = gptr(L.ap);
+    const auto prow = gptr(L.prow);
+    const auto w = lptr(L.w);
+    const auto li = lptr(L.li);
+    const auto lx = lptr(L.lx), ux = lptr(L.ux), ud = lptr(L.udiag);
+    const auto cr = lptr(L.cr);
+    const auto ue = lptr(L.ue);
+    const uint32_t n_void = uni(L.hdr[2]);
+    bool nz = false;
+    for (uint32_t v = tid; v < n_void; v += kWave) nz = nz || vals[gptr(L.void_slots)[v]] != 0;
+    if (wave_any(nz)) return true;
+    uint64_t nonfinite = 0, worst = 0;
+    bool bad = false, growth_bad = false;
+    struct Rec {
+        uint32_t x, y, z, w;
+    };
+    struct Ent {
+        uint32_t x, y;
+    };
+    auto rec = [&](uint32_t k) { return Rec{cr[4 * k], cr[4 * k + 1], cr[4 * k + 2], cr[4 * k + 3]}; };
+    auto ent = [&](uint32_t p) { return Ent{ue[2 * p], ue[2 * p + 1]}; };
+    // column 0 and 1 records, column 0's U entries, first L rows and terms of A
+    Rec c = rec(0), cn = rec(n >= 1 ? 1 : 0);
+    Ent e{0, 0};
+    if (c.x + tid < nu) e = ent(c.x + tid);
+    uint32_t kr = c.y + tid < nl ? li[c.y + tid] : 0u;
+    double sv = 0;
+    uint32_t sr = 0;
+    {
+        const uint32_t p0 = uni(c.z), p1 = uni(c.w) & 0x3fffffffu;
+        if (p0 + tid < p1) {
+            sv = vals[p0 + tid];
+            sr = prow[p0 + tid];
+        }
+    }
+    // entry j of the column: (i, L column start, length)
+    auto entry = [&](uint32_t u0, uint32_t j, uint32_t &i, uint32_t &a0, uint32_t &len) {
+        uint32_t x, y;
+        if (j < kWave) {
+            x = (uint32_t)__builtin_amdgcn_readlane((int)e.x, (int)j);
+            y = (uint32_t)__builtin_amdgcn_readlane((int)e.y, (int)j);
+        } else {
+            const Ent r = ent(u0 + j);
+            x = uni(r.x);
+            y = uni(r.y);
+        }
+        i = x & 0xffffu;
+        len = x >> 16;
+        a0 = y;
+    };
+    for (uint32_t k = 0; k < n; k++) {
+        const uint32_t u0 = uni(c.x), l0 = uni(c.y), p0 = uni(c.z), pw = uni(c.w);
+        const uint32_t p1 = pw & 0x3fffffffu, fk = pw >> 30;
+        const uint32_t u1 = uni(cn.x), l1 = uni(cn.y);
+        const uint32_t m = u1 - u0;
+        // ahead: the record after next, the next column's entries, L rows and terms of A
+        const Rec cnn = k + 2 <= n ? rec(k + 2) : cn;
+        Ent en{0, 0};
+        if (u1 + tid < nu) en = ent(u1 + tid);
+        const uint32_t krn = l1 + tid < nl ? li[l1 + tid] : 0u;
+        double svn = 0;
+        uint32_t srn = 0;
+        if (k + 1 < n) {
+            const uint32_t q0 = uni(cn.z), q1 = uni(cn.w) & 0x3fffffffu;
+            if (q0 + tid < q1) {
+                svn = vals[q0 + tid];
+                srn = prow[q0 + tid];
+            }
+        }
+        // this column's terms of A
+        if (p0 + tid < p1) w[sr] = sv;
+        for (uint32_t p = p0 + kWave + tid; p < p1; p += kWave) w[prow[p]] = vals[p];
+        // the U entries in order, each entry's first terms read one entry ahead
+        uint32_t r0 = 0;
+        double m0 = 0;
+        if (m > 0) {
+            uint32_t i, a0, len;
+            entry(u0, 0, i, a0, len);
+            if (tid < len) {
+                r0 = li[a0 + tid];
+                m0 = lx[a0 + tid];
+            }
+        }
+        for (uint32_t j = 0; j < m; j++) {
+            uint32_t i, a0, len;
+            entry(u0, j, i, a0, len);
+            uint32_t r1 = 0;
+            double m1 = 0;
+            if (j + 1 < m) {
+                uint32_t i1, b0, len1;
+                entry(u0, j + 1, i1, b0, len1);
+                if (tid < len1) {
+                    r1 = li[b0 + tid];
+                    m1 = lx[b0 + tid];
+                }
+            }
+            const bool on = tid < len;
+            const double wr = on ? w[r0] : 0.0;
+            const double uki = uni(w[i]);
+            nonfinite |= dbits(uki - uki);
+            if (on) {
+                const double t = m0 * uki;
+                w[r0] = wr - t;
+            }
+            for (uint32_t pl = a0 + kWave + tid; pl < a0 + len; pl += kWave) {
+                const uint32_t r = li[pl];
+                const double t = lx[pl] * uki;
+                w[r] = w[r] - t;
+            }
+            // row i is no target of its own L column
+            if (tid == 0) {
+                w[i] = 0;
+                ux[u0 + j] = uki;
+            }
+            r0 = r1;
+            m0 = m1;
+        }
+        const bool own = l0 + tid < l1;
+        if (fk & kVoidBit) {
+            if (tid == 0) {
+                ud[k] = 1;
+                for (uint32_t p = l0; p < l1; p++) {
+                    lx[p] = 0;
+                    w[li[p]] = 0;
+                }
+                w[k] = 0;
+                for (uint32_t j = 0; j < m; j++) {
+                    const Ent r = ent(u0 + j);
+                    const uint32_t a0 = r.y, a1 = r.y + (r.x >> 16);
+                    for (uint32_t pl = a0; pl < a1; pl++) w[li[pl]] = 0;
+                }
+            }
+        } else {
+            const double v0 = own ? w[kr] : 0.0;
+            double d = uni(w[k]);
+            const bool dead = d == 0 || !(fabs(d) < __builtin_inf());
+            bad = bad || dead;
+            if (dead) d = 1;
+            if (tid == 0) {
+                w[k] = 0;
+                ud[k] = d;
+            }
+            const bool scaled = (fk & kScaledBit) != 0;
+            if (growth_limit > 0 && !scaled) {
+                // each thread tests growth on its own terms (|d| >= g m holds
+                // for the largest m exactly when it holds for every term's)
+                const uint64_t db = dbits(fabs(d));
+                uint64_t cb = db;
+                if (own) {
+                    w[kr] = 0;
+                    const uint64_t vb = dbits(fabs(v0));
+                    cb = vb > cb ? vb : cb;
+                    const double lv = v0 / d;
+                    lx[l0 + tid] = lv;
+                    const uint64_t lb = dbits(fabs(lv));
+                    worst = lb > worst ? lb : worst;
+                }
+                for (uint32_t pl = l0 + kWave + tid; pl < l1; pl += kWave) {
+                    const uint32_t r = li[pl];
+                    const double v = w[r];
+                    w[r] = 0;
+                    const uint64_t vb = dbits(fabs(v));
+                    cb = vb > cb ? vb : cb;
+                    const double lv = v / d;
+                    lx[pl] = lv;
+                    const uint64_t lb = dbits(fabs(lv));
+                    worst = lb > worst ? lb : worst;
+                }
+                growth_bad = growth_bad || !(fabs(d) >= growth_limit * bitsd(cb));
+            } else {
+                if (own) {
+                    const double lv = v0 / d;
+                    nonfinite |= dbits(lv - lv);
+                    lx[l0 + tid] = lv;
+                    w[kr] = 0;
+                    const uint64_t lb = dbits(fabs(lv));
+                    if (!scaled) worst = lb > worst ? lb : worst;
+                }
+                for (uint32_t pl = l0 + kWave + tid; pl < l1; pl += kWave) {
+                    const uint32_t r = li[pl];
+                    const double lv = w[r] / d;
+                    nonfinite |= dbits(lv - lv);
+                    lx[pl] = lv;
+                    w[r] = 0;
+                    const uint64_t lb = dbits(fabs(lv));
+                    if (!scaled) worst = lb > worst ? lb : worst;
+                }
+            }
+        }
+        c = cn;
+        cn = cnn;
+        e = en;
+        kr = krn;
+        sv = svn;
+        sr = srn;
+    }
+    if (wave_any(nonfinite != 0)) return true;
+    worst = wave_max_u64(worst);
+    return bad || wave_any(growth_bad) || !(bitsd(worst) <= wide);
 }
 
 template <bool kRes>
@@ -392,17 +619,15 @@ __device__ inline auto pl_vals(double *p)
 }
 
 // The lane replay of its own factors on L.ap (lane_kernels.zig:49-118 with
-// the wide-multiplier test), spread over the wavefront: column k's scatter,
-// each U entry's update of the dense column (the targets of one L column
-// are distinct rows, so its terms go to different threads while the U
-// entries stay in their order), and the division of column k by its pivot
-// run one term per thread; every slot takes the serial replay's operations
-// in its order, so the factors are bitwise the serial replay's. LDS
-// operations of one wavefront complete in issue order, which carries each
-// step's writes to the next step's reads. A resident lane reads the next
-// column's terms of A while it works on this one. True when a peel test
-// fails (the lane then factors again).
-template <bool kRes>
+// the wide-multiplier test) with the values in global memory, spread over
+// the wavefront: column k's scatter, each U entry's update of the dense
+// column (the targets of one L column are distinct rows, so its terms go
+// to different threads while the U entries stay in their order), and the
+// division of column k by its pivot run one term per thread; every slot
+// takes the serial replay's operations in its order, so the factors are
+// bitwise the serial replay's. LDS operations of one wavefront complete in
+// issue order, which carries each step's writes to the next step's reads.
+// True when a peel test fails (the lane then factors again).
 __device__ bool priv_replay(const PrivLane &L, const uint32_t *col_ptr_, const uint32_t *q_, double growth_limit,
                             double wide, uint32_t tid)
 {
@@ -414,8 +639,7 @@ __device__ bool priv_replay(const PrivLane &L, const uint32_t *col_ptr_, const u
     const auto q = gptr(q_);
     const auto w = lptr(L.w);
     const auto lp = lptr(L.lp), up = lptr(L.up), li = lptr(L.li), ui = lptr(L.ui);
-    const auto lx = pl_vals<kRes>(L.lx), ux = pl_vals<kRes>(L.ux), ud = pl_vals<kRes>(L.udiag);
-    const auto kp = lptr(L.kp), fl = lptr(L.fl);
+    const auto lx = gptr(L.lx), ux = gptr(L.ux), ud = gptr(L.udiag);
     const uint32_t n_void = uni(L.hdr[2]);
     bool nz = false;
     for (uint32_t v = tid; v < n_void; v += kWave) nz = nz || vals[gptr(L.void_slots)[v]] != 0;
@@ -423,17 +647,6 @@ __device__ bool priv_replay(const PrivLane &L, const uint32_t *col_ptr_, const u
     uint64_t nonfinite = 0, worst = 0;
     bool bad = false, growth_bad = false;
     uint32_t u0 = uni(up[0]);
-    // resident: the next column's span and this thread's first term of it
-    uint32_t np0 = 0, np1 = 0, sr = 0;
-    double sv = 0;
-    if constexpr (kRes) {
-        np0 = uni(kp[0]);
-        np1 = uni(kp[1]);
-        if (np0 + tid < np1) {
-            sv = vals[np0 + tid];
-            sr = prow[np0 + tid];
-        }
-    }
     for (uint32_t k = 0; k < n; k++) {
         uint32_t p0, p1, fk;
         // this column's U entries (thread t: the t-th, its L column's span)
@@ -449,23 +662,7 @@ __device__ bool priv_replay(const PrivLane &L, const uint32_t *col_ptr_, const u
         }
         const bool own = l0 + tid < l1;
         const uint32_t kr = own ? li[l0 + tid] : 0u;
-        if constexpr (kRes) {
-            p0 = np0;
-            p1 = np1;
-            const double cv = sv;
-            const uint32_t cr = sr;
-            if (k + 1 < n) {
-                np0 = uni(kp[2 * k + 2]);
-                np1 = uni(kp[2 * k + 3]);
-                if (np0 + tid < np1) {
-                    sv = vals[np0 + tid];
-                    sr = prow[np0 + tid];
-                }
-            }
-            if (p0 + tid < p1) w[cr] = cv;
-            for (uint32_t p = p0 + kWave + tid; p < p1; p += kWave) w[prow[p]] = vals[p];
-            fk = uni(fl[k]);
-        } else {
+        {
             const uint32_t c = uni(q[k]);
             p0 = uni(col_ptr[c]);
             p1 = uni(col_ptr[c + 1]);
@@ -697,14 +894,17 @@ __device__ __forceinline__ int priv_factor_at(const PrivLU &P, uint32_t lane, co
 // The replay of the lane's own factors, and on a failed peel test its full
 // factor: kPlOk, kPlSingular or kPlFull (uniform). fresh: the last factor
 // met a singular matrix and left no usable LU.
-// The replay alone: true when a peel test fails (uniform).
+// The replay alone: true when a peel test fails (uniform); resident: the
+// values were in LDS.
 __device__ __forceinline__ bool priv_replay_at(const PrivLU &P, uint32_t lane, const uint32_t *col_ptr,
                                                const uint32_t *q, double *lds, uint32_t lds_doubles, double *tail,
-                                               uint32_t tail_n, uint32_t tid)
+                                               uint32_t tail_n, uint32_t tid, bool *resident = nullptr)
 {
     PrivLane L = priv_lane(P, lane, lds, lds_doubles);
-    const bool failed = priv_resident(L, tail, tail_n) ? priv_replay<true>(L, col_ptr, q, 1e-12, 1.0 / 1e-3, tid)
-                                                       : priv_replay<false>(L, col_ptr, q, 1e-12, 1.0 / 1e-3, tid);
+    const bool res = priv_resident(L, tail, tail_n);
+    if (resident) *resident = res;
+    const bool failed = res ? priv_replay_res(L, 1e-12, 1.0 / 1e-3, tid)
+                            : priv_replay(L, col_ptr, q, 1e-12, 1.0 / 1e-3, tid);
     wsync();
     return failed;
 }

@@ -1868,18 +1868,19 @@ u32 lds_bytes_for(u32 tv_cap, u32 n)
 u32 device_cu_count();
 
 // LDS of one lane in a launch of the own-LU kernels over `grid` lanes: when
-// the launch leaves LDS idle (few lanes per CU), room past the working set
-// for the lanes' own factor values (twice the lead factor's entries, the
-// diagonal, the column ranges and flags), so that their replays and solves
-// run out of LDS; otherwise the working set alone.
+// the launch leaves LDS idle (few lanes per CU), room (up to what is idle)
+// past the working set
+// for the lanes' own factor values and the replay's records (room for
+// twice the lead factor's entries), so that their replays and solves run
+// out of LDS; otherwise the working set alone.
 u32 own_lds_bytes(u32 lds, u32 grid, u32 n, u32 nnz_l, u32 nnz_u)
 {
     const u32 cu = device_cu_count();
     if (cu == 0 || grid == 0) return lds;
     const size_t per_cu = (grid + cu - 1) / cu;
-    const size_t want = lds + 8 * (2 * ((size_t)nnz_l + nnz_u) + 3 * (size_t)n + 16);
-    const size_t room = std::min<size_t>(65536, 160 * 1024 / per_cu);
-    return want <= room ? (u32)want : lds;
+    const size_t want = lds + 8 * (2 * (size_t)nnz_l + 4 * (size_t)nnz_u + 3 * (size_t)n + 16);
+    const size_t room = std::min<size_t>(65536, 160 * 1024 / per_cu) & ~(size_t)15;
+    return (u32)std::max<size_t>(lds, std::min(want, room));
 }
 
 

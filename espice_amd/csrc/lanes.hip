@@ -841,9 +841,11 @@ __device__ __forceinline__ uint32_t op_lane(const OpArgs &a, uint32_t lane, doub
             int rc = kPlOk;
             const bool fresh = (L.flags & kLadNeedFactor) != 0;
             L.flags &= ~kLadNeedFactor;
+            bool res = false;
             const bool failed =
-                fresh || priv_replay_at(a.priv, lane, dp.csc_ptr, dp.colq, tv, a.tv_cap, tail, tail_n, tid);
+                fresh || priv_replay_at(a.priv, lane, dp.csc_ptr, dp.colq, tv, a.tv_cap, tail, tail_n, tid, &res);
             lap(1);
+            if (res) pc[5]++;
             if (failed) {
                 rc = priv_factor_at(a.priv, lane, dp.csc_ptr, dp.csc_row, dp.colq, tv, a.tv_cap, tail, tail_n, tid);
                 lap(4);

@@ -476,7 +476,8 @@ esl_status esl_last_timing(esl_engine *e, esl_timing *out);
 /* Development hook (no counterpart in the reference): enable != 0 makes the
  * following esl_op_solve calls count shader cycles per lane in five phases
  * (assembly, refactor, solve, update + limiting + gates, the full factors of
- * lanes on their own LU); out receives the last solve's counters [count][5]
+ * lanes on their own LU) and, sixth, the lane's own-LU replays that ran with
+ * their values in LDS; out receives the last solve's counters [count][6]
  * when capacity allows. */
 esl_status esl_debug_profile(esl_engine *e, int enable, unsigned long long *out,
                              size_t capacity);
