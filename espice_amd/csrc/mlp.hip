@@ -134,24 +134,47 @@ __global__ void mlp_act_back(const float *dH, const float *Z, uint32_t count, fl
     dZb[e] = f2bf(v);
 }
 
-// db[c] = sum over rows of dZ[r][c], rows in order.
-__global__ void mlp_bias_grad(const float *dZ, uint32_t rows, uint32_t cols, float *db)
+// db[c] = sum over rows of dZ[r][c]: a block of kBiasCols columns x
+// kBiasGroups row groups; each thread sums its group's rows (reads of a
+// row coalesced across the block's columns), then the groups add up in LDS
+// in a fixed tree (deterministic).
+constexpr uint32_t kBiasCols = 64, kBiasGroups = 16;
+__global__ __launch_bounds__(kBiasCols * kBiasGroups) void mlp_bias_grad(const float *dZ, uint32_t rows,
+                                                                          uint32_t cols, float *db)
 {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cols) return;
+    __shared__ float part[kBiasGroups][kBiasCols];
+    const uint32_t x = threadIdx.x % kBiasCols, g = threadIdx.x / kBiasCols;
+    const uint32_t c = blockIdx.x * kBiasCols + x;
     float s = 0.f;
-    for (uint32_t r = 0; r < rows; r++) s += dZ[(size_t)r * cols + c];
-    db[c] = s;
+    if (c < cols)
+        for (uint32_t r = g; r < rows; r += kBiasGroups) s += dZ[(size_t)r * cols + c];
+    part[g][x] = s;
+    __syncthreads();
+    for (uint32_t h = kBiasGroups / 2; h > 0; h >>= 1) {
+        if (g < h) part[g][x] += part[g + h][x];
+        __syncthreads();
+    }
+    if (g == 0 && c < cols) db[c] = part[0][x];
 }
 
-// Sum of row losses into loss[0] (one thread; rows in order).
-__global__ void mlp_loss_sum(const float *row_loss, uint32_t rows, uint32_t cols, double *acc)
+// Sum of row losses into loss[0]: 256 threads, strided partial sums, then a
+// fixed tree in LDS (deterministic).
+__global__ __launch_bounds__(256) void mlp_loss_sum(const float *row_loss, uint32_t rows, uint32_t cols,
+                                                    double *acc)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ double part[256];
     double s = 0.0;
-    for (uint32_t r = 0; r < rows; r++) s += row_loss[r];
-    acc[0] += s / ((double)rows * (double)cols);
-    acc[1] += 1.0;
+    for (uint32_t r = threadIdx.x; r < rows; r += 256) s += row_loss[r];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        acc[0] += part[0] / ((double)rows * (double)cols);
+        acc[1] += 1.0;
+    }
 }
 
 // Adam (decoupled weight decay) on the FP32 master parameters; BF16 copies
@@ -227,7 +250,7 @@ hipError_t esl_mlp_launch_loss(const float *Y, const float *T, const uint32_t *i
 {
     hipLaunchKernelGGL(esl::mlp_loss, dim3(blocks(rows, 128)), dim3(128), 0, s, Y, T, idx, rows, cols, dYf, dYb,
                        row_loss);
-    hipLaunchKernelGGL(esl::mlp_loss_sum, dim3(1), dim3(64), 0, s, row_loss, rows, cols, acc);
+    hipLaunchKernelGGL(esl::mlp_loss_sum, dim3(1), dim3(256), 0, s, row_loss, rows, cols, acc);
     return hipGetLastError();
 }
 hipError_t esl_mlp_launch_act_back(const float *dH, const float *Z, uint32_t count, float *dZf, uint16_t *dZb,
@@ -238,7 +261,8 @@ hipError_t esl_mlp_launch_act_back(const float *dH, const float *Z, uint32_t cou
 }
 hipError_t esl_mlp_launch_bias_grad(const float *dZ, uint32_t rows, uint32_t cols, float *db, hipStream_t s)
 {
-    hipLaunchKernelGGL(esl::mlp_bias_grad, dim3(blocks(cols, 64)), dim3(64), 0, s, dZ, rows, cols, db);
+    hipLaunchKernelGGL(esl::mlp_bias_grad, dim3(blocks(cols, esl::kBiasCols)), dim3(esl::kBiasCols * esl::kBiasGroups),
+                       0, s, dZ, rows, cols, db);
     return hipGetLastError();
 }
 hipError_t esl_mlp_launch_adam(float *p, const float *grad, float *m, float *v, uint32_t count,
