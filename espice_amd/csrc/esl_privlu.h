@@ -402,8 +402,8 @@ __device__ inline void priv_stage_static(const PrivLane &L, const uint32_t *col_
 }
 
 // The replay of a resident lane (priv_replay's steps in its order): the
-// records of the next column, its U entries and its first L rows, and its
-// terms of A, are read while this column runs, and each U entry's first
+// records of the next column, its U entries and its first L rows are read
+// while this column runs, A's terms two columns ahead, and each U entry's first
 // terms while the entry before it runs, so an entry costs one LDS round
 // trip (its pivot-row value and its targets together). True when a peel
 // test fails.
@@ -436,13 +436,22 @@ __device__ bool priv_replay_res(const PrivLane &L, double growth_limit, double w
     Ent e{0, 0};
     if (c.x + tid < nu) e = ent(c.x + tid);
     uint32_t kr = c.y + tid < nl ? li[c.y + tid] : 0u;
-    double sv = 0;
-    uint32_t sr = 0;
+    // this thread's first term of A in columns k and k + 1 (read two columns
+    // ahead, so a column's global read has two columns' time to arrive)
+    double sv = 0, sv1 = 0;
+    uint32_t sr = 0, sr1 = 0;
     {
         const uint32_t p0 = uni(c.z), p1 = uni(c.w) & 0x3fffffffu;
         if (p0 + tid < p1) {
             sv = vals[p0 + tid];
             sr = prow[p0 + tid];
+        }
+        if (n >= 2) {
+            const uint32_t q0 = uni(cn.z), q1 = uni(cn.w) & 0x3fffffffu;
+            if (q0 + tid < q1) {
+                sv1 = vals[q0 + tid];
+                sr1 = prow[q0 + tid];
+            }
         }
     }
     // entry j of the column: (i, L column start, length)
@@ -470,13 +479,13 @@ __device__ bool priv_replay_res(const PrivLane &L, double growth_limit, double w
         Ent en{0, 0};
         if (u1 + tid < nu) en = ent(u1 + tid);
         const uint32_t krn = l1 + tid < nl ? li[l1 + tid] : 0u;
-        double svn = 0;
-        uint32_t srn = 0;
-        if (k + 1 < n) {
-            const uint32_t q0 = uni(cn.z), q1 = uni(cn.w) & 0x3fffffffu;
+        double sv2 = 0;
+        uint32_t sr2 = 0;
+        if (k + 2 < n) {
+            const uint32_t q0 = uni(cnn.z), q1 = uni(cnn.w) & 0x3fffffffu;
             if (q0 + tid < q1) {
-                svn = vals[q0 + tid];
-                srn = prow[q0 + tid];
+                sv2 = vals[q0 + tid];
+                sr2 = prow[q0 + tid];
             }
         }
         // this column's terms of A
@@ -603,8 +612,10 @@ __device__ bool priv_replay_res(const PrivLane &L, double growth_limit, double w
         cn = cnn;
         e = en;
         kr = krn;
-        sv = svn;
-        sr = srn;
+        sv = sv1;
+        sr = sr1;
+        sv1 = sv2;
+        sr1 = sr2;
     }
     if (wave_any(nonfinite != 0)) return true;
     worst = wave_max_u64(worst);
